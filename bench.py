@@ -1,0 +1,194 @@
+"""Benchmark: exact Z/p gemm, 8192 x 8192 x 8192, p = 65521, uint16 storage,
+C <- A*B + C on one MI355X with split-int8 MFMA; inputs resident in HBM.
+
+Prints one JSON line (see the keys below).  `value` is modular MACs/s of the
+int8 kernel family over the timed steps; `roofline` is the MFMA kernel alone,
+timed with HIP events on the library's stream; `variants` holds the plain
+VALU kernel and the FP64-carrier kernel on the same shape for comparison;
+`cpu_baseline` times the CPU restatement of LELA's ZpModule<uint16> gemm on a
+bounded block of rows of the same workload, one core.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N = 8192
+P = 65521
+P_FP64 = 67108859
+INT8_PEAK_TOPS = 5000.0      # MI355X dense INT8 MFMA, 5.0 POP/s
+FP64_PEAK_TFLOPS = 78.6
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    return ap.parse_args()
+
+
+def time_steps(ctx, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    ctx.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def cpu_baseline(seconds):
+    """CPU restatement (oracle) of ZpModule<uint16> gemm on whole rows of C."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+    from lela_amd import synth
+    lib = orc.load()
+    A = synth.zp_matrix(synth.seed_for(2, 0), 64, N, P, np.uint16)
+    B = synth.zp_matrix(synth.seed_for(2, 1), N, N, P, np.uint16)
+    C = synth.zp_matrix(synth.seed_for(2, 2), 64, N, P, np.uint16)
+
+    def run(rows):
+        t0 = time.perf_counter()
+        lib.orc_zp_gemm_u16(P, rows, N, N, 1, A.ctypes.data, N, B.ctypes.data, N, 1,
+                            C.ctypes.data, N)
+        return time.perf_counter() - t0
+
+    t1 = run(1)
+    rows = max(1, min(64, int(seconds / max(t1, 1e-6))))
+    t = run(rows)
+    macs = rows * N * N
+    return {"value": macs / t, "unit": "modular MACs/s", "cores": 1, "kind": "port",
+            "sample": f"{rows} of {N} rows of C for the {N}^3 uint16 gemm "
+                      f"({rows}*{N}*{N} MACs, {t:.1f} s), CPU restatement of "
+                      "ZpModule<uint16> dot + generic gemm, -O2, one core"}
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} (WORLD_SIZE={world}) is not "
+              "supported: every box here has one GPU and this path is not sharded.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import lela_amd as la
+    from lela_amd import synth
+
+    F = la.Modular(P, "uint16")
+    ctx = la.Context(F, device=0)
+    A = synth.zp_matrix(synth.seed_for(2, 0), N, N, P, np.uint16)
+    B = synth.zp_matrix(synth.seed_for(2, 1), N, N, P, np.uint16)
+    C = synth.zp_matrix(synth.seed_for(2, 2), N, N, P, np.uint16)
+    dA, dB, dC = (la.DeviceMatrix.from_host(ctx, X) for X in (A, B, C))
+    del A, B
+
+    def step_int8():
+        la.BLAS3.gemm(ctx, 1, dA, dB, 1, dC, algo=la.ALGO_INT8)
+
+    for _ in range(args.warmup):
+        step_int8()
+    ctx.synchronize()
+    ctx.prof_reset()
+    ctx.prof_enable(True)
+    t0 = time.perf_counter()
+    ctx.timer_start()
+    for _ in range(args.steps):
+        step_int8()
+    ev_ms = ctx.timer_stop()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.prof_enable(False)
+    launches, kernel_ms = ctx.prof_query("gemm_i8_mfma")
+    _, pack_ms = ctx.prof_query("pack_i8")
+    ms_per_step = wall * 1e3 / args.steps
+    W = float(N) ** 3
+    value = W / (ms_per_step * 1e-3)
+    avg_kernel_ms = kernel_ms / max(1, launches)
+    achieved = 8.0 * W / (avg_kernel_ms * 1e-3) / 1e12   # int8 ops: 4 limb MACs per modular MAC
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "gemm_i8_mfma_8192_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    out = {
+        "metric": "Z/p gemm modular MACs/s @8192^3 p=65521 (split-int8 MFMA)",
+        "value": value,
+        "unit": "modular MACs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int8",
+        "data": "synthetic (SplitMix64 uniform residues, seeds 0x4C454C4100000020..22)",
+        "config": {"workload": "Z/p gemm C <- A*B + C, m=n=k=8192, p=65521, Modular<uint16> "
+                               "storage, inputs resident in HBM",
+                   "m": N, "n": N, "k": N, "p": P, "storage": "uint16",
+                   "algo": "split-int8 MFMA (2x2 8-bit limbs, int32 accumulate)"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS,
+                     "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": traffic,
+                     "kernel": "gemm_i8_kernel", "kernel_avg_ms": avg_kernel_ms,
+                     "ops_per_launch": 8.0 * W},
+        "timing": {"event_ms_per_step": ev_ms / args.steps,
+                   "pack_ms_per_step": pack_ms / args.steps},
+    }
+
+    if not args.no_variants:
+        variants = {}
+        t = time_steps(ctx, lambda: la.BLAS3.gemm(ctx, 1, dA, dB, 1, dC, algo=la.ALGO_PLAIN), 2, 1)
+        variants["plain_u16_p65521"] = {"ms_per_step": t * 1e3, "value": W / t,
+                                        "unit": "modular MACs/s"}
+        out["int8_over_plain"] = value / variants["plain_u16_p65521"]["value"]
+        for d in (dA, dB, dC):
+            d.free()
+        F2 = la.Modular(P_FP64, "uint32")
+        ctx.F = F2
+        A2 = synth.zp_matrix(synth.seed_for(2, 5), N, N, P_FP64)
+        dA2 = la.DeviceMatrix.from_host(ctx, A2)
+        del A2
+        dB2 = la.DeviceMatrix.from_host(ctx, synth.zp_matrix(synth.seed_for(2, 6), N, N, P_FP64))
+        dC2 = la.DeviceMatrix.from_host(ctx, synth.zp_matrix(synth.seed_for(2, 7), N, N, P_FP64))
+        ctx.prof_reset()
+        ctx.prof_enable(True)
+        t = time_steps(ctx, lambda: la.BLAS3.gemm(ctx, 1, dA2, dB2, 1, dC2, algo=la.ALGO_FP64), 2, 1)
+        ctx.prof_enable(False)
+        cnt, kms = ctx.prof_query("gemm_f64_mfma")
+        fp64_kernel_ms = kms / max(1, cnt)
+        variants["fp64_u32_p67108859"] = {
+            "ms_per_step": t * 1e3, "value": W / t, "unit": "modular MACs/s",
+            "roofline": {"bound": "mfma", "achieved": 2 * 2 * W / (fp64_kernel_ms * 1e-3) / 1e12,
+                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": 2 * 2 * W / (fp64_kernel_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}}
+        for d in (dA2, dB2, dC2):
+            d.free()
+        out["variants"] = variants
+    else:
+        for d in (dA, dB, dC):
+            d.free()
+    ctx.close()
+
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

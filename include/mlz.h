@@ -1,0 +1,129 @@
+/* include/mlz.h -- C ABI of the MI355X exact linear-algebra backend
+ * (lela_amd/liblela_amd.so).
+ *
+ * This is the boundary a LELA module (HIPModule) binds:
+ * plain pointers and sizes, no C++ or HIP types.  Each entry point names the
+ * LELA routine whose semantics it reproduces (paths relative to the LELA
+ * source tree).
+ *
+ * Conventions
+ *  - Matrices are row-major; leading dimensions count elements (words for GF(2)).
+ *  - Z/p values are stored canonically in [0, p) for MLZ_U16 / MLZ_U32
+ *    (ModularTraits<uint16/uint32>, lela/ring/modular.h:158-255) and in the
+ *    symmetric range [-(p-1)/2, (p-1)/2] for MLZ_F64_SYM (ModularTraits<double>,
+ *    lela/ring/modular.h:288-327).  Inputs must already be valid residues, as
+ *    LELA assumes.
+ *  - GF(2) rows are uint64 words, bit j of word w = column 64*w + j (LSB first,
+ *    lela/vector/bit-iterator.h:249-311, lela/matrix/dense-zero-one.h:253-257).
+ *  - Pointer mode: by default (MLZ_PTR_HOST) matrix pointers are host memory;
+ *    the library stages them through device memory and returns when the result
+ *    is back on the host.  With mlz_ctx_set_ptr_mode(ctx, MLZ_PTR_DEVICE) they
+ *    are device pointers (from mlz_dev_alloc or any hipMalloc); calls are then
+ *    queued on the context's stream and return without waiting unless they
+ *    produce a host-side scalar (rank, pivots, error index).
+ *  - Nothing is retained after a call returns; a context is not thread-safe
+ *    (one per thread, like LELA's Context, lela/blas/context.h:62-71).
+ *  - Return codes below.  The LELA-side shim converts them back into
+ *    DiagonalEntryNotInvertible(idx), WrongMatrixForm(row) and LELAError.
+ */
+#ifndef MLZ_H
+#define MLZ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mlz_ctx mlz_ctx;
+
+/* Storage of Z/p elements. */
+typedef enum {
+	MLZ_U16 = 1,      /* Modular<uint16>, p < 2^16 */
+	MLZ_U32 = 2,      /* Modular<uint32>, p < 2^32 */
+	MLZ_F64_SYM = 3   /* Modular<double>, p < 2^26, symmetric representatives */
+} mlz_elem;
+
+/* gemm kernel families (all exact; they differ only in speed). */
+enum {
+	MLZ_ALGO_AUTO = 0,
+	MLZ_ALGO_PLAIN = 1,  /* VALU uint64 multiply-accumulate, one reduction per entry */
+	MLZ_ALGO_INT8 = 2,   /* 8-bit limbs on INT8 MFMA, int32 accumulation; p <= 2^16 */
+	MLZ_ALGO_FP64 = 3    /* FP64 MFMA as exact integer carrier; p < 2^26 */
+};
+
+enum {
+	MLZ_PTR_HOST = 0,
+	MLZ_PTR_DEVICE = 1
+};
+
+/* Return codes. */
+enum {
+	MLZ_OK = 0,
+	MLZ_E_ARG = 1,             /* bad argument (dimensions, modulus, null pointer) */
+	MLZ_E_HIP = 2,             /* HIP runtime error; see mlz_last_error */
+	MLZ_E_NOT_INVERTIBLE = 3,  /* DiagonalEntryNotInvertible / pivot not invertible */
+	MLZ_E_WRONG_FORM = 4,      /* FaugereLachartre WrongMatrixForm */
+	MLZ_E_NOMEM = 5,           /* device allocation failed */
+	MLZ_E_UNSUPPORTED = 6      /* algorithm not available for this modulus/element */
+};
+
+/* ---- context ---------------------------------------------------------- */
+
+/* One context per LELA Context: owns a HIP stream, device workspace and the
+ * profiling state.  device = HIP device ordinal.  config may be NULL. */
+int mlz_ctx_create(mlz_ctx **out, int device, const char *config);
+int mlz_ctx_destroy(mlz_ctx *ctx);
+int mlz_ctx_set_ptr_mode(mlz_ctx *ctx, int mode);
+int mlz_ctx_synchronize(mlz_ctx *ctx);
+/* Message for the last error on this context (static storage, never NULL). */
+const char *mlz_last_error(mlz_ctx *ctx);
+/* Library version string. */
+const char *mlz_version(void);
+
+/* ---- device memory helpers (for callers that keep data resident) ------- */
+int mlz_dev_alloc(mlz_ctx *ctx, size_t bytes, void **ptr);
+int mlz_dev_free(mlz_ctx *ctx, void *ptr);
+int mlz_memcpy_htod(mlz_ctx *ctx, void *dst, const void *src, size_t bytes);
+int mlz_memcpy_dtoh(mlz_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Release cached workspace buffers. */
+int mlz_ctx_trim(mlz_ctx *ctx);
+
+/* ---- profiling: HIP events around every kernel launch, on the context's
+ *      stream, accumulated per kernel name. ---------------------------------- */
+int mlz_prof_enable(mlz_ctx *ctx, int on);
+int mlz_prof_reset(mlz_ctx *ctx);
+/* Synchronises, then returns launch count and summed milliseconds for the
+ * kernels whose name contains `name`. */
+int mlz_prof_query(mlz_ctx *ctx, const char *name, uint64_t *count, double *ms);
+/* Events on the context's stream for timing a region. */
+int mlz_timer_start(mlz_ctx *ctx);
+int mlz_timer_stop(mlz_ctx *ctx, double *ms);
+
+/* ---- Z/p level 3 -------------------------------------------------------- */
+
+/* C <- a*A*B + b*C mod p; A m x k, B k x n, C m x n.
+ * Replaces BLAS3::gemm (lela/blas/level3.h:81-83) for Modular<T> through
+ * _gemm<Modular<T>, ZpModule<T>::Tag> -> GenericModule gemm_impl
+ * (lela/blas/level3-generic.tcc:285-322) with the ZpModule dots
+ * (lela/blas/level1-modular.tcc:25-63,135-158) and, for Modular<double>, the
+ * BLAS-carrier gemm (lela/blas/level3-modular.tcc:25-123). */
+int mlz_zp_gemm(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t n, size_t k,
+		uint64_t a, const void *A, size_t lda, const void *B, size_t ldb,
+		uint64_t b, void *C, size_t ldc, int algo);
+
+/* B <- a * A^-1 * B with A n x n triangular (upper != 0: upper), B n x nrhs.
+ * Replaces BLAS3::trsm (lela/blas/level3.h:120-122) ->
+ * _trsm<Ring, GenericModule::Tag>::trsm_impl (lela/blas/level3-generic.tcc:421-498).
+ * Non-invertible diagonal: returns MLZ_E_NOT_INVERTIBLE, *bad_diag = the index
+ * LELA's recursion reports (smallest for lower, largest for upper). */
+int mlz_zp_trsm(mlz_ctx *ctx, mlz_elem elem, uint64_t p, int upper, int unit_diag,
+		size_t n, size_t nrhs, uint64_t a, const void *A, size_t lda,
+		void *B, size_t ldb, size_t *bad_diag, int algo);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MLZ_H */

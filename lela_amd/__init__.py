@@ -1,0 +1,269 @@
+"""lela_amd -- MI355X backend for LELA's exact level-3 path.
+
+Host-side mirror of LELA's interface for the operations the HIP library
+implements, so that calls read like LELA's own (lela/blas/level3.h,
+lela/solutions/echelon-form.h, lela/algorithms/faugere-lachartre.h):
+
+    F = Modular(65521, storage="uint16")        # Modular<uint16>
+    ctx = Context(F)                            # Context<Ring, HIPModule>
+    BLAS3.gemm(ctx, 1, A, B, 1, C)              # C <- A*B + C   (numpy arrays)
+    BLAS3.trsm(ctx, 1, L, X, LowerTriangular, True)
+
+Errors are raised as the LELA exceptions they stand for
+(DiagonalEntryNotInvertible(idx), WrongMatrixForm(row), LELAError).
+Every call goes through liblela_amd.so; there is no other compute path.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import (ALGO_AUTO, ALGO_FP64, ALGO_INT8, ALGO_PLAIN, MLZ_F64_SYM, MLZ_U16,
+                   MLZ_U32, PTR_DEVICE, PTR_HOST)
+
+__all__ = [
+    "Modular", "Context", "BLAS3", "DeviceMatrix", "LELAError", "DiagonalEntryNotInvertible",
+    "WrongMatrixForm", "LowerTriangular", "UpperTriangular", "ALGO_AUTO", "ALGO_PLAIN",
+    "ALGO_INT8", "ALGO_FP64",
+]
+
+LowerTriangular = "lower"
+UpperTriangular = "upper"
+
+
+class LELAError(RuntimeError):
+    """LELA's generic error (lela/util/error.h)."""
+
+
+class DiagonalEntryNotInvertible(LELAError):
+    """Raised by trsm with the index of the offending diagonal entry
+    (lela/util/error.h:83-95)."""
+
+    def __init__(self, idx):
+        super().__init__(f"Diagonal entry {idx} not invertible")
+        self.idx = idx
+
+
+class WrongMatrixForm(LELAError):
+    """Raised by Faugere-Lachartre for a row whose head decreases
+    (lela/algorithms/faugere-lachartre.tcc:32-41)."""
+
+    def __init__(self, row):
+        super().__init__(f"Bad input-matrix for Faugere-Lachartre at row-index {row}")
+        self.row = row
+
+
+_STORAGE = {
+    "uint16": (MLZ_U16, np.uint16),
+    "uint32": (MLZ_U32, np.uint32),
+    "double": (MLZ_F64_SYM, np.float64),
+}
+
+
+class Modular:
+    """Z/p with LELA's storage conventions: uint16/uint32 hold [0, p);
+    double holds the symmetric range (lela/ring/modular.h:158-327)."""
+
+    def __init__(self, p, storage="uint32"):
+        if storage not in _STORAGE:
+            raise ValueError(f"storage must be one of {sorted(_STORAGE)}")
+        self.p = int(p)
+        self.storage = storage
+        self.elem, self.dtype = _STORAGE[storage]
+        limits = {"uint16": 1 << 16, "uint32": 1 << 32, "double": 1 << 26}
+        if not 2 <= self.p < limits[storage]:
+            raise ValueError(f"modulus {p} not valid for Modular<{storage}>")
+
+    def from_canonical(self, X):
+        """Canonical residues [0, p) -> this ring's storage."""
+        X = np.asarray(X)
+        if self.storage == "double":
+            Xs = X.astype(np.int64)
+            Xs = np.where(2 * Xs > self.p, Xs - self.p, Xs)
+            return Xs.astype(np.float64)
+        return X.astype(self.dtype)
+
+    def to_canonical(self, X):
+        """This ring's storage -> canonical residues as uint32."""
+        X = np.asarray(X)
+        if self.storage == "double":
+            Xi = X.astype(np.int64)
+            return np.where(Xi < 0, Xi + self.p, Xi).astype(np.uint32)
+        return X.astype(np.uint32)
+
+    def __repr__(self):
+        return f"Modular<{self.storage}>({self.p})"
+
+
+class Context:
+    """Context<Ring, HIPModule>: one HIP stream + device workspace
+    (lela/blas/context.h:73-85).  Not thread-safe, like LELA's Context."""
+
+    def __init__(self, F=None, device=0):
+        self.F = F
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self.lib.mlz_ctx_create(ctypes.byref(h), device, None)
+        if rc != 0:
+            raise LELAError(f"mlz_ctx_create failed with code {rc} (no usable GPU?)")
+        self.handle = h
+        self._ptr_mode = PTR_HOST
+
+    def close(self):
+        if self.handle:
+            self.lib.mlz_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def last_error(self):
+        return self.lib.mlz_last_error(self.handle).decode()
+
+    def check(self, rc, what, idx=None):
+        if rc == 0:
+            return
+        if rc == _lib.MLZ_E_NOT_INVERTIBLE and idx is not None:
+            raise DiagonalEntryNotInvertible(idx)
+        if rc == _lib.MLZ_E_WRONG_FORM and idx is not None:
+            raise WrongMatrixForm(idx)
+        raise LELAError(f"{what}: code {rc}: {self.last_error()}")
+
+    def set_ptr_mode(self, mode):
+        self.check(self.lib.mlz_ctx_set_ptr_mode(self.handle, mode), "set_ptr_mode")
+        self._ptr_mode = mode
+
+    def synchronize(self):
+        self.check(self.lib.mlz_ctx_synchronize(self.handle), "synchronize")
+
+    # -- profiling -------------------------------------------------------
+    def prof_enable(self, on=True):
+        self.check(self.lib.mlz_prof_enable(self.handle, int(on)), "prof_enable")
+
+    def prof_reset(self):
+        self.check(self.lib.mlz_prof_reset(self.handle), "prof_reset")
+
+    def prof_query(self, name):
+        c = ctypes.c_uint64()
+        ms = ctypes.c_double()
+        self.check(self.lib.mlz_prof_query(self.handle, name.encode(), ctypes.byref(c),
+                                           ctypes.byref(ms)), "prof_query")
+        return c.value, ms.value
+
+    def timer_start(self):
+        self.check(self.lib.mlz_timer_start(self.handle), "timer_start")
+
+    def timer_stop(self):
+        ms = ctypes.c_double()
+        self.check(self.lib.mlz_timer_stop(self.handle, ctypes.byref(ms)), "timer_stop")
+        return ms.value
+
+
+class DeviceMatrix:
+    """A matrix resident in device memory (for callers that keep data in HBM)."""
+
+    def __init__(self, ctx, rows, cols, dtype):
+        self.ctx = ctx
+        self.rows, self.cols = int(rows), int(cols)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.rows * self.cols * self.dtype.itemsize
+        ptr = ctypes.c_void_p()
+        ctx.check(ctx.lib.mlz_dev_alloc(ctx.handle, self.nbytes, ctypes.byref(ptr)), "dev_alloc")
+        self.ptr = ptr
+
+    @classmethod
+    def from_host(cls, ctx, X):
+        X = np.ascontiguousarray(X)
+        d = cls(ctx, X.shape[0], X.shape[1] if X.ndim > 1 else 1, X.dtype)
+        d.upload(X)
+        return d
+
+    def upload(self, X):
+        X = np.ascontiguousarray(X, dtype=self.dtype)
+        assert X.nbytes == self.nbytes
+        self.ctx.check(self.ctx.lib.mlz_memcpy_htod(self.ctx.handle, self.ptr, X.ctypes.data,
+                                                    self.nbytes), "memcpy_htod")
+
+    def download(self):
+        out = np.empty((self.rows, self.cols), dtype=self.dtype)
+        self.ctx.check(self.ctx.lib.mlz_memcpy_dtoh(self.ctx.handle, out.ctypes.data, self.ptr,
+                                                    self.nbytes), "memcpy_dtoh")
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx.lib.mlz_dev_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+    @property
+    def ld(self):
+        return self.cols
+
+
+def _host_ptr(X, dtype):
+    if not isinstance(X, np.ndarray) or X.dtype != dtype:
+        raise TypeError(f"expected numpy array of {dtype}")
+    if X.ndim != 2:
+        raise TypeError("expected a 2-D array")
+    if X.size == 0:
+        return X.ctypes.data, max(1, X.shape[1])
+    if (X.shape[1] > 1 and X.strides[1] != X.itemsize) or (X.shape[0] > 1 and X.strides[0] % X.itemsize):
+        raise TypeError("expected a row-major 2-D array (rows may be strided)")
+    if X.shape[0] == 1:
+        return X.ctypes.data, X.shape[1]
+        raise TypeError("expected a row-major 2-D array (rows may be strided)")
+    return X.ctypes.data, X.strides[0] // X.itemsize
+
+
+def _operand(ctx, X, dtype):
+    """(pointer, leading dimension, rows, cols) of a host array or DeviceMatrix."""
+    if isinstance(X, DeviceMatrix):
+        if ctx._ptr_mode != PTR_DEVICE:
+            ctx.set_ptr_mode(PTR_DEVICE)
+        return X.ptr, X.ld, X.rows, X.cols
+    if ctx._ptr_mode != PTR_HOST:
+        ctx.set_ptr_mode(PTR_HOST)
+    ptr, ld = _host_ptr(X, dtype)
+    return ptr, ld, X.shape[0], X.shape[1]
+
+
+class BLAS3:
+    """BLAS3 front door (lela/blas/level3.h) for Z/p on the HIP module."""
+
+    @staticmethod
+    def gemm(ctx, a, A, B, b, C, algo=ALGO_AUTO):
+        """C <- a*A*B + b*C (lela/blas/level3.h:81-83)."""
+        F = ctx.F
+        pa, lda, m, k = _operand(ctx, A, F.dtype)
+        pb, ldb, kb, n = _operand(ctx, B, F.dtype)
+        pc, ldc, mc, nc = _operand(ctx, C, F.dtype)
+        if kb != k or mc != m or nc != n:
+            raise LELAError("gemm: dimension mismatch")
+        rc = ctx.lib.mlz_zp_gemm(ctx.handle, F.elem, F.p, m, n, k, int(a) % F.p, pa, lda, pb, ldb,
+                                 int(b) % F.p, pc, ldc, algo)
+        ctx.check(rc, "gemm")
+        return C
+
+    @staticmethod
+    def trsm(ctx, a, A, B, type_, diagIsOne, algo=ALGO_AUTO):
+        """B <- a*A^-1*B, A triangular (lela/blas/level3.h:120-122)."""
+        F = ctx.F
+        pa, lda, n, n2 = _operand(ctx, A, F.dtype)
+        pb, ldb, nb, nrhs = _operand(ctx, B, F.dtype)
+        if n != n2 or nb != n:
+            raise LELAError("trsm: dimension mismatch")
+        bad = ctypes.c_size_t(0)
+        rc = ctx.lib.mlz_zp_trsm(ctx.handle, F.elem, F.p, int(type_ == UpperTriangular),
+                                 int(bool(diagIsOne)), n, nrhs, int(a) % F.p, pa, lda, pb, ldb,
+                                 ctypes.byref(bad), algo)
+        ctx.check(rc, "trsm", idx=bad.value if rc == _lib.MLZ_E_NOT_INVERTIBLE else None)
+        return B

@@ -1,0 +1,73 @@
+"""ctypes binding of liblela_amd.so, the C ABI declared in include/mlz.h.
+
+The library is the only compute path: if it is missing this module raises
+instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblela_amd.so")
+
+MLZ_U16, MLZ_U32, MLZ_F64_SYM = 1, 2, 3
+ALGO_AUTO, ALGO_PLAIN, ALGO_INT8, ALGO_FP64 = 0, 1, 2, 3
+PTR_HOST, PTR_DEVICE = 0, 1
+
+MLZ_OK = 0
+MLZ_E_ARG = 1
+MLZ_E_HIP = 2
+MLZ_E_NOT_INVERTIBLE = 3
+MLZ_E_WRONG_FORM = 4
+MLZ_E_NOMEM = 5
+MLZ_E_UNSUPPORTED = 6
+
+_c_size = ctypes.c_size_t
+_c_u64 = ctypes.c_uint64
+_c_int = ctypes.c_int
+_vp = ctypes.c_void_p
+_ctx = ctypes.c_void_p
+
+# name -> (restype, argtypes); every function declared in include/mlz.h.
+SIGNATURES = {
+    "mlz_ctx_create": (_c_int, [ctypes.POINTER(_ctx), _c_int, ctypes.c_char_p]),
+    "mlz_ctx_destroy": (_c_int, [_ctx]),
+    "mlz_ctx_set_ptr_mode": (_c_int, [_ctx, _c_int]),
+    "mlz_ctx_synchronize": (_c_int, [_ctx]),
+    "mlz_last_error": (ctypes.c_char_p, [_ctx]),
+    "mlz_version": (ctypes.c_char_p, []),
+    "mlz_dev_alloc": (_c_int, [_ctx, _c_size, ctypes.POINTER(_vp)]),
+    "mlz_dev_free": (_c_int, [_ctx, _vp]),
+    "mlz_memcpy_htod": (_c_int, [_ctx, _vp, _vp, _c_size]),
+    "mlz_memcpy_dtoh": (_c_int, [_ctx, _vp, _vp, _c_size]),
+    "mlz_ctx_trim": (_c_int, [_ctx]),
+    "mlz_prof_enable": (_c_int, [_ctx, _c_int]),
+    "mlz_prof_reset": (_c_int, [_ctx]),
+    "mlz_prof_query": (_c_int, [_ctx, ctypes.c_char_p, ctypes.POINTER(_c_u64),
+                                ctypes.POINTER(ctypes.c_double)]),
+    "mlz_timer_start": (_c_int, [_ctx]),
+    "mlz_timer_stop": (_c_int, [_ctx, ctypes.POINTER(ctypes.c_double)]),
+    "mlz_zp_gemm": (_c_int, [_ctx, _c_int, _c_u64, _c_size, _c_size, _c_size, _c_u64,
+                             _vp, _c_size, _vp, _c_size, _c_u64, _vp, _c_size, _c_int]),
+    "mlz_zp_trsm": (_c_int, [_ctx, _c_int, _c_u64, _c_int, _c_int, _c_size, _c_size,
+                             _c_u64, _vp, _c_size, _vp, _c_size, ctypes.POINTER(_c_size), _c_int]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads liblela_amd.so once; raises RuntimeError when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build the HIP library first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

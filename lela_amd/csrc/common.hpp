@@ -1,0 +1,210 @@
+// lela_amd/csrc/common.hpp -- context, workspace, error handling and modular
+// arithmetic shared by every kernel file of liblela_amd.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mlz.h"
+
+namespace mlz {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Barrett reduction modulo p < 2^32.
+// mu = floor((2^64-1)/p); for x < 2^64, q = hi64(x*mu) satisfies
+// x/p - 2.5 < q <= x/p, so r = x - q*p lies in [0, 3p) and two conditional
+// subtractions bring it to [0, p).
+// ---------------------------------------------------------------------------
+struct ModP {
+	uint64_t p;
+	uint64_t mu;
+
+	__host__ __device__ inline uint64_t reduce(uint64_t x) const
+	{
+#if defined(__HIP_DEVICE_COMPILE__)
+		uint64_t q = __umul64hi(x, mu);
+#else
+		uint64_t q = (uint64_t) (((unsigned __int128) x * mu) >> 64);
+#endif
+		uint64_t r = x - q * p;
+		r = r >= p ? r - p : r;
+		r = r >= p ? r - p : r;
+		return r;
+	}
+	// (x*y) mod p for x, y < p < 2^32.
+	__host__ __device__ inline uint32_t mul(uint32_t x, uint32_t y) const
+	{
+		return (uint32_t) reduce((uint64_t) x * y);
+	}
+	// (a*x + y) mod p for a, x, y < p < 2^32 (a*x + y < 2^64).
+	__host__ __device__ inline uint32_t axpy(uint32_t a, uint32_t x, uint32_t y) const
+	{
+		return (uint32_t) reduce((uint64_t) a * x + y);
+	}
+	__host__ __device__ inline uint32_t add(uint32_t x, uint32_t y) const
+	{
+		uint64_t s = (uint64_t) x + y;
+		return (uint32_t) (s >= p ? s - p : s);
+	}
+	__host__ __device__ inline uint32_t sub(uint32_t x, uint32_t y) const
+	{
+		return x >= y ? x - y : (uint32_t) (x + p - y);
+	}
+	__host__ __device__ inline uint32_t neg(uint32_t x) const
+	{
+		return x == 0 ? 0u : (uint32_t) (p - x);
+	}
+};
+
+inline ModP make_modp(uint64_t p)
+{
+	ModP m;
+	m.p = p;
+	m.mu = ~(uint64_t) 0 / p;
+	return m;
+}
+
+// Element storage <-> canonical uint32 in [0, p).
+template <int E> struct Elem;
+template <> struct Elem<MLZ_U16> {
+	typedef uint16_t T;
+	__device__ static inline uint32_t load(const T *ptr, uint32_t) { return *ptr; }
+	__device__ static inline void store(T *ptr, uint32_t v, uint32_t) { *ptr = (T) v; }
+};
+template <> struct Elem<MLZ_U32> {
+	typedef uint32_t T;
+	__device__ static inline uint32_t load(const T *ptr, uint32_t) { return *ptr; }
+	__device__ static inline void store(T *ptr, uint32_t v, uint32_t) { *ptr = v; }
+};
+template <> struct Elem<MLZ_F64_SYM> {
+	typedef double T;
+	// Symmetric representative s in [-(p-1)/2, (p-1)/2] -> canonical.
+	__device__ static inline uint32_t load(const T *ptr, uint32_t p)
+	{
+		long long s = (long long) *ptr;
+		return (uint32_t) (s < 0 ? s + (long long) p : s);
+	}
+	// Canonical -> symmetric: ModularTraits<double>::valid_rep keeps
+	// v <= p/2 (lela/ring/modular.h:310-326).
+	__device__ static inline void store(T *ptr, uint32_t v, uint32_t p)
+	{
+		*ptr = (2ull * v > (unsigned long long) p) ? (double) ((long long) v - (long long) p) : (double) v;
+	}
+};
+
+inline size_t elem_size(int e)
+{
+	return e == MLZ_U16 ? 2 : e == MLZ_U32 ? 4 : e == MLZ_F64_SYM ? 8 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Context.
+// ---------------------------------------------------------------------------
+enum WsSlot {
+	WS_STAGE_A, WS_STAGE_B, WS_STAGE_C,      // host-pointer staging
+	WS_PACK_A, WS_PACK_B,                    // gemm operand packing
+	WS_TRSM_0, WS_TRSM_1, WS_TRSM_2,         // trsm temporaries
+	WS_RREF_0, WS_RREF_1, WS_RREF_2, WS_RREF_3, WS_RREF_4, WS_RREF_5,
+	WS_FL_0, WS_FL_1, WS_FL_2, WS_FL_3, WS_FL_4, WS_FL_5, WS_FL_6, WS_FL_7,
+	WS_GF2_0, WS_GF2_1, WS_GF2_2,
+	WS_BATCH_0, WS_BATCH_1, WS_BATCH_2, WS_BATCH_3,
+	WS_SMALL,
+	WS_NUM_SLOTS
+};
+
+struct ProfRec {
+	std::string name;
+	uint64_t count;
+	double ms;
+};
+
+struct PendingEv {
+	const char *name;
+	hipEvent_t e0, e1;
+};
+
+}  // namespace mlz
+
+struct mlz_ctx {
+	int device;
+	hipStream_t stream;
+	int ptr_mode;
+	std::string last_error;
+	void *ws[mlz::WS_NUM_SLOTS];
+	size_t ws_bytes[mlz::WS_NUM_SLOTS];
+	bool prof_on;
+	std::vector<mlz::PendingEv> pending;
+	std::vector<hipEvent_t> free_events;
+	std::vector<mlz::ProfRec> prof;
+	hipEvent_t timer0, timer1;
+	int num_cus;
+	void *host_pinned;
+	size_t host_pinned_bytes;
+};
+
+namespace mlz {
+
+int set_error(mlz_ctx *ctx, int code, const char *fmt, ...);
+
+#define MLZ_HIP(ctx, expr)                                                              \
+	do {                                                                            \
+		hipError_t _e = (expr);                                                 \
+		if (_e != hipSuccess)                                                   \
+			return ::mlz::set_error((ctx), MLZ_E_HIP, "%s:%d %s: %s", __FILE__,     \
+						__LINE__, #expr, hipGetErrorString(_e)); \
+	} while (0)
+
+#define MLZ_TRY(expr)                           \
+	do {                                    \
+		int _rc = (expr);               \
+		if (_rc != MLZ_OK)              \
+			return _rc;             \
+	} while (0)
+
+// Device buffer of at least `bytes` in a workspace slot (grows, never shrinks
+// until mlz_ctx_trim).
+int ws_get(mlz_ctx *ctx, int slot, size_t bytes, void **out);
+
+// Profiling brackets around a kernel launch on ctx->stream.
+void prof_begin(mlz_ctx *ctx, const char *name, PendingEv *ev);
+void prof_end(mlz_ctx *ctx, PendingEv *ev);
+
+struct ProfScope {
+	mlz_ctx *ctx;
+	PendingEv ev;
+	ProfScope(mlz_ctx *c, const char *name) : ctx(c) { prof_begin(ctx, name, &ev); }
+	~ProfScope() { prof_end(ctx, &ev); }
+};
+
+int check_launch(mlz_ctx *ctx, const char *what);
+
+// Internal device-pointer entry points shared between the kernel files.
+// All pointers are device pointers; work is queued on ctx->stream.
+int zp_gemm_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, size_t k,
+		uint64_t a, const void *A, size_t lda, const void *B, size_t ldb,
+		uint64_t b, void *C, size_t ldc, int algo);
+int zp_scal_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, uint64_t a,
+		void *C, size_t ldc);
+int pick_gemm_algo(uint64_t p, int elem, size_t m, size_t n, size_t k, int algo);
+
+// Host <-> device staging of a strided matrix (rows x cols elements of esize
+// bytes, leading dimension ld elements) into a dense device buffer with
+// leading dimension cols.
+int stage_in(mlz_ctx *ctx, int slot, const void *host, size_t rows, size_t cols, size_t ld,
+	     size_t esize, void **dev);
+int stage_out(mlz_ctx *ctx, void *host, const void *dev, size_t rows, size_t cols, size_t ld,
+	      size_t esize);
+
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned) ((a + b - 1) / b); }
+inline size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
+
+}  // namespace mlz

@@ -1,0 +1,681 @@
+// lela_amd/csrc/zp_gemm.hip -- C <- a*A*B + b*C over Z/p on MI355X.
+//
+// Three exact kernel families (results are identical; only speed differs):
+//
+//  * INT8 (p <= 2^16).  Every residue v has a representative x in
+//    [-32896, 32639] (v itself if v <= 32639, else v - p), and x + 32896 is
+//    an unsigned 16-bit number u, so x = 256*(hi(u)-128) + (lo(u)-128) with
+//    both limbs in [-128, 127].  A*B = 65536*HH + 256*(HL+LH) + LL where each
+//    of HH, HL+LH, LL is an int8 x int8 -> int32 product on
+//    v_mfma_i32_32x32x32_i8.  |HL+LH| <= 2*16384*k, so k <= 65535 keeps every
+//    int32 partial sum exact; longer k is split into chunks that accumulate
+//    through C.  Limbs are packed once per call into MFMA-fragment order so
+//    that every global and LDS access is a contiguous 16-byte-per-lane load.
+//  * PLAIN.  VALU uint64 multiply-accumulate (v_mad_u64_u32) over LDS tiles
+//    with a single reduction per entry -- the delayed reduction of
+//    ZpModule's dot (lela/blas/level1-modular.tcc:25-63); for p^2*k >= 2^64
+//    the 2^64-wrap correction of lela/blas/level1-modular.tcc:135-158.
+//  * FP64 (p < 2^26).  Symmetric A entries times B split into two centred
+//    13-bit limbs, accumulated exactly on v_mfma_f64_16x16x4_f64 while the
+//    partial sums stay below 2^53 (the role of block_size in
+//    lela/ring/modular.h:661-669 and lela/blas/level3-modular.tcc:97-119).
+//
+// C update per entry follows lela/blas/level3-generic.tcc:309-311:
+// c <- b*c + a*d mod p, with d = (A*B)_{ij} mod p.
+#include "common.hpp"
+
+namespace mlz {
+
+// ===========================================================================
+// INT8 path
+// ===========================================================================
+
+// Fragment geometry for v_mfma_i32_32x32x32_i8: a fragment is 32 indices (rows
+// of A or columns of B) x 32 k-values, 16 bytes per lane; lane l holds index
+// l%32 and k-values 16*(l/32) .. 16*(l/32)+15.  A and B use the same
+// (lane, byte) -> k assignment, so the hardware's k order is irrelevant.
+// Packed operand layout: [iblock][kblock][limb(hi, lo)][64 lanes][16 bytes].
+constexpr int I8_FRAG = 1024;
+constexpr int I8_KMAX = 65504;   // multiple of 64, <= 65535
+
+__device__ inline uint32_t i8_limbs(uint32_t v, uint32_t p)
+{
+	// u = x + 32896 with x the representative in [-32896, 32639]; then the
+	// bytes of u ^ 0x8080 are the signed limbs (hi, lo).
+	int32_t x = v > 32639u ? (int32_t) v - (int32_t) p : (int32_t) v;
+	return (uint32_t) (x + 32896) ^ 0x8080u;
+}
+
+// Packs a 64 (index) x 64 (k) tile into 2 x 2 fragment pairs.
+// TRANS = false: element(i, kk) = src[i*ld + kk]   (A, row-major m x k)
+// TRANS = true : element(i, kk) = src[kk*ld + i]   (B, row-major k x n)
+template <int E, bool TRANS>
+__global__ __launch_bounds__(256) void pack_i8_kernel(const typename Elem<E>::T *__restrict__ src,
+						      size_t ld, int idim, int kdim, uint32_t p,
+						      int8_t *__restrict__ dst, int KB)
+{
+	__shared__ uint32_t tile[64][65];
+	const int tid = threadIdx.x;
+	const int i0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+#pragma unroll
+	for (int e = 0; e < 16; ++e) {
+		int idx = e * 256 + tid;
+		int i, kk;
+		if (!TRANS) { i = idx >> 6; kk = idx & 63; }
+		else        { kk = idx >> 6; i = idx & 63; }
+		int gi = i0 + i, gk = k0 + kk;
+		uint32_t v = 0;
+		if (gi < idim && gk < kdim) {
+			const typename Elem<E>::T *ptr = TRANS ? src + (size_t) gk * ld + gi : src + (size_t) gi * ld + gk;
+			v = Elem<E>::load(ptr, p);
+		}
+		tile[i][kk] = i8_limbs(v, p);
+	}
+	__syncthreads();
+	const int f = tid >> 6, lane = tid & 63;
+	const int isub = f >> 1, ksub = f & 1;
+	const int i = isub * 32 + (lane & 31);
+	const int kk0 = ksub * 32 + 16 * (lane >> 5);
+	uint32_t hw[4], lw[4];
+#pragma unroll
+	for (int q = 0; q < 4; ++q) {
+		uint32_t h = 0, l = 0;
+#pragma unroll
+		for (int s = 0; s < 4; ++s) {
+			uint32_t w = tile[i][kk0 + q * 4 + s];
+			h |= ((w >> 8) & 0xffu) << (8 * s);
+			l |= (w & 0xffu) << (8 * s);
+		}
+		hw[q] = h;
+		lw[q] = l;
+	}
+	const size_t ib = (size_t) (i0 / 32 + isub), kb = (size_t) (k0 / 32 + ksub);
+	int8_t *fr = dst + ((ib * KB + kb) * 2) * I8_FRAG + lane * 16;
+	*(v4i *) fr = v4i{(int) hw[0], (int) hw[1], (int) hw[2], (int) hw[3]};
+	*(v4i *) (fr + I8_FRAG) = v4i{(int) lw[0], (int) lw[1], (int) lw[2], (int) lw[3]};
+}
+
+// Maps a linear workgroup id to an output tile.  Workgroups are dispatched
+// round-robin over the 8 XCDs; the remap gives each XCD a contiguous range
+// of tiles (its own L2 working set), then tiles are visited in groups of
+// GM tile-rows so concurrently running workgroups share A and B panels.
+__device__ inline void tile_of(int bid, int tiles_m, int tiles_n, int &tm, int &tn)
+{
+	const int total = tiles_m * tiles_n;
+	if ((total & 7) == 0)
+		bid = (bid & 7) * (total >> 3) + (bid >> 3);
+	const int GM = 8;
+	const int per_group = GM * tiles_n;
+	const int group = bid / per_group;
+	const int first_m = group * GM;
+	const int gsz = min(tiles_m - first_m, GM);
+	const int in_group = bid - group * per_group;
+	tm = first_m + in_group % gsz;
+	tn = in_group / gsz;
+}
+
+// 128 x 128 output tile per workgroup, 4 waves in 2 x 2, each wave 64 x 64
+// (2 x 2 MFMA blocks of 32 x 32) with three int32 accumulators (HH, HL+LH,
+// LL).  The k loop stages 64 k-values (2 k-blocks x 2 limbs) of A and B per
+// step through double-buffered LDS.
+constexpr int I8_BM = 128, I8_BN = 128;
+
+template <int E>
+__global__ __launch_bounds__(256) void gemm_i8_kernel(const int8_t *__restrict__ Ap,
+						      const int8_t *__restrict__ Bp, int KB,
+						      typename Elem<E>::T *__restrict__ C, size_t ldc,
+						      int m, int n, uint32_t a, uint32_t b, ModP mp,
+						      uint64_t off, int tiles_m, int tiles_n)
+{
+	// [stage][operand][4 index-blocks x 2 k-blocks x 2 limbs x 1024 bytes]
+	__shared__ __attribute__((aligned(16))) int8_t lds[2][2][4 * 4 * I8_FRAG];
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int wr = wave >> 1, wc = wave & 1;
+	int tm, tn;
+	tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+
+	const size_t panel = (size_t) KB * 2 * I8_FRAG;   // bytes per 32-index block
+	const int8_t *Ag = Ap + (size_t) (tm * 4) * panel + tid * 16;
+	const int8_t *Bg = Bp + (size_t) (tn * 4) * panel + tid * 16;
+
+	v4i ra[4], rb[4];
+	v16i acc[3][2][2];
+#pragma unroll
+	for (int t = 0; t < 3; ++t)
+#pragma unroll
+		for (int i = 0; i < 2; ++i)
+#pragma unroll
+			for (int j = 0; j < 2; ++j)
+				acc[t][i][j] = v16i{0};
+
+	const int KS = KB / 2;
+#pragma unroll
+	for (int c = 0; c < 4; ++c) {
+		ra[c] = *(const v4i *) (Ag + c * panel);
+		rb[c] = *(const v4i *) (Bg + c * panel);
+	}
+	for (int ks = 0; ks < KS; ++ks) {
+		const int st = ks & 1;
+		int8_t *la = lds[st][0], *lb = lds[st][1];
+#pragma unroll
+		for (int c = 0; c < 4; ++c) {
+			*(v4i *) (la + c * 4096 + tid * 16) = ra[c];
+			*(v4i *) (lb + c * 4096 + tid * 16) = rb[c];
+		}
+		__syncthreads();
+		if (ks + 1 < KS) {
+			const size_t o = (size_t) (ks + 1) * 4096;
+#pragma unroll
+			for (int c = 0; c < 4; ++c) {
+				ra[c] = *(const v4i *) (Ag + c * panel + o);
+				rb[c] = *(const v4i *) (Bg + c * panel + o);
+			}
+		}
+#pragma unroll
+		for (int kk = 0; kk < 2; ++kk) {
+			v4i fa[2][2], fb[2][2];
+#pragma unroll
+			for (int i = 0; i < 2; ++i)
+#pragma unroll
+				for (int l = 0; l < 2; ++l) {
+					fa[i][l] = *(const v4i *) (la + (wr * 2 + i) * 4096 + kk * 2048 + l * 1024 + lane * 16);
+					fb[i][l] = *(const v4i *) (lb + (wc * 2 + i) * 4096 + kk * 2048 + l * 1024 + lane * 16);
+				}
+#pragma unroll
+			for (int i = 0; i < 2; ++i)
+#pragma unroll
+				for (int j = 0; j < 2; ++j)
+					acc[0][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][0], fb[j][0], acc[0][i][j], 0, 0, 0);
+#pragma unroll
+			for (int i = 0; i < 2; ++i)
+#pragma unroll
+				for (int j = 0; j < 2; ++j)
+					acc[1][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][0], fb[j][1], acc[1][i][j], 0, 0, 0);
+#pragma unroll
+			for (int i = 0; i < 2; ++i)
+#pragma unroll
+				for (int j = 0; j < 2; ++j)
+					acc[2][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][1], fb[j][1], acc[2][i][j], 0, 0, 0);
+#pragma unroll
+			for (int i = 0; i < 2; ++i)
+#pragma unroll
+				for (int j = 0; j < 2; ++j)
+					acc[1][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][1], fb[j][0], acc[1][i][j], 0, 0, 0);
+		}
+	}
+
+	// Epilogue.  32x32 accumulator layout: lane l, register r holds
+	// row 8*(r/4) + 4*(l/32) + r%4, column l%32.
+#pragma unroll
+	for (int i = 0; i < 2; ++i)
+#pragma unroll
+		for (int j = 0; j < 2; ++j) {
+			const int col = tn * I8_BN + wc * 64 + j * 32 + (lane & 31);
+#pragma unroll
+			for (int r = 0; r < 16; ++r) {
+				const int row = tm * I8_BM + wr * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+				if (row < m && col < n) {
+					int64_t v = (int64_t) acc[0][i][j][r] * 65536 + (int64_t) acc[1][i][j][r] * 256 +
+						    (int64_t) acc[2][i][j][r];
+					uint32_t d = (uint32_t) mp.reduce((uint64_t) (v + (int64_t) off));
+					typename Elem<E>::T *cp = C + (size_t) row * ldc + col;
+					uint64_t t = (uint64_t) a * d;
+					if (b)
+						t += (uint64_t) b * Elem<E>::load(cp, (uint32_t) mp.p);
+					Elem<E>::store(cp, (uint32_t) mp.reduce(t), (uint32_t) mp.p);
+				}
+			}
+		}
+}
+
+template <int E>
+static int gemm_i8_launch(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, uint32_t a,
+			  const void *A, size_t lda, const void *B, size_t ldb, uint32_t b, void *C,
+			  size_t ldc)
+{
+	typedef typename Elem<E>::T T;
+	const ModP mp = make_modp(p);
+	const size_t mpad = round_up(m, I8_BM), npad = round_up(n, I8_BN);
+	// |v| < 65536*2^31 + 256*2^31 + 2^31 < 2^48; off is a multiple of p above that.
+	const uint64_t off = ((((uint64_t) 1 << 48) + p - 1) / p) * p;
+	for (size_t k0 = 0; k0 < k; k0 += I8_KMAX) {
+		const size_t kc = std::min((size_t) I8_KMAX, k - k0);
+		const size_t kpad = round_up(kc, 64);
+		const int KB = (int) (kpad / 32);
+		void *ap = nullptr, *bp = nullptr;
+		MLZ_TRY(ws_get(ctx, WS_PACK_A, mpad * kpad * 2, &ap));
+		MLZ_TRY(ws_get(ctx, WS_PACK_B, npad * kpad * 2, &bp));
+		{
+			ProfScope ps(ctx, "pack_i8_A");
+			dim3 g(mpad / 64, kpad / 64);
+			pack_i8_kernel<E, false><<<g, 256, 0, ctx->stream>>>((const T *) A + k0, lda, (int) m,
+									      (int) kc, p, (int8_t *) ap, KB);
+		}
+		MLZ_TRY(check_launch(ctx, "pack_i8_kernel(A)"));
+		{
+			ProfScope ps(ctx, "pack_i8_B");
+			dim3 g(npad / 64, kpad / 64);
+			pack_i8_kernel<E, true><<<g, 256, 0, ctx->stream>>>((const T *) B + k0 * ldb, ldb, (int) n,
+									     (int) kc, p, (int8_t *) bp, KB);
+		}
+		MLZ_TRY(check_launch(ctx, "pack_i8_kernel(B)"));
+		const int tiles_m = (int) (mpad / I8_BM), tiles_n = (int) (npad / I8_BN);
+		{
+			ProfScope ps(ctx, "gemm_i8_mfma");
+			gemm_i8_kernel<E><<<tiles_m * tiles_n, 256, 0, ctx->stream>>>(
+				(const int8_t *) ap, (const int8_t *) bp, KB, (T *) C, ldc, (int) m, (int) n, a,
+				k0 == 0 ? b : 1u, mp, off, tiles_m, tiles_n);
+		}
+		MLZ_TRY(check_launch(ctx, "gemm_i8_kernel"));
+	}
+	return MLZ_OK;
+}
+
+// ===========================================================================
+// PLAIN path
+// ===========================================================================
+
+// 64 x 64 output tile, 16 x 16 threads, 4 x 4 entries per thread, k staged
+// 16 at a time through LDS as canonical uint32.
+template <int E, bool WIDE>
+__global__ __launch_bounds__(256) void gemm_plain_kernel(const typename Elem<E>::T *__restrict__ A, size_t lda,
+							 const typename Elem<E>::T *__restrict__ B, size_t ldb,
+							 typename Elem<E>::T *__restrict__ C, size_t ldc,
+							 int m, int n, int k, uint32_t a, uint32_t b, ModP mp,
+							 uint64_t two_64)
+{
+	__shared__ __attribute__((aligned(16))) uint32_t As[16][64];
+	__shared__ __attribute__((aligned(16))) uint32_t Bs[16][64];
+	const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+	const int row0 = blockIdx.y * 64, col0 = blockIdx.x * 64;
+	const uint32_t p = (uint32_t) mp.p;
+	uint64_t acc[4][4];
+#pragma unroll
+	for (int r = 0; r < 4; ++r)
+#pragma unroll
+		for (int c = 0; c < 4; ++c)
+			acc[r][c] = 0;
+	for (int k0 = 0; k0 < k; k0 += 16) {
+		// A tile 64 x 16: thread loads 4 entries, k fastest for coalescing.
+#pragma unroll
+		for (int e = 0; e < 4; ++e) {
+			int idx = e * 256 + threadIdx.x;
+			int r = idx >> 4, kk = idx & 15;
+			int gr = row0 + r, gk = k0 + kk;
+			As[kk][r] = (gr < m && gk < k) ? Elem<E>::load(A + (size_t) gr * lda + gk, p) : 0u;
+		}
+#pragma unroll
+		for (int e = 0; e < 4; ++e) {
+			int idx = e * 256 + threadIdx.x;
+			int kk = idx >> 6, c = idx & 63;
+			int gk = k0 + kk, gc = col0 + c;
+			Bs[kk][c] = (gk < k && gc < n) ? Elem<E>::load(B + (size_t) gk * ldb + gc, p) : 0u;
+		}
+		__syncthreads();
+#pragma unroll
+		for (int kk = 0; kk < 16; ++kk) {
+			uint4 av = *(const uint4 *) &As[kk][ty * 4];
+			uint4 bv = *(const uint4 *) &Bs[kk][tx * 4];
+			uint32_t ar[4] = {av.x, av.y, av.z, av.w};
+			uint32_t br[4] = {bv.x, bv.y, bv.z, bv.w};
+#pragma unroll
+			for (int r = 0; r < 4; ++r)
+#pragma unroll
+				for (int c = 0; c < 4; ++c) {
+					uint64_t t = (uint64_t) ar[r] * br[c];
+					if (WIDE) {
+						acc[r][c] += t;
+						if (acc[r][c] < t)
+							acc[r][c] += two_64;
+					} else {
+						acc[r][c] += t;
+					}
+				}
+		}
+		__syncthreads();
+	}
+#pragma unroll
+	for (int r = 0; r < 4; ++r) {
+		const int row = row0 + ty * 4 + r;
+		if (row >= m)
+			continue;
+#pragma unroll
+		for (int c = 0; c < 4; ++c) {
+			const int col = col0 + tx * 4 + c;
+			if (col >= n)
+				continue;
+			uint32_t d = (uint32_t) mp.reduce(acc[r][c]);
+			typename Elem<E>::T *cp = C + (size_t) row * ldc + col;
+			uint32_t cij = b ? mp.mul(Elem<E>::load(cp, p), b) : 0u;
+			Elem<E>::store(cp, mp.axpy(a, d, cij), p);
+		}
+	}
+}
+
+template <int E>
+static int gemm_plain_launch(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, uint32_t a,
+			     const void *A, size_t lda, const void *B, size_t ldb, uint32_t b, void *C,
+			     size_t ldc)
+{
+	typedef typename Elem<E>::T T;
+	const ModP mp = make_modp(p);
+	uint64_t t = 2;
+	for (int i = 0; i < 6; ++i)
+		t = (t * t) % p;  // 2^64 mod p (lela/ring/modular.h:616-625)
+	const unsigned __int128 bound = (unsigned __int128) (p - 1) * (p - 1) * (unsigned __int128) k;
+	const bool wide = bound >= ((unsigned __int128) 1 << 64);
+	dim3 g(ceil_div(n, 64), ceil_div(m, 64));
+	ProfScope ps(ctx, "gemm_plain");
+	if (wide)
+		gemm_plain_kernel<E, true><<<g, 256, 0, ctx->stream>>>((const T *) A, lda, (const T *) B, ldb,
+								       (T *) C, ldc, (int) m, (int) n, (int) k,
+								       a, b, mp, t);
+	else
+		gemm_plain_kernel<E, false><<<g, 256, 0, ctx->stream>>>((const T *) A, lda, (const T *) B, ldb,
+									(T *) C, ldc, (int) m, (int) n, (int) k,
+									a, b, mp, t);
+	return check_launch(ctx, "gemm_plain_kernel");
+}
+
+// ===========================================================================
+// FP64 path
+// ===========================================================================
+
+// v_mfma_f64_16x16x4_f64 fragments: A fragment 16 rows x 4 k, lane l holds
+// row l%16, k l/16; B fragment 4 k x 16 columns, lane l holds column l%16,
+// k l/16.  The 16 x 16 result: lane l, register r holds row l/16 + 4*r,
+// column l%16 (measured on gfx950).  Packed A: [rblock16][kblock4][64]
+// doubles (symmetric residues); packed B: [cblock16][kblock4][limb(hi, lo)][64].
+constexpr int F64_LIMB_BITS = 13;
+
+__device__ inline int32_t sym_of(uint32_t v, uint32_t p)
+{
+	return (2ull * v > (unsigned long long) p) ? (int32_t) ((int64_t) v - p) : (int32_t) v;
+}
+
+template <int E>
+__global__ __launch_bounds__(256) void pack_f64_a_kernel(const typename Elem<E>::T *__restrict__ src, size_t ld,
+							 int rows, int kdim, uint32_t p, double *__restrict__ dst,
+							 int KB4, int total)
+{
+	int t = blockIdx.x * 256 + threadIdx.x;
+	if (t >= total)
+		return;
+	// t enumerates (rb, kb, lane) in destination order.
+	int lane = t & 63, rest = t >> 6;
+	int kb = rest % KB4, rb = rest / KB4;
+	int row = rb * 16 + (lane & 15), kk = kb * 4 + (lane >> 4);
+	double v = 0.0;
+	if (row < rows && kk < kdim)
+		v = (double) sym_of(Elem<E>::load(src + (size_t) row * ld + kk, p), p);
+	dst[t] = v;
+}
+
+template <int E>
+__global__ __launch_bounds__(256) void pack_f64_b_kernel(const typename Elem<E>::T *__restrict__ src, size_t ld,
+							 int cols, int kdim, uint32_t p, double *__restrict__ dst,
+							 int KB4, int total)
+{
+	int t = blockIdx.x * 256 + threadIdx.x;
+	if (t >= total)
+		return;
+	int lane = t & 63, rest = t >> 6;
+	int kb = rest % KB4, cb = rest / KB4;
+	int col = cb * 16 + (lane & 15), kk = kb * 4 + (lane >> 4);
+	int32_t s = 0;
+	if (col < cols && kk < kdim)
+		s = sym_of(Elem<E>::load(src + (size_t) kk * ld + col, p), p);
+	// Centred split s = hi*2^13 + lo, lo in [-4096, 4095].
+	int32_t lo = ((s & 8191) ^ 4096) - 4096;
+	int32_t hi = (s - lo) >> F64_LIMB_BITS;
+	size_t base = ((size_t) cb * KB4 + kb) * 128 + lane;
+	dst[base] = (double) hi;
+	dst[base + 64] = (double) lo;
+}
+
+// Workgroup: 4 waves, each computing a 32 x 64 tile (2 x 4 blocks of 16 x 16,
+// two limbs).  Waves read fragments straight from the packed arrays (L1/L2
+// serve the reuse; the FP64 MFMA rate, not bandwidth, bounds this kernel).
+template <int E>
+__global__ __launch_bounds__(256) void gemm_f64_kernel(const double *__restrict__ Ap, const double *__restrict__ Bp,
+						       int KB4, typename Elem<E>::T *__restrict__ C, size_t ldc,
+						       int m, int n, uint32_t a, uint32_t b, ModP mp, int tiles_m,
+						       int tiles_n)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	int tm, tn;
+	tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+	// Workgroup tile 64 x 128: waves 2 x 2 of 32 x 64.
+	const int wr = wave >> 1, wc = wave & 1;
+	const int rb0 = tm * 4 + wr * 2;      // 16-row blocks
+	const int cb0 = tn * 8 + wc * 4;      // 16-column blocks
+	const double *Ag = Ap + (size_t) rb0 * KB4 * 64 + lane;
+	const double *Bg = Bp + (size_t) cb0 * KB4 * 128 + lane;
+	v4d acc[2][4][2];
+#pragma unroll
+	for (int i = 0; i < 2; ++i)
+#pragma unroll
+		for (int j = 0; j < 4; ++j)
+#pragma unroll
+			for (int l = 0; l < 2; ++l)
+				acc[i][j][l] = v4d{0.0, 0.0, 0.0, 0.0};
+	for (int kb = 0; kb < KB4; ++kb) {
+		double fa[2], fb[4][2];
+#pragma unroll
+		for (int i = 0; i < 2; ++i)
+			fa[i] = Ag[((size_t) i * KB4 + kb) * 64];
+#pragma unroll
+		for (int j = 0; j < 4; ++j) {
+			fb[j][0] = Bg[((size_t) j * KB4 + kb) * 128];
+			fb[j][1] = Bg[((size_t) j * KB4 + kb) * 128 + 64];
+		}
+#pragma unroll
+		for (int i = 0; i < 2; ++i)
+#pragma unroll
+			for (int j = 0; j < 4; ++j)
+#pragma unroll
+				for (int l = 0; l < 2; ++l)
+					acc[i][j][l] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j][l], acc[i][j][l], 0, 0, 0);
+	}
+	const uint32_t p = (uint32_t) mp.p;
+	// |partial sums| < 2^53; shift by a multiple of p before reducing.
+	const uint64_t off = ((((uint64_t) 1 << 53) + p - 1) / p) * p;
+#pragma unroll
+	for (int i = 0; i < 2; ++i)
+#pragma unroll
+		for (int j = 0; j < 4; ++j) {
+			const int col = (cb0 + j) * 16 + (lane & 15);
+#pragma unroll
+			for (int r = 0; r < 4; ++r) {
+				const int row = (rb0 + i) * 16 + (lane >> 4) + 4 * r;
+				if (row < m && col < n) {
+					int64_t hi = (int64_t) acc[i][j][0][r], lo = (int64_t) acc[i][j][1][r];
+					uint64_t h = mp.reduce((uint64_t) (hi + (int64_t) off));
+					uint64_t l = mp.reduce((uint64_t) (lo + (int64_t) off));
+					uint32_t d = (uint32_t) mp.reduce((h << F64_LIMB_BITS) + l);
+					typename Elem<E>::T *cp = C + (size_t) row * ldc + col;
+					uint32_t cij = b ? mp.mul(Elem<E>::load(cp, p), b) : 0u;
+					Elem<E>::store(cp, mp.axpy(a, d, cij), p);
+				}
+			}
+		}
+}
+
+template <int E>
+static int gemm_f64_launch(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, uint32_t a,
+			   const void *A, size_t lda, const void *B, size_t ldb, uint32_t b, void *C,
+			   size_t ldc)
+{
+	typedef typename Elem<E>::T T;
+	const ModP mp = make_modp(p);
+	// Exactness: |a| <= (p-1)/2, |limb| <= 4097, so each product is below
+	// amax*4097 and kchunk products sum below 2^53.
+	const uint64_t amax = (p - 1) / 2 + 1;
+	size_t kchunk = (size_t) ((((uint64_t) 1 << 53) - 1) / (amax * 4097ull));
+	kchunk = kchunk / 4 * 4;
+	if (kchunk < 4)
+		return set_error(ctx, MLZ_E_UNSUPPORTED, "FP64 carrier needs p < 2^26 (p=%u)", p);
+	const size_t mpad = round_up(m, 64), npad = round_up(n, 128);
+	for (size_t k0 = 0; k0 < k; k0 += kchunk) {
+		const size_t kc = std::min(kchunk, k - k0);
+		const int KB4 = (int) ceil_div(kc, 4);
+		void *ap = nullptr, *bp = nullptr;
+		MLZ_TRY(ws_get(ctx, WS_PACK_A, mpad * KB4 * 4 * sizeof(double), &ap));
+		MLZ_TRY(ws_get(ctx, WS_PACK_B, npad * KB4 * 4 * 2 * sizeof(double), &bp));
+		{
+			ProfScope ps(ctx, "pack_f64_A");
+			int total = (int) (mpad / 16 * KB4 * 64);
+			pack_f64_a_kernel<E><<<ceil_div(total, 256), 256, 0, ctx->stream>>>(
+				(const T *) A + k0, lda, (int) m, (int) kc, p, (double *) ap, KB4, total);
+		}
+		MLZ_TRY(check_launch(ctx, "pack_f64_a_kernel"));
+		{
+			ProfScope ps(ctx, "pack_f64_B");
+			int total = (int) (npad / 16 * KB4 * 64);
+			pack_f64_b_kernel<E><<<ceil_div(total, 256), 256, 0, ctx->stream>>>(
+				(const T *) B + k0 * ldb, ldb, (int) n, (int) kc, p, (double *) bp, KB4, total);
+		}
+		MLZ_TRY(check_launch(ctx, "pack_f64_b_kernel"));
+		const int tiles_m = (int) (mpad / 64), tiles_n = (int) (npad / 128);
+		{
+			ProfScope ps(ctx, "gemm_f64_mfma");
+			gemm_f64_kernel<E><<<tiles_m * tiles_n, 256, 0, ctx->stream>>>(
+				(const double *) ap, (const double *) bp, KB4, (T *) C, ldc, (int) m, (int) n, a,
+				k0 == 0 ? b : 1u, mp, tiles_m, tiles_n);
+		}
+		MLZ_TRY(check_launch(ctx, "gemm_f64_kernel"));
+	}
+	return MLZ_OK;
+}
+
+// ===========================================================================
+// scal and dispatch
+// ===========================================================================
+
+template <int E>
+__global__ void scal_kernel(typename Elem<E>::T *C, size_t ldc, int m, int n, uint32_t a, ModP mp)
+{
+	size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= (size_t) m * n)
+		return;
+	int r = (int) (idx / n), c = (int) (idx % n);
+	typename Elem<E>::T *cp = C + (size_t) r * ldc + c;
+	Elem<E>::store(cp, a ? mp.mul(Elem<E>::load(cp, (uint32_t) mp.p), a) : 0u, (uint32_t) mp.p);
+}
+
+int zp_scal_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, uint64_t a, void *C, size_t ldc)
+{
+	if (m == 0 || n == 0)
+		return MLZ_OK;
+	const ModP mp = make_modp(p);
+	unsigned g = ceil_div(m * n, 256);
+	if (elem == MLZ_U16)
+		scal_kernel<MLZ_U16><<<g, 256, 0, ctx->stream>>>((uint16_t *) C, ldc, (int) m, (int) n, (uint32_t) a, mp);
+	else if (elem == MLZ_U32)
+		scal_kernel<MLZ_U32><<<g, 256, 0, ctx->stream>>>((uint32_t *) C, ldc, (int) m, (int) n, (uint32_t) a, mp);
+	else
+		scal_kernel<MLZ_F64_SYM><<<g, 256, 0, ctx->stream>>>((double *) C, ldc, (int) m, (int) n, (uint32_t) a, mp);
+	return check_launch(ctx, "scal_kernel");
+}
+
+int pick_gemm_algo(uint64_t p, int elem, size_t m, size_t n, size_t k, int algo)
+{
+	(void) elem;
+	if (algo != MLZ_ALGO_AUTO)
+		return algo;
+	const size_t work = m * n * k;
+	if (p <= 65536 && work >= ((size_t) 1 << 18))
+		return MLZ_ALGO_INT8;
+	return MLZ_ALGO_PLAIN;
+}
+
+template <int E>
+static int gemm_dispatch(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, uint32_t a,
+			 const void *A, size_t lda, const void *B, size_t ldb, uint32_t b, void *C,
+			 size_t ldc, int algo)
+{
+	switch (algo) {
+	case MLZ_ALGO_INT8:
+		return gemm_i8_launch<E>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
+	case MLZ_ALGO_FP64:
+		return gemm_f64_launch<E>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
+	case MLZ_ALGO_PLAIN:
+		return gemm_plain_launch<E>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
+	default:
+		return set_error(ctx, MLZ_E_ARG, "unknown gemm algo %d", algo);
+	}
+}
+
+static int check_modulus(mlz_ctx *ctx, int elem, uint64_t p)
+{
+	if (p < 2)
+		return set_error(ctx, MLZ_E_ARG, "modulus %llu < 2", (unsigned long long) p);
+	if (elem == MLZ_U16 && p > 65535)
+		return set_error(ctx, MLZ_E_ARG, "Modular<uint16> needs p < 2^16 (ModularTraits<uint16>::valid_modulus)");
+	if (elem == MLZ_U32 && p > 0xffffffffull)
+		return set_error(ctx, MLZ_E_ARG, "Modular<uint32> needs p < 2^32");
+	if (elem == MLZ_F64_SYM && p >= 67108864ull)  // ModularTraits<double>::valid_modulus
+		return set_error(ctx, MLZ_E_ARG, "Modular<double> needs p < 2^26");
+	if (elem != MLZ_U16 && elem != MLZ_U32 && elem != MLZ_F64_SYM)
+		return set_error(ctx, MLZ_E_ARG, "unknown element type %d", elem);
+	return MLZ_OK;
+}
+
+int zp_gemm_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, size_t k, uint64_t a,
+		const void *A, size_t lda, const void *B, size_t ldb, uint64_t b, void *C, size_t ldc,
+		int algo)
+{
+	MLZ_TRY(check_modulus(ctx, elem, p));
+	if (m == 0 || n == 0)
+		return MLZ_OK;
+	a %= p;
+	b %= p;
+	if (k == 0 || a == 0)
+		return zp_scal_dev(ctx, elem, p, m, n, b, C, ldc);
+	if (m > (size_t) INT32_MAX || n > (size_t) INT32_MAX || k > (size_t) INT32_MAX)
+		return set_error(ctx, MLZ_E_ARG, "dimension too large");
+	algo = pick_gemm_algo(p, elem, m, n, k, algo);
+	if (algo == MLZ_ALGO_INT8 && p > 65536)
+		return set_error(ctx, MLZ_E_UNSUPPORTED, "INT8 limb gemm needs p <= 2^16");
+	if (algo == MLZ_ALGO_FP64 && p >= 67108864ull)
+		return set_error(ctx, MLZ_E_UNSUPPORTED, "FP64 carrier gemm needs p < 2^26");
+	const uint32_t pp = (uint32_t) p, aa = (uint32_t) a, bb = (uint32_t) b;
+	if (elem == MLZ_U16)
+		return gemm_dispatch<MLZ_U16>(ctx, pp, m, n, k, aa, A, lda, B, ldb, bb, C, ldc, algo);
+	if (elem == MLZ_U32)
+		return gemm_dispatch<MLZ_U32>(ctx, pp, m, n, k, aa, A, lda, B, ldb, bb, C, ldc, algo);
+	return gemm_dispatch<MLZ_F64_SYM>(ctx, pp, m, n, k, aa, A, lda, B, ldb, bb, C, ldc, algo);
+}
+
+}  // namespace mlz
+
+using namespace mlz;
+
+extern "C" int mlz_zp_gemm(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t n, size_t k,
+			   uint64_t a, const void *A, size_t lda, const void *B, size_t ldb, uint64_t b,
+			   void *C, size_t ldc, int algo)
+{
+	if (!ctx)
+		return MLZ_E_ARG;
+	MLZ_TRY(check_modulus(ctx, elem, p));
+	if ((m && k && !A) || (k && n && !B) || (m && n && !C))
+		return set_error(ctx, MLZ_E_ARG, "null matrix pointer");
+	if ((m && k && lda < k) || (k && n && ldb < n) || (m && n && ldc < n))
+		return set_error(ctx, MLZ_E_ARG, "leading dimension smaller than row length");
+	if (algo < MLZ_ALGO_AUTO || algo > MLZ_ALGO_FP64)
+		return set_error(ctx, MLZ_E_ARG, "unknown gemm algo %d", algo);
+	MLZ_HIP(ctx, hipSetDevice(ctx->device));
+	if (ctx->ptr_mode == MLZ_PTR_DEVICE)
+		return zp_gemm_dev(ctx, elem, p, m, n, k, a, A, lda, B, ldb, b, C, ldc, algo);
+	if (m == 0 || n == 0)
+		return MLZ_OK;
+	const size_t es = elem_size(elem);
+	void *dA = nullptr, *dB = nullptr, *dC = nullptr;
+	if (k) {
+		MLZ_TRY(stage_in(ctx, WS_STAGE_A, A, m, k, lda, es, &dA));
+		MLZ_TRY(stage_in(ctx, WS_STAGE_B, B, k, n, ldb, es, &dB));
+	}
+	MLZ_TRY(stage_in(ctx, WS_STAGE_C, C, m, n, ldc, es, &dC));
+	MLZ_TRY(zp_gemm_dev(ctx, elem, p, m, n, k, a, dA, k, dB, n, b, dC, n, algo));
+	return stage_out(ctx, C, dC, m, n, ldc, es);
+}

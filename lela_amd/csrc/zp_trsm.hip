@@ -1,0 +1,288 @@
+// lela_amd/csrc/zp_trsm.hip -- B <- a * A^-1 * B over Z/p, A triangular.
+//
+// Follows the recursion of _trsm<Ring, GenericModule::Tag>::trsm_impl
+// (lela/blas/level3-generic.tcc:421-498): split at l, solve one half, update
+// the other with gemm(-1, A21, X1, a, B2) (or A12 for upper), solve the rest.
+// The gemm updates run on the INT8 / FP64 / plain kernels of zp_gemm.hip.
+// The recursion splits at multiples of TRSM_LEAF; all diagonal blocks of that
+// size are inverted up front by one launch, and each leaf is then the small
+// product B_b <- a * inv(A_bb) * B_b.
+//
+// Non-invertible diagonal entries are found up front by one kernel (modular
+// inverse of every diagonal entry); the reported index is the one LELA's
+// recursion throws first: the smallest for lower, the largest for upper
+// (level3-generic.tcc:441-442,461-467,485-491).  B is left untouched then.
+#include "common.hpp"
+
+namespace mlz {
+
+constexpr int TRSM_LEAF = 64;
+
+// Modular inverse by the extended Euclidean algorithm; 0 if gcd(y, p) != 1.
+__device__ inline int inv_mod(uint32_t y, uint32_t p, uint32_t *out)
+{
+	int64_t r0 = y % p, r1 = p, s0 = 1, s1 = 0;
+	while (r1 != 0) {
+		int64_t q = r0 / r1, t;
+		t = r0 - q * r1; r0 = r1; r1 = t;
+		t = s0 - q * s1; s0 = s1; s1 = t;
+	}
+	if (r0 != 1)
+		return 0;
+	s0 %= (int64_t) p;
+	if (s0 < 0)
+		s0 += p;
+	*out = (uint32_t) s0;
+	return 1;
+}
+
+template <int E>
+__global__ void diag_inv_kernel(const typename Elem<E>::T *A, size_t lda, int n, uint32_t p,
+				uint32_t *dinv, unsigned long long *bad /* [min, max] */)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	uint32_t v = Elem<E>::load(A + (size_t) i * lda + i, p), x = 0;
+	if (!inv_mod(v, p, &x)) {
+		atomicMin(&bad[0], (unsigned long long) i);
+		atomicMax(&bad[1], (unsigned long long) i);
+		x = 0;
+	}
+	dinv[i] = x;
+}
+
+__global__ void init_bad_kernel(unsigned long long *bad)
+{
+	bad[0] = ~0ull;
+	bad[1] = 0ull;
+}
+
+// Inverts every diagonal block of TRSM_LEAF rows in one launch (one
+// workgroup per block): X <- A_bb^-1 by substitution on the identity, in LDS.
+// W receives block b's inverse in rows b*L .. b*L+nb-1, L columns each.
+template <int E, bool UPPER, bool UNIT>
+__global__ __launch_bounds__(256) void trsm_diag_block_inv_kernel(const typename Elem<E>::T *__restrict__ A,
+								  size_t lda, int n,
+								  const uint32_t *__restrict__ dinv,
+								  uint32_t *__restrict__ W, ModP mp)
+{
+	constexpr int L = TRSM_LEAF;
+	__shared__ uint32_t Ns[L][L + 1];   // -A strict triangle of the block
+	__shared__ uint32_t X[L][L + 1];
+	const uint32_t p = (uint32_t) mp.p;
+	const int b0 = blockIdx.x * L;
+	const int nb = min(L, n - b0);
+	for (int idx = threadIdx.x; idx < L * L; idx += blockDim.x) {
+		int r = idx / L, c = idx % L;
+		uint32_t v = 0;
+		bool tri = UPPER ? (c > r) : (c < r);
+		if (r < nb && c < nb && tri)
+			v = Elem<E>::load(A + (size_t) (b0 + r) * lda + b0 + c, p);
+		Ns[r][c] = v == 0 ? 0u : p - v;
+		X[r][c] = (r == c) ? 1u : 0u;
+	}
+	__syncthreads();
+	for (int s = 0; s < nb; ++s) {
+		const int i = UPPER ? nb - 1 - s : s;
+		if (!UNIT && threadIdx.x < L)
+			X[i][threadIdx.x] = mp.mul(X[i][threadIdx.x], dinv[b0 + i]);
+		__syncthreads();
+		// rows still to solve: r > i (lower) or r < i (upper)
+		const int cnt = nb - 1 - s;
+		for (int idx = threadIdx.x; idx < cnt * L; idx += blockDim.x) {
+			int rr = idx / L, c = idx % L;
+			int r = UPPER ? rr : i + 1 + rr;
+			X[r][c] = mp.axpy(Ns[r][i], X[i][c], X[r][c]);
+		}
+		__syncthreads();
+	}
+	for (int idx = threadIdx.x; idx < nb * L; idx += blockDim.x) {
+		int r = idx / L, c = idx % L;
+		W[(size_t) (b0 + r) * L + c] = X[r][c];
+	}
+}
+
+// Leaf solve B_b <- a * W_b * B_b for one diagonal block (nb <= L rows), in
+// place: each workgroup owns 64 right-hand-side columns.
+template <int E, bool WIDE>
+__global__ __launch_bounds__(256) void trsm_leaf_apply_kernel(const uint32_t *__restrict__ W,
+							      typename Elem<E>::T *__restrict__ B, size_t ldb,
+							      int nb, int nrhs, uint32_t a, ModP mp,
+							      uint64_t two_64)
+{
+	constexpr int L = TRSM_LEAF;
+	__shared__ uint32_t Ws[L][L + 1];
+	__shared__ uint32_t Bs[L][64 + 1];
+	const uint32_t p = (uint32_t) mp.p;
+	const int j0 = blockIdx.x * 64;
+	for (int idx = threadIdx.x; idx < L * L; idx += blockDim.x) {
+		int r = idx / L, c = idx % L;
+		Ws[r][c] = (r < nb && c < nb) ? W[(size_t) r * L + c] : 0u;
+	}
+	for (int idx = threadIdx.x; idx < L * 64; idx += blockDim.x) {
+		int r = idx / 64, c = idx % 64;
+		Bs[r][c] = (r < nb && j0 + c < nrhs) ? Elem<E>::load(B + (size_t) r * ldb + j0 + c, p) : 0u;
+	}
+	__syncthreads();
+	const int c = threadIdx.x & 63, r0 = threadIdx.x >> 6;
+	for (int q = 0; q < L / 4; ++q) {
+		const int r = r0 + 4 * q;
+		uint64_t acc = 0;
+		for (int t = 0; t < nb; ++t) {
+			uint64_t prod = (uint64_t) Ws[r][t] * Bs[t][c];
+			acc += prod;
+			if (WIDE && acc < prod)
+				acc += two_64;
+		}
+		if (r < nb && j0 + c < nrhs)
+			Elem<E>::store(B + (size_t) r * ldb + j0 + c, mp.mul((uint32_t) mp.reduce(acc), a), p);
+	}
+}
+
+template <int E>
+static int trsm_diag_inverses(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, const void *A,
+			      size_t lda, const uint32_t *dinv, uint32_t *W)
+{
+	typedef typename Elem<E>::T T;
+	const ModP mp = make_modp(p);
+	unsigned g = ceil_div(n, TRSM_LEAF);
+	ProfScope ps(ctx, "trsm_diag_block_inv");
+#define INV(U, N) trsm_diag_block_inv_kernel<E, U, N><<<g, 256, 0, ctx->stream>>>((const T *) A, lda, (int) n, dinv, W, mp)
+	if (upper) { if (unit) INV(true, true); else INV(true, false); }
+	else       { if (unit) INV(false, true); else INV(false, false); }
+#undef INV
+	return check_launch(ctx, "trsm_diag_block_inv_kernel");
+}
+
+template <int E>
+static int trsm_leaf(mlz_ctx *ctx, uint32_t p, size_t nb, size_t nrhs, uint32_t a, const uint32_t *W,
+		     void *B, size_t ldb)
+{
+	typedef typename Elem<E>::T T;
+	const ModP mp = make_modp(p);
+	uint64_t t = 2;
+	for (int i = 0; i < 6; ++i)
+		t = (t * t) % p;  // 2^64 mod p
+	const bool wide = (unsigned __int128) (p - 1) * (p - 1) * TRSM_LEAF >= ((unsigned __int128) 1 << 64);
+	dim3 g(ceil_div(nrhs, 64));
+	ProfScope ps(ctx, "trsm_leaf");
+	if (wide)
+		trsm_leaf_apply_kernel<E, true><<<g, 256, 0, ctx->stream>>>(W, (T *) B, ldb, (int) nb, (int) nrhs, a, mp, t);
+	else
+		trsm_leaf_apply_kernel<E, false><<<g, 256, 0, ctx->stream>>>(W, (T *) B, ldb, (int) nb, (int) nrhs, a, mp, t);
+	return check_launch(ctx, "trsm_leaf_apply_kernel");
+}
+
+template <int E>
+static int trsm_rec(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, size_t nrhs, uint32_t a,
+		    const typename Elem<E>::T *A, size_t lda, typename Elem<E>::T *B, size_t ldb,
+		    const uint32_t *W, int algo)
+{
+	if (n == 0)
+		return MLZ_OK;
+	if (n <= (size_t) TRSM_LEAF)
+		return trsm_leaf<E>(ctx, p, n, nrhs, a, W, B, ldb);
+	// Split at a multiple of the leaf size near n/2, so that every leaf is
+	// one of the precomputed diagonal blocks (any split gives the same unique
+	// solution; LELA splits at n/2).
+	size_t l = (n / 2 + TRSM_LEAF - 1) / TRSM_LEAF * TRSM_LEAF;
+	const typename Elem<E>::T *A11 = A, *A22 = A + l * lda + l;
+	typename Elem<E>::T *B1 = B, *B2 = B + l * ldb;
+	if (!upper) {
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, a, A11, lda, B1, ldb, W, algo));
+		MLZ_TRY(zp_gemm_dev(ctx, E, p, n - l, nrhs, l, p - 1, A + l * lda, lda, B1, ldb, a, B2, ldb, algo));
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n - l, nrhs, 1, A22, lda, B2, ldb, W + l * TRSM_LEAF, algo));
+	} else {
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n - l, nrhs, a, A22, lda, B2, ldb, W + l * TRSM_LEAF, algo));
+		MLZ_TRY(zp_gemm_dev(ctx, E, p, l, nrhs, n - l, p - 1, A + l, lda, B2, ldb, a, B1, ldb, algo));
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, 1, A11, lda, B1, ldb, W, algo));
+	}
+	return MLZ_OK;
+}
+
+int zp_trsm_dev(mlz_ctx *ctx, int elem, uint64_t p, int upper, int unit, size_t n, size_t nrhs,
+		uint64_t a, const void *A, size_t lda, void *B, size_t ldb, size_t *bad_diag, int algo)
+{
+	if (n == 0 || nrhs == 0)
+		return MLZ_OK;
+	a %= p;
+	if (a == 0)                                  // level3-generic.tcc:434-435
+		return zp_scal_dev(ctx, elem, p, n, nrhs, 0, B, ldb);
+	const size_t nblk = (n + TRSM_LEAF - 1) / TRSM_LEAF;
+	void *w = nullptr;
+	const size_t dinv_bytes = round_up(n * sizeof(uint32_t), 256);
+	const size_t w_bytes = nblk * TRSM_LEAF * TRSM_LEAF * sizeof(uint32_t);
+	MLZ_TRY(ws_get(ctx, WS_TRSM_0, dinv_bytes + w_bytes + 256, &w));
+	uint32_t *dinv = (uint32_t *) w;
+	uint32_t *W = (uint32_t *) ((char *) w + dinv_bytes);
+	unsigned long long *bad = (unsigned long long *) ((char *) W + w_bytes);
+	if (!unit) {
+		unsigned long long host[2];
+		init_bad_kernel<<<1, 1, 0, ctx->stream>>>(bad);
+		{
+			ProfScope ps(ctx, "trsm_diag_inv");
+			unsigned g = ceil_div(n, 256);
+			if (elem == MLZ_U16)
+				diag_inv_kernel<MLZ_U16><<<g, 256, 0, ctx->stream>>>((const uint16_t *) A, lda, (int) n, (uint32_t) p, dinv, bad);
+			else if (elem == MLZ_U32)
+				diag_inv_kernel<MLZ_U32><<<g, 256, 0, ctx->stream>>>((const uint32_t *) A, lda, (int) n, (uint32_t) p, dinv, bad);
+			else
+				diag_inv_kernel<MLZ_F64_SYM><<<g, 256, 0, ctx->stream>>>((const double *) A, lda, (int) n, (uint32_t) p, dinv, bad);
+		}
+		MLZ_TRY(check_launch(ctx, "diag_inv_kernel"));
+		MLZ_HIP(ctx, hipMemcpyAsync(host, bad, sizeof host, hipMemcpyDeviceToHost, ctx->stream));
+		MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+		if (host[0] != ~0ull) {
+			size_t idx = (size_t) (upper ? host[1] : host[0]);
+			if (bad_diag)
+				*bad_diag = idx;
+			return set_error(ctx, MLZ_E_NOT_INVERTIBLE, "diagonal entry %zu not invertible", idx);
+		}
+	}
+	const uint32_t pp = (uint32_t) p, aa = (uint32_t) a;
+	if (elem == MLZ_U16) {
+		MLZ_TRY(trsm_diag_inverses<MLZ_U16>(ctx, pp, upper, unit, n, A, lda, dinv, W));
+		return trsm_rec<MLZ_U16>(ctx, pp, upper, unit, n, nrhs, aa, (const uint16_t *) A, lda, (uint16_t *) B, ldb, W, algo);
+	}
+	if (elem == MLZ_U32) {
+		MLZ_TRY(trsm_diag_inverses<MLZ_U32>(ctx, pp, upper, unit, n, A, lda, dinv, W));
+		return trsm_rec<MLZ_U32>(ctx, pp, upper, unit, n, nrhs, aa, (const uint32_t *) A, lda, (uint32_t *) B, ldb, W, algo);
+	}
+	MLZ_TRY(trsm_diag_inverses<MLZ_F64_SYM>(ctx, pp, upper, unit, n, A, lda, dinv, W));
+	return trsm_rec<MLZ_F64_SYM>(ctx, pp, upper, unit, n, nrhs, aa, (const double *) A, lda, (double *) B, ldb, W, algo);
+}
+
+}  // namespace mlz
+
+using namespace mlz;
+
+extern "C" int mlz_zp_trsm(mlz_ctx *ctx, mlz_elem elem, uint64_t p, int upper, int unit_diag, size_t n,
+			   size_t nrhs, uint64_t a, const void *A, size_t lda, void *B, size_t ldb,
+			   size_t *bad_diag, int algo)
+{
+	if (!ctx)
+		return MLZ_E_ARG;
+	if (p < 2 || (elem == MLZ_U16 && p > 65535) || (elem == MLZ_U32 && p > 0xffffffffull) ||
+	    (elem == MLZ_F64_SYM && p >= 67108864ull) || elem_size(elem) == 0)
+		return set_error(ctx, MLZ_E_ARG, "bad modulus or element type");
+	if ((n && !A) || (n && nrhs && !B))
+		return set_error(ctx, MLZ_E_ARG, "null matrix pointer");
+	if ((n && lda < n) || (n && nrhs && ldb < nrhs))
+		return set_error(ctx, MLZ_E_ARG, "leading dimension smaller than row length");
+	if (n > (size_t) INT32_MAX || nrhs > (size_t) INT32_MAX)
+		return set_error(ctx, MLZ_E_ARG, "dimension too large");
+	MLZ_HIP(ctx, hipSetDevice(ctx->device));
+	if (ctx->ptr_mode == MLZ_PTR_DEVICE)
+		return zp_trsm_dev(ctx, elem, p, upper, unit_diag, n, nrhs, a, A, lda, B, ldb, bad_diag, algo);
+	if (n == 0 || nrhs == 0)
+		return MLZ_OK;
+	const size_t es = elem_size(elem);
+	void *dA = nullptr, *dB = nullptr;
+	MLZ_TRY(stage_in(ctx, WS_STAGE_A, A, n, n, lda, es, &dA));
+	MLZ_TRY(stage_in(ctx, WS_STAGE_B, B, n, nrhs, ldb, es, &dB));
+	int rc = zp_trsm_dev(ctx, elem, p, upper, unit_diag, n, nrhs, a, dA, n, dB, nrhs, bad_diag, algo);
+	if (rc != MLZ_OK)
+		return rc;
+	return stage_out(ctx, B, dB, n, nrhs, ldb, es);
+}

@@ -1,0 +1,89 @@
+/* oracle/oracle.h -- CPU restatement of LELA's exact level-3 path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * backend (lela_amd/liblela_amd.so); only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it.  The product never links it.
+ *
+ * Every function restates the algorithm of the LELA routine it cites
+ * (paths relative to the LELA source tree).  Storage is canonical: Z/p
+ * values are uint32 in [0, p); GF(2) matrices are rows of uint64 words,
+ * bit j of word w holding column 64*w + j (LSB first,
+ * lela/vector/bit-iterator.h:249-311, lela/matrix/dense-zero-one.h:253-257).
+ *
+ * Pinning (what checks each function beyond retyping its formula) is listed
+ * per function in tests/test_oracle.py; nothing here is "parity unpinned"
+ * except orc_fl_echelonize's det output for the D block (see its comment).
+ */
+#ifndef ORACLE_H
+#define ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+	ORC_OK = 0,
+	ORC_E_ARG = 1,
+	ORC_E_NOT_INVERTIBLE = 3,  /* DiagonalEntryNotInvertible / LELAError("Could not invert pivot") */
+	ORC_E_WRONG_FORM = 4       /* FaugereLachartre WrongMatrixForm */
+};
+
+/* C <- a*A*B + b*C over Z/p, uint32 storage (Modular<uint32> + ZpModule<uint32>).
+ * Per-entry dot with the 2^64-wrap correction of lela/blas/level1-modular.tcc:135-158,
+ * C update of lela/blas/level3-generic.tcc:301-319. */
+int orc_zp_gemm_u32(uint32_t p, size_t m, size_t n, size_t k, uint32_t a,
+		    const uint32_t *A, size_t lda, const uint32_t *B, size_t ldb,
+		    uint32_t b, uint32_t *C, size_t ldc);
+
+/* Same product for Modular<uint16> + ZpModule<uint16>: k-blocked delayed
+ * reduction with block_size = floor((2^64-1)/(p-1)^2)
+ * (lela/ring/modular.h:578-580, lela/blas/level1-modular.tcc:25-63). */
+int orc_zp_gemm_u16(uint16_t p, size_t m, size_t n, size_t k, uint16_t a,
+		    const uint16_t *A, size_t lda, const uint16_t *B, size_t ldb,
+		    uint16_t b, uint16_t *C, size_t ldc);
+
+/* B <- a * A^-1 * B, A n x n triangular, B n x nrhs (lela/blas/level3-generic.tcc:421-498).
+ * On a non-invertible diagonal returns ORC_E_NOT_INVERTIBLE with *bad_diag set to
+ * the index LELA's recursion reports (B is then partially updated, as in LELA). */
+int orc_zp_trsm(uint32_t p, int upper, int unit_diag, size_t n, size_t nrhs, uint32_t a,
+		const uint32_t *A, size_t lda, uint32_t *B, size_t ldb, size_t *bad_diag);
+
+/* Elimination::echelonize / echelonize_reduced with DensePivotStrategy
+ * (lela/algorithms/elimination.tcc:47-264, lela/algorithms/pivot-strategy.tcc:19-41).
+ * pivots[0..rank) receives the pivot columns.  det as LELA accumulates it. */
+int orc_zp_echelonize(uint32_t p, size_t m, size_t n, uint32_t *A, size_t lda, int reduced,
+		      size_t *rank, uint32_t *pivots, uint32_t *det);
+
+/* GF(2): C <- a*A*B + b*C (a, b in {0,1}); row/dot form of
+ * lela/blas/level3-generic.tcc:243-263 + lela/blas/level2-gf2.tcc:28-55 +
+ * lela/blas/level1-gf2.tcc:26-52.  Leading dimensions in words. */
+int orc_gf2_gemm(size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda_w,
+		 const uint64_t *B, size_t ldb_w, int b, uint64_t *C, size_t ldc_w);
+
+/* GF(2) Elimination::echelonize[_reduced] with the Dense01 pivot search
+ * (lela/algorithms/pivot-strategy.tcc:44-67) and word-XOR axpy
+ * (lela/blas/level1-gf2.tcc:282-299). */
+int orc_gf2_echelonize(size_t m, size_t n, uint64_t *A, size_t lda_w, int reduced,
+		       size_t *rank, uint32_t *pivots);
+
+/* FaugereLachartre<Modular<uint32>>::echelonize(R, X, rank, det, reduced=true)
+ * (lela/algorithms/faugere-lachartre.tcc:47-146,735-917) on dense storage.
+ * R (m x n, ldr) receives the reduced row-echelon form.  On WrongMatrixForm
+ * returns ORC_E_WRONG_FORM with *bad_row set.  det follows :119 for the X scan
+ * and the non-reduced Elimination form of D for the D scan (the D part of det
+ * is parity unpinned: it depends on LELA's non-canonical echelon form). */
+int orc_fl_echelonize(uint32_t p, size_t m, size_t n, const uint32_t *X, size_t ldx,
+		      uint32_t *R, size_t ldr, size_t *rank, uint32_t *det, size_t *bad_row);
+
+/* Modular inverse by the extended Euclidean algorithm (lela/ring/modular.h:460-474, :536-557).
+ * Returns 0 when gcd(y, p) != 1. */
+int orc_inv(uint32_t p, uint32_t y, uint32_t *x);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

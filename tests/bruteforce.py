@@ -1,0 +1,139 @@
+"""Independent brute-force references (Python integers / numpy object arrays).
+
+These do not follow LELA's loop structure: products are exact big-integer
+matmuls reduced once, and row reduction picks the *last* nonzero candidate row
+as pivot (LELA picks the first), which still yields the unique reduced
+row-echelon form.  They pin the oracle on small inputs.
+"""
+import numpy as np
+
+
+def matmul_mod(A, B, p):
+    A = np.asarray(A).astype(object)
+    B = np.asarray(B).astype(object)
+    if A.shape[1] == 0:
+        return np.zeros((A.shape[0], B.shape[1]), dtype=object)
+    return (A.dot(B)) % p
+
+
+def gemm_mod(p, a, A, B, b, C):
+    return ((a * matmul_mod(A, B, p)) + b * np.asarray(C).astype(object)) % p
+
+
+def rref_mod(A, p):
+    """Reduced row-echelon form over Z/p (p prime), last-row pivoting."""
+    M = [[int(x) % p for x in row] for row in np.asarray(A)]
+    m = len(M)
+    n = len(M[0]) if m else 0
+    r = 0
+    pivots = []
+    for c in range(n):
+        cand = [i for i in range(r, m) if M[i][c] % p]
+        if not cand:
+            continue
+        i = cand[-1]
+        M[r], M[i] = M[i], M[r]
+        inv = pow(M[r][c], -1, p)
+        M[r] = [(x * inv) % p for x in M[r]]
+        for j in range(m):
+            if j != r and M[j][c]:
+                f = M[j][c]
+                M[j] = [(x - f * y) % p for x, y in zip(M[j], M[r])]
+        pivots.append(c)
+        r += 1
+        if r == m:
+            break
+    return np.array(M, dtype=object).reshape(m, n), r, pivots
+
+
+def det_mod(A, p):
+    """Determinant mod prime p by elimination over the field."""
+    M = [[int(x) % p for x in row] for row in np.asarray(A)]
+    n = len(M)
+    d = 1
+    for c in range(n):
+        piv = next((i for i in range(c, n) if M[i][c]), None)
+        if piv is None:
+            return 0
+        if piv != c:
+            M[c], M[piv] = M[piv], M[c]
+            d = -d
+        d = d * M[c][c] % p
+        inv = pow(M[c][c], -1, p)
+        for i in range(c + 1, n):
+            f = M[i][c] * inv % p
+            if f:
+                M[i] = [(x - f * y) % p for x, y in zip(M[i], M[c])]
+    return d % p
+
+
+def is_rref(R, p):
+    """Structural check of the standard-layout RREF; returns the pivot list."""
+    R = np.asarray(R).astype(object) % p
+    m, n = R.shape
+    pivots = []
+    last = -1
+    zero_seen = False
+    for i in range(m):
+        nz = [c for c in range(n) if R[i, c] != 0]
+        if not nz:
+            zero_seen = True
+            continue
+        assert not zero_seen, "nonzero row after a zero row"
+        c = nz[0]
+        assert c > last, "pivot columns not increasing"
+        assert R[i, c] == 1, "pivot not 1"
+        col = R[:, c]
+        assert sum(1 for x in col if x != 0) == 1, "pivot column not a unit vector"
+        pivots.append(c)
+        last = c
+    return pivots
+
+
+# ----------------------------------------------------------------- GF(2)
+
+def gf2_unpack(W, ncols):
+    W = np.asarray(W, dtype=np.uint64)
+    rows, words = W.shape
+    bits = np.zeros((rows, words * 64), dtype=np.uint8)
+    for b in range(64):
+        bits[:, b::64] = ((W >> np.uint64(b)) & np.uint64(1)).astype(np.uint8)
+    return bits[:, :ncols]
+
+
+def gf2_pack(bits):
+    bits = np.asarray(bits, dtype=np.uint64)
+    rows, ncols = bits.shape
+    words = (ncols + 63) // 64
+    padded = np.zeros((rows, words * 64), dtype=np.uint64)
+    padded[:, :ncols] = bits
+    W = np.zeros((rows, words), dtype=np.uint64)
+    for b in range(64):
+        W |= padded[:, b::64] << np.uint64(b)
+    return W
+
+
+def gf2_matmul_bits(A, B):
+    return (np.asarray(A, dtype=np.int64) @ np.asarray(B, dtype=np.int64)) & 1
+
+
+def gf2_rref_bits(A):
+    M = np.array(A, dtype=np.uint8) & 1
+    m, n = M.shape
+    r = 0
+    pivots = []
+    for c in range(n):
+        cand = np.nonzero(M[r:, c])[0]
+        if cand.size == 0:
+            continue
+        i = r + cand[-1]
+        if i != r:
+            M[[r, i]] = M[[i, r]]
+        rows = np.nonzero(M[:, c])[0]
+        rows = rows[rows != r]
+        M[rows] ^= M[r]
+        pivots.append(c)
+        r += 1
+        if r == m:
+            break
+    return M, r, pivots

@@ -1,0 +1,24 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU (run with -m gpu)")
+
+
+@pytest.fixture(scope="session")
+def gpu_ctx():
+    """One HIP context for the whole GPU session (its ring is set per test)."""
+    import lela_amd
+    ctx = lela_amd.Context(None)
+    yield ctx
+    ctx.close()
