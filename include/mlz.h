@@ -122,6 +122,18 @@ int mlz_zp_trsm(mlz_ctx *ctx, mlz_elem elem, uint64_t p, int upper, int unit_dia
 		size_t n, size_t nrhs, uint64_t a, const void *A, size_t lda,
 		void *B, size_t ldb, size_t *bad_diag, int algo);
 
+/* Reduced row-echelon form of A (m x n) in place, standard layout: rows
+ * 0..rank-1 hold the pivot rows (pivot 1, increasing pivot columns, zeros in
+ * every other row of a pivot column), rows rank..m-1 are zero.  pivots[0..rank)
+ * receives the pivot columns (array of min(m, n) entries, may be NULL).
+ * Replaces EchelonForm::echelonize(A, reduced=true) (lela/solutions/echelon-form.h:103-154)
+ * -> Elimination::echelonize_reduced (lela/algorithms/elimination.tcc:141-264) /
+ * GaussJordan::echelonize_reduced (lela/algorithms/gauss-jordan.tcc:413-466).
+ * algo selects the gemm family used for the block updates.
+ * MLZ_E_NOT_INVERTIBLE: a pivot has no inverse (composite modulus). */
+int mlz_zp_rref(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t n, void *A, size_t lda,
+		size_t *rank, uint32_t *pivots, int algo);
+
 #ifdef __cplusplus
 }
 #endif

@@ -267,3 +267,32 @@ class BLAS3:
                                  ctypes.byref(bad), algo)
         ctx.check(rc, "trsm", idx=bad.value if rc == _lib.MLZ_E_NOT_INVERTIBLE else None)
         return B
+
+
+class EchelonForm:
+    """EchelonForm<Ring, HIPModule> (lela/solutions/echelon-form.h:31-164) for
+    dense Z/p matrices.  echelonize(A, reduced=True) replaces A by its reduced
+    row-echelon form and returns it; rank(A) gives the rank recorded for A."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self._rank_table = {}
+        self.pivots = None
+
+    def echelonize(self, A, reduced=True, algo=ALGO_AUTO):
+        if not reduced:
+            raise LELAError("only the reduced form is computed on the GPU "
+                            "(the non-reduced form depends on LELA's pivot order)")
+        ctx, F = self.ctx, self.ctx.F
+        pa, lda, m, n = _operand(ctx, A, F.dtype)
+        rank = ctypes.c_size_t(0)
+        piv = np.zeros(max(1, min(m, n)), dtype=np.uint32)
+        rc = ctx.lib.mlz_zp_rref(ctx.handle, F.elem, F.p, m, n, pa, lda, ctypes.byref(rank),
+                                 piv.ctypes.data, algo)
+        ctx.check(rc, "rref")
+        self._rank_table[id(A)] = rank.value
+        self.pivots = piv[:rank.value].copy()
+        return A
+
+    def rank(self, A):
+        return self._rank_table[id(A)]

@@ -50,6 +50,8 @@ SIGNATURES = {
                              _vp, _c_size, _vp, _c_size, _c_u64, _vp, _c_size, _c_int]),
     "mlz_zp_trsm": (_c_int, [_ctx, _c_int, _c_u64, _c_int, _c_int, _c_size, _c_size,
                              _c_u64, _vp, _c_size, _vp, _c_size, ctypes.POINTER(_c_size), _c_int]),
+    "mlz_zp_rref": (_c_int, [_ctx, _c_int, _c_u64, _c_size, _c_size, _vp, _c_size,
+                             ctypes.POINTER(_c_size), _vp, _c_int]),
 }
 
 _lib = None

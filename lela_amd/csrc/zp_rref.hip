@@ -1,0 +1,503 @@
+// lela_amd/csrc/zp_rref.hip -- reduced row-echelon form over Z/p on MI355X.
+//
+// Replaces the pivot-row reduction of Elimination::echelonize_reduced
+// (lela/algorithms/elimination.tcc:141-264) and GaussJordan::echelonize_reduced
+// (lela/algorithms/gauss-jordan.tcc:413-466) behind EchelonForm::echelonize
+// (lela/solutions/echelon-form.h:103-154).  The reduced row-echelon form, the
+// rank and the pivot columns are unique, so any pivot order gives LELA's
+// result; this one is built for the GPU:
+//
+// Blocked Gauss-Jordan over column panels of RREF_W columns.  Per panel
+//  1. pivot search: a tournament over the rows that are not yet pivots, in ONE
+//     kernel -- each workgroup reduces a 256-row chunk of the panel to echelon
+//     form in LDS and keeps its pivot rows (at most RREF_W); workgroups then
+//     merge pairwise up a tree (the second to arrive at a node merges, via an
+//     atomic arrival counter).  The surviving rows span the panel's row space,
+//     so their pivot columns J are the panel's rank profile.
+//  2. W = P_J^-1 * P (P = the k pivot rows from column c on, P_J its columns J):
+//     one workgroup inverts P_J, a gemm forms W (identity on J).
+//  3. every other row i: A_i <- A_i - A_i[J] * W, one gemm over all rows (the
+//     new pivot rows get a zero multiplier row), then W is written into the
+//     pivot rows.
+// Rank, pivot rows and pivot columns stay on the device; the host only checks
+// the rank every few panels to stop early.  A final kernel writes the pivot
+// rows in pivot-column order followed by zero rows (standard layout).
+#include "common.hpp"
+
+namespace mlz {
+
+constexpr int RREF_W = 64;        // panel width (columns)
+constexpr int RREF_CHUNK = 256;   // rows per tournament leaf
+constexpr int RREF_MAXLVL = 24;
+
+__device__ inline uint32_t inv_mod_dev(uint32_t y, uint32_t p)
+{
+	int64_t r0 = y % p, r1 = p, s0 = 1, s1 = 0;
+	while (r1 != 0) {
+		int64_t q = r0 / r1, t;
+		t = r0 - q * r1; r0 = r1; r1 = t;
+		t = s0 - q * s1; s0 = s1; s1 = t;
+	}
+	if (r0 != 1)
+		return 0;
+	s0 %= (int64_t) p;
+	if (s0 < 0)
+		s0 += p;
+	return (uint32_t) s0;
+}
+
+// Device-side state of one RREF call.
+struct RrefState {
+	int *is_pivot;       // [m]: 0 = not a pivot row, else 1 + panel index
+	int *rank;           // [1]
+	int *piv_rows;       // [min(m,n)] in pivot-column order
+	int *piv_cols;       // [min(m,n)]
+	int *k;              // [1] pivots found in the current panel
+	int *J;              // [RREF_W] panel-relative pivot columns
+	int *prow;           // [RREF_W] their rows
+	int *bad;            // [1] set when a pivot is not invertible (composite p)
+	int *lists_rows;     // [levels][nodes0][RREF_W]
+	int *lists_cols;
+	int *lists_cnt;      // [levels][nodes0]
+	int *counters;       // [levels][nodes0]
+};
+
+// Echelon form of nr <= 256 rows x wc <= RREF_W columns held in LDS; records the
+// pivot rows (their ids) and pivot columns in order.  All threads return the
+// same count.
+__device__ int lds_echelon(uint32_t (*M)[RREF_W + 1], const int *rowid, int *alive, int nr, int wc,
+			   const ModP &mp, int *out_rows, int *out_cols, int *s_min, uint32_t *s_inv,
+			   uint32_t *f, int *bad)
+{
+	const int tid = threadIdx.x;
+	int cnt = 0;
+	for (int col = 0; col < wc; ++col) {
+		if (tid == 0)
+			*s_min = 0x7fffffff;
+		__syncthreads();
+		for (int r = tid; r < nr; r += blockDim.x)
+			if (alive[r] && M[r][col] != 0)
+				atomicMin(s_min, r);
+		__syncthreads();
+		const int pr = *s_min;
+		if (pr == 0x7fffffff) {
+			__syncthreads();
+			continue;
+		}
+		if (tid == 0) {
+			uint32_t x = inv_mod_dev(M[pr][col], (uint32_t) mp.p);
+			if (x == 0)
+				*bad = 1;
+			*s_inv = x;
+			alive[pr] = 0;
+			out_rows[cnt] = rowid[pr];
+			out_cols[cnt] = col;
+		}
+		__syncthreads();
+		const uint32_t xi = *s_inv;
+		for (int r = tid; r < nr; r += blockDim.x)
+			f[r] = (alive[r] && M[r][col]) ? mp.mul(M[r][col], xi) : 0u;
+		__syncthreads();
+		const int cc = col + 1 + (tid % RREF_W);
+		if (cc < wc) {
+			const uint32_t pv = M[pr][cc];
+			for (int r = tid / RREF_W; r < nr; r += blockDim.x / RREF_W)
+				if (f[r])
+					M[r][cc] = mp.sub(M[r][cc], mp.mul(f[r], pv));
+		}
+		__syncthreads();
+		++cnt;
+	}
+	return cnt;
+}
+
+__device__ inline int nodes_at(int nodes0, int level)
+{
+	int n = nodes0;
+	for (int l = 0; l < level; ++l)
+		n = (n + 1) / 2;
+	return n;
+}
+
+__global__ __launch_bounds__(256) void rref_tourney_kernel(const uint32_t *__restrict__ A, size_t lda, int m,
+							   int c, int wc, ModP mp, RrefState st, int nodes0)
+{
+	__shared__ uint32_t M[RREF_CHUNK][RREF_W + 1];
+	__shared__ int rowid[RREF_CHUNK], alive[RREF_CHUNK];
+	__shared__ uint32_t f[RREF_CHUNK];
+	__shared__ int outr[RREF_W], outc[RREF_W];
+	__shared__ int s_min, s_go, s_nr;
+	__shared__ uint32_t s_inv;
+	const int tid = threadIdx.x;
+	int node = blockIdx.x, level = 0;
+
+	// Leaf: rows [node*CHUNK, ...) that are not pivots yet.
+	const int r0 = node * RREF_CHUNK;
+	const int nr0 = min(RREF_CHUNK, m - r0);
+	for (int idx = tid; idx < nr0 * RREF_W; idx += blockDim.x) {
+		int r = idx / RREF_W, cc = idx % RREF_W;
+		M[r][cc] = cc < wc ? A[(size_t) (r0 + r) * lda + c + cc] : 0u;
+	}
+	for (int r = tid; r < nr0; r += blockDim.x) {
+		rowid[r] = r0 + r;
+		alive[r] = st.is_pivot[r0 + r] == 0;
+	}
+	__syncthreads();
+	int cnt = lds_echelon(M, rowid, alive, nr0, wc, mp, outr, outc, &s_min, &s_inv, f, st.bad);
+
+	for (;;) {
+		const int here = nodes_at(nodes0, level);
+		int *lr = st.lists_rows + ((size_t) level * nodes0 + node) * RREF_W;
+		int *lc = st.lists_cols + ((size_t) level * nodes0 + node) * RREF_W;
+		if (here == 1) {
+			// Root: publish the panel's pivots and reset the arrival counters.
+			for (int t = tid; t < RREF_W; t += blockDim.x) {
+				st.J[t] = t < cnt ? outc[t] : 0;
+				st.prow[t] = t < cnt ? outr[t] : 0;
+			}
+			if (tid == 0)
+				*st.k = cnt;
+			for (int t = tid; t < RREF_MAXLVL * nodes0; t += blockDim.x)
+				st.counters[t] = 0;
+			return;
+		}
+		for (int t = tid; t < cnt; t += blockDim.x) {
+			lr[t] = outr[t];
+			lc[t] = outc[t];
+		}
+		if (tid == 0)
+			st.lists_cnt[(size_t) level * nodes0 + node] = cnt;
+		__threadfence();
+		__syncthreads();
+		const int parent = node >> 1, sibling = node ^ 1;
+		if (tid == 0) {
+			int go = 1;
+			if (sibling < here) {
+				int old = atomicAdd(&st.counters[(size_t) level * nodes0 + parent], 1);
+				go = old == 1;
+			}
+			s_go = go;
+		}
+		__syncthreads();
+		if (!s_go)
+			return;
+		__threadfence();
+		// Merge both children's lists (re-reading the rows' panel values).
+		const int c0 = parent * 2, c1 = parent * 2 + 1;
+		if (tid == 0) {
+			int n0 = ((volatile int *) st.lists_cnt)[(size_t) level * nodes0 + c0];
+			int n1 = c1 < here ? ((volatile int *) st.lists_cnt)[(size_t) level * nodes0 + c1] : 0;
+			s_nr = n0 + n1;
+			s_min = n0;
+		}
+		__syncthreads();
+		const int nr = s_nr, n0 = s_min;
+		__syncthreads();
+		for (int r = tid; r < nr; r += blockDim.x) {
+			const int child = r < n0 ? c0 : c1;
+			const int pos = r < n0 ? r : r - n0;
+			rowid[r] = ((volatile int *) st.lists_rows)[((size_t) level * nodes0 + child) * RREF_W + pos];
+			alive[r] = 1;
+		}
+		__syncthreads();
+		for (int idx = tid; idx < nr * RREF_W; idx += blockDim.x) {
+			int r = idx / RREF_W, cc = idx % RREF_W;
+			M[r][cc] = cc < wc ? A[(size_t) rowid[r] * lda + c + cc] : 0u;
+		}
+		__syncthreads();
+		cnt = lds_echelon(M, rowid, alive, nr, wc, mp, outr, outc, &s_min, &s_inv, f, st.bad);
+		++level;
+		node = parent;
+	}
+}
+
+// P[t][col] = A[prow[t]][c + col] (t < k, else 0); marks the new pivot rows and
+// appends them to the pivot lists.
+__global__ void rref_gather_kernel(const uint32_t *__restrict__ A, size_t lda, int nc, int c, int panel,
+				   RrefState st, uint32_t *__restrict__ P)
+{
+	const int t = blockIdx.y;
+	const int k = *st.k;
+	const int col = blockIdx.x * blockDim.x + threadIdx.x;
+	if (col < nc)
+		P[(size_t) t * nc + col] = t < k ? A[(size_t) st.prow[t] * lda + c + col] : 0u;
+	if (blockIdx.x == 0 && threadIdx.x == 0 && t < k) {
+		const int r = *st.rank;
+		st.is_pivot[st.prow[t]] = panel + 1;
+		st.piv_rows[r + t] = st.prow[t];
+		st.piv_cols[r + t] = c + st.J[t];
+	}
+}
+
+// Inverse of P_J (k x k, padded with the identity to RREF_W) by Gauss-Jordan.
+__global__ __launch_bounds__(256) void rref_inv_kernel(const uint32_t *__restrict__ P, int nc, ModP mp,
+						       RrefState st, uint32_t *__restrict__ Pinv)
+{
+	constexpr int W = RREF_W;
+	__shared__ uint32_t X[W][2 * W + 1];
+	__shared__ uint32_t f[W];
+	__shared__ int s_piv;
+	__shared__ uint32_t s_inv;
+	const int tid = threadIdx.x;
+	const int k = *st.k;
+	for (int idx = tid; idx < W * 2 * W; idx += blockDim.x) {
+		int r = idx / (2 * W), cc = idx % (2 * W);
+		uint32_t v;
+		if (cc < W)
+			v = (r < k && cc < k) ? P[(size_t) r * nc + st.J[cc]] : (r == cc ? 1u : 0u);
+		else
+			v = (cc - W == r) ? 1u : 0u;
+		X[r][cc] = v;
+	}
+	__syncthreads();
+	for (int s = 0; s < k; ++s) {
+		if (tid == 0) {
+			int pr = -1;
+			for (int r = s; r < k; ++r)
+				if (X[r][s] != 0) { pr = r; break; }
+			s_piv = pr;
+			s_inv = pr >= 0 ? inv_mod_dev(X[pr][s], (uint32_t) mp.p) : 0u;
+			if (pr < 0 || s_inv == 0)
+				*st.bad = 1;
+		}
+		__syncthreads();
+		const int pr = s_piv;
+		if (pr < 0)
+			break;
+		if (pr != s)
+			for (int cc = tid; cc < 2 * W; cc += blockDim.x) {
+				uint32_t t = X[s][cc];
+				X[s][cc] = X[pr][cc];
+				X[pr][cc] = t;
+			}
+		__syncthreads();
+		for (int cc = tid; cc < 2 * W; cc += blockDim.x)
+			X[s][cc] = mp.mul(X[s][cc], s_inv);
+		__syncthreads();
+		for (int r = tid; r < W; r += blockDim.x)
+			f[r] = r == s ? 0u : X[r][s];
+		__syncthreads();
+		for (int idx = tid; idx < W * 2 * W; idx += blockDim.x) {
+			int r = idx / (2 * W), cc = idx % (2 * W);
+			if (f[r])
+				X[r][cc] = mp.sub(X[r][cc], mp.mul(f[r], X[s][cc]));
+		}
+		__syncthreads();
+	}
+	for (int idx = tid; idx < W * W; idx += blockDim.x) {
+		int r = idx / W, cc = idx % W;
+		Pinv[idx] = X[r][W + cc];
+	}
+}
+
+// G[i][s] = A[i][c + J[s]] for s < k, except 0 on this panel's pivot rows.
+__global__ void rref_gatherG_kernel(const uint32_t *__restrict__ A, size_t lda, int m, int c, int panel,
+				    RrefState st, uint32_t *__restrict__ G)
+{
+	const int i = blockIdx.x * (blockDim.x / RREF_W) + threadIdx.x / RREF_W;
+	const int s = threadIdx.x % RREF_W;
+	if (i >= m)
+		return;
+	const int k = *st.k;
+	uint32_t v = 0;
+	if (s < k && st.is_pivot[i] != panel + 1)
+		v = A[(size_t) i * lda + c + st.J[s]];
+	G[(size_t) i * RREF_W + s] = v;
+}
+
+// Writes W into the pivot rows; the last block advances the rank.
+__global__ void rref_scatter_kernel(uint32_t *__restrict__ A, size_t lda, int nc, int c, RrefState st,
+				    const uint32_t *__restrict__ W)
+{
+	const int t = blockIdx.y;
+	const int k = *st.k;
+	const int col = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < k && col < nc)
+		A[(size_t) st.prow[t] * lda + c + col] = W[(size_t) t * nc + col];
+}
+
+__global__ void rref_advance_kernel(RrefState st)
+{
+	*st.rank += *st.k;
+}
+
+template <int E>
+__global__ void rref_load_kernel(const typename Elem<E>::T *__restrict__ src, size_t ld, int m, int n,
+				 uint32_t p, uint32_t *__restrict__ dst)
+{
+	size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= (size_t) m * n)
+		return;
+	int r = (int) (idx / n), cc = (int) (idx % n);
+	dst[idx] = Elem<E>::load(src + (size_t) r * ld + cc, p);
+}
+
+// Output row t: pivot row t for t < rank, zero row after.
+template <int E>
+__global__ void rref_store_kernel(const uint32_t *__restrict__ W, int m, int n, uint32_t p, RrefState st,
+				  typename Elem<E>::T *__restrict__ dst, size_t ld)
+{
+	size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= (size_t) m * n)
+		return;
+	int t = (int) (idx / n), cc = (int) (idx % n);
+	uint32_t v = t < *st.rank ? W[(size_t) st.piv_rows[t] * n + cc] : 0u;
+	Elem<E>::store(dst + (size_t) t * ld + cc, v, p);
+}
+
+__global__ void rref_init_kernel(RrefState st, int m, int cnt_counters)
+{
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < m)
+		st.is_pivot[i] = 0;
+	if (i < cnt_counters)
+		st.counters[i] = 0;
+	if (i == 0) {
+		*st.rank = 0;
+		*st.k = 0;
+		*st.bad = 0;
+	}
+}
+
+int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A, size_t lda, size_t *rank_out,
+		uint32_t *pivots_out, int algo)
+{
+	if (m == 0 || n == 0) {
+		if (rank_out)
+			*rank_out = 0;
+		return MLZ_OK;
+	}
+	const ModP mp = make_modp(p);
+	const uint32_t pp = (uint32_t) p;
+	const int nodes0 = (int) ceil_div(m, RREF_CHUNK);
+	const size_t mn = m * n;
+	const size_t rmax = std::min(m, n);
+	// workspace
+	void *w0, *w1, *w2, *w3;
+	MLZ_TRY(ws_get(ctx, WS_RREF_0, mn * sizeof(uint32_t), &w0));                 // working matrix
+	MLZ_TRY(ws_get(ctx, WS_RREF_1, 2 * (size_t) RREF_W * n * sizeof(uint32_t) +
+				      (size_t) RREF_W * RREF_W * sizeof(uint32_t), &w1));  // P, W, Pinv
+	MLZ_TRY(ws_get(ctx, WS_RREF_2, m * RREF_W * sizeof(uint32_t), &w2));         // G
+	const size_t ints = m + 1 + 2 * rmax + 1 + 2 * RREF_W + 1 +
+			    3 * (size_t) RREF_MAXLVL * nodes0 * RREF_W + 2 * (size_t) RREF_MAXLVL * nodes0;
+	MLZ_TRY(ws_get(ctx, WS_RREF_3, ints * sizeof(int), &w3));
+	uint32_t *Wm = (uint32_t *) w0;
+	uint32_t *P = (uint32_t *) w1, *Wb = P + (size_t) RREF_W * n, *Pinv = Wb + (size_t) RREF_W * n;
+	uint32_t *G = (uint32_t *) w2;
+	RrefState st;
+	int *q = (int *) w3;
+	st.is_pivot = q; q += m;
+	st.rank = q; q += 1;
+	st.piv_rows = q; q += rmax;
+	st.piv_cols = q; q += rmax;
+	st.k = q; q += 1;
+	st.J = q; q += RREF_W;
+	st.prow = q; q += RREF_W;
+	st.bad = q; q += 1;
+	st.lists_rows = q; q += (size_t) RREF_MAXLVL * nodes0 * RREF_W;
+	st.lists_cols = q; q += (size_t) RREF_MAXLVL * nodes0 * RREF_W;
+	q += (size_t) RREF_MAXLVL * nodes0 * RREF_W;
+	st.lists_cnt = q; q += (size_t) RREF_MAXLVL * nodes0;
+	st.counters = q;
+	if (nodes0 > (1 << (RREF_MAXLVL - 2)))
+		return set_error(ctx, MLZ_E_ARG, "too many rows for the pivot tournament");
+
+	{
+		const int cc = RREF_MAXLVL * nodes0;
+		rref_init_kernel<<<ceil_div(std::max(m, (size_t) cc), 256), 256, 0, ctx->stream>>>(st, (int) m, cc);
+		ProfScope ps(ctx, "rref_load");
+		unsigned g = ceil_div(mn, 256);
+		if (elem == MLZ_U16)
+			rref_load_kernel<MLZ_U16><<<g, 256, 0, ctx->stream>>>((const uint16_t *) A, lda, (int) m, (int) n, pp, Wm);
+		else if (elem == MLZ_U32)
+			rref_load_kernel<MLZ_U32><<<g, 256, 0, ctx->stream>>>((const uint32_t *) A, lda, (int) m, (int) n, pp, Wm);
+		else
+			rref_load_kernel<MLZ_F64_SYM><<<g, 256, 0, ctx->stream>>>((const double *) A, lda, (int) m, (int) n, pp, Wm);
+		MLZ_TRY(check_launch(ctx, "rref_load_kernel"));
+	}
+	int panel = 0;
+	for (size_t c = 0; c < n; c += RREF_W, ++panel) {
+		const int wc = (int) std::min((size_t) RREF_W, n - c);
+		const int nc = (int) (n - c);
+		{
+			ProfScope ps(ctx, "rref_tourney");
+			rref_tourney_kernel<<<nodes0, 256, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, nodes0);
+		}
+		MLZ_TRY(check_launch(ctx, "rref_tourney_kernel"));
+		rref_gather_kernel<<<dim3(ceil_div(nc, 256), RREF_W), 256, 0, ctx->stream>>>(Wm, n, nc, (int) c, panel, st, P);
+		rref_inv_kernel<<<1, 256, 0, ctx->stream>>>(P, nc, mp, st, Pinv);
+		MLZ_TRY(check_launch(ctx, "rref_inv_kernel"));
+		MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, RREF_W, nc, RREF_W, 1, Pinv, RREF_W, P, nc, 0, Wb, nc, algo));
+		rref_gatherG_kernel<<<ceil_div(m, 256 / RREF_W), 256, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, panel, st, G);
+		MLZ_TRY(check_launch(ctx, "rref_gatherG_kernel"));
+		MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, m, nc, RREF_W, p - 1, G, RREF_W, Wb, nc, 1, Wm + c, n, algo));
+		rref_scatter_kernel<<<dim3(ceil_div(nc, 256), RREF_W), 256, 0, ctx->stream>>>(Wm, n, nc, (int) c, st, Wb);
+		rref_advance_kernel<<<1, 1, 0, ctx->stream>>>(st);
+		MLZ_TRY(check_launch(ctx, "rref_scatter_kernel"));
+		if ((panel & 7) == 7 || c + RREF_W >= n) {
+			int r = 0;
+			MLZ_HIP(ctx, hipMemcpyAsync(&r, st.rank, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+			MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+			if ((size_t) r >= m)
+				break;
+		}
+	}
+	int r = 0, bad = 0;
+	MLZ_HIP(ctx, hipMemcpyAsync(&r, st.rank, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+	MLZ_HIP(ctx, hipMemcpyAsync(&bad, st.bad, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+	MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+	if (bad)
+		return set_error(ctx, MLZ_E_NOT_INVERTIBLE, "Could not invert pivot-element in the ring");
+	{
+		ProfScope ps(ctx, "rref_store");
+		unsigned g = ceil_div(mn, 256);
+		if (elem == MLZ_U16)
+			rref_store_kernel<MLZ_U16><<<g, 256, 0, ctx->stream>>>(Wm, (int) m, (int) n, pp, st, (uint16_t *) A, lda);
+		else if (elem == MLZ_U32)
+			rref_store_kernel<MLZ_U32><<<g, 256, 0, ctx->stream>>>(Wm, (int) m, (int) n, pp, st, (uint32_t *) A, lda);
+		else
+			rref_store_kernel<MLZ_F64_SYM><<<g, 256, 0, ctx->stream>>>(Wm, (int) m, (int) n, pp, st, (double *) A, lda);
+		MLZ_TRY(check_launch(ctx, "rref_store_kernel"));
+	}
+	if (pivots_out && r > 0) {
+		std::vector<int> pc(r);
+		MLZ_HIP(ctx, hipMemcpyAsync(pc.data(), st.piv_cols, r * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+		MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+		for (int i = 0; i < r; ++i)
+			pivots_out[i] = (uint32_t) pc[i];
+	}
+	if (rank_out)
+		*rank_out = (size_t) r;
+	return MLZ_OK;
+}
+
+}  // namespace mlz
+
+using namespace mlz;
+
+extern "C" int mlz_zp_rref(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t n, void *A, size_t lda,
+			   size_t *rank, uint32_t *pivots, int algo)
+{
+	if (!ctx)
+		return MLZ_E_ARG;
+	if (p < 2 || (elem == MLZ_U16 && p > 65535) || (elem == MLZ_U32 && p > 0xffffffffull) ||
+	    (elem == MLZ_F64_SYM && p >= 67108864ull) || elem_size(elem) == 0)
+		return set_error(ctx, MLZ_E_ARG, "bad modulus or element type");
+	if (m && n && (!A || lda < n))
+		return set_error(ctx, MLZ_E_ARG, "bad matrix pointer or leading dimension");
+	if (m > (size_t) INT32_MAX / 2 || n > (size_t) INT32_MAX / 2 || m * n > ((size_t) 1 << 34))
+		return set_error(ctx, MLZ_E_ARG, "matrix too large");
+	MLZ_HIP(ctx, hipSetDevice(ctx->device));
+	if (ctx->ptr_mode == MLZ_PTR_DEVICE)
+		return zp_rref_dev(ctx, elem, p, m, n, A, lda, rank, pivots, algo);
+	if (m == 0 || n == 0) {
+		if (rank)
+			*rank = 0;
+		return MLZ_OK;
+	}
+	const size_t es = elem_size(elem);
+	void *dA = nullptr;
+	MLZ_TRY(stage_in(ctx, WS_STAGE_A, A, m, n, lda, es, &dA));
+	MLZ_TRY(zp_rref_dev(ctx, elem, p, m, n, dA, n, rank, pivots, algo));
+	return stage_out(ctx, A, dA, m, n, lda, es);
+}

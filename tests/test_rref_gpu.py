@@ -1,0 +1,104 @@
+"""GPU parity of mlz_zp_rref against the oracle's Elimination::echelonize_reduced
+(bit-exact RREF, rank and pivot columns): random, low-rank, sparse, tall,
+wide, zero and ragged matrices, every storage and gemm family."""
+import numpy as np
+import pytest
+
+import bruteforce as bf
+import lela_amd as la
+import oracle_lib as orc
+from lela_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def lowrank(seed, m, n, r, p):
+    X = synth.zp_matrix(seed, m, r, p).astype(np.uint64)
+    Y = synth.zp_matrix(seed + 1, r, n, p).astype(np.uint64)
+    out = np.zeros((m, n), dtype=np.uint64)
+    for i in range(0, m, 256):
+        out[i:i + 256] = (X[i:i + 256] @ Y) % np.uint64(p) if p < (1 << 20) else \
+            np.array(bf.matmul_mod(X[i:i + 256], Y, p), dtype=np.uint64)
+    return out.astype(np.uint32)
+
+
+def sparse(seed, m, n, p, density):
+    rng = np.random.default_rng(seed)
+    A = synth.zp_matrix(seed, m, n, p)
+    A[rng.random((m, n)) >= density] = 0
+    return A
+
+
+def run_rref(ctx, p, storage, A, algo=la.ALGO_AUTO):
+    F = la.Modular(p, storage)
+    ctx.F = F
+    X = F.from_canonical(A)
+    EF = la.EchelonForm(ctx)
+    EF.echelonize(X, reduced=True, algo=algo)
+    return F.to_canonical(X), EF.rank(X), EF.pivots
+
+
+def check(ctx, p, storage, A, algo=la.ALGO_AUTO):
+    got, rank, piv = run_rref(ctx, p, storage, A, algo)
+    rc, want, wrank, wpiv, _ = orc.zp_echelonize(p, A, reduced=True)
+    assert rc == 0
+    assert rank == wrank and list(piv) == list(wpiv)
+    assert np.array_equal(got, want)
+
+
+CASES = [
+    ("square", lambda p: synth.zp_matrix(1, 100, 100, p)),
+    ("lowrank", lambda p: lowrank(2, 150, 170, 37, p)),
+    ("tall", lambda p: synth.zp_matrix(3, 700, 90, p)),
+    ("wide", lambda p: synth.zp_matrix(4, 60, 400, p)),
+    ("sparse1pct", lambda p: sparse(5, 300, 260, p, 0.01)),
+    ("sparse10pct", lambda p: sparse(6, 257, 333, p, 0.10)),
+    ("zero", lambda p: np.zeros((70, 80), dtype=np.uint32)),
+    ("one", lambda p: np.ones((1, 1), dtype=np.uint32)),
+    ("ragged", lambda p: lowrank(7, 513, 129, 100, p)),
+]
+
+
+@pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("p", [2, 3, 65521, 2147483647])
+def test_rref_matches_oracle(gpu_ctx, name, make, p):
+    print(f"rref {name} p={p}", flush=True)
+    check(gpu_ctx, p, "uint32", make(p))
+
+
+@pytest.mark.parametrize("storage,p", [("uint16", 65521), ("double", 67108859), ("uint16", 251)])
+def test_rref_storages(gpu_ctx, storage, p):
+    check(gpu_ctx, p, storage, lowrank(8, 200, 230, 150, p))
+
+
+@pytest.mark.parametrize("algo", [la.ALGO_PLAIN, la.ALGO_INT8, la.ALGO_FP64])
+def test_rref_gemm_families(gpu_ctx, algo):
+    check(gpu_ctx, 65521, "uint32", lowrank(9, 300, 280, 200, 65521), algo=algo)
+
+
+def test_rref_many_pivot_chunks(gpu_ctx):
+    # more than 256 rows per panel with pivots spread over several chunks and a
+    # rank deficit inside panels
+    p = 65521
+    A = lowrank(10, 1500, 600, 420, p)
+    A[::7] = 0
+    check(gpu_ctx, p, "uint32", A)
+
+
+def test_rref_structure_large(gpu_ctx):
+    """2000 x 3000 rank 1500: RREF structure, rank and row-space checks
+    (size-independent properties; the oracle is too slow at full size)."""
+    p = 65521
+    A = lowrank(11, 2000, 3000, 1500, p)
+    got, rank, piv = run_rref(gpu_ctx, p, "uint32", A)
+    assert rank == 1500
+    R = got[:rank].astype(np.uint64)
+    P = np.uint64(p)
+    # pivot columns are an identity block, rows below rank are zero
+    assert np.array_equal(R[:, piv], np.eye(rank, dtype=np.uint64))
+    assert not got[rank:].any()
+    assert all(R[i, :piv[i]].sum() == 0 for i in range(0, rank, 97))
+    # every row of A is the combination A[i, piv] of the RREF rows
+    sel = np.arange(0, 2000, 37)
+    recon = (A[sel][:, piv].astype(np.uint64) @ R) % P
+    assert np.array_equal(recon, A[sel].astype(np.uint64))
