@@ -134,6 +134,42 @@ int mlz_zp_trsm(mlz_ctx *ctx, mlz_elem elem, uint64_t p, int upper, int unit_dia
 int mlz_zp_rref(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t n, void *A, size_t lda,
 		size_t *rank, uint32_t *pivots, int algo);
 
+/* Faugere-Lachartre reduction of an F4-shaped matrix X (m x n, CSR: row_ptr
+ * [m+1], col [nnz] strictly increasing within each row, val [nnz] in the
+ * element storage) to its reduced row-echelon form R (m x n dense, ldr; may be
+ * NULL to compute only rank/pivots).  pivots receives rank column indices
+ * (array of min(m, n) entries, may be NULL).  det receives the product of the
+ * heads of the scanned rows of X (lela/algorithms/faugere-lachartre.tcc:119).
+ * Replaces FaugereLachartre<Modular<T>>::echelonize(R, X, rank, det, reduced=true)
+ * (lela/algorithms/faugere-lachartre.tcc:735-917).
+ * MLZ_E_WRONG_FORM: a head decreases at row *bad_row (WrongMatrixForm, :80-81).
+ * MLZ_E_NOT_INVERTIBLE: a head has no inverse mod p (LELA then returns with R
+ * unchanged, :790-799); *rank is the number of pivot rows of X. */
+int mlz_zp_fl_echelonize(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t n,
+			 const uint64_t *row_ptr, const uint32_t *col, const void *val, void *R,
+			 size_t ldr, size_t *rank, uint32_t *pivots, uint64_t *det, size_t *bad_row,
+			 int algo);
+
+/* ---- GF(2) ------------------------------------------------------------- */
+
+/* C <- a*A*B + b*C over GF(2) (a, b in {0, 1}); A m x k, B k x n, C m x n bits,
+ * leading dimensions in 64-bit words.  Bits of C beyond column n are left as
+ * they are; bits of A beyond k and of B beyond n are ignored.
+ * Replaces BLAS3::gemm for GF2: _gemm<GF2, M4RIModule::Tag> -> mzd_mul /
+ * mzd_addmul (lela/blas/level3-m4ri.h:89-108, lela/blas/level3-m4ri.tcc:61-86)
+ * and the no-M4RI GenericModule path (lela/blas/level3-generic.tcc:243-263,
+ * lela/blas/level2-gf2.tcc:28-55). */
+int mlz_gf2_gemm(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda_w,
+		 const uint64_t *B, size_t ldb_w, int b, uint64_t *C, size_t ldc_w);
+
+/* Reduced row-echelon form over GF(2) in place (standard layout as
+ * mlz_zp_rref; bits beyond column n are written as zero).
+ * Replaces EchelonForm<GF2>::echelonize(A, reduced=true)
+ * (lela/solutions/echelon-form-gf2.h:97-134: mzd_echelonize with M4RI,
+ * GaussJordan / Elimination without it). */
+int mlz_gf2_rref(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda_w, size_t *rank,
+		 uint32_t *pivots);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,7 +24,8 @@ from ._lib import (ALGO_AUTO, ALGO_FP64, ALGO_INT8, ALGO_PLAIN, MLZ_F64_SYM, MLZ
 __all__ = [
     "Modular", "Context", "BLAS3", "DeviceMatrix", "LELAError", "DiagonalEntryNotInvertible",
     "WrongMatrixForm", "LowerTriangular", "UpperTriangular", "ALGO_AUTO", "ALGO_PLAIN",
-    "ALGO_INT8", "ALGO_FP64",
+    "ALGO_INT8", "ALGO_FP64", "EchelonForm", "GF2", "Dense01Matrix", "FaugereLachartre",
+    "dense_to_csr",
 ]
 
 LowerTriangular = "lower"
@@ -243,6 +244,8 @@ class BLAS3:
     def gemm(ctx, a, A, B, b, C, algo=ALGO_AUTO):
         """C <- a*A*B + b*C (lela/blas/level3.h:81-83)."""
         F = ctx.F
+        if isinstance(F, GF2):
+            return _gf2_gemm(ctx, a, A, B, b, C)
         pa, lda, m, k = _operand(ctx, A, F.dtype)
         pb, ldb, kb, n = _operand(ctx, B, F.dtype)
         pc, ldc, mc, nc = _operand(ctx, C, F.dtype)
@@ -280,6 +283,10 @@ class EchelonForm:
         self.pivots = None
 
     def echelonize(self, A, reduced=True, algo=ALGO_AUTO):
+        if isinstance(self.ctx.F, GF2):
+            if not reduced:
+                raise LELAError("only the reduced form is computed on the GPU")
+            return _gf2_echelonize(self, A)
         if not reduced:
             raise LELAError("only the reduced form is computed on the GPU "
                             "(the non-reduced form depends on LELA's pivot order)")
@@ -296,3 +303,131 @@ class EchelonForm:
 
     def rank(self, A):
         return self._rank_table[id(A)]
+
+
+class GF2:
+    """The field with two elements (lela/ring/gf2.h:48-293); matrices are
+    Dense01Matrix objects (64-bit words, LSB first)."""
+    p = 2
+
+    def __repr__(self):
+        return "GF2"
+
+
+class Dense01Matrix:
+    """Bit-packed dense GF(2) matrix: `words` is a (rows x ceil(cols/64))
+    uint64 array (or a DeviceMatrix of words); bit j of word w is column
+    64*w + j (lela/matrix/dense-zero-one.h:132-360)."""
+
+    def __init__(self, words, ncols):
+        self.words = words
+        self.ncols = int(ncols)
+
+    def rowdim(self):
+        return self.words.rows if isinstance(self.words, DeviceMatrix) else self.words.shape[0]
+
+    def coldim(self):
+        return self.ncols
+
+
+def _gf2_operand(ctx, X):
+    if not isinstance(X, Dense01Matrix):
+        raise TypeError("GF(2) operands must be Dense01Matrix")
+    ptr, ld, rows, _ = _operand(ctx, X.words, np.dtype(np.uint64))
+    return ptr, ld, rows, X.ncols
+
+
+def _gf2_gemm(ctx, a, A, B, b, C):
+    pa, lda, m, k = _gf2_operand(ctx, A)
+    pb, ldb, kb, n = _gf2_operand(ctx, B)
+    pc, ldc, mc, nc = _gf2_operand(ctx, C)
+    if kb != k or mc != m or nc != n:
+        raise LELAError("gemm: dimension mismatch")
+    ctx.check(ctx.lib.mlz_gf2_gemm(ctx.handle, m, n, k, int(a) & 1, pa, lda, pb, ldb, int(b) & 1,
+                                   pc, ldc), "gf2 gemm")
+    return C
+
+
+def _gf2_echelonize(EF, A):
+    ctx = EF.ctx
+    pa, lda, m, n = _gf2_operand(ctx, A)
+    rank = ctypes.c_size_t(0)
+    piv = np.zeros(max(1, min(m, n)), dtype=np.uint32)
+    ctx.check(ctx.lib.mlz_gf2_rref(ctx.handle, m, n, pa, lda, ctypes.byref(rank), piv.ctypes.data),
+              "gf2 rref")
+    EF._rank_table[id(A)] = rank.value
+    EF.pivots = piv[:rank.value].copy()
+    return A
+
+
+def dense_to_csr(X):
+    """Row-compressed form (row_ptr uint64, col uint32, values) of a dense
+    matrix, dropping zeros -- LELA's sparse rows store only nonzeros
+    (lela/vector/sparse.h:311-491)."""
+    X = np.asarray(X)
+    rows, cols = np.nonzero(X)
+    row_ptr = np.zeros(X.shape[0] + 1, dtype=np.uint64)
+    np.add.at(row_ptr, rows + 1, 1)
+    row_ptr = np.cumsum(row_ptr, dtype=np.uint64)
+    return row_ptr, cols.astype(np.uint32), X[rows, cols]
+
+
+class FaugereLachartre:
+    """FaugereLachartre<Modular<T>, HIPModule> (lela/algorithms/faugere-lachartre.h:40-81)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.pivots = None
+
+    def echelonize(self, R, X, reduced=True, algo=ALGO_AUTO, shape=None):
+        """R <- reduced row-echelon form of X; returns (rank, det).
+
+        X is a dense array or a CSR triple (row_ptr, col, val) with `shape`.
+        det is the product of the heads of X's scanned rows
+        (faugere-lachartre.tcc:119).  WrongMatrixForm is raised for a
+        decreasing head; when a head is not invertible (composite modulus)
+        LELA reports it and returns with R unchanged, and so does this call."""
+        if not reduced:
+            raise LELAError("only the reduced form is computed on the GPU")
+        ctx, F = self.ctx, self.ctx.F
+        if isinstance(X, tuple):
+            row_ptr, col, val = X
+            m, n = shape
+        else:
+            m, n = X.shape
+            row_ptr, col, val = dense_to_csr(X)
+        on_device = isinstance(R, DeviceMatrix) or isinstance(row_ptr, DeviceMatrix)
+        if on_device:
+            # resident inputs and output: every pointer must be device memory
+            if not all(isinstance(v, DeviceMatrix) for v in (R, row_ptr, col, val)):
+                raise TypeError("device FL needs R, row_ptr, col and val as DeviceMatrix")
+            if ctx._ptr_mode != PTR_DEVICE:
+                ctx.set_ptr_mode(PTR_DEVICE)
+            return self._call(m, n, row_ptr.ptr, col.ptr, val.ptr, R.ptr, R.ld, algo)
+        row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint64)
+        col = np.ascontiguousarray(col, dtype=np.uint32)
+        val = np.ascontiguousarray(val, dtype=F.dtype)
+        if R is not None:
+            pr, ldr, rm, rn = _operand(ctx, R, F.dtype)
+            if (rm, rn) != (m, n):
+                raise LELAError("R must have the shape of X")
+        else:
+            pr, ldr = None, n
+            if ctx._ptr_mode != PTR_HOST:
+                ctx.set_ptr_mode(PTR_HOST)
+        return self._call(m, n, row_ptr.ctypes.data, col.ctypes.data, val.ctypes.data, pr, ldr, algo)
+
+    def _call(self, m, n, row_ptr, col, val, pr, ldr, algo):
+        ctx, F = self.ctx, self.ctx.F
+        rank = ctypes.c_size_t(0)
+        det = ctypes.c_uint64(0)
+        bad = ctypes.c_size_t(0)
+        piv = np.zeros(max(1, min(m, n)), dtype=np.uint32)
+        rc = ctx.lib.mlz_zp_fl_echelonize(ctx.handle, F.elem, F.p, m, n, row_ptr, col, val, pr, ldr,
+                                          ctypes.byref(rank), piv.ctypes.data, ctypes.byref(det),
+                                          ctypes.byref(bad), algo)
+        if rc == _lib.MLZ_E_NOT_INVERTIBLE:
+            return rank.value, det.value
+        ctx.check(rc, "fl_echelonize", idx=bad.value if rc == _lib.MLZ_E_WRONG_FORM else None)
+        self.pivots = piv[:rank.value].copy()
+        return rank.value, det.value

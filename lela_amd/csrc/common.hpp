@@ -194,6 +194,14 @@ int zp_gemm_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, size_t k
 		uint64_t b, void *C, size_t ldc, int algo);
 int zp_scal_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, uint64_t a,
 		void *C, size_t ldc);
+int zp_trsm_dev(mlz_ctx *ctx, int elem, uint64_t p, int upper, int unit, size_t n, size_t nrhs,
+		uint64_t a, const void *A, size_t lda, void *B, size_t ldb, size_t *bad_diag, int algo);
+int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A, size_t lda,
+		size_t *rank_out, uint32_t *pivots_out, int algo);
+int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda,
+		 const uint64_t *B, size_t ldb, int b, uint64_t *C, size_t ldc);
+int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size_t *rank_out,
+		 uint32_t *pivots_out);
 int pick_gemm_algo(uint64_t p, int elem, size_t m, size_t n, size_t k, int algo);
 
 // Host <-> device staging of a strided matrix (rows x cols elements of esize

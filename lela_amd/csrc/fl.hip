@@ -1,0 +1,360 @@
+// lela_amd/csrc/fl.hip -- Faugere-Lachartre reduction of an F4-shaped matrix
+// to reduced row-echelon form on MI355X.
+//
+// Follows FaugereLachartre::echelonize(R, X, rank, det, reduced=true)
+// (lela/algorithms/faugere-lachartre.tcc:735-917):
+//   1. row scan of setup_splicer (:47-146): each row's head (first nonzero
+//      column); a head larger than the last starts a pivot row (A|B), a
+//      repeated head marks a C|D row, a smaller head is WrongMatrixForm(row),
+//      the first zero row ends the scan; det *= every scanned head (:119).
+//      This scan is sequential and O(m), so it runs on the host from the
+//      heads a kernel computes.
+//   2. MatrixGrid1 split (:148-275) into dense A (pivot rows x pivot columns,
+//      upper triangular), B, C, D by one scatter kernel over the nonzeros.
+//   3. B <- A^-1 B   (:791, upper, non-unit; zp_trsm.hip)
+//   4. D <- D - C B  (:805, Schur complement; zp_gemm.hip)
+//   5. D <- RREF(D)  (:822-852; zp_rref.hip) -- LELA takes the non-reduced
+//      echelon form and reduces it with D1^-1 D2 (:861); RREF(D) is that
+//      result directly.
+//   6. B <- B - B[:, J_D] RREF(D)  (:883-892, B2 - B1 D1^-1 D2)
+//   7. assembly of the standard-layout RREF (MatrixGrid3, :901; :363-504).
+// det: the factor from the X scan (:119) is returned; the factor LELA adds
+// from the D scan (:833) depends on its non-canonical echelon form of D and
+// is not reproduced.
+#include "common.hpp"
+
+#include <algorithm>
+
+namespace mlz {
+
+// head[i] = first column whose value is nonzero mod p, -1 for a zero row;
+// hval[i] = that value (canonical).
+template <int E>
+__global__ void fl_heads_kernel(const uint64_t *__restrict__ row_ptr, const uint32_t *__restrict__ col,
+				const typename Elem<E>::T *__restrict__ val, int m, uint32_t p, int *head,
+				uint32_t *hval)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= m)
+		return;
+	int h = -1;
+	uint32_t hv = 0;
+	for (uint64_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+		uint32_t v = Elem<E>::load(val + e, p) % p;
+		if (v != 0) {
+			h = (int) col[e];
+			hv = v;
+			break;
+		}
+	}
+	head[i] = h;
+	hval[i] = hv;
+}
+
+__global__ void fl_zero_kernel(uint32_t *__restrict__ X, size_t count)
+{
+	const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < count)
+		X[i] = 0;
+}
+
+// rowmap[i] = index among pivot rows (>= 0) or -(1 + index among the others);
+// colmap[j] likewise for columns.  A: r x r, B: r x nr, C: mr x r, D: mr x nr.
+template <int E>
+__global__ void fl_scatter_kernel(const uint64_t *__restrict__ row_ptr, const uint32_t *__restrict__ col,
+				  const typename Elem<E>::T *__restrict__ val, int m, uint32_t p,
+				  const int *__restrict__ rowmap, const int *__restrict__ colmap, int r, int nr,
+				  uint32_t *__restrict__ A, uint32_t *__restrict__ B, uint32_t *__restrict__ C,
+				  uint32_t *__restrict__ D)
+{
+	const int i = blockIdx.x;
+	if (i >= m)
+		return;
+	const int ri = rowmap[i];
+	for (uint64_t e = row_ptr[i] + threadIdx.x; e < row_ptr[i + 1]; e += blockDim.x) {
+		const uint32_t v = Elem<E>::load(val + e, p) % p;
+		const int cj = colmap[col[e]];
+		if (ri >= 0) {
+			if (cj >= 0)
+				A[(size_t) ri * r + cj] = v;
+			else
+				B[(size_t) ri * nr + (-1 - cj)] = v;
+		} else {
+			if (cj >= 0)
+				C[(size_t) (-1 - ri) * r + cj] = v;
+			else
+				D[(size_t) (-1 - ri) * nr + (-1 - cj)] = v;
+		}
+	}
+}
+
+// G[i][s] = B[i][jd[s]]  (r x rd)
+__global__ void fl_gather_cols_kernel(const uint32_t *__restrict__ B, int r, int nr, const int *__restrict__ jd,
+				      int rd, uint32_t *__restrict__ G)
+{
+	const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= (size_t) r * rd)
+		return;
+	const int i = (int) (idx / rd), s = (int) (idx % rd);
+	G[idx] = B[(size_t) i * nr + jd[s]];
+}
+
+// Output row t: src[t] >= 0 -> X pivot row src[t] (B row), pivot column
+// pcol[t]; src[t] < 0 -> D pivot row -1-src[t] (RREF(D) row); t >= rank -> 0.
+template <int E>
+__global__ void fl_assemble_kernel(const uint32_t *__restrict__ B, const uint32_t *__restrict__ D, int nr,
+				   const int *__restrict__ colmap, const int *__restrict__ src,
+				   const int *__restrict__ pcol, int rank, int m, int n, uint32_t p,
+				   typename Elem<E>::T *__restrict__ R, size_t ldr)
+{
+	const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= (size_t) m * n)
+		return;
+	const int t = (int) (idx / n), j = (int) (idx % n);
+	uint32_t v = 0;
+	if (t < rank) {
+		const int cj = colmap[j];
+		const int s = src[t];
+		if (cj >= 0)
+			v = (s >= 0 && j == pcol[t]) ? 1u : 0u;
+		else
+			v = s >= 0 ? B[(size_t) s * nr + (-1 - cj)] : D[(size_t) (-1 - s) * nr + (-1 - cj)];
+	}
+	Elem<E>::store(R + (size_t) t * ldr + j, v % p, p);
+}
+
+struct FlResult {
+	size_t rank;
+	uint64_t det;
+	size_t bad_row;
+};
+
+template <int E>
+static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *d_rowptr, const uint32_t *d_col,
+		  const void *d_val, void *R, size_t ldr, uint32_t *pivots_out, FlResult *res, int algo)
+{
+	typedef typename Elem<E>::T T;
+	const ModP mp = make_modp(p);
+	// 1. heads (device) -> row scan (host)
+	void *w;
+	MLZ_TRY(ws_get(ctx, WS_FL_0, (2 * m + n + 16) * sizeof(int) + m * sizeof(uint32_t), &w));
+	int *d_head = (int *) w;
+	uint32_t *d_hval = (uint32_t *) (d_head + m);
+	int *d_rowmap = (int *) (d_hval + m);
+	int *d_colmap = d_rowmap + m;
+	fl_heads_kernel<E><<<ceil_div(m, 256), 256, 0, ctx->stream>>>(d_rowptr, d_col, (const T *) d_val, (int) m, p,
+								       d_head, d_hval);
+	MLZ_TRY(check_launch(ctx, "fl_heads_kernel"));
+	std::vector<int> head(m);
+	std::vector<uint32_t> hval(m);
+	MLZ_HIP(ctx, hipMemcpyAsync(head.data(), d_head, m * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+	MLZ_HIP(ctx, hipMemcpyAsync(hval.data(), d_hval, m * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+	MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+
+	std::vector<char> is_piv_row(m, 0), is_piv_col(n, 0);
+	uint64_t det = 1 % p;
+	long last = -1;
+	size_t r = 0;
+	for (size_t i = 0; i < m; ++i) {
+		const long h = head[i];
+		if (h < 0)
+			break;
+		if (h < last) {
+			res->bad_row = i;
+			return set_error(ctx, MLZ_E_WRONG_FORM, "Bad input-matrix for Faugere-Lachartre at row-index %zu", i);
+		}
+		if (h > last) {
+			is_piv_row[i] = 1;
+			is_piv_col[h] = 1;
+			last = h;
+			++r;
+		}
+		det = mp.mul((uint32_t) det, hval[i]);
+	}
+	res->det = det;
+	const size_t mr = m - r, nr = n - r;
+	std::vector<int> rowmap(m), colmap(n), piv_cols_x;
+	piv_cols_x.reserve(r);
+	std::vector<int> res_cols;
+	res_cols.reserve(nr);
+	for (size_t i = 0, a = 0, b = 0; i < m; ++i)
+		rowmap[i] = is_piv_row[i] ? (int) a++ : -1 - (int) b++;
+	for (size_t j = 0, a = 0, b = 0; j < n; ++j) {
+		if (is_piv_col[j]) {
+			colmap[j] = (int) a++;
+			piv_cols_x.push_back((int) j);
+		} else {
+			colmap[j] = -1 - (int) b++;
+			res_cols.push_back((int) j);
+		}
+	}
+	MLZ_HIP(ctx, hipMemcpyAsync(d_rowmap, rowmap.data(), m * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+	MLZ_HIP(ctx, hipMemcpyAsync(d_colmap, colmap.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+
+	// 2. dense blocks
+	void *wa, *wb, *wc, *wd;
+	MLZ_TRY(ws_get(ctx, WS_FL_1, r * r * 4 + 256, &wa));
+	MLZ_TRY(ws_get(ctx, WS_FL_2, r * nr * 4 + 256, &wb));
+	MLZ_TRY(ws_get(ctx, WS_FL_3, mr * r * 4 + 256, &wc));
+	MLZ_TRY(ws_get(ctx, WS_FL_4, mr * nr * 4 + 256, &wd));
+	uint32_t *dA = (uint32_t *) wa, *dB = (uint32_t *) wb, *dC = (uint32_t *) wc, *dD = (uint32_t *) wd;
+	{
+		ProfScope ps(ctx, "fl_split");
+		const size_t cnt[4] = {r * r, r * nr, mr * r, mr * nr};
+		uint32_t *bufs[4] = {dA, dB, dC, dD};
+		for (int q = 0; q < 4; ++q)
+			if (cnt[q])
+				fl_zero_kernel<<<ceil_div(cnt[q], 256), 256, 0, ctx->stream>>>(bufs[q], cnt[q]);
+		fl_scatter_kernel<E><<<(unsigned) m, 128, 0, ctx->stream>>>(d_rowptr, d_col, (const T *) d_val, (int) m, p,
+									   d_rowmap, d_colmap, (int) r, (int) nr, dA, dB,
+									   dC, dD);
+		MLZ_TRY(check_launch(ctx, "fl_scatter_kernel"));
+	}
+	// 3. B <- A^-1 B
+	if (r && nr) {
+		size_t bad = 0;
+		int rc = zp_trsm_dev(ctx, MLZ_U32, p, 1, 0, r, nr, 1, dA, r, dB, nr, &bad, algo);
+		if (rc != MLZ_OK) {
+			res->rank = r;
+			return rc;
+		}
+	}
+	// 4. D <- D - C B
+	if (mr && nr && r)
+		MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, mr, nr, r, p - 1, dC, r, dB, nr, 1, dD, nr, algo));
+	// 5. RREF(D)
+	size_t rd = 0;
+	std::vector<uint32_t> jd(std::min(mr, nr) + 1);
+	if (mr && nr)
+		MLZ_TRY(zp_rref_dev(ctx, MLZ_U32, p, mr, nr, dD, nr, &rd, jd.data(), algo));
+	// 6. B <- B - B[:, J_D] RREF(D)[0:rd]
+	if (r && rd) {
+		void *wg, *wj;
+		MLZ_TRY(ws_get(ctx, WS_FL_5, r * rd * 4 + 256, &wg));
+		MLZ_TRY(ws_get(ctx, WS_FL_6, rd * sizeof(int) + 256, &wj));
+		std::vector<int> jdi(jd.begin(), jd.begin() + rd);
+		MLZ_HIP(ctx, hipMemcpyAsync(wj, jdi.data(), rd * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+		fl_gather_cols_kernel<<<ceil_div(r * rd, 256), 256, 0, ctx->stream>>>(dB, (int) r, (int) nr, (const int *) wj,
+										     (int) rd, (uint32_t *) wg);
+		MLZ_TRY(check_launch(ctx, "fl_gather_cols_kernel"));
+		MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, r, nr, rd, p - 1, wg, rd, dD, nr, 1, dB, nr, algo));
+		MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));   // jdi leaves scope
+	}
+	// 7. assembly: merge the X pivots and the D pivots by column
+	const size_t rank = r + rd;
+	std::vector<int> src(rank + 1), pcol(rank + 1);
+	{
+		size_t ia = 0, id = 0, t = 0;
+		while (ia < r || id < rd) {
+			const long ca = ia < r ? piv_cols_x[ia] : (long) n + 1;
+			const long cd = id < rd ? res_cols[jd[id]] : (long) n + 1;
+			if (ca < cd) {
+				src[t] = (int) ia;
+				pcol[t] = (int) ca;
+				++ia;
+			} else {
+				src[t] = -1 - (int) id;
+				pcol[t] = (int) cd;
+				++id;
+			}
+			++t;
+		}
+	}
+	void *wm;
+	MLZ_TRY(ws_get(ctx, WS_FL_7, 2 * (rank + 1) * sizeof(int), &wm));
+	int *d_src = (int *) wm, *d_pcol = d_src + rank + 1;
+	MLZ_HIP(ctx, hipMemcpyAsync(d_src, src.data(), (rank + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+	MLZ_HIP(ctx, hipMemcpyAsync(d_pcol, pcol.data(), (rank + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+	if (R) {
+		ProfScope ps(ctx, "fl_assemble");
+		fl_assemble_kernel<E><<<ceil_div(m * n, 256), 256, 0, ctx->stream>>>(dB, dD, (int) nr, d_colmap, d_src, d_pcol,
+										    (int) rank, (int) m, (int) n, p, (T *) R,
+										    ldr);
+		MLZ_TRY(check_launch(ctx, "fl_assemble_kernel"));
+	}
+	MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+	if (pivots_out)
+		for (size_t t = 0; t < rank; ++t)
+			pivots_out[t] = (uint32_t) pcol[t];
+	res->rank = rank;
+	return MLZ_OK;
+}
+
+}  // namespace mlz
+
+using namespace mlz;
+
+extern "C" int mlz_zp_fl_echelonize(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t n,
+				    const uint64_t *row_ptr, const uint32_t *col, const void *val, void *R,
+				    size_t ldr, size_t *rank, uint32_t *pivots, uint64_t *det, size_t *bad_row,
+				    int algo)
+{
+	if (!ctx)
+		return MLZ_E_ARG;
+	if (p < 2 || (elem == MLZ_U16 && p > 65535) || (elem == MLZ_U32 && p > 0xffffffffull) ||
+	    (elem == MLZ_F64_SYM && p >= 67108864ull) || elem_size(elem) == 0)
+		return set_error(ctx, MLZ_E_ARG, "bad modulus or element type");
+	if (!row_ptr || (m && n && R && ldr < n))
+		return set_error(ctx, MLZ_E_ARG, "bad CSR or output pointer");
+	if (m > (size_t) INT32_MAX / 2 || n > (size_t) INT32_MAX / 2)
+		return set_error(ctx, MLZ_E_ARG, "dimension too large");
+	MLZ_HIP(ctx, hipSetDevice(ctx->device));
+	FlResult res = {0, 1 % p, 0};
+	int rc;
+	const size_t es = elem_size(elem);
+	if (m == 0 || n == 0) {
+		if (rank) *rank = 0;
+		if (det) *det = 1 % p;
+		return MLZ_OK;
+	}
+	const uint64_t *d_rowptr;
+	const uint32_t *d_col;
+	const void *d_val;
+	void *dR = R;
+	if (ctx->ptr_mode == MLZ_PTR_DEVICE) {
+		d_rowptr = row_ptr;
+		d_col = col;
+		d_val = val;
+	} else {
+		// host CSR: validate (sorted, in-range columns) and stage
+		const uint64_t nnz = row_ptr[m];
+		for (size_t i = 0; i < m; ++i) {
+			if (row_ptr[i + 1] < row_ptr[i])
+				return set_error(ctx, MLZ_E_ARG, "row_ptr not monotone at row %zu", i);
+			for (uint64_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e)
+				if (col[e] >= n || (e > row_ptr[i] && col[e] <= col[e - 1]))
+					return set_error(ctx, MLZ_E_ARG, "row %zu: columns not sorted or out of range", i);
+		}
+		void *a, *b, *c;
+		MLZ_TRY(ws_get(ctx, WS_STAGE_A, (m + 1) * 8, &a));
+		MLZ_TRY(ws_get(ctx, WS_STAGE_B, nnz * 4 + 8, &b));
+		MLZ_TRY(ws_get(ctx, WS_STAGE_C, nnz * es + 8, &c));
+		MLZ_HIP(ctx, hipMemcpyAsync(a, row_ptr, (m + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+		if (nnz) {
+			MLZ_HIP(ctx, hipMemcpyAsync(b, col, nnz * 4, hipMemcpyHostToDevice, ctx->stream));
+			MLZ_HIP(ctx, hipMemcpyAsync(c, val, nnz * es, hipMemcpyHostToDevice, ctx->stream));
+		}
+		d_rowptr = (const uint64_t *) a;
+		d_col = (const uint32_t *) b;
+		d_val = c;
+		dR = nullptr;
+		if (R)
+			MLZ_TRY(ws_get(ctx, WS_BATCH_0, m * n * es, &dR));
+	}
+	const uint32_t pp = (uint32_t) p;
+	const size_t ldr_dev = ctx->ptr_mode == MLZ_PTR_DEVICE ? ldr : n;
+	if (elem == MLZ_U16)
+		rc = fl_run<MLZ_U16>(ctx, pp, m, n, d_rowptr, d_col, d_val, dR, ldr_dev, pivots, &res, algo);
+	else if (elem == MLZ_U32)
+		rc = fl_run<MLZ_U32>(ctx, pp, m, n, d_rowptr, d_col, d_val, dR, ldr_dev, pivots, &res, algo);
+	else
+		rc = fl_run<MLZ_F64_SYM>(ctx, pp, m, n, d_rowptr, d_col, d_val, dR, ldr_dev, pivots, &res, algo);
+	if (rank) *rank = res.rank;
+	if (det) *det = res.det;
+	if (bad_row) *bad_row = res.bad_row;
+	if (rc != MLZ_OK)
+		return rc;
+	if (ctx->ptr_mode != MLZ_PTR_DEVICE && R)
+		return stage_out(ctx, R, dR, m, n, ldr, es);
+	return MLZ_OK;
+}

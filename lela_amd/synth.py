@@ -62,6 +62,67 @@ def zp_matrix(seed, rows, cols, p, dtype=np.uint32):
     return uniform_mod(seed, rows * cols, p).astype(dtype).reshape(rows, cols)
 
 
+def fl_matrix(seed, p, na=15000, nc=5000, nb=15000, band=600, nnz_a=150, density=0.02,
+              chunk=500):
+    """Faugere-Lachartre-shaped X = [A B; C D] in CSR (row_ptr, col, val uint32):
+    A na x na upper triangular, A[i][i] = 1 plus nnz_a-1 distinct columns drawn
+    from (i, min(i+band, na)); B, C, D entries nonzero with probability
+    `density`, values uniform in [1, p).  Each C|D row whose first nonzero is in
+    column h is placed right after A's row h (F4 row order); an all-zero C part
+    is redrawn so the row cannot become a spurious pivot row."""
+    rng = np.random.default_rng(seed)
+    n = na + nb
+    a_rows = []
+    for start in range(0, na, chunk):
+        stop = min(na, start + chunk)
+        Bmask = rng.random((stop - start, nb)) < density
+        for i in range(start, stop):
+            hi = min(i + band, na)
+            avail = hi - (i + 1)
+            k = min(nnz_a - 1, avail)
+            off = rng.choice(avail, size=k, replace=False) + i + 1 if k > 0 else np.zeros(0, np.int64)
+            bcols = np.nonzero(Bmask[i - start])[0] + na
+            cols = np.concatenate(([i], np.sort(off), bcols)).astype(np.uint32)
+            vals = rng.integers(1, p, size=cols.size, dtype=np.uint64).astype(np.uint32)
+            vals[0] = 1
+            a_rows.append((cols, vals))
+    c_rows = []
+    for start in range(0, nc, chunk):
+        stop = min(nc, start + chunk)
+        Cmask = rng.random((stop - start, na)) < density
+        Dmask = rng.random((stop - start, nb)) < density
+        for t in range(stop - start):
+            while not Cmask[t].any():
+                Cmask[t] = rng.random(na) < density
+            cols = np.concatenate((np.nonzero(Cmask[t])[0], np.nonzero(Dmask[t])[0] + na))
+            vals = rng.integers(1, p, size=cols.size, dtype=np.uint64).astype(np.uint32)
+            c_rows.append((cols.astype(np.uint32), vals))
+    # F4 order: C rows after the A row of their head, stable in generation order
+    heads = np.array([r[0][0] for r in c_rows], dtype=np.int64)
+    order = np.argsort(heads, kind="stable")
+    rows = []
+    ci = 0
+    for i in range(na):
+        rows.append(a_rows[i])
+        while ci < nc and heads[order[ci]] == i:
+            rows.append(c_rows[order[ci]])
+            ci += 1
+    lens = np.array([r[0].size for r in rows], dtype=np.uint64)
+    row_ptr = np.zeros(len(rows) + 1, dtype=np.uint64)
+    row_ptr[1:] = np.cumsum(lens)
+    col = np.concatenate([r[0] for r in rows])
+    val = np.concatenate([r[1] for r in rows])
+    return (row_ptr, col, val), (len(rows), n)
+
+
+def csr_to_dense(csr, shape, dtype=np.uint32):
+    row_ptr, col, val = csr
+    X = np.zeros(shape, dtype=dtype)
+    rows = np.repeat(np.arange(shape[0]), np.diff(row_ptr.astype(np.int64)))
+    X[rows, col.astype(np.int64)] = val
+    return X
+
+
 def gf2_matrix(seed, rows, cols):
     """rows x cols Bernoulli(1/2) bit matrix, packed LSB-first into uint64 words
     (row stride ceil(cols/64) words, unused tail bits zero)."""

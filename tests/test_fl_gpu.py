@@ -1,0 +1,121 @@
+"""GPU parity of mlz_zp_fl_echelonize against the oracle's restatement of
+FaugereLachartre::echelonize (bit-exact RREF and rank; det of the X scan),
+the reference test's own assertion that FL equals Elimination's reduced form
+(test-faugere-lachartre.C:227-243), WrongMatrixForm, and the 20000 x 30000
+workload through size-independent checks."""
+import numpy as np
+import pytest
+
+import lela_amd as la
+import oracle_lib as orc
+from lela_amd import synth
+from test_oracle import f4_matrix
+
+pytestmark = pytest.mark.gpu
+
+
+def x_scan_det(X, p):
+    """Product of the heads of the rows scanned by setup_splicer (:61-81,119)."""
+    d = 1 % p
+    for row in X:
+        nz = np.nonzero(row)[0]
+        if nz.size == 0:
+            break
+        d = d * int(row[nz[0]]) % p
+    return d
+
+
+def run_fl(ctx, p, storage, X, algo=la.ALGO_AUTO):
+    F = la.Modular(p, storage)
+    ctx.F = F
+    R = np.zeros(X.shape, dtype=F.dtype)
+    FL = la.FaugereLachartre(ctx)
+    rank, det = FL.echelonize(R, F.from_canonical(X), algo=algo)
+    return F.to_canonical(R), rank, det, FL.pivots
+
+
+@pytest.mark.parametrize("p", [101, 65521, 2147483647])
+@pytest.mark.parametrize("m,n,seed", [(24, 32, 1), (96, 128, 3), (30, 30, 4), (200, 260, 5), (300, 200, 6)])
+def test_fl_matches_oracle(gpu_ctx, p, m, n, seed):
+    X = f4_matrix(seed, m, n, p)
+    got, rank, det, piv = run_fl(gpu_ctx, p, "uint32", X)
+    rc, want, wrank, _, _ = orc.fl_echelonize(p, X)
+    assert rc == 0
+    assert rank == wrank and np.array_equal(got, want)
+    # the reference test's check: FL result == Elimination's reduced form
+    rc2, E, erank, epiv, _ = orc.zp_echelonize(p, X, reduced=True)
+    assert np.array_equal(got, E) and list(piv) == list(epiv)
+    assert det == x_scan_det(X, p)
+
+
+@pytest.mark.parametrize("storage,p", [("uint16", 65521), ("double", 67108859)])
+def test_fl_storages(gpu_ctx, storage, p):
+    X = f4_matrix(7, 150, 220, p)
+    got, rank, _, _ = run_fl(gpu_ctx, p, storage, X)
+    rc, want, wrank, _, _ = orc.fl_echelonize(p, X)
+    assert rank == wrank and np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("algo", [la.ALGO_PLAIN, la.ALGO_INT8])
+def test_fl_small_config4_shape(gpu_ctx, algo):
+    """The config-4 generator at 1/20 scale, against the oracle."""
+    p = 65521
+    csr, shape = synth.fl_matrix(11, p, na=750, nc=250, nb=750, band=60, nnz_a=15)
+    X = synth.csr_to_dense(csr, shape)
+    got, rank, det, _ = run_fl(gpu_ctx, p, "uint32", X, algo=algo)
+    rc, want, wrank, _, _ = orc.fl_echelonize(p, X)
+    assert rc == 0 and rank == wrank == 1000
+    assert np.array_equal(got, want)
+    assert det == x_scan_det(X, p)
+
+
+def test_fl_wrong_matrix_form(gpu_ctx):
+    p = 101
+    X = np.zeros((4, 6), dtype=np.uint32)
+    X[0, 1] = 1
+    X[1, 2] = 1
+    X[2, 0] = 1
+    X[3, 4] = 1
+    with pytest.raises(la.WrongMatrixForm) as ei:
+        run_fl(gpu_ctx, p, "uint32", X)
+    assert ei.value.row == 2
+
+
+def test_fl_zero_row_ends_scan(gpu_ctx):
+    # rows after the first zero row join the C|D part (faugere-lachartre.tcc:64-70, :137-142)
+    p = 101
+    X = f4_matrix(12, 40, 50, p)
+    X[10] = 0
+    X[25, :] = 0
+    X[25, 3] = 5                   # a small head after the zero row: not WrongMatrixForm
+    got, rank, _, _ = run_fl(gpu_ctx, p, "uint32", X)
+    rc, want, wrank, _, _ = orc.fl_echelonize(p, X)
+    assert rc == 0 and rank == wrank and np.array_equal(got, want)
+
+
+def test_fl_config4_full(gpu_ctx):
+    """20000 x 30000, p = 65521: rank, RREF structure on sampled rows, and
+    sampled rows of X lie in the row space of R."""
+    p = 65521
+    csr, shape = synth.fl_matrix(synth.seed_for(4, 0), p)
+    F = la.Modular(p, "uint32")
+    gpu_ctx.F = F
+    FL = la.FaugereLachartre(gpu_ctx)
+    R = np.zeros(shape, dtype=np.uint32)
+    rank, det = FL.echelonize(R, csr, shape=shape)
+    piv = FL.pivots
+    assert rank == 20000
+    assert np.all(np.diff(piv.astype(np.int64)) > 0)
+    assert not R[rank:].any()
+    P = np.uint64(p)
+    row_ptr, col, val = csr
+    for i in range(0, shape[0], 997):
+        assert R[i, piv[i]] == 1 and R[i, :piv[i]].sum() == 0
+        assert np.count_nonzero(R[i, piv]) == 1
+        x = np.zeros(shape[1], dtype=np.uint64)
+        s, e = int(row_ptr[i]), int(row_ptr[i + 1])
+        x[col[s:e].astype(np.int64)] = val[s:e]
+        coef = x[piv]
+        nzc = np.nonzero(coef)[0]
+        recon = (coef[nzc] @ R[nzc].astype(np.uint64)) % P
+        assert np.array_equal(recon, x)

@@ -1,0 +1,120 @@
+"""GPU parity of the GF(2) kernels (mlz_gf2_gemm, mlz_gf2_rref) against the
+oracle (bit-exact), plus the 16384^2 workloads through size-independent
+checks: Freivalds for the product; rank, RREF structure and row-space
+membership for the echelon form of a rank-12000 product."""
+import numpy as np
+import pytest
+
+import bruteforce as bf
+import lela_amd as la
+import oracle_lib as orc
+from lela_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_gemm(ctx, m, n, k, a, A, B, b, C):
+    ctx.F = la.GF2()
+    Cw = C.copy()
+    la.BLAS3.gemm(ctx, a, la.Dense01Matrix(A, k), la.Dense01Matrix(B, n), b, la.Dense01Matrix(Cw, n))
+    return Cw
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (5, 70, 9), (64, 64, 64), (130, 300, 129),
+                                   (1100, 257, 200), (300, 1030, 700), (2049, 64, 63)])
+@pytest.mark.parametrize("a,b", [(1, 1), (1, 0), (0, 1), (0, 0)])
+def test_gf2_gemm_matches_oracle(gpu_ctx, m, n, k, a, b):
+    A, B, C = synth.gf2_matrix(1, m, k), synth.gf2_matrix(2, k, n), synth.gf2_matrix(3, m, n)
+    got = gpu_gemm(gpu_ctx, m, n, k, a, A, B, b, C)
+    want = orc.gf2_gemm(m, n, k, a, A, B, b, C)
+    assert np.array_equal(got, want)
+
+
+def test_gf2_gemm_ignores_padding_bits(gpu_ctx):
+    m, n, k = 40, 100, 70
+    A, B, C = synth.gf2_matrix(4, m, k), synth.gf2_matrix(5, k, n), synth.gf2_matrix(6, m, n)
+    want = orc.gf2_gemm(m, n, k, 1, A, B, 1, C)
+    A2, B2 = A.copy(), B.copy()
+    A2[:, -1] |= ~np.uint64((1 << (k % 64)) - 1)       # garbage beyond column k
+    B2[:, -1] |= ~np.uint64((1 << (n % 64)) - 1)
+    got = gpu_gemm(gpu_ctx, m, n, k, 1, A2, B2, 1, C)
+    assert np.array_equal(bf.gf2_unpack(got, n), bf.gf2_unpack(want, n))
+    assert np.array_equal(got[:, -1] >> np.uint64(n % 64), C[:, -1] >> np.uint64(n % 64))
+
+
+def gpu_rref(ctx, m, n, A):
+    ctx.F = la.GF2()
+    R = la.Dense01Matrix(A.copy(), n)
+    EF = la.EchelonForm(ctx)
+    EF.echelonize(R)
+    return R.words, EF.rank(R), EF.pivots
+
+
+def lowrank_gf2(seed, m, n, r):
+    X, Y = synth.gf2_matrix(seed, m, r), synth.gf2_matrix(seed + 1, r, n)
+    Z = np.zeros((m, (n + 63) // 64), dtype=np.uint64)
+    return orc.gf2_gemm(m, n, r, 1, X, Y, 0, Z)
+
+
+@pytest.mark.parametrize("m,n,kind", [(10, 70, "rand"), (70, 10, "rand"), (300, 300, "rand"),
+                                      (500, 700, "low"), (1500, 400, "low"), (257, 1000, "sparse"),
+                                      (64, 64, "zero"), (2100, 900, "low")])
+def test_gf2_rref_matches_oracle(gpu_ctx, m, n, kind):
+    if kind == "rand":
+        A = synth.gf2_matrix(7, m, n)
+    elif kind == "low":
+        A = lowrank_gf2(8, m, n, min(m, n) // 3)
+    elif kind == "sparse":
+        A = synth.gf2_matrix(9, m, n) & synth.gf2_matrix(10, m, n) & synth.gf2_matrix(11, m, n)
+    else:
+        A = np.zeros((m, (n + 63) // 64), dtype=np.uint64)
+    got, rank, piv = gpu_rref(gpu_ctx, m, n, A)
+    want, wrank, wpiv = orc.gf2_echelonize(m, n, A, reduced=True)
+    assert rank == wrank and list(piv) == list(wpiv)
+    assert np.array_equal(got, want)
+
+
+def _gf2_matvec(W, ncols, x_bits):
+    """(W as bits) @ x mod 2 for a packed matrix W and a 0/1 vector x."""
+    out = np.zeros(W.shape[0], dtype=np.int64)
+    for start in range(0, W.shape[0], 1024):
+        out[start:start + 1024] = bf.gf2_unpack(W[start:start + 1024], ncols).astype(np.int64) @ x_bits
+    return out & 1
+
+
+def test_gf2_gemm_full_size_16384(gpu_ctx):
+    n = 16384
+    A = synth.gf2_matrix(synth.seed_for(3, 0), n, n)
+    B = synth.gf2_matrix(synth.seed_for(3, 1), n, n)
+    C = np.zeros_like(A)
+    got = gpu_gemm(gpu_ctx, n, n, n, 1, A, B, 0, C)
+    rng = np.random.default_rng(12)
+    for _ in range(2):
+        x = rng.integers(0, 2, size=n).astype(np.int64)
+        assert np.array_equal(_gf2_matvec(got, n, x), _gf2_matvec(A, n, _gf2_matvec(B, n, x)))
+
+
+def test_gf2_rref_rank12000_16384(gpu_ctx):
+    n, r = 16384, 12000
+    X = synth.gf2_matrix(synth.seed_for(3, 2), n, r)
+    Y = synth.gf2_matrix(synth.seed_for(3, 3), r, n)
+    M = gpu_gemm(gpu_ctx, n, n, r, 1, X, Y, 0, np.zeros((n, n // 64), dtype=np.uint64))
+    R, rank, piv = gpu_rref(gpu_ctx, n, n, M)
+    assert rank == r
+    assert not R[rank:].any()
+    assert np.all(np.diff(piv.astype(np.int64)) > 0)
+    # pivot columns hold the identity on the first rank rows (sampled rows)
+    for i in range(0, rank, 499):
+        bits = bf.gf2_unpack(R[i:i + 1], n)[0]
+        assert bits[piv[i]] == 1 and bits[:piv[i]].sum() == 0
+        assert bits[piv].sum() == 1
+    # sampled rows of M lie in the row space: row = sum of R rows at its pivot bits
+    Rb = None
+    for i in range(0, n, 1777):
+        row = bf.gf2_unpack(M[i:i + 1], n)[0]
+        sel = np.nonzero(row[piv])[0]
+        acc = np.zeros(n // 64, dtype=np.uint64)
+        for t in sel:
+            acc ^= R[t]
+        assert np.array_equal(acc, M[i])
+    del Rb
