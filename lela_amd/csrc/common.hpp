@@ -27,6 +27,9 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 struct ModP {
 	uint64_t p;
 	uint64_t mu;
+	double pinv;     // 1/p
+	int small;       // p < 2^26: products are exact in a double
+	int prime;       // p is prime: inverses by Fermat
 
 	__host__ __device__ inline uint64_t reduce(uint64_t x) const
 	{
@@ -40,10 +43,54 @@ struct ModP {
 		r = r >= p ? r - p : r;
 		return r;
 	}
-	// (x*y) mod p for x, y < p < 2^32.
+	// (x*y) mod p for x, y < p < 2^32.  For p < 2^26 the product is exact in a
+	// double; q = floor(x*y/p) is then off by at most one and the remainder
+	// x*y - q*p is exact (fma), so one correction each way suffices.
 	__host__ __device__ inline uint32_t mul(uint32_t x, uint32_t y) const
 	{
+		if (small) {
+			const double prod = (double) x * (double) y;
+			const double q = floor(prod * pinv);
+			double r = fma(-q, (double) p, prod);
+			r = r < 0.0 ? r + (double) p : r;
+			r = r >= (double) p ? r - (double) p : r;
+			return (uint32_t) r;
+		}
 		return (uint32_t) reduce((uint64_t) x * y);
+	}
+	// x^e mod p
+	__host__ __device__ inline uint32_t pow(uint32_t x, uint64_t e) const
+	{
+		uint32_t r = (uint32_t) (1 % p);
+		while (e) {
+			if (e & 1)
+				r = mul(r, x);
+			x = mul(x, x);
+			e >>= 1;
+		}
+		return r;
+	}
+	// Inverse of x mod p, 0 when none exists: Fermat for prime p, else the
+	// extended Euclidean algorithm.
+	__host__ __device__ inline uint32_t inv(uint32_t x) const
+	{
+		x = (uint32_t) (x % p);
+		if (x == 0)
+			return 0;
+		if (prime)
+			return pow(x, p - 2);
+		int64_t r0 = x, r1 = (int64_t) p, s0 = 1, s1 = 0;
+		while (r1 != 0) {
+			int64_t q = r0 / r1, t;
+			t = r0 - q * r1; r0 = r1; r1 = t;
+			t = s0 - q * s1; s0 = s1; s1 = t;
+		}
+		if (r0 != 1)
+			return 0;
+		s0 %= (int64_t) p;
+		if (s0 < 0)
+			s0 += (int64_t) p;
+		return (uint32_t) s0;
 	}
 	// (a*x + y) mod p for a, x, y < p < 2^32 (a*x + y < 2^64).
 	__host__ __device__ inline uint32_t axpy(uint32_t a, uint32_t x, uint32_t y) const
@@ -65,11 +112,50 @@ struct ModP {
 	}
 };
 
+// Deterministic Miller-Rabin for 32-bit n (bases 2, 7, 61 suffice below 2^32).
+inline bool is_prime_u32(uint64_t n)
+{
+	if (n < 2)
+		return false;
+	for (uint64_t q : {2ull, 3ull, 5ull, 7ull, 11ull, 13ull, 61ull})
+		if (n % q == 0)
+			return n == q;
+	uint64_t d = n - 1;
+	int s = 0;
+	while ((d & 1) == 0) {
+		d >>= 1;
+		++s;
+	}
+	for (uint64_t a : {2ull, 7ull, 61ull}) {
+		uint64_t x = 1, b = a % n, e = d;
+		while (e) {
+			if (e & 1)
+				x = (uint64_t) ((unsigned __int128) x * b % n);
+			b = (uint64_t) ((unsigned __int128) b * b % n);
+			e >>= 1;
+		}
+		if (x == 1 || x == n - 1)
+			continue;
+		bool comp = true;
+		for (int r = 1; r < s && comp; ++r) {
+			x = (uint64_t) ((unsigned __int128) x * x % n);
+			if (x == n - 1)
+				comp = false;
+		}
+		if (comp)
+			return false;
+	}
+	return true;
+}
+
 inline ModP make_modp(uint64_t p)
 {
 	ModP m;
 	m.p = p;
 	m.mu = ~(uint64_t) 0 / p;
+	m.pinv = 1.0 / (double) p;
+	m.small = p < (1ull << 26);
+	m.prime = is_prime_u32(p);
 	return m;
 }
 

@@ -20,6 +20,8 @@
 // (lela/solutions/echelon-form-gf2.h:97-134).
 #include "common.hpp"
 
+#include <climits>
+
 namespace mlz {
 
 constexpr int G2_TW = 4;                    // C words per workgroup tile (256 columns)
@@ -124,11 +126,12 @@ int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64
 
 constexpr int G2P = 4;              // panel width in words (256 columns)
 constexpr int G2_PB = G2P * 64;     // panel width in bits
-constexpr int G2_CHUNK = 1024;      // rows per tournament leaf
+constexpr int G2_SRPT = 2;           // candidate rows per thread in the pivot search
+constexpr int G2_CHUNK = 256 * G2_SRPT;  // rows per fast-path set / tournament leaf
 constexpr int G2_MAXLVL = 24;
 
 struct Gf2State {
-	int *is_pivot, *rank, *piv_rows, *piv_cols, *k, *J, *prow, *lists_rows, *lists_cnt, *counters;
+	int *is_pivot, *rank, *piv_rows, *piv_cols, *k, *J, *prow, *lists_rows, *lists_cnt, *counters, *need_tree;
 };
 
 __device__ inline int bit_of(const uint64_t *row, int col)
@@ -136,40 +139,76 @@ __device__ inline int bit_of(const uint64_t *row, int col)
 	return (int) ((row[col >> 6] >> (col & 63)) & 1ull);
 }
 
-// Echelon form of nr rows x wb bits (rows of G2P words in LDS); returns pivots.
-__device__ int gf2_lds_echelon(uint64_t (*M)[G2P], const int *rowid, int *alive, int nr, int wb, int *out_rows,
-			       int *out_cols, int *s_min, int *flip)
+__device__ inline int wave_min(int x)
+{
+#pragma unroll
+	for (int off = 32; off >= 1; off >>= 1)
+		x = min(x, __shfl_xor(x, off, 64));
+	return x;
+}
+
+// Gaussian elimination over GF(2) with the rows in registers: thread t holds
+// rows t + 256*q (q < RPT), WPR words each.  Bit columns 0..nsearch-1 are
+// processed in order; the pivot of a column is the live row of lowest index
+// with that bit set.  reduced = false clears the bit in live rows only
+// (echelon form); reduced = true clears it in every other row (Gauss-Jordan).
+// Two barriers per column: the arrival slot s_min alternates between two
+// words so it can be reset while the other is in use.
+template <int RPT, int WPR>
+__device__ int gf2_reg_eliminate(uint64_t (&v)[RPT][WPR], bool (&alive)[RPT], int nrows, int nsearch, bool reduced,
+				 uint64_t *pivot, int *s_min, int *out_rows, int *out_cols)
 {
 	const int tid = threadIdx.x;
+	if (tid == 0) {
+		s_min[0] = INT_MAX;
+		s_min[1] = INT_MAX;
+	}
+	__syncthreads();
 	int cnt = 0;
-	for (int col = 0; col < wb; ++col) {
+	for (int col = 0; col < nsearch; ++col) {
+		int cand = INT_MAX;
+#pragma unroll
+		for (int q = RPT - 1; q >= 0; --q) {
+			const int r = tid + 256 * q;
+			if (r < nrows && alive[q] && ((v[q][col >> 6] >> (col & 63)) & 1ull))
+				cand = r;
+		}
+		cand = wave_min(cand);
+		if ((tid & 63) == 0 && cand != INT_MAX)
+			atomicMin(&s_min[col & 1], cand);
+		__syncthreads();
+		const int pr = s_min[col & 1];
 		if (tid == 0)
-			*s_min = 0x7fffffff;
+			s_min[(col + 1) & 1] = INT_MAX;
+		if (pr != INT_MAX) {
+#pragma unroll
+			for (int q = 0; q < RPT; ++q)
+				if (tid + 256 * q == pr) {
+#pragma unroll
+					for (int w = 0; w < WPR; ++w)
+						pivot[w] = v[q][w];
+					alive[q] = false;
+				}
+			if (tid == 0) {
+				out_rows[cnt] = pr;
+				out_cols[cnt] = col;
+			}
+		}
 		__syncthreads();
-		for (int r = tid; r < nr; r += blockDim.x)
-			if (alive[r] && bit_of(M[r], col))
-				atomicMin(s_min, r);
-		__syncthreads();
-		const int pr = *s_min;
-		__syncthreads();
-		if (pr == 0x7fffffff)
+		if (pr == INT_MAX)
 			continue;
-		for (int r = tid; r < nr; r += blockDim.x)
-			flip[r] = (r != pr && alive[r] && bit_of(M[r], col));
-		if (tid == 0) {
-			alive[pr] = 0;
-			out_rows[cnt] = rowid[pr];
-			out_cols[cnt] = col;
+#pragma unroll
+		for (int q = 0; q < RPT; ++q) {
+			const int r = tid + 256 * q;
+			if (r != pr && (reduced || alive[q]) && ((v[q][col >> 6] >> (col & 63)) & 1ull)) {
+#pragma unroll
+				for (int w = 0; w < WPR; ++w)
+					v[q][w] ^= pivot[w];
+			}
 		}
-		__syncthreads();
-		for (int idx = tid; idx < nr * G2P; idx += blockDim.x) {
-			const int r = idx / G2P, w = idx % G2P;
-			if (flip[r])
-				M[r][w] ^= M[pr][w];
-		}
-		__syncthreads();
 		++cnt;
 	}
+	__syncthreads();
 	return cnt;
 }
 
@@ -181,27 +220,94 @@ __device__ inline int g2_nodes_at(int nodes0, int level)
 	return n;
 }
 
+// Fast path: the first G2_CHUNK rows that are not pivots yet.
+__global__ __launch_bounds__(256) void gf2_fast_kernel(const uint64_t *__restrict__ A, size_t lda, int m, int cw,
+						       int nw, int wb, Gf2State st)
+{
+	__shared__ int rowid[G2_CHUNK], outr[G2_PB], outc[G2_PB];
+	__shared__ uint64_t pivot[G2P];
+	__shared__ int s_min[2], s_cnt, s_wave[4];
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	if (tid == 0)
+		s_cnt = 0;
+	__syncthreads();
+	int scanned_all = 1;
+	for (int base = 0; base < m; base += 256) {
+		const int row = base + tid;
+		const bool live = row < m && st.is_pivot[row] == 0;
+		const unsigned long long bal = __ballot(live);
+		if (lane == 0)
+			s_wave[wave] = __popcll(bal);
+		__syncthreads();
+		int before = s_cnt;
+		for (int w = 0; w < wave; ++w)
+			before += s_wave[w];
+		const int slot = before + __popcll(bal & ((1ull << lane) - 1));
+		if (live && slot < G2_CHUNK)
+			rowid[slot] = row;
+		__syncthreads();
+		if (tid == 0)
+			s_cnt += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+		__syncthreads();
+		if (s_cnt >= G2_CHUNK) {
+			scanned_all = (s_cnt == G2_CHUNK && base + 256 >= m);
+			break;
+		}
+	}
+	const int nr = min(s_cnt, G2_CHUNK);
+	uint64_t v[G2_SRPT][G2P];
+	bool alive[G2_SRPT];
+#pragma unroll
+	for (int q = 0; q < G2_SRPT; ++q) {
+		const int r = tid + 256 * q;
+		alive[q] = r < nr;
+#pragma unroll
+		for (int w = 0; w < G2P; ++w)
+			v[q][w] = (r < nr && cw + w < nw) ? A[(size_t) rowid[r] * lda + cw + w] : 0ull;
+	}
+	const int cnt = gf2_reg_eliminate<G2_SRPT, G2P>(v, alive, nr, wb, false, pivot, s_min, outr, outc);
+	const bool done = cnt == wb || scanned_all;
+	if (done)
+		for (int t = tid; t < G2_PB; t += blockDim.x) {
+			st.J[t] = t < cnt ? outc[t] : 0;
+			st.prow[t] = t < cnt ? rowid[outr[t]] : 0;
+		}
+	if (tid == 0) {
+		if (done)
+			*st.k = cnt;
+		*st.need_tree = done ? 0 : 1;
+	}
+}
+
 __global__ __launch_bounds__(256) void gf2_tourney_kernel(const uint64_t *__restrict__ A, size_t lda, int m,
 							  int cw, int nw, int wb, Gf2State st, int nodes0)
 {
-	__shared__ uint64_t M[G2_CHUNK][G2P];
-	__shared__ int rowid[G2_CHUNK], alive[G2_CHUNK], flip[G2_CHUNK];
-	__shared__ int outr[G2_PB], outc[G2_PB];
-	__shared__ int s_min, s_go, s_nr, s_n0;
+	__shared__ int rowid[G2_CHUNK], outr[G2_PB], outc[G2_PB];
+	__shared__ uint64_t pivot[G2P];
+	__shared__ int s_min[2], s_go, s_nr, s_n0;
+	if (*st.need_tree == 0)
+		return;
 	const int tid = threadIdx.x;
 	int node = blockIdx.x, level = 0;
+	uint64_t v[G2_SRPT][G2P];
+	bool alive[G2_SRPT];
 	const int r0 = node * G2_CHUNK;
 	const int nr0 = min(G2_CHUNK, m - r0);
-	for (int idx = tid; idx < nr0 * G2P; idx += blockDim.x) {
-		const int r = idx / G2P, w = idx % G2P;
-		M[r][w] = cw + w < nw ? A[(size_t) (r0 + r) * lda + cw + w] : 0ull;
-	}
-	for (int r = tid; r < nr0; r += blockDim.x) {
+	for (int r = tid; r < G2_CHUNK; r += blockDim.x)
 		rowid[r] = r0 + r;
-		alive[r] = st.is_pivot[r0 + r] == 0;
+#pragma unroll
+	for (int q = 0; q < G2_SRPT; ++q) {
+		const int r = tid + 256 * q;
+		alive[q] = r < nr0 && st.is_pivot[r0 + r] == 0;
+#pragma unroll
+		for (int w = 0; w < G2P; ++w)
+			v[q][w] = (r < nr0 && cw + w < nw) ? A[(size_t) (r0 + r) * lda + cw + w] : 0ull;
 	}
 	__syncthreads();
-	int cnt = gf2_lds_echelon(M, rowid, alive, nr0, wb, outr, outc, &s_min, flip);
+	int cnt = gf2_reg_eliminate<G2_SRPT, G2P>(v, alive, nr0, wb, false, pivot, s_min, outr, outc);
+	for (int t = tid; t < cnt; t += blockDim.x)
+		outr[t] = rowid[outr[t]];
+	__syncthreads();
 	for (;;) {
 		const int here = g2_nodes_at(nodes0, level);
 		if (here == 1) {
@@ -246,15 +352,20 @@ __global__ __launch_bounds__(256) void gf2_tourney_kernel(const uint64_t *__rest
 			const int child = r < n0 ? c0 : c1;
 			const int pos = r < n0 ? r : r - n0;
 			rowid[r] = ((volatile int *) st.lists_rows)[((size_t) level * nodes0 + child) * G2_PB + pos];
-			alive[r] = 1;
 		}
 		__syncthreads();
-		for (int idx = tid; idx < nr * G2P; idx += blockDim.x) {
-			const int r = idx / G2P, w = idx % G2P;
-			M[r][w] = cw + w < nw ? A[(size_t) rowid[r] * lda + cw + w] : 0ull;
+#pragma unroll
+		for (int q = 0; q < G2_SRPT; ++q) {
+			const int r = tid + 256 * q;
+			alive[q] = r < nr;
+#pragma unroll
+			for (int w = 0; w < G2P; ++w)
+				v[q][w] = (r < nr && cw + w < nw) ? A[(size_t) rowid[r] * lda + cw + w] : 0ull;
 		}
+		cnt = gf2_reg_eliminate<G2_SRPT, G2P>(v, alive, nr, wb, false, pivot, s_min, outr, outc);
+		for (int t = tid; t < cnt; t += blockDim.x)
+			outr[t] = rowid[outr[t]];
 		__syncthreads();
-		cnt = gf2_lds_echelon(M, rowid, alive, nr, wb, outr, outc, &s_min, flip);
 		++level;
 		node = parent;
 	}
@@ -277,64 +388,41 @@ __global__ void gf2_gather_kernel(const uint64_t *__restrict__ A, size_t lda, in
 	}
 }
 
-// Inverse of P_J (k x k bits, identity-padded to G2_PB) by Gauss-Jordan; rows
-// of [P_J | I] are 2*G2P words.
+// Inverse of P_J (k x k bits, identity-padded to G2_PB) by Gauss-Jordan on
+// [P_J | I], one row of 2*G2P words per thread.
 __global__ __launch_bounds__(256) void gf2_inv_kernel(const uint64_t *__restrict__ P, int nwc, Gf2State st,
 						      uint64_t *__restrict__ Pinv)
 {
-	__shared__ uint64_t X[G2_PB][2 * G2P];
-	__shared__ int flip[G2_PB];
-	__shared__ int s_piv;
+	__shared__ int outr[G2_PB], outc[G2_PB], pos[G2_PB];
+	__shared__ uint64_t pivot[2 * G2P];
+	__shared__ int s_min[2];
 	const int tid = threadIdx.x;
 	const int k = *st.k;
-	for (int r = tid; r < G2_PB; r += blockDim.x) {
-		uint64_t row[2 * G2P];
+	uint64_t v[1][2 * G2P];
+	bool alive[1] = {true};
+	{
+		const int r = tid;
+#pragma unroll
 		for (int w = 0; w < 2 * G2P; ++w)
-			row[w] = 0;
+			v[0][w] = 0;
 		if (r < k) {
 			const uint64_t *src = P + (size_t) r * nwc;
 			for (int s = 0; s < k; ++s)
 				if (bit_of(src, st.J[s]))
-					row[s >> 6] |= 1ull << (s & 63);
+					v[0][s >> 6] |= 1ull << (s & 63);
 		} else {
-			row[r >> 6] |= 1ull << (r & 63);
+			v[0][r >> 6] |= 1ull << (r & 63);
 		}
-		row[G2P + (r >> 6)] |= 1ull << (r & 63);
-		for (int w = 0; w < 2 * G2P; ++w)
-			X[r][w] = row[w];
+		v[0][G2P + (r >> 6)] |= 1ull << (r & 63);
+		pos[r] = r;
 	}
+	const int cnt = gf2_reg_eliminate<1, 2 * G2P>(v, alive, G2_PB, k, true, pivot, s_min, outr, outc);
+	for (int s = tid; s < cnt; s += blockDim.x)
+		pos[outr[s]] = outc[s];
 	__syncthreads();
-	for (int s = 0; s < k; ++s) {
-		if (tid == 0) {
-			int pr = -1;
-			for (int r = s; r < k; ++r)
-				if ((X[r][s >> 6] >> (s & 63)) & 1ull) { pr = r; break; }
-			s_piv = pr;
-		}
-		__syncthreads();
-		const int pr = s_piv;
-		if (pr < 0)
-			break;
-		if (pr != s && tid < 2 * G2P) {
-			uint64_t t = X[s][tid];
-			X[s][tid] = X[pr][tid];
-			X[pr][tid] = t;
-		}
-		__syncthreads();
-		for (int r = tid; r < G2_PB; r += blockDim.x)
-			flip[r] = r != s && ((X[r][s >> 6] >> (s & 63)) & 1ull);
-		__syncthreads();
-		for (int idx = tid; idx < G2_PB * 2 * G2P; idx += blockDim.x) {
-			const int r = idx / (2 * G2P), w = idx % (2 * G2P);
-			if (flip[r])
-				X[r][w] ^= X[s][w];
-		}
-		__syncthreads();
-	}
-	for (int idx = tid; idx < G2_PB * G2P; idx += blockDim.x) {
-		const int r = idx / G2P, w = idx % G2P;
-		Pinv[idx] = X[r][G2P + w];
-	}
+#pragma unroll
+	for (int w = 0; w < G2P; ++w)
+		Pinv[(size_t) pos[tid] * G2P + w] = v[0][G2P + w];
 }
 
 // G[i] = bits J of row i (compressed to k bits), zero on this panel's pivots.
@@ -385,6 +473,7 @@ __global__ void gf2_init_kernel(Gf2State st, int m, int cnt_counters)
 	if (i == 0) {
 		*st.rank = 0;
 		*st.k = 0;
+		*st.need_tree = 0;
 	}
 }
 
@@ -427,7 +516,7 @@ int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size
 	MLZ_TRY(ws_get(ctx, WS_GF2_1, 2 * (size_t) G2_PB * nw * 8 + (size_t) G2_PB * G2P * 8, &w1));
 	MLZ_TRY(ws_get(ctx, WS_GF2_2, m * G2P * 8, &w2));
 	const size_t ints = m + 1 + 2 * rmax + 1 + 2 * G2_PB + (size_t) G2_MAXLVL * nodes0 * G2_PB +
-			    2 * (size_t) G2_MAXLVL * nodes0;
+			    2 * (size_t) G2_MAXLVL * nodes0 + 1;
 	MLZ_TRY(ws_get(ctx, WS_SMALL, ints * sizeof(int), &w3));
 	uint64_t *Wm = (uint64_t *) w0;
 	uint64_t *P = (uint64_t *) w1, *Wb = P + (size_t) G2_PB * nw, *Pinv = Wb + (size_t) G2_PB * nw;
@@ -443,6 +532,7 @@ int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size
 	st.prow = q; q += G2_PB;
 	st.lists_rows = q; q += (size_t) G2_MAXLVL * nodes0 * G2_PB;
 	st.lists_cnt = q; q += (size_t) G2_MAXLVL * nodes0;
+	st.need_tree = q; q += 1;
 	st.counters = q;
 	{
 		const int cc = G2_MAXLVL * nodes0;
@@ -455,7 +545,8 @@ int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size
 		const int wb = (int) std::min((size_t) G2_PB, n - cw * 64);
 		const int nwc = (int) (nw - cw);
 		{
-			ProfScope ps(ctx, "gf2_tourney");
+			ProfScope ps(ctx, "gf2_pivots");
+			gf2_fast_kernel<<<1, 256, 0, ctx->stream>>>(Wm, nw, (int) m, (int) cw, (int) nw, wb, st);
 			gf2_tourney_kernel<<<nodes0, 256, 0, ctx->stream>>>(Wm, nw, (int) m, (int) cw, (int) nw, wb, st, nodes0);
 		}
 		MLZ_TRY(check_launch(ctx, "gf2_tourney_kernel"));

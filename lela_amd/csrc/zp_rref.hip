@@ -8,43 +8,35 @@
 // result; this one is built for the GPU:
 //
 // Blocked Gauss-Jordan over column panels of RREF_W columns.  Per panel
-//  1. pivot search: a tournament over the rows that are not yet pivots, in ONE
-//     kernel -- each workgroup reduces a 256-row chunk of the panel to echelon
-//     form in LDS and keeps its pivot rows (at most RREF_W); workgroups then
-//     merge pairwise up a tree (the second to arrive at a node merges, via an
-//     atomic arrival counter).  The surviving rows span the panel's row space,
-//     so their pivot columns J are the panel's rank profile.
-//  2. W = P_J^-1 * P (P = the k pivot rows from column c on, P_J its columns J):
-//     one workgroup inverts P_J, a gemm forms W (identity on J).
+//  1. pivot search among the rows that are not pivots yet, restricted to the
+//     panel: one workgroup reduces the first 2*RREF_W such rows to echelon
+//     form; if that reaches full panel rank (or covers every remaining row) its
+//     pivot rows are the answer.  Otherwise a tournament in one kernel: each
+//     workgroup reduces a 256-row chunk and keeps its pivot rows, and
+//     workgroups merge pairwise up a tree (the second to arrive at a node
+//     merges, via an atomic arrival counter).  Either way the chosen rows span
+//     the panel's row space, so their pivot columns J are its rank profile.
+//  2. W = P_J^-1 * P (P = the k pivot rows from column c on, P_J its columns
+//     J): one workgroup inverts P_J, a gemm forms W (identity on J).
 //  3. every other row i: A_i <- A_i - A_i[J] * W, one gemm over all rows (the
 //     new pivot rows get a zero multiplier row), then W is written into the
 //     pivot rows.
+// Eliminations inside a workgroup keep the matrix in registers: thread
+// (g, c) owns column c of the rows of group g, so a column step needs three
+// barriers and each thread's updates are independent.
 // Rank, pivot rows and pivot columns stay on the device; the host only checks
 // the rank every few panels to stop early.  A final kernel writes the pivot
 // rows in pivot-column order followed by zero rows (standard layout).
 #include "common.hpp"
 
+#include <climits>
+
 namespace mlz {
 
 constexpr int RREF_W = 64;        // panel width (columns)
-constexpr int RREF_CHUNK = 256;   // rows per tournament leaf
+constexpr int RREF_CHUNK = 128;   // rows per tournament leaf (= two merged lists)
+constexpr int RREF_FAST = 128;    // candidate rows of the fast path
 constexpr int RREF_MAXLVL = 24;
-
-__device__ inline uint32_t inv_mod_dev(uint32_t y, uint32_t p)
-{
-	int64_t r0 = y % p, r1 = p, s0 = 1, s1 = 0;
-	while (r1 != 0) {
-		int64_t q = r0 / r1, t;
-		t = r0 - q * r1; r0 = r1; r1 = t;
-		t = s0 - q * s1; s0 = s1; s1 = t;
-	}
-	if (r0 != 1)
-		return 0;
-	s0 %= (int64_t) p;
-	if (s0 < 0)
-		s0 += p;
-	return (uint32_t) s0;
-}
 
 // Device-side state of one RREF call.
 struct RrefState {
@@ -56,58 +48,94 @@ struct RrefState {
 	int *J;              // [RREF_W] panel-relative pivot columns
 	int *prow;           // [RREF_W] their rows
 	int *bad;            // [1] set when a pivot is not invertible (composite p)
+	int *need_tree;      // [1] the fast path did not settle the panel
 	int *lists_rows;     // [levels][nodes0][RREF_W]
-	int *lists_cols;
 	int *lists_cnt;      // [levels][nodes0]
 	int *counters;       // [levels][nodes0]
 };
 
-// Echelon form of nr <= 256 rows x wc <= RREF_W columns held in LDS; records the
-// pivot rows (their ids) and pivot columns in order.  All threads return the
-// same count.
-__device__ int lds_echelon(uint32_t (*M)[RREF_W + 1], const int *rowid, int *alive, int nr, int wc,
-			   const ModP &mp, int *out_rows, int *out_cols, int *s_min, uint32_t *s_inv,
-			   uint32_t *f, int *bad)
+// Elimination with the matrix in registers.  The block has NG = 256/NC row
+// groups of RPG rows; thread (g = tid/NC, c = tid%NC) holds v[i] = entry
+// (row g*RPG + i, column c).  Columns 0..nsearch-1 are processed in order; the
+// pivot of a column is the live row of lowest index with a nonzero there.
+// reduced = false: live rows are cleared below/above (echelon form, values not
+// normalised).  reduced = true: Gauss-Jordan, every other row is cleared and
+// the pivot row scaled to 1.  Returns the pivot count; out_rows/out_cols list
+// the pivots in order.  alive[] must be set for rows < nrows on entry.
+template <int NC, int RPG>
+__device__ int reg_eliminate(uint32_t (&v)[RPG], int nrows, int nsearch, bool reduced, const ModP &mp,
+			     int *alive, uint32_t *f, uint32_t *pivrow, int *s_min, uint32_t *s_inv,
+			     int *out_rows, int *out_cols, int *bad)
 {
-	const int tid = threadIdx.x;
+	const int tid = threadIdx.x, c = tid % NC, g = tid / NC, rbase = g * RPG;
+	if (tid == 0)
+		*s_min = INT_MAX;
+	__syncthreads();
 	int cnt = 0;
-	for (int col = 0; col < wc; ++col) {
-		if (tid == 0)
-			*s_min = 0x7fffffff;
-		__syncthreads();
-		for (int r = tid; r < nr; r += blockDim.x)
-			if (alive[r] && M[r][col] != 0)
-				atomicMin(s_min, r);
+	for (int col = 0; col < nsearch; ++col) {
+		if (c == col) {
+			int found = INT_MAX;
+#pragma unroll
+			for (int i = RPG - 1; i >= 0; --i) {
+				const int r = rbase + i;
+				if (r < nrows && v[i] != 0 && alive[r])
+					found = r;
+			}
+			if (found != INT_MAX)
+				atomicMin(s_min, found);
+		}
 		__syncthreads();
 		const int pr = *s_min;
-		if (pr == 0x7fffffff) {
+		if (pr == INT_MAX) {
 			__syncthreads();
 			continue;
 		}
+		if (pr / RPG == g) {
+			uint32_t val = 0;
+#pragma unroll
+			for (int i = 0; i < RPG; ++i)
+				if (rbase + i == pr)
+					val = v[i];
+			pivrow[c] = val;
+		}
+		__syncthreads();
+		if (c == col) {
+			const uint32_t xinv = mp.inv(pivrow[col]);
+			if (g == pr / RPG) {
+				*s_inv = xinv;
+				if (xinv == 0)
+					*bad = 1;
+			}
+#pragma unroll
+			for (int i = 0; i < RPG; ++i) {
+				const int r = rbase + i;
+				uint32_t fr = 0;
+				if (r != pr && r < nrows && v[i] != 0 && (reduced || alive[r]))
+					fr = mp.mul(v[i], xinv);
+				f[r] = fr;
+			}
+		}
 		if (tid == 0) {
-			uint32_t x = inv_mod_dev(M[pr][col], (uint32_t) mp.p);
-			if (x == 0)
-				*bad = 1;
-			*s_inv = x;
 			alive[pr] = 0;
-			out_rows[cnt] = rowid[pr];
+			out_rows[cnt] = pr;
 			out_cols[cnt] = col;
+			*s_min = INT_MAX;
 		}
 		__syncthreads();
+		const uint32_t pc = pivrow[c];
 		const uint32_t xi = *s_inv;
-		for (int r = tid; r < nr; r += blockDim.x)
-			f[r] = (alive[r] && M[r][col]) ? mp.mul(M[r][col], xi) : 0u;
-		__syncthreads();
-		const int cc = col + 1 + (tid % RREF_W);
-		if (cc < wc) {
-			const uint32_t pv = M[pr][cc];
-			for (int r = tid / RREF_W; r < nr; r += blockDim.x / RREF_W)
-				if (f[r])
-					M[r][cc] = mp.sub(M[r][cc], mp.mul(f[r], pv));
+#pragma unroll
+		for (int i = 0; i < RPG; ++i) {
+			const int r = rbase + i;
+			const uint32_t fr = f[r];
+			if (fr)
+				v[i] = mp.sub(v[i], mp.mul(fr, pc));
+			if (reduced && r == pr)
+				v[i] = mp.mul(v[i], xi);
 		}
-		__syncthreads();
 		++cnt;
 	}
+	__syncthreads();
 	return cnt;
 }
 
@@ -119,36 +147,104 @@ __device__ inline int nodes_at(int nodes0, int level)
 	return n;
 }
 
+// Fast path: the first RREF_FAST rows that are not pivots yet (in row order).
+__global__ __launch_bounds__(256) void rref_fast_kernel(const uint32_t *__restrict__ A, size_t lda, int m, int c,
+							int wc, ModP mp, RrefState st)
+{
+	constexpr int RPG = RREF_FAST / 4;
+	__shared__ int rowid[RREF_FAST], alive[RREF_FAST], outr[RREF_W], outc[RREF_W];
+	__shared__ uint32_t f[RREF_FAST], pivrow[RREF_W];
+	__shared__ int s_min, s_cnt, s_wave[4];
+	__shared__ uint32_t s_inv;
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	if (tid == 0)
+		s_cnt = 0;
+	__syncthreads();
+	int scanned_all = 1;
+	for (int base = 0; base < m; base += 256) {
+		const int row = base + tid;
+		const bool live = row < m && st.is_pivot[row] == 0;
+		const unsigned long long bal = __ballot(live);
+		if (lane == 0)
+			s_wave[wave] = __popcll(bal);
+		__syncthreads();
+		int before = s_cnt;
+		for (int w = 0; w < wave; ++w)
+			before += s_wave[w];
+		const int slot = before + __popcll(bal & ((1ull << lane) - 1));
+		if (live && slot < RREF_FAST)
+			rowid[slot] = row;
+		__syncthreads();
+		if (tid == 0)
+			s_cnt += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+		__syncthreads();
+		if (s_cnt >= RREF_FAST) {
+			// rows beyond the collected ones remain unseen unless this was the end
+			scanned_all = (s_cnt == RREF_FAST && base + 256 >= m);
+			break;
+		}
+	}
+	const int nr = min(s_cnt, RREF_FAST);
+	for (int r = tid; r < RREF_FAST; r += blockDim.x)
+		alive[r] = r < nr;
+	const int cc = tid % RREF_W, g = tid / RREF_W;
+	uint32_t v[RPG];
+#pragma unroll
+	for (int i = 0; i < RPG; ++i) {
+		const int r = g * RPG + i;
+		v[i] = (r < nr && cc < wc) ? A[(size_t) rowid[r] * lda + c + cc] : 0u;
+	}
+	const int cnt = reg_eliminate<RREF_W, RPG>(v, nr, wc, false, mp, alive, f, pivrow, &s_min, &s_inv, outr, outc,
+						   st.bad);
+	const bool done = cnt == wc || scanned_all;
+	for (int t = tid; t < RREF_W; t += blockDim.x) {
+		if (done) {
+			st.J[t] = t < cnt ? outc[t] : 0;
+			st.prow[t] = t < cnt ? rowid[outr[t]] : 0;
+		}
+	}
+	if (tid == 0) {
+		if (done)
+			*st.k = cnt;
+		*st.need_tree = done ? 0 : 1;
+	}
+}
+
 __global__ __launch_bounds__(256) void rref_tourney_kernel(const uint32_t *__restrict__ A, size_t lda, int m,
 							   int c, int wc, ModP mp, RrefState st, int nodes0)
 {
-	__shared__ uint32_t M[RREF_CHUNK][RREF_W + 1];
+	constexpr int RPG = RREF_CHUNK / 4;
 	__shared__ int rowid[RREF_CHUNK], alive[RREF_CHUNK];
-	__shared__ uint32_t f[RREF_CHUNK];
+	__shared__ uint32_t f[RREF_CHUNK], pivrow[RREF_W];
 	__shared__ int outr[RREF_W], outc[RREF_W];
-	__shared__ int s_min, s_go, s_nr;
+	__shared__ int s_min, s_go, s_nr, s_n0;
 	__shared__ uint32_t s_inv;
-	const int tid = threadIdx.x;
+	if (*st.need_tree == 0)
+		return;
+	const int tid = threadIdx.x, cc = tid % RREF_W, g = tid / RREF_W;
 	int node = blockIdx.x, level = 0;
+	uint32_t v[RPG];
 
 	// Leaf: rows [node*CHUNK, ...) that are not pivots yet.
 	const int r0 = node * RREF_CHUNK;
 	const int nr0 = min(RREF_CHUNK, m - r0);
-	for (int idx = tid; idx < nr0 * RREF_W; idx += blockDim.x) {
-		int r = idx / RREF_W, cc = idx % RREF_W;
-		M[r][cc] = cc < wc ? A[(size_t) (r0 + r) * lda + c + cc] : 0u;
-	}
-	for (int r = tid; r < nr0; r += blockDim.x) {
+	for (int r = tid; r < RREF_CHUNK; r += blockDim.x) {
 		rowid[r] = r0 + r;
-		alive[r] = st.is_pivot[r0 + r] == 0;
+		alive[r] = r < nr0 && st.is_pivot[r0 + r] == 0;
+	}
+#pragma unroll
+	for (int i = 0; i < RPG; ++i) {
+		const int r = g * RPG + i;
+		v[i] = (r < nr0 && cc < wc) ? A[(size_t) (r0 + r) * lda + c + cc] : 0u;
 	}
 	__syncthreads();
-	int cnt = lds_echelon(M, rowid, alive, nr0, wc, mp, outr, outc, &s_min, &s_inv, f, st.bad);
+	int cnt = reg_eliminate<RREF_W, RPG>(v, nr0, wc, false, mp, alive, f, pivrow, &s_min, &s_inv, outr, outc, st.bad);
+	for (int t = tid; t < cnt; t += blockDim.x)
+		outr[t] = rowid[outr[t]];
+	__syncthreads();
 
 	for (;;) {
 		const int here = nodes_at(nodes0, level);
-		int *lr = st.lists_rows + ((size_t) level * nodes0 + node) * RREF_W;
-		int *lc = st.lists_cols + ((size_t) level * nodes0 + node) * RREF_W;
 		if (here == 1) {
 			// Root: publish the panel's pivots and reset the arrival counters.
 			for (int t = tid; t < RREF_W; t += blockDim.x) {
@@ -161,10 +257,9 @@ __global__ __launch_bounds__(256) void rref_tourney_kernel(const uint32_t *__res
 				st.counters[t] = 0;
 			return;
 		}
-		for (int t = tid; t < cnt; t += blockDim.x) {
+		int *lr = st.lists_rows + ((size_t) level * nodes0 + node) * RREF_W;
+		for (int t = tid; t < cnt; t += blockDim.x)
 			lr[t] = outr[t];
-			lc[t] = outc[t];
-		}
 		if (tid == 0)
 			st.lists_cnt[(size_t) level * nodes0 + node] = cnt;
 		__threadfence();
@@ -172,10 +267,8 @@ __global__ __launch_bounds__(256) void rref_tourney_kernel(const uint32_t *__res
 		const int parent = node >> 1, sibling = node ^ 1;
 		if (tid == 0) {
 			int go = 1;
-			if (sibling < here) {
-				int old = atomicAdd(&st.counters[(size_t) level * nodes0 + parent], 1);
-				go = old == 1;
-			}
+			if (sibling < here)
+				go = atomicAdd(&st.counters[(size_t) level * nodes0 + parent], 1) == 1;
 			s_go = go;
 		}
 		__syncthreads();
@@ -185,27 +278,31 @@ __global__ __launch_bounds__(256) void rref_tourney_kernel(const uint32_t *__res
 		// Merge both children's lists (re-reading the rows' panel values).
 		const int c0 = parent * 2, c1 = parent * 2 + 1;
 		if (tid == 0) {
-			int n0 = ((volatile int *) st.lists_cnt)[(size_t) level * nodes0 + c0];
-			int n1 = c1 < here ? ((volatile int *) st.lists_cnt)[(size_t) level * nodes0 + c1] : 0;
+			const int n0 = ((volatile int *) st.lists_cnt)[(size_t) level * nodes0 + c0];
+			const int n1 = c1 < here ? ((volatile int *) st.lists_cnt)[(size_t) level * nodes0 + c1] : 0;
 			s_nr = n0 + n1;
-			s_min = n0;
+			s_n0 = n0;
 		}
 		__syncthreads();
-		const int nr = s_nr, n0 = s_min;
-		__syncthreads();
-		for (int r = tid; r < nr; r += blockDim.x) {
-			const int child = r < n0 ? c0 : c1;
-			const int pos = r < n0 ? r : r - n0;
-			rowid[r] = ((volatile int *) st.lists_rows)[((size_t) level * nodes0 + child) * RREF_W + pos];
-			alive[r] = 1;
+		const int nr = s_nr, n0 = s_n0;
+		for (int r = tid; r < RREF_CHUNK; r += blockDim.x) {
+			if (r < nr) {
+				const int child = r < n0 ? c0 : c1;
+				const int pos = r < n0 ? r : r - n0;
+				rowid[r] = ((volatile int *) st.lists_rows)[((size_t) level * nodes0 + child) * RREF_W + pos];
+			}
+			alive[r] = r < nr;
 		}
 		__syncthreads();
-		for (int idx = tid; idx < nr * RREF_W; idx += blockDim.x) {
-			int r = idx / RREF_W, cc = idx % RREF_W;
-			M[r][cc] = cc < wc ? A[(size_t) rowid[r] * lda + c + cc] : 0u;
+#pragma unroll
+		for (int i = 0; i < RPG; ++i) {
+			const int r = g * RPG + i;
+			v[i] = (r < nr && cc < wc) ? A[(size_t) rowid[r] * lda + c + cc] : 0u;
 		}
+		cnt = reg_eliminate<RREF_W, RPG>(v, nr, wc, false, mp, alive, f, pivrow, &s_min, &s_inv, outr, outc, st.bad);
+		for (int t = tid; t < cnt; t += blockDim.x)
+			outr[t] = rowid[outr[t]];
 		__syncthreads();
-		cnt = lds_echelon(M, rowid, alive, nr, wc, mp, outr, outc, &s_min, &s_inv, f, st.bad);
 		++level;
 		node = parent;
 	}
@@ -229,64 +326,46 @@ __global__ void rref_gather_kernel(const uint32_t *__restrict__ A, size_t lda, i
 	}
 }
 
-// Inverse of P_J (k x k, padded with the identity to RREF_W) by Gauss-Jordan.
+// Inverse of P_J (k x k, padded with the identity to RREF_W) by Gauss-Jordan on
+// [P_J | I] held in registers (128 columns x 2 groups of 32 rows).
 __global__ __launch_bounds__(256) void rref_inv_kernel(const uint32_t *__restrict__ P, int nc, ModP mp,
 						       RrefState st, uint32_t *__restrict__ Pinv)
 {
-	constexpr int W = RREF_W;
-	__shared__ uint32_t X[W][2 * W + 1];
-	__shared__ uint32_t f[W];
-	__shared__ int s_piv;
+	constexpr int W = RREF_W, NC = 2 * RREF_W, RPG = RREF_W / 2;
+	__shared__ int alive[W], outr[W], outc[W], pos[W];
+	__shared__ uint32_t f[W], pivrow[NC];
+	__shared__ int s_min;
 	__shared__ uint32_t s_inv;
-	const int tid = threadIdx.x;
+	const int tid = threadIdx.x, c = tid % NC, g = tid / NC;
 	const int k = *st.k;
-	for (int idx = tid; idx < W * 2 * W; idx += blockDim.x) {
-		int r = idx / (2 * W), cc = idx % (2 * W);
-		uint32_t v;
-		if (cc < W)
-			v = (r < k && cc < k) ? P[(size_t) r * nc + st.J[cc]] : (r == cc ? 1u : 0u);
+	uint32_t v[RPG];
+#pragma unroll
+	for (int i = 0; i < RPG; ++i) {
+		const int r = g * RPG + i;
+		uint32_t x;
+		if (c < W)
+			x = (r < k && c < k) ? P[(size_t) r * nc + st.J[c]] : (r == c ? 1u : 0u);
 		else
-			v = (cc - W == r) ? 1u : 0u;
-		X[r][cc] = v;
+			x = (c - W == r) ? 1u : 0u;
+		v[i] = x;
+	}
+	for (int r = tid; r < W; r += blockDim.x) {
+		alive[r] = 1;
+		pos[r] = r;
 	}
 	__syncthreads();
-	for (int s = 0; s < k; ++s) {
-		if (tid == 0) {
-			int pr = -1;
-			for (int r = s; r < k; ++r)
-				if (X[r][s] != 0) { pr = r; break; }
-			s_piv = pr;
-			s_inv = pr >= 0 ? inv_mod_dev(X[pr][s], (uint32_t) mp.p) : 0u;
-			if (pr < 0 || s_inv == 0)
-				*st.bad = 1;
+	const int cnt = reg_eliminate<NC, RPG>(v, W, k, true, mp, alive, f, pivrow, &s_min, &s_inv, outr, outc, st.bad);
+	if (tid == 0 && cnt < k)
+		*st.bad = 1;
+	for (int s = tid; s < cnt; s += blockDim.x)
+		pos[outr[s]] = outc[s];
+	__syncthreads();
+	if (c >= W) {
+#pragma unroll
+		for (int i = 0; i < RPG; ++i) {
+			const int r = g * RPG + i;
+			Pinv[(size_t) pos[r] * W + (c - W)] = v[i];
 		}
-		__syncthreads();
-		const int pr = s_piv;
-		if (pr < 0)
-			break;
-		if (pr != s)
-			for (int cc = tid; cc < 2 * W; cc += blockDim.x) {
-				uint32_t t = X[s][cc];
-				X[s][cc] = X[pr][cc];
-				X[pr][cc] = t;
-			}
-		__syncthreads();
-		for (int cc = tid; cc < 2 * W; cc += blockDim.x)
-			X[s][cc] = mp.mul(X[s][cc], s_inv);
-		__syncthreads();
-		for (int r = tid; r < W; r += blockDim.x)
-			f[r] = r == s ? 0u : X[r][s];
-		__syncthreads();
-		for (int idx = tid; idx < W * 2 * W; idx += blockDim.x) {
-			int r = idx / (2 * W), cc = idx % (2 * W);
-			if (f[r])
-				X[r][cc] = mp.sub(X[r][cc], mp.mul(f[r], X[s][cc]));
-		}
-		__syncthreads();
-	}
-	for (int idx = tid; idx < W * W; idx += blockDim.x) {
-		int r = idx / W, cc = idx % W;
-		Pinv[idx] = X[r][W + cc];
 	}
 }
 
@@ -305,7 +384,7 @@ __global__ void rref_gatherG_kernel(const uint32_t *__restrict__ A, size_t lda, 
 	G[(size_t) i * RREF_W + s] = v;
 }
 
-// Writes W into the pivot rows; the last block advances the rank.
+// Writes W into the pivot rows.
 __global__ void rref_scatter_kernel(uint32_t *__restrict__ A, size_t lda, int nc, int c, RrefState st,
 				    const uint32_t *__restrict__ W)
 {
@@ -356,6 +435,7 @@ __global__ void rref_init_kernel(RrefState st, int m, int cnt_counters)
 		*st.rank = 0;
 		*st.k = 0;
 		*st.bad = 0;
+		*st.need_tree = 0;
 	}
 }
 
@@ -372,14 +452,16 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 	const int nodes0 = (int) ceil_div(m, RREF_CHUNK);
 	const size_t mn = m * n;
 	const size_t rmax = std::min(m, n);
+	if (nodes0 > (1 << (RREF_MAXLVL - 2)))
+		return set_error(ctx, MLZ_E_ARG, "too many rows for the pivot tournament");
 	// workspace
 	void *w0, *w1, *w2, *w3;
 	MLZ_TRY(ws_get(ctx, WS_RREF_0, mn * sizeof(uint32_t), &w0));                 // working matrix
 	MLZ_TRY(ws_get(ctx, WS_RREF_1, 2 * (size_t) RREF_W * n * sizeof(uint32_t) +
 				      (size_t) RREF_W * RREF_W * sizeof(uint32_t), &w1));  // P, W, Pinv
 	MLZ_TRY(ws_get(ctx, WS_RREF_2, m * RREF_W * sizeof(uint32_t), &w2));         // G
-	const size_t ints = m + 1 + 2 * rmax + 1 + 2 * RREF_W + 1 +
-			    3 * (size_t) RREF_MAXLVL * nodes0 * RREF_W + 2 * (size_t) RREF_MAXLVL * nodes0;
+	const size_t ints = m + 1 + 2 * rmax + 1 + 2 * RREF_W + 2 + (size_t) RREF_MAXLVL * nodes0 * RREF_W +
+			    2 * (size_t) RREF_MAXLVL * nodes0;
 	MLZ_TRY(ws_get(ctx, WS_RREF_3, ints * sizeof(int), &w3));
 	uint32_t *Wm = (uint32_t *) w0;
 	uint32_t *P = (uint32_t *) w1, *Wb = P + (size_t) RREF_W * n, *Pinv = Wb + (size_t) RREF_W * n;
@@ -394,13 +476,10 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 	st.J = q; q += RREF_W;
 	st.prow = q; q += RREF_W;
 	st.bad = q; q += 1;
+	st.need_tree = q; q += 1;
 	st.lists_rows = q; q += (size_t) RREF_MAXLVL * nodes0 * RREF_W;
-	st.lists_cols = q; q += (size_t) RREF_MAXLVL * nodes0 * RREF_W;
-	q += (size_t) RREF_MAXLVL * nodes0 * RREF_W;
 	st.lists_cnt = q; q += (size_t) RREF_MAXLVL * nodes0;
 	st.counters = q;
-	if (nodes0 > (1 << (RREF_MAXLVL - 2)))
-		return set_error(ctx, MLZ_E_ARG, "too many rows for the pivot tournament");
 
 	{
 		const int cc = RREF_MAXLVL * nodes0;
@@ -420,12 +499,16 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 		const int wc = (int) std::min((size_t) RREF_W, n - c);
 		const int nc = (int) (n - c);
 		{
-			ProfScope ps(ctx, "rref_tourney");
+			ProfScope ps(ctx, "rref_pivots");
+			rref_fast_kernel<<<1, 256, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st);
 			rref_tourney_kernel<<<nodes0, 256, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, nodes0);
 		}
 		MLZ_TRY(check_launch(ctx, "rref_tourney_kernel"));
 		rref_gather_kernel<<<dim3(ceil_div(nc, 256), RREF_W), 256, 0, ctx->stream>>>(Wm, n, nc, (int) c, panel, st, P);
-		rref_inv_kernel<<<1, 256, 0, ctx->stream>>>(P, nc, mp, st, Pinv);
+		{
+			ProfScope ps(ctx, "rref_inv");
+			rref_inv_kernel<<<1, 256, 0, ctx->stream>>>(P, nc, mp, st, Pinv);
+		}
 		MLZ_TRY(check_launch(ctx, "rref_inv_kernel"));
 		MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, RREF_W, nc, RREF_W, 1, Pinv, RREF_W, P, nc, 0, Wb, nc, algo));
 		rref_gatherG_kernel<<<ceil_div(m, 256 / RREF_W), 256, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, panel, st, G);
