@@ -46,17 +46,27 @@ struct ModP {
 	// (x*y) mod p for x, y < p < 2^32.  For p < 2^26 the product is exact in a
 	// double; q = floor(x*y/p) is then off by at most one and the remainder
 	// x*y - q*p is exact (fma), so one correction each way suffices.
+	__host__ __device__ inline uint32_t mul_small(uint32_t x, uint32_t y) const
+	{
+		const double prod = (double) x * (double) y;
+		const double q = floor(prod * pinv);
+		double r = fma(-q, (double) p, prod);
+		r = r < 0.0 ? r + (double) p : r;
+		r = r >= (double) p ? r - (double) p : r;
+		return (uint32_t) r;
+	}
 	__host__ __device__ inline uint32_t mul(uint32_t x, uint32_t y) const
 	{
-		if (small) {
-			const double prod = (double) x * (double) y;
-			const double q = floor(prod * pinv);
-			double r = fma(-q, (double) p, prod);
-			r = r < 0.0 ? r + (double) p : r;
-			r = r >= (double) p ? r - (double) p : r;
-			return (uint32_t) r;
-		}
+		if (small)
+			return mul_small(x, y);
 		return (uint32_t) reduce((uint64_t) x * y);
+	}
+	// Multiplication with the path fixed at compile time (kernels templated
+	// on SMALL = (p < 2^26) avoid evaluating both paths).
+	template <bool SMALL>
+	__host__ __device__ inline uint32_t mulT(uint32_t x, uint32_t y) const
+	{
+		return SMALL ? mul_small(x, y) : (uint32_t) reduce((uint64_t) x * y);
 	}
 	// x^e mod p
 	__host__ __device__ inline uint32_t pow(uint32_t x, uint64_t e) const

@@ -12,7 +12,7 @@
 //     panel: one workgroup reduces the first 2*RREF_W such rows to echelon
 //     form; if that reaches full panel rank (or covers every remaining row) its
 //     pivot rows are the answer.  Otherwise a tournament in one kernel: each
-//     workgroup reduces a 256-row chunk and keeps its pivot rows, and
+//     workgroup reduces a 128-row chunk and keeps its pivot rows, and
 //     workgroups merge pairwise up a tree (the second to arrive at a node
 //     merges, via an atomic arrival counter).  Either way the chosen rows span
 //     the panel's row space, so their pivot columns J are its rank profile.
@@ -22,8 +22,11 @@
 //     new pivot rows get a zero multiplier row), then W is written into the
 //     pivot rows.
 // Eliminations inside a workgroup keep the matrix in registers: thread
-// (g, c) owns column c of the rows of group g, so a column step needs three
-// barriers and each thread's updates are independent.
+// (g, c) owns column c of the rows of group g, so a column step needs two
+// barriers and each thread's updates are independent.  Steps are fraction
+// free (no inverse on the critical path); the pivots of P_J are checked for
+// invertibility when it is inverted (LELA's check,
+// lela/algorithms/elimination.tcc:217-220).
 // Rank, pivot rows and pivot columns stay on the device; the host only checks
 // the rank every few panels to stop early.  A final kernel writes the pivot
 // rows in pivot-column order followed by zero rows (standard layout).
@@ -54,22 +57,29 @@ struct RrefState {
 	int *counters;       // [levels][nodes0]
 };
 
-// Elimination with the matrix in registers.  The block has NG = 256/NC row
-// groups of RPG rows; thread (g = tid/NC, c = tid%NC) holds v[i] = entry
+// Elimination with the matrix in registers.  The block has NG = blockDim/NC
+// row groups of RPG rows; thread (g = tid/NC, c = tid%NC) holds v[i] = entry
 // (row g*RPG + i, column c).  Columns 0..nsearch-1 are processed in order; the
 // pivot of a column is the live row of lowest index with a nonzero there.
 // reduced = false: live rows are cleared below/above (echelon form, values not
 // normalised).  reduced = true: Gauss-Jordan, every other row is cleared and
 // the pivot row scaled to 1.  Returns the pivot count; out_rows/out_cols list
 // the pivots in order.  alive[] must be set for rows < nrows on entry.
-template <int NC, int RPG>
+template <int NC, int RPG, bool SMALL>
 __device__ int reg_eliminate(uint32_t (&v)[RPG], int nrows, int nsearch, bool reduced, const ModP &mp,
-			     int *alive, uint32_t *f, uint32_t *pivrow, int *s_min, uint32_t *s_inv,
-			     int *out_rows, int *out_cols, int *bad)
+			     int *alive, uint32_t *f, uint32_t *pivrow, int *s_min, int *out_rows, int *out_cols,
+			     int *bad)
 {
+	// Fraction-free steps: a row r that has x_r != 0 in the pivot column
+	// becomes x * row_r - x_r * pivot_row, x the pivot value (x is invertible
+	// for prime p, so row spaces and ranks are unchanged).  No inverse sits
+	// on the critical path; reduced mode normalises the pivot rows once at
+	// the end, all inverses in parallel.  s_min is two words used in turn.
 	const int tid = threadIdx.x, c = tid % NC, g = tid / NC, rbase = g * RPG;
-	if (tid == 0)
-		*s_min = INT_MAX;
+	if (tid == 0) {
+		s_min[0] = INT_MAX;
+		s_min[1] = INT_MAX;
+	}
 	__syncthreads();
 	int cnt = 0;
 	for (int col = 0; col < nsearch; ++col) {
@@ -82,60 +92,77 @@ __device__ int reg_eliminate(uint32_t (&v)[RPG], int nrows, int nsearch, bool re
 					found = r;
 			}
 			if (found != INT_MAX)
-				atomicMin(s_min, found);
+				atomicMin(&s_min[col & 1], found);
 		}
 		__syncthreads();
-		const int pr = *s_min;
-		if (pr == INT_MAX) {
-			__syncthreads();
+		const int pr = s_min[col & 1];
+		if (tid == 0)
+			s_min[(col + 1) & 1] = INT_MAX;
+		if (pr != INT_MAX) {
+			if (pr / RPG == g) {
+				uint32_t val = 0;
+#pragma unroll
+				for (int i = 0; i < RPG; ++i)
+					if (rbase + i == pr)
+						val = v[i];
+				pivrow[c] = val;
+			}
+			if (c == col) {
+#pragma unroll
+				for (int i = 0; i < RPG; ++i) {
+					const int r = rbase + i;
+					f[r] = (r != pr && r < nrows && (reduced || alive[r])) ? v[i] : 0u;
+				}
+			}
+			if (tid == 0) {
+				alive[pr] = 0;
+				out_rows[cnt] = pr;
+				out_cols[cnt] = col;
+			}
+		}
+		__syncthreads();
+		if (pr == INT_MAX)
 			continue;
-		}
-		if (pr / RPG == g) {
-			uint32_t val = 0;
-#pragma unroll
-			for (int i = 0; i < RPG; ++i)
-				if (rbase + i == pr)
-					val = v[i];
-			pivrow[c] = val;
-		}
-		__syncthreads();
-		if (c == col) {
-			const uint32_t xinv = mp.inv(pivrow[col]);
-			if (g == pr / RPG) {
-				*s_inv = xinv;
-				if (xinv == 0)
-					*bad = 1;
-			}
-#pragma unroll
-			for (int i = 0; i < RPG; ++i) {
-				const int r = rbase + i;
-				uint32_t fr = 0;
-				if (r != pr && r < nrows && v[i] != 0 && (reduced || alive[r]))
-					fr = mp.mul(v[i], xinv);
-				f[r] = fr;
-			}
-		}
-		if (tid == 0) {
-			alive[pr] = 0;
-			out_rows[cnt] = pr;
-			out_cols[cnt] = col;
-			*s_min = INT_MAX;
-		}
-		__syncthreads();
-		const uint32_t pc = pivrow[c];
-		const uint32_t xi = *s_inv;
+		const uint32_t pc = pivrow[c], x = pivrow[col];
 #pragma unroll
 		for (int i = 0; i < RPG; ++i) {
-			const int r = rbase + i;
-			const uint32_t fr = f[r];
+			const uint32_t fr = f[rbase + i];
 			if (fr)
-				v[i] = mp.sub(v[i], mp.mul(fr, pc));
-			if (reduced && r == pr)
-				v[i] = mp.mul(v[i], xi);
+				v[i] = mp.sub(mp.mulT<SMALL>(x, v[i]), mp.mulT<SMALL>(fr, pc));
 		}
 		++cnt;
 	}
 	__syncthreads();
+	if (reduced) {
+		// pivrow -> column-to-pivot map, f -> inverse of each pivot row's lead
+		int *colpos = (int *) pivrow;
+		for (int t = tid; t < NC; t += blockDim.x)
+			colpos[t] = -1;
+		__syncthreads();
+		for (int s = tid; s < cnt; s += blockDim.x)
+			colpos[out_cols[s]] = s;
+		__syncthreads();
+		const int s = colpos[c];
+		if (s >= 0 && out_rows[s] / RPG == g) {
+			uint32_t d = 0;
+#pragma unroll
+			for (int i = 0; i < RPG; ++i)
+				if (rbase + i == out_rows[s])
+					d = v[i];
+			const uint32_t di = mp.inv(d);
+			if (di == 0)
+				*bad = 1;
+			f[out_rows[s]] = di;
+		}
+		__syncthreads();
+#pragma unroll
+		for (int i = 0; i < RPG; ++i) {
+			const int r = rbase + i;
+			if (r < nrows && !alive[r])
+				v[i] = mp.mulT<SMALL>(v[i], f[r]);
+		}
+		__syncthreads();
+	}
 	return cnt;
 }
 
@@ -148,20 +175,21 @@ __device__ inline int nodes_at(int nodes0, int level)
 }
 
 // Fast path: the first RREF_FAST rows that are not pivots yet (in row order).
-__global__ __launch_bounds__(256) void rref_fast_kernel(const uint32_t *__restrict__ A, size_t lda, int m, int c,
+template <bool SMALL>
+__global__ __launch_bounds__(1024) void rref_fast_kernel(const uint32_t *__restrict__ A, size_t lda, int m, int c,
 							int wc, ModP mp, RrefState st)
 {
-	constexpr int RPG = RREF_FAST / 4;
+	constexpr int RPG = RREF_FAST / (1024 / RREF_W);
 	__shared__ int rowid[RREF_FAST], alive[RREF_FAST], outr[RREF_W], outc[RREF_W];
 	__shared__ uint32_t f[RREF_FAST], pivrow[RREF_W];
-	__shared__ int s_min, s_cnt, s_wave[4];
-	__shared__ uint32_t s_inv;
+	__shared__ int s_min[2], s_cnt, s_wave[16];
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	if (tid == 0)
 		s_cnt = 0;
 	__syncthreads();
 	int scanned_all = 1;
-	for (int base = 0; base < m; base += 256) {
+	const int nwaves = blockDim.x >> 6;
+	for (int base = 0; base < m; base += blockDim.x) {
 		const int row = base + tid;
 		const bool live = row < m && st.is_pivot[row] == 0;
 		const unsigned long long bal = __ballot(live);
@@ -176,11 +204,12 @@ __global__ __launch_bounds__(256) void rref_fast_kernel(const uint32_t *__restri
 			rowid[slot] = row;
 		__syncthreads();
 		if (tid == 0)
-			s_cnt += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+			for (int w = 0; w < nwaves; ++w)
+				s_cnt += s_wave[w];
 		__syncthreads();
 		if (s_cnt >= RREF_FAST) {
 			// rows beyond the collected ones remain unseen unless this was the end
-			scanned_all = (s_cnt == RREF_FAST && base + 256 >= m);
+			scanned_all = (s_cnt == RREF_FAST && base + (int) blockDim.x >= m);
 			break;
 		}
 	}
@@ -194,8 +223,7 @@ __global__ __launch_bounds__(256) void rref_fast_kernel(const uint32_t *__restri
 		const int r = g * RPG + i;
 		v[i] = (r < nr && cc < wc) ? A[(size_t) rowid[r] * lda + c + cc] : 0u;
 	}
-	const int cnt = reg_eliminate<RREF_W, RPG>(v, nr, wc, false, mp, alive, f, pivrow, &s_min, &s_inv, outr, outc,
-						   st.bad);
+	const int cnt = reg_eliminate<RREF_W, RPG, SMALL>(v, nr, wc, false, mp, alive, f, pivrow, s_min, outr, outc, st.bad);
 	const bool done = cnt == wc || scanned_all;
 	for (int t = tid; t < RREF_W; t += blockDim.x) {
 		if (done) {
@@ -210,15 +238,15 @@ __global__ __launch_bounds__(256) void rref_fast_kernel(const uint32_t *__restri
 	}
 }
 
-__global__ __launch_bounds__(256) void rref_tourney_kernel(const uint32_t *__restrict__ A, size_t lda, int m,
+template <bool SMALL>
+__global__ __launch_bounds__(1024) void rref_tourney_kernel(const uint32_t *__restrict__ A, size_t lda, int m,
 							   int c, int wc, ModP mp, RrefState st, int nodes0)
 {
-	constexpr int RPG = RREF_CHUNK / 4;
+	constexpr int RPG = RREF_CHUNK / (1024 / RREF_W);
 	__shared__ int rowid[RREF_CHUNK], alive[RREF_CHUNK];
 	__shared__ uint32_t f[RREF_CHUNK], pivrow[RREF_W];
 	__shared__ int outr[RREF_W], outc[RREF_W];
-	__shared__ int s_min, s_go, s_nr, s_n0;
-	__shared__ uint32_t s_inv;
+	__shared__ int s_min[2], s_go, s_nr, s_n0;
 	if (*st.need_tree == 0)
 		return;
 	const int tid = threadIdx.x, cc = tid % RREF_W, g = tid / RREF_W;
@@ -238,7 +266,7 @@ __global__ __launch_bounds__(256) void rref_tourney_kernel(const uint32_t *__res
 		v[i] = (r < nr0 && cc < wc) ? A[(size_t) (r0 + r) * lda + c + cc] : 0u;
 	}
 	__syncthreads();
-	int cnt = reg_eliminate<RREF_W, RPG>(v, nr0, wc, false, mp, alive, f, pivrow, &s_min, &s_inv, outr, outc, st.bad);
+	int cnt = reg_eliminate<RREF_W, RPG, SMALL>(v, nr0, wc, false, mp, alive, f, pivrow, s_min, outr, outc, st.bad);
 	for (int t = tid; t < cnt; t += blockDim.x)
 		outr[t] = rowid[outr[t]];
 	__syncthreads();
@@ -299,7 +327,7 @@ __global__ __launch_bounds__(256) void rref_tourney_kernel(const uint32_t *__res
 			const int r = g * RPG + i;
 			v[i] = (r < nr && cc < wc) ? A[(size_t) rowid[r] * lda + c + cc] : 0u;
 		}
-		cnt = reg_eliminate<RREF_W, RPG>(v, nr, wc, false, mp, alive, f, pivrow, &s_min, &s_inv, outr, outc, st.bad);
+		cnt = reg_eliminate<RREF_W, RPG, SMALL>(v, nr, wc, false, mp, alive, f, pivrow, s_min, outr, outc, st.bad);
 		for (int t = tid; t < cnt; t += blockDim.x)
 			outr[t] = rowid[outr[t]];
 		__syncthreads();
@@ -328,14 +356,14 @@ __global__ void rref_gather_kernel(const uint32_t *__restrict__ A, size_t lda, i
 
 // Inverse of P_J (k x k, padded with the identity to RREF_W) by Gauss-Jordan on
 // [P_J | I] held in registers (128 columns x 2 groups of 32 rows).
-__global__ __launch_bounds__(256) void rref_inv_kernel(const uint32_t *__restrict__ P, int nc, ModP mp,
+template <bool SMALL>
+__global__ __launch_bounds__(1024) void rref_inv_kernel(const uint32_t *__restrict__ P, int nc, ModP mp,
 						       RrefState st, uint32_t *__restrict__ Pinv)
 {
-	constexpr int W = RREF_W, NC = 2 * RREF_W, RPG = RREF_W / 2;
+	constexpr int W = RREF_W, NC = 2 * RREF_W, RPG = RREF_W / (1024 / NC);
 	__shared__ int alive[W], outr[W], outc[W], pos[W];
 	__shared__ uint32_t f[W], pivrow[NC];
-	__shared__ int s_min;
-	__shared__ uint32_t s_inv;
+	__shared__ int s_min[2];
 	const int tid = threadIdx.x, c = tid % NC, g = tid / NC;
 	const int k = *st.k;
 	uint32_t v[RPG];
@@ -354,7 +382,7 @@ __global__ __launch_bounds__(256) void rref_inv_kernel(const uint32_t *__restric
 		pos[r] = r;
 	}
 	__syncthreads();
-	const int cnt = reg_eliminate<NC, RPG>(v, W, k, true, mp, alive, f, pivrow, &s_min, &s_inv, outr, outc, st.bad);
+	const int cnt = reg_eliminate<NC, RPG, SMALL>(v, W, k, true, mp, alive, f, pivrow, s_min, outr, outc, st.bad);
 	if (tid == 0 && cnt < k)
 		*st.bad = 1;
 	for (int s = tid; s < cnt; s += blockDim.x)
@@ -500,14 +528,22 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 		const int nc = (int) (n - c);
 		{
 			ProfScope ps(ctx, "rref_pivots");
-			rref_fast_kernel<<<1, 256, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st);
-			rref_tourney_kernel<<<nodes0, 256, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, nodes0);
+			if (mp.small) {
+				rref_fast_kernel<true><<<1, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st);
+				rref_tourney_kernel<true><<<nodes0, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, nodes0);
+			} else {
+				rref_fast_kernel<false><<<1, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st);
+				rref_tourney_kernel<false><<<nodes0, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, nodes0);
+			}
 		}
 		MLZ_TRY(check_launch(ctx, "rref_tourney_kernel"));
 		rref_gather_kernel<<<dim3(ceil_div(nc, 256), RREF_W), 256, 0, ctx->stream>>>(Wm, n, nc, (int) c, panel, st, P);
 		{
 			ProfScope ps(ctx, "rref_inv");
-			rref_inv_kernel<<<1, 256, 0, ctx->stream>>>(P, nc, mp, st, Pinv);
+			if (mp.small)
+				rref_inv_kernel<true><<<1, 1024, 0, ctx->stream>>>(P, nc, mp, st, Pinv);
+			else
+				rref_inv_kernel<false><<<1, 1024, 0, ctx->stream>>>(P, nc, mp, st, Pinv);
 		}
 		MLZ_TRY(check_launch(ctx, "rref_inv_kernel"));
 		MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, RREF_W, nc, RREF_W, 1, Pinv, RREF_W, P, nc, 0, Wb, nc, algo));
