@@ -76,6 +76,11 @@ enum {
 int mlz_ctx_create(mlz_ctx **out, int device, const char *config);
 int mlz_ctx_destroy(mlz_ctx *ctx);
 int mlz_ctx_set_ptr_mode(mlz_ctx *ctx, int mode);
+/* Tuning knobs (results never depend on them):
+ *   "gemm.i8.variant"  INT8 gemm workgroup tile: 0 by shape, 1 = 128x128,
+ *                      2 = 128x256, 3 = 256x128, 4 = 128x128 with two
+ *                      workgroups per CU. */
+int mlz_ctx_set_option(mlz_ctx *ctx, const char *key, int64_t value);
 int mlz_ctx_synchronize(mlz_ctx *ctx);
 /* Message for the last error on this context (static storage, never NULL). */
 const char *mlz_last_error(mlz_ctx *ctx);

@@ -139,6 +139,10 @@ class Context:
             raise WrongMatrixForm(idx)
         raise LELAError(f"{what}: code {rc}: {self.last_error()}")
 
+    def set_option(self, key, value):
+        """Tuning knob of the context (mlz_ctx_set_option); never changes results."""
+        self.check(self.lib.mlz_ctx_set_option(self.handle, key.encode(), int(value)), "set_option")
+
     def set_ptr_mode(self, mode):
         self.check(self.lib.mlz_ctx_set_ptr_mode(self.handle, mode), "set_ptr_mode")
         self._ptr_mode = mode

@@ -32,6 +32,7 @@ SIGNATURES = {
     "mlz_ctx_create": (_c_int, [ctypes.POINTER(_ctx), _c_int, ctypes.c_char_p]),
     "mlz_ctx_destroy": (_c_int, [_ctx]),
     "mlz_ctx_set_ptr_mode": (_c_int, [_ctx, _c_int]),
+    "mlz_ctx_set_option": (_c_int, [_ctx, ctypes.c_char_p, ctypes.c_int64]),
     "mlz_ctx_synchronize": (_c_int, [_ctx]),
     "mlz_last_error": (ctypes.c_char_p, [_ctx]),
     "mlz_version": (ctypes.c_char_p, []),

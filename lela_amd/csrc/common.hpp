@@ -245,6 +245,7 @@ struct mlz_ctx {
 	int num_cus;
 	void *host_pinned;
 	size_t host_pinned_bytes;
+	int i8_variant;      // INT8 gemm tile variant, 0 = chosen by shape
 };
 
 namespace mlz {

@@ -164,6 +164,7 @@ int mlz_ctx_create(mlz_ctx **out, int device, const char *config)
 	}
 	ctx->host_pinned = nullptr;
 	ctx->host_pinned_bytes = 0;
+	ctx->i8_variant = 0;
 	// Default priority only: streams of different priorities are not used.
 	if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
 		delete ctx;
@@ -211,6 +212,19 @@ int mlz_ctx_set_ptr_mode(mlz_ctx *ctx, int mode)
 		return MLZ_E_ARG;
 	ctx->ptr_mode = mode;
 	return MLZ_OK;
+}
+
+int mlz_ctx_set_option(mlz_ctx *ctx, const char *key, int64_t value)
+{
+	if (!ctx || !key)
+		return MLZ_E_ARG;
+	if (strcmp(key, "gemm.i8.variant") == 0) {
+		if (value < 0 || value > 6)
+			return set_error(ctx, MLZ_E_ARG, "gemm.i8.variant must be 0..6");
+		ctx->i8_variant = (int) value;
+		return MLZ_OK;
+	}
+	return set_error(ctx, MLZ_E_ARG, "unknown option '%s'", key);
 }
 
 int mlz_ctx_synchronize(mlz_ctx *ctx)

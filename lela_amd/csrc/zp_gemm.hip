@@ -49,7 +49,42 @@ __device__ inline uint32_t i8_limbs(uint32_t v, uint32_t p)
 // Packs a 64 (index) x 64 (k) tile into 2 x 2 fragment pairs.
 // TRANS = false: element(i, kk) = src[i*ld + kk]   (A, row-major m x k)
 // TRANS = true : element(i, kk) = src[kk*ld + i]   (B, row-major k x n)
-template <int E, bool TRANS>
+// Vector loads need a 16-byte aligned base and row stride.
+inline bool aligned16(const void *p, size_t row_bytes)
+{
+	return ((uintptr_t) p % 16) == 0 && (row_bytes % 16) == 0;
+}
+
+// Eight consecutive elements of the contiguous dimension, one vector load
+// when the whole run is inside the matrix and aligned (VEC), else per element.
+template <int E, bool VEC>
+__device__ inline void load8(const typename Elem<E>::T *ptr, int valid, uint32_t p, uint32_t (&out)[8])
+{
+	typedef typename Elem<E>::T T;
+	if (VEC && valid >= 8) {
+		if (sizeof(T) == 2) {
+			const uint4 w = *(const uint4 *) ptr;
+			const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+			for (int q = 0; q < 4; ++q) {
+				out[2 * q] = ws[q] & 0xffffu;
+				out[2 * q + 1] = ws[q] >> 16;
+			}
+			return;
+		}
+		if (sizeof(T) == 4) {
+			const uint4 w0 = ((const uint4 *) ptr)[0], w1 = ((const uint4 *) ptr)[1];
+			out[0] = w0.x; out[1] = w0.y; out[2] = w0.z; out[3] = w0.w;
+			out[4] = w1.x; out[5] = w1.y; out[6] = w1.z; out[7] = w1.w;
+			return;
+		}
+	}
+#pragma unroll
+	for (int q = 0; q < 8; ++q)
+		out[q] = q < valid ? Elem<E>::load(ptr + q, p) : 0u;
+}
+
+template <int E, bool TRANS, bool VEC>
 __global__ __launch_bounds__(256) void pack_i8_kernel(const typename Elem<E>::T *__restrict__ src,
 						      size_t ld, int idim, int kdim, uint32_t p,
 						      int8_t *__restrict__ dst, int KB)
@@ -57,19 +92,27 @@ __global__ __launch_bounds__(256) void pack_i8_kernel(const typename Elem<E>::T 
 	__shared__ uint32_t tile[64][65];
 	const int tid = threadIdx.x;
 	const int i0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+	// 64 x 64 tile = 512 runs of 8 along the contiguous dimension, 2 per thread
 #pragma unroll
-	for (int e = 0; e < 16; ++e) {
-		int idx = e * 256 + tid;
-		int i, kk;
-		if (!TRANS) { i = idx >> 6; kk = idx & 63; }
-		else        { kk = idx >> 6; i = idx & 63; }
-		int gi = i0 + i, gk = k0 + kk;
-		uint32_t v = 0;
-		if (gi < idim && gk < kdim) {
-			const typename Elem<E>::T *ptr = TRANS ? src + (size_t) gk * ld + gi : src + (size_t) gi * ld + gk;
-			v = Elem<E>::load(ptr, p);
+	for (int e = 0; e < 2; ++e) {
+		const int idx = e * 256 + tid;
+		const int line = idx >> 3, off = (idx & 7) * 8;
+		uint32_t v[8];
+		if (!TRANS) {          // line = row i, runs along k
+			const int gi = i0 + line, gk = k0 + off;
+			const int valid = gi < idim ? min(8, kdim - gk) : 0;
+			load8<E, VEC>(src + (size_t) gi * ld + gk, valid, p, v);
+#pragma unroll
+			for (int q = 0; q < 8; ++q)
+				tile[line][off + q] = i8_limbs(q < valid ? v[q] : 0u, p);
+		} else {               // line = k, runs along i
+			const int gk = k0 + line, gi = i0 + off;
+			const int valid = gk < kdim ? min(8, idim - gi) : 0;
+			load8<E, VEC>(src + (size_t) gk * ld + gi, valid, p, v);
+#pragma unroll
+			for (int q = 0; q < 8; ++q)
+				tile[off + q][line] = i8_limbs(q < valid ? v[q] : 0u, p);
 		}
-		tile[i][kk] = i8_limbs(v, p);
 	}
 	__syncthreads();
 	const int f = tid >> 6, lane = tid & 63;
@@ -114,106 +157,115 @@ __device__ inline void tile_of(int bid, int tiles_m, int tiles_n, int &tm, int &
 	tn = in_group / gsz;
 }
 
-// 128 x 128 output tile per workgroup, 4 waves in 2 x 2, each wave 64 x 64
-// (2 x 2 MFMA blocks of 32 x 32) with three int32 accumulators (HH, HL+LH,
-// LL).  The k loop stages 64 k-values (2 k-blocks x 2 limbs) of A and B per
-// step through double-buffered LDS.
-constexpr int I8_BM = 128, I8_BN = 128;
+// Workgroup of WM x WN waves; each wave computes (32*WTM) x (32*WTN)
+// outputs as WTM x WTN MFMA blocks of 32 x 32, with three int32 accumulators
+// (HH, HL+LH, LL) each.  The k loop stages 32*KK k-values (KK k-blocks x 2
+// limbs) of A and B per step through double-buffered LDS, the next step's
+// global loads in flight during the current step's MFMAs.
+template <int WTM, int WTN, int KK, int WM = 2, int WN = 2>
+struct I8Tile {
+	static constexpr int THREADS = 64 * WM * WN;
+	static constexpr int BM = 32 * WTM * WM, BN = 32 * WTN * WN;  // workgroup tile
+	static constexpr int RBA = WTM * WM, RBB = WTN * WN;          // 32-index blocks
+	static constexpr int CHUNK = KK * 2048;                       // bytes per block per step
+	static constexpr int STAGE = (RBA + RBB) * CHUNK;             // LDS bytes per step
+	static constexpr int LOADS = STAGE / (THREADS * 16);          // 16-byte loads per thread
+	static_assert(STAGE % (THREADS * 16) == 0, "stage must split into whole loads");
+};
 
-template <int E>
-__global__ __launch_bounds__(256) void gemm_i8_kernel(const int8_t *__restrict__ Ap,
-						      const int8_t *__restrict__ Bp, int KB,
-						      typename Elem<E>::T *__restrict__ C, size_t ldc,
-						      int m, int n, uint32_t a, uint32_t b, ModP mp,
-						      uint64_t off, int tiles_m, int tiles_n)
+template <int E, int WTM, int WTN, int KK, int OCC, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_kernel(const int8_t *__restrict__ Ap,
+							   const int8_t *__restrict__ Bp, int KB,
+							   typename Elem<E>::T *__restrict__ C, size_t ldc,
+							   int m, int n, uint32_t a, uint32_t b, ModP mp,
+							   uint64_t off, int tiles_m, int tiles_n)
 {
-	// [stage][operand][4 index-blocks x 2 k-blocks x 2 limbs x 1024 bytes]
-	__shared__ __attribute__((aligned(16))) int8_t lds[2][2][4 * 4 * I8_FRAG];
+	typedef I8Tile<WTM, WTN, KK, WM, WN> T;
+	// [stage][A: RBA blocks | B: RBB blocks] x [KK k-blocks x 2 limbs x 1024 bytes]
+	__shared__ __attribute__((aligned(16))) int8_t lds[2][T::STAGE];
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-	const int wr = wave >> 1, wc = wave & 1;
+	const int wr = wave / WN, wc = wave % WN;
 	int tm, tn;
 	tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
 
 	const size_t panel = (size_t) KB * 2 * I8_FRAG;   // bytes per 32-index block
-	const int8_t *Ag = Ap + (size_t) (tm * 4) * panel + tid * 16;
-	const int8_t *Bg = Bp + (size_t) (tn * 4) * panel + tid * 16;
+	const int8_t *Ag = Ap + (size_t) (tm * T::RBA) * panel;
+	const int8_t *Bg = Bp + (size_t) (tn * T::RBB) * panel;
+	// Load c of a step covers stage bytes [c*16*THREADS, (c+1)*16*THREADS);
+	// thread tid moves bytes g = c*16*THREADS + tid*16, i.e. block g / CHUNK at
+	// offset g % CHUNK.
+	const int8_t *src[T::LOADS];
+#pragma unroll
+	for (int c = 0; c < T::LOADS; ++c) {
+		const int g = c * 16 * T::THREADS + tid * 16;
+		const int blk = g / T::CHUNK, o = g % T::CHUNK;
+		src[c] = blk < T::RBA ? Ag + (size_t) blk * panel + o : Bg + (size_t) (blk - T::RBA) * panel + o;
+	}
 
-	v4i ra[4], rb[4];
-	v16i acc[3][2][2];
+	v4i stage[T::LOADS];
+	v16i acc[3][WTM][WTN];
 #pragma unroll
 	for (int t = 0; t < 3; ++t)
 #pragma unroll
-		for (int i = 0; i < 2; ++i)
+		for (int i = 0; i < WTM; ++i)
 #pragma unroll
-			for (int j = 0; j < 2; ++j)
+			for (int j = 0; j < WTN; ++j)
 				acc[t][i][j] = v16i{0};
 
-	const int KS = KB / 2;
+	const int KS = KB / KK;
 #pragma unroll
-	for (int c = 0; c < 4; ++c) {
-		ra[c] = *(const v4i *) (Ag + c * panel);
-		rb[c] = *(const v4i *) (Bg + c * panel);
-	}
+	for (int c = 0; c < T::LOADS; ++c)
+		stage[c] = *(const v4i *) src[c];
 	for (int ks = 0; ks < KS; ++ks) {
-		const int st = ks & 1;
-		int8_t *la = lds[st][0], *lb = lds[st][1];
+		int8_t *la = lds[ks & 1], *lb = la + T::RBA * T::CHUNK;
 #pragma unroll
-		for (int c = 0; c < 4; ++c) {
-			*(v4i *) (la + c * 4096 + tid * 16) = ra[c];
-			*(v4i *) (lb + c * 4096 + tid * 16) = rb[c];
-		}
+		for (int c = 0; c < T::LOADS; ++c)
+			*(v4i *) (la + c * 16 * T::THREADS + tid * 16) = stage[c];
 		__syncthreads();
 		if (ks + 1 < KS) {
-			const size_t o = (size_t) (ks + 1) * 4096;
+			const size_t o = (size_t) (ks + 1) * T::CHUNK;
 #pragma unroll
-			for (int c = 0; c < 4; ++c) {
-				ra[c] = *(const v4i *) (Ag + c * panel + o);
-				rb[c] = *(const v4i *) (Bg + c * panel + o);
-			}
+			for (int c = 0; c < T::LOADS; ++c)
+				stage[c] = *(const v4i *) (src[c] + o);
 		}
 #pragma unroll
-		for (int kk = 0; kk < 2; ++kk) {
-			v4i fa[2][2], fb[2][2];
+		for (int kk = 0; kk < KK; ++kk) {
+			v4i fa[WTM][2];
 #pragma unroll
-			for (int i = 0; i < 2; ++i)
+			for (int i = 0; i < WTM; ++i)
 #pragma unroll
-				for (int l = 0; l < 2; ++l) {
-					fa[i][l] = *(const v4i *) (la + (wr * 2 + i) * 4096 + kk * 2048 + l * 1024 + lane * 16);
-					fb[i][l] = *(const v4i *) (lb + (wc * 2 + i) * 4096 + kk * 2048 + l * 1024 + lane * 16);
-				}
+				for (int l = 0; l < 2; ++l)
+					fa[i][l] = *(const v4i *) (la + (wr * WTM + i) * T::CHUNK + kk * 2048 + l * 1024 + lane * 16);
 #pragma unroll
-			for (int i = 0; i < 2; ++i)
+			for (int j = 0; j < WTN; ++j) {
+				const v4i fb0 = *(const v4i *) (lb + (wc * WTN + j) * T::CHUNK + kk * 2048 + lane * 16);
+				const v4i fb1 = *(const v4i *) (lb + (wc * WTN + j) * T::CHUNK + kk * 2048 + 1024 + lane * 16);
 #pragma unroll
-				for (int j = 0; j < 2; ++j)
-					acc[0][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][0], fb[j][0], acc[0][i][j], 0, 0, 0);
+				for (int i = 0; i < WTM; ++i)
+					acc[0][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][0], fb0, acc[0][i][j], 0, 0, 0);
 #pragma unroll
-			for (int i = 0; i < 2; ++i)
+				for (int i = 0; i < WTM; ++i)
+					acc[1][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][0], fb1, acc[1][i][j], 0, 0, 0);
 #pragma unroll
-				for (int j = 0; j < 2; ++j)
-					acc[1][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][0], fb[j][1], acc[1][i][j], 0, 0, 0);
+				for (int i = 0; i < WTM; ++i)
+					acc[2][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][1], fb1, acc[2][i][j], 0, 0, 0);
 #pragma unroll
-			for (int i = 0; i < 2; ++i)
-#pragma unroll
-				for (int j = 0; j < 2; ++j)
-					acc[2][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][1], fb[j][1], acc[2][i][j], 0, 0, 0);
-#pragma unroll
-			for (int i = 0; i < 2; ++i)
-#pragma unroll
-				for (int j = 0; j < 2; ++j)
-					acc[1][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][1], fb[j][0], acc[1][i][j], 0, 0, 0);
+				for (int i = 0; i < WTM; ++i)
+					acc[1][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][1], fb0, acc[1][i][j], 0, 0, 0);
+			}
 		}
 	}
 
 	// Epilogue.  32x32 accumulator layout: lane l, register r holds
 	// row 8*(r/4) + 4*(l/32) + r%4, column l%32.
 #pragma unroll
-	for (int i = 0; i < 2; ++i)
+	for (int i = 0; i < WTM; ++i)
 #pragma unroll
-		for (int j = 0; j < 2; ++j) {
-			const int col = tn * I8_BN + wc * 64 + j * 32 + (lane & 31);
+		for (int j = 0; j < WTN; ++j) {
+			const int col = tn * T::BN + wc * 32 * WTN + j * 32 + (lane & 31);
 #pragma unroll
 			for (int r = 0; r < 16; ++r) {
-				const int row = tm * I8_BM + wr * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+				const int row = tm * T::BM + wr * 32 * WTM + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
 				if (row < m && col < n) {
 					int64_t v = (int64_t) acc[0][i][j][r] * 65536 + (int64_t) acc[1][i][j][r] * 256 +
 						    (int64_t) acc[2][i][j][r];
@@ -228,14 +280,24 @@ __global__ __launch_bounds__(256) void gemm_i8_kernel(const int8_t *__restrict__
 		}
 }
 
-template <int E>
-static int gemm_i8_launch(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, uint32_t a,
-			  const void *A, size_t lda, const void *B, size_t ldb, uint32_t b, void *C,
-			  size_t ldc)
+// Tile variants of the INT8 kernel (index = ctx option "gemm.i8.variant"):
+//   1: 128 x 128, 4 waves, 64 k per step
+//   2: 128 x 256, 4 waves of 64 x 128, 32 k per step
+//   3: 256 x 128, 4 waves of 128 x 64, 32 k per step
+//   4: 128 x 128, 4 waves, 32 k per step, 2 waves/SIMD
+//   5: 128 x 256, 8 waves of 64 x 64, 32 k per step, 2 waves/SIMD
+//   6: 256 x 128, 8 waves of 64 x 64, 32 k per step, 2 waves/SIMD
+enum { I8V_AUTO = 0, I8V_COUNT = 7 };
+
+template <int E, int WTM, int WTN, int KK, int OCC, int WM = 2, int WN = 2>
+static int gemm_i8_launch_t(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, uint32_t a,
+			    const void *A, size_t lda, const void *B, size_t ldb, uint32_t b, void *C,
+			    size_t ldc)
 {
 	typedef typename Elem<E>::T T;
+	typedef I8Tile<WTM, WTN, KK, WM, WN> Tile;
 	const ModP mp = make_modp(p);
-	const size_t mpad = round_up(m, I8_BM), npad = round_up(n, I8_BN);
+	const size_t mpad = round_up(m, Tile::BM), npad = round_up(n, Tile::BN);
 	// |v| < 65536*2^31 + 256*2^31 + 2^31 < 2^48; off is a multiple of p above that.
 	const uint64_t off = ((((uint64_t) 1 << 48) + p - 1) / p) * p;
 	for (size_t k0 = 0; k0 < k; k0 += I8_KMAX) {
@@ -248,27 +310,72 @@ static int gemm_i8_launch(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k
 		{
 			ProfScope ps(ctx, "pack_i8_A");
 			dim3 g(mpad / 64, kpad / 64);
-			pack_i8_kernel<E, false><<<g, 256, 0, ctx->stream>>>((const T *) A + k0, lda, (int) m,
-									      (int) kc, p, (int8_t *) ap, KB);
+			const T *src = (const T *) A + k0;
+			if (aligned16(src, lda * sizeof(T)))
+				pack_i8_kernel<E, false, true><<<g, 256, 0, ctx->stream>>>(src, lda, (int) m, (int) kc, p,
+											   (int8_t *) ap, KB);
+			else
+				pack_i8_kernel<E, false, false><<<g, 256, 0, ctx->stream>>>(src, lda, (int) m, (int) kc, p,
+											    (int8_t *) ap, KB);
 		}
 		MLZ_TRY(check_launch(ctx, "pack_i8_kernel(A)"));
 		{
 			ProfScope ps(ctx, "pack_i8_B");
 			dim3 g(npad / 64, kpad / 64);
-			pack_i8_kernel<E, true><<<g, 256, 0, ctx->stream>>>((const T *) B + k0 * ldb, ldb, (int) n,
-									     (int) kc, p, (int8_t *) bp, KB);
+			const T *src = (const T *) B + k0 * ldb;
+			if (aligned16(src, ldb * sizeof(T)))
+				pack_i8_kernel<E, true, true><<<g, 256, 0, ctx->stream>>>(src, ldb, (int) n, (int) kc, p,
+											  (int8_t *) bp, KB);
+			else
+				pack_i8_kernel<E, true, false><<<g, 256, 0, ctx->stream>>>(src, ldb, (int) n, (int) kc, p,
+											   (int8_t *) bp, KB);
 		}
 		MLZ_TRY(check_launch(ctx, "pack_i8_kernel(B)"));
-		const int tiles_m = (int) (mpad / I8_BM), tiles_n = (int) (npad / I8_BN);
+		const int tiles_m = (int) (mpad / Tile::BM), tiles_n = (int) (npad / Tile::BN);
 		{
 			ProfScope ps(ctx, "gemm_i8_mfma");
-			gemm_i8_kernel<E><<<tiles_m * tiles_n, 256, 0, ctx->stream>>>(
+			gemm_i8_kernel<E, WTM, WTN, KK, OCC, WM, WN><<<tiles_m * tiles_n, Tile::THREADS, 0, ctx->stream>>>(
 				(const int8_t *) ap, (const int8_t *) bp, KB, (T *) C, ldc, (int) m, (int) n, a,
 				k0 == 0 ? b : 1u, mp, off, tiles_m, tiles_n);
 		}
 		MLZ_TRY(check_launch(ctx, "gemm_i8_kernel"));
 	}
 	return MLZ_OK;
+}
+
+// Variant choice when none is forced (measured at 8192^3: 5 > 6 > 4 > 1;
+// 2 and 3 spill registers): the 8-wave 128 x 256 tile once the output fills
+// the 256 CUs with it, else the 128 x 128 tile at two workgroups per CU.
+static int i8_variant_for(size_t m, size_t n)
+{
+	const size_t tiles_wide = ceil_div(m, 128) * ceil_div(n, 256);
+	if (tiles_wide >= 256)
+		return 5;
+	return 4;
+}
+
+template <int E>
+static int gemm_i8_launch(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, uint32_t a,
+			  const void *A, size_t lda, const void *B, size_t ldb, uint32_t b, void *C,
+			  size_t ldc)
+{
+	int v = ctx->i8_variant;
+	if (v <= I8V_AUTO || v >= I8V_COUNT)
+		v = i8_variant_for(m, n);
+	switch (v) {
+	case 2:
+		return gemm_i8_launch_t<E, 2, 4, 1, 1>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
+	case 3:
+		return gemm_i8_launch_t<E, 4, 2, 1, 1>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
+	case 4:
+		return gemm_i8_launch_t<E, 2, 2, 1, 2>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
+	case 5:
+		return gemm_i8_launch_t<E, 2, 2, 1, 2, 2, 4>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
+	case 6:
+		return gemm_i8_launch_t<E, 2, 2, 1, 2, 4, 2>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
+	default:
+		return gemm_i8_launch_t<E, 2, 2, 2, 1>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
+	}
 }
 
 // ===========================================================================

@@ -81,6 +81,21 @@ def test_gemm_strided_views(gpu_ctx, algo):
     assert np.array_equal(Cp[mask], outside[mask])
 
 
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("storage", ["uint16", "uint32", "double"])
+def test_gemm_int8_tile_variants(gpu_ctx, variant, storage):
+    """Every INT8 workgroup-tile variant gives the same exact product."""
+    p = 65521
+    gpu_ctx.set_option("gemm.i8.variant", variant)
+    try:
+        for (m, n, k) in [(300, 517, 130), (129, 257, 64), (1, 1, 33)]:
+            A, B, C = synth.zp_matrix(41, m, k, p), synth.zp_matrix(42, k, n, p), synth.zp_matrix(43, m, n, p)
+            got = run_gemm(gpu_ctx, p, storage, "int8", 7, A, B, 3, C)
+            assert np.array_equal(got, orc.zp_gemm(p, 7, A, B, 3, C)), (variant, m, n, k)
+    finally:
+        gpu_ctx.set_option("gemm.i8.variant", 0)
+
+
 def test_gemm_int8_k_chunks(gpu_ctx):
     # k > 65504 splits the int32-exact accumulation into chunks through C.
     p = 65521
