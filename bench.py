@@ -1,12 +1,14 @@
 """Benchmark: exact Z/p gemm, 8192 x 8192 x 8192, p = 65521, uint16 storage,
 C <- A*B + C on one MI355X with split-int8 MFMA; inputs resident in HBM.
 
-Prints one JSON line (see the keys below).  `value` is modular MACs/s of the
-int8 kernel family over the timed steps; `roofline` is the MFMA kernel alone,
-timed with HIP events on the library's stream; `variants` holds the plain
-VALU kernel and the FP64-carrier kernel on the same shape for comparison;
-`cpu_baseline` times the CPU restatement of LELA's ZpModule<uint16> gemm on a
-bounded block of rows of the same workload, one core.
+Prints one JSON line.  `value` is modular MACs/s of the int8 kernel family
+over the timed steps; `roofline` covers the MFMA kernel alone, timed with HIP
+events on the library's stream; `variants` holds the plain VALU kernel and the
+FP64-carrier kernel on the same shape; `fl_echelon` is the Faugere-Lachartre
+reduction of the 20000 x 30000 F4-shaped matrix (seconds, MFMA families vs
+plain kernels); `rows` times the other level-3 workloads; `cpu_baseline`
+times the CPU restatement of LELA's ZpModule<uint16> gemm on a bounded block
+of rows of the same workload, one core.
 """
 import argparse
 import json
@@ -30,8 +32,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-variants", action="store_true", help="skip plain/FP64 gemm, FL and other rows")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="also run one device-to-device copy of BYTES (counter calibration)")
     return ap.parse_args()
 
 
@@ -71,6 +75,73 @@ def cpu_baseline(seconds):
             "sample": f"{rows} of {N} rows of C for the {N}^3 uint16 gemm "
                       f"({rows}*{N}*{N} MACs, {t:.1f} s), CPU restatement of "
                       "ZpModule<uint16> dot + generic gemm, -O2, one core"}
+
+
+def bench_fl(la, synth, np):
+    """Config 4: Faugere-Lachartre on the 20000 x 30000 F4-shaped matrix."""
+    ctx = la.Context(la.Modular(P, "uint32"), device=0)
+    csr, shape = synth.fl_matrix(synth.seed_for(4, 0), P)
+    drp = la.DeviceMatrix.from_host(ctx, csr[0].reshape(-1, 1))
+    dc = la.DeviceMatrix.from_host(ctx, csr[1].reshape(-1, 1))
+    dv = la.DeviceMatrix.from_host(ctx, csr[2].reshape(-1, 1))
+    dR = la.DeviceMatrix(ctx, shape[0], shape[1], np.uint32)
+    FL = la.FaugereLachartre(ctx)
+    out = {"shape": list(shape), "nnz": int(csr[0][-1])}
+    for algo, name in [(la.ALGO_AUTO, "mfma"), (la.ALGO_PLAIN, "plain")]:
+        res = {}
+        t = time_steps(ctx, lambda: res.update(r=FL.echelonize(dR, (drp, dc, dv), shape=shape,
+                                                                 algo=algo)), 2, 1)
+        out[f"{name}_s"] = t
+        out["rank"] = int(res["r"][0])
+    out["speedup"] = out["plain_s"] / out["mfma_s"]
+    for d in (drp, dc, dv, dR):
+        d.free()
+    ctx.close()
+    return out
+
+
+def bench_rows(la, synth, np):
+    """trsm 8192 x 1024 and the GF(2) 16384^2 gemm and rank-12000 RREF."""
+    out = {}
+    ctx = la.Context(la.Modular(P, "uint16"), device=0)
+    L = np.tril(synth.zp_matrix(synth.seed_for(2, 3), N, N, P, np.uint16), -1)
+    L[np.arange(N), np.arange(N)] = 1
+    dL = la.DeviceMatrix.from_host(ctx, L)
+    dB = la.DeviceMatrix.from_host(ctx, synth.zp_matrix(synth.seed_for(2, 4), N, 1024, P, np.uint16))
+    t = time_steps(ctx, lambda: la.BLAS3.trsm(ctx, 1, dL, dB, la.LowerTriangular, True), 3, 1)
+    out["trsm_8192x1024_ms"] = t * 1e3
+    out["trsm_modular_MACs_per_s"] = N * (N - 1) / 2 * 1024 / t
+    dL.free()
+    dB.free()
+    n = 16384
+    ctx.F = la.GF2()
+    dA = la.DeviceMatrix.from_host(ctx, synth.gf2_matrix(synth.seed_for(3, 0), n, n))
+    dB = la.DeviceMatrix.from_host(ctx, synth.gf2_matrix(synth.seed_for(3, 1), n, n))
+    dC = la.DeviceMatrix(ctx, n, n // 64, np.uint64)
+    t = time_steps(ctx, lambda: la.BLAS3.gemm(ctx, 1, la.Dense01Matrix(dA, n), la.Dense01Matrix(dB, n), 0,
+                                              la.Dense01Matrix(dC, n)), 3, 1)
+    out["gf2_gemm_16384_ms"] = t * 1e3
+    out["gf2_bit_MACs_per_s"] = n ** 3 / t
+    r = 12000
+    dX = la.DeviceMatrix.from_host(ctx, synth.gf2_matrix(synth.seed_for(3, 2), n, r))
+    dY = la.DeviceMatrix.from_host(ctx, synth.gf2_matrix(synth.seed_for(3, 3), r, n))
+    la.BLAS3.gemm(ctx, 1, la.Dense01Matrix(dX, r), la.Dense01Matrix(dY, n), 0, la.Dense01Matrix(dC, n))
+    ctx.synchronize()
+    M = dC.download()
+    EF = la.EchelonForm(ctx)
+    times = []
+    for _ in range(2):
+        dC.upload(M)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        EF.echelonize(la.Dense01Matrix(dC, n))
+        ctx.synchronize()
+        times.append(time.perf_counter() - t0)
+    out["gf2_rref_16384_rank12000_ms"] = min(times) * 1e3
+    for d in (dA, dB, dC, dX, dY):
+        d.free()
+    ctx.close()
+    return out
 
 
 def main():
@@ -118,6 +189,14 @@ def main():
     avg_kernel_ms = kernel_ms / max(1, launches)
     achieved = 8.0 * W / (avg_kernel_ms * 1e-3) / 1e12   # int8 ops: 4 limb MACs per modular MAC
 
+    if args.copy_calibration:
+        nb = int(args.copy_calibration)
+        s = la.DeviceMatrix(ctx, 1, nb, np.uint8)
+        d = la.DeviceMatrix(ctx, 1, nb, np.uint8)
+        ctx.check(ctx.lib.mlz_memcpy_dtod(ctx.handle, d.ptr, s.ptr, nb), "memcpy_dtod")
+        s.free()
+        d.free()
+
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "gemm_i8_mfma_8192_traffic.json")
     if os.path.exists(tfile):
@@ -125,7 +204,7 @@ def main():
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
     out = {
-        "metric": "Z/p gemm modular MACs/s @8192^3 p=65521 (split-int8 MFMA)",
+        "metric": "Z/p gemm modular MACs/s @8192^3 p=65521 (split-int8 MFMA) + F-L echelon s",
         "value": value,
         "unit": "modular MACs/s",
         "n_gpus": 1,
@@ -144,7 +223,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS,
                      "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": traffic,
                      "kernel": "gemm_i8_kernel", "kernel_avg_ms": avg_kernel_ms,
-                     "ops_per_launch": 8.0 * W},
+                     "ops_per_launch": 8.0 * W, "op": "int8 multiply-add counted as 2 ops"},
         "timing": {"event_ms_per_step": ev_ms / args.steps,
                    "pack_ms_per_step": pack_ms / args.steps},
     }
@@ -159,9 +238,7 @@ def main():
             d.free()
         F2 = la.Modular(P_FP64, "uint32")
         ctx.F = F2
-        A2 = synth.zp_matrix(synth.seed_for(2, 5), N, N, P_FP64)
-        dA2 = la.DeviceMatrix.from_host(ctx, A2)
-        del A2
+        dA2 = la.DeviceMatrix.from_host(ctx, synth.zp_matrix(synth.seed_for(2, 5), N, N, P_FP64))
         dB2 = la.DeviceMatrix.from_host(ctx, synth.zp_matrix(synth.seed_for(2, 6), N, N, P_FP64))
         dC2 = la.DeviceMatrix.from_host(ctx, synth.zp_matrix(synth.seed_for(2, 7), N, N, P_FP64))
         ctx.prof_reset()
@@ -170,18 +247,21 @@ def main():
         ctx.prof_enable(False)
         cnt, kms = ctx.prof_query("gemm_f64_mfma")
         fp64_kernel_ms = kms / max(1, cnt)
+        tf = 2 * 2 * W / (fp64_kernel_ms * 1e-3) / 1e12     # 2 limbs x (mul + add)
         variants["fp64_u32_p67108859"] = {
             "ms_per_step": t * 1e3, "value": W / t, "unit": "modular MACs/s",
-            "roofline": {"bound": "mfma", "achieved": 2 * 2 * W / (fp64_kernel_ms * 1e-3) / 1e12,
-                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": 2 * 2 * W / (fp64_kernel_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}}
+            "roofline": {"bound": "mfma", "achieved": tf, "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFLOPS}}
         for d in (dA2, dB2, dC2):
             d.free()
         out["variants"] = variants
+        ctx.close()
+        out["fl_echelon"] = bench_fl(la, synth, np)
+        out["rows"] = bench_rows(la, synth, np)
     else:
         for d in (dA, dB, dC):
             d.free()
-    ctx.close()
+        ctx.close()
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)

@@ -78,8 +78,9 @@ int mlz_ctx_destroy(mlz_ctx *ctx);
 int mlz_ctx_set_ptr_mode(mlz_ctx *ctx, int mode);
 /* Tuning knobs (results never depend on them):
  *   "gemm.i8.variant"  INT8 gemm workgroup tile: 0 by shape, 1 = 128x128,
- *                      2 = 128x256, 3 = 256x128, 4 = 128x128 with two
- *                      workgroups per CU. */
+ *                      2 = 128x256, 3 = 256x128 (4 waves each), 4 = 128x128
+ *                      with two workgroups per CU, 5 = 128x256 and
+ *                      6 = 256x128 with 8 waves and two workgroups per CU. */
 int mlz_ctx_set_option(mlz_ctx *ctx, const char *key, int64_t value);
 int mlz_ctx_synchronize(mlz_ctx *ctx);
 /* Message for the last error on this context (static storage, never NULL). */
@@ -92,6 +93,8 @@ int mlz_dev_alloc(mlz_ctx *ctx, size_t bytes, void **ptr);
 int mlz_dev_free(mlz_ctx *ctx, void *ptr);
 int mlz_memcpy_htod(mlz_ctx *ctx, void *dst, const void *src, size_t bytes);
 int mlz_memcpy_dtoh(mlz_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Device-to-device copy on the context's stream (waits for completion). */
+int mlz_memcpy_dtod(mlz_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* Release cached workspace buffers. */
 int mlz_ctx_trim(mlz_ctx *ctx);
 

@@ -40,6 +40,7 @@ SIGNATURES = {
     "mlz_dev_free": (_c_int, [_ctx, _vp]),
     "mlz_memcpy_htod": (_c_int, [_ctx, _vp, _vp, _c_size]),
     "mlz_memcpy_dtoh": (_c_int, [_ctx, _vp, _vp, _c_size]),
+    "mlz_memcpy_dtod": (_c_int, [_ctx, _vp, _vp, _c_size]),
     "mlz_ctx_trim": (_c_int, [_ctx]),
     "mlz_prof_enable": (_c_int, [_ctx, _c_int]),
     "mlz_prof_reset": (_c_int, [_ctx]),

@@ -279,6 +279,15 @@ int mlz_memcpy_dtoh(mlz_ctx *ctx, void *dst, const void *src, size_t bytes)
 	return MLZ_OK;
 }
 
+int mlz_memcpy_dtod(mlz_ctx *ctx, void *dst, const void *src, size_t bytes)
+{
+	if (!ctx)
+		return MLZ_E_ARG;
+	MLZ_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+	MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+	return MLZ_OK;
+}
+
 int mlz_ctx_trim(mlz_ctx *ctx)
 {
 	if (!ctx)
