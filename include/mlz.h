@@ -178,6 +178,101 @@ int mlz_gf2_gemm(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64
 int mlz_gf2_rref(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda_w, size_t *rank,
 		 uint32_t *pivots);
 
+/* ---- batches of independent Z/p instances ------------------------------ */
+/* Many small gemm and echelon-form instances of uneven shapes (the blocks a
+ * Faugere-Lachartre splitting produces, or independent solves) run together
+ * in persistent work-list launches, optionally captured once as a hipGraph
+ * and replayed.  Each instance runs one of several exact kernel variants;
+ * the choice changes speed only.  LELA has no batch interface: a caller makes
+ * one BLAS3::gemm / EchelonForm::echelonize call per instance
+ * (lela/blas/level3.h:81-83, lela/solutions/echelon-form.h:103-154); the
+ * per-instance semantics here are those of mlz_zp_gemm and mlz_zp_rref.
+ * Matrices are device pointers (data resident in HBM). */
+enum { MLZ_TASK_GEMM = 1, MLZ_TASK_RREF = 2 };
+#define MLZ_MAX_VARIANTS 4
+/* gemm variants: 0 plain VALU 64x64, 1 sparse-A VALU 16x256, 2 INT8 MFMA
+ * 64x64, 3 INT8 MFMA 128x128 (2 and 3 need p <= 2^16 and k <= 65504).
+ * rref variants: 0 panels of 64 columns, 1 panels of 32 columns (m <= 4096). */
+typedef struct mlz_task {
+	int op;                /* MLZ_TASK_GEMM or MLZ_TASK_RREF */
+	int variant;           /* in: -1 = chosen by the policy, else forced; out: the variant run */
+	uint32_t m, n, k;      /* gemm: A m x k, B k x n, C m x n; rref: A m x n (k unused) */
+	float density;         /* caller's estimate of the nonzero fraction (cost-model input) */
+	uint64_t a, b;         /* gemm: C <- a*A*B + b*C */
+	void *A;               /* gemm: A; rref: the matrix, reduced in place */
+	size_t lda;
+	const void *B;
+	size_t ldb;
+	void *C;
+	size_t ldc;
+	uint32_t *pivots;      /* rref: device array of min(m, n) pivot columns, may be NULL */
+	size_t rank;           /* rref: out */
+} mlz_task;
+
+enum {
+	MLZ_POLICY_FIXED = 0,  /* every task names its variant */
+	MLZ_POLICY_TABLE = 1,  /* autotuner table (built by mlz_tune); a shape rule where it is empty */
+	MLZ_POLICY_MODEL = 2   /* cost model (mlz_model_train); the table for an op without a model */
+};
+enum {
+	MLZ_BATCH_GRAPH = 1,   /* capture the launches once as a hipGraph; mlz_batch_launch replays it */
+	MLZ_BATCH_TIMING = 2,  /* record each task's kernel time (see mlz_batch_finish) */
+	MLZ_BATCH_SERIAL = 4   /* one stream: kernels of different variants do not overlap */
+};
+
+typedef struct mlz_batch mlz_batch;
+
+/* Chooses a variant per task (tasks[i].variant is set on return), builds the
+ * work lists and device workspace, and with MLZ_BATCH_GRAPH captures the
+ * launch sequence.  The task array must stay unchanged while the plan lives. */
+int mlz_batch_plan(mlz_ctx *ctx, mlz_elem elem, uint64_t p, mlz_task *tasks, size_t ntasks, int policy,
+		   int flags, mlz_batch **plan);
+/* Queues one execution of the plan on the context's stream. */
+int mlz_batch_launch(mlz_ctx *ctx, mlz_batch *plan);
+/* Waits for the last execution; sets tasks[i].rank for echelon-form tasks and,
+ * with MLZ_BATCH_TIMING and cost_us != NULL, cost_us[i] = the task's summed
+ * workgroup time divided by the resident workgroups per CU of its kernels
+ * (microseconds of one CU).  MLZ_E_NOT_INVERTIBLE: some pivot had no inverse
+ * (composite p); *bad_task (may be NULL) = the first such task. */
+int mlz_batch_finish(mlz_ctx *ctx, mlz_batch *plan, mlz_task *tasks, float *cost_us, size_t *bad_task);
+int mlz_batch_destroy(mlz_ctx *ctx, mlz_batch *plan);
+/* plan + launch + finish + destroy. */
+int mlz_batch_run(mlz_ctx *ctx, mlz_elem elem, uint64_t p, mlz_task *tasks, size_t ntasks, int policy);
+
+/* Autotuner: runs the tasks once per variant (serial, timed, reps times,
+ * median kept) and writes cost_us[i * MLZ_MAX_VARIANTS + v] (NaN where v does
+ * not apply); the context's table then maps each shape bucket
+ * (floor(log2 m), floor(log2 n), floor(log2 k)) and op to the variant of
+ * least total cost over the tasks in it.  The tasks' matrices are
+ * overwritten (gemm C, rref A): pass scratch copies. */
+int mlz_tune(mlz_ctx *ctx, mlz_elem elem, uint64_t p, mlz_task *tasks, size_t ntasks, int reps, float *cost_us);
+/* Table entries (op 1 or 2): 16 x 16 x 16 bytes, index (lm*16 + ln)*16 + lk,
+ * -1 = empty. */
+int mlz_tune_table_get(mlz_ctx *ctx, int op, int8_t *table);
+int mlz_tune_table_set(mlz_ctx *ctx, int op, const int8_t *table);
+
+/* Cost model: per op an MLP 5 -> 32 -> 32 -> MLZ_MAX_VARIANTS (ReLU, FP32)
+ * that predicts log2 of each variant's cost from the task features below.
+ * Weights: W1[5][32] b1[32] W2[32][32] b2[32] W3[32][4] b3[4] (1380 floats). */
+#define MLZ_MODEL_PARAMS 1380
+/* Features of a task: (log2 m - 7.5)/1.5, (log2 n - 7.5)/1.5, (log2 k' - 7.5)/1.5
+ * with k' = k for gemm and min(m, n) for rref, 2*density - 1, (log2 p - 16)/8. */
+int mlz_task_features(const mlz_task *task, uint64_t p, float *x5);
+/* Trains the op's model on the GPU (full-batch Adam, `steps` steps, learning
+ * rate lr, weights initialised from seed) on x [n][5], y [n][MLZ_MAX_VARIANTS]
+ * (log2 microseconds) and mask [n][MLZ_MAX_VARIANTS] (1 = measured); the loss
+ * is the masked mean squared error.  *loss (may be NULL) = final loss. */
+int mlz_model_train(mlz_ctx *ctx, int op, const float *x, const float *y, const uint8_t *mask, size_t n,
+		    int steps, float lr, uint64_t seed, float *loss);
+int mlz_model_get(mlz_ctx *ctx, int op, float *params);
+int mlz_model_set(mlz_ctx *ctx, int op, const float *params);
+/* Predictions y [n][MLZ_MAX_VARIANTS] of the op's model (GPU inference kernel). */
+int mlz_model_predict(mlz_ctx *ctx, int op, const float *x, size_t n, float *y);
+/* Loss and its gradient (MLZ_MODEL_PARAMS floats) at params on (x, y, mask),
+ * computed by the training kernels. */
+int mlz_model_grad(mlz_ctx *ctx, const float *params, const float *x, const float *y, const uint8_t *mask,
+		   size_t n, float *grad, float *loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -225,7 +225,6 @@ def _host_ptr(X, dtype):
         raise TypeError("expected a row-major 2-D array (rows may be strided)")
     if X.shape[0] == 1:
         return X.ctypes.data, X.shape[1]
-        raise TypeError("expected a row-major 2-D array (rows may be strided)")
     return X.ctypes.data, X.strides[0] // X.itemsize
 
 

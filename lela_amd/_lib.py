@@ -60,7 +60,52 @@ SIGNATURES = {
     "mlz_zp_fl_echelonize": (_c_int, [_ctx, _c_int, _c_u64, _c_size, _c_size, _vp, _vp, _vp, _vp,
                                       _c_size, ctypes.POINTER(_c_size), _vp, ctypes.POINTER(_c_u64),
                                       ctypes.POINTER(_c_size), _c_int]),
+    "mlz_batch_plan": (_c_int, [_ctx, _c_int, _c_u64, _vp, _c_size, _c_int, _c_int, ctypes.POINTER(_vp)]),
+    "mlz_batch_launch": (_c_int, [_ctx, _vp]),
+    "mlz_batch_finish": (_c_int, [_ctx, _vp, _vp, _vp, ctypes.POINTER(_c_size)]),
+    "mlz_batch_destroy": (_c_int, [_ctx, _vp]),
+    "mlz_batch_run": (_c_int, [_ctx, _c_int, _c_u64, _vp, _c_size, _c_int]),
+    "mlz_tune": (_c_int, [_ctx, _c_int, _c_u64, _vp, _c_size, _c_int, _vp]),
+    "mlz_tune_table_get": (_c_int, [_ctx, _c_int, _vp]),
+    "mlz_tune_table_set": (_c_int, [_ctx, _c_int, _vp]),
+    "mlz_task_features": (_c_int, [_vp, _c_u64, _vp]),
+    "mlz_model_train": (_c_int, [_ctx, _c_int, _vp, _vp, _vp, _c_size, _c_int, ctypes.c_float, _c_u64,
+                                 ctypes.POINTER(ctypes.c_float)]),
+    "mlz_model_get": (_c_int, [_ctx, _c_int, _vp]),
+    "mlz_model_set": (_c_int, [_ctx, _c_int, _vp]),
+    "mlz_model_predict": (_c_int, [_ctx, _c_int, _vp, _c_size, _vp]),
+    "mlz_model_grad": (_c_int, [_ctx, _vp, _vp, _vp, _vp, _c_size, _vp, ctypes.POINTER(ctypes.c_float)]),
 }
+
+TASK_GEMM, TASK_RREF = 1, 2
+MAX_VARIANTS = 4
+MODEL_PARAMS = 1380
+POLICY_FIXED, POLICY_TABLE, POLICY_MODEL = 0, 1, 2
+BATCH_GRAPH, BATCH_TIMING, BATCH_SERIAL = 1, 2, 4
+GEMM_PLAIN, GEMM_SPARSE, GEMM_I8S, GEMM_I8L = 0, 1, 2, 3
+RREF_W64, RREF_W32 = 0, 1
+
+
+class Task(ctypes.Structure):
+    """struct mlz_task of include/mlz.h (same field order and types)."""
+    _fields_ = [
+        ("op", ctypes.c_int),
+        ("variant", ctypes.c_int),
+        ("m", ctypes.c_uint32),
+        ("n", ctypes.c_uint32),
+        ("k", ctypes.c_uint32),
+        ("density", ctypes.c_float),
+        ("a", ctypes.c_uint64),
+        ("b", ctypes.c_uint64),
+        ("A", ctypes.c_void_p),
+        ("lda", ctypes.c_size_t),
+        ("B", ctypes.c_void_p),
+        ("ldb", ctypes.c_size_t),
+        ("C", ctypes.c_void_p),
+        ("ldc", ctypes.c_size_t),
+        ("pivots", ctypes.c_void_p),
+        ("rank", ctypes.c_size_t),
+    ]
 
 _lib = None
 

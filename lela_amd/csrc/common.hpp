@@ -246,6 +246,7 @@ struct mlz_ctx {
 	void *host_pinned;
 	size_t host_pinned_bytes;
 	int i8_variant;      // INT8 gemm tile variant, 0 = chosen by shape
+	struct mlz_tuner *tuner;   // autotuner table and cost-model weights (batch.hip), lazily created
 };
 
 namespace mlz {
@@ -300,6 +301,8 @@ int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64
 int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size_t *rank_out,
 		 uint32_t *pivots_out);
 int pick_gemm_algo(uint64_t p, int elem, size_t m, size_t n, size_t k, int algo);
+int check_modulus(mlz_ctx *ctx, int elem, uint64_t p);
+void tuner_free(mlz_ctx *ctx);
 
 // Host <-> device staging of a strided matrix (rows x cols elements of esize
 // bytes, leading dimension ld elements) into a dense device buffer with

@@ -165,6 +165,7 @@ int mlz_ctx_create(mlz_ctx **out, int device, const char *config)
 	ctx->host_pinned = nullptr;
 	ctx->host_pinned_bytes = 0;
 	ctx->i8_variant = 0;
+	ctx->tuner = nullptr;
 	// Default priority only: streams of different priorities are not used.
 	if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
 		delete ctx;
@@ -188,6 +189,7 @@ int mlz_ctx_destroy(mlz_ctx *ctx)
 		return MLZ_E_ARG;
 	(void) hipSetDevice(ctx->device);
 	(void) hipStreamSynchronize(ctx->stream);
+	tuner_free(ctx);
 	for (int i = 0; i < WS_NUM_SLOTS; ++i)
 		if (ctx->ws[i])
 			(void) hipFree(ctx->ws[i]);

@@ -22,7 +22,7 @@
 //
 // C update per entry follows lela/blas/level3-generic.tcc:309-311:
 // c <- b*c + a*d mod p, with d = (A*B)_{ij} mod p.
-#include "common.hpp"
+#include "batch.hpp"
 
 namespace mlz {
 
@@ -84,14 +84,14 @@ __device__ inline void load8(const typename Elem<E>::T *ptr, int valid, uint32_t
 		out[q] = q < valid ? Elem<E>::load(ptr + q, p) : 0u;
 }
 
+// Packs the 64 x 64 tile at (i0, k0) with 256 threads; tile is 64 x 65 words
+// of LDS.  Shared by the single-matrix and the batched pack kernels.
 template <int E, bool TRANS, bool VEC>
-__global__ __launch_bounds__(256) void pack_i8_kernel(const typename Elem<E>::T *__restrict__ src,
-						      size_t ld, int idim, int kdim, uint32_t p,
-						      int8_t *__restrict__ dst, int KB)
+__device__ __forceinline__ void i8_pack_tile(const typename Elem<E>::T *__restrict__ src, size_t ld, int idim,
+					     int kdim, uint32_t p, int8_t *__restrict__ dst, int KB, int i0, int k0,
+					     uint32_t (*tile)[65])
 {
-	__shared__ uint32_t tile[64][65];
 	const int tid = threadIdx.x;
-	const int i0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
 	// 64 x 64 tile = 512 runs of 8 along the contiguous dimension, 2 per thread
 #pragma unroll
 	for (int e = 0; e < 2; ++e) {
@@ -138,6 +138,15 @@ __global__ __launch_bounds__(256) void pack_i8_kernel(const typename Elem<E>::T 
 	*(v4i *) (fr + I8_FRAG) = v4i{(int) lw[0], (int) lw[1], (int) lw[2], (int) lw[3]};
 }
 
+template <int E, bool TRANS, bool VEC>
+__global__ __launch_bounds__(256) void pack_i8_kernel(const typename Elem<E>::T *__restrict__ src,
+						      size_t ld, int idim, int kdim, uint32_t p,
+						      int8_t *__restrict__ dst, int KB)
+{
+	__shared__ uint32_t tile[64][65];
+	i8_pack_tile<E, TRANS, VEC>(src, ld, idim, kdim, p, dst, KB, blockIdx.x * 64, blockIdx.y * 64, tile);
+}
+
 // Maps a linear workgroup id to an output tile.  Workgroups are dispatched
 // round-robin over the 8 XCDs; the remap gives each XCD a contiguous range
 // of tiles (its own L2 working set), then tiles are visited in groups of
@@ -173,20 +182,17 @@ struct I8Tile {
 	static_assert(STAGE % (THREADS * 16) == 0, "stage must split into whole loads");
 };
 
-template <int E, int WTM, int WTN, int KK, int OCC, int WM, int WN>
-__global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_kernel(const int8_t *__restrict__ Ap,
-							   const int8_t *__restrict__ Bp, int KB,
-							   typename Elem<E>::T *__restrict__ C, size_t ldc,
-							   int m, int n, uint32_t a, uint32_t b, ModP mp,
-							   uint64_t off, int tiles_m, int tiles_n)
+// One output tile (tm, tn) of the INT8 product; lds holds 2 * STAGE bytes
+// (double-buffered [A: RBA blocks | B: RBB blocks] x [KK k-blocks x 2 limbs x
+// 1024 bytes]).  Shared by the single-matrix kernel and the batched one.
+template <int E, int WTM, int WTN, int KK, int WM, int WN>
+__device__ __forceinline__ void i8_tile(const int8_t *__restrict__ Ap, const int8_t *__restrict__ Bp, int KB,
+					typename Elem<E>::T *__restrict__ C, size_t ldc, int m, int n, uint32_t a,
+					uint32_t b, const ModP &mp, uint64_t off, int tm, int tn, int8_t *lds_base)
 {
 	typedef I8Tile<WTM, WTN, KK, WM, WN> T;
-	// [stage][A: RBA blocks | B: RBB blocks] x [KK k-blocks x 2 limbs x 1024 bytes]
-	__shared__ __attribute__((aligned(16))) int8_t lds[2][T::STAGE];
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int wr = wave / WN, wc = wave % WN;
-	int tm, tn;
-	tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
 
 	const size_t panel = (size_t) KB * 2 * I8_FRAG;   // bytes per 32-index block
 	const int8_t *Ag = Ap + (size_t) (tm * T::RBA) * panel;
@@ -217,7 +223,7 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_kernel(const int8_t
 	for (int c = 0; c < T::LOADS; ++c)
 		stage[c] = *(const v4i *) src[c];
 	for (int ks = 0; ks < KS; ++ks) {
-		int8_t *la = lds[ks & 1], *lb = la + T::RBA * T::CHUNK;
+		int8_t *la = lds_base + (ks & 1) * T::STAGE, *lb = la + T::RBA * T::CHUNK;
 #pragma unroll
 		for (int c = 0; c < T::LOADS; ++c)
 			*(v4i *) (la + c * 16 * T::THREADS + tid * 16) = stage[c];
@@ -278,6 +284,19 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_kernel(const int8_t
 				}
 			}
 		}
+}
+
+template <int E, int WTM, int WTN, int KK, int OCC, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_kernel(const int8_t *__restrict__ Ap,
+							   const int8_t *__restrict__ Bp, int KB,
+							   typename Elem<E>::T *__restrict__ C, size_t ldc,
+							   int m, int n, uint32_t a, uint32_t b, ModP mp,
+							   uint64_t off, int tiles_m, int tiles_n)
+{
+	__shared__ __attribute__((aligned(16))) int8_t lds[2 * I8Tile<WTM, WTN, KK, WM, WN>::STAGE];
+	int tm, tn;
+	tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+	i8_tile<E, WTM, WTN, KK, WM, WN>(Ap, Bp, KB, C, ldc, m, n, a, b, mp, off, tm, tn, lds);
 }
 
 // Tile variants of the INT8 kernel (index = ctx option "gemm.i8.variant"):
@@ -382,19 +401,16 @@ static int gemm_i8_launch(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k
 // PLAIN path
 // ===========================================================================
 
-// 64 x 64 output tile, 16 x 16 threads, 4 x 4 entries per thread, k staged
-// 16 at a time through LDS as canonical uint32.
+// The output tile at (row0, col0) of the plain kernel; As and Bs are 16 x 64
+// words of LDS each.  Shared by the single-matrix and the batched kernels.
 template <int E, bool WIDE>
-__global__ __launch_bounds__(256) void gemm_plain_kernel(const typename Elem<E>::T *__restrict__ A, size_t lda,
-							 const typename Elem<E>::T *__restrict__ B, size_t ldb,
-							 typename Elem<E>::T *__restrict__ C, size_t ldc,
-							 int m, int n, int k, uint32_t a, uint32_t b, ModP mp,
-							 uint64_t two_64)
+__device__ __forceinline__ void plain_tile(const typename Elem<E>::T *__restrict__ A, size_t lda,
+					   const typename Elem<E>::T *__restrict__ B, size_t ldb,
+					   typename Elem<E>::T *__restrict__ C, size_t ldc, int m, int n, int k,
+					   uint32_t a, uint32_t b, const ModP &mp, uint64_t two_64, int row0, int col0,
+					   uint32_t (*As)[64], uint32_t (*Bs)[64])
 {
-	__shared__ __attribute__((aligned(16))) uint32_t As[16][64];
-	__shared__ __attribute__((aligned(16))) uint32_t Bs[16][64];
 	const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-	const int row0 = blockIdx.y * 64, col0 = blockIdx.x * 64;
 	const uint32_t p = (uint32_t) mp.p;
 	uint64_t acc[4][4];
 #pragma unroll
@@ -457,6 +473,21 @@ __global__ __launch_bounds__(256) void gemm_plain_kernel(const typename Elem<E>:
 			Elem<E>::store(cp, mp.axpy(a, d, cij), p);
 		}
 	}
+}
+
+// 64 x 64 output tile, 16 x 16 threads, 4 x 4 entries per thread, k staged
+// 16 at a time through LDS as canonical uint32.
+template <int E, bool WIDE>
+__global__ __launch_bounds__(256) void gemm_plain_kernel(const typename Elem<E>::T *__restrict__ A, size_t lda,
+							 const typename Elem<E>::T *__restrict__ B, size_t ldb,
+							 typename Elem<E>::T *__restrict__ C, size_t ldc,
+							 int m, int n, int k, uint32_t a, uint32_t b, ModP mp,
+							 uint64_t two_64)
+{
+	__shared__ __attribute__((aligned(16))) uint32_t As[16][64];
+	__shared__ __attribute__((aligned(16))) uint32_t Bs[16][64];
+	plain_tile<E, WIDE>(A, lda, B, ldb, C, ldc, m, n, k, a, b, mp, two_64, blockIdx.y * 64, blockIdx.x * 64, As,
+			    Bs);
 }
 
 template <int E>
@@ -713,7 +744,7 @@ static int gemm_dispatch(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k,
 	}
 }
 
-static int check_modulus(mlz_ctx *ctx, int elem, uint64_t p)
+int check_modulus(mlz_ctx *ctx, int elem, uint64_t p)
 {
 	if (p < 2)
 		return set_error(ctx, MLZ_E_ARG, "modulus %llu < 2", (unsigned long long) p);
@@ -752,6 +783,261 @@ int zp_gemm_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, size_t k
 	if (elem == MLZ_U32)
 		return gemm_dispatch<MLZ_U32>(ctx, pp, m, n, k, aa, A, lda, B, ldb, bb, C, ldc, algo);
 	return gemm_dispatch<MLZ_F64_SYM>(ctx, pp, m, n, k, aa, A, lda, B, ldb, bb, C, ldc, algo);
+}
+
+// ===========================================================================
+// Batched kernels: persistent workgroups take work items (one output tile of
+// one instance, or one 64 x 64 pack tile) from an atomic counter until the
+// list is exhausted.  The lists are ordered largest instance first, so the
+// tail of the launch holds the small tiles.  With L.tcost set, every item adds
+// its workgroup's duration (s_memrealtime, 100 MHz) to its instance's total.
+// ===========================================================================
+
+// Pack items: x = 0 (A) / 1 (B) | 2 (16-byte aligned rows), y = 64-index block,
+// z = 64-k block.
+template <int E>
+__global__ __launch_bounds__(256) void batch_pack_i8_kernel(const BatchGemm *__restrict__ tasks, BatchLaunch L,
+							    uint32_t p)
+{
+	typedef typename Elem<E>::T T;
+	__shared__ uint32_t tile[64][65];
+	__shared__ int slot;
+	for (;;) {
+		const int it = batch_next(L, &slot);
+		if (it < 0)
+			break;
+		const BatchItem w = L.items[it];
+		const BatchGemm &t = tasks[w.task];
+		const uint64_t t0 = wall_clock64();
+		const bool isB = w.x & 1, vec = w.x & 2;
+		if (!isB) {
+			if (vec)
+				i8_pack_tile<E, false, true>((const T *) t.A, t.lda, t.m, t.k, p, t.Ap, t.KB, w.y * 64, w.z * 64, tile);
+			else
+				i8_pack_tile<E, false, false>((const T *) t.A, t.lda, t.m, t.k, p, t.Ap, t.KB, w.y * 64, w.z * 64, tile);
+		} else {
+			if (vec)
+				i8_pack_tile<E, true, true>((const T *) t.B, t.ldb, t.n, t.k, p, t.Bp, t.KB, w.y * 64, w.z * 64, tile);
+			else
+				i8_pack_tile<E, true, false>((const T *) t.B, t.ldb, t.n, t.k, p, t.Bp, t.KB, w.y * 64, w.z * 64, tile);
+		}
+		__syncthreads();
+		batch_charge(L, w.task, t0);
+	}
+}
+
+// INT8 tile items: x = tile row, y = tile column.
+template <int E, int WTM, int WTN, int KK, int OCC>
+__global__ __launch_bounds__(256, OCC) void batch_i8_kernel(const BatchGemm *__restrict__ tasks, BatchLaunch L,
+							    ModP mp, uint64_t off)
+{
+	typedef typename Elem<E>::T T;
+	__shared__ __attribute__((aligned(16))) int8_t lds[2 * I8Tile<WTM, WTN, KK>::STAGE];
+	__shared__ int slot;
+	for (;;) {
+		const int it = batch_next(L, &slot);
+		if (it < 0)
+			break;
+		const BatchItem w = L.items[it];
+		const BatchGemm &t = tasks[w.task];
+		const uint64_t t0 = wall_clock64();
+		i8_tile<E, WTM, WTN, KK, 2, 2>(t.Ap, t.Bp, t.KB, (T *) t.C, t.ldc, t.m, t.n, t.a, t.b, mp, off, w.x, w.y,
+					       lds);
+		__syncthreads();
+		batch_charge(L, w.task, t0);
+	}
+}
+
+// Plain tile items: x = 64-row block, y = 64-column block.
+template <int E, bool WIDE>
+__global__ __launch_bounds__(256) void batch_plain_kernel(const BatchGemm *__restrict__ tasks, BatchLaunch L, ModP mp,
+							  uint64_t two_64)
+{
+	typedef typename Elem<E>::T T;
+	__shared__ __attribute__((aligned(16))) uint32_t As[16][64];
+	__shared__ __attribute__((aligned(16))) uint32_t Bs[16][64];
+	__shared__ int slot;
+	for (;;) {
+		const int it = batch_next(L, &slot);
+		if (it < 0)
+			break;
+		const BatchItem w = L.items[it];
+		const BatchGemm &t = tasks[w.task];
+		const uint64_t t0 = wall_clock64();
+		plain_tile<E, WIDE>((const T *) t.A, t.lda, (const T *) t.B, t.ldb, (T *) t.C, t.ldc, t.m, t.n, t.k, t.a,
+				    t.b, mp, two_64, w.x * 64, w.y * 64, As, Bs);
+		__syncthreads();
+		batch_charge(L, w.task, t0);
+	}
+}
+
+// Sparse-A tile items: x = 16-row block, y = 256-column block.  Wave w takes
+// rows 4w..4w+3 of the block; for each row it reads A 64 entries at a time,
+// and for every nonzero a_il (found by ballot, the loop is wave-uniform) adds
+// a_il * B[l][cols] to the lane's four columns (lane, lane+64, +128, +192).
+// Work per row is proportional to its nonzeros, not to k.
+template <int E, bool WIDE>
+__global__ __launch_bounds__(256) void batch_sparse_kernel(const BatchGemm *__restrict__ tasks, BatchLaunch L,
+							   ModP mp, uint64_t two_64)
+{
+	typedef typename Elem<E>::T T;
+	__shared__ int slot;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const uint32_t p = (uint32_t) mp.p;
+	for (;;) {
+		const int it = batch_next(L, &slot);
+		if (it < 0)
+			break;
+		const BatchItem w = L.items[it];
+		const BatchGemm &t = tasks[w.task];
+		const uint64_t t0 = wall_clock64();
+		const T *A = (const T *) t.A, *B = (const T *) t.B;
+		T *C = (T *) t.C;
+		const int col0 = w.y * 256 + lane;
+		for (int rr = 0; rr < 4; ++rr) {
+			const int row = w.x * 16 + wave * 4 + rr;
+			if (row >= t.m)
+				break;
+			uint64_t acc[4] = {0, 0, 0, 0};
+			const T *arow = A + (size_t) row * t.lda;
+			for (int kc = 0; kc < t.k; kc += 64) {
+				const int l = kc + lane;
+				const uint32_t x = l < t.k ? Elem<E>::load(arow + l, p) : 0u;
+				uint64_t mask = __ballot(x != 0);
+				while (mask) {
+					const int s = __ffsll((unsigned long long) mask) - 1;
+					mask &= mask - 1;
+					const uint32_t xv = (uint32_t) __builtin_amdgcn_readlane((int) x, s);
+					const T *brow = B + (size_t) (kc + s) * t.ldb;
+#pragma unroll
+					for (int q = 0; q < 4; ++q) {
+						const int col = col0 + 64 * q;
+						if (col < t.n) {
+							const uint64_t pr = (uint64_t) xv * Elem<E>::load(brow + col, p);
+							acc[q] += pr;
+							if (WIDE && acc[q] < pr)
+								acc[q] += two_64;
+						}
+					}
+				}
+			}
+#pragma unroll
+			for (int q = 0; q < 4; ++q) {
+				const int col = col0 + 64 * q;
+				if (col < t.n) {
+					const uint32_t d = (uint32_t) mp.reduce(acc[q]);
+					T *cp = C + (size_t) row * t.ldc + col;
+					const uint32_t cij = t.b ? mp.mul(Elem<E>::load(cp, p), t.b) : 0u;
+					Elem<E>::store(cp, mp.axpy(t.a, d, cij), p);
+				}
+			}
+		}
+		__syncthreads();
+		batch_charge(L, w.task, t0);
+	}
+}
+
+static uint64_t two64_mod(uint32_t p)
+{
+	uint64_t t = 2;
+	for (int i = 0; i < 6; ++i)
+		t = (t * t) % p;
+	return t;
+}
+
+static bool plain_wide(uint32_t p, uint64_t kmax)
+{
+	return (unsigned __int128) (p - 1) * (p - 1) * (unsigned __int128) kmax >= ((unsigned __int128) 1 << 64);
+}
+
+template <int E>
+static int batch_gemm_launch_t(hipStream_t stream, int variant, uint32_t p, uint64_t kmax, const BatchGemm *tasks,
+			       BatchLaunch L, int grid)
+{
+	const ModP mp = make_modp(p);
+	const uint64_t off = ((((uint64_t) 1 << 48) + p - 1) / p) * p;
+	const uint64_t t64 = two64_mod(p);
+	const bool wide = plain_wide(p, kmax);
+	switch (variant) {
+	case BG_PLAIN:
+		if (wide)
+			batch_plain_kernel<E, true><<<grid, 256, 0, stream>>>(tasks, L, mp, t64);
+		else
+			batch_plain_kernel<E, false><<<grid, 256, 0, stream>>>(tasks, L, mp, t64);
+		break;
+	case BG_SPARSE:
+		if (wide)
+			batch_sparse_kernel<E, true><<<grid, 256, 0, stream>>>(tasks, L, mp, t64);
+		else
+			batch_sparse_kernel<E, false><<<grid, 256, 0, stream>>>(tasks, L, mp, t64);
+		break;
+	case BG_I8S:
+		batch_i8_kernel<E, 1, 1, 2, 4><<<grid, 256, 0, stream>>>(tasks, L, mp, off);
+		break;
+	case BG_I8L:
+		batch_i8_kernel<E, 2, 2, 1, 2><<<grid, 256, 0, stream>>>(tasks, L, mp, off);
+		break;
+	default:
+		return MLZ_E_ARG;
+	}
+	return hipGetLastError() == hipSuccess ? MLZ_OK : MLZ_E_HIP;
+}
+
+int batch_gemm_launch(hipStream_t stream, int elem, int variant, uint32_t p, uint64_t kmax, const BatchGemm *tasks,
+		      BatchLaunch L, int grid)
+{
+	if (elem == MLZ_U16)
+		return batch_gemm_launch_t<MLZ_U16>(stream, variant, p, kmax, tasks, L, grid);
+	if (elem == MLZ_U32)
+		return batch_gemm_launch_t<MLZ_U32>(stream, variant, p, kmax, tasks, L, grid);
+	return batch_gemm_launch_t<MLZ_F64_SYM>(stream, variant, p, kmax, tasks, L, grid);
+}
+
+int batch_pack_i8_launch(hipStream_t stream, int elem, uint32_t p, const BatchGemm *tasks, BatchLaunch L, int grid)
+{
+	if (elem == MLZ_U16)
+		batch_pack_i8_kernel<MLZ_U16><<<grid, 256, 0, stream>>>(tasks, L, p);
+	else if (elem == MLZ_U32)
+		batch_pack_i8_kernel<MLZ_U32><<<grid, 256, 0, stream>>>(tasks, L, p);
+	else
+		batch_pack_i8_kernel<MLZ_F64_SYM><<<grid, 256, 0, stream>>>(tasks, L, p);
+	return hipGetLastError() == hipSuccess ? MLZ_OK : MLZ_E_HIP;
+}
+
+template <typename K>
+static int occ_of(K kernel, int threads)
+{
+	int n = 0;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, threads, 0) != hipSuccess || n < 1)
+		n = 1;
+	return n;
+}
+
+template <int E>
+static int batch_gemm_occ(int variant, uint32_t p, uint64_t kmax)
+{
+	const bool wide = plain_wide(p, kmax);
+	switch (variant) {
+	case BG_PLAIN:
+		return wide ? occ_of(batch_plain_kernel<E, true>, 256) : occ_of(batch_plain_kernel<E, false>, 256);
+	case BG_SPARSE:
+		return wide ? occ_of(batch_sparse_kernel<E, true>, 256) : occ_of(batch_sparse_kernel<E, false>, 256);
+	case BG_I8S:
+		return occ_of(batch_i8_kernel<E, 1, 1, 2, 4>, 256);
+	case BG_I8L:
+		return occ_of(batch_i8_kernel<E, 2, 2, 1, 2>, 256);
+	default:
+		return occ_of(batch_pack_i8_kernel<E>, 256);
+	}
+}
+
+int batch_gemm_wgs_per_cu(int elem, int variant, uint32_t p, uint64_t kmax)
+{
+	if (elem == MLZ_U16)
+		return batch_gemm_occ<MLZ_U16>(variant, p, kmax);
+	if (elem == MLZ_U32)
+		return batch_gemm_occ<MLZ_U32>(variant, p, kmax);
+	return batch_gemm_occ<MLZ_F64_SYM>(variant, p, kmax);
 }
 
 }  // namespace mlz

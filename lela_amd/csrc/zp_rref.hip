@@ -30,9 +30,10 @@
 // Rank, pivot rows and pivot columns stay on the device; the host only checks
 // the rank every few panels to stop early.  A final kernel writes the pivot
 // rows in pivot-column order followed by zero rows (standard layout).
-#include "common.hpp"
+#include "batch.hpp"
 
 #include <climits>
+#include <type_traits>
 
 namespace mlz {
 
@@ -588,6 +589,381 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 	if (rank_out)
 		*rank_out = (size_t) r;
 	return MLZ_OK;
+}
+
+// ===========================================================================
+// Batched echelon forms: one workgroup of BR_THREADS threads per instance.
+//
+// Instances of a batch are small (up to BR_MAXM rows) and many, so each one
+// runs start to finish inside one workgroup taken from the work list, with
+// all synchronisation by barriers.  Per panel of W columns:
+//  1. pivot search: the rows that are not pivots are visited in order; each
+//     round eliminates the current basis of the panel (at most W rows, kept
+//     first so they stay pivots) together with the next live rows, up to
+//     BR_CH rows, in registers (reg_eliminate), and keeps the pivot rows.  It
+//     stops at full panel rank or when every row has been seen; the basis then
+//     spans the panel's live rows, so its pivot columns J are their rank
+//     profile.
+//  2. P_J^-1 from [P_J | I] in registers, then Wb = P_J^-1 * P over columns
+//     c..n-1 (identity on J);
+//  3. every other row i, 64-row block at a time: its multipliers A_i[J] go to
+//     LDS, then A_i <- A_i - A_i[J] * Wb; finally Wb is written into the pivot
+//     rows.
+// Products are summed exactly before one reduction per entry: in double
+// precision when W*(p-1)^2 < 2^53 (FMA, full-rate FP64), else in uint64 with
+// the 2^64-wrap correction.
+// ===========================================================================
+constexpr int BR_THREADS = 1024;
+constexpr int BR_CH = 128;   // rows per pivot-search elimination
+constexpr int BR_RB = 64;    // rows per update block (multipliers in LDS)
+constexpr int BR_KC = 8;     // multipliers per register chunk
+constexpr int BR_RQ = 8;     // rows per update unit of one wave
+
+template <int W, bool FMA>
+struct BrLds {
+	typedef typename std::conditional<FMA, double, uint32_t>::type GT;
+	int rowid[BR_CH], alive[BR_CH], tmp[BR_CH];
+	uint32_t f[BR_CH];
+	uint32_t pivrow[2 * W];
+	int outr[W], outc[W], J[W], pos[W];
+	long long srcoff[W];
+	int s_min[2];
+	int slot, nr, cursor;
+	uint8_t is_piv[BR_MAXM];     // 0 live, 1 pivot of an earlier panel, 2 of this panel
+	uint8_t rownz[BR_RB];
+	uint32_t Pinv[W][W];
+	GT G[BR_RB][W];
+};
+
+// x (an exact integer below 2^53) mod p for p < 2^26.
+__device__ __forceinline__ uint32_t br_mod_f64(double x, const ModP &mp)
+{
+	const double pd = (double) mp.p;
+	const double q = floor(x * mp.pinv);
+	double r = fma(-q, pd, x);
+	r = r < 0.0 ? r + pd : r;
+	r = r >= pd ? r - pd : r;
+	return (uint32_t) r;
+}
+
+// out[r * ld + j] (SUB: -)= sum_{s < k} G[r][s] * src[srcoff[s] + j] mod p for
+// rows r < nrows with rownz[r] set and columns j < ncols.  Units of BR_RQ rows
+// x 64 columns are spread over the waves; a lane owns one column and keeps
+// BR_KC source values in registers while the multipliers (uniform across the
+// wave) are read from LDS.
+template <int W, bool FMA, bool SUB>
+__device__ __forceinline__ void br_apply(BrLds<W, FMA> &S, int k, int nrows, int ncols, const uint32_t *src,
+					 uint32_t *out, size_t ld, const ModP &mp, uint64_t two_64)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int nchunks = (ncols + 63) >> 6, nq = (nrows + BR_RQ - 1) / BR_RQ;
+	const uint32_t p = (uint32_t) mp.p;
+	for (int u = wave; u < nchunks * nq; u += BR_THREADS / 64) {
+		const int rq = u / nchunks, jc = u - rq * nchunks;
+		const int j = jc * 64 + lane, r0 = rq * BR_RQ;
+		const bool colok = j < ncols;
+		if (FMA) {
+			double acc[BR_RQ];
+#pragma unroll
+			for (int r = 0; r < BR_RQ; ++r)
+				acc[r] = 0.0;
+			for (int s0 = 0; s0 < k; s0 += BR_KC) {
+				double wv[BR_KC];
+#pragma unroll
+				for (int q = 0; q < BR_KC; ++q)
+					wv[q] = (s0 + q < k && colok) ? (double) src[S.srcoff[s0 + q] + j] : 0.0;
+#pragma unroll
+				for (int r = 0; r < BR_RQ; ++r) {
+					if (r0 + r < nrows && S.rownz[r0 + r]) {
+						const double *g = (const double *) &S.G[r0 + r][s0];
+#pragma unroll
+						for (int q = 0; q < BR_KC; ++q)
+							acc[r] = fma(g[q], wv[q], acc[r]);
+					}
+				}
+			}
+#pragma unroll
+			for (int r = 0; r < BR_RQ; ++r) {
+				if (r0 + r < nrows && S.rownz[r0 + r] && colok) {
+					uint32_t *o = out + (size_t) (r0 + r) * ld + j;
+					const uint32_t x = br_mod_f64(acc[r], mp);
+					*o = SUB ? mp.sub(*o, x) : x;
+				}
+			}
+		} else {
+			uint64_t acc[BR_RQ];
+#pragma unroll
+			for (int r = 0; r < BR_RQ; ++r)
+				acc[r] = 0;
+			for (int s0 = 0; s0 < k; s0 += BR_KC) {
+				uint32_t wv[BR_KC];
+#pragma unroll
+				for (int q = 0; q < BR_KC; ++q)
+					wv[q] = (s0 + q < k && colok) ? src[S.srcoff[s0 + q] + j] : 0u;
+#pragma unroll
+				for (int r = 0; r < BR_RQ; ++r) {
+					if (r0 + r < nrows && S.rownz[r0 + r]) {
+						const uint32_t *g = (const uint32_t *) &S.G[r0 + r][s0];
+#pragma unroll
+						for (int q = 0; q < BR_KC; ++q) {
+							const uint64_t t = (uint64_t) g[q] * wv[q];
+							acc[r] += t;
+							if (acc[r] < t)
+								acc[r] += two_64;
+						}
+					}
+				}
+			}
+#pragma unroll
+			for (int r = 0; r < BR_RQ; ++r) {
+				if (r0 + r < nrows && S.rownz[r0 + r] && colok) {
+					uint32_t *o = out + (size_t) (r0 + r) * ld + j;
+					const uint32_t x = (uint32_t) mp.reduce(acc[r]);
+					*o = SUB ? mp.sub(*o, x) : x;
+				}
+			}
+		}
+		(void) p;
+	}
+}
+
+template <int E, int W, bool SMALL, bool FMA>
+__global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref *__restrict__ tasks, BatchLaunch L,
+								ModP mp, uint64_t two_64, int *__restrict__ ranks,
+								int *__restrict__ bad)
+{
+	typedef typename Elem<E>::T T;
+	typedef typename BrLds<W, FMA>::GT GT;
+	constexpr int RPG = BR_CH / (BR_THREADS / W);             // pivot search: rows per thread
+	constexpr int NCI = 2 * W, RPGI = W / (BR_THREADS / NCI);  // inverse: rows per thread
+	__shared__ BrLds<W, FMA> S;
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const uint32_t p = (uint32_t) mp.p;
+	for (;;) {
+		const int it = batch_next(L, &S.slot);
+		if (it < 0)
+			break;
+		const BatchItem w = L.items[it];
+		const BatchRref &t = tasks[w.task];
+		const uint64_t t0 = wall_clock64();
+		const int m = t.m, n = t.n;
+		uint32_t *Wm = t.Wm, *Wb = t.Wb;
+		int *badp = bad + w.task;
+		for (int r = wave; r < m; r += BR_THREADS / 64)
+			for (int c = lane; c < n; c += 64)
+				Wm[(size_t) r * n + c] = Elem<E>::load((const T *) t.A + (size_t) r * t.lda + c, p);
+		for (int r = tid; r < m; r += BR_THREADS)
+			S.is_piv[r] = 0;
+		__syncthreads();
+		int rank = 0;
+		for (int c = 0; c < n && rank < m; c += W) {
+			const int wc = min(W, n - c), nc = n - c;
+			// ---- 1. pivot search
+			int cnt = 0;
+			for (bool first = true;; first = false) {
+				if (wave == 0) {
+					int have = cnt, cur = first ? 0 : S.cursor;
+					while (have < BR_CH && cur < m) {
+						const int row = cur + lane;
+						const bool live = row < m && S.is_piv[row] == 0;
+						const uint64_t bal = __ballot(live);
+						const int before = __popcll(bal & ((1ull << lane) - 1ull));
+						const int room = BR_CH - have, tot = __popcll(bal);
+						if (live && before < room)
+							S.rowid[have + before] = row;
+						if (tot <= room) {
+							have += tot;
+							cur += 64;
+						} else {
+							uint64_t b = bal;
+							for (int q = 0; q < room - 1; ++q)
+								b &= b - 1;
+							have = BR_CH;
+							cur += __ffsll((unsigned long long) b);
+						}
+					}
+					if (lane == 0) {
+						S.nr = have;
+						S.cursor = cur;
+					}
+				}
+				__syncthreads();
+				const int nr = S.nr;
+				const bool seen_all = S.cursor >= m;
+				for (int r = tid; r < BR_CH; r += BR_THREADS)
+					S.alive[r] = r < nr;
+				{
+					const int cc = tid % W, g = tid / W;
+					uint32_t v[RPG];
+#pragma unroll
+					for (int i = 0; i < RPG; ++i) {
+						const int r = g * RPG + i;
+						v[i] = (r < nr && cc < wc) ? Wm[(size_t) S.rowid[r] * n + c + cc] : 0u;
+					}
+					cnt = reg_eliminate<W, RPG, SMALL>(v, nr, wc, false, mp, S.alive, S.f, S.pivrow, S.s_min,
+									   S.outr, S.outc, badp);
+				}
+				for (int s = tid; s < cnt; s += BR_THREADS)
+					S.tmp[s] = S.rowid[S.outr[s]];
+				__syncthreads();
+				for (int s = tid; s < cnt; s += BR_THREADS) {
+					S.rowid[s] = S.tmp[s];
+					S.J[s] = S.outc[s];
+				}
+				__syncthreads();
+				if (cnt == wc || seen_all)
+					break;
+			}
+			const int k = cnt;
+			if (k == 0)
+				continue;
+			// ---- 2. P_J^-1, then Wb = P_J^-1 * P
+			{
+				const int ci = tid % NCI, gi = tid / NCI;
+				uint32_t v[RPGI];
+#pragma unroll
+				for (int i = 0; i < RPGI; ++i) {
+					const int r = gi * RPGI + i;
+					uint32_t x;
+					if (ci < W)
+						x = (r < k && ci < k) ? Wm[(size_t) S.rowid[r] * n + c + S.J[ci]] : (r == ci ? 1u : 0u);
+					else
+						x = (ci - W == r) ? 1u : 0u;
+					v[i] = x;
+				}
+				for (int r = tid; r < W; r += BR_THREADS) {
+					S.alive[r] = 1;
+					S.pos[r] = r;
+				}
+				const int ki = reg_eliminate<NCI, RPGI, SMALL>(v, W, k, true, mp, S.alive, S.f, S.pivrow, S.s_min,
+									       S.outr, S.outc, badp);
+				if (tid == 0 && ki < k)
+					*badp = 1;
+				for (int s = tid; s < ki; s += BR_THREADS)
+					S.pos[S.outr[s]] = S.outc[s];
+				__syncthreads();
+				if (ci >= W) {
+#pragma unroll
+					for (int i = 0; i < RPGI; ++i)
+						S.Pinv[S.pos[gi * RPGI + i]][ci - W] = v[i];
+				}
+			}
+			for (int s = tid; s < k; s += BR_THREADS) {
+				S.srcoff[s] = (long long) S.rowid[s] * n + c;
+				S.is_piv[S.rowid[s]] = 2;
+				t.piv_rows[rank + s] = S.rowid[s];
+				t.piv_cols[rank + s] = c + S.J[s];
+			}
+			for (int r = tid; r < BR_RB; r += BR_THREADS)
+				S.rownz[r] = 1;
+			__syncthreads();
+			for (int e = tid; e < k * W; e += BR_THREADS) {
+				const int r = e / W, s = e % W;
+				S.G[r][s] = (GT) (s < k ? S.Pinv[r][s] : 0u);
+			}
+			__syncthreads();
+			br_apply<W, FMA, false>(S, k, k, nc, Wm, Wb, n, mp, two_64);
+			__syncthreads();
+			// ---- 3. eliminate column set J from every other row
+			for (int s = tid; s < k; s += BR_THREADS)
+				S.srcoff[s] = (long long) s * n;
+			for (int i0 = 0; i0 < m; i0 += BR_RB) {
+				const int nb = min(BR_RB, m - i0);
+				for (int e = tid; e < nb * W; e += BR_THREADS) {
+					const int r = e / W, s = e % W;
+					const int i = i0 + r;
+					uint32_t g = 0;
+					if (s < k && S.is_piv[i] != 2)
+						g = Wm[(size_t) i * n + c + S.J[s]];
+					S.G[r][s] = (GT) g;
+				}
+				__syncthreads();
+				// rownz[r]: some multiplier of row i0 + r is nonzero (a wave per row)
+				for (int r = wave; r < nb; r += BR_THREADS / 64) {
+					const bool nz = lane < k && S.G[r][lane] != (GT) 0;
+					const uint64_t bal = __ballot(nz);
+					if (lane == 0)
+						S.rownz[r] = bal != 0;
+				}
+				__syncthreads();
+				br_apply<W, FMA, true>(S, k, nb, nc, Wb, Wm + (size_t) i0 * n + c, n, mp, two_64);
+				__syncthreads();
+			}
+			for (int e = tid; e < k * nc; e += BR_THREADS) {
+				const int s = e / nc, j = e % nc;
+				Wm[(size_t) S.rowid[s] * n + c + j] = Wb[(size_t) s * n + j];
+			}
+			for (int s = tid; s < k; s += BR_THREADS)
+				S.is_piv[S.rowid[s]] = 1;
+			for (int r = tid; r < BR_RB; r += BR_THREADS)
+				S.rownz[r] = 1;
+			rank += k;
+			__syncthreads();
+		}
+		// ---- standard layout: pivot rows in pivot-column order, then zero rows
+		for (int r = wave; r < m; r += BR_THREADS / 64) {
+			const int src = r < rank ? t.piv_rows[r] : -1;
+			for (int c = lane; c < n; c += 64) {
+				const uint32_t x = src >= 0 ? Wm[(size_t) src * n + c] : 0u;
+				Elem<E>::store((T *) t.A + (size_t) r * t.lda + c, x, p);
+			}
+		}
+		if (t.pivots)
+			for (int s = tid; s < rank; s += BR_THREADS)
+				t.pivots[s] = (uint32_t) t.piv_cols[s];
+		if (tid == 0)
+			ranks[w.task] = rank;
+		__syncthreads();
+		batch_charge(L, w.task, t0);
+	}
+}
+
+template <int E, int W>
+static void batch_rref_launch_w(hipStream_t stream, const ModP &mp, uint64_t two_64, const BatchRref *tasks,
+				int *ranks, int *bad, BatchLaunch L, int grid)
+{
+	const bool small = mp.p < (1ull << 26);
+	const bool fma = (unsigned __int128) W * (mp.p - 1) * (mp.p - 1) < ((unsigned __int128) 1 << 53);
+	if (fma)
+		batch_rref_kernel<E, W, true, true><<<grid, BR_THREADS, 0, stream>>>(tasks, L, mp, two_64, ranks, bad);
+	else if (small)
+		batch_rref_kernel<E, W, true, false><<<grid, BR_THREADS, 0, stream>>>(tasks, L, mp, two_64, ranks, bad);
+	else
+		batch_rref_kernel<E, W, false, false><<<grid, BR_THREADS, 0, stream>>>(tasks, L, mp, two_64, ranks, bad);
+}
+
+template <int E>
+static void batch_rref_launch_e(hipStream_t stream, int variant, const ModP &mp, uint64_t two_64,
+				const BatchRref *tasks, int *ranks, int *bad, BatchLaunch L, int grid)
+{
+	if (variant == BR_W32)
+		batch_rref_launch_w<E, 32>(stream, mp, two_64, tasks, ranks, bad, L, grid);
+	else
+		batch_rref_launch_w<E, 64>(stream, mp, two_64, tasks, ranks, bad, L, grid);
+}
+
+int batch_rref_launch(hipStream_t stream, int elem, int variant, uint32_t p, const BatchRref *tasks, int *ranks,
+		      int *bad, BatchLaunch L, int grid)
+{
+	const ModP mp = make_modp(p);
+	uint64_t two_64 = 2;
+	for (int i = 0; i < 6; ++i)
+		two_64 = (two_64 * two_64) % p;
+	if (elem == MLZ_U16)
+		batch_rref_launch_e<MLZ_U16>(stream, variant, mp, two_64, tasks, ranks, bad, L, grid);
+	else if (elem == MLZ_U32)
+		batch_rref_launch_e<MLZ_U32>(stream, variant, mp, two_64, tasks, ranks, bad, L, grid);
+	else
+		batch_rref_launch_e<MLZ_F64_SYM>(stream, variant, mp, two_64, tasks, ranks, bad, L, grid);
+	return hipGetLastError() == hipSuccess ? MLZ_OK : MLZ_E_HIP;
+}
+
+int batch_rref_wgs_per_cu(int elem, int variant, uint32_t p)
+{
+	(void) elem;
+	(void) variant;
+	(void) p;
+	return 1;   // 1024-thread workgroups: one per CU
 }
 
 }  // namespace mlz

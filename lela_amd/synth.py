@@ -132,3 +132,34 @@ def gf2_matrix(seed, rows, cols):
     if r:
         W[:, -1] &= np.uint64((1 << r) - 1)
     return W
+
+
+def batch_instances(seed, count=4096, p=65521, lo=32, hi=1024, dtype=np.uint16, sparse_lo=0.01,
+                    sparse_hi=0.20):
+    """Uneven batch of independent Z/p instances (as dicts).
+
+    Instance i has m, n, k uniform in [lo, hi].  i % 4 in {0, 1}: gemm
+    C <- A*B + C (A m x k, B k x n, C m x n uniform); otherwise the reduced
+    echelon form of an m x n matrix A.  Odd i are sparse: entries of A (and B)
+    nonzero with probability d uniform in [sparse_lo, sparse_hi], values
+    uniform in [1, p); even i are dense uniform residues (d = 1)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        m, n, k = (int(v) for v in rng.integers(lo, hi + 1, size=3))
+        sparse = i % 2 == 1
+        d = float(rng.uniform(sparse_lo, sparse_hi)) if sparse else 1.0
+
+        def mat(r, c):
+            if not sparse:
+                return rng.integers(0, p, size=(r, c), dtype=np.int64).astype(dtype)
+            mask = rng.random((r, c)) < d
+            vals = rng.integers(1, p, size=(r, c), dtype=np.int64)
+            return np.where(mask, vals, 0).astype(dtype)
+
+        if i % 4 in (0, 1):
+            out.append(dict(op=1, m=m, n=n, k=k, density=d, A=mat(m, k), B=mat(k, n),
+                            C=rng.integers(0, p, size=(m, n), dtype=np.int64).astype(dtype)))
+        else:
+            out.append(dict(op=2, m=m, n=n, k=min(m, n), density=d, A=mat(m, n)))
+    return out

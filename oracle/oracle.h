@@ -82,6 +82,19 @@ int orc_fl_echelonize(uint32_t p, size_t m, size_t n, const uint32_t *X, size_t 
  * Returns 0 when gcd(y, p) != 1. */
 int orc_inv(uint32_t p, uint32_t y, uint32_t *x);
 
+/* ---- batch cost model (no LELA counterpart: restated from its definition) ---
+ * MLP 5 -> 32 -> 32 -> 4 with ReLU; params laid out W1[5][32] b1[32]
+ * W2[32][32] b2[32] W3[32][4] b3[4] (1380 floats).  Arithmetic in double.
+ * Loss: masked mean squared error (1/M) sum mask*(y - t)^2, M = sum mask. */
+#define ORC_MLP_PARAMS 1380
+int orc_mlp_forward(const float *params, const float *x, size_t n, double *y);
+int orc_mlp_loss_grad(const float *params, const float *x, const float *t, const uint8_t *mask, size_t n,
+		      double *grad, double *loss);
+/* `steps` full-batch Adam steps (beta1 0.9, beta2 0.999, eps 1e-8) from params
+ * (updated in place, kept in double inside); loss_hist[s] = loss before step s. */
+int orc_mlp_adam(float *params, const float *x, const float *t, const uint8_t *mask, size_t n, int steps,
+		 double lr, double *loss_hist);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,8 +37,12 @@ def load():
     lib.orc_fl_echelonize.argtypes = [u32, _sz, _sz, _vp, _sz, _vp, _sz, ctypes.POINTER(_sz),
                                       ctypes.POINTER(u32), ctypes.POINTER(_sz)]
     lib.orc_inv.argtypes = [u32, u32, ctypes.POINTER(u32)]
+    lib.orc_mlp_forward.argtypes = [_vp, _vp, _sz, _vp]
+    lib.orc_mlp_loss_grad.argtypes = [_vp, _vp, _vp, _vp, _sz, _vp, ctypes.POINTER(ctypes.c_double)]
+    lib.orc_mlp_adam.argtypes = [_vp, _vp, _vp, _vp, _sz, i, ctypes.c_double, _vp]
     for name in ("orc_zp_gemm_u32", "orc_zp_gemm_u16", "orc_zp_trsm", "orc_zp_echelonize",
-                 "orc_gf2_gemm", "orc_gf2_echelonize", "orc_fl_echelonize", "orc_inv"):
+                 "orc_gf2_gemm", "orc_gf2_echelonize", "orc_fl_echelonize", "orc_inv",
+                 "orc_mlp_forward", "orc_mlp_loss_grad", "orc_mlp_adam"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -123,3 +127,38 @@ def inv(p, y):
     x = ctypes.c_uint32(0)
     ok = lib.orc_inv(p, y, ctypes.byref(x))
     return x.value if ok else None
+
+
+MLP_PARAMS = 1380
+
+
+def mlp_forward(params, x):
+    """Predictions [n, 4] of the cost model (double precision)."""
+    lib = load()
+    params, x = _c(params, np.float32), _c(x, np.float32)
+    y = np.zeros((x.shape[0], 4), dtype=np.float64)
+    assert lib.orc_mlp_forward(params.ctypes.data, x.ctypes.data, x.shape[0], y.ctypes.data) == ORC_OK
+    return y
+
+
+def mlp_loss_grad(params, x, t, mask):
+    """(loss, gradient[1380]) of the masked mean squared error."""
+    lib = load()
+    params, x, t = _c(params, np.float32), _c(x, np.float32), _c(t, np.float32)
+    mask = _c(mask, np.uint8)
+    g = np.zeros(MLP_PARAMS, dtype=np.float64)
+    loss = ctypes.c_double(0.0)
+    assert lib.orc_mlp_loss_grad(params.ctypes.data, x.ctypes.data, t.ctypes.data, mask.ctypes.data,
+                                 x.shape[0], g.ctypes.data, ctypes.byref(loss)) == ORC_OK
+    return loss.value, g
+
+
+def mlp_adam(params, x, t, mask, steps, lr):
+    """(params after `steps` Adam steps, loss before each step)."""
+    lib = load()
+    w = _c(params, np.float32).copy()
+    x, t, mask = _c(x, np.float32), _c(t, np.float32), _c(mask, np.uint8)
+    hist = np.zeros(max(1, steps), dtype=np.float64)
+    assert lib.orc_mlp_adam(w.ctypes.data, x.ctypes.data, t.ctypes.data, mask.ctypes.data, x.shape[0],
+                            int(steps), float(lr), hist.ctypes.data) == ORC_OK
+    return w, hist[:steps]

@@ -52,3 +52,24 @@ def test_no_gpu_context_fails_loudly():
         return                      # no GPU: a clear error, no fallback
     ctx.close()
     pytest.skip("GPU present")
+
+
+def test_task_struct_layout_matches_header(tmp_path):
+    """struct mlz_task as gcc lays it out == the ctypes Task of the binding."""
+    import shutil
+    import subprocess
+    from lela_amd._lib import Task
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    fields = [f for f, _ in Task._fields_]
+    src = tmp_path / "layout.c"
+    body = "\n".join(f'printf("{f} %zu\\n", offsetof(mlz_task, {f}));' for f in fields)
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "mlz.h"\nint main(void){\n'
+                   + body + '\nprintf("size %zu\\n", sizeof(mlz_task));\nreturn 0;}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                        check=True).stdout.splitlines())
+    for f in fields:
+        assert int(out[f]) == getattr(Task, f).offset, f
+    assert int(out["size"]) == ctypes.sizeof(Task)

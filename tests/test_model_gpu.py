@@ -1,0 +1,128 @@
+"""The batch cost model on the GPU: its hand-written kernels against the CPU
+oracle (oracle/mlp.c, double precision), training, and what it gains over
+the autotuner's table on measured costs.
+
+Tolerances (float32 kernels vs the double-precision oracle):
+  gradient: |g - g_oracle| <= 2e-4 * max|g_oracle| + 1e-6 per entry;
+  loss: relative 1e-5; predictions: |y - y_oracle| <= 1e-4 * (1 + |y_oracle|);
+  Adam trajectory (30 steps): loss history relative 1e-3, parameters 5e-3."""
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+from lela_amd import Modular, synth
+from lela_amd import batch as lb
+
+pytestmark = pytest.mark.gpu
+
+
+def sample(rng, n):
+    x = rng.uniform(-1.5, 1.5, (n, 5)).astype(np.float32)
+    t = (np.sin(x @ rng.uniform(-1, 1, (5, 4))) * 3 + 5).astype(np.float32)
+    mask = (rng.random((n, 4)) < 0.8).astype(np.uint8)
+    return x, t, mask
+
+
+def test_gradient_matches_oracle(gpu_ctx):
+    rng = np.random.default_rng(21)
+    for n in (1, 127, 128, 129, 1000):
+        w = rng.uniform(-0.4, 0.4, lb.MODEL_PARAMS).astype(np.float32)
+        x, t, mask = sample(rng, n)
+        g, loss = lb.CostModel.grad(gpu_ctx, w, x, t, mask)
+        lo, go = orc.mlp_loss_grad(w, x, t, mask)
+        assert loss == pytest.approx(lo, rel=1e-5)
+        tol = 2e-4 * np.abs(go).max() + 1e-6
+        assert np.abs(g - go).max() <= tol, (n, np.abs(g - go).max(), tol)
+
+
+def test_prediction_matches_oracle(gpu_ctx):
+    rng = np.random.default_rng(22)
+    w = rng.uniform(-0.4, 0.4, lb.MODEL_PARAMS).astype(np.float32)
+    lb.CostModel.set(gpu_ctx, lb.TASK_GEMM, w)
+    assert np.array_equal(lb.CostModel.get(gpu_ctx, lb.TASK_GEMM), w)
+    x, _, _ = sample(rng, 777)
+    y = lb.CostModel.predict(gpu_ctx, lb.TASK_GEMM, x)
+    yo = orc.mlp_forward(w, x)
+    assert np.all(np.abs(y - yo) <= 1e-4 * (1 + np.abs(yo)))
+
+
+def test_training_follows_oracle_adam(gpu_ctx):
+    rng = np.random.default_rng(23)
+    x, t, mask = sample(rng, 300)
+    # steps = 0 stores the initial weights the trainer would start from
+    lb.CostModel.train(gpu_ctx, lb.TASK_RREF, x, t, mask, steps=0, lr=3e-3, seed=77)
+    w0 = lb.CostModel.get(gpu_ctx, lb.TASK_RREF)
+    w_or, hist = orc.mlp_adam(w0, x, t, mask, 30, 3e-3)
+    loss = lb.CostModel.train(gpu_ctx, lb.TASK_RREF, x, t, mask, steps=30, lr=3e-3, seed=77)
+    w_gpu = lb.CostModel.get(gpu_ctx, lb.TASK_RREF)
+    assert np.abs(w_gpu - w_or).max() < 5e-3
+    lo, _ = orc.mlp_loss_grad(w_or, x, t, mask)
+    assert loss == pytest.approx(lo, rel=1e-3)
+    assert hist[-1] < hist[0]
+
+
+def test_training_fits_a_learnable_target(gpu_ctx):
+    rng = np.random.default_rng(24)
+    x, t, mask = sample(rng, 2000)
+    lb.CostModel.train(gpu_ctx, lb.TASK_GEMM, x, t, mask, steps=0, lr=3e-3, seed=5)
+    l0, _ = orc.mlp_loss_grad(lb.CostModel.get(gpu_ctx, lb.TASK_GEMM), x, t, mask)
+    loss = lb.CostModel.train(gpu_ctx, lb.TASK_GEMM, x, t, mask, steps=1500, lr=3e-3, seed=5)
+    assert loss < 0.1 * l0
+
+
+def workload(ctx, F, seed, count, hi):
+    raw = synth.batch_instances(seed, count=count, p=F.p, hi=hi)
+    ins = [lb.Instance(d["op"], d["A"], d.get("B"), d.get("C"), density=d["density"]) for d in raw]
+    return lb.Workload(ctx, F, ins)
+
+
+def features(ws, p):
+    return np.stack([lb.task_features(t, p) for t in ws.tasks])
+
+
+def fit_models(ctx, ws, cost, p, steps=2000):
+    """Trains each op's model on log2 of the measured costs."""
+    x = features(ws, p)
+    ops = np.array([t.op for t in ws.tasks])
+    for op in (lb.TASK_GEMM, lb.TASK_RREF):
+        sel = ops == op
+        y = np.log2(np.where(np.isnan(cost[sel]), 1.0, np.maximum(cost[sel], 1e-3))).astype(np.float32)
+        mask = (~np.isnan(cost[sel])).astype(np.uint8)
+        lb.CostModel.train(ctx, op, x[sel], y, mask, steps=steps, lr=3e-3, seed=op)
+
+
+def chosen(ctx, F, ws, policy):
+    ws.set_variants([-1] * ws.n)
+    plan = lb.Plan(ctx, F, ws.tasks, policy)
+    v = plan.variants()
+    plan.close()
+    return v
+
+
+@pytest.mark.timeout(900)
+def test_model_beats_table_on_measured_cost(gpu_ctx):
+    """Autotuner and model are built from the same measurements of a training
+    batch; on a different batch the model's choices cost less in total
+    (measured cost of every variant of every held-out instance)."""
+    F = Modular(65521, "uint16")
+    train = workload(gpu_ctx, F, 1001, 600, 640)
+    test = workload(gpu_ctx, F, 2002, 600, 640)
+    try:
+        c_train = lb.tune(gpu_ctx, F, train.tasks, reps=3)
+        fit_models(gpu_ctx, train, c_train, F.p)
+        tables = {op: lb.table_get(gpu_ctx, op) for op in (lb.TASK_GEMM, lb.TASK_RREF)}
+        c_test = lb.tune(gpu_ctx, F, test.tasks, reps=3)
+        for op, tab in tables.items():        # back to the training batch's table
+            lb.table_set(gpu_ctx, op, tab)
+        v_tab = chosen(gpu_ctx, F, test, lb.POLICY_TABLE)
+        v_mod = chosen(gpu_ctx, F, test, lb.POLICY_MODEL)
+        idx = np.arange(test.n)
+        tot_tab = float(np.nansum(c_test[idx, v_tab]))
+        tot_mod = float(np.nansum(c_test[idx, v_mod]))
+        tot_best = float(np.nansum(np.nanmin(c_test, axis=1)))
+        print(f"held-out CU-us: table {tot_tab:.0f} model {tot_mod:.0f} best {tot_best:.0f}", flush=True)
+        assert not np.isnan(c_test[idx, v_mod]).any() and not np.isnan(c_test[idx, v_tab]).any()
+        assert tot_mod <= tot_tab
+    finally:
+        train.free()
+        test.free()
