@@ -6,7 +6,9 @@ over the timed steps; `roofline` covers the MFMA kernel alone, timed with HIP
 events on the library's stream; `variants` holds the plain VALU kernel and the
 FP64-carrier kernel on the same shape; `fl_echelon` is the Faugere-Lachartre
 reduction of the 20000 x 30000 F4-shaped matrix (seconds, MFMA families vs
-plain kernels); `rows` times the other level-3 workloads; `cpu_baseline`
+plain kernels); `rows` times the other level-3 workloads; `batched` runs
+4096 uneven gemm/echelon-form instances in captured batches (cost model,
+autotuner table, plain kernels, one call per instance); `cpu_baseline`
 times the CPU restatement of LELA's ZpModule<uint16> gemm on a bounded block
 of rows of the same workload, one core.
 """
@@ -96,6 +98,82 @@ def bench_fl(la, synth, np):
     out["speedup"] = out["plain_s"] / out["mfma_s"]
     for d in (drp, dc, dv, dR):
         d.free()
+    ctx.close()
+    return out
+
+
+def bench_batch(la, synth, np):
+    """Config 5: 4096 uneven instances (m, n, k in [32, 1024], half sparse,
+    gemm and echelon forms mixed), resident in HBM, one captured graph per
+    policy.  The autotuner table and the cost model are fitted on a separate
+    1024-instance training batch (outside the timed region)."""
+    from lela_amd import batch as lb
+    F = la.Modular(P, "uint16")
+    ctx = la.Context(F, device=0)
+
+    def workload(seed, count):
+        raw = synth.batch_instances(seed, count=count, p=P)
+        ins = [lb.Instance(d["op"], d["A"], d.get("B"), d.get("C"), density=d["density"]) for d in raw]
+        return lb.Workload(ctx, F, ins)
+
+    train = workload(synth.seed_for(5, 1), 1024)
+    _, losses = lb.fit(ctx, F, train.tasks, reps=3, steps=3000)
+    train.free()
+    ev = workload(synth.seed_for(5, 0), 4096)
+    ops = np.array([t.op for t in ev.tasks])
+    gemm_macs = float(sum(t.m * t.n * t.k for t in ev.tasks if t.op == lb.TASK_GEMM))
+    out = {"instances": ev.n, "gemm": int((ops == 1).sum()), "rref": int((ops == 2).sum()),
+           "gemm_modular_MACs": gemm_macs, "model_train_loss": {str(k): v for k, v in losses.items()}}
+
+    def timed(policy, variants=None, reps=5):
+        ev.set_variants(variants if variants is not None else [-1] * ev.n)
+        plan = lb.Plan(ctx, F, ev.tasks, policy, graph=True)
+        v = plan.variants()
+        ts = []
+        for r in range(reps + 1):
+            ev.upload()
+            ctx.synchronize()
+            ctx.timer_start()
+            plan.launch()
+            ms = ctx.timer_stop()
+            plan.finish()
+            if r:
+                ts.append(ms)
+        plan.close()
+        return float(np.median(ts)), v
+
+    chosen = {}
+    for name, policy, var in (("model", lb.POLICY_MODEL, None), ("table", lb.POLICY_TABLE, None),
+                              ("plain", lb.POLICY_FIXED, [0] * ev.n)):
+        ms, v = timed(policy, var)
+        chosen[name] = v
+        out[f"{name}_ms"] = ms
+    out["instances_per_s"] = ev.n / (out["model_ms"] * 1e-3)
+    out["model_over_plain"] = out["plain_ms"] / out["model_ms"]
+    # one library call per instance, as a caller looping over LELA calls would
+    import ctypes
+    ev.upload()
+    ctx.synchronize()
+    ctx.set_ptr_mode(la.PTR_DEVICE)
+    rank = ctypes.c_size_t()
+    t0 = time.perf_counter()
+    for t in ev.tasks:
+        if t.op == lb.TASK_GEMM:
+            rc = ctx.lib.mlz_zp_gemm(ctx.handle, F.elem, P, t.m, t.n, t.k, t.a, t.A, t.lda, t.B, t.ldb, t.b,
+                                     t.C, t.ldc, 0)
+        else:
+            rc = ctx.lib.mlz_zp_rref(ctx.handle, F.elem, P, t.m, t.n, t.A, t.lda, ctypes.byref(rank), None, 0)
+        ctx.check(rc, "per-instance call")
+    ctx.synchronize()
+    out["per_instance_calls_ms"] = (time.perf_counter() - t0) * 1e3
+    out["batched_over_per_instance"] = out["per_instance_calls_ms"] / out["model_ms"]
+    # measured cost (CU-microseconds) of every variant of every held-out instance
+    cost = lb.tune(ctx, F, ev.tasks, reps=2)
+    idx = np.arange(ev.n)
+    out["cu_us"] = {name: float(np.nansum(cost[idx, v])) for name, v in chosen.items()}
+    out["cu_us"]["best"] = float(np.nansum(np.nanmin(cost, axis=1)))
+    out["model_gain_over_table"] = out["cu_us"]["table"] / out["cu_us"]["model"]
+    ev.free()
     ctx.close()
     return out
 
@@ -258,6 +336,7 @@ def main():
         ctx.close()
         out["fl_echelon"] = bench_fl(la, synth, np)
         out["rows"] = bench_rows(la, synth, np)
+        out["batched"] = bench_batch(la, synth, np)
     else:
         for d in (dA, dB, dC):
             d.free()

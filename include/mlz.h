@@ -80,7 +80,9 @@ int mlz_ctx_set_ptr_mode(mlz_ctx *ctx, int mode);
  *   "gemm.i8.variant"  INT8 gemm workgroup tile: 0 by shape, 1 = 128x128,
  *                      2 = 128x256, 3 = 256x128 (4 waves each), 4 = 128x128
  *                      with two workgroups per CU, 5 = 128x256 and
- *                      6 = 256x128 with 8 waves and two workgroups per CU. */
+ *                      6 = 256x128 with 8 waves and two workgroups per CU.
+ *   "rref.update.blocks" 32-column blocks per wave of the echelon form's
+ *                      trailing update on INT8 limbs: 1 (default), 2 or 4. */
 int mlz_ctx_set_option(mlz_ctx *ctx, const char *key, int64_t value);
 int mlz_ctx_synchronize(mlz_ctx *ctx);
 /* Message for the last error on this context (static storage, never NULL). */

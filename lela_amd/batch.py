@@ -25,7 +25,7 @@ __all__ = [
     "Instance", "Workload", "Plan", "tune", "table_get", "table_set", "task_features", "CostModel",
     "TASK_GEMM", "TASK_RREF", "POLICY_FIXED", "POLICY_TABLE", "POLICY_MODEL", "GEMM_PLAIN",
     "GEMM_SPARSE", "GEMM_I8S", "GEMM_I8L", "RREF_W64", "RREF_W32", "MAX_VARIANTS",
-    "MODEL_PARAMS", "nvariants", "variant_ok",
+    "MODEL_PARAMS", "nvariants", "variant_ok", "fit",
 ]
 
 _ALIGN = 256
@@ -285,3 +285,23 @@ class CostModel:
                                          mask.ctypes.data, x.shape[0], g.ctypes.data, ctypes.byref(loss)),
                   "model_grad")
         return g, loss.value
+
+
+def fit(ctx, F, tasks, reps=3, steps=3000, lr=3e-3):
+    """Autotunes on `tasks` (mlz_tune: measured cost of every variant, which
+    also fills the context's table), then trains each op's cost model on
+    log2 of those costs.  Returns the cost matrix [ntasks, MAX_VARIANTS]
+    and the final training loss per op."""
+    cost = tune(ctx, F, tasks, reps=reps)
+    x = np.stack([task_features(t, F.p) for t in tasks])
+    ops = np.array([t.op for t in tasks])
+    losses = {}
+    for op in (TASK_GEMM, TASK_RREF):
+        sel = ops == op
+        if not sel.any():
+            continue
+        c = cost[sel]
+        mask = ~np.isnan(c)
+        y = np.log2(np.where(mask, np.maximum(c, 1e-3), 1.0)).astype(np.float32)
+        losses[op] = CostModel.train(ctx, op, x[sel], y, mask.astype(np.uint8), steps=steps, lr=lr, seed=op)
+    return cost, losses

@@ -68,6 +68,28 @@ struct ModP {
 	{
 		return SMALL ? mul_small(x, y) : (uint32_t) reduce((uint64_t) x * y);
 	}
+	// x mod p for an integer-valued double |x| < 2^53 and p < 2^26: the
+	// quotient estimate floor(x/p) is off by at most one, the remainder
+	// x - q*p is exact (fma), one correction each way.
+	__host__ __device__ inline uint32_t mod_f64(double x) const
+	{
+		const double pd = (double) p;
+		const double q = floor(x * pinv);
+		double r = fma(-q, pd, x);
+		r = r < 0.0 ? r + pd : r;
+		r = r >= pd ? r - pd : r;
+		return (uint32_t) r;
+	}
+	// (x*y - f*g) mod p for x, y, f, g < p.  SMALL (p < 2^26): both products
+	// are exact doubles and so is their difference (|.| < 2^52), one
+	// reduction; otherwise two Barrett products.
+	template <bool SMALL>
+	__host__ __device__ inline uint32_t mul_subT(uint32_t x, uint32_t y, uint32_t f, uint32_t g) const
+	{
+		if (SMALL)
+			return mod_f64(fma((double) x, (double) y, -((double) f * (double) g)));
+		return sub(mulT<false>(x, y), mulT<false>(f, g));
+	}
 	// x^e mod p
 	__host__ __device__ inline uint32_t pow(uint32_t x, uint64_t e) const
 	{
@@ -246,6 +268,7 @@ struct mlz_ctx {
 	void *host_pinned;
 	size_t host_pinned_bytes;
 	int i8_variant;      // INT8 gemm tile variant, 0 = chosen by shape
+	int upd_nblk;        // 32-column blocks per wave of the rank-k update (1, 2, 4)
 	struct mlz_tuner *tuner;   // autotuner table and cost-model weights (batch.hip), lazily created
 };
 
@@ -294,6 +317,11 @@ int zp_scal_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, uint64_t
 		void *C, size_t ldc);
 int zp_trsm_dev(mlz_ctx *ctx, int elem, uint64_t p, int upper, int unit, size_t n, size_t nrhs,
 		uint64_t a, const void *A, size_t lda, void *B, size_t ldb, size_t *bad_diag, int algo);
+int zp_trsm_prepare(mlz_ctx *ctx, int elem, uint64_t p, int upper, int unit, size_t n, const void *A, size_t lda,
+		    size_t *bad_diag, const uint32_t **W_out);
+int zp_trsm_apply(mlz_ctx *ctx, int elem, uint64_t p, int upper, int unit, size_t n, size_t nrhs, uint64_t a,
+		  const void *A, size_t lda, void *B, size_t ldb, const uint32_t *W, int algo);
+constexpr size_t TRSM_LEAF_ROWS = 64;   // block size of zp_trsm_prepare's inverses
 int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A, size_t lda,
 		size_t *rank_out, uint32_t *pivots_out, int algo);
 int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda,
@@ -301,6 +329,8 @@ int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64
 int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size_t *rank_out,
 		 uint32_t *pivots_out);
 int pick_gemm_algo(uint64_t p, int elem, size_t m, size_t n, size_t k, int algo);
+int zp_rank_update_i8_dev(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, const uint32_t *G, size_t ldg,
+			  const uint32_t *W, size_t ldw, uint32_t *C, size_t ldc);
 int check_modulus(mlz_ctx *ctx, int elem, uint64_t p);
 void tuner_free(mlz_ctx *ctx);
 

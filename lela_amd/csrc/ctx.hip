@@ -165,6 +165,7 @@ int mlz_ctx_create(mlz_ctx **out, int device, const char *config)
 	ctx->host_pinned = nullptr;
 	ctx->host_pinned_bytes = 0;
 	ctx->i8_variant = 0;
+	ctx->upd_nblk = 1;
 	ctx->tuner = nullptr;
 	// Default priority only: streams of different priorities are not used.
 	if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -224,6 +225,12 @@ int mlz_ctx_set_option(mlz_ctx *ctx, const char *key, int64_t value)
 		if (value < 0 || value > 6)
 			return set_error(ctx, MLZ_E_ARG, "gemm.i8.variant must be 0..6");
 		ctx->i8_variant = (int) value;
+		return MLZ_OK;
+	}
+	if (strcmp(key, "rref.update.blocks") == 0) {
+		if (value != 1 && value != 2 && value != 4)
+			return set_error(ctx, MLZ_E_ARG, "rref.update.blocks must be 1, 2 or 4");
+		ctx->upd_nblk = (int) value;
 		return MLZ_OK;
 	}
 	return set_error(ctx, MLZ_E_ARG, "unknown option '%s'", key);

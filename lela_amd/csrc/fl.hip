@@ -11,7 +11,10 @@
 //      heads a kernel computes.
 //   2. MatrixGrid1 split (:148-275) into dense A (pivot rows x pivot columns,
 //      upper triangular), B, C, D by one scatter kernel over the nonzeros.
-//   3. B <- A^-1 B   (:791, upper, non-unit; zp_trsm.hip)
+//   3. B <- A^-1 B   (:791, upper, non-unit): blocked back-substitution that
+//      skips the zero columns right of each block-row's last nonzero (the
+//      pivot block of an F4 matrix is sparse; LELA hands trsm a sparse A);
+//      diagonal blocks by zp_trsm.hip, off-diagonal slabs by zp_gemm.hip.
 //   4. D <- D - C B  (:805, Schur complement; zp_gemm.hip)
 //   5. D <- RREF(D)  (:822-852; zp_rref.hip) -- LELA takes the non-reduced
 //      echelon form and reduces it with D1^-1 D2 (:861); RREF(D) is that
@@ -88,6 +91,25 @@ __global__ void fl_scatter_kernel(const uint64_t *__restrict__ row_ptr, const ui
 	}
 }
 
+// last[ri] = largest A-column index with a nonzero entry in pivot row ri (its
+// head at least, since A is upper triangular with the heads on the diagonal).
+template <int E>
+__global__ void fl_lastcol_kernel(const uint64_t *__restrict__ row_ptr, const uint32_t *__restrict__ col,
+				  const typename Elem<E>::T *__restrict__ val, int m, uint32_t p,
+				  const int *__restrict__ rowmap, const int *__restrict__ colmap, int *__restrict__ last)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= m || rowmap[i] < 0)
+		return;
+	int mx = rowmap[i];
+	for (uint64_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+		const int cj = colmap[col[e]];
+		if (cj > mx && Elem<E>::load(val + e, p) % p != 0)
+			mx = cj;
+	}
+	last[rowmap[i]] = mx;
+}
+
 // G[i][s] = B[i][jd[s]]  (r x rd)
 __global__ void fl_gather_cols_kernel(const uint32_t *__restrict__ B, int r, int nr, const int *__restrict__ jd,
 				      int rd, uint32_t *__restrict__ G)
@@ -107,21 +129,25 @@ __global__ void fl_assemble_kernel(const uint32_t *__restrict__ B, const uint32_
 				   const int *__restrict__ pcol, int rank, int m, int n, uint32_t p,
 				   typename Elem<E>::T *__restrict__ R, size_t ldr)
 {
-	const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-	if (idx >= (size_t) m * n)
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= n)
 		return;
-	const int t = (int) (idx / n), j = (int) (idx % n);
-	uint32_t v = 0;
-	if (t < rank) {
-		const int cj = colmap[j];
-		const int s = src[t];
-		if (cj >= 0)
-			v = (s >= 0 && j == pcol[t]) ? 1u : 0u;
-		else
-			v = s >= 0 ? B[(size_t) s * nr + (-1 - cj)] : D[(size_t) (-1 - s) * nr + (-1 - cj)];
+	const int cj = colmap[j];
+	for (int t = blockIdx.y; t < m; t += gridDim.y) {
+		uint32_t v = 0;
+		if (t < rank) {
+			const int s = src[t];
+			if (cj >= 0)
+				v = (s >= 0 && j == pcol[t]) ? 1u : 0u;
+			else
+				v = s >= 0 ? B[(size_t) s * nr + (-1 - cj)] : D[(size_t) (-1 - s) * nr + (-1 - cj)];
+		}
+		Elem<E>::store(R + (size_t) t * ldr + j, v % p, p);
 	}
-	Elem<E>::store(R + (size_t) t * ldr + j, v % p, p);
 }
+
+constexpr size_t FL_TRSM_BLOCK = 512;   // rows per block of the pivot-block solve (multiple of TRSM_LEAF_ROWS)
+static_assert(FL_TRSM_BLOCK % TRSM_LEAF_ROWS == 0, "solve blocks must align with the inverted diagonal blocks");
 
 struct FlResult {
 	size_t rank;
@@ -210,13 +236,43 @@ static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *
 									   dC, dD);
 		MLZ_TRY(check_launch(ctx, "fl_scatter_kernel"));
 	}
-	// 3. B <- A^-1 B
+	// 3. B <- A^-1 B, as blocked back-substitution over FL_TRSM_BLOCK-row
+	//    blocks from the bottom: B_I <- B_I - A[I, I+1..c1) X[I+1..c1), then
+	//    X_I = A_II^-1 B_I, where c1 is one past the last nonzero column of
+	//    block-row I.  F4 pivot blocks are sparse and banded, so the columns
+	//    right of c1 (known zeros) are skipped; for a dense A this is the
+	//    ordinary blocked solve.
 	if (r && nr) {
-		size_t bad = 0;
-		int rc = zp_trsm_dev(ctx, MLZ_U32, p, 1, 0, r, nr, 1, dA, r, dB, nr, &bad, algo);
-		if (rc != MLZ_OK) {
-			res->rank = r;
-			return rc;
+		void *wl;
+		MLZ_TRY(ws_get(ctx, WS_FL_5, r * sizeof(int) + 256, &wl));
+		fl_lastcol_kernel<E><<<ceil_div(m, 256), 256, 0, ctx->stream>>>(d_rowptr, d_col, (const T *) d_val, (int) m,
+										 p, d_rowmap, d_colmap, (int *) wl);
+		MLZ_TRY(check_launch(ctx, "fl_lastcol_kernel"));
+		std::vector<int> last(r);
+		MLZ_HIP(ctx, hipMemcpyAsync(last.data(), wl, r * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+		MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+		// every diagonal block's inverse in one launch, and LELA's
+		// non-invertible-head check over all of A
+		const uint32_t *Winv = nullptr;
+		{
+			size_t bad = 0;
+			const int rc = zp_trsm_prepare(ctx, MLZ_U32, p, 1, 0, r, dA, r, &bad, &Winv);
+			if (rc != MLZ_OK) {
+				res->rank = r;
+				return rc;
+			}
+		}
+		const size_t nb = (r + FL_TRSM_BLOCK - 1) / FL_TRSM_BLOCK;
+		for (size_t I = nb; I-- > 0;) {
+			const size_t i0 = I * FL_TRSM_BLOCK, i1 = std::min(r, i0 + FL_TRSM_BLOCK);
+			size_t c1 = i1;
+			for (size_t i = i0; i < i1; ++i)
+				c1 = std::max(c1, (size_t) last[i] + 1);
+			if (c1 > i1)
+				MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, i1 - i0, nr, c1 - i1, p - 1, dA + i0 * r + i1, r,
+						    dB + i1 * nr, nr, 1, dB + i0 * nr, nr, algo));
+			MLZ_TRY(zp_trsm_apply(ctx, MLZ_U32, p, 1, 0, i1 - i0, nr, 1, dA + i0 * r + i0, r, dB + i0 * nr, nr,
+					      Winv + i0 * TRSM_LEAF_ROWS, algo));
 		}
 	}
 	// 4. D <- D - C B
@@ -267,9 +323,8 @@ static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *
 	MLZ_HIP(ctx, hipMemcpyAsync(d_pcol, pcol.data(), (rank + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
 	if (R) {
 		ProfScope ps(ctx, "fl_assemble");
-		fl_assemble_kernel<E><<<ceil_div(m * n, 256), 256, 0, ctx->stream>>>(dB, dD, (int) nr, d_colmap, d_src, d_pcol,
-										    (int) rank, (int) m, (int) n, p, (T *) R,
-										    ldr);
+		fl_assemble_kernel<E><<<dim3(ceil_div(n, 256), (unsigned) std::min<size_t>(m, 8192)), 256, 0, ctx->stream>>>(
+			dB, dD, (int) nr, d_colmap, d_src, d_pcol, (int) rank, (int) m, (int) n, p, (T *) R, ldr);
 		MLZ_TRY(check_launch(ctx, "fl_assemble_kernel"));
 	}
 	MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -188,7 +188,7 @@ struct I8Tile {
 template <int E, int WTM, int WTN, int KK, int WM, int WN>
 __device__ __forceinline__ void i8_tile(const int8_t *__restrict__ Ap, const int8_t *__restrict__ Bp, int KB,
 					typename Elem<E>::T *__restrict__ C, size_t ldc, int m, int n, uint32_t a,
-					uint32_t b, const ModP &mp, uint64_t off, int tm, int tn, int8_t *lds_base)
+					uint32_t b, const ModP &mp, int tm, int tn, int8_t *lds_base)
 {
 	typedef I8Tile<WTM, WTN, KK, WM, WN> T;
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -273,14 +273,16 @@ __device__ __forceinline__ void i8_tile(const int8_t *__restrict__ Ap, const int
 			for (int r = 0; r < 16; ++r) {
 				const int row = tm * T::BM + wr * 32 * WTM + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
 				if (row < m && col < n) {
-					int64_t v = (int64_t) acc[0][i][j][r] * 65536 + (int64_t) acc[1][i][j][r] * 256 +
-						    (int64_t) acc[2][i][j][r];
-					uint32_t d = (uint32_t) mp.reduce((uint64_t) (v + (int64_t) off));
+					// 65536*HH + 256*(HL+LH) + LL has |.| < 2^48 and a*d + b*c < 2^33
+					// (p <= 2^16): both are exact doubles, reduced by mod_f64.
+					const double v = fma((double) acc[0][i][j][r], 65536.0,
+							     fma((double) acc[1][i][j][r], 256.0, (double) acc[2][i][j][r]));
+					const double d = (double) mp.mod_f64(v);
 					typename Elem<E>::T *cp = C + (size_t) row * ldc + col;
-					uint64_t t = (uint64_t) a * d;
+					double t = (double) a * d;
 					if (b)
-						t += (uint64_t) b * Elem<E>::load(cp, (uint32_t) mp.p);
-					Elem<E>::store(cp, (uint32_t) mp.reduce(t), (uint32_t) mp.p);
+						t = fma((double) b, (double) Elem<E>::load(cp, (uint32_t) mp.p), t);
+					Elem<E>::store(cp, mp.mod_f64(t), (uint32_t) mp.p);
 				}
 			}
 		}
@@ -291,12 +293,12 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_kernel(const int8_t
 							   const int8_t *__restrict__ Bp, int KB,
 							   typename Elem<E>::T *__restrict__ C, size_t ldc,
 							   int m, int n, uint32_t a, uint32_t b, ModP mp,
-							   uint64_t off, int tiles_m, int tiles_n)
+							   int tiles_m, int tiles_n)
 {
 	__shared__ __attribute__((aligned(16))) int8_t lds[2 * I8Tile<WTM, WTN, KK, WM, WN>::STAGE];
 	int tm, tn;
 	tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
-	i8_tile<E, WTM, WTN, KK, WM, WN>(Ap, Bp, KB, C, ldc, m, n, a, b, mp, off, tm, tn, lds);
+	i8_tile<E, WTM, WTN, KK, WM, WN>(Ap, Bp, KB, C, ldc, m, n, a, b, mp, tm, tn, lds);
 }
 
 // Tile variants of the INT8 kernel (index = ctx option "gemm.i8.variant"):
@@ -317,8 +319,6 @@ static int gemm_i8_launch_t(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t
 	typedef I8Tile<WTM, WTN, KK, WM, WN> Tile;
 	const ModP mp = make_modp(p);
 	const size_t mpad = round_up(m, Tile::BM), npad = round_up(n, Tile::BN);
-	// |v| < 65536*2^31 + 256*2^31 + 2^31 < 2^48; off is a multiple of p above that.
-	const uint64_t off = ((((uint64_t) 1 << 48) + p - 1) / p) * p;
 	for (size_t k0 = 0; k0 < k; k0 += I8_KMAX) {
 		const size_t kc = std::min((size_t) I8_KMAX, k - k0);
 		const size_t kpad = round_up(kc, 64);
@@ -355,7 +355,7 @@ static int gemm_i8_launch_t(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t
 			ProfScope ps(ctx, "gemm_i8_mfma");
 			gemm_i8_kernel<E, WTM, WTN, KK, OCC, WM, WN><<<tiles_m * tiles_n, Tile::THREADS, 0, ctx->stream>>>(
 				(const int8_t *) ap, (const int8_t *) bp, KB, (T *) C, ldc, (int) m, (int) n, a,
-				k0 == 0 ? b : 1u, mp, off, tiles_m, tiles_n);
+				k0 == 0 ? b : 1u, mp, tiles_m, tiles_n);
 		}
 		MLZ_TRY(check_launch(ctx, "gemm_i8_kernel"));
 	}
@@ -395,6 +395,128 @@ static int gemm_i8_launch(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k
 	default:
 		return gemm_i8_launch_t<E, 2, 2, 2, 1>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
 	}
+}
+
+// ===========================================================================
+// Rank-k update C <- C - G*W for small k (the trailing update of a panel of
+// the echelon form): the output is read and written once per call while k
+// is tiny, so the tiled kernel's LDS staging and barriers are overhead.
+// Each wave takes one 32-row block and a strip of NBLK 32-column blocks;
+// fragments come straight from the packed operands (16 bytes per lane, no
+// LDS), the limb products go through the same three accumulators, and the
+// epilogue subtracts in place.  No barriers: a wave past the last row block
+// just returns.
+// ===========================================================================
+template <int UPD_NBLK>
+__global__ __launch_bounds__(256) void rank_update_i8_kernel(const int8_t *__restrict__ Gp,
+							     const int8_t *__restrict__ Wp, int KB,
+							     uint32_t *__restrict__ C, size_t ldc, int m, int n,
+							     ModP mp, int strips_n)
+{
+	const int lane = threadIdx.x & 63;
+	const int wid = blockIdx.x * 4 + (threadIdx.x >> 6);
+	const int rb = wid / strips_n, sn = wid - rb * strips_n;
+	if (rb * 32 >= m)
+		return;
+	const size_t panel = (size_t) KB * 2 * I8_FRAG;
+	const int8_t *ga = Gp + (size_t) rb * panel + lane * 16;
+	const int cb0 = sn * UPD_NBLK;
+	const int nblk = min(UPD_NBLK, (n + 31) / 32 - cb0);
+	v16i acc[3][UPD_NBLK];
+#pragma unroll
+	for (int t = 0; t < 3; ++t)
+#pragma unroll
+		for (int j = 0; j < UPD_NBLK; ++j)
+			acc[t][j] = v16i{0};
+	for (int kb = 0; kb < KB; ++kb) {
+		const v4i ah = *(const v4i *) (ga + (size_t) kb * 2 * I8_FRAG);
+		const v4i al = *(const v4i *) (ga + (size_t) kb * 2 * I8_FRAG + I8_FRAG);
+#pragma unroll
+		for (int j = 0; j < UPD_NBLK; ++j) {
+			if (j < nblk) {
+				const int8_t *gb = Wp + (size_t) (cb0 + j) * panel + (size_t) kb * 2 * I8_FRAG + lane * 16;
+				const v4i bh = *(const v4i *) gb, bl = *(const v4i *) (gb + I8_FRAG);
+				acc[0][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bh, acc[0][j], 0, 0, 0);
+				acc[1][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bl, acc[1][j], 0, 0, 0);
+				acc[2][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl, acc[2][j], 0, 0, 0);
+				acc[1][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, acc[1][j], 0, 0, 0);
+			}
+		}
+	}
+	const uint32_t p = (uint32_t) mp.p;
+#pragma unroll
+	for (int j = 0; j < UPD_NBLK; ++j) {
+		const int col = (cb0 + j) * 32 + (lane & 31);
+		if (j >= nblk || col >= n)
+			continue;
+#pragma unroll
+		for (int r = 0; r < 16; ++r) {
+			const int row = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+			if (row < m) {
+				const double v = fma((double) acc[0][j][r], 65536.0,
+						     fma((double) acc[1][j][r], 256.0, (double) acc[2][j][r]));
+				const uint32_t d = mp.mod_f64(v);
+				uint32_t *cp = C + (size_t) row * ldc + col;
+				const uint32_t c = *cp;
+				*cp = c >= d ? c - d : c + p - d;
+			}
+		}
+	}
+}
+
+// C (m x n, canonical uint32, leading dimension ldc) <- C - G*W mod p with G
+// m x k (ldg) and W k x n (ldw), p <= 2^16, k <= I8_KMAX.
+int zp_rank_update_i8_dev(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, const uint32_t *G, size_t ldg,
+			  const uint32_t *W, size_t ldw, uint32_t *C, size_t ldc)
+{
+	if (m == 0 || n == 0 || k == 0)
+		return MLZ_OK;
+	if (p > 65536 || k > (size_t) I8_KMAX)
+		return set_error(ctx, MLZ_E_UNSUPPORTED, "rank update needs p <= 2^16 and k <= %d", I8_KMAX);
+	const ModP mp = make_modp(p);
+	const size_t mpad = round_up(m, 64), npad = round_up(n, 64), kpad = round_up(k, 64);
+	const int KB = (int) (kpad / 32);
+	void *gp = nullptr, *wp = nullptr;
+	MLZ_TRY(ws_get(ctx, WS_PACK_A, mpad * kpad * 2, &gp));
+	MLZ_TRY(ws_get(ctx, WS_PACK_B, npad * kpad * 2, &wp));
+	{
+		ProfScope ps(ctx, "pack_i8_A");
+		dim3 g(mpad / 64, kpad / 64);
+		if (aligned16(G, ldg * 4))
+			pack_i8_kernel<MLZ_U32, false, true><<<g, 256, 0, ctx->stream>>>(G, ldg, (int) m, (int) k, p,
+											 (int8_t *) gp, KB);
+		else
+			pack_i8_kernel<MLZ_U32, false, false><<<g, 256, 0, ctx->stream>>>(G, ldg, (int) m, (int) k, p,
+											  (int8_t *) gp, KB);
+	}
+	{
+		ProfScope ps(ctx, "pack_i8_B");
+		dim3 g(npad / 64, kpad / 64);
+		if (aligned16(W, ldw * 4))
+			pack_i8_kernel<MLZ_U32, true, true><<<g, 256, 0, ctx->stream>>>(W, ldw, (int) n, (int) k, p,
+											(int8_t *) wp, KB);
+		else
+			pack_i8_kernel<MLZ_U32, true, false><<<g, 256, 0, ctx->stream>>>(W, ldw, (int) n, (int) k, p,
+											 (int8_t *) wp, KB);
+	}
+	MLZ_TRY(check_launch(ctx, "pack_i8_kernel"));
+	// strip width (32-column blocks per wave): wider strips reuse each row
+	// block's fragments more, narrower ones leave room for more waves
+	const int nblk = ctx->upd_nblk == 2 || ctx->upd_nblk == 4 ? ctx->upd_nblk : 1;
+	const int strips_n = (int) ceil_div(n, 32 * nblk);
+	const size_t waves = (mpad / 32) * (size_t) strips_n;
+	const unsigned grid = ceil_div(waves, 4);
+	ProfScope ps(ctx, "rank_update_i8");
+	if (nblk == 1)
+		rank_update_i8_kernel<1><<<grid, 256, 0, ctx->stream>>>((const int8_t *) gp, (const int8_t *) wp, KB, C, ldc,
+									(int) m, (int) n, mp, strips_n);
+	else if (nblk == 4)
+		rank_update_i8_kernel<4><<<grid, 256, 0, ctx->stream>>>((const int8_t *) gp, (const int8_t *) wp, KB, C, ldc,
+									(int) m, (int) n, mp, strips_n);
+	else
+		rank_update_i8_kernel<2><<<grid, 256, 0, ctx->stream>>>((const int8_t *) gp, (const int8_t *) wp, KB, C, ldc,
+									(int) m, (int) n, mp, strips_n);
+	return check_launch(ctx, "rank_update_i8_kernel");
 }
 
 // ===========================================================================
@@ -829,7 +951,7 @@ __global__ __launch_bounds__(256) void batch_pack_i8_kernel(const BatchGemm *__r
 // INT8 tile items: x = tile row, y = tile column.
 template <int E, int WTM, int WTN, int KK, int OCC>
 __global__ __launch_bounds__(256, OCC) void batch_i8_kernel(const BatchGemm *__restrict__ tasks, BatchLaunch L,
-							    ModP mp, uint64_t off)
+							    ModP mp)
 {
 	typedef typename Elem<E>::T T;
 	__shared__ __attribute__((aligned(16))) int8_t lds[2 * I8Tile<WTM, WTN, KK>::STAGE];
@@ -841,8 +963,7 @@ __global__ __launch_bounds__(256, OCC) void batch_i8_kernel(const BatchGemm *__r
 		const BatchItem w = L.items[it];
 		const BatchGemm &t = tasks[w.task];
 		const uint64_t t0 = wall_clock64();
-		i8_tile<E, WTM, WTN, KK, 2, 2>(t.Ap, t.Bp, t.KB, (T *) t.C, t.ldc, t.m, t.n, t.a, t.b, mp, off, w.x, w.y,
-					       lds);
+		i8_tile<E, WTM, WTN, KK, 2, 2>(t.Ap, t.Bp, t.KB, (T *) t.C, t.ldc, t.m, t.n, t.a, t.b, mp, w.x, w.y, lds);
 		__syncthreads();
 		batch_charge(L, w.task, t0);
 	}
@@ -955,7 +1076,6 @@ static int batch_gemm_launch_t(hipStream_t stream, int variant, uint32_t p, uint
 			       BatchLaunch L, int grid)
 {
 	const ModP mp = make_modp(p);
-	const uint64_t off = ((((uint64_t) 1 << 48) + p - 1) / p) * p;
 	const uint64_t t64 = two64_mod(p);
 	const bool wide = plain_wide(p, kmax);
 	switch (variant) {
@@ -972,10 +1092,10 @@ static int batch_gemm_launch_t(hipStream_t stream, int variant, uint32_t p, uint
 			batch_sparse_kernel<E, false><<<grid, 256, 0, stream>>>(tasks, L, mp, t64);
 		break;
 	case BG_I8S:
-		batch_i8_kernel<E, 1, 1, 2, 4><<<grid, 256, 0, stream>>>(tasks, L, mp, off);
+		batch_i8_kernel<E, 1, 1, 2, 4><<<grid, 256, 0, stream>>>(tasks, L, mp);
 		break;
 	case BG_I8L:
-		batch_i8_kernel<E, 2, 2, 1, 2><<<grid, 256, 0, stream>>>(tasks, L, mp, off);
+		batch_i8_kernel<E, 2, 2, 1, 2><<<grid, 256, 0, stream>>>(tasks, L, mp);
 		break;
 	default:
 		return MLZ_E_ARG;

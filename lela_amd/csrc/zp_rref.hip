@@ -9,18 +9,21 @@
 //
 // Blocked Gauss-Jordan over column panels of RREF_W columns.  Per panel
 //  1. pivot search among the rows that are not pivots yet, restricted to the
-//     panel: one workgroup reduces the first 2*RREF_W such rows to echelon
-//     form; if that reaches full panel rank (or covers every remaining row) its
-//     pivot rows are the answer.  Otherwise a tournament in one kernel: each
+//     panel: one workgroup reduces the first RREF_W such rows together with
+//     the identity; if that reaches full panel rank its rows are the pivot
+//     rows and step 2's inverse comes out of the same elimination (if it
+//     covers every remaining row, its pivots are the answer).  Otherwise a
+//     tournament in one kernel: each
 //     workgroup reduces a 128-row chunk and keeps its pivot rows, and
 //     workgroups merge pairwise up a tree (the second to arrive at a node
 //     merges, via an atomic arrival counter).  Either way the chosen rows span
 //     the panel's row space, so their pivot columns J are its rank profile.
 //  2. W = P_J^-1 * P (P = the k pivot rows from column c on, P_J its columns
-//     J): one workgroup inverts P_J, a gemm forms W (identity on J).
+//     J): one workgroup inverts P_J (unless step 1 did), a gemm forms W
+//     (identity on J).
 //  3. every other row i: A_i <- A_i - A_i[J] * W, one gemm over all rows (the
-//     new pivot rows get a zero multiplier row), then W is written into the
-//     pivot rows.
+//     new pivot rows get a zero multiplier row; for p <= 2^16 a rank-64
+//     update on INT8 limbs), then W is written into the pivot rows.
 // Eliminations inside a workgroup keep the matrix in registers: thread
 // (g, c) owns column c of the rows of group g, so a column step needs two
 // barriers and each thread's updates are independent.  Steps are fraction
@@ -39,7 +42,6 @@ namespace mlz {
 
 constexpr int RREF_W = 64;        // panel width (columns)
 constexpr int RREF_CHUNK = 128;   // rows per tournament leaf (= two merged lists)
-constexpr int RREF_FAST = 128;    // candidate rows of the fast path
 constexpr int RREF_MAXLVL = 24;
 
 // Device-side state of one RREF call.
@@ -53,6 +55,7 @@ struct RrefState {
 	int *prow;           // [RREF_W] their rows
 	int *bad;            // [1] set when a pivot is not invertible (composite p)
 	int *need_tree;      // [1] the fast path did not settle the panel
+	int *inv_done;       // [1] the fast path also produced P_J^-1
 	int *lists_rows;     // [levels][nodes0][RREF_W]
 	int *lists_cnt;      // [levels][nodes0]
 	int *counters;       // [levels][nodes0]
@@ -124,12 +127,17 @@ __device__ int reg_eliminate(uint32_t (&v)[RPG], int nrows, int nsearch, bool re
 		__syncthreads();
 		if (pr == INT_MAX)
 			continue;
-		const uint32_t pc = pivrow[c], x = pivrow[col];
+		// In echelon mode columns up to col are never read again (the pivot
+		// test of a later column only reads that column), so they are not
+		// updated.
+		if (reduced || c > col) {
+			const uint32_t pc = pivrow[c], x = pivrow[col];
 #pragma unroll
-		for (int i = 0; i < RPG; ++i) {
-			const uint32_t fr = f[rbase + i];
-			if (fr)
-				v[i] = mp.sub(mp.mulT<SMALL>(x, v[i]), mp.mulT<SMALL>(fr, pc));
+			for (int i = 0; i < RPG; ++i) {
+				const uint32_t fr = f[rbase + i];
+				if (fr)
+					v[i] = mp.mul_subT<SMALL>(x, v[i], fr, pc);
+			}
 		}
 		++cnt;
 	}
@@ -175,14 +183,21 @@ __device__ inline int nodes_at(int nodes0, int level)
 	return n;
 }
 
-// Fast path: the first RREF_FAST rows that are not pivots yet (in row order).
+// Fast path, one workgroup: S = the first wc rows that are not pivots yet (in
+// row order), on the panel's wc columns, is reduced together with the
+// identity, [S | I] -> [Pi | R].  If S has full panel rank its rows are the
+// panel's pivot rows and P_J^-1 comes out directly: with Q the order of the
+// rows by pivot column, P_J = Q S, so P_J^-1 = S^-1 Q^T, whose entry (c, t)
+// is R[row of pivot c][row of pivot t].  If S is rank deficient but holds
+// every live row, the pivots are still final and the inverse kernel builds
+// P_J^-1; otherwise the tournament decides.
 template <bool SMALL>
 __global__ __launch_bounds__(1024) void rref_fast_kernel(const uint32_t *__restrict__ A, size_t lda, int m, int c,
-							int wc, ModP mp, RrefState st)
+							int wc, ModP mp, RrefState st, uint32_t *__restrict__ Pinv)
 {
-	constexpr int RPG = RREF_FAST / (1024 / RREF_W);
-	__shared__ int rowid[RREF_FAST], alive[RREF_FAST], outr[RREF_W], outc[RREF_W];
-	__shared__ uint32_t f[RREF_FAST], pivrow[RREF_W];
+	constexpr int W = RREF_W, NC = 2 * RREF_W, RPG = W / (1024 / NC);
+	__shared__ int rowid[W], alive[W], outr[W], outc[W], pos[W];
+	__shared__ uint32_t f[W], pivrow[NC];
 	__shared__ int s_min[2], s_cnt, s_wave[16];
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	if (tid == 0)
@@ -201,31 +216,53 @@ __global__ __launch_bounds__(1024) void rref_fast_kernel(const uint32_t *__restr
 		for (int w = 0; w < wave; ++w)
 			before += s_wave[w];
 		const int slot = before + __popcll(bal & ((1ull << lane) - 1));
-		if (live && slot < RREF_FAST)
+		if (live && slot < wc)
 			rowid[slot] = row;
 		__syncthreads();
 		if (tid == 0)
 			for (int w = 0; w < nwaves; ++w)
 				s_cnt += s_wave[w];
 		__syncthreads();
-		if (s_cnt >= RREF_FAST) {
+		if (s_cnt >= wc) {
 			// rows beyond the collected ones remain unseen unless this was the end
-			scanned_all = (s_cnt == RREF_FAST && base + (int) blockDim.x >= m);
+			scanned_all = (s_cnt == wc && base + (int) blockDim.x >= m);
 			break;
 		}
 	}
-	const int nr = min(s_cnt, RREF_FAST);
-	for (int r = tid; r < RREF_FAST; r += blockDim.x)
+	const int nr = min(s_cnt, wc);
+	for (int r = tid; r < W; r += blockDim.x)
 		alive[r] = r < nr;
-	const int cc = tid % RREF_W, g = tid / RREF_W;
+	const int ci = tid % NC, g = tid / NC;
 	uint32_t v[RPG];
 #pragma unroll
 	for (int i = 0; i < RPG; ++i) {
 		const int r = g * RPG + i;
-		v[i] = (r < nr && cc < wc) ? A[(size_t) rowid[r] * lda + c + cc] : 0u;
+		if (ci < W)
+			v[i] = (r < nr && ci < wc) ? A[(size_t) rowid[r] * lda + c + ci] : 0u;
+		else
+			v[i] = (ci - W == r) ? 1u : 0u;
 	}
-	const int cnt = reg_eliminate<RREF_W, RPG, SMALL>(v, nr, wc, false, mp, alive, f, pivrow, s_min, outr, outc, st.bad);
-	const bool done = cnt == wc || scanned_all;
+	const int cnt = reg_eliminate<NC, RPG, SMALL>(v, W, wc, true, mp, alive, f, pivrow, s_min, outr, outc, st.bad);
+	const bool full = cnt == wc;
+	const bool done = full || scanned_all;
+	if (full) {
+		for (int s = tid; s < cnt; s += blockDim.x)
+			pos[outr[s]] = s;
+		__syncthreads();
+		if (ci >= W) {
+#pragma unroll
+			for (int i = 0; i < RPG; ++i) {
+				const int r = g * RPG + i;
+				if (r < wc && ci - W < wc)
+					Pinv[(size_t) pos[r] * W + pos[ci - W]] = v[i];
+			}
+		}
+		for (int e = tid; e < W * W; e += blockDim.x) {
+			const int r = e / W, q = e % W;
+			if (r >= wc || q >= wc)
+				Pinv[e] = r == q ? 1u : 0u;
+		}
+	}
 	for (int t = tid; t < RREF_W; t += blockDim.x) {
 		if (done) {
 			st.J[t] = t < cnt ? outc[t] : 0;
@@ -236,6 +273,7 @@ __global__ __launch_bounds__(1024) void rref_fast_kernel(const uint32_t *__restr
 		if (done)
 			*st.k = cnt;
 		*st.need_tree = done ? 0 : 1;
+		*st.inv_done = full ? 1 : 0;
 	}
 }
 
@@ -365,6 +403,8 @@ __global__ __launch_bounds__(1024) void rref_inv_kernel(const uint32_t *__restri
 	__shared__ int alive[W], outr[W], outc[W], pos[W];
 	__shared__ uint32_t f[W], pivrow[NC];
 	__shared__ int s_min[2];
+	if (*st.inv_done)
+		return;
 	const int tid = threadIdx.x, c = tid % NC, g = tid / NC;
 	const int k = *st.k;
 	uint32_t v[RPG];
@@ -465,6 +505,7 @@ __global__ void rref_init_kernel(RrefState st, int m, int cnt_counters)
 		*st.k = 0;
 		*st.bad = 0;
 		*st.need_tree = 0;
+		*st.inv_done = 0;
 	}
 }
 
@@ -489,7 +530,7 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 	MLZ_TRY(ws_get(ctx, WS_RREF_1, 2 * (size_t) RREF_W * n * sizeof(uint32_t) +
 				      (size_t) RREF_W * RREF_W * sizeof(uint32_t), &w1));  // P, W, Pinv
 	MLZ_TRY(ws_get(ctx, WS_RREF_2, m * RREF_W * sizeof(uint32_t), &w2));         // G
-	const size_t ints = m + 1 + 2 * rmax + 1 + 2 * RREF_W + 2 + (size_t) RREF_MAXLVL * nodes0 * RREF_W +
+	const size_t ints = m + 1 + 2 * rmax + 1 + 2 * RREF_W + 3 + (size_t) RREF_MAXLVL * nodes0 * RREF_W +
 			    2 * (size_t) RREF_MAXLVL * nodes0;
 	MLZ_TRY(ws_get(ctx, WS_RREF_3, ints * sizeof(int), &w3));
 	uint32_t *Wm = (uint32_t *) w0;
@@ -506,6 +547,7 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 	st.prow = q; q += RREF_W;
 	st.bad = q; q += 1;
 	st.need_tree = q; q += 1;
+	st.inv_done = q; q += 1;
 	st.lists_rows = q; q += (size_t) RREF_MAXLVL * nodes0 * RREF_W;
 	st.lists_cnt = q; q += (size_t) RREF_MAXLVL * nodes0;
 	st.counters = q;
@@ -523,6 +565,8 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 			rref_load_kernel<MLZ_F64_SYM><<<g, 256, 0, ctx->stream>>>((const double *) A, lda, (int) m, (int) n, pp, Wm);
 		MLZ_TRY(check_launch(ctx, "rref_load_kernel"));
 	}
+	// trailing updates on INT8 limbs whenever the gemm family allows them
+	const bool upd_i8 = p <= 65536 && (algo == MLZ_ALGO_AUTO || algo == MLZ_ALGO_INT8);
 	int panel = 0;
 	for (size_t c = 0; c < n; c += RREF_W, ++panel) {
 		const int wc = (int) std::min((size_t) RREF_W, n - c);
@@ -530,10 +574,10 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 		{
 			ProfScope ps(ctx, "rref_pivots");
 			if (mp.small) {
-				rref_fast_kernel<true><<<1, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st);
+				rref_fast_kernel<true><<<1, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, Pinv);
 				rref_tourney_kernel<true><<<nodes0, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, nodes0);
 			} else {
-				rref_fast_kernel<false><<<1, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st);
+				rref_fast_kernel<false><<<1, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, Pinv);
 				rref_tourney_kernel<false><<<nodes0, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, nodes0);
 			}
 		}
@@ -550,7 +594,10 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 		MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, RREF_W, nc, RREF_W, 1, Pinv, RREF_W, P, nc, 0, Wb, nc, algo));
 		rref_gatherG_kernel<<<ceil_div(m, 256 / RREF_W), 256, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, panel, st, G);
 		MLZ_TRY(check_launch(ctx, "rref_gatherG_kernel"));
-		MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, m, nc, RREF_W, p - 1, G, RREF_W, Wb, nc, 1, Wm + c, n, algo));
+		if (upd_i8)
+			MLZ_TRY(zp_rank_update_i8_dev(ctx, pp, m, nc, RREF_W, G, RREF_W, Wb, nc, Wm + c, n));
+		else
+			MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, m, nc, RREF_W, p - 1, G, RREF_W, Wb, nc, 1, Wm + c, n, algo));
 		rref_scatter_kernel<<<dim3(ceil_div(nc, 256), RREF_W), 256, 0, ctx->stream>>>(Wm, n, nc, (int) c, st, Wb);
 		rref_advance_kernel<<<1, 1, 0, ctx->stream>>>(st);
 		MLZ_TRY(check_launch(ctx, "rref_scatter_kernel"));
@@ -758,28 +805,34 @@ __global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref 
 		int rank = 0;
 		for (int c = 0; c < n && rank < m; c += W) {
 			const int wc = min(W, n - c), nc = n - c;
-			// ---- 1. pivot search
+			// ---- 1. pivot search.  Round 0 reduces [S | I] for S = the first wc
+			// live rows; full panel rank gives the pivot rows and P_J^-1 at once.
+			// Otherwise rounds of up to BR_CH rows from the top find the rank
+			// profile, and step 2 inverts P_J.
 			int cnt = 0;
-			for (bool first = true;; first = false) {
+			bool have_inv = false;
+			for (int round = 0;; ++round) {
+				const bool fast = round == 0;
+				const int lim = fast ? wc : BR_CH;
 				if (wave == 0) {
-					int have = cnt, cur = first ? 0 : S.cursor;
-					while (have < BR_CH && cur < m) {
+					int have = fast || round == 1 ? 0 : cnt, cur = fast || round == 1 ? 0 : S.cursor;
+					while (have < lim && cur < m) {
 						const int row = cur + lane;
 						const bool live = row < m && S.is_piv[row] == 0;
 						const uint64_t bal = __ballot(live);
 						const int before = __popcll(bal & ((1ull << lane) - 1ull));
-						const int room = BR_CH - have, tot = __popcll(bal);
+						const int room = lim - have, tot = __popcll(bal);
 						if (live && before < room)
 							S.rowid[have + before] = row;
 						if (tot <= room) {
 							have += tot;
 							cur += 64;
 						} else {
-							uint64_t b = bal;
+							uint64_t bb = bal;
 							for (int q = 0; q < room - 1; ++q)
-								b &= b - 1;
-							have = BR_CH;
-							cur += __ffsll((unsigned long long) b);
+								bb &= bb - 1;
+							have = lim;
+							cur += __ffsll((unsigned long long) bb);
 						}
 					}
 					if (lane == 0) {
@@ -790,6 +843,48 @@ __global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref 
 				__syncthreads();
 				const int nr = S.nr;
 				const bool seen_all = S.cursor >= m;
+				if (fast) {
+					const int ci = tid % NCI, gi = tid / NCI;
+					uint32_t v[RPGI];
+#pragma unroll
+					for (int i = 0; i < RPGI; ++i) {
+						const int r = gi * RPGI + i;
+						if (ci < W)
+							v[i] = (r < nr && ci < wc) ? Wm[(size_t) S.rowid[r] * n + c + ci] : 0u;
+						else
+							v[i] = (ci - W == r) ? 1u : 0u;
+					}
+					for (int r = tid; r < W; r += BR_THREADS)
+						S.alive[r] = r < nr;
+					const int kf = reg_eliminate<NCI, RPGI, SMALL>(v, W, wc, true, mp, S.alive, S.f, S.pivrow,
+										       S.s_min, S.outr, S.outc, badp);
+					if (kf == wc) {
+						// P_J = Q S, P_J^-1 = S^-1 Q^T: entry (c, t) = R[row of pivot c][row of pivot t]
+						for (int q = tid; q < kf; q += BR_THREADS) {
+							S.pos[S.outr[q]] = q;
+							S.tmp[q] = S.rowid[S.outr[q]];
+						}
+						__syncthreads();
+						if (ci >= W) {
+#pragma unroll
+							for (int i = 0; i < RPGI; ++i) {
+								const int r = gi * RPGI + i;
+								if (r < wc && ci - W < wc)
+									S.Pinv[S.pos[r]][S.pos[ci - W]] = v[i];
+							}
+						}
+						for (int q = tid; q < kf; q += BR_THREADS) {
+							S.rowid[q] = S.tmp[q];
+							S.J[q] = S.outc[q];
+						}
+						__syncthreads();
+						cnt = kf;
+						have_inv = true;
+						break;
+					}
+					__syncthreads();
+					continue;   // rank deficient: search from the top
+				}
 				for (int r = tid; r < BR_CH; r += BR_THREADS)
 					S.alive[r] = r < nr;
 				{
@@ -803,12 +898,12 @@ __global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref 
 					cnt = reg_eliminate<W, RPG, SMALL>(v, nr, wc, false, mp, S.alive, S.f, S.pivrow, S.s_min,
 									   S.outr, S.outc, badp);
 				}
-				for (int s = tid; s < cnt; s += BR_THREADS)
-					S.tmp[s] = S.rowid[S.outr[s]];
+				for (int q = tid; q < cnt; q += BR_THREADS)
+					S.tmp[q] = S.rowid[S.outr[q]];
 				__syncthreads();
-				for (int s = tid; s < cnt; s += BR_THREADS) {
-					S.rowid[s] = S.tmp[s];
-					S.J[s] = S.outc[s];
+				for (int q = tid; q < cnt; q += BR_THREADS) {
+					S.rowid[q] = S.tmp[q];
+					S.J[q] = S.outc[q];
 				}
 				__syncthreads();
 				if (cnt == wc || seen_all)
@@ -817,8 +912,8 @@ __global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref 
 			const int k = cnt;
 			if (k == 0)
 				continue;
-			// ---- 2. P_J^-1, then Wb = P_J^-1 * P
-			{
+			// ---- 2. P_J^-1 (unless round 0 produced it), then Wb = P_J^-1 * P
+			if (!have_inv) {
 				const int ci = tid % NCI, gi = tid / NCI;
 				uint32_t v[RPGI];
 #pragma unroll
@@ -839,8 +934,8 @@ __global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref 
 									       S.outr, S.outc, badp);
 				if (tid == 0 && ki < k)
 					*badp = 1;
-				for (int s = tid; s < ki; s += BR_THREADS)
-					S.pos[S.outr[s]] = S.outc[s];
+				for (int q = tid; q < ki; q += BR_THREADS)
+					S.pos[S.outr[q]] = S.outc[q];
 				__syncthreads();
 				if (ci >= W) {
 #pragma unroll

@@ -16,7 +16,7 @@
 
 namespace mlz {
 
-constexpr int TRSM_LEAF = 64;
+constexpr int TRSM_LEAF = (int) TRSM_LEAF_ROWS;
 
 // Modular inverse by the extended Euclidean algorithm; 0 if gcd(y, p) != 1.
 __device__ inline int inv_mod(uint32_t y, uint32_t p, uint32_t *out)
@@ -201,14 +201,12 @@ static int trsm_rec(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, siz
 	return MLZ_OK;
 }
 
-int zp_trsm_dev(mlz_ctx *ctx, int elem, uint64_t p, int upper, int unit, size_t n, size_t nrhs,
-		uint64_t a, const void *A, size_t lda, void *B, size_t ldb, size_t *bad_diag, int algo)
+// Diagonal checks and the inverses of all TRSM_LEAF-row diagonal blocks of A
+// (n x n): on MLZ_OK *W points at workspace holding block b's inverse in rows
+// b*TRSM_LEAF .., valid until the next call that uses WS_TRSM_0.
+int zp_trsm_prepare(mlz_ctx *ctx, int elem, uint64_t p, int upper, int unit, size_t n, const void *A, size_t lda,
+		    size_t *bad_diag, const uint32_t **W_out)
 {
-	if (n == 0 || nrhs == 0)
-		return MLZ_OK;
-	a %= p;
-	if (a == 0)                                  // level3-generic.tcc:434-435
-		return zp_scal_dev(ctx, elem, p, n, nrhs, 0, B, ldb);
 	const size_t nblk = (n + TRSM_LEAF - 1) / TRSM_LEAF;
 	void *w = nullptr;
 	const size_t dinv_bytes = round_up(n * sizeof(uint32_t), 256);
@@ -240,17 +238,46 @@ int zp_trsm_dev(mlz_ctx *ctx, int elem, uint64_t p, int upper, int unit, size_t 
 			return set_error(ctx, MLZ_E_NOT_INVERTIBLE, "diagonal entry %zu not invertible", idx);
 		}
 	}
-	const uint32_t pp = (uint32_t) p, aa = (uint32_t) a;
-	if (elem == MLZ_U16) {
+	const uint32_t pp = (uint32_t) p;
+	if (elem == MLZ_U16)
 		MLZ_TRY(trsm_diag_inverses<MLZ_U16>(ctx, pp, upper, unit, n, A, lda, dinv, W));
-		return trsm_rec<MLZ_U16>(ctx, pp, upper, unit, n, nrhs, aa, (const uint16_t *) A, lda, (uint16_t *) B, ldb, W, algo);
-	}
-	if (elem == MLZ_U32) {
+	else if (elem == MLZ_U32)
 		MLZ_TRY(trsm_diag_inverses<MLZ_U32>(ctx, pp, upper, unit, n, A, lda, dinv, W));
+	else
+		MLZ_TRY(trsm_diag_inverses<MLZ_F64_SYM>(ctx, pp, upper, unit, n, A, lda, dinv, W));
+	*W_out = W;
+	return MLZ_OK;
+}
+
+// The solve proper with the block inverses W of zp_trsm_prepare (for a
+// diagonal sub-block starting at row i0 of the prepared matrix, pass
+// W + i0 * TRSM_LEAF; i0 a multiple of TRSM_LEAF).
+int zp_trsm_apply(mlz_ctx *ctx, int elem, uint64_t p, int upper, int unit, size_t n, size_t nrhs, uint64_t a,
+		  const void *A, size_t lda, void *B, size_t ldb, const uint32_t *W, int algo)
+{
+	if (n == 0 || nrhs == 0)
+		return MLZ_OK;
+	a %= p;
+	if (a == 0)                                  // level3-generic.tcc:434-435
+		return zp_scal_dev(ctx, elem, p, n, nrhs, 0, B, ldb);
+	const uint32_t pp = (uint32_t) p, aa = (uint32_t) a;
+	if (elem == MLZ_U16)
+		return trsm_rec<MLZ_U16>(ctx, pp, upper, unit, n, nrhs, aa, (const uint16_t *) A, lda, (uint16_t *) B, ldb, W, algo);
+	if (elem == MLZ_U32)
 		return trsm_rec<MLZ_U32>(ctx, pp, upper, unit, n, nrhs, aa, (const uint32_t *) A, lda, (uint32_t *) B, ldb, W, algo);
-	}
-	MLZ_TRY(trsm_diag_inverses<MLZ_F64_SYM>(ctx, pp, upper, unit, n, A, lda, dinv, W));
 	return trsm_rec<MLZ_F64_SYM>(ctx, pp, upper, unit, n, nrhs, aa, (const double *) A, lda, (double *) B, ldb, W, algo);
+}
+
+int zp_trsm_dev(mlz_ctx *ctx, int elem, uint64_t p, int upper, int unit, size_t n, size_t nrhs,
+		uint64_t a, const void *A, size_t lda, void *B, size_t ldb, size_t *bad_diag, int algo)
+{
+	if (n == 0 || nrhs == 0)
+		return MLZ_OK;
+	if (a % p == 0)                              // level3-generic.tcc:434-435
+		return zp_scal_dev(ctx, elem, p, n, nrhs, 0, B, ldb);
+	const uint32_t *W = nullptr;
+	MLZ_TRY(zp_trsm_prepare(ctx, elem, p, upper, unit, n, A, lda, bad_diag, &W));
+	return zp_trsm_apply(ctx, elem, p, upper, unit, n, nrhs, a, A, lda, B, ldb, W, algo);
 }
 
 }  // namespace mlz

@@ -69,6 +69,28 @@ def test_fl_small_config4_shape(gpu_ctx, algo):
     assert det == x_scan_det(X, p)
 
 
+@pytest.mark.parametrize("algo", [la.ALGO_AUTO, la.ALGO_PLAIN])
+def test_fl_multiblock_banded_pivot_block(gpu_ctx, algo):
+    """1000 pivot rows (two blocks of the pivot-block solve), banded A."""
+    p = 65521
+    csr, shape = synth.fl_matrix(13, p, na=1000, nc=300, nb=1000, band=90, nnz_a=20)
+    X = synth.csr_to_dense(csr, shape)
+    got, rank, det, _ = run_fl(gpu_ctx, p, "uint32", X, algo=algo)
+    rc, want, wrank, _, _ = orc.fl_echelonize(p, X)
+    assert rc == 0 and rank == wrank
+    assert np.array_equal(got, want)
+
+
+def test_fl_multiblock_dense_pivot_block(gpu_ctx):
+    """A dense upper-triangular pivot block spanning several solve blocks."""
+    p = 65521
+    X = f4_matrix(14, 1100, 1300, p)
+    got, rank, det, _ = run_fl(gpu_ctx, p, "uint32", X)
+    rc, want, wrank, _, _ = orc.fl_echelonize(p, X)
+    assert rc == 0 and rank == wrank and rank > 600
+    assert np.array_equal(got, want)
+
+
 def test_fl_wrong_matrix_form(gpu_ctx):
     p = 101
     X = np.zeros((4, 6), dtype=np.uint32)
