@@ -194,7 +194,9 @@ enum { MLZ_TASK_GEMM = 1, MLZ_TASK_RREF = 2 };
 #define MLZ_MAX_VARIANTS 4
 /* gemm variants: 0 plain VALU 64x64, 1 sparse-A VALU 16x256, 2 INT8 MFMA
  * 64x64, 3 INT8 MFMA 128x128 (2 and 3 need p <= 2^16 and k <= 65504).
- * rref variants: 0 panels of 64 columns, 1 panels of 32 columns (m <= 4096). */
+ * rref variants (m <= 4096, one workgroup per instance): 0 panels of 64
+ * columns, 1 panels of 32 columns (VALU updates); 2 and 3 the same with the
+ * trailing updates on INT8 MFMA (p <= 2^16). */
 typedef struct mlz_task {
 	int op;                /* MLZ_TASK_GEMM or MLZ_TASK_RREF */
 	int variant;           /* in: -1 = chosen by the policy, else forced; out: the variant run */

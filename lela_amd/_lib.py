@@ -83,7 +83,7 @@ MODEL_PARAMS = 1380
 POLICY_FIXED, POLICY_TABLE, POLICY_MODEL = 0, 1, 2
 BATCH_GRAPH, BATCH_TIMING, BATCH_SERIAL = 1, 2, 4
 GEMM_PLAIN, GEMM_SPARSE, GEMM_I8S, GEMM_I8L = 0, 1, 2, 3
-RREF_W64, RREF_W32 = 0, 1
+RREF_W64, RREF_W32, RREF_I8_W64, RREF_I8_W32 = 0, 1, 2, 3
 
 
 class Task(ctypes.Structure):

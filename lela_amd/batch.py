@@ -19,12 +19,14 @@ import numpy as np
 from . import _lib
 from ._lib import (BATCH_GRAPH, BATCH_SERIAL, BATCH_TIMING, GEMM_I8L, GEMM_I8S, GEMM_PLAIN,
                    GEMM_SPARSE, MAX_VARIANTS, MODEL_PARAMS, POLICY_FIXED, POLICY_MODEL,
-                   POLICY_TABLE, PTR_DEVICE, RREF_W32, RREF_W64, TASK_GEMM, TASK_RREF, Task)
+                   POLICY_TABLE, PTR_DEVICE, RREF_I8_W32, RREF_I8_W64, RREF_W32, RREF_W64, TASK_GEMM,
+                   TASK_RREF, Task)
 
 __all__ = [
     "Instance", "Workload", "Plan", "tune", "table_get", "table_set", "task_features", "CostModel",
     "TASK_GEMM", "TASK_RREF", "POLICY_FIXED", "POLICY_TABLE", "POLICY_MODEL", "GEMM_PLAIN",
-    "GEMM_SPARSE", "GEMM_I8S", "GEMM_I8L", "RREF_W64", "RREF_W32", "MAX_VARIANTS",
+    "GEMM_SPARSE", "GEMM_I8S", "GEMM_I8L", "RREF_W64", "RREF_W32", "RREF_I8_W64", "RREF_I8_W32",
+    "MAX_VARIANTS",
     "MODEL_PARAMS", "nvariants", "variant_ok", "fit",
 ]
 
@@ -32,7 +34,7 @@ _ALIGN = 256
 
 
 def nvariants(op):
-    return 4 if op == TASK_GEMM else 2
+    return 4
 
 
 def variant_ok(op, v, p, k=1):
@@ -40,7 +42,9 @@ def variant_ok(op, v, p, k=1):
         if v in (GEMM_I8S, GEMM_I8L):
             return p <= 65536 and k <= 65504
         return 0 <= v < 4
-    return 0 <= v < 2
+    if v in (RREF_I8_W64, RREF_I8_W32):
+        return p <= 65536
+    return 0 <= v < 4
 
 
 class Instance:

@@ -13,7 +13,8 @@
 // Execution of a plan:
 //   zero counters -> fork -> branch 0: INT8 pack, INT8 64x64, INT8 128x128
 //                            branch 1: plain      branch 2: sparse-A
-//                            branch 3: rref W64   branch 4: rref W32
+//                            branch 3: rref 64-column panels (VALU, MFMA)
+//                            branch 4: rref 32-column panels (VALU, MFMA)
 //                 -> join
 // The branches are independent streams joined by events, so in a captured
 // graph they are parallel nodes; MLZ_BATCH_SERIAL puts everything on the
@@ -94,6 +95,8 @@ static bool variant_ok(const mlz_task &t, int v, uint64_t p)
 			return p <= 65536 && t.k <= 65504;
 		return true;
 	}
+	if (v == BR_I8_W64 || v == BR_I8_W32)
+		return p <= 65536;
 	return v >= 0 && v < BR_NVAR;
 }
 
@@ -101,7 +104,7 @@ static bool variant_ok(const mlz_task &t, int v, uint64_t p)
 static int rule_variant(const mlz_task &t, uint64_t p)
 {
 	if (t.op == MLZ_TASK_RREF)
-		return BR_W64;
+		return p <= 65536 ? BR_I8_W64 : BR_W32;
 	if (p <= 65536 && t.k <= 65504)
 		return (uint64_t) t.m * t.n * t.k >= ((uint64_t) 1 << 24) ? BG_I8L : BG_I8S;
 	return BG_PLAIN;
@@ -394,7 +397,7 @@ static int plan_build(mlz_ctx *ctx, int elem, uint64_t p, mlz_task *tasks, size_
 	const size_t rdesc_off = lay.take(nr * sizeof(BatchRref));
 	std::vector<BatchGemm> gd(ng);
 	std::vector<BatchRref> rd(nr);
-	std::vector<size_t> ap_off(nt, 0), bp_off(nt, 0), wm_off(nt, 0), wb_off(nt, 0), pr_off(nt, 0);
+	std::vector<size_t> ap_off(nt, 0), bp_off(nt, 0), wm_off(nt, 0), wb_off(nt, 0), wbp_off(nt, 0), pr_off(nt, 0);
 	for (size_t oi = 0; oi < nt; ++oi) {
 		const size_t i = order[oi];
 		const mlz_task &t = tasks[i];
@@ -443,6 +446,7 @@ static int plan_build(mlz_ctx *ctx, int elem, uint64_t p, mlz_task *tasks, size_
 			r.n = (int) t.n;
 			wm_off[i] = lay.take((size_t) t.m * t.n * 4);
 			wb_off[i] = lay.take((size_t) br_width(v) * t.n * 4);
+			wbp_off[i] = br_mfma(v) ? lay.take(br_wbp_bytes(v, t.n)) : 0;
 			pr_off[i] = lay.take(2 * (size_t) std::min(t.m, t.n) * 4);
 			items[RR0 + v].push_back({(int) rref_of[i], 0, 0, 0});
 		}
@@ -469,6 +473,7 @@ static int plan_build(mlz_ctx *ctx, int elem, uint64_t p, mlz_task *tasks, size_
 			BatchRref &r = rd[rref_of[i]];
 			r.Wm = (uint32_t *) (A0 + wm_off[i]);
 			r.Wb = (uint32_t *) (A0 + wb_off[i]);
+			r.Wbp = br_mfma(tasks[i].variant) ? (int8_t *) (A0 + wbp_off[i]) : nullptr;
 			r.piv_rows = (int *) (A0 + pr_off[i]);
 			r.piv_cols = r.piv_rows + std::min(tasks[i].m, tasks[i].n);
 		}
@@ -528,7 +533,7 @@ static int plan_build(mlz_ctx *ctx, int elem, uint64_t p, mlz_task *tasks, size_
 			add(1, v, gemm_branch[v], v, tc_gemm, (int) b->occ_gemm[v]);
 	for (int v = 0; v < BR_NVAR; ++v)
 		if (!items[RR0 + v].empty())
-			add(2, v, 3 + v, RR0 + v, tc_rref, (int) b->occ_rref[v]);
+			add(2, v, 3 + (v & 1), RR0 + v, tc_rref, (int) b->occ_rref[v]);
 	(void) bad_rref;
 
 	if (!(flags & MLZ_BATCH_SERIAL)) {

@@ -15,11 +15,14 @@ enum {
 	BG_I8L = 3,      // INT8 MFMA limbs, 128 x 128 tiles (packed operands)
 	BG_NVAR = 4
 };
-// Kernel variants an echelon-form instance can run with.
+// Kernel variants an echelon-form instance can run with (one workgroup per
+// instance; the trailing update of each panel on VALU or on INT8 MFMA).
 enum {
-	BR_W64 = 0,      // one workgroup per instance, 64-column panels
-	BR_W32 = 1,      // one workgroup per instance, 32-column panels
-	BR_NVAR = 2
+	BR_W64 = 0,      // 64-column panels, FP64 (or uint64) VALU update
+	BR_W32 = 1,      // 32-column panels, VALU update
+	BR_I8_W64 = 2,   // 64-column panels, INT8 MFMA update (p <= 2^16)
+	BR_I8_W32 = 3,   // 32-column panels, INT8 MFMA update (p <= 2^16)
+	BR_NVAR = 4
 };
 
 constexpr int BR_MAXM = 4096;    // rows of a batched echelon-form instance (pivot flags in LDS)
@@ -43,6 +46,7 @@ struct BatchRref {
 	uint32_t *pivots;      // caller's pivot-column array (min(m, n)) or null
 	uint32_t *Wm;          // m x n canonical working copy
 	uint32_t *Wb;          // panel-width x n rows of P_J^-1 * P
+	int8_t *Wbp;           // Wb as INT8 limb fragments (MFMA variants), else null
 	int *piv_rows, *piv_cols;   // min(m, n) each
 	int m, n;
 };
@@ -94,6 +98,9 @@ int batch_rref_wgs_per_cu(int elem, int variant, uint32_t p);
 // Tile geometry of the gemm variants.
 inline int bg_tile_m(int v) { return v == BG_SPARSE ? 16 : v == BG_I8L ? 128 : 64; }
 inline int bg_tile_n(int v) { return v == BG_SPARSE ? 256 : v == BG_I8L ? 128 : 64; }
-inline int br_width(int v) { return v == BR_W32 ? 32 : 64; }
+inline int br_width(int v) { return v == BR_W32 || v == BR_I8_W32 ? 32 : 64; }
+inline bool br_mfma(int v) { return v == BR_I8_W64 || v == BR_I8_W32; }
+// Bytes of the packed Wb of an MFMA variant: [32-column block][k block][limb][64 lanes][16 bytes].
+inline size_t br_wbp_bytes(int v, size_t n) { return (size_t) br_width(v) * ((n + 31) / 32 * 32) * 2; }
 
 }  // namespace mlz

@@ -191,6 +191,16 @@ inline ModP make_modp(uint64_t p)
 	return m;
 }
 
+// Signed 8-bit limbs of a residue v < p <= 2^16 for the INT8 MFMA kernels:
+// u = x + 32896 with x the representative in [-32896, 32639] (v itself if
+// v <= 32639, else v - p); the bytes of u ^ 0x8080 are the limbs (hi, lo), so
+// x = 256*hi + lo with both in [-128, 127].  Zero maps to (0, 0).
+__device__ inline uint32_t i8_limbs(uint32_t v, uint32_t p)
+{
+	int32_t x = v > 32639u ? (int32_t) v - (int32_t) p : (int32_t) v;
+	return (uint32_t) (x + 32896) ^ 0x8080u;
+}
+
 // Element storage <-> canonical uint32 in [0, p).
 template <int E> struct Elem;
 template <> struct Elem<MLZ_U16> {

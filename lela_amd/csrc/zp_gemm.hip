@@ -38,14 +38,6 @@ namespace mlz {
 constexpr int I8_FRAG = 1024;
 constexpr int I8_KMAX = 65504;   // multiple of 64, <= 65535
 
-__device__ inline uint32_t i8_limbs(uint32_t v, uint32_t p)
-{
-	// u = x + 32896 with x the representative in [-32896, 32639]; then the
-	// bytes of u ^ 0x8080 are the signed limbs (hi, lo).
-	int32_t x = v > 32639u ? (int32_t) v - (int32_t) p : (int32_t) v;
-	return (uint32_t) (x + 32896) ^ 0x8080u;
-}
-
 // Packs a 64 (index) x 64 (k) tile into 2 x 2 fragment pairs.
 // TRANS = false: element(i, kk) = src[i*ld + kk]   (A, row-major m x k)
 // TRANS = true : element(i, kk) = src[kk*ld + i]   (B, row-major k x n)

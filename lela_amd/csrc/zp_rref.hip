@@ -639,7 +639,9 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 }
 
 // ===========================================================================
-// Batched echelon forms: one workgroup of BR_THREADS threads per instance.
+// Batched echelon forms: one workgroup per instance (NT threads: 1024 for the
+// VALU-update variants, 512 for the MFMA ones, whose accumulators need the
+// larger register budget).
 //
 // Instances of a batch are small (up to BR_MAXM rows) and many, so each one
 // runs start to finish inside one workgroup taken from the work list, with
@@ -660,15 +662,17 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 // precision when W*(p-1)^2 < 2^53 (FMA, full-rate FP64), else in uint64 with
 // the 2^64-wrap correction.
 // ===========================================================================
-constexpr int BR_THREADS = 1024;
 constexpr int BR_CH = 128;   // rows per pivot-search elimination
 constexpr int BR_RB = 64;    // rows per update block (multipliers in LDS)
 constexpr int BR_KC = 8;     // multipliers per register chunk
 constexpr int BR_RQ = 8;     // rows per update unit of one wave
 
-template <int W, bool FMA>
+template <int W, bool FMA, bool I8>
 struct BrLds {
 	typedef typename std::conditional<FMA, double, uint32_t>::type GT;
+	// MFMA update: the 64-row block's multipliers as INT8 A fragments,
+	// [32-row block][k block][limb][64 lanes][16 bytes]
+	__attribute__((aligned(16))) int8_t Ga[I8 ? BR_RB / 32 : 1][I8 ? W / 32 : 1][2][64][16];
 	int rowid[BR_CH], alive[BR_CH], tmp[BR_CH];
 	uint32_t f[BR_CH];
 	uint32_t pivrow[2 * W];
@@ -698,14 +702,14 @@ __device__ __forceinline__ uint32_t br_mod_f64(double x, const ModP &mp)
 // x 64 columns are spread over the waves; a lane owns one column and keeps
 // BR_KC source values in registers while the multipliers (uniform across the
 // wave) are read from LDS.
-template <int W, bool FMA, bool SUB>
-__device__ __forceinline__ void br_apply(BrLds<W, FMA> &S, int k, int nrows, int ncols, const uint32_t *src,
+template <int NT, int W, bool FMA, bool I8, bool SUB>
+__device__ __forceinline__ void br_apply(BrLds<W, FMA, I8> &S, int k, int nrows, int ncols, const uint32_t *src,
 					 uint32_t *out, size_t ld, const ModP &mp, uint64_t two_64)
 {
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int nchunks = (ncols + 63) >> 6, nq = (nrows + BR_RQ - 1) / BR_RQ;
 	const uint32_t p = (uint32_t) mp.p;
-	for (int u = wave; u < nchunks * nq; u += BR_THREADS / 64) {
+	for (int u = wave; u < nchunks * nq; u += NT / 64) {
 		const int rq = u / nchunks, jc = u - rq * nchunks;
 		const int j = jc * 64 + lane, r0 = rq * BR_RQ;
 		const bool colok = j < ncols;
@@ -774,16 +778,86 @@ __device__ __forceinline__ void br_apply(BrLds<W, FMA> &S, int k, int nrows, int
 	}
 }
 
-template <int E, int W, bool SMALL, bool FMA>
-__global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref *__restrict__ tasks, BatchLaunch L,
+// Wb (k x nc, rows k..W-1 taken as zero) -> INT8 B fragments
+// [32-column block][k block][limb][64 lanes][16 bytes]: lane l holds column
+// l%32 and k-values 16*(l/32) .. +15 of its k block.
+template <int NT, int W>
+__device__ __forceinline__ void br_pack_wb(const uint32_t *__restrict__ Wb, size_t n, int k, int nc, uint32_t p,
+					   int8_t *__restrict__ Wbp)
+{
+	constexpr int KB = W / 32;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int ncb = (nc + 31) / 32;
+	for (int u = wave; u < ncb * KB; u += NT / 64) {
+		const int cb = u / KB, kb = u - cb * KB;
+		const int col = cb * 32 + (lane & 31), k0 = kb * 32 + 16 * (lane >> 5);
+		uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
+#pragma unroll
+		for (int q = 0; q < 16; ++q) {
+			const int kk = k0 + q;
+			const uint32_t x = (kk < k && col < nc) ? Wb[(size_t) kk * n + col] : 0u;
+			const uint32_t l = i8_limbs(x, p);
+			hw[q >> 2] |= ((l >> 8) & 0xffu) << (8 * (q & 3));
+			lw[q >> 2] |= (l & 0xffu) << (8 * (q & 3));
+		}
+		int8_t *fr = Wbp + (size_t) (cb * KB + kb) * 2 * 1024 + lane * 16;
+		*(v4i *) fr = v4i{(int) hw[0], (int) hw[1], (int) hw[2], (int) hw[3]};
+		*(v4i *) (fr + 1024) = v4i{(int) lw[0], (int) lw[1], (int) lw[2], (int) lw[3]};
+	}
+}
+
+// C (nb <= BR_RB rows x nc columns, row stride n) -= G * Wb on INT8 MFMA: G
+// from S.Ga, Wb from Wbp.  Units of 32 x 32 outputs are spread over the
+// waves; rows whose multipliers are all zero are not written.
+template <int NT, int W, bool FMA>
+__device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int nc, const int8_t *__restrict__ Wbp,
+					     uint32_t *__restrict__ C, size_t n, const ModP &mp)
+{
+	constexpr int KB = W / 32;
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int ncb = (nc + 31) / 32, nrb = (nb + 31) / 32;
+	const uint32_t p = (uint32_t) mp.p;
+	for (int u = wave; u < nrb * ncb; u += NT / 64) {
+		const int rb = u / ncb, cb = u - rb * ncb;
+		v16i acc0 = v16i{0}, acc1 = v16i{0}, acc2 = v16i{0};
+#pragma unroll
+		for (int kb = 0; kb < KB; ++kb) {
+			const v4i ah = *(const v4i *) &S.Ga[rb][kb][0][lane][0];
+			const v4i al = *(const v4i *) &S.Ga[rb][kb][1][lane][0];
+			const int8_t *gb = Wbp + (size_t) (cb * KB + kb) * 2 * 1024 + lane * 16;
+			const v4i bh = *(const v4i *) gb, bl = *(const v4i *) (gb + 1024);
+			acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bh, acc0, 0, 0, 0);
+			acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bl, acc1, 0, 0, 0);
+			acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl, acc2, 0, 0, 0);
+			acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, acc1, 0, 0, 0);
+		}
+		const int col = cb * 32 + (lane & 31);
+		if (col >= nc)
+			continue;
+#pragma unroll
+		for (int r = 0; r < 16; ++r) {
+			const int row = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+			if (row < nb && S.rownz[row]) {
+				const double v = fma((double) acc0[r], 65536.0, fma((double) acc1[r], 256.0, (double) acc2[r]));
+				const uint32_t d = mp.mod_f64(v);
+				uint32_t *cp = C + (size_t) row * n + col;
+				const uint32_t x = *cp;
+				*cp = x >= d ? x - d : x + p - d;
+			}
+		}
+	}
+}
+
+template <int E, int W, bool SMALL, bool FMA, bool I8, int NT>
+__global__ __launch_bounds__(NT) void batch_rref_kernel(const BatchRref *__restrict__ tasks, BatchLaunch L,
 								ModP mp, uint64_t two_64, int *__restrict__ ranks,
 								int *__restrict__ bad)
 {
 	typedef typename Elem<E>::T T;
-	typedef typename BrLds<W, FMA>::GT GT;
-	constexpr int RPG = BR_CH / (BR_THREADS / W);             // pivot search: rows per thread
-	constexpr int NCI = 2 * W, RPGI = W / (BR_THREADS / NCI);  // inverse: rows per thread
-	__shared__ BrLds<W, FMA> S;
+	typedef typename BrLds<W, FMA, I8>::GT GT;
+	constexpr int RPG = BR_CH / (NT / W);             // pivot search: rows per thread
+	constexpr int NCI = 2 * W, RPGI = W / (NT / NCI);  // inverse: rows per thread
+	__shared__ BrLds<W, FMA, I8> S;
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const uint32_t p = (uint32_t) mp.p;
 	for (;;) {
@@ -796,10 +870,10 @@ __global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref 
 		const int m = t.m, n = t.n;
 		uint32_t *Wm = t.Wm, *Wb = t.Wb;
 		int *badp = bad + w.task;
-		for (int r = wave; r < m; r += BR_THREADS / 64)
+		for (int r = wave; r < m; r += NT / 64)
 			for (int c = lane; c < n; c += 64)
 				Wm[(size_t) r * n + c] = Elem<E>::load((const T *) t.A + (size_t) r * t.lda + c, p);
-		for (int r = tid; r < m; r += BR_THREADS)
+		for (int r = tid; r < m; r += NT)
 			S.is_piv[r] = 0;
 		__syncthreads();
 		int rank = 0;
@@ -854,13 +928,13 @@ __global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref 
 						else
 							v[i] = (ci - W == r) ? 1u : 0u;
 					}
-					for (int r = tid; r < W; r += BR_THREADS)
+					for (int r = tid; r < W; r += NT)
 						S.alive[r] = r < nr;
 					const int kf = reg_eliminate<NCI, RPGI, SMALL>(v, W, wc, true, mp, S.alive, S.f, S.pivrow,
 										       S.s_min, S.outr, S.outc, badp);
 					if (kf == wc) {
 						// P_J = Q S, P_J^-1 = S^-1 Q^T: entry (c, t) = R[row of pivot c][row of pivot t]
-						for (int q = tid; q < kf; q += BR_THREADS) {
+						for (int q = tid; q < kf; q += NT) {
 							S.pos[S.outr[q]] = q;
 							S.tmp[q] = S.rowid[S.outr[q]];
 						}
@@ -873,7 +947,7 @@ __global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref 
 									S.Pinv[S.pos[r]][S.pos[ci - W]] = v[i];
 							}
 						}
-						for (int q = tid; q < kf; q += BR_THREADS) {
+						for (int q = tid; q < kf; q += NT) {
 							S.rowid[q] = S.tmp[q];
 							S.J[q] = S.outc[q];
 						}
@@ -885,7 +959,7 @@ __global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref 
 					__syncthreads();
 					continue;   // rank deficient: search from the top
 				}
-				for (int r = tid; r < BR_CH; r += BR_THREADS)
+				for (int r = tid; r < BR_CH; r += NT)
 					S.alive[r] = r < nr;
 				{
 					const int cc = tid % W, g = tid / W;
@@ -898,10 +972,10 @@ __global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref 
 					cnt = reg_eliminate<W, RPG, SMALL>(v, nr, wc, false, mp, S.alive, S.f, S.pivrow, S.s_min,
 									   S.outr, S.outc, badp);
 				}
-				for (int q = tid; q < cnt; q += BR_THREADS)
+				for (int q = tid; q < cnt; q += NT)
 					S.tmp[q] = S.rowid[S.outr[q]];
 				__syncthreads();
-				for (int q = tid; q < cnt; q += BR_THREADS) {
+				for (int q = tid; q < cnt; q += NT) {
 					S.rowid[q] = S.tmp[q];
 					S.J[q] = S.outc[q];
 				}
@@ -926,7 +1000,7 @@ __global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref 
 						x = (ci - W == r) ? 1u : 0u;
 					v[i] = x;
 				}
-				for (int r = tid; r < W; r += BR_THREADS) {
+				for (int r = tid; r < W; r += NT) {
 					S.alive[r] = 1;
 					S.pos[r] = r;
 				}
@@ -934,7 +1008,7 @@ __global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref 
 									       S.outr, S.outc, badp);
 				if (tid == 0 && ki < k)
 					*badp = 1;
-				for (int q = tid; q < ki; q += BR_THREADS)
+				for (int q = tid; q < ki; q += NT)
 					S.pos[S.outr[q]] = S.outc[q];
 				__syncthreads();
 				if (ci >= W) {
@@ -943,28 +1017,56 @@ __global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref 
 						S.Pinv[S.pos[gi * RPGI + i]][ci - W] = v[i];
 				}
 			}
-			for (int s = tid; s < k; s += BR_THREADS) {
+			for (int s = tid; s < k; s += NT) {
 				S.srcoff[s] = (long long) S.rowid[s] * n + c;
 				S.is_piv[S.rowid[s]] = 2;
 				t.piv_rows[rank + s] = S.rowid[s];
 				t.piv_cols[rank + s] = c + S.J[s];
 			}
-			for (int r = tid; r < BR_RB; r += BR_THREADS)
+			for (int r = tid; r < BR_RB; r += NT)
 				S.rownz[r] = 1;
 			__syncthreads();
-			for (int e = tid; e < k * W; e += BR_THREADS) {
+			for (int e = tid; e < k * W; e += NT) {
 				const int r = e / W, s = e % W;
 				S.G[r][s] = (GT) (s < k ? S.Pinv[r][s] : 0u);
 			}
 			__syncthreads();
-			br_apply<W, FMA, false>(S, k, k, nc, Wm, Wb, n, mp, two_64);
+			br_apply<NT, W, FMA, I8, false>(S, k, k, nc, Wm, Wb, n, mp, two_64);
 			__syncthreads();
+			if constexpr (I8) {
+				br_pack_wb<NT, W>(Wb, n, k, nc, p, t.Wbp);
+				__syncthreads();
+			}
 			// ---- 3. eliminate column set J from every other row
-			for (int s = tid; s < k; s += BR_THREADS)
+			for (int s = tid; s < k; s += NT)
 				S.srcoff[s] = (long long) s * n;
 			for (int i0 = 0; i0 < m; i0 += BR_RB) {
 				const int nb = min(BR_RB, m - i0);
-				for (int e = tid; e < nb * W; e += BR_THREADS) {
+				if constexpr (I8) {
+					// multipliers straight into A fragments: element (r, s) is byte
+					// s%16 of lane r%32 + 32*((s%32)/16) of fragment (r/32, s/32)
+					for (int r = tid; r < BR_RB; r += NT)
+						S.rownz[r] = 0;
+					__syncthreads();
+					for (int e = tid; e < BR_RB * W; e += NT) {
+						const int r = e / W, s = e % W;
+						const int i = i0 + r;
+						uint32_t g = 0;
+						if (r < nb && s < k && S.is_piv[i] != 2)
+							g = Wm[(size_t) i * n + c + S.J[s]];
+						if (g)
+							S.rownz[r] = 1;
+						const uint32_t l = i8_limbs(g, p);
+						const int ln = (r & 31) + 32 * ((s & 31) >> 4), by = s & 15;
+						S.Ga[r >> 5][s >> 5][0][ln][by] = (int8_t) (l >> 8);
+						S.Ga[r >> 5][s >> 5][1][ln][by] = (int8_t) l;
+					}
+					__syncthreads();
+					br_update_i8<NT, W, FMA>(S, nb, nc, t.Wbp, Wm + (size_t) i0 * n + c, n, mp);
+					__syncthreads();
+					continue;
+				}
+				for (int e = tid; e < nb * W; e += NT) {
 					const int r = e / W, s = e % W;
 					const int i = i0 + r;
 					uint32_t g = 0;
@@ -974,29 +1076,29 @@ __global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref 
 				}
 				__syncthreads();
 				// rownz[r]: some multiplier of row i0 + r is nonzero (a wave per row)
-				for (int r = wave; r < nb; r += BR_THREADS / 64) {
+				for (int r = wave; r < nb; r += NT / 64) {
 					const bool nz = lane < k && S.G[r][lane] != (GT) 0;
 					const uint64_t bal = __ballot(nz);
 					if (lane == 0)
 						S.rownz[r] = bal != 0;
 				}
 				__syncthreads();
-				br_apply<W, FMA, true>(S, k, nb, nc, Wb, Wm + (size_t) i0 * n + c, n, mp, two_64);
+				br_apply<NT, W, FMA, I8, true>(S, k, nb, nc, Wb, Wm + (size_t) i0 * n + c, n, mp, two_64);
 				__syncthreads();
 			}
-			for (int e = tid; e < k * nc; e += BR_THREADS) {
+			for (int e = tid; e < k * nc; e += NT) {
 				const int s = e / nc, j = e % nc;
 				Wm[(size_t) S.rowid[s] * n + c + j] = Wb[(size_t) s * n + j];
 			}
-			for (int s = tid; s < k; s += BR_THREADS)
+			for (int s = tid; s < k; s += NT)
 				S.is_piv[S.rowid[s]] = 1;
-			for (int r = tid; r < BR_RB; r += BR_THREADS)
+			for (int r = tid; r < BR_RB; r += NT)
 				S.rownz[r] = 1;
 			rank += k;
 			__syncthreads();
 		}
 		// ---- standard layout: pivot rows in pivot-column order, then zero rows
-		for (int r = wave; r < m; r += BR_THREADS / 64) {
+		for (int r = wave; r < m; r += NT / 64) {
 			const int src = r < rank ? t.piv_rows[r] : -1;
 			for (int c = lane; c < n; c += 64) {
 				const uint32_t x = src >= 0 ? Wm[(size_t) src * n + c] : 0u;
@@ -1004,7 +1106,7 @@ __global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref 
 			}
 		}
 		if (t.pivots)
-			for (int s = tid; s < rank; s += BR_THREADS)
+			for (int s = tid; s < rank; s += NT)
 				t.pivots[s] = (uint32_t) t.piv_cols[s];
 		if (tid == 0)
 			ranks[w.task] = rank;
@@ -1014,27 +1116,31 @@ __global__ __launch_bounds__(BR_THREADS) void batch_rref_kernel(const BatchRref 
 }
 
 template <int E, int W>
-static void batch_rref_launch_w(hipStream_t stream, const ModP &mp, uint64_t two_64, const BatchRref *tasks,
-				int *ranks, int *bad, BatchLaunch L, int grid)
+static void batch_rref_launch_w(hipStream_t stream, bool mfma, const ModP &mp, uint64_t two_64,
+				const BatchRref *tasks, int *ranks, int *bad, BatchLaunch L, int grid)
 {
 	const bool small = mp.p < (1ull << 26);
 	const bool fma = (unsigned __int128) W * (mp.p - 1) * (mp.p - 1) < ((unsigned __int128) 1 << 53);
-	if (fma)
-		batch_rref_kernel<E, W, true, true><<<grid, BR_THREADS, 0, stream>>>(tasks, L, mp, two_64, ranks, bad);
+	if (mfma)   // p <= 2^16: the fused and FP64 paths apply
+		batch_rref_kernel<E, W, true, true, true, 512><<<grid, 512, 0, stream>>>(tasks, L, mp, two_64, ranks, bad);
+	else if (fma)
+		batch_rref_kernel<E, W, true, true, false, 1024><<<grid, 1024, 0, stream>>>(tasks, L, mp, two_64, ranks, bad);
 	else if (small)
-		batch_rref_kernel<E, W, true, false><<<grid, BR_THREADS, 0, stream>>>(tasks, L, mp, two_64, ranks, bad);
+		batch_rref_kernel<E, W, true, false, false, 1024><<<grid, 1024, 0, stream>>>(tasks, L, mp, two_64, ranks,
+												  bad);
 	else
-		batch_rref_kernel<E, W, false, false><<<grid, BR_THREADS, 0, stream>>>(tasks, L, mp, two_64, ranks, bad);
+		batch_rref_kernel<E, W, false, false, false, 1024><<<grid, 1024, 0, stream>>>(tasks, L, mp, two_64, ranks,
+												   bad);
 }
 
 template <int E>
 static void batch_rref_launch_e(hipStream_t stream, int variant, const ModP &mp, uint64_t two_64,
 				const BatchRref *tasks, int *ranks, int *bad, BatchLaunch L, int grid)
 {
-	if (variant == BR_W32)
-		batch_rref_launch_w<E, 32>(stream, mp, two_64, tasks, ranks, bad, L, grid);
+	if (br_width(variant) == 32)
+		batch_rref_launch_w<E, 32>(stream, br_mfma(variant), mp, two_64, tasks, ranks, bad, L, grid);
 	else
-		batch_rref_launch_w<E, 64>(stream, mp, two_64, tasks, ranks, bad, L, grid);
+		batch_rref_launch_w<E, 64>(stream, br_mfma(variant), mp, two_64, tasks, ranks, bad, L, grid);
 }
 
 int batch_rref_launch(hipStream_t stream, int elem, int variant, uint32_t p, const BatchRref *tasks, int *ranks,
