@@ -167,7 +167,9 @@ def bench_batch(la, synth, np):
     ctx.synchronize()
     out["per_instance_calls_ms"] = (time.perf_counter() - t0) * 1e3
     out["batched_over_per_instance"] = out["per_instance_calls_ms"] / out["model_ms"]
-    # measured cost (CU-microseconds) of every variant of every held-out instance
+    # measured cost (CU-microseconds) of every variant of every held-out instance,
+    # on the original inputs (the calls above reduced the matrices in place)
+    ev.upload()
     cost = lb.tune(ctx, F, ev.tasks, reps=2)
     idx = np.arange(ev.n)
     out["cu_us"] = {name: float(np.nansum(cost[idx, v])) for name, v in chosen.items()}

@@ -44,7 +44,7 @@ struct BatchRref {
 	void *A;               // caller matrix, m x n, leading dimension lda
 	uint64_t lda;
 	uint32_t *pivots;      // caller's pivot-column array (min(m, n)) or null
-	uint32_t *Wm;          // m x n canonical working copy
+	void *Wm;              // m x n canonical working copy (uint16 for the MFMA variants, else uint32)
 	uint32_t *Wb;          // panel-width x n rows of P_J^-1 * P
 	int8_t *Wbp;           // Wb as INT8 limb fragments (MFMA variants), else null
 	int *piv_rows, *piv_cols;   // min(m, n) each

@@ -663,7 +663,8 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 // the 2^64-wrap correction.
 // ===========================================================================
 constexpr int BR_CH = 128;   // rows per pivot-search elimination
-constexpr int BR_RB = 64;    // rows per update block (multipliers in LDS)
+constexpr int BR_RB = 64;    // rows per update block (multipliers in LDS), VALU update
+constexpr int BR_RB8 = 256;  // rows per update block, MFMA update (multipliers as limbs)
 constexpr int BR_KC = 8;     // multipliers per register chunk
 constexpr int BR_RQ = 8;     // rows per update unit of one wave
 
@@ -672,7 +673,7 @@ struct BrLds {
 	typedef typename std::conditional<FMA, double, uint32_t>::type GT;
 	// MFMA update: the 64-row block's multipliers as INT8 A fragments,
 	// [32-row block][k block][limb][64 lanes][16 bytes]
-	__attribute__((aligned(16))) int8_t Ga[I8 ? BR_RB / 32 : 1][I8 ? W / 32 : 1][2][64][16];
+	__attribute__((aligned(16))) int8_t Ga[I8 ? BR_RB8 / 32 : 1][I8 ? W / 32 : 1][2][64][16];
 	int rowid[BR_CH], alive[BR_CH], tmp[BR_CH];
 	uint32_t f[BR_CH];
 	uint32_t pivrow[2 * W];
@@ -681,7 +682,7 @@ struct BrLds {
 	int s_min[2];
 	int slot, nr, cursor;
 	uint8_t is_piv[BR_MAXM];     // 0 live, 1 pivot of an earlier panel, 2 of this panel
-	uint8_t rownz[BR_RB];
+	uint8_t rownz[I8 ? BR_RB8 : BR_RB];
 	uint32_t Pinv[W][W];
 	GT G[BR_RB][W];
 };
@@ -702,9 +703,9 @@ __device__ __forceinline__ uint32_t br_mod_f64(double x, const ModP &mp)
 // x 64 columns are spread over the waves; a lane owns one column and keeps
 // BR_KC source values in registers while the multipliers (uniform across the
 // wave) are read from LDS.
-template <int NT, int W, bool FMA, bool I8, bool SUB>
-__device__ __forceinline__ void br_apply(BrLds<W, FMA, I8> &S, int k, int nrows, int ncols, const uint32_t *src,
-					 uint32_t *out, size_t ld, const ModP &mp, uint64_t two_64)
+template <int NT, int W, bool FMA, bool I8, bool SUB, typename ST, typename OT>
+__device__ __forceinline__ void br_apply(BrLds<W, FMA, I8> &S, int k, int nrows, int ncols, const ST *src, OT *out,
+					 size_t ld, const ModP &mp, uint64_t two_64)
 {
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int nchunks = (ncols + 63) >> 6, nq = (nrows + BR_RQ - 1) / BR_RQ;
@@ -736,9 +737,9 @@ __device__ __forceinline__ void br_apply(BrLds<W, FMA, I8> &S, int k, int nrows,
 #pragma unroll
 			for (int r = 0; r < BR_RQ; ++r) {
 				if (r0 + r < nrows && S.rownz[r0 + r] && colok) {
-					uint32_t *o = out + (size_t) (r0 + r) * ld + j;
+					OT *o = out + (size_t) (r0 + r) * ld + j;
 					const uint32_t x = br_mod_f64(acc[r], mp);
-					*o = SUB ? mp.sub(*o, x) : x;
+					*o = (OT) (SUB ? mp.sub(*o, x) : x);
 				}
 			}
 		} else {
@@ -768,9 +769,9 @@ __device__ __forceinline__ void br_apply(BrLds<W, FMA, I8> &S, int k, int nrows,
 #pragma unroll
 			for (int r = 0; r < BR_RQ; ++r) {
 				if (r0 + r < nrows && S.rownz[r0 + r] && colok) {
-					uint32_t *o = out + (size_t) (r0 + r) * ld + j;
+					OT *o = out + (size_t) (r0 + r) * ld + j;
 					const uint32_t x = (uint32_t) mp.reduce(acc[r]);
-					*o = SUB ? mp.sub(*o, x) : x;
+					*o = (OT) (SUB ? mp.sub(*o, x) : x);
 				}
 			}
 		}
@@ -806,12 +807,12 @@ __device__ __forceinline__ void br_pack_wb(const uint32_t *__restrict__ Wb, size
 	}
 }
 
-// C (nb <= BR_RB rows x nc columns, row stride n) -= G * Wb on INT8 MFMA: G
+// C (nb <= BR_RB8 rows x nc columns, row stride n) -= G * Wb on INT8 MFMA: G
 // from S.Ga, Wb from Wbp.  Units of 32 x 32 outputs are spread over the
 // waves; rows whose multipliers are all zero are not written.
-template <int NT, int W, bool FMA>
+template <int NT, int W, bool FMA, typename CT>
 __device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int nc, const int8_t *__restrict__ Wbp,
-					     uint32_t *__restrict__ C, size_t n, const ModP &mp)
+					     CT *__restrict__ C, size_t n, const ModP &mp)
 {
 	constexpr int KB = W / 32;
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -819,6 +820,15 @@ __device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int
 	const uint32_t p = (uint32_t) mp.p;
 	for (int u = wave; u < nrb * ncb; u += NT / 64) {
 		const int rb = u / ncb, cb = u - rb * ncb;
+		const int col = cb * 32 + (lane & 31);
+		// the unit's outputs are loaded first so that their latency overlaps
+		// the MFMAs
+		uint32_t cv[16];
+#pragma unroll
+		for (int r = 0; r < 16; ++r) {
+			const int row = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+			cv[r] = (row < nb && col < nc) ? C[(size_t) row * n + col] : 0u;
+		}
 		v16i acc0 = v16i{0}, acc1 = v16i{0}, acc2 = v16i{0};
 #pragma unroll
 		for (int kb = 0; kb < KB; ++kb) {
@@ -831,7 +841,6 @@ __device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int
 			acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl, acc2, 0, 0, 0);
 			acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, acc1, 0, 0, 0);
 		}
-		const int col = cb * 32 + (lane & 31);
 		if (col >= nc)
 			continue;
 #pragma unroll
@@ -840,9 +849,8 @@ __device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int
 			if (row < nb && S.rownz[row]) {
 				const double v = fma((double) acc0[r], 65536.0, fma((double) acc1[r], 256.0, (double) acc2[r]));
 				const uint32_t d = mp.mod_f64(v);
-				uint32_t *cp = C + (size_t) row * n + col;
-				const uint32_t x = *cp;
-				*cp = x >= d ? x - d : x + p - d;
+				const uint32_t x = cv[r];
+				C[(size_t) row * n + col] = (CT) (x >= d ? x - d : x + p - d);
 			}
 		}
 	}
@@ -855,6 +863,8 @@ __global__ __launch_bounds__(NT) void batch_rref_kernel(const BatchRref *__restr
 {
 	typedef typename Elem<E>::T T;
 	typedef typename BrLds<W, FMA, I8>::GT GT;
+	// working copy: residues fit 16 bits for the MFMA variants (p <= 2^16)
+	typedef typename std::conditional<I8, uint16_t, uint32_t>::type WT;
 	constexpr int RPG = BR_CH / (NT / W);             // pivot search: rows per thread
 	constexpr int NCI = 2 * W, RPGI = W / (NT / NCI);  // inverse: rows per thread
 	__shared__ BrLds<W, FMA, I8> S;
@@ -868,11 +878,12 @@ __global__ __launch_bounds__(NT) void batch_rref_kernel(const BatchRref *__restr
 		const BatchRref &t = tasks[w.task];
 		const uint64_t t0 = wall_clock64();
 		const int m = t.m, n = t.n;
-		uint32_t *Wm = t.Wm, *Wb = t.Wb;
+		WT *Wm = (WT *) t.Wm;
+		uint32_t *Wb = t.Wb;
 		int *badp = bad + w.task;
 		for (int r = wave; r < m; r += NT / 64)
 			for (int c = lane; c < n; c += 64)
-				Wm[(size_t) r * n + c] = Elem<E>::load((const T *) t.A + (size_t) r * t.lda + c, p);
+				Wm[(size_t) r * n + c] = (WT) Elem<E>::load((const T *) t.A + (size_t) r * t.lda + c, p);
 		for (int r = tid; r < m; r += NT)
 			S.is_piv[r] = 0;
 		__syncthreads();
@@ -1040,15 +1051,16 @@ __global__ __launch_bounds__(NT) void batch_rref_kernel(const BatchRref *__restr
 			// ---- 3. eliminate column set J from every other row
 			for (int s = tid; s < k; s += NT)
 				S.srcoff[s] = (long long) s * n;
-			for (int i0 = 0; i0 < m; i0 += BR_RB) {
-				const int nb = min(BR_RB, m - i0);
+			constexpr int RB = I8 ? BR_RB8 : BR_RB;
+			for (int i0 = 0; i0 < m; i0 += RB) {
+				const int nb = min(RB, m - i0);
 				if constexpr (I8) {
 					// multipliers straight into A fragments: element (r, s) is byte
 					// s%16 of lane r%32 + 32*((s%32)/16) of fragment (r/32, s/32)
-					for (int r = tid; r < BR_RB; r += NT)
+					for (int r = tid; r < RB; r += NT)
 						S.rownz[r] = 0;
 					__syncthreads();
-					for (int e = tid; e < BR_RB * W; e += NT) {
+					for (int e = tid; e < ((nb + 31) & ~31) * W; e += NT) {
 						const int r = e / W, s = e % W;
 						const int i = i0 + r;
 						uint32_t g = 0;
@@ -1062,7 +1074,7 @@ __global__ __launch_bounds__(NT) void batch_rref_kernel(const BatchRref *__restr
 						S.Ga[r >> 5][s >> 5][1][ln][by] = (int8_t) l;
 					}
 					__syncthreads();
-					br_update_i8<NT, W, FMA>(S, nb, nc, t.Wbp, Wm + (size_t) i0 * n + c, n, mp);
+					br_update_i8<NT, W, FMA, WT>(S, nb, nc, t.Wbp, Wm + (size_t) i0 * n + c, n, mp);
 					__syncthreads();
 					continue;
 				}
@@ -1088,11 +1100,11 @@ __global__ __launch_bounds__(NT) void batch_rref_kernel(const BatchRref *__restr
 			}
 			for (int e = tid; e < k * nc; e += NT) {
 				const int s = e / nc, j = e % nc;
-				Wm[(size_t) S.rowid[s] * n + c + j] = Wb[(size_t) s * n + j];
+				Wm[(size_t) S.rowid[s] * n + c + j] = (WT) Wb[(size_t) s * n + j];
 			}
 			for (int s = tid; s < k; s += NT)
 				S.is_piv[S.rowid[s]] = 1;
-			for (int r = tid; r < BR_RB; r += NT)
+			for (int r = tid; r < (I8 ? BR_RB8 : BR_RB); r += NT)
 				S.rownz[r] = 1;
 			rank += k;
 			__syncthreads();
