@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip plain/FP64 gemm, FL and other rows")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--only", choices=["fl", "rows", "batched"], default=None,
+                    help="run one secondary leg alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run one device-to-device copy of BYTES (counter calibration)")
     return ap.parse_args()
@@ -236,6 +238,11 @@ def main():
     import numpy as np
     import lela_amd as la
     from lela_amd import synth
+
+    if args.only:
+        leg = {"fl": bench_fl, "rows": bench_rows, "batched": bench_batch}[args.only]
+        print(json.dumps({"leg": args.only, args.only: leg(la, synth, np)}), flush=True)
+        return
 
     F = la.Modular(P, "uint16")
     ctx = la.Context(F, device=0)
