@@ -176,7 +176,6 @@ struct mlz_batch {
 	hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
-	int last_rc = MLZ_OK;
 };
 
 namespace mlz {
@@ -489,7 +488,6 @@ static int plan_build(mlz_ctx *ctx, int elem, uint64_t p, mlz_task *tasks, size_
 		b->rref_slot[i] = tasks[i].op == MLZ_TASK_RREF ? (int) rref_of[i] : -1;
 	// tcost is indexed by descriptor too: gemm descriptors [0, ng), rref [ng, ng + nr)
 	unsigned long long *tc_gemm = b->d_tcost, *tc_rref = b->d_tcost + ng, *tc_pack = b->d_tcost + nt;
-	int *bad_rref = b->d_bad;
 
 	auto fail = [&](int rc) {
 		free_plan(b);
@@ -534,7 +532,6 @@ static int plan_build(mlz_ctx *ctx, int elem, uint64_t p, mlz_task *tasks, size_
 	for (int v = 0; v < BR_NVAR; ++v)
 		if (!items[RR0 + v].empty())
 			add(2, v, 3 + (v & 1), RR0 + v, tc_rref, (int) b->occ_rref[v]);
-	(void) bad_rref;
 
 	if (!(flags & MLZ_BATCH_SERIAL)) {
 		for (auto &s : b->side)
