@@ -81,6 +81,8 @@ int mlz_ctx_set_ptr_mode(mlz_ctx *ctx, int mode);
  *                      2 = 128x256, 3 = 256x128 (4 waves each), 4 = 128x128
  *                      with two workgroups per CU, 5 = 128x256 and
  *                      6 = 256x128 with 8 waves and two workgroups per CU.
+ *   "rref.search.threads" threads of the echelon form's fused pivot search
+ *                      and inverse (one workgroup per panel): 256, 512 or 1024.
  *   "rref.update.blocks" 32-column blocks per wave of the echelon form's
  *                      trailing update on INT8 limbs: 1 (default), 2 or 4. */
 int mlz_ctx_set_option(mlz_ctx *ctx, const char *key, int64_t value);

@@ -279,6 +279,7 @@ struct mlz_ctx {
 	size_t host_pinned_bytes;
 	int i8_variant;      // INT8 gemm tile variant, 0 = chosen by shape
 	int upd_nblk;        // 32-column blocks per wave of the rank-k update (1, 2, 4)
+	int rref_fast_threads;   // threads of the echelon form's fused pivot-search kernel (256, 512, 1024)
 	struct mlz_tuner *tuner;   // autotuner table and cost-model weights (batch.hip), lazily created
 };
 

@@ -166,6 +166,7 @@ int mlz_ctx_create(mlz_ctx **out, int device, const char *config)
 	ctx->host_pinned_bytes = 0;
 	ctx->i8_variant = 0;
 	ctx->upd_nblk = 1;
+	ctx->rref_fast_threads = 1024;
 	ctx->tuner = nullptr;
 	// Default priority only: streams of different priorities are not used.
 	if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -225,6 +226,12 @@ int mlz_ctx_set_option(mlz_ctx *ctx, const char *key, int64_t value)
 		if (value < 0 || value > 6)
 			return set_error(ctx, MLZ_E_ARG, "gemm.i8.variant must be 0..6");
 		ctx->i8_variant = (int) value;
+		return MLZ_OK;
+	}
+	if (strcmp(key, "rref.search.threads") == 0) {
+		if (value != 256 && value != 512 && value != 1024)
+			return set_error(ctx, MLZ_E_ARG, "rref.search.threads must be 256, 512 or 1024");
+		ctx->rref_fast_threads = (int) value;
 		return MLZ_OK;
 	}
 	if (strcmp(key, "rref.update.blocks") == 0) {

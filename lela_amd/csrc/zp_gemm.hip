@@ -414,6 +414,17 @@ __global__ __launch_bounds__(256) void rank_update_i8_kernel(const int8_t *__res
 	const int8_t *ga = Gp + (size_t) rb * panel + lane * 16;
 	const int cb0 = sn * UPD_NBLK;
 	const int nblk = min(UPD_NBLK, (n + 31) / 32 - cb0);
+	// the strip's outputs are loaded first so that their latency overlaps the MFMAs
+	uint32_t cv[UPD_NBLK][16];
+#pragma unroll
+	for (int j = 0; j < UPD_NBLK; ++j) {
+		const int col = (cb0 + j) * 32 + (lane & 31);
+#pragma unroll
+		for (int r = 0; r < 16; ++r) {
+			const int row = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+			cv[j][r] = (j < nblk && row < m && col < n) ? C[(size_t) row * ldc + col] : 0u;
+		}
+	}
 	v16i acc[3][UPD_NBLK];
 #pragma unroll
 	for (int t = 0; t < 3; ++t)
@@ -448,9 +459,8 @@ __global__ __launch_bounds__(256) void rank_update_i8_kernel(const int8_t *__res
 				const double v = fma((double) acc[0][j][r], 65536.0,
 						     fma((double) acc[1][j][r], 256.0, (double) acc[2][j][r]));
 				const uint32_t d = mp.mod_f64(v);
-				uint32_t *cp = C + (size_t) row * ldc + col;
-				const uint32_t c = *cp;
-				*cp = c >= d ? c - d : c + p - d;
+				const uint32_t c = cv[j][r];
+				C[(size_t) row * ldc + col] = c >= d ? c - d : c + p - d;
 			}
 		}
 	}

@@ -191,11 +191,11 @@ __device__ inline int nodes_at(int nodes0, int level)
 // is R[row of pivot c][row of pivot t].  If S is rank deficient but holds
 // every live row, the pivots are still final and the inverse kernel builds
 // P_J^-1; otherwise the tournament decides.
-template <bool SMALL>
-__global__ __launch_bounds__(1024) void rref_fast_kernel(const uint32_t *__restrict__ A, size_t lda, int m, int c,
-							int wc, ModP mp, RrefState st, uint32_t *__restrict__ Pinv)
+template <bool SMALL, int TPB>
+__global__ __launch_bounds__(TPB) void rref_fast_kernel(const uint32_t *__restrict__ A, size_t lda, int m, int c,
+						       int wc, ModP mp, RrefState st, uint32_t *__restrict__ Pinv)
 {
-	constexpr int W = RREF_W, NC = 2 * RREF_W, RPG = W / (1024 / NC);
+	constexpr int W = RREF_W, NC = 2 * RREF_W, RPG = W / (TPB / NC);
 	__shared__ int rowid[W], alive[W], outr[W], outc[W], pos[W];
 	__shared__ uint32_t f[W], pivrow[NC];
 	__shared__ int s_min[2], s_cnt, s_wave[16];
@@ -574,10 +574,15 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 		{
 			ProfScope ps(ctx, "rref_pivots");
 			if (mp.small) {
-				rref_fast_kernel<true><<<1, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, Pinv);
+				if (ctx->rref_fast_threads == 256)
+					rref_fast_kernel<true, 256><<<1, 256, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, Pinv);
+				else if (ctx->rref_fast_threads == 512)
+					rref_fast_kernel<true, 512><<<1, 512, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, Pinv);
+				else
+					rref_fast_kernel<true, 1024><<<1, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, Pinv);
 				rref_tourney_kernel<true><<<nodes0, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, nodes0);
 			} else {
-				rref_fast_kernel<false><<<1, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, Pinv);
+				rref_fast_kernel<false, 1024><<<1, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, Pinv);
 				rref_tourney_kernel<false><<<nodes0, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, nodes0);
 			}
 		}

@@ -102,3 +102,20 @@ def test_rref_structure_large(gpu_ctx):
     sel = np.arange(0, 2000, 37)
     recon = (A[sel][:, piv].astype(np.uint64) @ R) % P
     assert np.array_equal(recon, A[sel].astype(np.uint64))
+
+
+@pytest.mark.parametrize("threads", [256, 512, 1024])
+@pytest.mark.parametrize("blocks", [1, 2, 4])
+def test_rref_tuning_options(gpu_ctx, threads, blocks):
+    """The pivot-search workgroup size and the update strip width change speed only."""
+    p = 65521
+    gpu_ctx.set_option("rref.search.threads", threads)
+    gpu_ctx.set_option("rref.update.blocks", blocks)
+    try:
+        for A in (synth.zp_matrix(31, 300, 500, p), lowrank(32, 260, 300, 70, p), sparse(33, 200, 330, p, 0.02)):
+            got, rank, piv = run_rref(gpu_ctx, p, "uint32", A)
+            rc, want, wrank, wpiv, _ = orc.zp_echelonize(p, A, reduced=True)
+            assert rank == wrank and np.array_equal(got, want) and list(piv) == list(wpiv)
+    finally:
+        gpu_ctx.set_option("rref.search.threads", 1024)
+        gpu_ctx.set_option("rref.update.blocks", 1)
