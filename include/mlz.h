@@ -1,7 +1,7 @@
 /* include/mlz.h -- C ABI of the MI355X exact linear-algebra backend
  * (lela_amd/liblela_amd.so).
  *
- * This is the boundary a LELA module (HIPModule) binds:
+ * This is the boundary a LELA module (HIPModule, include/lela_amd/hip-module.h) binds:
  * plain pointers and sizes, no C++ or HIP types.  Each entry point names the
  * LELA routine whose semantics it reproduces (paths relative to the LELA
  * source tree).

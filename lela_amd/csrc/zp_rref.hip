@@ -895,10 +895,10 @@ __global__ __launch_bounds__(NT) void batch_rref_kernel(const BatchRref *__restr
 		int rank = 0;
 		for (int c = 0; c < n && rank < m; c += W) {
 			const int wc = min(W, n - c), nc = n - c;
-			// ---- 1. pivot search.  Round 0 reduces [S | I] for S = the first wc
-			// live rows; full panel rank gives the pivot rows and P_J^-1 at once.
-			// Otherwise rounds of up to BR_CH rows from the top find the rank
-			// profile, and step 2 inverts P_J.
+			// ---- 1. pivot search.  The first pass reduces [S | I] for S = the
+			// first wc live rows; full panel rank gives the pivot rows and P_J^-1
+			// at once.  Otherwise passes of up to BR_CH rows from the top find the
+			// rank profile, and step 2 inverts P_J.
 			int cnt = 0;
 			bool have_inv = false;
 			for (int round = 0;; ++round) {
@@ -1002,7 +1002,7 @@ __global__ __launch_bounds__(NT) void batch_rref_kernel(const BatchRref *__restr
 			const int k = cnt;
 			if (k == 0)
 				continue;
-			// ---- 2. P_J^-1 (unless round 0 produced it), then Wb = P_J^-1 * P
+			// ---- 2. P_J^-1 (unless the first pass produced it), then Wb = P_J^-1 * P
 			if (!have_inv) {
 				const int ci = tid % NCI, gi = tid / NCI;
 				uint32_t v[RPGI];
