@@ -6,7 +6,8 @@
 // The gemm updates run on the INT8 / FP64 / plain kernels of zp_gemm.hip.
 // The recursion splits at multiples of TRSM_LEAF; all diagonal blocks of that
 // size are inverted up front by one launch, and each leaf is then the small
-// product B_b <- a * inv(A_bb) * B_b.
+// product B_b <- a * inv(A_bb) * B_b (on INT8 MFMA for p <= 2^16 unless the plain or FP64 path
+// was asked for).
 //
 // Non-invertible diagonal entries are found up front by one kernel (modular
 // inverse of every diagonal entry); the reported index is the one LELA's
@@ -140,6 +141,64 @@ __global__ __launch_bounds__(256) void trsm_leaf_apply_kernel(const uint32_t *__
 	}
 }
 
+// The same leaf on INT8 MFMA (p <= 2^16): a workgroup of 4 waves owns 64
+// right-hand-side columns.  W_b and the 64 x 64 block of B are turned into
+// limb fragments in LDS (A operand: element (r, k) is byte k%16 of lane
+// r%32 + 32*((k%32)/16) of fragment (r/32, k/32); B operand likewise with the
+// column in place of r), each wave computes one 32 x 32 block of W_b * B_b with
+// eight MFMAs, and the block is written back after the whole strip was read.
+template <int E>
+__global__ __launch_bounds__(256) void trsm_leaf_i8_kernel(const uint32_t *__restrict__ W,
+							   typename Elem<E>::T *__restrict__ B, size_t ldb, int nb,
+							   int nrhs, uint32_t a, ModP mp)
+{
+	constexpr int L = TRSM_LEAF;
+	__shared__ __attribute__((aligned(16))) int8_t Wf[2][2][2][64][16];   // [row block][k block][limb]
+	__shared__ __attribute__((aligned(16))) int8_t Bf[2][2][2][64][16];   // [column block][k block][limb]
+	const uint32_t p = (uint32_t) mp.p;
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int j0 = blockIdx.x * 64;
+	for (int e = tid; e < L * L; e += blockDim.x) {
+		const int r = e / L, k = e % L;
+		const uint32_t wv = (r < nb && k < nb) ? W[(size_t) r * L + k] : 0u;
+		const uint32_t lw = i8_limbs(wv, p);
+		const int ln = (r & 31) + 32 * ((k & 31) >> 4), by = k & 15;
+		Wf[r >> 5][k >> 5][0][ln][by] = (int8_t) (lw >> 8);
+		Wf[r >> 5][k >> 5][1][ln][by] = (int8_t) lw;
+		// B element (k = e / L, column e % L): rows of B are contiguous in the columns
+		const int kb = e / L, c = e % L;
+		const uint32_t bv = (kb < nb && j0 + c < nrhs) ? Elem<E>::load(B + (size_t) kb * ldb + j0 + c, p) : 0u;
+		const uint32_t lb = i8_limbs(bv, p);
+		const int lnb = (c & 31) + 32 * ((kb & 31) >> 4), byb = kb & 15;
+		Bf[c >> 5][kb >> 5][0][lnb][byb] = (int8_t) (lb >> 8);
+		Bf[c >> 5][kb >> 5][1][lnb][byb] = (int8_t) lb;
+	}
+	__syncthreads();
+	const int rb = wave >> 1, cb = wave & 1;
+	v16i acc0 = v16i{0}, acc1 = v16i{0}, acc2 = v16i{0};
+#pragma unroll
+	for (int kb = 0; kb < 2; ++kb) {
+		const v4i ah = *(const v4i *) &Wf[rb][kb][0][lane][0], al = *(const v4i *) &Wf[rb][kb][1][lane][0];
+		const v4i bh = *(const v4i *) &Bf[cb][kb][0][lane][0], bl = *(const v4i *) &Bf[cb][kb][1][lane][0];
+		acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bh, acc0, 0, 0, 0);
+		acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bl, acc1, 0, 0, 0);
+		acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl, acc2, 0, 0, 0);
+		acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, acc1, 0, 0, 0);
+	}
+	const int col = j0 + cb * 32 + (lane & 31);
+	if (col >= nrhs)
+		return;
+#pragma unroll
+	for (int r = 0; r < 16; ++r) {
+		const int row = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+		if (row < nb) {
+			const double v = fma((double) acc0[r], 65536.0, fma((double) acc1[r], 256.0, (double) acc2[r]));
+			const uint32_t d = mp.mod_f64(v);
+			Elem<E>::store(B + (size_t) row * ldb + col, mp.mod_f64((double) a * (double) d), p);
+		}
+	}
+}
+
 template <int E>
 static int trsm_diag_inverses(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, const void *A,
 			      size_t lda, const uint32_t *dinv, uint32_t *W)
@@ -157,7 +216,7 @@ static int trsm_diag_inverses(mlz_ctx *ctx, uint32_t p, int upper, int unit, siz
 
 template <int E>
 static int trsm_leaf(mlz_ctx *ctx, uint32_t p, size_t nb, size_t nrhs, uint32_t a, const uint32_t *W,
-		     void *B, size_t ldb)
+		     void *B, size_t ldb, int algo)
 {
 	typedef typename Elem<E>::T T;
 	const ModP mp = make_modp(p);
@@ -167,7 +226,9 @@ static int trsm_leaf(mlz_ctx *ctx, uint32_t p, size_t nb, size_t nrhs, uint32_t 
 	const bool wide = (unsigned __int128) (p - 1) * (p - 1) * TRSM_LEAF >= ((unsigned __int128) 1 << 64);
 	dim3 g(ceil_div(nrhs, 64));
 	ProfScope ps(ctx, "trsm_leaf");
-	if (wide)
+	if (p <= 65536 && (algo == MLZ_ALGO_AUTO || algo == MLZ_ALGO_INT8))
+		trsm_leaf_i8_kernel<E><<<g, 256, 0, ctx->stream>>>(W, (T *) B, ldb, (int) nb, (int) nrhs, a, mp);
+	else if (wide)
 		trsm_leaf_apply_kernel<E, true><<<g, 256, 0, ctx->stream>>>(W, (T *) B, ldb, (int) nb, (int) nrhs, a, mp, t);
 	else
 		trsm_leaf_apply_kernel<E, false><<<g, 256, 0, ctx->stream>>>(W, (T *) B, ldb, (int) nb, (int) nrhs, a, mp, t);
@@ -182,7 +243,7 @@ static int trsm_rec(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, siz
 	if (n == 0)
 		return MLZ_OK;
 	if (n <= (size_t) TRSM_LEAF)
-		return trsm_leaf<E>(ctx, p, n, nrhs, a, W, B, ldb);
+		return trsm_leaf<E>(ctx, p, n, nrhs, a, W, B, ldb, algo);
 	// Split at a multiple of the leaf size near n/2, so that every leaf is
 	// one of the precomputed diagonal blocks (any split gives the same unique
 	// solution; LELA splits at n/2).
