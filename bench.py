@@ -309,7 +309,7 @@ def main():
                    "algo": "split-int8 MFMA (2x2 8-bit limbs, int32 accumulate)"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS,
                      "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": traffic,
-                     "kernel": "gemm_i8_kernel", "kernel_avg_ms": avg_kernel_ms,
+                     "kernel": "gemm_i8_async_kernel", "kernel_avg_ms": avg_kernel_ms,
                      "ops_per_launch": 8.0 * W, "op": "int8 multiply-add counted as 2 ops"},
         "timing": {"event_ms_per_step": ev_ms / args.steps,
                    "pack_ms_per_step": pack_ms / args.steps},

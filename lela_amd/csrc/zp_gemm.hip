@@ -174,6 +174,71 @@ struct I8Tile {
 	static_assert(STAGE % (THREADS * 16) == 0, "stage must split into whole loads");
 };
 
+// The MFMAs of one staged k-step: A blocks at la, B blocks at lb, each
+// KK k-blocks x 2 limbs x 1024 bytes (CHUNK bytes per 32-index block).
+template <class T, int WTM, int WTN, int KK>
+__device__ __forceinline__ void i8_stage_mfma(const int8_t *la, const int8_t *lb, int wr, int wc, int lane,
+					      v16i (&acc)[3][WTM][WTN])
+{
+#pragma unroll
+	for (int kk = 0; kk < KK; ++kk) {
+		v4i fa[WTM][2];
+#pragma unroll
+		for (int i = 0; i < WTM; ++i)
+#pragma unroll
+			for (int l = 0; l < 2; ++l)
+				fa[i][l] = *(const v4i *) (la + (wr * WTM + i) * T::CHUNK + kk * 2048 + l * 1024 + lane * 16);
+#pragma unroll
+		for (int j = 0; j < WTN; ++j) {
+			const v4i fb0 = *(const v4i *) (lb + (wc * WTN + j) * T::CHUNK + kk * 2048 + lane * 16);
+			const v4i fb1 = *(const v4i *) (lb + (wc * WTN + j) * T::CHUNK + kk * 2048 + 1024 + lane * 16);
+#pragma unroll
+			for (int i = 0; i < WTM; ++i)
+				acc[0][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][0], fb0, acc[0][i][j], 0, 0, 0);
+#pragma unroll
+			for (int i = 0; i < WTM; ++i)
+				acc[1][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][0], fb1, acc[1][i][j], 0, 0, 0);
+#pragma unroll
+			for (int i = 0; i < WTM; ++i)
+				acc[2][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][1], fb1, acc[2][i][j], 0, 0, 0);
+#pragma unroll
+			for (int i = 0; i < WTM; ++i)
+				acc[1][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][1], fb0, acc[1][i][j], 0, 0, 0);
+		}
+	}
+}
+
+// Epilogue of a workgroup tile.  32x32 accumulator layout: lane l,
+// register r holds row 8*(r/4) + 4*(l/32) + r%4, column l%32.
+template <int E, class T, int WTM, int WTN>
+__device__ __forceinline__ void i8_epilogue(const v16i (&acc)[3][WTM][WTN], typename Elem<E>::T *__restrict__ C,
+					    size_t ldc, int m, int n, uint32_t a, uint32_t b, const ModP &mp, int tm,
+					    int tn, int wr, int wc, int lane)
+{
+#pragma unroll
+	for (int i = 0; i < WTM; ++i)
+#pragma unroll
+		for (int j = 0; j < WTN; ++j) {
+			const int col = tn * T::BN + wc * 32 * WTN + j * 32 + (lane & 31);
+#pragma unroll
+			for (int r = 0; r < 16; ++r) {
+				const int row = tm * T::BM + wr * 32 * WTM + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+				if (row < m && col < n) {
+					// 65536*HH + 256*(HL+LH) + LL has |.| < 2^48 and a*d + b*c < 2^33
+					// (p <= 2^16): both are exact doubles, reduced by mod_f64.
+					const double v = fma((double) acc[0][i][j][r], 65536.0,
+							     fma((double) acc[1][i][j][r], 256.0, (double) acc[2][i][j][r]));
+					const double d = (double) mp.mod_f64(v);
+					typename Elem<E>::T *cp = C + (size_t) row * ldc + col;
+					double t = (double) a * d;
+					if (b)
+						t = fma((double) b, (double) Elem<E>::load(cp, (uint32_t) mp.p), t);
+					Elem<E>::store(cp, mp.mod_f64(t), (uint32_t) mp.p);
+				}
+			}
+		}
+}
+
 // One output tile (tm, tn) of the INT8 product; lds holds 2 * STAGE bytes
 // (double-buffered [A: RBA blocks | B: RBB blocks] x [KK k-blocks x 2 limbs x
 // 1024 bytes]).  Shared by the single-matrix kernel and the batched one.
@@ -226,58 +291,9 @@ __device__ __forceinline__ void i8_tile(const int8_t *__restrict__ Ap, const int
 			for (int c = 0; c < T::LOADS; ++c)
 				stage[c] = *(const v4i *) (src[c] + o);
 		}
-#pragma unroll
-		for (int kk = 0; kk < KK; ++kk) {
-			v4i fa[WTM][2];
-#pragma unroll
-			for (int i = 0; i < WTM; ++i)
-#pragma unroll
-				for (int l = 0; l < 2; ++l)
-					fa[i][l] = *(const v4i *) (la + (wr * WTM + i) * T::CHUNK + kk * 2048 + l * 1024 + lane * 16);
-#pragma unroll
-			for (int j = 0; j < WTN; ++j) {
-				const v4i fb0 = *(const v4i *) (lb + (wc * WTN + j) * T::CHUNK + kk * 2048 + lane * 16);
-				const v4i fb1 = *(const v4i *) (lb + (wc * WTN + j) * T::CHUNK + kk * 2048 + 1024 + lane * 16);
-#pragma unroll
-				for (int i = 0; i < WTM; ++i)
-					acc[0][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][0], fb0, acc[0][i][j], 0, 0, 0);
-#pragma unroll
-				for (int i = 0; i < WTM; ++i)
-					acc[1][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][0], fb1, acc[1][i][j], 0, 0, 0);
-#pragma unroll
-				for (int i = 0; i < WTM; ++i)
-					acc[2][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][1], fb1, acc[2][i][j], 0, 0, 0);
-#pragma unroll
-				for (int i = 0; i < WTM; ++i)
-					acc[1][i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[i][1], fb0, acc[1][i][j], 0, 0, 0);
-			}
-		}
+		i8_stage_mfma<T, WTM, WTN, KK>(la, lb, wr, wc, lane, acc);
 	}
-
-	// Epilogue.  32x32 accumulator layout: lane l, register r holds
-	// row 8*(r/4) + 4*(l/32) + r%4, column l%32.
-#pragma unroll
-	for (int i = 0; i < WTM; ++i)
-#pragma unroll
-		for (int j = 0; j < WTN; ++j) {
-			const int col = tn * T::BN + wc * 32 * WTN + j * 32 + (lane & 31);
-#pragma unroll
-			for (int r = 0; r < 16; ++r) {
-				const int row = tm * T::BM + wr * 32 * WTM + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-				if (row < m && col < n) {
-					// 65536*HH + 256*(HL+LH) + LL has |.| < 2^48 and a*d + b*c < 2^33
-					// (p <= 2^16): both are exact doubles, reduced by mod_f64.
-					const double v = fma((double) acc[0][i][j][r], 65536.0,
-							     fma((double) acc[1][i][j][r], 256.0, (double) acc[2][i][j][r]));
-					const double d = (double) mp.mod_f64(v);
-					typename Elem<E>::T *cp = C + (size_t) row * ldc + col;
-					double t = (double) a * d;
-					if (b)
-						t = fma((double) b, (double) Elem<E>::load(cp, (uint32_t) mp.p), t);
-					Elem<E>::store(cp, mp.mod_f64(t), (uint32_t) mp.p);
-				}
-			}
-		}
+	i8_epilogue<E, T, WTM, WTN>(acc, C, ldc, m, n, a, b, mp, tm, tn, wr, wc, lane);
 }
 
 template <int E, int WTM, int WTN, int KK, int OCC, int WM, int WN>
@@ -293,6 +309,116 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_kernel(const int8_t
 	i8_tile<E, WTM, WTN, KK, WM, WN>(Ap, Bp, KB, C, ldc, m, n, a, b, mp, tm, tn, lds);
 }
 
+// The same tile with the k-steps staged by direct global -> LDS loads
+// (global_load_lds_dwordx4: lane l of a wave writes its 16 bytes at the
+// wave's LDS address + 16*l, which is the register-staged layout above)
+// through NS buffers, so that NS-1 steps are in flight while one is
+// multiplied and no registers hold staged data.  The buffers are distinct
+// LDS arrays, so the compiler can tell the loads into one from the reads of
+// another and does not drain the loads in flight before each read.  Step j
+// waits for its own loads with vmcnt (per wave, completed in issue order),
+// then a barrier makes every wave's part visible and guarantees that the
+// buffer refilled next (the one step j-1 read) is no longer read.
+#define MLZ_AS_GLOBAL __attribute__((address_space(1)))
+#define MLZ_AS_LDS __attribute__((address_space(3)))
+
+// s_waitcnt vmcnt(N) with no expcnt / lgkmcnt condition (gfx9 encoding:
+// vmcnt in bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
+{
+	static_assert(N >= 0 && N < 64, "vmcnt has 6 bits");
+	__builtin_amdgcn_s_waitcnt((N & 0xf) | ((N >> 4) << 14) | (0x7 << 4) | (0xf << 8));
+}
+
+// This wave's LDS reads complete (lgkmcnt(0)), then the workgroup barrier.
+// Global loads in flight are not waited for (the fence of __syncthreads()
+// would drain them); the empty asm statements keep the compiler from moving
+// memory accesses across the barrier.
+__device__ __forceinline__ void lds_barrier()
+{
+	__builtin_amdgcn_s_waitcnt(0xf | (0x3 << 14) | (0x7 << 4));
+	asm volatile("" ::: "memory");
+	__builtin_amdgcn_s_barrier();
+	asm volatile("" ::: "memory");
+}
+
+template <int E, int WTM, int WTN, int KK, int OCC, int WM, int WN, int NS>
+__global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_async_kernel(const int8_t *__restrict__ Ap,
+								 const int8_t *__restrict__ Bp, int KB,
+								 typename Elem<E>::T *__restrict__ C, size_t ldc,
+								 int m, int n, uint32_t a, uint32_t b, ModP mp,
+								 int tiles_m, int tiles_n)
+{
+	typedef I8Tile<WTM, WTN, KK, WM, WN> T;
+	static_assert(NS == 3 || NS == 4, "three or four stage buffers");
+	__shared__ __attribute__((aligned(16))) int8_t s0[T::STAGE], s1[T::STAGE], s2[T::STAGE],
+		s3[NS > 3 ? T::STAGE : 16];
+	int tm, tn;
+	tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int wr = wave / WN, wc = wave % WN;
+	const int wave_off = __builtin_amdgcn_readfirstlane(wave) * 1024;
+
+	const size_t panel = (size_t) KB * 2 * I8_FRAG;
+	const int8_t *Ag = Ap + (size_t) (tm * T::RBA) * panel;
+	const int8_t *Bg = Bp + (size_t) (tn * T::RBB) * panel;
+	const int8_t *src[T::LOADS];
+#pragma unroll
+	for (int c = 0; c < T::LOADS; ++c) {
+		const int g = c * 16 * T::THREADS + tid * 16;
+		const int blk = g / T::CHUNK, o = g % T::CHUNK;
+		src[c] = blk < T::RBA ? Ag + (size_t) blk * panel + o : Bg + (size_t) (blk - T::RBA) * panel + o;
+	}
+	auto buf = [&](int s) -> int8_t * { return s == 0 ? s0 : s == 1 ? s1 : s == 2 ? s2 : s3; };
+	auto issue = [&](int step, int8_t *dst) {
+		const size_t o = (size_t) step * T::CHUNK;
+#pragma unroll
+		for (int c = 0; c < T::LOADS; ++c)
+			__builtin_amdgcn_global_load_lds((const MLZ_AS_GLOBAL void *) (src[c] + o),
+							 (MLZ_AS_LDS void *) (dst + c * 16 * T::THREADS + wave_off), 16, 0, 0);
+	};
+
+	v16i acc[3][WTM][WTN];
+#pragma unroll
+	for (int t = 0; t < 3; ++t)
+#pragma unroll
+		for (int i = 0; i < WTM; ++i)
+#pragma unroll
+			for (int j = 0; j < WTN; ++j)
+				acc[t][i][j] = v16i{0};
+
+	// Every step issues one stage of loads, so the wait count is the same
+	// at every step: past the last step the loads repeat step KS - 1 into the
+	// buffer just read, which nothing reads again.  (Conditional issues
+	// would also make the compiler drain all loads before some reads.)
+	// The main loop runs whole groups of NS steps with no branch inside, so
+	// that the compiler sees the same loads in flight at every read; the
+	// remaining steps follow it.
+	const int KS = KB / KK;
+	auto step = [&](int j, int s) {
+		wait_vmcnt<T::LOADS * (NS - 2)>();   // step j's loads are done
+		lds_barrier();
+		issue(min(j + NS - 1, KS - 1), buf((s + NS - 1) % NS));
+		i8_stage_mfma<T, WTM, WTN, KK>(buf(s), buf(s) + T::RBA * T::CHUNK, wr, wc, lane, acc);
+	};
+#pragma unroll
+	for (int s = 0; s < NS - 1; ++s)
+		issue(min(s, KS - 1), buf(s));
+	int ks0 = 0;
+	for (; ks0 + NS <= KS; ks0 += NS) {
+#pragma unroll
+		for (int s = 0; s < NS; ++s)
+			step(ks0 + s, s);
+	}
+#pragma unroll
+	for (int s = 0; s < NS - 1; ++s)
+		if (ks0 + s < KS)
+			step(ks0 + s, s);
+	wait_vmcnt<0>();   // no load into LDS outlives the workgroup
+	i8_epilogue<E, T, WTM, WTN>(acc, C, ldc, m, n, a, b, mp, tm, tn, wr, wc, lane);
+}
+
 // Tile variants of the INT8 kernel (index = ctx option "gemm.i8.variant"):
 //   1: 128 x 128, 4 waves, 64 k per step
 //   2: 128 x 256, 4 waves of 64 x 128, 32 k per step
@@ -300,9 +426,13 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_kernel(const int8_t
 //   4: 128 x 128, 4 waves, 32 k per step, 2 waves/SIMD
 //   5: 128 x 256, 8 waves of 64 x 64, 32 k per step, 2 waves/SIMD
 //   6: 256 x 128, 8 waves of 64 x 64, 32 k per step, 2 waves/SIMD
-enum { I8V_AUTO = 0, I8V_COUNT = 7 };
+//   7: as 5, staged by direct loads through 3 LDS buffers
+//   8: as 5, staged by direct loads through 4 LDS buffers
+//   9: as 5 with 64 k per step, direct loads through 3 LDS buffers
+//  10: as 4, staged by direct loads through 4 LDS buffers
+enum { I8V_AUTO = 0, I8V_COUNT = 11 };
 
-template <int E, int WTM, int WTN, int KK, int OCC, int WM = 2, int WN = 2>
+template <int E, int WTM, int WTN, int KK, int OCC, int WM = 2, int WN = 2, int NS = 0>
 static int gemm_i8_launch_t(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, uint32_t a,
 			    const void *A, size_t lda, const void *B, size_t ldb, uint32_t b, void *C,
 			    size_t ldc)
@@ -345,24 +475,33 @@ static int gemm_i8_launch_t(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t
 		const int tiles_m = (int) (mpad / Tile::BM), tiles_n = (int) (npad / Tile::BN);
 		{
 			ProfScope ps(ctx, "gemm_i8_mfma");
-			gemm_i8_kernel<E, WTM, WTN, KK, OCC, WM, WN><<<tiles_m * tiles_n, Tile::THREADS, 0, ctx->stream>>>(
-				(const int8_t *) ap, (const int8_t *) bp, KB, (T *) C, ldc, (int) m, (int) n, a,
-				k0 == 0 ? b : 1u, mp, tiles_m, tiles_n);
+			if constexpr (NS == 0)
+				gemm_i8_kernel<E, WTM, WTN, KK, OCC, WM, WN><<<tiles_m * tiles_n, Tile::THREADS, 0, ctx->stream>>>(
+					(const int8_t *) ap, (const int8_t *) bp, KB, (T *) C, ldc, (int) m, (int) n, a,
+					k0 == 0 ? b : 1u, mp, tiles_m, tiles_n);
+			else
+				gemm_i8_async_kernel<E, WTM, WTN, KK, OCC, WM, WN, NS>
+					<<<tiles_m * tiles_n, Tile::THREADS, 0, ctx->stream>>>(
+						(const int8_t *) ap, (const int8_t *) bp, KB, (T *) C, ldc, (int) m, (int) n,
+						a, k0 == 0 ? b : 1u, mp, tiles_m, tiles_n);
 		}
 		MLZ_TRY(check_launch(ctx, "gemm_i8_kernel"));
 	}
 	return MLZ_OK;
 }
 
-// Variant choice when none is forced (measured at 8192^3: 5 > 6 > 4 > 1;
-// 2 and 3 spill registers): the 8-wave 128 x 256 tile once the output fills
-// the 256 CUs with it, else the 128 x 128 tile at two workgroups per CU.
-static int i8_variant_for(size_t m, size_t n)
+// Variant choice when none is forced.  The multi-buffered variants beat the
+// register-staged ones at every measured shape (8192^3: 8 > 7 > 10 > 5 > 4;
+// 2 and 3 spill registers).  The 8-wave 128 x 256 tile with four buffers wins
+// once the output fills the 256 CUs with it and k is long (4096^3, 1024 x
+// 8192 x 8192, 8192^3); with shorter k or fewer tiles the 128 x 128 tile at
+// two workgroups per CU does (2048^3, 8192 x 8192 x 1024, 3000 x 5000 x 2000).
+static int i8_variant_for(size_t m, size_t n, size_t k)
 {
 	const size_t tiles_wide = ceil_div(m, 128) * ceil_div(n, 256);
-	if (tiles_wide >= 256)
-		return 5;
-	return 4;
+	if (tiles_wide >= 256 && k >= 4096)
+		return 8;
+	return 10;
 }
 
 template <int E>
@@ -372,7 +511,7 @@ static int gemm_i8_launch(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k
 {
 	int v = ctx->i8_variant;
 	if (v <= I8V_AUTO || v >= I8V_COUNT)
-		v = i8_variant_for(m, n);
+		v = i8_variant_for(m, n, k);
 	switch (v) {
 	case 2:
 		return gemm_i8_launch_t<E, 2, 4, 1, 1>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
@@ -384,6 +523,14 @@ static int gemm_i8_launch(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k
 		return gemm_i8_launch_t<E, 2, 2, 1, 2, 2, 4>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
 	case 6:
 		return gemm_i8_launch_t<E, 2, 2, 1, 2, 4, 2>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
+	case 7:
+		return gemm_i8_launch_t<E, 2, 2, 1, 2, 2, 4, 3>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
+	case 8:
+		return gemm_i8_launch_t<E, 2, 2, 1, 2, 2, 4, 4>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
+	case 9:
+		return gemm_i8_launch_t<E, 2, 2, 2, 2, 2, 4, 3>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
+	case 10:
+		return gemm_i8_launch_t<E, 2, 2, 1, 2, 2, 2, 4>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
 	default:
 		return gemm_i8_launch_t<E, 2, 2, 2, 1>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
 	}

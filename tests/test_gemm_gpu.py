@@ -81,14 +81,18 @@ def test_gemm_strided_views(gpu_ctx, algo):
     assert np.array_equal(Cp[mask], outside[mask])
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", list(range(1, 11)))
 @pytest.mark.parametrize("storage", ["uint16", "uint32", "double"])
 def test_gemm_int8_tile_variants(gpu_ctx, variant, storage):
-    """Every INT8 workgroup-tile variant gives the same exact product."""
+    """Every INT8 workgroup-tile variant gives the same exact product.  The k
+    values give 2 .. 32 k-steps, so the multi-buffered variants (7-10) run
+    with fewer steps than buffers, whole groups only, and groups plus every
+    possible remainder."""
     p = 65521
     gpu_ctx.set_option("gemm.i8.variant", variant)
     try:
-        for (m, n, k) in [(300, 517, 130), (129, 257, 64), (1, 1, 33)]:
+        for (m, n, k) in [(300, 517, 130), (129, 257, 64), (1, 1, 33), (200, 300, 1000), (64, 64, 448),
+                          (130, 260, 96)]:
             A, B, C = synth.zp_matrix(41, m, k, p), synth.zp_matrix(42, k, n, p), synth.zp_matrix(43, m, n, p)
             got = run_gemm(gpu_ctx, p, storage, "int8", 7, A, B, 3, C)
             assert np.array_equal(got, orc.zp_gemm(p, 7, A, B, 3, C)), (variant, m, n, k)
