@@ -215,14 +215,37 @@ __device__ __forceinline__ void i8_epilogue(const v16i (&acc)[3][WTM][WTN], type
 					    size_t ldc, int m, int n, uint32_t a, uint32_t b, const ModP &mp, int tm,
 					    int tn, int wr, int wc, int lane)
 {
+	const uint32_t p = (uint32_t) mp.p;
 #pragma unroll
 	for (int i = 0; i < WTM; ++i)
 #pragma unroll
 		for (int j = 0; j < WTN; ++j) {
 			const int col = tn * T::BN + wc * 32 * WTN + j * 32 + (lane & 31);
+			const int row0 = tm * T::BM + wr * 32 * WTM + i * 32;
+			if (row0 + 32 <= m && col - (lane & 31) + 32 <= n) {
+				// Whole 32 x 32 block inside C: the 16 loads of C are issued
+				// together (one wait instead of one per element).
+				typename Elem<E>::T *cb = C + (size_t) (row0 + 4 * (lane >> 5)) * ldc + col;
+				uint32_t cv[16];
+				if (b) {
+#pragma unroll
+					for (int r = 0; r < 16; ++r)
+						cv[r] = Elem<E>::load(cb + (size_t) (8 * (r >> 2) + (r & 3)) * ldc, p);
+				}
+#pragma unroll
+				for (int r = 0; r < 16; ++r) {
+					const double v = fma((double) acc[0][i][j][r], 65536.0,
+							     fma((double) acc[1][i][j][r], 256.0, (double) acc[2][i][j][r]));
+					double t = (double) a * (double) mp.mod_f64(v);
+					if (b)
+						t = fma((double) b, (double) cv[r], t);
+					Elem<E>::store(cb + (size_t) (8 * (r >> 2) + (r & 3)) * ldc, mp.mod_f64(t), p);
+				}
+				continue;
+			}
 #pragma unroll
 			for (int r = 0; r < 16; ++r) {
-				const int row = tm * T::BM + wr * 32 * WTM + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+				const int row = row0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
 				if (row < m && col < n) {
 					// 65536*HH + 256*(HL+LH) + LL has |.| < 2^48 and a*d + b*c < 2^33
 					// (p <= 2^16): both are exact doubles, reduced by mod_f64.

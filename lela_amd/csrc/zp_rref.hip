@@ -41,6 +41,7 @@
 namespace mlz {
 
 constexpr int RREF_W = 64;        // panel width (columns)
+constexpr size_t RREF_NB = 256;   // column block of the blocked trailing update
 constexpr int RREF_CHUNK = 128;   // rows per tournament leaf (= two merged lists)
 constexpr int RREF_MAXLVL = 24;
 
@@ -56,6 +57,7 @@ struct RrefState {
 	int *bad;            // [1] set when a pivot is not invertible (composite p)
 	int *need_tree;      // [1] the fast path did not settle the panel
 	int *inv_done;       // [1] the fast path also produced P_J^-1
+	int *rank0;          // [1] rank when the current column block began
 	int *lists_rows;     // [levels][nodes0][RREF_W]
 	int *lists_cnt;      // [levels][nodes0]
 	int *counters;       // [levels][nodes0]
@@ -376,20 +378,26 @@ __global__ __launch_bounds__(1024) void rref_tourney_kernel(const uint32_t *__re
 }
 
 // P[t][col] = A[prow[t]][c + col] (t < k, else 0); marks the new pivot rows and
-// appends them to the pivot lists.
+// appends them to the pivot lists (pivot column coff + c + J[t]).  With a
+// tracker block at column inj of A (inj >= 0), pivot t also gets 1 added in
+// tracker column inj + (pivots found since the block began) + t.
 __global__ void rref_gather_kernel(const uint32_t *__restrict__ A, size_t lda, int nc, int c, int panel,
-				   RrefState st, uint32_t *__restrict__ P)
+				   RrefState st, uint32_t *__restrict__ P, int coff, int inj, uint32_t p)
 {
 	const int t = blockIdx.y;
 	const int k = *st.k;
 	const int col = blockIdx.x * blockDim.x + threadIdx.x;
-	if (col < nc)
-		P[(size_t) t * nc + col] = t < k ? A[(size_t) st.prow[t] * lda + c + col] : 0u;
+	if (col < nc) {
+		uint32_t x = t < k ? A[(size_t) st.prow[t] * lda + c + col] : 0u;
+		if (inj >= 0 && t < k && c + col == inj + (*st.rank - *st.rank0) + t)
+			x = x + 1 == p ? 0u : x + 1;
+		P[(size_t) t * nc + col] = x;
+	}
 	if (blockIdx.x == 0 && threadIdx.x == 0 && t < k) {
 		const int r = *st.rank;
 		st.is_pivot[st.prow[t]] = panel + 1;
 		st.piv_rows[r + t] = st.prow[t];
-		st.piv_cols[r + t] = c + st.J[t];
+		st.piv_cols[r + t] = coff + c + st.J[t];
 	}
 }
 
@@ -469,6 +477,49 @@ __global__ void rref_advance_kernel(RrefState st)
 	*st.rank += *st.k;
 }
 
+// Column blocks: X (m x xw) <- [A[:, c0 .. c0+nbw) | 0], and rank0 <- rank.
+__global__ void rref_block_load_kernel(const uint32_t *__restrict__ A, size_t lda, int m, int c0, int nbw,
+				       uint32_t *__restrict__ X, int xw, RrefState st)
+{
+	const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx == 0)
+		*st.rank0 = *st.rank;
+	if (idx >= (size_t) m * xw)
+		return;
+	const int i = (int) (idx / xw), j = (int) (idx % xw);
+	X[idx] = j < nbw ? A[(size_t) i * lda + c0 + j] : 0u;
+}
+
+// The block's columns back into A.
+__global__ void rref_block_store_kernel(const uint32_t *__restrict__ X, int xw, int m, int nbw,
+					uint32_t *__restrict__ A, size_t lda, int c0)
+{
+	const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (idx >= (size_t) m * nbw)
+		return;
+	const int i = (int) (idx / nbw), j = (int) (idx % nbw);
+	A[(size_t) i * lda + c0 + j] = X[(size_t) i * xw + j];
+}
+
+// After a column block: TR[t] = the trailing columns (from c0 + nbw on) of
+// the block's t-th pivot row (zero for t beyond the block's pivots), and the
+// tracker entry of that row loses the 1 its pivot step added, so that the
+// trailing columns of every row become A + E * TR.
+__global__ void rref_block_trail_kernel(uint32_t *__restrict__ X, int xw, int nbw, const uint32_t *__restrict__ A,
+					size_t lda, int c0, int nt, RrefState st, uint32_t *__restrict__ TR, uint32_t p)
+{
+	const int t = blockIdx.y;
+	const int kb = *st.rank - *st.rank0;
+	const int col = blockIdx.x * blockDim.x + threadIdx.x;
+	const int row = t < kb ? st.piv_rows[*st.rank0 + t] : -1;
+	if (col < nt)
+		TR[(size_t) t * nt + col] = row >= 0 ? A[(size_t) row * lda + c0 + nbw + col] : 0u;
+	if (blockIdx.x == 0 && threadIdx.x == 0 && row >= 0) {
+		uint32_t *e = X + (size_t) row * xw + nbw + t;
+		*e = *e == 0 ? p - 1 : *e - 1;
+	}
+}
+
 template <int E>
 __global__ void rref_load_kernel(const typename Elem<E>::T *__restrict__ src, size_t ld, int m, int n,
 				 uint32_t p, uint32_t *__restrict__ dst)
@@ -502,6 +553,7 @@ __global__ void rref_init_kernel(RrefState st, int m, int cnt_counters)
 		st.counters[i] = 0;
 	if (i == 0) {
 		*st.rank = 0;
+		*st.rank0 = 0;
 		*st.k = 0;
 		*st.bad = 0;
 		*st.need_tree = 0;
@@ -530,7 +582,7 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 	MLZ_TRY(ws_get(ctx, WS_RREF_1, 2 * (size_t) RREF_W * n * sizeof(uint32_t) +
 				      (size_t) RREF_W * RREF_W * sizeof(uint32_t), &w1));  // P, W, Pinv
 	MLZ_TRY(ws_get(ctx, WS_RREF_2, m * RREF_W * sizeof(uint32_t), &w2));         // G
-	const size_t ints = m + 1 + 2 * rmax + 1 + 2 * RREF_W + 3 + (size_t) RREF_MAXLVL * nodes0 * RREF_W +
+	const size_t ints = m + 1 + 2 * rmax + 1 + 2 * RREF_W + 4 + (size_t) RREF_MAXLVL * nodes0 * RREF_W +
 			    2 * (size_t) RREF_MAXLVL * nodes0;
 	MLZ_TRY(ws_get(ctx, WS_RREF_3, ints * sizeof(int), &w3));
 	uint32_t *Wm = (uint32_t *) w0;
@@ -548,6 +600,7 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 	st.bad = q; q += 1;
 	st.need_tree = q; q += 1;
 	st.inv_done = q; q += 1;
+	st.rank0 = q; q += 1;
 	st.lists_rows = q; q += (size_t) RREF_MAXLVL * nodes0 * RREF_W;
 	st.lists_cnt = q; q += (size_t) RREF_MAXLVL * nodes0;
 	st.counters = q;
@@ -568,26 +621,30 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 	// trailing updates on INT8 limbs whenever the gemm family allows them
 	const bool upd_i8 = p <= 65536 && (algo == MLZ_ALGO_AUTO || algo == MLZ_ALGO_INT8);
 	int panel = 0;
-	for (size_t c = 0; c < n; c += RREF_W, ++panel) {
-		const int wc = (int) std::min((size_t) RREF_W, n - c);
-		const int nc = (int) (n - c);
+	// One panel of X (row stride ldx, xn columns): pivots of columns
+	// [c, c + RREF_W), then every row's columns from c on are updated.
+	// coff = X's column offset in A; inj = tracker column of X or -1.
+	auto panel_step = [&](uint32_t *X, size_t ldx, size_t xn, size_t c, int coff, int inj) -> int {
+		const int wc = (int) std::min((size_t) RREF_W, xn - c);
+		const int nc = (int) (xn - c);
 		{
 			ProfScope ps(ctx, "rref_pivots");
 			if (mp.small) {
 				if (ctx->rref_fast_threads == 256)
-					rref_fast_kernel<true, 256><<<1, 256, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, Pinv);
+					rref_fast_kernel<true, 256><<<1, 256, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp, st, Pinv);
 				else if (ctx->rref_fast_threads == 512)
-					rref_fast_kernel<true, 512><<<1, 512, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, Pinv);
+					rref_fast_kernel<true, 512><<<1, 512, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp, st, Pinv);
 				else
-					rref_fast_kernel<true, 1024><<<1, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, Pinv);
-				rref_tourney_kernel<true><<<nodes0, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, nodes0);
+					rref_fast_kernel<true, 1024><<<1, 1024, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp, st, Pinv);
+				rref_tourney_kernel<true><<<nodes0, 1024, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp, st, nodes0);
 			} else {
-				rref_fast_kernel<false, 1024><<<1, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, Pinv);
-				rref_tourney_kernel<false><<<nodes0, 1024, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, wc, mp, st, nodes0);
+				rref_fast_kernel<false, 1024><<<1, 1024, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp, st, Pinv);
+				rref_tourney_kernel<false><<<nodes0, 1024, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp, st, nodes0);
 			}
 		}
 		MLZ_TRY(check_launch(ctx, "rref_tourney_kernel"));
-		rref_gather_kernel<<<dim3(ceil_div(nc, 256), RREF_W), 256, 0, ctx->stream>>>(Wm, n, nc, (int) c, panel, st, P);
+		rref_gather_kernel<<<dim3(ceil_div(nc, 256), RREF_W), 256, 0, ctx->stream>>>(X, ldx, nc, (int) c, panel, st, P,
+											coff, inj, pp);
 		{
 			ProfScope ps(ctx, "rref_inv");
 			if (mp.small)
@@ -597,19 +654,71 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 		}
 		MLZ_TRY(check_launch(ctx, "rref_inv_kernel"));
 		MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, RREF_W, nc, RREF_W, 1, Pinv, RREF_W, P, nc, 0, Wb, nc, algo));
-		rref_gatherG_kernel<<<ceil_div(m, 256 / RREF_W), 256, 0, ctx->stream>>>(Wm, n, (int) m, (int) c, panel, st, G);
+		rref_gatherG_kernel<<<ceil_div(m, 256 / RREF_W), 256, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, panel, st, G);
 		MLZ_TRY(check_launch(ctx, "rref_gatherG_kernel"));
 		if (upd_i8)
-			MLZ_TRY(zp_rank_update_i8_dev(ctx, pp, m, nc, RREF_W, G, RREF_W, Wb, nc, Wm + c, n));
+			MLZ_TRY(zp_rank_update_i8_dev(ctx, pp, m, nc, RREF_W, G, RREF_W, Wb, nc, X + c, ldx));
 		else
-			MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, m, nc, RREF_W, p - 1, G, RREF_W, Wb, nc, 1, Wm + c, n, algo));
-		rref_scatter_kernel<<<dim3(ceil_div(nc, 256), RREF_W), 256, 0, ctx->stream>>>(Wm, n, nc, (int) c, st, Wb);
+			MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, m, nc, RREF_W, p - 1, G, RREF_W, Wb, nc, 1, X + c, ldx, algo));
+		rref_scatter_kernel<<<dim3(ceil_div(nc, 256), RREF_W), 256, 0, ctx->stream>>>(X, ldx, nc, (int) c, st, Wb);
 		rref_advance_kernel<<<1, 1, 0, ctx->stream>>>(st);
 		MLZ_TRY(check_launch(ctx, "rref_scatter_kernel"));
-		if ((panel & 7) == 7 || c + RREF_W >= n) {
+		++panel;
+		return MLZ_OK;
+	};
+	auto rank_now = [&](int *r) -> int {
+		MLZ_HIP(ctx, hipMemcpyAsync(r, st.rank, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+		MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+		return MLZ_OK;
+	};
+	if (n < 2 * RREF_NB) {
+		// narrow matrices: panels over the whole width
+		for (size_t c = 0; c < n; c += RREF_W) {
+			MLZ_TRY(panel_step(Wm, n, n, c, 0, -1));
+			if ((panel & 7) == 0 || c + RREF_W >= n) {
+				int r = 0;
+				MLZ_TRY(rank_now(&r));
+				if ((size_t) r >= m)
+					break;
+			}
+		}
+	} else {
+		// Column blocks of RREF_NB: the block's panels update only the block
+		// (X = its columns, plus a tracker block E that starts at zero and
+		// receives a 1 in column j when the block's j-th pivot row is chosen).
+		// Every row's trailing columns then depend on the original ones only
+		// through the block's pivot rows TR: they become A + E * TR (with the
+		// pivot rows' own 1 taken out), one gemm of inner size RREF_NB instead
+		// of RREF_NB / RREF_W rank-RREF_W passes over the trailing matrix.
+		const size_t xmax = 2 * RREF_NB;
+		void *w4, *w5;
+		MLZ_TRY(ws_get(ctx, WS_RREF_4, m * xmax * sizeof(uint32_t), &w4));
+		MLZ_TRY(ws_get(ctx, WS_RREF_5, RREF_NB * (n - RREF_NB) * sizeof(uint32_t), &w5));
+		uint32_t *X = (uint32_t *) w4, *TR = (uint32_t *) w5;
+		for (size_t c0 = 0; c0 < n; c0 += RREF_NB) {
+			const size_t nbw = std::min(RREF_NB, n - c0), nt = n - c0 - nbw;
+			const size_t xw = nt ? 2 * nbw : nbw;
+			{
+				ProfScope ps(ctx, "rref_block");
+				rref_block_load_kernel<<<ceil_div(m * xw, 256), 256, 0, ctx->stream>>>(Wm, n, (int) m, (int) c0,
+													  (int) nbw, X, (int) xw, st);
+			}
+			MLZ_TRY(check_launch(ctx, "rref_block_load_kernel"));
+			for (size_t c = 0; c < nbw; c += RREF_W)
+				MLZ_TRY(panel_step(X, xw, nt ? xw : nbw, c, (int) c0, nt ? (int) nbw : -1));
+			{
+				ProfScope ps(ctx, "rref_block");
+				rref_block_store_kernel<<<ceil_div(m * nbw, 256), 256, 0, ctx->stream>>>(X, (int) xw, (int) m,
+													    (int) nbw, Wm, n, (int) c0);
+				if (nt)
+					rref_block_trail_kernel<<<dim3(ceil_div(nt, 256), nbw), 256, 0, ctx->stream>>>(
+						X, (int) xw, (int) nbw, Wm, n, (int) c0, (int) nt, st, TR, pp);
+			}
+			MLZ_TRY(check_launch(ctx, "rref_block_trail_kernel"));
+			if (nt)
+				MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, m, nt, nbw, 1, X + nbw, xw, TR, nt, 1, Wm + c0 + nbw, n, algo));
 			int r = 0;
-			MLZ_HIP(ctx, hipMemcpyAsync(&r, st.rank, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-			MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+			MLZ_TRY(rank_now(&r));
 			if ((size_t) r >= m)
 				break;
 		}
