@@ -246,6 +246,7 @@ enum WsSlot {
 	WS_GF2_0, WS_GF2_1, WS_GF2_2,
 	WS_BATCH_0, WS_BATCH_1, WS_BATCH_2, WS_BATCH_3,
 	WS_SMALL,
+	WS_INVTAB,                               // inverses mod p (p <= 2^16), see invtab_p
 	WS_NUM_SLOTS
 };
 
@@ -280,6 +281,7 @@ struct mlz_ctx {
 	int i8_variant;      // INT8 gemm tile variant, 0 = chosen by shape
 	int upd_nblk;        // 32-column blocks per wave of the rank-k update (1, 2, 4)
 	int rref_fast_threads;   // threads of the echelon form's fused pivot-search kernel (256, 512, 1024)
+	uint32_t invtab_p;       // modulus whose inverse table WS_INVTAB holds (0 = none)
 	struct mlz_tuner *tuner;   // autotuner table and cost-model weights (batch.hip), lazily created
 };
 

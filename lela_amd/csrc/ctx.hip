@@ -167,6 +167,7 @@ int mlz_ctx_create(mlz_ctx **out, int device, const char *config)
 	ctx->i8_variant = 0;
 	ctx->upd_nblk = 1;
 	ctx->rref_fast_threads = 1024;
+	ctx->invtab_p = 0;
 	ctx->tuner = nullptr;
 	// Default priority only: streams of different priorities are not used.
 	if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {

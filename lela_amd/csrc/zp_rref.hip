@@ -25,11 +25,14 @@
 //     new pivot rows get a zero multiplier row; for p <= 2^16 a rank-64
 //     update on INT8 limbs), then W is written into the pivot rows.
 // Eliminations inside a workgroup keep the matrix in registers: thread
-// (g, c) owns column c of the rows of group g, so a column step needs two
-// barriers and each thread's updates are independent.  Steps are fraction
-// free (no inverse on the critical path); the pivots of P_J are checked for
-// invertibility when it is inverted (LELA's check,
-// lela/algorithms/elimination.tcc:217-220).
+// (g, c) owns column c of the rows of group g, so each thread's updates are
+// independent.  Steps are fraction free (no inverse on the critical path),
+// except in step 1's fast path for p <= 2^16: there the pivot row is
+// normalised with an inverse from a table in LDS and the other rows are
+// updated without reduction (one fma per entry).  Pivots are checked for
+// invertibility (LELA's check, lela/algorithms/elimination.tcc:217-220).
+// For n >= 512 the panels run over 256-column blocks: a panel updates only
+// its block, and the columns right of the block get one gemm per block.
 // Rank, pivot rows and pivot columns stay on the device; the host only checks
 // the rank every few panels to stop early.  A final kernel writes the pivot
 // rows in pivot-column order followed by zero rows (standard layout).
@@ -177,6 +180,108 @@ __device__ int reg_eliminate(uint32_t (&v)[RPG], int nrows, int nsearch, bool re
 	return cnt;
 }
 
+// Gauss-Jordan with lazy reduction, for p <= 2^16 (reduced mode only).  The
+// entries are integer-valued doubles.  A chosen pivot row is normalised at
+// once (times the inverse of its lead) and every other row r becomes
+// v_r - f_r * g with f_r = v_r[col] mod p and g the normalised pivot row,
+// both below p, so after s steps |v| < p + s * (p-1)^2 < 2^53 for s <= 64:
+// one fma per entry and step and no reduction until the end (where v is
+// reduced to [0, p)).  The pivot search runs on wave 0: the owners of the
+// column put its entries in s_col, lane r reduces entry r, a ballot gives
+// the pivot row (the lowest live row with a nonzero entry) and its lane
+// looks up the lead's inverse in inv_tab (LDS); a lead without inverse
+// (composite p) sets *bad.  Three barriers per column and no chain of
+// per-row work in a single thread (a search by the column's owner threads,
+// with a reduction per entry, was twice as slow).  Requires nrows <= 64.
+template <int NC, int RPG>
+__device__ int reg_gj_wave(double (&v)[RPG], int nrows, int nsearch, const ModP &mp, int *alive, uint32_t *f,
+			   uint32_t *pivrow, double *s_col, int *s_pr, uint32_t *s_y, int *out_rows, int *out_cols,
+			   int *bad, const uint16_t *inv_tab)
+{
+	const int tid = threadIdx.x, c = tid % NC, g = tid / NC, rbase = g * RPG;
+	int cnt = 0;
+	for (int col = 0; col < nsearch; ++col) {
+		if (c == col) {
+#pragma unroll
+			for (int i = 0; i < RPG; ++i)
+				s_col[rbase + i] = v[i];
+		}
+		__syncthreads();
+		if (tid < 64) {
+			const int r = tid;
+			const uint32_t x = r < nrows ? mp.mod_f64(s_col[r]) : 0u;
+			const bool cand = r < nrows && x != 0 && alive[r];
+			const unsigned long long bal = __ballot(cand);
+			const int pr = bal ? __ffsll((long long) bal) - 1 : INT_MAX;
+			f[r] = r != pr ? x : 0u;
+			if (r == pr) {
+				const uint32_t y = inv_tab[x];
+				*s_y = y;
+				if (y == 0)
+					*bad = 1;
+				alive[r] = 0;
+				out_rows[cnt] = r;
+				out_cols[cnt] = col;
+			}
+			if (r == 0)
+				*s_pr = pr;
+		}
+		__syncthreads();
+		const int pr = *s_pr;
+		if (pr == INT_MAX)
+			continue;
+		if (pr / RPG == g) {
+			double xv = 0.0;
+#pragma unroll
+			for (int i = 0; i < RPG; ++i)
+				if (rbase + i == pr)
+					xv = v[i];
+			const uint32_t gv = mp.mul_small(mp.mod_f64(xv), *s_y);
+			pivrow[c] = gv;
+#pragma unroll
+			for (int i = 0; i < RPG; ++i)
+				if (rbase + i == pr)
+					v[i] = (double) gv;
+		}
+		__syncthreads();
+		const double gc = (double) pivrow[c];
+#pragma unroll
+		for (int i = 0; i < RPG; ++i) {
+			const uint32_t fr = f[rbase + i];
+			if (fr)
+				v[i] = fma(-(double) fr, gc, v[i]);
+		}
+		++cnt;
+	}
+#pragma unroll
+	for (int i = 0; i < RPG; ++i)
+		v[i] = (double) mp.mod_f64(v[i]);
+	__syncthreads();
+	return cnt;
+}
+
+// Inverses mod p for p <= 2^16 (0 where none exists).
+__global__ void invtab_kernel(uint16_t *__restrict__ tab, ModP mp)
+{
+	const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+	if (x < mp.p)
+		tab[x] = (uint16_t) mp.inv(x);
+}
+
+// The context's inverse table for p (built on first use per modulus).
+static int get_invtab(mlz_ctx *ctx, const ModP &mp, const uint16_t **out)
+{
+	void *w = nullptr;
+	MLZ_TRY(ws_get(ctx, WS_INVTAB, 65536 * sizeof(uint16_t), &w));
+	if (ctx->invtab_p != (uint32_t) mp.p) {
+		invtab_kernel<<<ceil_div(mp.p, 256), 256, 0, ctx->stream>>>((uint16_t *) w, mp);
+		MLZ_TRY(check_launch(ctx, "invtab_kernel"));
+		ctx->invtab_p = (uint32_t) mp.p;
+	}
+	*out = (const uint16_t *) w;
+	return MLZ_OK;
+}
+
 __device__ inline int nodes_at(int nodes0, int level)
 {
 	int n = nodes0;
@@ -193,15 +298,24 @@ __device__ inline int nodes_at(int nodes0, int level)
 // is R[row of pivot c][row of pivot t].  If S is rank deficient but holds
 // every live row, the pivots are still final and the inverse kernel builds
 // P_J^-1; otherwise the tournament decides.
-template <bool SMALL, int TPB>
+//
+// LAZY (p <= 2^16): reg_gj_wave with the inverse table copied into LDS;
+// otherwise fraction-free steps with a reduction per step (any p).
+template <bool SMALL, int TPB, bool LAZY>
 __global__ __launch_bounds__(TPB) void rref_fast_kernel(const uint32_t *__restrict__ A, size_t lda, int m, int c,
-						       int wc, ModP mp, RrefState st, uint32_t *__restrict__ Pinv)
+						       int wc, ModP mp, RrefState st, uint32_t *__restrict__ Pinv,
+						       const uint16_t *__restrict__ invtab)
 {
 	constexpr int W = RREF_W, NC = 2 * RREF_W, RPG = W / (TPB / NC);
 	__shared__ int rowid[W], alive[W], outr[W], outc[W], pos[W];
-	__shared__ uint32_t f[W], pivrow[NC];
-	__shared__ int s_min[2], s_cnt, s_wave[16];
+	__shared__ uint32_t f[W], pivrow[NC], s_y;
+	__shared__ int s_min[2], s_cnt, s_wave[16], s_pr;
+	__shared__ __attribute__((aligned(16))) uint16_t tab[LAZY ? 65536 : 8];
+	__shared__ double s_col[W];
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	if (LAZY)
+		for (int e = tid; e < (int) ((mp.p + 7) / 8); e += TPB)
+			((uint4 *) tab)[e] = ((const uint4 *) invtab)[e];
 	if (tid == 0)
 		s_cnt = 0;
 	__syncthreads();
@@ -244,7 +358,19 @@ __global__ __launch_bounds__(TPB) void rref_fast_kernel(const uint32_t *__restri
 		else
 			v[i] = (ci - W == r) ? 1u : 0u;
 	}
-	const int cnt = reg_eliminate<NC, RPG, SMALL>(v, W, wc, true, mp, alive, f, pivrow, s_min, outr, outc, st.bad);
+	int cnt;
+	if (!LAZY) {
+		cnt = reg_eliminate<NC, RPG, SMALL>(v, W, wc, true, mp, alive, f, pivrow, s_min, outr, outc, st.bad);
+	} else {
+		double vd[RPG];
+#pragma unroll
+		for (int i = 0; i < RPG; ++i)
+			vd[i] = (double) v[i];
+		cnt = reg_gj_wave<NC, RPG>(vd, W, wc, mp, alive, f, pivrow, s_col, &s_pr, &s_y, outr, outc, st.bad, tab);
+#pragma unroll
+		for (int i = 0; i < RPG; ++i)
+			v[i] = (uint32_t) vd[i];
+	}
 	const bool full = cnt == wc;
 	const bool done = full || scanned_all;
 	if (full) {
@@ -620,6 +746,10 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 	}
 	// trailing updates on INT8 limbs whenever the gemm family allows them
 	const bool upd_i8 = p <= 65536 && (algo == MLZ_ALGO_AUTO || algo == MLZ_ALGO_INT8);
+	// pivot search with lazy reduction and the inverse table for p <= 2^16
+	const uint16_t *invtab = nullptr;
+	if (p <= 65536)
+		MLZ_TRY(get_invtab(ctx, mp, &invtab));
 	int panel = 0;
 	// One panel of X (row stride ldx, xn columns): pivots of columns
 	// [c, c + RREF_W), then every row's columns from c on are updated.
@@ -629,18 +759,21 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 		const int nc = (int) (xn - c);
 		{
 			ProfScope ps(ctx, "rref_pivots");
+#define FAST(TPB, LAZY) rref_fast_kernel<true, TPB, LAZY><<<1, TPB, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, \
+										       mp, st, Pinv, invtab)
 			if (mp.small) {
-				if (ctx->rref_fast_threads == 256)
-					rref_fast_kernel<true, 256><<<1, 256, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp, st, Pinv);
-				else if (ctx->rref_fast_threads == 512)
-					rref_fast_kernel<true, 512><<<1, 512, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp, st, Pinv);
+				const int threads = ctx->rref_fast_threads;
+				if (invtab)
+					threads == 256 ? FAST(256, true) : threads == 512 ? FAST(512, true) : FAST(1024, true);
 				else
-					rref_fast_kernel<true, 1024><<<1, 1024, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp, st, Pinv);
+					threads == 256 ? FAST(256, false) : threads == 512 ? FAST(512, false) : FAST(1024, false);
 				rref_tourney_kernel<true><<<nodes0, 1024, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp, st, nodes0);
 			} else {
-				rref_fast_kernel<false, 1024><<<1, 1024, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp, st, Pinv);
+				rref_fast_kernel<false, 1024, false><<<1, 1024, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp,
+												   st, Pinv, nullptr);
 				rref_tourney_kernel<false><<<nodes0, 1024, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp, st, nodes0);
 			}
+#undef FAST
 		}
 		MLZ_TRY(check_launch(ctx, "rref_tourney_kernel"));
 		rref_gather_kernel<<<dim3(ceil_div(nc, 256), RREF_W), 256, 0, ctx->stream>>>(X, ldx, nc, (int) c, panel, st, P,

@@ -65,7 +65,7 @@ CASES = [
 
 
 @pytest.mark.parametrize("name,make", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("p", [2, 3, 65521, 2147483647])
+@pytest.mark.parametrize("p", [2, 3, 65521, 1000003, 8388593, 2147483647])
 def test_rref_matches_oracle(gpu_ctx, name, make, p):
     print(f"rref {name} p={p}", flush=True)
     check(gpu_ctx, p, "uint32", make(p))
@@ -126,3 +126,12 @@ def test_rref_tuning_options(gpu_ctx, threads, blocks):
     finally:
         gpu_ctx.set_option("rref.search.threads", 1024)
         gpu_ctx.set_option("rref.update.blocks", 1)
+
+
+def test_rref_composite_modulus(gpu_ctx):
+    """p = 15: a pivot without inverse is reported (LELA's check,
+    elimination.tcc:217-220); with invertible pivots the result is the
+    oracle's."""
+    with pytest.raises(la.LELAError):
+        run_rref(gpu_ctx, 15, "uint32", np.array([[3, 1], [6, 2]], dtype=np.uint32))
+    check(gpu_ctx, 15, "uint32", np.array([[2, 1, 7], [1, 1, 4]], dtype=np.uint32))
