@@ -344,6 +344,10 @@ int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size
 int pick_gemm_algo(uint64_t p, int elem, size_t m, size_t n, size_t k, int algo);
 int zp_rank_update_i8_dev(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, const uint32_t *G, size_t ldg,
 			  const uint32_t *W, size_t ldw, uint32_t *C, size_t ldc);
+// The same update with G (m x 32*KB) and W (32*KB x n) already packed as INT8
+// limb fragments ([32-index block][k block][limb][64 lanes][16 bytes]).
+int zp_rank_update_i8_packed(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, int KB, const int8_t *Gp,
+			     const int8_t *Wp, uint32_t *C, size_t ldc);
 int check_modulus(mlz_ctx *ctx, int elem, uint64_t p);
 void tuner_free(mlz_ctx *ctx);
 

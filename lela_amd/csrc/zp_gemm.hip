@@ -645,7 +645,6 @@ int zp_rank_update_i8_dev(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k
 		return MLZ_OK;
 	if (p > 65536 || k > (size_t) I8_KMAX)
 		return set_error(ctx, MLZ_E_UNSUPPORTED, "rank update needs p <= 2^16 and k <= %d", I8_KMAX);
-	const ModP mp = make_modp(p);
 	const size_t mpad = round_up(m, 64), npad = round_up(n, 64), kpad = round_up(k, 64);
 	const int KB = (int) (kpad / 32);
 	void *gp = nullptr, *wp = nullptr;
@@ -672,6 +671,16 @@ int zp_rank_update_i8_dev(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k
 											 (int8_t *) wp, KB);
 	}
 	MLZ_TRY(check_launch(ctx, "pack_i8_kernel"));
+	return zp_rank_update_i8_packed(ctx, p, m, n, KB, (const int8_t *) gp, (const int8_t *) wp, C, ldc);
+}
+
+int zp_rank_update_i8_packed(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, int KB, const int8_t *gp,
+			     const int8_t *wp, uint32_t *C, size_t ldc)
+{
+	if (m == 0 || n == 0)
+		return MLZ_OK;
+	const ModP mp = make_modp(p);
+	const size_t mpad = round_up(m, 64);
 	// strip width (32-column blocks per wave): wider strips reuse each row
 	// block's fragments more, narrower ones leave room for more waves
 	const int nblk = ctx->upd_nblk == 2 || ctx->upd_nblk == 4 ? ctx->upd_nblk : 1;

@@ -527,10 +527,129 @@ __global__ void rref_gather_kernel(const uint32_t *__restrict__ A, size_t lda, i
 	}
 }
 
-// Inverse of P_J (k x k, padded with the identity to RREF_W) by Gauss-Jordan on
-// [P_J | I] held in registers (128 columns x 2 groups of 32 rows).
+// The panel step after the pivot search, fused for p <= 2^16 (INT8 trailing
+// update).  Workgroup j owns P's columns [64j, 64j + 64): it gathers them from
+// the pivot rows (plus the tracker's 1, as rref_gather_kernel), forms
+// W = P_J^-1 * P for them (64 terms below 2^32 each: exact in a double),
+// writes W into the pivot rows (the update leaves those rows unchanged: their
+// multipliers are zero) and packs W as the update's INT8 B fragments
+// (KB = 2).  Workgroup 0 also marks the pivot rows and appends the pivot
+// lists.
+__global__ __launch_bounds__(256) void rref_wb_i8_kernel(uint32_t *__restrict__ A, size_t lda, int nc, int c,
+							 int panel, RrefState st, const uint32_t *__restrict__ Pinv,
+							 int8_t *__restrict__ Wp, int coff, int inj, ModP mp)
+{
+	constexpr int W = RREF_W;
+	__shared__ uint32_t Ps[W][W + 1], Pi[W][W + 1];
+	__shared__ int prow[W];
+	const uint32_t p = (uint32_t) mp.p;
+	const int tid = threadIdx.x;
+	const int j0 = blockIdx.x * W;
+	const int k = *st.k;
+	const int kb = *st.rank - *st.rank0;
+	if (tid < W)
+		prow[tid] = tid < k ? st.prow[tid] : -1;
+	__syncthreads();
+	for (int e = tid; e < W * W; e += 256) {
+		const int t = e / W, col = e % W, gc = j0 + col;
+		uint32_t x = 0;
+		if (t < k && gc < nc) {
+			x = A[(size_t) prow[t] * lda + c + gc];
+			if (inj >= 0 && c + gc == inj + kb + t)
+				x = x + 1 == p ? 0u : x + 1;
+		}
+		Ps[t][col] = x;
+		Pi[t][col] = Pinv[e];
+	}
+	__syncthreads();
+	// W on v_mfma_f64_16x16x4f64: wave w computes rows 16w .. 16w+15 as four
+	// 16 x 16 blocks.  A fragment: lane l holds (row l%16, k l/16); B
+	// fragment: (k l/16, column l%16); result register r of lane l: (row
+	// l/16 + 4r, column l%16).
+	{
+		const int lane = tid & 63, w = tid >> 6;
+		typedef double v4d __attribute__((ext_vector_type(4)));
+		v4d acc[4];
+#pragma unroll
+		for (int j = 0; j < 4; ++j)
+			acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+		for (int s = 0; s < W; s += 4) {
+			const double a = (double) Pi[16 * w + (lane & 15)][s + (lane >> 4)];
+#pragma unroll
+			for (int j = 0; j < 4; ++j)
+				acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, (double) Ps[s + (lane >> 4)][16 * j + (lane & 15)],
+									     acc[j], 0, 0, 0);
+		}
+		__syncthreads();
+#pragma unroll
+		for (int j = 0; j < 4; ++j)
+#pragma unroll
+			for (int r = 0; r < 4; ++r)
+				Ps[16 * w + (lane >> 4) + 4 * r][16 * j + (lane & 15)] = mp.mod_f64(acc[j][r]);
+	}
+	__syncthreads();
+	for (int e = tid; e < W * W; e += 256) {
+		const int t = e / W, cc = e % W, gc = j0 + cc;
+		if (t < k && gc < nc)
+			A[(size_t) prow[t] * lda + c + gc] = Ps[t][cc];
+	}
+	// B fragments: piece (column block cb, k block q, lane) holds column
+	// 32cb + lane%32 and k = 32q + 16(lane/32) .. +15
+	{
+		const int lane = tid & 63, q = (tid >> 6) & 1, cb = tid >> 7;
+		const int cc = 32 * cb + (lane & 31), t = 32 * q + 16 * (lane >> 5);
+		uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
+#pragma unroll
+		for (int b = 0; b < 16; ++b) {
+			const uint32_t l = i8_limbs(Ps[t + b][cc], p);
+			hw[b >> 2] |= ((l >> 8) & 0xffu) << (8 * (b & 3));
+			lw[b >> 2] |= (l & 0xffu) << (8 * (b & 3));
+		}
+		int8_t *fr = Wp + ((size_t) (2 * blockIdx.x + cb) * 2 + q) * 2 * 1024 + lane * 16;
+		*(uint4 *) fr = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+		*(uint4 *) (fr + 1024) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+	}
+	if (blockIdx.x == 0 && tid < k) {
+		const int r = *st.rank;
+		st.is_pivot[prow[tid]] = panel + 1;
+		st.piv_rows[r + tid] = prow[tid];
+		st.piv_cols[r + tid] = coff + c + st.J[tid];
+	}
+}
+
+// G[i][s] = A[i][c + J[s]] (s < k; zero on this panel's pivot rows), packed
+// as the update's INT8 A fragments (KB = 2), then rank += k.  Thread: the
+// 16-byte piece (row block, k block, lane) of one workgroup's 64 rows.
+__global__ __launch_bounds__(256) void rref_gpack_i8_kernel(const uint32_t *__restrict__ A, size_t lda, int m,
+							    int c, int panel, RrefState st, int8_t *__restrict__ Gp,
+							    uint32_t p)
+{
+	const int tid = threadIdx.x, lane = tid & 63, q = (tid >> 6) & 1, rb = 2 * blockIdx.x + (tid >> 7);
+	const int k = *st.k;
+	const int i = 32 * rb + (lane & 31), s0 = 32 * q + 16 * (lane >> 5);
+	const bool live = i < m && st.is_pivot[i] != panel + 1;
+	uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
+#pragma unroll
+	for (int b = 0; b < 16; ++b) {
+		const int s = s0 + b;
+		const uint32_t x = (live && s < k) ? A[(size_t) i * lda + c + st.J[s]] : 0u;
+		const uint32_t l = i8_limbs(x, p);
+		hw[b >> 2] |= ((l >> 8) & 0xffu) << (8 * (b & 3));
+		lw[b >> 2] |= (l & 0xffu) << (8 * (b & 3));
+	}
+	int8_t *fr = Gp + ((size_t) rb * 2 + q) * 2 * 1024 + lane * 16;
+	*(uint4 *) fr = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+	*(uint4 *) (fr + 1024) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+	if (blockIdx.x == 0 && tid == 0)
+		*st.rank += k;
+}
+
+// Inverse of P_J (k x k, padded with the identity to RREF_W; entry (r, s) =
+// A[prow[r]][c + J[s]]) by Gauss-Jordan on [P_J | I] held in registers (128
+// columns x 2 groups of 32 rows).
 template <bool SMALL>
-__global__ __launch_bounds__(1024) void rref_inv_kernel(const uint32_t *__restrict__ P, int nc, ModP mp,
+__global__ __launch_bounds__(1024) void rref_inv_kernel(const uint32_t *__restrict__ A, size_t lda, int c, ModP mp,
 						       RrefState st, uint32_t *__restrict__ Pinv)
 {
 	constexpr int W = RREF_W, NC = 2 * RREF_W, RPG = RREF_W / (1024 / NC);
@@ -539,17 +658,17 @@ __global__ __launch_bounds__(1024) void rref_inv_kernel(const uint32_t *__restri
 	__shared__ int s_min[2];
 	if (*st.inv_done)
 		return;
-	const int tid = threadIdx.x, c = tid % NC, g = tid / NC;
+	const int tid = threadIdx.x, cc = tid % NC, g = tid / NC;
 	const int k = *st.k;
 	uint32_t v[RPG];
 #pragma unroll
 	for (int i = 0; i < RPG; ++i) {
 		const int r = g * RPG + i;
 		uint32_t x;
-		if (c < W)
-			x = (r < k && c < k) ? P[(size_t) r * nc + st.J[c]] : (r == c ? 1u : 0u);
+		if (cc < W)
+			x = (r < k && cc < k) ? A[(size_t) st.prow[r] * lda + c + st.J[cc]] : (r == cc ? 1u : 0u);
 		else
-			x = (c - W == r) ? 1u : 0u;
+			x = (cc - W == r) ? 1u : 0u;
 		v[i] = x;
 	}
 	for (int r = tid; r < W; r += blockDim.x) {
@@ -563,11 +682,11 @@ __global__ __launch_bounds__(1024) void rref_inv_kernel(const uint32_t *__restri
 	for (int s = tid; s < cnt; s += blockDim.x)
 		pos[outr[s]] = outc[s];
 	__syncthreads();
-	if (c >= W) {
+	if (cc >= W) {
 #pragma unroll
 		for (int i = 0; i < RPG; ++i) {
 			const int r = g * RPG + i;
-			Pinv[(size_t) pos[r] * W + (c - W)] = v[i];
+			Pinv[(size_t) pos[r] * W + (cc - W)] = v[i];
 		}
 	}
 }
@@ -705,15 +824,18 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 	// workspace
 	void *w0, *w1, *w2, *w3;
 	MLZ_TRY(ws_get(ctx, WS_RREF_0, mn * sizeof(uint32_t), &w0));                 // working matrix
-	MLZ_TRY(ws_get(ctx, WS_RREF_1, 2 * (size_t) RREF_W * n * sizeof(uint32_t) +
-				      (size_t) RREF_W * RREF_W * sizeof(uint32_t), &w1));  // P, W, Pinv
-	MLZ_TRY(ws_get(ctx, WS_RREF_2, m * RREF_W * sizeof(uint32_t), &w2));         // G
+	// P (or W's INT8 fragments), W, Pinv; G (or its INT8 fragments)
+	const size_t pbytes = std::max((size_t) RREF_W * n * sizeof(uint32_t), round_up(n, 64) * 2 * RREF_W);
+	MLZ_TRY(ws_get(ctx, WS_RREF_1, pbytes + (size_t) RREF_W * n * sizeof(uint32_t) +
+				      (size_t) RREF_W * RREF_W * sizeof(uint32_t), &w1));
+	MLZ_TRY(ws_get(ctx, WS_RREF_2, std::max(m * RREF_W * sizeof(uint32_t), round_up(m, 64) * 2 * RREF_W), &w2));
 	const size_t ints = m + 1 + 2 * rmax + 1 + 2 * RREF_W + 4 + (size_t) RREF_MAXLVL * nodes0 * RREF_W +
 			    2 * (size_t) RREF_MAXLVL * nodes0;
 	MLZ_TRY(ws_get(ctx, WS_RREF_3, ints * sizeof(int), &w3));
 	uint32_t *Wm = (uint32_t *) w0;
-	uint32_t *P = (uint32_t *) w1, *Wb = P + (size_t) RREF_W * n, *Pinv = Wb + (size_t) RREF_W * n;
+	uint32_t *P = (uint32_t *) w1, *Wb = (uint32_t *) ((char *) w1 + pbytes), *Pinv = Wb + (size_t) RREF_W * n;
 	uint32_t *G = (uint32_t *) w2;
+	int8_t *Wp = (int8_t *) P, *Gp = (int8_t *) G;
 	RrefState st;
 	int *q = (int *) w3;
 	st.is_pivot = q; q += m;
@@ -776,23 +898,33 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 #undef FAST
 		}
 		MLZ_TRY(check_launch(ctx, "rref_tourney_kernel"));
-		rref_gather_kernel<<<dim3(ceil_div(nc, 256), RREF_W), 256, 0, ctx->stream>>>(X, ldx, nc, (int) c, panel, st, P,
-											coff, inj, pp);
 		{
 			ProfScope ps(ctx, "rref_inv");
 			if (mp.small)
-				rref_inv_kernel<true><<<1, 1024, 0, ctx->stream>>>(P, nc, mp, st, Pinv);
+				rref_inv_kernel<true><<<1, 1024, 0, ctx->stream>>>(X, ldx, (int) c, mp, st, Pinv);
 			else
-				rref_inv_kernel<false><<<1, 1024, 0, ctx->stream>>>(P, nc, mp, st, Pinv);
+				rref_inv_kernel<false><<<1, 1024, 0, ctx->stream>>>(X, ldx, (int) c, mp, st, Pinv);
 		}
 		MLZ_TRY(check_launch(ctx, "rref_inv_kernel"));
+		if (upd_i8) {
+			// W formed, written and packed in one kernel, G gathered and
+			// packed in another, then the INT8 update
+			ProfScope ps(ctx, "rref_update");
+			rref_wb_i8_kernel<<<ceil_div(nc, RREF_W), 256, 0, ctx->stream>>>(X, ldx, nc, (int) c, panel, st, Pinv,
+											 Wp, coff, inj, mp);
+			rref_gpack_i8_kernel<<<ceil_div(m, 64), 256, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, panel, st,
+											 Gp, pp);
+			MLZ_TRY(check_launch(ctx, "rref_gpack_i8_kernel"));
+			MLZ_TRY(zp_rank_update_i8_packed(ctx, pp, m, nc, 2, Gp, Wp, X + c, ldx));
+			++panel;
+			return MLZ_OK;
+		}
+		rref_gather_kernel<<<dim3(ceil_div(nc, 256), RREF_W), 256, 0, ctx->stream>>>(X, ldx, nc, (int) c, panel, st, P,
+											coff, inj, pp);
 		MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, RREF_W, nc, RREF_W, 1, Pinv, RREF_W, P, nc, 0, Wb, nc, algo));
 		rref_gatherG_kernel<<<ceil_div(m, 256 / RREF_W), 256, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, panel, st, G);
 		MLZ_TRY(check_launch(ctx, "rref_gatherG_kernel"));
-		if (upd_i8)
-			MLZ_TRY(zp_rank_update_i8_dev(ctx, pp, m, nc, RREF_W, G, RREF_W, Wb, nc, X + c, ldx));
-		else
-			MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, m, nc, RREF_W, p - 1, G, RREF_W, Wb, nc, 1, X + c, ldx, algo));
+		MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, m, nc, RREF_W, p - 1, G, RREF_W, Wb, nc, 1, X + c, ldx, algo));
 		rref_scatter_kernel<<<dim3(ceil_div(nc, 256), RREF_W), 256, 0, ctx->stream>>>(X, ldx, nc, (int) c, st, Wb);
 		rref_advance_kernel<<<1, 1, 0, ctx->stream>>>(st);
 		MLZ_TRY(check_launch(ctx, "rref_scatter_kernel"));
