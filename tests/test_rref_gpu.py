@@ -56,11 +56,12 @@ CASES = [
     ("zero", lambda p: np.zeros((70, 80), dtype=np.uint32)),
     ("one", lambda p: np.ones((1, 1), dtype=np.uint32)),
     ("ragged", lambda p: lowrank(7, 513, 129, 100, p)),
-    # n >= 512: 256-column blocks with a trailing gemm per block
+    # n >= 1024: 512-column blocks with a trailing gemm per block (the last
+    # block partial)
     ("blocks_wide", lambda p: synth.zp_matrix(12, 120, 1100, p)),
-    ("blocks_lowrank", lambda p: lowrank(13, 400, 900, 300, p)),
-    ("blocks_sparse", lambda p: sparse(14, 300, 700, p, 0.02)),
-    ("blocks_tall", lambda p: synth.zp_matrix(15, 700, 530, p)),
+    ("blocks_lowrank", lambda p: lowrank(13, 400, 1300, 300, p)),
+    ("blocks_sparse", lambda p: sparse(14, 300, 1100, p, 0.02)),
+    ("blocks_square", lambda p: synth.zp_matrix(15, 600, 1030, p)),
 ]
 
 
@@ -74,13 +75,13 @@ def test_rref_matches_oracle(gpu_ctx, name, make, p):
 @pytest.mark.parametrize("storage,p", [("uint16", 65521), ("double", 67108859), ("uint16", 251)])
 def test_rref_storages(gpu_ctx, storage, p):
     check(gpu_ctx, p, storage, lowrank(8, 200, 230, 150, p))
-    check(gpu_ctx, p, storage, lowrank(16, 260, 777, 240, p))
+    check(gpu_ctx, p, storage, lowrank(16, 260, 1100, 240, p))
 
 
 @pytest.mark.parametrize("algo", [la.ALGO_PLAIN, la.ALGO_INT8, la.ALGO_FP64])
 def test_rref_gemm_families(gpu_ctx, algo):
     check(gpu_ctx, 65521, "uint32", lowrank(9, 300, 280, 200, 65521), algo=algo)
-    check(gpu_ctx, 65521, "uint32", lowrank(17, 350, 800, 330, 65521), algo=algo)
+    check(gpu_ctx, 65521, "uint32", lowrank(17, 350, 1200, 330, 65521), algo=algo)
 
 
 def test_rref_many_pivot_chunks(gpu_ctx):
