@@ -187,72 +187,99 @@ __device__ int reg_eliminate(uint32_t (&v)[RPG], int nrows, int nsearch, bool re
 // v_r - f_r * g with f_r = v_r[col] mod p and g the normalised pivot row,
 // both below p, so after s steps |v| < p + s * (p-1)^2 < 2^53 for s <= 64:
 // one fma per entry and step and no reduction until the end (where v is
-// reduced to [0, p)).  The pivot search runs on wave 0: the owners of the
-// column put its entries in s_col, lane r reduces entry r, a ballot gives
-// the pivot row (the lowest live row with a nonzero entry) and its lane
-// looks up the lead's inverse in inv_tab (LDS); a lead without inverse
-// (composite p) sets *bad.  Three barriers per column and no chain of
-// per-row work in a single thread (a search by the column's owner threads,
-// with a reduction per entry, was twice as slow).  Requires nrows <= 64.
+// reduced to [0, p)).
+//
+// The pivot search runs on wave 0 (lane r = row r) one step ahead: the
+// owners of column col+2 publish it (Q) after their update of step col, and
+// after step col's pivot row is known wave 0 forms column col+1's values
+// itself, Q - f * g (the same fma the owners apply), reduces them, picks the
+// lowest live nonzero row by a ballot, and its lane looks up the lead's
+// inverse in inv_tab (LDS).  Two barriers per column: A (pivot, multipliers,
+// inverse published) and B (normalised pivot row published).  Q, the
+// multipliers (as doubles), the pivot row index and the inverse are double
+// buffered by column parity.  A lead without inverse (composite p) sets
+// *bad.  Requires nrows <= 64.
 template <int NC, int RPG>
-__device__ int reg_gj_wave(double (&v)[RPG], int nrows, int nsearch, const ModP &mp, int *alive, uint32_t *f,
-			   uint32_t *pivrow, double *s_col, int *s_pr, uint32_t *s_y, int *out_rows, int *out_cols,
+__device__ int reg_gj_wave(double (&v)[RPG], int nrows, int nsearch, const ModP &mp, double (*Q)[64],
+			   double (*fd)[64], int *s_pr, uint32_t *s_y, uint32_t *pivrow, int *out_rows, int *out_cols,
 			   int *bad, const uint16_t *inv_tab)
 {
 	const int tid = threadIdx.x, c = tid % NC, g = tid / NC, rbase = g * RPG;
-	int cnt = 0;
-	for (int col = 0; col < nsearch; ++col) {
-		if (c == col) {
+	// wave 0's search over the column values x (lane = row); cnt_w and
+	// alive_w are wave 0's own view
+	bool alive_w = tid < nrows;
+	int cnt_w = 0;
+	auto search = [&](int col, double x) {
+		const uint32_t xr = mp.mod_f64(x);
+		const bool cand = alive_w && xr != 0;
+		const unsigned long long bal = __ballot(cand);
+		const int pr = bal ? __ffsll((long long) bal) - 1 : -1;
+		const int b = col & 1;
+		fd[b][tid] = tid != pr ? (double) xr : 0.0;
+		if (tid == pr) {
+			const uint32_t y = inv_tab[xr];
+			s_y[b] = y;
+			if (y == 0)
+				*bad = 1;
+			out_rows[cnt_w] = pr;
+			out_cols[cnt_w] = col;
+		}
+		if (tid == 0)
+			s_pr[b] = pr;
+		if (pr >= 0) {
+			alive_w = alive_w && tid != pr;
+			++cnt_w;
+		}
+	};
+	// prologue: columns 0 and 1 published, column 0 searched
+	for (int q = 0; q < 2 && q < nsearch; ++q)
+		if (c == q) {
 #pragma unroll
 			for (int i = 0; i < RPG; ++i)
-				s_col[rbase + i] = v[i];
+				Q[q][rbase + i] = v[i];
 		}
-		__syncthreads();
-		if (tid < 64) {
-			const int r = tid;
-			const uint32_t x = r < nrows ? mp.mod_f64(s_col[r]) : 0u;
-			const bool cand = r < nrows && x != 0 && alive[r];
-			const unsigned long long bal = __ballot(cand);
-			const int pr = bal ? __ffsll((long long) bal) - 1 : INT_MAX;
-			f[r] = r != pr ? x : 0u;
-			if (r == pr) {
-				const uint32_t y = inv_tab[x];
-				*s_y = y;
-				if (y == 0)
-					*bad = 1;
-				alive[r] = 0;
-				out_rows[cnt] = r;
-				out_cols[cnt] = col;
-			}
-			if (r == 0)
-				*s_pr = pr;
-		}
-		__syncthreads();
-		const int pr = *s_pr;
-		if (pr == INT_MAX)
-			continue;
-		if (pr / RPG == g) {
+	__syncthreads();
+	if (tid < 64)
+		search(0, Q[0][tid]);
+	int cnt = 0;
+	for (int col = 0; col < nsearch; ++col) {
+		const int b = col & 1;
+		__syncthreads();                                   // A: step col's pivot is known
+		const int pr = s_pr[b];
+		if (pr >= 0 && pr / RPG == g) {
+			const uint32_t y = s_y[b];
 			double xv = 0.0;
 #pragma unroll
 			for (int i = 0; i < RPG; ++i)
 				if (rbase + i == pr)
 					xv = v[i];
-			const uint32_t gv = mp.mul_small(mp.mod_f64(xv), *s_y);
+			const uint32_t gv = mp.mul_small(mp.mod_f64(xv), y);
 			pivrow[c] = gv;
 #pragma unroll
 			for (int i = 0; i < RPG; ++i)
 				if (rbase + i == pr)
 					v[i] = (double) gv;
 		}
-		__syncthreads();
-		const double gc = (double) pivrow[c];
+		__syncthreads();                                   // B: the pivot row is published
+		if (pr >= 0) {
+			const double gc = (double) pivrow[c];
 #pragma unroll
-		for (int i = 0; i < RPG; ++i) {
-			const uint32_t fr = f[rbase + i];
-			if (fr)
-				v[i] = fma(-(double) fr, gc, v[i]);
+			for (int i = 0; i < RPG; ++i)
+				v[i] = fma(-fd[b][rbase + i], gc, v[i]);
+			++cnt;
 		}
-		++cnt;
+		if (col + 2 < nsearch && c == col + 2) {
+#pragma unroll
+			for (int i = 0; i < RPG; ++i)
+				Q[b][rbase + i] = v[i];
+		}
+		if (col + 1 < nsearch && tid < 64) {
+			// the pivot row's own entry is its normalised value
+			double x = Q[(col + 1) & 1][tid];
+			if (pr >= 0)
+				x = tid == pr ? (double) pivrow[col + 1] : fma(-fd[b][tid], (double) pivrow[col + 1], x);
+			search(col + 1, x);
+		}
 	}
 #pragma unroll
 	for (int i = 0; i < RPG; ++i)
@@ -309,10 +336,10 @@ __global__ __launch_bounds__(TPB) void rref_fast_kernel(const uint32_t *__restri
 {
 	constexpr int W = RREF_W, NC = 2 * RREF_W, RPG = W / (TPB / NC);
 	__shared__ int rowid[W], alive[W], outr[W], outc[W], pos[W];
-	__shared__ uint32_t f[W], pivrow[NC], s_y;
-	__shared__ int s_min[2], s_cnt, s_wave[16], s_pr;
+	__shared__ uint32_t f[W], pivrow[NC], s_y[2];
+	__shared__ int s_min[2], s_cnt, s_wave[16], s_pr[2];
 	__shared__ __attribute__((aligned(16))) uint16_t tab[LAZY ? 65536 : 8];
-	__shared__ double s_col[W];
+	__shared__ double s_q[LAZY ? 2 : 1][64], s_fd[LAZY ? 2 : 1][64];
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	if (LAZY)
 		for (int e = tid; e < (int) ((mp.p + 7) / 8); e += TPB)
@@ -367,7 +394,7 @@ __global__ __launch_bounds__(TPB) void rref_fast_kernel(const uint32_t *__restri
 #pragma unroll
 		for (int i = 0; i < RPG; ++i)
 			vd[i] = (double) v[i];
-		cnt = reg_gj_wave<NC, RPG>(vd, W, wc, mp, alive, f, pivrow, s_col, &s_pr, &s_y, outr, outc, st.bad, tab);
+		cnt = reg_gj_wave<NC, RPG>(vd, nr, wc, mp, s_q, s_fd, s_pr, s_y, pivrow, outr, outc, st.bad, tab);
 #pragma unroll
 		for (int i = 0; i < RPG; ++i)
 			v[i] = (uint32_t) vd[i];
