@@ -110,6 +110,16 @@ __global__ void fl_lastcol_kernel(const uint64_t *__restrict__ row_ptr, const ui
 	last[rowmap[i]] = mx;
 }
 
+// B[i][j0 .. j0 + w) = 0 for every row i < r (grid.y strides over the rows).
+__global__ void fl_zero_cols_kernel(uint32_t *__restrict__ B, int r, int nr, int j0, int w)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= w)
+		return;
+	for (int i = blockIdx.y; i < r; i += gridDim.y)
+		B[(size_t) i * nr + j0 + j] = 0u;
+}
+
 // G[i][s] = B[i][jd[s]]  (r x rd)
 __global__ void fl_gather_cols_kernel(const uint32_t *__restrict__ B, int r, int nr, const int *__restrict__ jd,
 				      int rd, uint32_t *__restrict__ G)
@@ -515,7 +525,23 @@ static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *
 		fl_gather_cols_kernel<<<ceil_div(r * rd, 256), 256, 0, ctx->stream>>>(dB, (int) r, (int) nr, (const int *) wj,
 										     (int) rd, (uint32_t *) wg);
 		MLZ_TRY(check_launch(ctx, "fl_gather_cols_kernel"));
-		MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, r, nr, rd, p - 1, wg, rd, dD, nr, 1, dB, nr, algo));
+		// On D's pivot columns the product is B's own column (RREF(D) has the
+		// identity there), so those columns of B become zero.  When they are
+		// one contiguous range (D of full row rank has them first), only the
+		// other columns go through the gemm and the range is cleared.
+		const size_t j0 = jd[0];
+		if (jd[rd - 1] - j0 == rd - 1) {
+			if (j0)
+				MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, r, j0, rd, p - 1, wg, rd, dD, nr, 1, dB, nr, algo));
+			if (j0 + rd < nr)
+				MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, r, nr - j0 - rd, rd, p - 1, wg, rd, dD + j0 + rd, nr, 1,
+						    dB + j0 + rd, nr, algo));
+			fl_zero_cols_kernel<<<dim3(ceil_div(rd, 256), std::min(r, (size_t) 65535)), 256, 0, ctx->stream>>>(
+				dB, (int) r, (int) nr, (int) j0, (int) rd);
+			MLZ_TRY(check_launch(ctx, "fl_zero_cols_kernel"));
+		} else {
+			MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, r, nr, rd, p - 1, wg, rd, dD, nr, 1, dB, nr, algo));
+		}
 		MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));   // jdi leaves scope
 	}
 	// 7. assembly: merge the X pivots and the D pivots by column

@@ -91,6 +91,23 @@ def test_fl_multiblock_dense_pivot_block(gpu_ctx):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("which", ["first", "middle"])
+def test_fl_zero_columns_in_d(gpu_ctx, which):
+    """Columns that are zero in B and D: D's pivot columns start after them
+    ("first") or skip one ("middle"), both forms of the back-substitution's
+    pivot-column handling."""
+    p = 65521
+    X = f4_matrix(21, 200, 260, p)
+    heads = {int(np.nonzero(row)[0][0]) for row in X if row.any()}
+    free = [j for j in range(X.shape[1]) if j not in heads]
+    cols = free[:3] if which == "first" else free[len(free) // 3: len(free) // 3 + 2]
+    X[:, cols] = 0
+    got, rank, det, piv = run_fl(gpu_ctx, p, "uint32", X)
+    rc, want, wrank, _, _ = orc.fl_echelonize(p, X)
+    assert rc == 0 and rank == wrank
+    assert np.array_equal(got, want)
+
+
 def test_fl_wrong_matrix_form(gpu_ctx):
     p = 101
     X = np.zeros((4, 6), dtype=np.uint32)
