@@ -152,7 +152,7 @@ __global__ void fl_assemble_kernel(const uint32_t *__restrict__ B, const uint32_
 			else
 				v = s >= 0 ? B[(size_t) s * nr + (-1 - cj)] : D[(size_t) (-1 - s) * nr + (-1 - cj)];
 		}
-		Elem<E>::store(R + (size_t) t * ldr + j, v % p, p);
+		Elem<E>::store(R + (size_t) t * ldr + j, v, p);   // B and D hold residues below p
 	}
 }
 
