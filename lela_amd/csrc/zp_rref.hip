@@ -288,6 +288,121 @@ __device__ int reg_gj_wave(double (&v)[RPG], int nrows, int nsearch, const ModP 
 	return cnt;
 }
 
+// Inverse of an 8 x 8 block B (LDS, row stride ldb, residues below p) on one
+// wave, by fraction-free Gauss-Jordan on [B | I] with row pivoting inside the
+// block: lane 8r + c holds entries (r, c) and (r, 8 + c), rows exchange values
+// by cross-lane reads, so no step needs a workgroup barrier.  On success
+// T[8 j + s] = (B^-1)[j][s]; returns false (on every lane) when B is
+// singular or a lead has no inverse (composite p).
+__device__ bool wave_inv8(const uint32_t *B, int ldb, uint32_t *T, const ModP &mp, const uint16_t *inv_tab)
+{
+	const int lane = threadIdx.x & 63, r = lane >> 3, c = lane & 7;
+	uint32_t m0 = B[r * ldb + c], m1 = r == c ? 1u : 0u;
+	bool used = false;
+	int pj[8];
+#pragma unroll
+	for (int j = 0; j < 8; ++j) {
+		const unsigned long long bal = __ballot(c == j && !used && m0 != 0);
+		if (bal == 0)
+			return false;
+		const int lp = __ffsll((long long) bal) - 1, pr = lp >> 3;
+		pj[j] = pr;
+		const uint32_t x = __shfl(m0, lp);
+		const uint32_t fr = __shfl(m0, 8 * r + j);
+		const uint32_t p0 = __shfl(m0, 8 * pr + c), p1 = __shfl(m1, 8 * pr + c);
+		if (r == pr) {
+			used = true;
+		} else {
+			m0 = mp.mul_subT<true>(x, m0, fr, p0);
+			m1 = mp.mul_subT<true>(x, m1, fr, p1);
+		}
+	}
+	// row pj[j] is the pivot of column j, with its lead in column j
+	uint32_t lead = 0;
+	int myj = 0;
+#pragma unroll
+	for (int j = 0; j < 8; ++j) {
+		const uint32_t lj = __shfl(m0, 8 * pj[j] + j);
+		if (r == pj[j]) {
+			lead = lj;
+			myj = j;
+		}
+	}
+	const uint32_t y = inv_tab[lead];
+	if (__ballot(y == 0))
+		return false;
+	T[8 * myj + c] = mp.mul_small(m1, y);
+	return true;
+}
+
+// [S | I] -> [I | S^-1] for a 64 x 64 S by Gauss-Jordan over eight 8-column
+// blocks (1024 threads; thread (g, c) holds column c of rows 8g .. 8g+7 as
+// integer-valued doubles).  Block q: the block's rows and columns go to LDS
+// reduced, one wave inverts the 8 x 8 diagonal block (wave_inv8), the block's
+// rows become U = T * X[block rows] and every other row r gets
+// X[r] - X[r, block] * U (8 products below p^2 each: after eight blocks
+// |X| < p + 64 p^2 < 2^53, exact).  Row 8q + j ends as the pivot row of
+// column 8q + j.  Returns false (uniformly, with v partly transformed) when
+// a diagonal block is singular.
+__device__ bool block_gj64(double (&v)[8], const ModP &mp, const uint16_t *tab, uint32_t (*Ls)[8],
+			   uint32_t (*XR)[128], uint32_t (*U)[128], uint32_t *T, int *s_ok)
+{
+	const int tid = threadIdx.x, c = tid & 127, g = tid >> 7;
+	for (int q = 0; q < 8; ++q) {
+		if (c >= 8 * q && c < 8 * q + 8) {
+#pragma unroll
+			for (int i = 0; i < 8; ++i)
+				Ls[8 * g + i][c - 8 * q] = mp.mod_f64(v[i]);
+		}
+		if (g == q) {
+#pragma unroll
+			for (int i = 0; i < 8; ++i)
+				XR[i][c] = mp.mod_f64(v[i]);
+		}
+		__syncthreads();
+		if (tid < 64) {
+			const bool ok = wave_inv8(&Ls[8 * q][0], 8, T, mp, tab);
+			if (tid == 0)
+				*s_ok = ok;
+		}
+		__syncthreads();
+		if (!*s_ok)
+			return false;
+		{
+			const int j = g;
+			const uint4 t0 = *(const uint4 *) &T[8 * j], t1 = *(const uint4 *) &T[8 * j + 4];
+			const uint32_t tj[8] = {t0.x, t0.y, t0.z, t0.w, t1.x, t1.y, t1.z, t1.w};
+			double a = 0.0;
+#pragma unroll
+			for (int s = 0; s < 8; ++s)
+				a = fma((double) tj[s], (double) XR[s][c], a);
+			U[j][c] = mp.mod_f64(a);
+		}
+		__syncthreads();
+		double uc[8];
+#pragma unroll
+		for (int s = 0; s < 8; ++s)
+			uc[s] = (double) U[s][c];
+#pragma unroll
+		for (int i = 0; i < 8; ++i) {
+			const int r = 8 * g + i;
+			if (g == q) {
+				v[i] = uc[i];
+			} else {
+				const uint4 l0 = *(const uint4 *) &Ls[r][0], l1 = *(const uint4 *) &Ls[r][4];
+				const uint32_t lr[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+				double acc = 0.0;
+#pragma unroll
+				for (int s = 0; s < 8; ++s)
+					acc = fma((double) lr[s], uc[s], acc);
+				v[i] -= acc;
+			}
+		}
+		__syncthreads();
+	}
+	return true;
+}
+
 // Inverses mod p for p <= 2^16 (0 where none exists).
 __global__ void invtab_kernel(uint16_t *__restrict__ tab, ModP mp)
 {
@@ -340,6 +455,10 @@ __global__ __launch_bounds__(TPB) void rref_fast_kernel(const uint32_t *__restri
 	__shared__ int s_min[2], s_cnt, s_wave[16], s_pr[2];
 	__shared__ __attribute__((aligned(16))) uint16_t tab[LAZY ? 65536 : 8];
 	__shared__ double s_q[LAZY ? 2 : 1][64], s_fd[LAZY ? 2 : 1][64];
+	constexpr bool BLK = LAZY && TPB == 1024;   // block Gauss-Jordan first (full 64 x 64 S)
+	__shared__ __attribute__((aligned(16))) uint32_t b_l[BLK ? 64 : 1][8], b_xr[BLK ? 8 : 1][128],
+		b_u[BLK ? 8 : 1][128], b_t[BLK ? 64 : 4];
+	__shared__ int b_ok;
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	if (LAZY)
 		for (int e = tid; e < (int) ((mp.p + 7) / 8); e += TPB)
@@ -394,7 +513,29 @@ __global__ __launch_bounds__(TPB) void rref_fast_kernel(const uint32_t *__restri
 #pragma unroll
 		for (int i = 0; i < RPG; ++i)
 			vd[i] = (double) v[i];
-		cnt = reg_gj_wave<NC, RPG>(vd, nr, wc, mp, s_q, s_fd, s_pr, s_y, pivrow, outr, outc, st.bad, tab);
+		bool blk = false;
+		if constexpr (BLK) {
+			if (nr == W && wc == W) {
+				blk = block_gj64(vd, mp, tab, b_l, b_xr, b_u, b_t, &b_ok);
+				if (blk) {
+					// row t is the pivot row of column t
+					for (int s = tid; s < W; s += TPB) {
+						outr[s] = s;
+						outc[s] = s;
+					}
+#pragma unroll
+					for (int i = 0; i < RPG; ++i)
+						vd[i] = (double) mp.mod_f64(vd[i]);
+					cnt = W;
+				} else {
+#pragma unroll
+					for (int i = 0; i < RPG; ++i)
+						vd[i] = (double) v[i];
+				}
+			}
+		}
+		if (!blk)
+			cnt = reg_gj_wave<NC, RPG>(vd, nr, wc, mp, s_q, s_fd, s_pr, s_y, pivrow, outr, outc, st.bad, tab);
 #pragma unroll
 		for (int i = 0; i < RPG; ++i)
 			v[i] = (uint32_t) vd[i];
