@@ -158,14 +158,51 @@ template <int RPT, int WPR>
 __device__ int gf2_reg_eliminate(uint64_t (&v)[RPT][WPR], bool (&alive)[RPT], int nrows, int nsearch, bool reduced,
 				 uint64_t *pivot, int *s_min, int *out_rows, int *out_cols)
 {
+	// Columns where no live row has a bit cannot hold a pivot (XORs of live
+	// rows keep them zero there): an OR over the live rows skips them, which
+	// makes rank-exhausted panels and zero blocks nearly free.
+	__shared__ unsigned long long s_or[WPR];
 	const int tid = threadIdx.x;
 	if (tid == 0) {
 		s_min[0] = INT_MAX;
 		s_min[1] = INT_MAX;
 	}
+	if (tid < WPR)
+		s_or[tid] = 0ull;
 	__syncthreads();
+	{
+		uint64_t orv[WPR];
+#pragma unroll
+		for (int w = 0; w < WPR; ++w)
+			orv[w] = 0ull;
+#pragma unroll
+		for (int q = 0; q < RPT; ++q)
+			if (tid + 256 * q < nrows && alive[q]) {
+#pragma unroll
+				for (int w = 0; w < WPR; ++w)
+					orv[w] |= v[q][w];
+			}
+#pragma unroll
+		for (int w = 0; w < WPR; ++w) {
+			uint32_t lo = (uint32_t) orv[w], hi = (uint32_t) (orv[w] >> 32);
+#pragma unroll
+			for (int off = 32; off >= 1; off >>= 1) {
+				lo |= (uint32_t) __shfl_xor((int) lo, off, 64);
+				hi |= (uint32_t) __shfl_xor((int) hi, off, 64);
+			}
+			if ((tid & 63) == 0 && (lo | hi))
+				atomicOr(&s_or[w], ((unsigned long long) hi << 32) | lo);
+		}
+	}
+	__syncthreads();
+	uint64_t live_cols[WPR];
+#pragma unroll
+	for (int w = 0; w < WPR; ++w)
+		live_cols[w] = s_or[w];
 	int cnt = 0;
 	for (int col = 0; col < nsearch; ++col) {
+		if (!((live_cols[col >> 6] >> (col & 63)) & 1ull))
+			continue;
 		int cand = INT_MAX;
 #pragma unroll
 		for (int q = RPT - 1; q >= 0; --q) {
