@@ -132,6 +132,7 @@ constexpr int G2_MAXLVL = 24;
 
 struct Gf2State {
 	int *is_pivot, *rank, *piv_rows, *piv_cols, *k, *J, *prow, *lists_rows, *lists_cnt, *counters, *need_tree;
+	int *inv_done;   // the fast path also produced P_J^-1
 };
 
 __device__ inline int bit_of(const uint64_t *row, int col)
@@ -154,9 +155,14 @@ __device__ inline int wave_min(int x)
 // (echelon form); reduced = true clears it in every other row (Gauss-Jordan).
 // Two barriers per column: the arrival slot s_min alternates between two
 // words so it can be reset while the other is in use.
-template <int RPT, int WPR>
+// Optionally a second block of words per row (x, XW words) is carried along
+// by every row operation without being searched (the identity part of a
+// fast-path elimination); keeping it apart leaves the searched words
+// addressable by selects rather than scratch memory.
+template <int RPT, int WPR, int XW = 0>
 __device__ int gf2_reg_eliminate(uint64_t (&v)[RPT][WPR], bool (&alive)[RPT], int nrows, int nsearch, bool reduced,
-				 uint64_t *pivot, int *s_min, int *out_rows, int *out_cols)
+				 uint64_t *pivot, int *s_min, int *out_rows, int *out_cols,
+				 uint64_t (*x)[XW > 0 ? XW : 1] = nullptr)
 {
 	// Columns where no live row has a bit cannot hold a pivot (XORs of live
 	// rows keep them zero there): an OR over the live rows skips them, which
@@ -224,6 +230,9 @@ __device__ int gf2_reg_eliminate(uint64_t (&v)[RPT][WPR], bool (&alive)[RPT], in
 #pragma unroll
 					for (int w = 0; w < WPR; ++w)
 						pivot[w] = v[q][w];
+#pragma unroll
+					for (int w = 0; w < XW; ++w)
+						pivot[WPR + w] = x[q][w];
 					alive[q] = false;
 				}
 			if (tid == 0) {
@@ -241,6 +250,9 @@ __device__ int gf2_reg_eliminate(uint64_t (&v)[RPT][WPR], bool (&alive)[RPT], in
 #pragma unroll
 				for (int w = 0; w < WPR; ++w)
 					v[q][w] ^= pivot[w];
+#pragma unroll
+				for (int w = 0; w < XW; ++w)
+					x[q][w] ^= pivot[WPR + w];
 			}
 		}
 		++cnt;
@@ -257,12 +269,18 @@ __device__ inline int g2_nodes_at(int nodes0, int level)
 	return n;
 }
 
-// Fast path: the first G2_CHUNK rows that are not pivots yet.
+// Fast path: the first G2_CHUNK rows S that are not pivots yet, reduced
+// (Gauss-Jordan) together with the identity, [S | I].  If S has full panel
+// rank its pivot rows are the panel's, and P_J^-1 comes out of the same
+// elimination: pivot rows only ever receive other pivot rows, so the
+// identity part of the pivot row of column J[t] has bits on pivot rows only,
+// and bit s of it is entry (t, pos of row s) of P_J^-1.
+constexpr int G2_IW = G2_CHUNK / 64;   // identity words per row
 __global__ __launch_bounds__(256) void gf2_fast_kernel(const uint64_t *__restrict__ A, size_t lda, int m, int cw,
-						       int nw, int wb, Gf2State st)
+						       int nw, int wb, Gf2State st, uint64_t *__restrict__ Pinv)
 {
-	__shared__ int rowid[G2_CHUNK], outr[G2_PB], outc[G2_PB];
-	__shared__ uint64_t pivot[G2P];
+	__shared__ int rowid[G2_CHUNK], outr[G2_PB], outc[G2_PB], pos[G2_CHUNK];
+	__shared__ uint64_t pivot[G2P + G2_IW];
 	__shared__ int s_min[2], s_cnt, s_wave[4];
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	if (tid == 0)
@@ -292,7 +310,7 @@ __global__ __launch_bounds__(256) void gf2_fast_kernel(const uint64_t *__restric
 		}
 	}
 	const int nr = min(s_cnt, G2_CHUNK);
-	uint64_t v[G2_SRPT][G2P];
+	uint64_t v[G2_SRPT][G2P], xi[G2_SRPT][G2_IW];
 	bool alive[G2_SRPT];
 #pragma unroll
 	for (int q = 0; q < G2_SRPT; ++q) {
@@ -301,9 +319,50 @@ __global__ __launch_bounds__(256) void gf2_fast_kernel(const uint64_t *__restric
 #pragma unroll
 		for (int w = 0; w < G2P; ++w)
 			v[q][w] = (r < nr && cw + w < nw) ? A[(size_t) rowid[r] * lda + cw + w] : 0ull;
+#pragma unroll
+		for (int w = 0; w < G2_IW; ++w)
+			xi[q][w] = (r >> 6) == w ? 1ull << (r & 63) : 0ull;
 	}
-	const int cnt = gf2_reg_eliminate<G2_SRPT, G2P>(v, alive, nr, wb, false, pivot, s_min, outr, outc);
-	const bool done = cnt == wb || scanned_all;
+	const int cnt = gf2_reg_eliminate<G2_SRPT, G2P, G2_IW>(v, alive, nr, wb, true, pivot, s_min, outr, outc, xi);
+	const bool full = cnt == wb;
+	const bool done = full || scanned_all;
+	if (full) {
+		// P_J^-1 row t (pivot of column outc[t]) from the identity part of row outr[t]
+		for (int s = tid; s < G2_CHUNK; s += blockDim.x)
+			pos[s] = -1;
+		__syncthreads();
+		for (int t = tid; t < cnt; t += blockDim.x)
+			pos[outr[t]] = t;
+		__syncthreads();
+#pragma unroll
+		for (int q = 0; q < G2_SRPT; ++q) {
+			const int r = tid + 256 * q;
+			const int t = pos[r];
+			if (t >= 0) {
+				uint64_t out[G2P] = {0, 0, 0, 0};
+#pragma unroll
+				for (int w = 0; w < G2_IW; ++w) {
+					uint64_t bits = xi[q][w];
+					while (bits) {
+						const int s = 64 * w + __ffsll((long long) bits) - 1;
+						bits &= bits - 1;
+						const int ts = pos[s];
+#pragma unroll
+						for (int w2 = 0; w2 < G2P; ++w2)
+							out[w2] |= (ts >> 6) == w2 ? 1ull << (ts & 63) : 0ull;
+					}
+				}
+#pragma unroll
+				for (int w = 0; w < G2P; ++w)
+					Pinv[(size_t) t * G2P + w] = out[w];
+			}
+		}
+		// rows beyond the panel width (a narrow last panel): identity
+		for (int t = cnt + tid; t < G2_PB; t += blockDim.x)
+#pragma unroll
+			for (int w = 0; w < G2P; ++w)
+				Pinv[(size_t) t * G2P + w] = (t >> 6) == w ? 1ull << (t & 63) : 0ull;
+	}
 	if (done)
 		for (int t = tid; t < G2_PB; t += blockDim.x) {
 			st.J[t] = t < cnt ? outc[t] : 0;
@@ -313,6 +372,7 @@ __global__ __launch_bounds__(256) void gf2_fast_kernel(const uint64_t *__restric
 		if (done)
 			*st.k = cnt;
 		*st.need_tree = done ? 0 : 1;
+		*st.inv_done = full ? 1 : 0;
 	}
 }
 
@@ -433,6 +493,8 @@ __global__ __launch_bounds__(256) void gf2_inv_kernel(const uint64_t *__restrict
 	__shared__ int outr[G2_PB], outc[G2_PB], pos[G2_PB];
 	__shared__ uint64_t pivot[2 * G2P];
 	__shared__ int s_min[2];
+	if (*st.inv_done)
+		return;
 	const int tid = threadIdx.x;
 	const int k = *st.k;
 	uint64_t v[1][2 * G2P];
@@ -511,6 +573,7 @@ __global__ void gf2_init_kernel(Gf2State st, int m, int cnt_counters)
 		*st.rank = 0;
 		*st.k = 0;
 		*st.need_tree = 0;
+		*st.inv_done = 0;
 	}
 }
 
@@ -553,7 +616,7 @@ int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size
 	MLZ_TRY(ws_get(ctx, WS_GF2_1, 2 * (size_t) G2_PB * nw * 8 + (size_t) G2_PB * G2P * 8, &w1));
 	MLZ_TRY(ws_get(ctx, WS_GF2_2, m * G2P * 8, &w2));
 	const size_t ints = m + 1 + 2 * rmax + 1 + 2 * G2_PB + (size_t) G2_MAXLVL * nodes0 * G2_PB +
-			    2 * (size_t) G2_MAXLVL * nodes0 + 1;
+			    2 * (size_t) G2_MAXLVL * nodes0 + 2;
 	MLZ_TRY(ws_get(ctx, WS_SMALL, ints * sizeof(int), &w3));
 	uint64_t *Wm = (uint64_t *) w0;
 	uint64_t *P = (uint64_t *) w1, *Wb = P + (size_t) G2_PB * nw, *Pinv = Wb + (size_t) G2_PB * nw;
@@ -570,6 +633,7 @@ int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size
 	st.lists_rows = q; q += (size_t) G2_MAXLVL * nodes0 * G2_PB;
 	st.lists_cnt = q; q += (size_t) G2_MAXLVL * nodes0;
 	st.need_tree = q; q += 1;
+	st.inv_done = q; q += 1;
 	st.counters = q;
 	{
 		const int cc = G2_MAXLVL * nodes0;
@@ -583,7 +647,7 @@ int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size
 		const int nwc = (int) (nw - cw);
 		{
 			ProfScope ps(ctx, "gf2_pivots");
-			gf2_fast_kernel<<<1, 256, 0, ctx->stream>>>(Wm, nw, (int) m, (int) cw, (int) nw, wb, st);
+			gf2_fast_kernel<<<1, 256, 0, ctx->stream>>>(Wm, nw, (int) m, (int) cw, (int) nw, wb, st, Pinv);
 			gf2_tourney_kernel<<<nodes0, 256, 0, ctx->stream>>>(Wm, nw, (int) m, (int) cw, (int) nw, wb, st, nodes0);
 		}
 		MLZ_TRY(check_launch(ctx, "gf2_tourney_kernel"));
