@@ -140,14 +140,6 @@ __device__ inline int bit_of(const uint64_t *row, int col)
 	return (int) ((row[col >> 6] >> (col & 63)) & 1ull);
 }
 
-__device__ inline int wave_min(int x)
-{
-#pragma unroll
-	for (int off = 32; off >= 1; off >>= 1)
-		x = min(x, __shfl_xor(x, off, 64));
-	return x;
-}
-
 // Gaussian elimination over GF(2) with the rows in registers: thread t holds
 // rows t + 256*q (q < RPT), WPR words each.  Bit columns 0..nsearch-1 are
 // processed in order; the pivot of a column is the live row of lowest index
@@ -209,14 +201,17 @@ __device__ int gf2_reg_eliminate(uint64_t (&v)[RPT][WPR], bool (&alive)[RPT], in
 	for (int col = 0; col < nsearch; ++col) {
 		if (!((live_cols[col >> 6] >> (col & 63)) & 1ull))
 			continue;
+		// the wave's lowest candidate row: rows tid + 256q grow with q and,
+		// within q, with the lane, so the first q with a candidate decides
 		int cand = INT_MAX;
 #pragma unroll
-		for (int q = RPT - 1; q >= 0; --q) {
+		for (int q = 0; q < RPT; ++q) {
 			const int r = tid + 256 * q;
-			if (r < nrows && alive[q] && ((v[q][col >> 6] >> (col & 63)) & 1ull))
-				cand = r;
+			const unsigned long long bal =
+				__ballot(r < nrows && alive[q] && ((v[q][col >> 6] >> (col & 63)) & 1ull));
+			if (cand == INT_MAX && bal)
+				cand = 256 * q + (tid & ~63) + __ffsll((long long) bal) - 1;
 		}
-		cand = wave_min(cand);
 		if ((tid & 63) == 0 && cand != INT_MAX)
 			atomicMin(&s_min[col & 1], cand);
 		__syncthreads();
@@ -325,7 +320,25 @@ __global__ __launch_bounds__(256) void gf2_fast_kernel(const uint64_t *__restric
 	}
 	const int cnt = gf2_reg_eliminate<G2_SRPT, G2P, G2_IW>(v, alive, nr, wb, true, pivot, s_min, outr, outc, xi);
 	const bool full = cnt == wb;
-	const bool done = full || scanned_all;
+	bool all_zero = false;
+	if (cnt == 0 && !scanned_all) {
+		// no bit in the first rows: if no live row has a bit in the panel
+		// the panel has no pivot, and the tournament is not needed
+		__shared__ int s_any;
+		if (tid == 0)
+			s_any = 0;
+		__syncthreads();
+		int any = 0;
+		for (int row = tid; row < m && !any; row += blockDim.x)
+			if (st.is_pivot[row] == 0)
+				for (int w = 0; w < G2P && cw + w < nw; ++w)
+					any |= A[(size_t) row * lda + cw + w] != 0ull;
+		if (any)
+			s_any = 1;
+		__syncthreads();
+		all_zero = s_any == 0;
+	}
+	const bool done = full || scanned_all || all_zero;
 	if (full) {
 		// P_J^-1 row t (pivot of column outc[t]) from the identity part of row outr[t]
 		for (int s = tid; s < G2_CHUNK; s += blockDim.x)
