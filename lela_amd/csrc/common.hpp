@@ -335,6 +335,11 @@ int zp_trsm_prepare(mlz_ctx *ctx, int elem, uint64_t p, int upper, int unit, siz
 int zp_trsm_apply(mlz_ctx *ctx, int elem, uint64_t p, int upper, int unit, size_t n, size_t nrhs, uint64_t a,
 		  const void *A, size_t lda, void *B, size_t ldb, const uint32_t *W, int algo);
 constexpr size_t TRSM_LEAF_ROWS = 64;   // block size of zp_trsm_prepare's inverses
+// X = a A^-1 B by 64-row blocks on INT8 MFMA (p <= 2^16), W from
+// zp_trsm_prepare for the same A; bend (upper only, or null = n): one past
+// the last column block I's band needs (a banded A skips the zero columns).
+int zp_trsm_blocked_i8(mlz_ctx *ctx, int elem, uint64_t p, int upper, size_t n, size_t nrhs, uint64_t a,
+		       const void *A, size_t lda, void *B, size_t ldb, const uint32_t *W, const int *bend);
 int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A, size_t lda,
 		size_t *rank_out, uint32_t *pivots_out, int algo);
 int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda,

@@ -156,183 +156,6 @@ __global__ void fl_assemble_kernel(const uint32_t *__restrict__ B, const uint32_
 	}
 }
 
-// ---------------------------------------------------------------------------
-// Step 3 on INT8 MFMA (p <= 2^16): one launch per 64-row block of A, from the
-// bottom.  Packed operands ([32-index block][k block][limb][64 lanes][16 B],
-// as zp_gemm.hip's INT8 kernels): the band A[I, i1 .. i1 + 32*kbn) of every
-// block (its rows, the columns right of its diagonal block up to its last
-// nonzero), the 64 x 64 inverses of the diagonal blocks, and X as it is
-// solved (B operand: column blocks, then row blocks).
-
-// 16-byte piece (hi and lo limb vectors) of the 16 values v[0..15].
-__device__ inline void fl_piece(const uint32_t (&v)[16], uint32_t p, int8_t *dst)
-{
-	uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
-#pragma unroll
-	for (int b = 0; b < 16; ++b) {
-		const uint32_t l = i8_limbs(v[b], p);
-		hw[b >> 2] |= ((l >> 8) & 0xffu) << (8 * (b & 3));
-		lw[b >> 2] |= (l & 0xffu) << (8 * (b & 3));
-	}
-	*(uint4 *) dst = make_uint4(hw[0], hw[1], hw[2], hw[3]);
-	*(uint4 *) (dst + 1024) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
-}
-
-// Band fragments: workgroup (q, I) packs k blocks 2q, 2q+1 of block I's band
-// (thread = piece (row block, k block, lane)); columns >= r are zero.
-__global__ __launch_bounds__(256) void fl_band_pack_kernel(const uint32_t *__restrict__ A, int r,
-							   const int *__restrict__ kbn, const size_t *__restrict__ off,
-							   int8_t *__restrict__ band, uint32_t p)
-{
-	const int I = blockIdx.y, kb = 2 * blockIdx.x + ((threadIdx.x >> 6) & 1);
-	const int n_kb = kbn[I];
-	if (kb >= n_kb)
-		return;
-	const int lane = threadIdx.x & 63, rb = threadIdx.x >> 7;
-	const int i = 64 * I + 32 * rb + (lane & 31);
-	const int i1 = min(64 * I + 64, r);
-	const int k0 = i1 + 32 * kb + 16 * (lane >> 5);
-	uint32_t v[16];
-#pragma unroll
-	for (int b = 0; b < 16; ++b)
-		v[b] = (i < r && k0 + b < r) ? A[(size_t) i * r + k0 + b] : 0u;
-	fl_piece(v, p, band + off[I] + ((size_t) rb * n_kb + kb) * 2048 + lane * 16);
-}
-
-// Diagonal-block inverse fragments (W from zp_trsm_prepare: block I's inverse
-// in rows 64I .. of a 64-column array, valid for its first nb rows/columns).
-__global__ __launch_bounds__(256) void fl_winv_pack_kernel(const uint32_t *__restrict__ W, int r,
-							   int8_t *__restrict__ wp, uint32_t p)
-{
-	const int I = blockIdx.x, nb = min(64, r - 64 * I);
-	const int lane = threadIdx.x & 63, kb = (threadIdx.x >> 6) & 1, rb = threadIdx.x >> 7;
-	const int t = 32 * rb + (lane & 31), s0 = 32 * kb + 16 * (lane >> 5);
-	uint32_t v[16];
-#pragma unroll
-	for (int b = 0; b < 16; ++b)
-		v[b] = (t < nb && s0 + b < nb) ? W[(size_t) (64 * I + t) * 64 + s0 + b] : 0u;
-	fl_piece(v, p, wp + (size_t) I * 8192 + (rb * 2 + kb) * 2048 + lane * 16);
-}
-
-// Block I = rows [i0, i0 + nbr) of X = A^-1 B; workgroup j owns columns
-// [64j, 64j + 64), wave (rb, cb) one 32 x 32 block.  T = B_I - band * X[i1..)
-// (kbn k blocks of packed X), then X_I = W_I T, written to B in place and,
-// packed, to Xp (rkb row blocks per column block) for the blocks above.
-__global__ __launch_bounds__(256) void fl_block_solve_kernel(uint32_t *__restrict__ B, int nr, int i0, int nbr,
-							     int kbn, const int8_t *__restrict__ band,
-							     const int8_t *__restrict__ winv, int8_t *__restrict__ Xp,
-							     int rkb, ModP mp)
-{
-	__shared__ uint32_t T[64][65];
-	const uint32_t p = (uint32_t) mp.p;
-	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, rb = wave >> 1, cb = wave & 1;
-	const int cbg = 2 * blockIdx.x + cb, col = 32 * cbg + (lane & 31);
-	const int i1 = i0 + 64;
-	// B_I's entries are loaded first: their latency overlaps the band product
-	uint32_t bv[16];
-	if (nbr == 64 && 64 * (int) blockIdx.x + 64 <= nr) {
-#pragma unroll
-		for (int r = 0; r < 16; ++r)
-			bv[r] = B[(size_t) (i0 + 32 * rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * nr + col];
-	} else {
-#pragma unroll
-		for (int r = 0; r < 16; ++r) {
-			const int row = 32 * rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-			bv[r] = (row < nbr && col < nr) ? B[(size_t) (i0 + row) * nr + col] : 0u;
-		}
-	}
-	v16i a0 = v16i{0}, a1 = v16i{0}, a2 = v16i{0};
-	{
-		const int8_t *ga = band + (size_t) rb * kbn * 2048 + lane * 16;
-		const int8_t *gx = Xp + ((size_t) cbg * rkb + i1 / 32) * 2048 + lane * 16;
-		// kbn is even; four k blocks' fragments are loaded before their
-		// MFMAs so that the loads' latency is paid once per group
-		int kb = 0;
-		for (; kb + 4 <= kbn; kb += 4) {
-			v4i ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-			for (int q = 0; q < 4; ++q) {
-				const size_t o = (size_t) (kb + q) * 2048;
-				ah[q] = *(const v4i *) (ga + o);
-				al[q] = *(const v4i *) (ga + o + 1024);
-				bh[q] = *(const v4i *) (gx + o);
-				bl[q] = *(const v4i *) (gx + o + 1024);
-			}
-#pragma unroll
-			for (int q = 0; q < 4; ++q) {
-				a0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah[q], bh[q], a0, 0, 0, 0);
-				a1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah[q], bl[q], a1, 0, 0, 0);
-				a2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al[q], bl[q], a2, 0, 0, 0);
-				a1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al[q], bh[q], a1, 0, 0, 0);
-			}
-		}
-		for (; kb < kbn; ++kb) {
-			const size_t o = (size_t) kb * 2048;
-			const v4i ah = *(const v4i *) (ga + o), al = *(const v4i *) (ga + o + 1024);
-			const v4i bh = *(const v4i *) (gx + o), bl = *(const v4i *) (gx + o + 1024);
-			a0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bh, a0, 0, 0, 0);
-			a1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bl, a1, 0, 0, 0);
-			a2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl, a2, 0, 0, 0);
-			a1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, a1, 0, 0, 0);
-		}
-	}
-	// T = B_I - band product; rows >= nbr and columns >= nr are zero
-#pragma unroll
-	for (int r = 0; r < 16; ++r) {
-		const int row = 32 * rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-		uint32_t tv = 0;
-		if (row < nbr && col < nr) {
-			const uint32_t d = mp.mod_f64(fma((double) a0[r], 65536.0, fma((double) a1[r], 256.0, (double) a2[r])));
-			tv = bv[r] >= d ? bv[r] - d : bv[r] + p - d;
-		}
-		T[row][32 * cb + (lane & 31)] = tv;
-	}
-	__syncthreads();
-	// X_I = W_I T: B fragments of T built from LDS, W_I's from winv
-	a0 = v16i{0};
-	a1 = v16i{0};
-	a2 = v16i{0};
-#pragma unroll
-	for (int kb = 0; kb < 2; ++kb) {
-		uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
-		const int s0 = 32 * kb + 16 * (lane >> 5), cc = 32 * cb + (lane & 31);
-#pragma unroll
-		for (int b = 0; b < 16; ++b) {
-			const uint32_t l = i8_limbs(T[s0 + b][cc], p);
-			hw[b >> 2] |= ((l >> 8) & 0xffu) << (8 * (b & 3));
-			lw[b >> 2] |= (l & 0xffu) << (8 * (b & 3));
-		}
-		const v4i bh = v4i{(int) hw[0], (int) hw[1], (int) hw[2], (int) hw[3]};
-		const v4i bl = v4i{(int) lw[0], (int) lw[1], (int) lw[2], (int) lw[3]};
-		const int8_t *gw = winv + (rb * 2 + kb) * 2048 + lane * 16;
-		const v4i ah = *(const v4i *) gw, al = *(const v4i *) (gw + 1024);
-		a0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bh, a0, 0, 0, 0);
-		a1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bl, a1, 0, 0, 0);
-		a2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl, a2, 0, 0, 0);
-		a1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, a1, 0, 0, 0);
-	}
-	__syncthreads();
-#pragma unroll
-	for (int r = 0; r < 16; ++r) {
-		const int row = 32 * rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-		const uint32_t x = mp.mod_f64(fma((double) a0[r], 65536.0, fma((double) a1[r], 256.0, (double) a2[r])));
-		if (row < nbr && col < nr)
-			B[(size_t) (i0 + row) * nr + col] = x;
-		T[row][32 * cb + (lane & 31)] = x;
-	}
-	__syncthreads();
-	// X_I packed: piece (column block, row block, lane) of this workgroup
-	{
-		const int cb2 = tid >> 7, kb2 = (tid >> 6) & 1;
-		const int cc = 32 * cb2 + (lane & 31), s0 = 32 * kb2 + 16 * (lane >> 5);
-		uint32_t v[16];
-#pragma unroll
-		for (int b = 0; b < 16; ++b)
-			v[b] = T[s0 + b][cc];
-		fl_piece(v, p, Xp + ((size_t) (2 * blockIdx.x + cb2) * rkb + i0 / 32 + kb2) * 2048 + lane * 16);
-	}
-}
-
 constexpr size_t FL_TRSM_BLOCK = 512;   // rows per block of the pivot-block solve (multiple of TRSM_LEAF_ROWS)
 static_assert(FL_TRSM_BLOCK % TRSM_LEAF_ROWS == 0, "solve blocks must align with the inverted diagonal blocks");
 
@@ -452,48 +275,19 @@ static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *
 		const size_t nb = (r + FL_TRSM_BLOCK - 1) / FL_TRSM_BLOCK;
 		const bool i8 = p <= 65536 && (algo == MLZ_ALGO_AUTO || algo == MLZ_ALGO_INT8) && r <= 65504;
 		if (i8) {
-			// 64-row blocks on INT8 MFMA: pack the bands, the diagonal
-			// inverses, then one solve launch per block from the bottom
-			const size_t nb64 = (r + 63) / 64, rpad = nb64 * 64, npad = round_up(nr, 64);
-			std::vector<int> kbn(nb64);
-			std::vector<size_t> off(nb64 + 1, 0);
-			int kbmax = 0;
+			// 64-row blocks on INT8 MFMA, each block's band ending one past
+			// its rows' last nonzero column
+			const size_t nb64 = (r + 63) / 64;
+			std::vector<int> c1(nb64);
 			for (size_t I = 0; I < nb64; ++I) {
 				const size_t i0 = 64 * I, i1 = std::min(r, i0 + 64);
-				size_t c1 = i1;
+				size_t c = i1;
 				for (size_t i = i0; i < i1; ++i)
-					c1 = std::max(c1, (size_t) last[i] + 1);
-				kbn[I] = (int) (2 * ((c1 - i1 + 63) / 64));
-				kbmax = std::max(kbmax, kbn[I]);
-				off[I + 1] = off[I] + (size_t) 2 * kbn[I] * 2048;
+					c = std::max(c, (size_t) last[i] + 1);
+				c1[I] = (int) c;
 			}
-			void *wx, *wbp;
-			MLZ_TRY(ws_get(ctx, WS_TRSM_1, rpad * npad * 2, &wx));
-			const size_t meta = round_up(nb64 * sizeof(int), 256) + round_up((nb64 + 1) * sizeof(size_t), 256);
-			MLZ_TRY(ws_get(ctx, WS_TRSM_2, meta + nb64 * 8192 + off[nb64] + 256, &wbp));
-			int *d_kbn = (int *) wbp;
-			size_t *d_off = (size_t *) ((char *) wbp + round_up(nb64 * sizeof(int), 256));
-			int8_t *winvp = (int8_t *) wbp + meta, *band = winvp + nb64 * 8192;
-			MLZ_HIP(ctx, hipMemcpyAsync(d_kbn, kbn.data(), nb64 * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-			MLZ_HIP(ctx, hipMemcpyAsync(d_off, off.data(), (nb64 + 1) * sizeof(size_t), hipMemcpyHostToDevice,
-						    ctx->stream));
-			const ModP mp = make_modp(p);
-			{
-				ProfScope ps(ctx, "fl_solve");
-				if (kbmax)
-					fl_band_pack_kernel<<<dim3(kbmax / 2, nb64), 256, 0, ctx->stream>>>(dA, (int) r, d_kbn, d_off,
-													      band, p);
-				fl_winv_pack_kernel<<<nb64, 256, 0, ctx->stream>>>(Winv, (int) r, winvp, p);
-				MLZ_TRY(check_launch(ctx, "fl_winv_pack_kernel"));
-				for (size_t I = nb64; I-- > 0;) {
-					const size_t i0 = 64 * I;
-					fl_block_solve_kernel<<<npad / 64, 256, 0, ctx->stream>>>(
-						dB, (int) nr, (int) i0, (int) std::min((size_t) 64, r - i0), kbn[I], band + off[I],
-						winvp + I * 8192, (int8_t *) wx, (int) (rpad / 32), mp);
-				}
-				MLZ_TRY(check_launch(ctx, "fl_block_solve_kernel"));
-				MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));   // kbn, off leave scope
-			}
+			ProfScope ps(ctx, "fl_solve");
+			MLZ_TRY(zp_trsm_blocked_i8(ctx, MLZ_U32, p, 1, r, nr, 1, dA, r, dB, nr, Winv, c1.data()));
 		}
 		for (size_t I = i8 ? 0 : nb; I-- > 0;) {
 			const size_t i0 = I * FL_TRSM_BLOCK, i1 = std::min(r, i0 + FL_TRSM_BLOCK);

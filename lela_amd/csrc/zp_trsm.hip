@@ -235,6 +235,296 @@ static int trsm_leaf(mlz_ctx *ctx, uint32_t p, size_t nb, size_t nrhs, uint32_t 
 	return check_launch(ctx, "trsm_leaf_apply_kernel");
 }
 
+// ---------------------------------------------------------------------------
+// Blocked solve on INT8 MFMA (p <= 2^16) with one launch per 64-row block:
+// block I's rows of X = a * A^-1 * B are W_I (a B_I - A[I, band] X[band]),
+// with W_I the inverse of its diagonal block and the band the solved rows it
+// depends on (upper: the rows below it up to the block's last nonzero
+// column; lower: all rows above it).  Packed operands ([32-index block]
+// [k block][limb][64 lanes][16 B], as zp_gemm.hip's INT8 kernels): every
+// block's band, the diagonal inverses, and X as it is solved (B operand:
+// column blocks, then row blocks).
+
+// 16-byte piece (hi and lo limb vectors) of the 16 residues v[0..15].
+__device__ inline void i8_piece(const uint32_t (&v)[16], uint32_t p, int8_t *dst)
+{
+	uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
+#pragma unroll
+	for (int b = 0; b < 16; ++b) {
+		const uint32_t l = i8_limbs(v[b], p);
+		hw[b >> 2] |= ((l >> 8) & 0xffu) << (8 * (b & 3));
+		lw[b >> 2] |= (l & 0xffu) << (8 * (b & 3));
+	}
+	*(uint4 *) dst = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+	*(uint4 *) (dst + 1024) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+}
+
+// Block I's band of a dense triangular A: k blocks, first column, end
+// column, and the byte offset of its fragments (blocks packed in order).
+__host__ __device__ inline void dense_band(int upper, int n, int nb64, int I, int &kbn, int &kc0, int &kend,
+					   size_t &off)
+{
+	const int i0 = 64 * I, i1 = i0 + 64 < n ? i0 + 64 : n;
+	if (upper) {
+		kc0 = i1;
+		kend = n;
+		kbn = 2 * (nb64 - 1 - I);
+		off = (size_t) 8192 * ((size_t) I * (nb64 - 1) - (size_t) I * (I - 1) / 2);
+	} else {
+		kc0 = 0;
+		kend = i0;
+		kbn = 2 * I;
+		off = (size_t) 4096 * I * (I - 1);
+	}
+}
+
+// Band fragments: workgroup (q, I) packs k blocks 2q, 2q+1 of block I's band
+// (columns kc0 .., zero from kend on; from meta/off, or dense_band when meta
+// is null); thread = piece (row block, k block, lane).
+template <int E>
+__global__ __launch_bounds__(256) void trsm_band_pack_kernel(const typename Elem<E>::T *__restrict__ A, size_t lda,
+							     int n, const int *__restrict__ meta, const size_t *__restrict__ moff,
+							     int upper, int8_t *__restrict__ band, uint32_t p)
+{
+	const int I = blockIdx.y, kb = 2 * blockIdx.x + ((threadIdx.x >> 6) & 1);
+	int n_kb, kc0, kend;
+	size_t off;
+	if (meta) {
+		n_kb = meta[3 * I];
+		kc0 = meta[3 * I + 1];
+		kend = meta[3 * I + 2];
+		off = moff[I];
+	} else {
+		dense_band(upper, n, (int) gridDim.y, I, n_kb, kc0, kend, off);
+	}
+	if (kb >= n_kb)
+		return;
+	const int lane = threadIdx.x & 63, rb = threadIdx.x >> 7;
+	const int i = 64 * I + 32 * rb + (lane & 31);
+	const int k0 = kc0 + 32 * kb + 16 * (lane >> 5);
+	uint32_t v[16];
+#pragma unroll
+	for (int b = 0; b < 16; ++b)
+		v[b] = (i < n && k0 + b < kend) ? Elem<E>::load(A + (size_t) i * lda + k0 + b, p) : 0u;
+	i8_piece(v, p, band + off + ((size_t) rb * n_kb + kb) * 2048 + lane * 16);
+}
+
+// Diagonal-block inverse fragments (W from zp_trsm_prepare: block I's inverse
+// in rows 64I .. of a 64-column array, valid for its first nb rows/columns).
+__global__ __launch_bounds__(256) void trsm_winv_pack_kernel(const uint32_t *__restrict__ W, int n,
+							     int8_t *__restrict__ wp, uint32_t p)
+{
+	const int I = blockIdx.x, nb = min(64, n - 64 * I);
+	const int lane = threadIdx.x & 63, kb = (threadIdx.x >> 6) & 1, rb = threadIdx.x >> 7;
+	const int t = 32 * rb + (lane & 31), s0 = 32 * kb + 16 * (lane >> 5);
+	uint32_t v[16];
+#pragma unroll
+	for (int b = 0; b < 16; ++b)
+		v[b] = (t < nb && s0 + b < nb) ? W[(size_t) (64 * I + t) * 64 + s0 + b] : 0u;
+	i8_piece(v, p, wp + (size_t) I * 8192 + (rb * 2 + kb) * 2048 + lane * 16);
+}
+
+// Block rows [i0, i0 + nbr) of X; workgroup j owns columns [64j, 64j + 64),
+// wave (rb, cb) one 32 x 32 block.  T = a B_I - band * X[band rows] (kbn k
+// blocks of packed X from row block kb0), then X_I = W_I T, written to B in
+// place and, packed, to Xp (rkb row blocks per column block).
+template <int E>
+__global__ __launch_bounds__(256) void trsm_block_solve_kernel(typename Elem<E>::T *__restrict__ B, size_t ldb,
+							       int nrhs, int i0, int nbr, int kbn, int kb0,
+							       const int8_t *__restrict__ band,
+							       const int8_t *__restrict__ winv, int8_t *__restrict__ Xp,
+							       int rkb, uint32_t a, ModP mp)
+{
+	__shared__ uint32_t Ts[64][65];
+	const uint32_t p = (uint32_t) mp.p;
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, rb = wave >> 1, cb = wave & 1;
+	const int cbg = 2 * blockIdx.x + cb, col = 32 * cbg + (lane & 31);
+	// B_I's entries are loaded first: their latency overlaps the band product
+	uint32_t bv[16];
+	if (nbr == 64 && 64 * (int) blockIdx.x + 64 <= nrhs) {
+#pragma unroll
+		for (int r = 0; r < 16; ++r)
+			bv[r] = Elem<E>::load(B + (size_t) (i0 + 32 * rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * ldb + col, p);
+	} else {
+#pragma unroll
+		for (int r = 0; r < 16; ++r) {
+			const int row = 32 * rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+			bv[r] = (row < nbr && col < nrhs) ? Elem<E>::load(B + (size_t) (i0 + row) * ldb + col, p) : 0u;
+		}
+	}
+	v16i a0 = v16i{0}, a1 = v16i{0}, a2 = v16i{0};
+	{
+		const int8_t *ga = band + (size_t) rb * kbn * 2048 + lane * 16;
+		const int8_t *gx = Xp + ((size_t) cbg * rkb + kb0) * 2048 + lane * 16;
+		// kbn is even; four k blocks' fragments are loaded before their
+		// MFMAs so that the loads' latency is paid once per group
+		int kb = 0;
+		for (; kb + 4 <= kbn; kb += 4) {
+			v4i ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+			for (int q = 0; q < 4; ++q) {
+				const size_t o = (size_t) (kb + q) * 2048;
+				ah[q] = *(const v4i *) (ga + o);
+				al[q] = *(const v4i *) (ga + o + 1024);
+				bh[q] = *(const v4i *) (gx + o);
+				bl[q] = *(const v4i *) (gx + o + 1024);
+			}
+#pragma unroll
+			for (int q = 0; q < 4; ++q) {
+				a0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah[q], bh[q], a0, 0, 0, 0);
+				a1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah[q], bl[q], a1, 0, 0, 0);
+				a2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al[q], bl[q], a2, 0, 0, 0);
+				a1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al[q], bh[q], a1, 0, 0, 0);
+			}
+		}
+		for (; kb < kbn; ++kb) {
+			const size_t o = (size_t) kb * 2048;
+			const v4i ah = *(const v4i *) (ga + o), al = *(const v4i *) (ga + o + 1024);
+			const v4i bh = *(const v4i *) (gx + o), bl = *(const v4i *) (gx + o + 1024);
+			a0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bh, a0, 0, 0, 0);
+			a1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bl, a1, 0, 0, 0);
+			a2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl, a2, 0, 0, 0);
+			a1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, a1, 0, 0, 0);
+		}
+	}
+	// T = a B_I - band product; rows >= nbr and columns >= nrhs are zero
+#pragma unroll
+	for (int r = 0; r < 16; ++r) {
+		const int row = 32 * rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+		uint32_t tv = 0;
+		if (row < nbr && col < nrhs) {
+			const uint32_t d = mp.mod_f64(fma((double) a0[r], 65536.0, fma((double) a1[r], 256.0, (double) a2[r])));
+			const uint32_t ab = a == 1 ? bv[r] : mp.mod_f64((double) a * (double) bv[r]);
+			tv = ab >= d ? ab - d : ab + p - d;
+		}
+		Ts[row][32 * cb + (lane & 31)] = tv;
+	}
+	__syncthreads();
+	// X_I = W_I T: B fragments of T built from LDS, W_I's from winv
+	a0 = v16i{0};
+	a1 = v16i{0};
+	a2 = v16i{0};
+#pragma unroll
+	for (int kb = 0; kb < 2; ++kb) {
+		uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
+		const int s0 = 32 * kb + 16 * (lane >> 5), cc = 32 * cb + (lane & 31);
+#pragma unroll
+		for (int b = 0; b < 16; ++b) {
+			const uint32_t l = i8_limbs(Ts[s0 + b][cc], p);
+			hw[b >> 2] |= ((l >> 8) & 0xffu) << (8 * (b & 3));
+			lw[b >> 2] |= (l & 0xffu) << (8 * (b & 3));
+		}
+		const v4i bh = v4i{(int) hw[0], (int) hw[1], (int) hw[2], (int) hw[3]};
+		const v4i bl = v4i{(int) lw[0], (int) lw[1], (int) lw[2], (int) lw[3]};
+		const int8_t *gw = winv + (rb * 2 + kb) * 2048 + lane * 16;
+		const v4i ah = *(const v4i *) gw, al = *(const v4i *) (gw + 1024);
+		a0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bh, a0, 0, 0, 0);
+		a1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bl, a1, 0, 0, 0);
+		a2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl, a2, 0, 0, 0);
+		a1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, a1, 0, 0, 0);
+	}
+	__syncthreads();
+#pragma unroll
+	for (int r = 0; r < 16; ++r) {
+		const int row = 32 * rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+		const uint32_t x = mp.mod_f64(fma((double) a0[r], 65536.0, fma((double) a1[r], 256.0, (double) a2[r])));
+		if (row < nbr && col < nrhs)
+			Elem<E>::store(B + (size_t) (i0 + row) * ldb + col, x, p);
+		Ts[row][32 * cb + (lane & 31)] = x;
+	}
+	__syncthreads();
+	// X_I packed: piece (column block, row block, lane) of this workgroup
+	{
+		const int cb2 = tid >> 7, kb2 = (tid >> 6) & 1;
+		const int cc = 32 * cb2 + (lane & 31), s0 = 32 * kb2 + 16 * (lane >> 5);
+		uint32_t v[16];
+#pragma unroll
+		for (int b = 0; b < 16; ++b)
+			v[b] = Ts[s0 + b][cc];
+		i8_piece(v, p, Xp + ((size_t) (2 * blockIdx.x + cb2) * rkb + i0 / 32 + kb2) * 2048 + lane * 16);
+	}
+}
+
+// X = a A^-1 B by 64-row blocks on INT8 MFMA (p <= 2^16, n <= 65504), W from
+// zp_trsm_prepare for this A.  upper: blocks from the bottom, block I's band
+// from its diagonal block's end to bend[I] (default n); lower: from the top,
+// band = all rows above the block.
+template <int E>
+static int trsm_blocked_i8_t(mlz_ctx *ctx, uint32_t p, int upper, size_t n, size_t nrhs, uint32_t a,
+			     const typename Elem<E>::T *A, size_t lda, typename Elem<E>::T *B, size_t ldb,
+			     const uint32_t *W, const int *bend)
+{
+	if (n == 0 || nrhs == 0)
+		return MLZ_OK;
+	const size_t nb64 = (n + 63) / 64, rpad = nb64 * 64, npad = round_up(nrhs, 64);
+	std::vector<int> meta(3 * nb64);
+	std::vector<size_t> off(nb64 + 1, 0);
+	int kbmax = 0;
+	for (size_t I = 0; I < nb64; ++I) {
+		int kbn, kc0, kend;
+		size_t o;
+		dense_band(upper, (int) n, (int) nb64, (int) I, kbn, kc0, kend, o);
+		if (bend) {   // banded: the band ends at bend[I]
+			kend = std::max(bend[I], kc0);
+			kbn = 2 * ((kend - kc0 + 63) / 64);
+			o = off[I];
+		}
+		meta[3 * I] = kbn;
+		meta[3 * I + 1] = kc0;
+		meta[3 * I + 2] = kend;
+		off[I] = o;
+		off[I + 1] = o + (size_t) 2 * kbn * 2048;
+		kbmax = std::max(kbmax, kbn);
+	}
+	void *wx, *wbp;
+	MLZ_TRY(ws_get(ctx, WS_TRSM_1, rpad * npad * 2, &wx));
+	const size_t mbytes = round_up(3 * nb64 * sizeof(int), 256) + round_up((nb64 + 1) * sizeof(size_t), 256);
+	MLZ_TRY(ws_get(ctx, WS_TRSM_2, mbytes + nb64 * 8192 + off[nb64] + 256, &wbp));
+	int *d_meta = (int *) wbp;
+	size_t *d_off = (size_t *) ((char *) wbp + round_up(3 * nb64 * sizeof(int), 256));
+	int8_t *winvp = (int8_t *) wbp + mbytes, *band = winvp + nb64 * 8192;
+	if (bend) {
+		// host tables (one call per FL): uploaded, and waited for below
+		MLZ_HIP(ctx, hipMemcpyAsync(d_meta, meta.data(), 3 * nb64 * sizeof(int), hipMemcpyHostToDevice,
+					    ctx->stream));
+		MLZ_HIP(ctx, hipMemcpyAsync(d_off, off.data(), (nb64 + 1) * sizeof(size_t), hipMemcpyHostToDevice,
+					    ctx->stream));
+	}
+	const ModP mp = make_modp(p);
+	if (kbmax)
+		trsm_band_pack_kernel<E><<<dim3(kbmax / 2, nb64), 256, 0, ctx->stream>>>(
+			A, lda, (int) n, bend ? d_meta : nullptr, bend ? d_off : nullptr, upper, band, p);
+	trsm_winv_pack_kernel<<<nb64, 256, 0, ctx->stream>>>(W, (int) n, winvp, p);
+	MLZ_TRY(check_launch(ctx, "trsm_winv_pack_kernel"));
+	for (size_t s = 0; s < nb64; ++s) {
+		const size_t I = upper ? nb64 - 1 - s : s;
+		const size_t i0 = 64 * I;
+		trsm_block_solve_kernel<E><<<npad / 64, 256, 0, ctx->stream>>>(
+			B, ldb, (int) nrhs, (int) i0, (int) std::min((size_t) 64, n - i0), meta[3 * I], meta[3 * I + 1] / 32,
+			band + off[I], winvp + I * 8192, (int8_t *) wx, (int) (rpad / 32), a, mp);
+	}
+	MLZ_TRY(check_launch(ctx, "trsm_block_solve_kernel"));
+	if (bend)
+		MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));   // meta and off leave scope
+	return MLZ_OK;
+}
+
+int zp_trsm_blocked_i8(mlz_ctx *ctx, int elem, uint64_t p, int upper, size_t n, size_t nrhs, uint64_t a,
+		       const void *A, size_t lda, void *B, size_t ldb, const uint32_t *W, const int *bend)
+{
+	const uint32_t pp = (uint32_t) p, aa = (uint32_t) (a % p);
+	if (elem == MLZ_U16)
+		return trsm_blocked_i8_t<MLZ_U16>(ctx, pp, upper, n, nrhs, aa, (const uint16_t *) A, lda, (uint16_t *) B, ldb,
+						  W, bend);
+	if (elem == MLZ_U32)
+		return trsm_blocked_i8_t<MLZ_U32>(ctx, pp, upper, n, nrhs, aa, (const uint32_t *) A, lda, (uint32_t *) B, ldb,
+						  W, bend);
+	return trsm_blocked_i8_t<MLZ_F64_SYM>(ctx, pp, upper, n, nrhs, aa, (const double *) A, lda, (double *) B, ldb, W,
+					      bend);
+}
+
+constexpr size_t TRSM_BLOCK_I8 = 1024;
+
 template <int E>
 static int trsm_rec(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, size_t nrhs, uint32_t a,
 		    const typename Elem<E>::T *A, size_t lda, typename Elem<E>::T *B, size_t ldb,
@@ -244,6 +534,10 @@ static int trsm_rec(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, siz
 		return MLZ_OK;
 	if (n <= (size_t) TRSM_LEAF)
 		return trsm_leaf<E>(ctx, p, n, nrhs, a, W, B, ldb, algo);
+	// diagonal blocks up to TRSM_BLOCK_I8 rows: one launch per 64-row block
+	// instead of the recursion's small gemms and leaves
+	if (n <= TRSM_BLOCK_I8 && p <= 65536 && (algo == MLZ_ALGO_AUTO || algo == MLZ_ALGO_INT8))
+		return trsm_blocked_i8_t<E>(ctx, p, upper, n, nrhs, a, A, lda, B, ldb, W, nullptr);
 	// Split at a multiple of the leaf size near n/2, so that every leaf is
 	// one of the precomputed diagonal blocks (any split gives the same unique
 	// solution; LELA splits at n/2).
