@@ -523,7 +523,7 @@ int zp_trsm_blocked_i8(mlz_ctx *ctx, int elem, uint64_t p, int upper, size_t n, 
 					      bend);
 }
 
-constexpr size_t TRSM_BLOCK_I8 = 1024;
+constexpr size_t TRSM_BLOCK_I8 = 1024;   // 512: same speed at 8192 x 1024; 2048: 20% slower
 
 template <int E>
 static int trsm_rec(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, size_t nrhs, uint32_t a,
