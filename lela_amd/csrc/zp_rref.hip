@@ -1404,8 +1404,16 @@ __device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int
 	}
 }
 
+// Waves per SIMD the batched echelon kernel is compiled for: the INT8 variant
+// with 32-column panels fits two 512-thread workgroups per CU (2 x 44 KB of
+// LDS, 128 VGPRs), so one instance's pivot search overlaps the other's update.
+constexpr int br_waves_per_simd(int W, bool I8, int NT)
+{
+	return I8 && W == 32 ? 4 : NT / 256;
+}
+
 template <int E, int W, bool SMALL, bool FMA, bool I8, int NT>
-__global__ __launch_bounds__(NT) void batch_rref_kernel(const BatchRref *__restrict__ tasks, BatchLaunch L,
+__global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_kernel(const BatchRref *__restrict__ tasks, BatchLaunch L,
 								ModP mp, uint64_t two_64, int *__restrict__ ranks,
 								int *__restrict__ bad)
 {
@@ -1722,9 +1730,10 @@ int batch_rref_launch(hipStream_t stream, int elem, int variant, uint32_t p, con
 int batch_rref_wgs_per_cu(int elem, int variant, uint32_t p)
 {
 	(void) elem;
-	(void) variant;
 	(void) p;
-	return 1;   // 1024-thread workgroups: one per CU
+	// INT8 with 32-column panels: two 512-thread workgroups per CU; the others
+	// take a CU each (1024 threads, or 90 KB of LDS for INT8 with 64 columns)
+	return variant == BR_I8_W32 ? 2 : 1;
 }
 
 }  // namespace mlz
