@@ -444,7 +444,7 @@ static int plan_build(mlz_ctx *ctx, int elem, uint64_t p, mlz_task *tasks, size_
 			r.m = (int) t.m;
 			r.n = (int) t.n;
 			wm_off[i] = lay.take((size_t) t.m * t.n * (br_mfma(v) ? 2 : 4));
-			wb_off[i] = lay.take((size_t) br_width(v) * t.n * 4);
+			wb_off[i] = br_mfma(v) ? 0 : lay.take((size_t) br_width(v) * t.n * 4);   // VALU variants only
 			wbp_off[i] = br_mfma(v) ? lay.take(br_wbp_bytes(v, t.n)) : 0;
 			pr_off[i] = lay.take(2 * (size_t) std::min(t.m, t.n) * 4);
 			items[RR0 + v].push_back({(int) rref_of[i], 0, 0, 0});
@@ -471,7 +471,7 @@ static int plan_build(mlz_ctx *ctx, int elem, uint64_t p, mlz_task *tasks, size_
 		} else {
 			BatchRref &r = rd[rref_of[i]];
 			r.Wm = A0 + wm_off[i];
-			r.Wb = (uint32_t *) (A0 + wb_off[i]);
+			r.Wb = br_mfma(tasks[i].variant) ? nullptr : (uint32_t *) (A0 + wb_off[i]);
 			r.Wbp = br_mfma(tasks[i].variant) ? (int8_t *) (A0 + wbp_off[i]) : nullptr;
 			r.piv_rows = (int *) (A0 + pr_off[i]);
 			r.piv_cols = r.piv_rows + std::min(tasks[i].m, tasks[i].n);

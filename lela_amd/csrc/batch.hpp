@@ -45,7 +45,7 @@ struct BatchRref {
 	uint64_t lda;
 	uint32_t *pivots;      // caller's pivot-column array (min(m, n)) or null
 	void *Wm;              // m x n canonical working copy (uint16 for the MFMA variants, else uint32)
-	uint32_t *Wb;          // panel-width x n rows of P_J^-1 * P
+	uint32_t *Wb;          // panel-width x n rows of P_J^-1 * P (VALU variants), else null
 	int8_t *Wbp;           // Wb as INT8 limb fragments (MFMA variants), else null
 	int *piv_rows, *piv_cols;   // min(m, n) each
 	int m, n;

@@ -1232,7 +1232,7 @@ struct BrLds {
 	uint8_t is_piv[BR_MAXM];     // 0 live, 1 pivot of an earlier panel, 2 of this panel
 	uint8_t rownz[I8 ? BR_RB8 : BR_RB];
 	uint32_t Pinv[W][W];
-	GT G[BR_RB][W];
+	GT G[I8 ? 1 : BR_RB][W];   // multipliers of the VALU update
 };
 
 // x (an exact integer below 2^53) mod p for p < 2^26.
@@ -1327,79 +1327,164 @@ __device__ __forceinline__ void br_apply(BrLds<W, FMA, I8> &S, int k, int nrows,
 	}
 }
 
-// Wb (k x nc, rows k..W-1 taken as zero) -> INT8 B fragments
-// [32-column block][k block][limb][64 lanes][16 bytes]: lane l holds column
-// l%32 and k-values 16*(l/32) .. +15 of its k block.
-template <int NT, int W>
-__device__ __forceinline__ void br_pack_wb(const uint32_t *__restrict__ Wb, size_t n, int k, int nc, uint32_t p,
-					   int8_t *__restrict__ Wbp)
+// Wb = P_J^-1 * P on INT8 MFMA, P the k pivot rows of the working copy
+// (columns c .. c+nc-1), P_J^-1 as A fragments in S.Ga (entries outside k x k
+// zero).  A wave takes a 32-column block: it loads the block's P fragments
+// straight from the pivot rows, multiplies, writes the reduced pivot rows back
+// into the working copy and the block's Wb fragments (the update's B operand,
+// [32-column block][k block][limb][64 lanes][16 bytes], lane l holding column
+// l%32 and k-values 16*(l/32) .. +15) into Wbp.  Rows k .. W-1 of Wb are zero.
+template <int NT, int W, bool FMA>
+__device__ __forceinline__ void br_wb_i8(BrLds<W, FMA, true> &S, int k, int nc, uint16_t *__restrict__ Wm, size_t n,
+					 int c, const ModP &mp, int8_t *__restrict__ Wbp)
 {
 	constexpr int KB = W / 32;
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int cl = lane & 31, hi = lane >> 5;
 	const int ncb = (nc + 31) / 32;
-	for (int u = wave; u < ncb * KB; u += NT / 64) {
-		const int cb = u / KB, kb = u - cb * KB;
-		const int col = cb * 32 + (lane & 31), k0 = kb * 32 + 16 * (lane >> 5);
-		uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
+	const uint32_t p = (uint32_t) mp.p;
+	for (int cb = wave; cb < ncb; cb += NT / 64) {
+		const int col = cb * 32 + cl;
+		const bool colok = col < nc;
+		uint16_t *const base = Wm + c + col;
+		v4i bh[KB], bl[KB];
 #pragma unroll
-		for (int q = 0; q < 16; ++q) {
-			const int kk = k0 + q;
-			const uint32_t x = (kk < k && col < nc) ? Wb[(size_t) kk * n + col] : 0u;
-			const uint32_t l = i8_limbs(x, p);
-			hw[q >> 2] |= ((l >> 8) & 0xffu) << (8 * (q & 3));
-			lw[q >> 2] |= (l & 0xffu) << (8 * (q & 3));
+		for (int kb = 0; kb < KB; ++kb) {
+			uint32_t x[16];
+#pragma unroll
+			for (int q = 0; q < 16; ++q) {
+				const int s = kb * 32 + 16 * hi + q;
+				x[q] = (s < k && colok) ? base[(size_t) S.rowid[s] * n] : 0u;
+			}
+			uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
+#pragma unroll
+			for (int q = 0; q < 16; ++q) {
+				const uint32_t l = i8_limbs(x[q], p);
+				hw[q >> 2] |= ((l >> 8) & 0xffu) << (8 * (q & 3));
+				lw[q >> 2] |= (l & 0xffu) << (8 * (q & 3));
+			}
+			bh[kb] = v4i{(int) hw[0], (int) hw[1], (int) hw[2], (int) hw[3]};
+			bl[kb] = v4i{(int) lw[0], (int) lw[1], (int) lw[2], (int) lw[3]};
 		}
-		int8_t *fr = Wbp + (size_t) (cb * KB + kb) * 2 * 1024 + lane * 16;
-		*(v4i *) fr = v4i{(int) hw[0], (int) hw[1], (int) hw[2], (int) hw[3]};
-		*(v4i *) (fr + 1024) = v4i{(int) lw[0], (int) lw[1], (int) lw[2], (int) lw[3]};
+#pragma unroll
+		for (int rb = 0; rb < KB; ++rb) {
+			v16i acc0 = v16i{0}, acc1 = v16i{0}, acc2 = v16i{0};
+#pragma unroll
+			for (int kb = 0; kb < KB; ++kb) {
+				const v4i ah = *(const v4i *) &S.Ga[rb][kb][0][lane][0];
+				const v4i al = *(const v4i *) &S.Ga[rb][kb][1][lane][0];
+				acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bh[kb], acc0, 0, 0, 0);
+				acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bl[kb], acc1, 0, 0, 0);
+				acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl[kb], acc2, 0, 0, 0);
+				acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh[kb], acc1, 0, 0, 0);
+			}
+			// lane l holds column cl, rows 8(r/4) + 4 hi + r%4 of the block
+			uint32_t d[16];
+#pragma unroll
+			for (int r = 0; r < 16; ++r) {
+				const double v = fma((double) acc0[r], 65536.0, fma((double) acc1[r], 256.0, (double) acc2[r]));
+				d[r] = mp.mod_f64(v);
+				const int s = rb * 32 + 8 * (r >> 2) + 4 * hi + (r & 3);
+				if (s < k && colok)
+					base[(size_t) S.rowid[s] * n] = (uint16_t) d[r];
+			}
+			// fragment order wants rows 16 hi + q in lane l: the halves swap
+			// rows 4-7 / 12-15 (lower half) against 16-19 / 24-27 (upper)
+			uint32_t f[16];
+#pragma unroll
+			for (int i = 0; i < 8; ++i) {
+				const uint32_t y = (uint32_t) __shfl_xor((int) (hi ? d[i] : d[8 + i]), 32);
+				const int q = (i & 3) + 8 * (i >> 2);   // y's place: rows 4-7 / 12-15 or 16-19 / 24-27
+				if (hi) {
+					f[q] = y;
+					f[q + 4] = d[8 + i];
+				} else {
+					f[q] = d[(i & 3) + 4 * (i >> 2)];
+					f[q + 4] = y;
+				}
+			}
+			uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
+#pragma unroll
+			for (int q = 0; q < 16; ++q) {
+				const uint32_t l = i8_limbs(f[q], p);
+				hw[q >> 2] |= ((l >> 8) & 0xffu) << (8 * (q & 3));
+				lw[q >> 2] |= (l & 0xffu) << (8 * (q & 3));
+			}
+			int8_t *fr = Wbp + (size_t) (cb * KB + rb) * 2 * 1024 + lane * 16;
+			*(v4i *) fr = v4i{(int) hw[0], (int) hw[1], (int) hw[2], (int) hw[3]};
+			*(v4i *) (fr + 1024) = v4i{(int) lw[0], (int) lw[1], (int) lw[2], (int) lw[3]};
+		}
 	}
 }
 
 // C (nb <= BR_RB8 rows x nc columns, row stride n) -= G * Wb on INT8 MFMA: G
 // from S.Ga, Wb from Wbp.  Units of 32 x 32 outputs are spread over the
-// waves; rows whose multipliers are all zero are not written.
+// waves; rows whose multipliers are all zero are not written.  Software
+// pipelined: a unit's outputs and Wb fragments are loaded while the previous
+// unit's MFMAs and epilogue run (addresses clamped into the block, so the
+// loads need no branch; clamped values are never written back).
 template <int NT, int W, bool FMA, typename CT>
 __device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int nc, const int8_t *__restrict__ Wbp,
 					     CT *__restrict__ C, size_t n, const ModP &mp)
 {
 	constexpr int KB = W / 32;
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	const int ncb = (nc + 31) / 32, nrb = (nb + 31) / 32;
+	const int ncb = (nc + 31) / 32, nrb = (nb + 31) / 32, nu = nrb * ncb;
+	const int cl = lane & 31, r4 = 4 * (lane >> 5);
 	const uint32_t p = (uint32_t) mp.p;
-	for (int u = wave; u < nrb * ncb; u += NT / 64) {
+	uint32_t cv[16];
+	v4i bh[KB], bl[KB];
+	auto fetch = [&](int u, uint32_t(&cx)[16], v4i(&hx)[KB], v4i(&lx)[KB]) {
 		const int rb = u / ncb, cb = u - rb * ncb;
-		const int col = cb * 32 + (lane & 31);
-		// the unit's outputs are loaded first so that their latency overlaps
-		// the MFMAs
-		uint32_t cv[16];
+		const CT *cc = C + min(cb * 32 + cl, nc - 1);
 #pragma unroll
-		for (int r = 0; r < 16; ++r) {
-			const int row = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-			cv[r] = (row < nb && col < nc) ? C[(size_t) row * n + col] : 0u;
+		for (int r = 0; r < 16; ++r)
+			cx[r] = cc[(size_t) min(rb * 32 + 8 * (r >> 2) + r4 + (r & 3), nb - 1) * n];
+#pragma unroll
+		for (int kb = 0; kb < KB; ++kb) {
+			const int8_t *gb = Wbp + (size_t) (cb * KB + kb) * 2 * 1024 + lane * 16;
+			hx[kb] = *(const v4i *) gb;
+			lx[kb] = *(const v4i *) (gb + 1024);
 		}
+	};
+	if (wave < nu)
+		fetch(wave, cv, bh, bl);
+	for (int u = wave; u < nu; u += NT / 64) {
+		const int rb = u / ncb, cb = u - rb * ncb;
+		const int col = cb * 32 + cl;
+		uint32_t cn[16];
+		v4i nh[KB], nl[KB];
+		fetch(min(u + NT / 64, nu - 1), cn, nh, nl);
 		v16i acc0 = v16i{0}, acc1 = v16i{0}, acc2 = v16i{0};
 #pragma unroll
 		for (int kb = 0; kb < KB; ++kb) {
 			const v4i ah = *(const v4i *) &S.Ga[rb][kb][0][lane][0];
 			const v4i al = *(const v4i *) &S.Ga[rb][kb][1][lane][0];
-			const int8_t *gb = Wbp + (size_t) (cb * KB + kb) * 2 * 1024 + lane * 16;
-			const v4i bh = *(const v4i *) gb, bl = *(const v4i *) (gb + 1024);
-			acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bh, acc0, 0, 0, 0);
-			acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bl, acc1, 0, 0, 0);
-			acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl, acc2, 0, 0, 0);
-			acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, acc1, 0, 0, 0);
+			acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bh[kb], acc0, 0, 0, 0);
+			acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bl[kb], acc1, 0, 0, 0);
+			acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl[kb], acc2, 0, 0, 0);
+			acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh[kb], acc1, 0, 0, 0);
 		}
-		if (col >= nc)
-			continue;
+		if (col < nc) {
 #pragma unroll
-		for (int r = 0; r < 16; ++r) {
-			const int row = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-			if (row < nb && S.rownz[row]) {
-				const double v = fma((double) acc0[r], 65536.0, fma((double) acc1[r], 256.0, (double) acc2[r]));
-				const uint32_t d = mp.mod_f64(v);
-				const uint32_t x = cv[r];
-				C[(size_t) row * n + col] = (CT) (x >= d ? x - d : x + p - d);
+			for (int r = 0; r < 16; ++r) {
+				const int row = rb * 32 + 8 * (r >> 2) + r4 + (r & 3);
+				if (row < nb && S.rownz[row]) {
+					const double v =
+						fma((double) acc0[r], 65536.0, fma((double) acc1[r], 256.0, (double) acc2[r]));
+					const uint32_t d = mp.mod_f64(v);
+					const uint32_t x = cv[r];
+					C[(size_t) row * n + col] = (CT) (x >= d ? x - d : x + p - d);
+				}
 			}
+		}
+#pragma unroll
+		for (int r = 0; r < 16; ++r)
+			cv[r] = cn[r];
+#pragma unroll
+		for (int kb = 0; kb < KB; ++kb) {
+			bh[kb] = nh[kb];
+			bl[kb] = nl[kb];
 		}
 	}
 }
@@ -1593,15 +1678,26 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 			for (int r = tid; r < BR_RB; r += NT)
 				S.rownz[r] = 1;
 			__syncthreads();
-			for (int e = tid; e < k * W; e += NT) {
-				const int r = e / W, s = e % W;
-				S.G[r][s] = (GT) (s < k ? S.Pinv[r][s] : 0u);
-			}
-			__syncthreads();
-			br_apply<NT, W, FMA, I8, false>(S, k, k, nc, Wm, Wb, n, mp, two_64);
-			__syncthreads();
 			if constexpr (I8) {
-				br_pack_wb<NT, W>(Wb, n, k, nc, p, t.Wbp);
+				// P_J^-1 as A fragments, then Wb on MFMA straight into the pivot
+				// rows and the update's fragments
+				for (int e = tid; e < W * W; e += NT) {
+					const int r = e / W, s = e % W;
+					const uint32_t l = i8_limbs(r < k && s < k ? S.Pinv[r][s] : 0u, p);
+					const int ln = (r & 31) + 32 * ((s & 31) >> 4), by = s & 15;
+					S.Ga[r >> 5][s >> 5][0][ln][by] = (int8_t) (l >> 8);
+					S.Ga[r >> 5][s >> 5][1][ln][by] = (int8_t) l;
+				}
+				__syncthreads();
+				br_wb_i8<NT, W, FMA>(S, k, nc, Wm, n, c, mp, t.Wbp);
+				__syncthreads();
+			} else {
+				for (int e = tid; e < k * W; e += NT) {
+					const int r = e / W, s = e % W;
+					S.G[r][s] = (GT) (s < k ? S.Pinv[r][s] : 0u);
+				}
+				__syncthreads();
+				br_apply<NT, W, FMA, I8, false>(S, k, k, nc, Wm, Wb, n, mp, two_64);
 				__syncthreads();
 			}
 			// ---- 3. eliminate column set J from every other row
@@ -1632,32 +1728,33 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 					__syncthreads();
 					br_update_i8<NT, W, FMA, WT>(S, nb, nc, t.Wbp, Wm + (size_t) i0 * n + c, n, mp);
 					__syncthreads();
-					continue;
+				} else {
+					for (int e = tid; e < nb * W; e += NT) {
+						const int r = e / W, s = e % W;
+						const int i = i0 + r;
+						uint32_t g = 0;
+						if (s < k && S.is_piv[i] != 2)
+							g = Wm[(size_t) i * n + c + S.J[s]];
+						S.G[r][s] = (GT) g;
+					}
+					__syncthreads();
+					// rownz[r]: some multiplier of row i0 + r is nonzero (a wave per row)
+					for (int r = wave; r < nb; r += NT / 64) {
+						const bool nz = lane < k && S.G[r][lane] != (GT) 0;
+						const uint64_t bal = __ballot(nz);
+						if (lane == 0)
+							S.rownz[r] = bal != 0;
+					}
+					__syncthreads();
+					br_apply<NT, W, FMA, I8, true>(S, k, nb, nc, Wb, Wm + (size_t) i0 * n + c, n, mp, two_64);
+					__syncthreads();
 				}
-				for (int e = tid; e < nb * W; e += NT) {
-					const int r = e / W, s = e % W;
-					const int i = i0 + r;
-					uint32_t g = 0;
-					if (s < k && S.is_piv[i] != 2)
-						g = Wm[(size_t) i * n + c + S.J[s]];
-					S.G[r][s] = (GT) g;
-				}
-				__syncthreads();
-				// rownz[r]: some multiplier of row i0 + r is nonzero (a wave per row)
-				for (int r = wave; r < nb; r += NT / 64) {
-					const bool nz = lane < k && S.G[r][lane] != (GT) 0;
-					const uint64_t bal = __ballot(nz);
-					if (lane == 0)
-						S.rownz[r] = bal != 0;
-				}
-				__syncthreads();
-				br_apply<NT, W, FMA, I8, true>(S, k, nb, nc, Wb, Wm + (size_t) i0 * n + c, n, mp, two_64);
-				__syncthreads();
 			}
-			for (int e = tid; e < k * nc; e += NT) {
-				const int s = e / nc, j = e % nc;
-				Wm[(size_t) S.rowid[s] * n + c + j] = (WT) Wb[(size_t) s * n + j];
-			}
+			if constexpr (!I8)
+				for (int e = tid; e < k * nc; e += NT) {
+					const int s = e / nc, j = e % nc;
+					Wm[(size_t) S.rowid[s] * n + c + j] = (WT) Wb[(size_t) s * n + j];
+				}
 			for (int s = tid; s < k; s += NT)
 				S.is_piv[S.rowid[s]] = 1;
 			for (int r = tid; r < (I8 ? BR_RB8 : BR_RB); r += NT)
