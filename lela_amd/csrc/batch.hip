@@ -443,7 +443,7 @@ static int plan_build(mlz_ctx *ctx, int elem, uint64_t p, mlz_task *tasks, size_
 			r.pivots = t.pivots;
 			r.m = (int) t.m;
 			r.n = (int) t.n;
-			wm_off[i] = lay.take((size_t) t.m * t.n * (br_mfma(v) ? 2 : 4));
+			wm_off[i] = lay.take((size_t) t.m * br_ldw((int) t.n, br_mfma(v)) * (br_mfma(v) ? 2 : 4));
 			wb_off[i] = br_mfma(v) ? 0 : lay.take((size_t) br_width(v) * t.n * 4);   // VALU variants only
 			wbp_off[i] = br_mfma(v) ? lay.take(br_wbp_bytes(v, t.n)) : 0;
 			pr_off[i] = lay.take(2 * (size_t) std::min(t.m, t.n) * 4);

@@ -100,6 +100,9 @@ inline int bg_tile_m(int v) { return v == BG_SPARSE ? 16 : v == BG_I8L ? 128 : 6
 inline int bg_tile_n(int v) { return v == BG_SPARSE ? 256 : v == BG_I8L ? 128 : 64; }
 inline int br_width(int v) { return v == BR_W32 || v == BR_I8_W32 ? 32 : 64; }
 inline bool br_mfma(int v) { return v == BR_I8_W64 || v == BR_I8_W32; }
+// Row stride of an instance's working copy: padded to a multiple of 4 entries
+// for the MFMA variants.
+__host__ __device__ inline int br_ldw(int n, bool mfma) { return mfma ? (n + 3) & ~3 : n; }
 // Bytes of the packed Wb of an MFMA variant: [32-column block][k block][limb][64 lanes][16 bytes].
 inline size_t br_wbp_bytes(int v, size_t n) { return (size_t) br_width(v) * ((n + 31) / 32 * 32) * 2; }
 

@@ -1417,29 +1417,33 @@ __device__ __forceinline__ void br_wb_i8(BrLds<W, FMA, true> &S, int k, int nc, 
 	}
 }
 
-// C (nb <= BR_RB8 rows x nc columns, row stride n) -= G * Wb on INT8 MFMA: G
-// from S.Ga, Wb from Wbp.  Units of 32 x 32 outputs are spread over the
-// waves; rows whose multipliers are all zero are not written.  Software
-// pipelined: a unit's outputs and Wb fragments are loaded while the previous
-// unit's MFMAs and epilogue run (addresses clamped into the block, so the
-// loads need no branch; clamped values are never written back).
-template <int NT, int W, bool FMA, typename CT>
+// C (nb <= BR_RB8 rows x nc columns; row stride ldc a multiple of 4 and rows
+// 8-byte aligned) -= G * Wb on INT8 MFMA, G from S.Ga, Wb from Wbp.  It is
+// computed transposed, C^T -= Wb^T G^T: Wbp's fragments are the A operand and
+// S.Ga's the B operand as they stand, and lane l of a 32 x 32 unit then owns
+// row l%32 of C and four runs of four consecutive columns, 8 g + 4 (l/32) +
+// 0..3, loaded and stored as 8-byte vectors.  Rows whose multipliers are all
+// zero are not written; the row padding (columns nc .. ldc-1) may be.
+// Software pipelined: a unit's outputs and Wb fragments are loaded while the
+// previous unit's MFMAs and epilogue run (addresses clamped into the block,
+// so the loads need no branch; clamped values are never written back).
+template <int NT, int W, bool FMA>
 __device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int nc, const int8_t *__restrict__ Wbp,
-					     CT *__restrict__ C, size_t n, const ModP &mp)
+					     uint16_t *__restrict__ C, size_t ldc, const ModP &mp)
 {
 	constexpr int KB = W / 32;
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int ncb = (nc + 31) / 32, nrb = (nb + 31) / 32, nu = nrb * ncb;
-	const int cl = lane & 31, r4 = 4 * (lane >> 5);
+	const int rl = lane & 31, c4 = 4 * (lane >> 5), clast = (nc - 1) & ~3;
 	const uint32_t p = (uint32_t) mp.p;
-	uint32_t cv[16];
+	uint2 cv[4];
 	v4i bh[KB], bl[KB];
-	auto fetch = [&](int u, uint32_t(&cx)[16], v4i(&hx)[KB], v4i(&lx)[KB]) {
+	auto fetch = [&](int u, uint2(&cx)[4], v4i(&hx)[KB], v4i(&lx)[KB]) {
 		const int rb = u / ncb, cb = u - rb * ncb;
-		const CT *cc = C + min(cb * 32 + cl, nc - 1);
+		const uint16_t *cc = C + (size_t) min(rb * 32 + rl, nb - 1) * ldc;
 #pragma unroll
-		for (int r = 0; r < 16; ++r)
-			cx[r] = cc[(size_t) min(rb * 32 + 8 * (r >> 2) + r4 + (r & 3), nb - 1) * n];
+		for (int g = 0; g < 4; ++g)
+			cx[g] = *(const uint2 *) (cc + min(cb * 32 + 8 * g + c4, clast));
 #pragma unroll
 		for (int kb = 0; kb < KB; ++kb) {
 			const int8_t *gb = Wbp + (size_t) (cb * KB + kb) * 2 * 1024 + lane * 16;
@@ -1451,8 +1455,7 @@ __device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int
 		fetch(wave, cv, bh, bl);
 	for (int u = wave; u < nu; u += NT / 64) {
 		const int rb = u / ncb, cb = u - rb * ncb;
-		const int col = cb * 32 + cl;
-		uint32_t cn[16];
+		uint2 cn[4];
 		v4i nh[KB], nl[KB];
 		fetch(min(u + NT / 64, nu - 1), cn, nh, nl);
 		v16i acc0 = v16i{0}, acc1 = v16i{0}, acc2 = v16i{0};
@@ -1460,27 +1463,36 @@ __device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int
 		for (int kb = 0; kb < KB; ++kb) {
 			const v4i ah = *(const v4i *) &S.Ga[rb][kb][0][lane][0];
 			const v4i al = *(const v4i *) &S.Ga[rb][kb][1][lane][0];
-			acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bh[kb], acc0, 0, 0, 0);
-			acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bl[kb], acc1, 0, 0, 0);
-			acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl[kb], acc2, 0, 0, 0);
-			acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh[kb], acc1, 0, 0, 0);
+			acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(bh[kb], ah, acc0, 0, 0, 0);
+			acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(bl[kb], ah, acc1, 0, 0, 0);
+			acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(bl[kb], al, acc2, 0, 0, 0);
+			acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(bh[kb], al, acc1, 0, 0, 0);
 		}
-		if (col < nc) {
+		const int row = rb * 32 + rl;
+		if (row < nb && S.rownz[row]) {
+			uint16_t *const out = C + (size_t) row * ldc;
 #pragma unroll
-			for (int r = 0; r < 16; ++r) {
-				const int row = rb * 32 + 8 * (r >> 2) + r4 + (r & 3);
-				if (row < nb && S.rownz[row]) {
-					const double v =
-						fma((double) acc0[r], 65536.0, fma((double) acc1[r], 256.0, (double) acc2[r]));
-					const uint32_t d = mp.mod_f64(v);
-					const uint32_t x = cv[r];
-					C[(size_t) row * n + col] = (CT) (x >= d ? x - d : x + p - d);
+			for (int g = 0; g < 4; ++g) {
+				const int col0 = cb * 32 + 8 * g + c4;
+				if (col0 < nc) {
+					const uint32_t xw[2] = {cv[g].x, cv[g].y};
+					uint32_t yw[2] = {0u, 0u};
+#pragma unroll
+					for (int j = 0; j < 4; ++j) {
+						const int r = 4 * g + j;
+						const double v = fma((double) acc0[r], 65536.0,
+								     fma((double) acc1[r], 256.0, (double) acc2[r]));
+						const uint32_t d = mp.mod_f64(v);
+						const uint32_t x = (xw[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+						yw[j >> 1] |= (x >= d ? x - d : x + p - d) << (16 * (j & 1));
+					}
+					*(uint2 *) (out + col0) = make_uint2(yw[0], yw[1]);
 				}
 			}
 		}
 #pragma unroll
-		for (int r = 0; r < 16; ++r)
-			cv[r] = cn[r];
+		for (int g = 0; g < 4; ++g)
+			cv[g] = cn[g];
 #pragma unroll
 		for (int kb = 0; kb < KB; ++kb) {
 			bh[kb] = nh[kb];
@@ -1519,12 +1531,15 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 		const BatchRref &t = tasks[w.task];
 		const uint64_t t0 = wall_clock64();
 		const int m = t.m, n = t.n;
+		// working copy, rows padded to a multiple of 4 entries for the MFMA
+		// variants (8-byte aligned rows for the update's vector accesses)
+		const int ldw = br_ldw(n, I8);
 		WT *Wm = (WT *) t.Wm;
 		uint32_t *Wb = t.Wb;
 		int *badp = bad + w.task;
 		for (int r = wave; r < m; r += NT / 64)
 			for (int c = lane; c < n; c += 64)
-				Wm[(size_t) r * n + c] = (WT) Elem<E>::load((const T *) t.A + (size_t) r * t.lda + c, p);
+				Wm[(size_t) r * ldw + c] = (WT) Elem<E>::load((const T *) t.A + (size_t) r * t.lda + c, p);
 		for (int r = tid; r < m; r += NT)
 			S.is_piv[r] = 0;
 		__syncthreads();
@@ -1576,7 +1591,7 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 					for (int i = 0; i < RPGI; ++i) {
 						const int r = gi * RPGI + i;
 						if (ci < W)
-							v[i] = (r < nr && ci < wc) ? Wm[(size_t) S.rowid[r] * n + c + ci] : 0u;
+							v[i] = (r < nr && ci < wc) ? Wm[(size_t) S.rowid[r] * ldw + c + ci] : 0u;
 						else
 							v[i] = (ci - W == r) ? 1u : 0u;
 					}
@@ -1619,7 +1634,7 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 #pragma unroll
 					for (int i = 0; i < RPG; ++i) {
 						const int r = g * RPG + i;
-						v[i] = (r < nr && cc < wc) ? Wm[(size_t) S.rowid[r] * n + c + cc] : 0u;
+						v[i] = (r < nr && cc < wc) ? Wm[(size_t) S.rowid[r] * ldw + c + cc] : 0u;
 					}
 					cnt = reg_eliminate<W, RPG, SMALL>(v, nr, wc, false, mp, S.alive, S.f, S.pivrow, S.s_min,
 									   S.outr, S.outc, badp);
@@ -1647,7 +1662,7 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 					const int r = gi * RPGI + i;
 					uint32_t x;
 					if (ci < W)
-						x = (r < k && ci < k) ? Wm[(size_t) S.rowid[r] * n + c + S.J[ci]] : (r == ci ? 1u : 0u);
+						x = (r < k && ci < k) ? Wm[(size_t) S.rowid[r] * ldw + c + S.J[ci]] : (r == ci ? 1u : 0u);
 					else
 						x = (ci - W == r) ? 1u : 0u;
 					v[i] = x;
@@ -1670,7 +1685,7 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 				}
 			}
 			for (int s = tid; s < k; s += NT) {
-				S.srcoff[s] = (long long) S.rowid[s] * n + c;
+				S.srcoff[s] = (long long) S.rowid[s] * ldw + c;
 				S.is_piv[S.rowid[s]] = 2;
 				t.piv_rows[rank + s] = S.rowid[s];
 				t.piv_cols[rank + s] = c + S.J[s];
@@ -1689,7 +1704,7 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 					S.Ga[r >> 5][s >> 5][1][ln][by] = (int8_t) l;
 				}
 				__syncthreads();
-				br_wb_i8<NT, W, FMA>(S, k, nc, Wm, n, c, mp, t.Wbp);
+				br_wb_i8<NT, W, FMA>(S, k, nc, Wm, ldw, c, mp, t.Wbp);
 				__syncthreads();
 			} else {
 				for (int e = tid; e < k * W; e += NT) {
@@ -1717,7 +1732,7 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 						const int i = i0 + r;
 						uint32_t g = 0;
 						if (r < nb && s < k && S.is_piv[i] != 2)
-							g = Wm[(size_t) i * n + c + S.J[s]];
+							g = Wm[(size_t) i * ldw + c + S.J[s]];
 						if (g)
 							S.rownz[r] = 1;
 						const uint32_t l = i8_limbs(g, p);
@@ -1726,7 +1741,7 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 						S.Ga[r >> 5][s >> 5][1][ln][by] = (int8_t) l;
 					}
 					__syncthreads();
-					br_update_i8<NT, W, FMA, WT>(S, nb, nc, t.Wbp, Wm + (size_t) i0 * n + c, n, mp);
+					br_update_i8<NT, W, FMA>(S, nb, nc, t.Wbp, (uint16_t *) Wm + (size_t) i0 * ldw + c, ldw, mp);
 					__syncthreads();
 				} else {
 					for (int e = tid; e < nb * W; e += NT) {
@@ -1734,7 +1749,7 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 						const int i = i0 + r;
 						uint32_t g = 0;
 						if (s < k && S.is_piv[i] != 2)
-							g = Wm[(size_t) i * n + c + S.J[s]];
+							g = Wm[(size_t) i * ldw + c + S.J[s]];
 						S.G[r][s] = (GT) g;
 					}
 					__syncthreads();
@@ -1746,14 +1761,14 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 							S.rownz[r] = bal != 0;
 					}
 					__syncthreads();
-					br_apply<NT, W, FMA, I8, true>(S, k, nb, nc, Wb, Wm + (size_t) i0 * n + c, n, mp, two_64);
+					br_apply<NT, W, FMA, I8, true>(S, k, nb, nc, Wb, Wm + (size_t) i0 * ldw + c, ldw, mp, two_64);
 					__syncthreads();
 				}
 			}
 			if constexpr (!I8)
 				for (int e = tid; e < k * nc; e += NT) {
 					const int s = e / nc, j = e % nc;
-					Wm[(size_t) S.rowid[s] * n + c + j] = (WT) Wb[(size_t) s * n + j];
+					Wm[(size_t) S.rowid[s] * ldw + c + j] = (WT) Wb[(size_t) s * n + j];
 				}
 			for (int s = tid; s < k; s += NT)
 				S.is_piv[S.rowid[s]] = 1;
@@ -1766,7 +1781,7 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 		for (int r = wave; r < m; r += NT / 64) {
 			const int src = r < rank ? t.piv_rows[r] : -1;
 			for (int c = lane; c < n; c += 64) {
-				const uint32_t x = src >= 0 ? Wm[(size_t) src * n + c] : 0u;
+				const uint32_t x = src >= 0 ? Wm[(size_t) src * ldw + c] : 0u;
 				Elem<E>::store((T *) t.A + (size_t) r * t.lda + c, x, p);
 			}
 		}
