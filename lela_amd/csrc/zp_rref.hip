@@ -1246,6 +1246,22 @@ __device__ __forceinline__ uint32_t br_mod_f64(double x, const ModP &mp)
 	return (uint32_t) r;
 }
 
+// (x - (65536 hi + 256 mid + lo)) mod p for x in [0, p), p <= 2^16 and INT8
+// MFMA limb sums of up to 64-term dot products of signed bytes (|hi|, |lo| <=
+// 2^20, |mid| <= 2^21).  mid and lo fold into x in int32 (|t| < 2^30); the
+// rest is one exact double (|v| < 2^37) and a floor quotient that is off by at
+// most one, so r + p lies in [0, 3p) and two unsigned min steps finish it.
+__device__ __forceinline__ uint32_t br_sub_mod(uint32_t x, int hi, int mid, int lo, const ModP &mp)
+{
+	const int t = (int) x - (mid * 256 + lo);
+	const double v = fma((double) hi, -65536.0, (double) t);
+	const double q = floor(v * mp.pinv);
+	const uint32_t p = (uint32_t) mp.p;
+	uint32_t u = (uint32_t) ((int) fma(-q, (double) p, v) + (int) p);
+	u = min(u, u - p);
+	return min(u, u - p);
+}
+
 // out[r * ld + j] (SUB: -)= sum_{s < k} G[r][s] * src[srcoff[s] + j] mod p for
 // rows r < nrows with rownz[r] set and columns j < ncols.  Units of BR_RQ rows
 // x 64 columns are spread over the waves; a lane owns one column and keeps
@@ -1435,7 +1451,6 @@ __device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 	const int ncb = (nc + 31) / 32, nrb = (nb + 31) / 32, nu = nrb * ncb;
 	const int rl = lane & 31, c4 = 4 * (lane >> 5), clast = (nc - 1) & ~3;
-	const uint32_t p = (uint32_t) mp.p;
 	uint2 cv[4];
 	v4i bh[KB], bl[KB];
 	auto fetch = [&](int u, uint2(&cx)[4], v4i(&hx)[KB], v4i(&lx)[KB]) {
@@ -1480,11 +1495,8 @@ __device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int
 #pragma unroll
 					for (int j = 0; j < 4; ++j) {
 						const int r = 4 * g + j;
-						const double v = fma((double) acc0[r], 65536.0,
-								     fma((double) acc1[r], 256.0, (double) acc2[r]));
-						const uint32_t d = mp.mod_f64(v);
 						const uint32_t x = (xw[j >> 1] >> (16 * (j & 1))) & 0xffffu;
-						yw[j >> 1] |= (x >= d ? x - d : x + p - d) << (16 * (j & 1));
+						yw[j >> 1] |= br_sub_mod(x, acc0[r], acc1[r], acc2[r], mp) << (16 * (j & 1));
 					}
 					*(uint2 *) (out + col0) = make_uint2(yw[0], yw[1]);
 				}
