@@ -1739,18 +1739,29 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 					for (int r = tid; r < RB; r += NT)
 						S.rownz[r] = 0;
 					__syncthreads();
-					for (int e = tid; e < ((nb + 31) & ~31) * W; e += NT) {
-						const int r = e / W, s = e % W;
+					// four consecutive multipliers of a row per thread: four loads in
+					// flight, one 32-bit LDS word per limb
+					for (int e = tid; e < ((nb + 31) & ~31) * (W / 4); e += NT) {
+						const int r = e / (W / 4), s0 = 4 * (e % (W / 4));
 						const int i = i0 + r;
-						uint32_t g = 0;
-						if (r < nb && s < k && S.is_piv[i] != 2)
-							g = Wm[(size_t) i * ldw + c + S.J[s]];
-						if (g)
+						const bool live = r < nb && S.is_piv[i] != 2;
+						const WT *row = Wm + (size_t) i * ldw + c;
+						uint32_t g[4];
+#pragma unroll
+						for (int j = 0; j < 4; ++j)
+							g[j] = live && s0 + j < k ? row[S.J[s0 + j]] : 0u;
+						if (g[0] | g[1] | g[2] | g[3])
 							S.rownz[r] = 1;
-						const uint32_t l = i8_limbs(g, p);
-						const int ln = (r & 31) + 32 * ((s & 31) >> 4), by = s & 15;
-						S.Ga[r >> 5][s >> 5][0][ln][by] = (int8_t) (l >> 8);
-						S.Ga[r >> 5][s >> 5][1][ln][by] = (int8_t) l;
+						uint32_t hw = 0, lw = 0;
+#pragma unroll
+						for (int j = 0; j < 4; ++j) {
+							const uint32_t l = i8_limbs(g[j], p);
+							hw |= ((l >> 8) & 0xffu) << (8 * j);
+							lw |= (l & 0xffu) << (8 * j);
+						}
+						const int ln = (r & 31) + 32 * ((s0 & 31) >> 4), by = s0 & 15;
+						*(uint32_t *) &S.Ga[r >> 5][s0 >> 5][0][ln][by] = hw;
+						*(uint32_t *) &S.Ga[r >> 5][s0 >> 5][1][ln][by] = lw;
 					}
 					__syncthreads();
 					br_update_i8<NT, W, FMA>(S, nb, nc, t.Wbp, (uint16_t *) Wm + (size_t) i0 * ldw + c, ldw, mp);
