@@ -64,14 +64,17 @@ struct BatchLaunch {
 	unsigned long long *tcost;     // per-task sum of workgroup durations (10 ns ticks), or null
 };
 
-// Next work item for the whole workgroup (-1 when the list is exhausted).
-// Ends with a barrier, so slot may be rewritten on the next call only after
-// every thread has read it.
-__device__ __forceinline__ int batch_next(const BatchLaunch &L, int *slot)
+// Next work items for the whole workgroup: the first of n consecutive items
+// claimed with one atomic (-1 when the list is exhausted; the claim may run
+// past the end, the caller stops at L.nitems).  Claiming several small items
+// at once keeps the single counter from serialising the launch.  Ends with a
+// barrier, so slot may be rewritten on the next call only after every thread
+// has read it.
+__device__ __forceinline__ int batch_next(const BatchLaunch &L, int *slot, int n = 1)
 {
 	__syncthreads();
 	if (threadIdx.x == 0)
-		*slot = atomicAdd(L.queue, 1);
+		*slot = atomicAdd(L.queue, n);
 	__syncthreads();
 	// uniform across the workgroup: descriptor loads can go through the scalar unit
 	const int it = __builtin_amdgcn_readfirstlane(*slot);

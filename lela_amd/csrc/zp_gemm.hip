@@ -1096,6 +1096,11 @@ int zp_gemm_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, size_t k
 // its workgroup's duration (s_memrealtime, 100 MHz) to its instance's total.
 // ===========================================================================
 
+// Items claimed per atomic by the pack and INT8 tile kernels (their items
+// are small and numerous).
+constexpr int BATCH_PACK_CHUNK = 8;
+constexpr int BATCH_I8_CHUNK = 2;
+
 // Pack items: x = 0 (A) / 1 (B) | 2 (16-byte aligned rows), y = 64-index block,
 // z = 64-k block.
 template <int E>
@@ -1106,26 +1111,33 @@ __global__ __launch_bounds__(256) void batch_pack_i8_kernel(const BatchGemm *__r
 	__shared__ uint32_t tile[64][65];
 	__shared__ int slot;
 	for (;;) {
-		const int it = batch_next(L, &slot);
-		if (it < 0)
+		const int c0 = batch_next(L, &slot, BATCH_PACK_CHUNK);
+		if (c0 < 0)
 			break;
-		const BatchItem w = L.items[it];
-		const BatchGemm &t = tasks[w.task];
-		const uint64_t t0 = wall_clock64();
-		const bool isB = w.x & 1, vec = w.x & 2;
-		if (!isB) {
-			if (vec)
-				i8_pack_tile<E, false, true>((const T *) t.A, t.lda, t.m, t.k, p, t.Ap, t.KB, w.y * 64, w.z * 64, tile);
-			else
-				i8_pack_tile<E, false, false>((const T *) t.A, t.lda, t.m, t.k, p, t.Ap, t.KB, w.y * 64, w.z * 64, tile);
-		} else {
-			if (vec)
-				i8_pack_tile<E, true, true>((const T *) t.B, t.ldb, t.n, t.k, p, t.Bp, t.KB, w.y * 64, w.z * 64, tile);
-			else
-				i8_pack_tile<E, true, false>((const T *) t.B, t.ldb, t.n, t.k, p, t.Bp, t.KB, w.y * 64, w.z * 64, tile);
+		const int c1 = min(c0 + BATCH_PACK_CHUNK, L.nitems);
+		for (int it = c0; it < c1; ++it) {
+			const BatchItem w = L.items[it];
+			const BatchGemm &t = tasks[w.task];
+			const uint64_t t0 = wall_clock64();
+			const bool isB = w.x & 1, vec = w.x & 2;
+			if (!isB) {
+				if (vec)
+					i8_pack_tile<E, false, true>((const T *) t.A, t.lda, t.m, t.k, p, t.Ap, t.KB, w.y * 64,
+								     w.z * 64, tile);
+				else
+					i8_pack_tile<E, false, false>((const T *) t.A, t.lda, t.m, t.k, p, t.Ap, t.KB, w.y * 64,
+								      w.z * 64, tile);
+			} else {
+				if (vec)
+					i8_pack_tile<E, true, true>((const T *) t.B, t.ldb, t.n, t.k, p, t.Bp, t.KB, w.y * 64,
+								    w.z * 64, tile);
+				else
+					i8_pack_tile<E, true, false>((const T *) t.B, t.ldb, t.n, t.k, p, t.Bp, t.KB, w.y * 64,
+								     w.z * 64, tile);
+			}
+			__syncthreads();
+			batch_charge(L, w.task, t0);
 		}
-		__syncthreads();
-		batch_charge(L, w.task, t0);
 	}
 }
 
@@ -1138,15 +1150,19 @@ __global__ __launch_bounds__(256, OCC) void batch_i8_kernel(const BatchGemm *__r
 	__shared__ __attribute__((aligned(16))) int8_t lds[2 * I8Tile<WTM, WTN, KK>::STAGE];
 	__shared__ int slot;
 	for (;;) {
-		const int it = batch_next(L, &slot);
-		if (it < 0)
+		const int c0 = batch_next(L, &slot, BATCH_I8_CHUNK);
+		if (c0 < 0)
 			break;
-		const BatchItem w = L.items[it];
-		const BatchGemm &t = tasks[w.task];
-		const uint64_t t0 = wall_clock64();
-		i8_tile<E, WTM, WTN, KK, 2, 2>(t.Ap, t.Bp, t.KB, (T *) t.C, t.ldc, t.m, t.n, t.a, t.b, mp, w.x, w.y, lds);
-		__syncthreads();
-		batch_charge(L, w.task, t0);
+		const int c1 = min(c0 + BATCH_I8_CHUNK, L.nitems);
+		for (int it = c0; it < c1; ++it) {
+			const BatchItem w = L.items[it];
+			const BatchGemm &t = tasks[w.task];
+			const uint64_t t0 = wall_clock64();
+			i8_tile<E, WTM, WTN, KK, 2, 2>(t.Ap, t.Bp, t.KB, (T *) t.C, t.ldc, t.m, t.n, t.a, t.b, mp, w.x, w.y,
+						       lds);
+			__syncthreads();
+			batch_charge(L, w.task, t0);
+		}
 	}
 }
 
