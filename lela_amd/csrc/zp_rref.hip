@@ -1249,16 +1249,16 @@ __device__ __forceinline__ uint32_t br_mod_f64(double x, const ModP &mp)
 // (x - (65536 hi + 256 mid + lo)) mod p for x in [0, p), p <= 2^16 and INT8
 // MFMA limb sums of up to 64-term dot products of signed bytes (|hi|, |lo| <=
 // 2^20, |mid| <= 2^21).  mid and lo fold into x in int32 (|t| < 2^30); the
-// rest is one exact double (|v| < 2^37) and a floor quotient that is off by at
-// most one, so r + p lies in [0, 3p) and two unsigned min steps finish it.
+// rest is one exact double (|v| < 2^37) and a round-to-nearest quotient: the
+// product v * (1/p) is within 2^-17 of v / p, so |v - q p| <= p/2 + 1, r + p
+// lies in [0, 2p) and one unsigned min step finishes it.
 __device__ __forceinline__ uint32_t br_sub_mod(uint32_t x, int hi, int mid, int lo, const ModP &mp)
 {
 	const int t = (int) x - (mid * 256 + lo);
 	const double v = fma((double) hi, -65536.0, (double) t);
-	const double q = floor(v * mp.pinv);
+	const double q = rint(v * mp.pinv);
 	const uint32_t p = (uint32_t) mp.p;
-	uint32_t u = (uint32_t) ((int) fma(-q, (double) p, v) + (int) p);
-	u = min(u, u - p);
+	const uint32_t u = (uint32_t) ((int) fma(-q, (double) p, v) + (int) p);
 	return min(u, u - p);
 }
 
@@ -1739,17 +1739,27 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 					for (int r = tid; r < RB; r += NT)
 						S.rownz[r] = 0;
 					__syncthreads();
-					// four consecutive multipliers of a row per thread: four loads in
-					// flight, one 32-bit LDS word per limb
+					// four consecutive multipliers of a row per thread, one 32-bit LDS
+					// word per limb.  A full-rank panel has J[s] = s (the search takes
+					// the columns in order), and the four are one 8-byte load of the
+					// row (rows are 8-byte aligned, the padding keeps it in bounds).
+					const bool ident = k == wc;
 					for (int e = tid; e < ((nb + 31) & ~31) * (W / 4); e += NT) {
 						const int r = e / (W / 4), s0 = 4 * (e % (W / 4));
 						const int i = i0 + r;
 						const bool live = r < nb && S.is_piv[i] != 2;
 						const WT *row = Wm + (size_t) i * ldw + c;
 						uint32_t g[4];
+						if (ident) {
+							const uint2 v = live && s0 < k ? *(const uint2 *) (row + s0) : make_uint2(0u, 0u);
 #pragma unroll
-						for (int j = 0; j < 4; ++j)
-							g[j] = live && s0 + j < k ? row[S.J[s0 + j]] : 0u;
+							for (int j = 0; j < 4; ++j)
+								g[j] = s0 + j < k ? ((j < 2 ? v.x : v.y) >> (16 * (j & 1))) & 0xffffu : 0u;
+						} else {
+#pragma unroll
+							for (int j = 0; j < 4; ++j)
+								g[j] = live && s0 + j < k ? row[S.J[s0 + j]] : 0u;
+						}
 						if (g[0] | g[1] | g[2] | g[3])
 							S.rownz[r] = 1;
 						uint32_t hw = 0, lw = 0;
