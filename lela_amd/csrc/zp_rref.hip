@@ -1448,13 +1448,12 @@ __device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int
 					     uint16_t *__restrict__ C, size_t ldc, const ModP &mp)
 {
 	constexpr int KB = W / 32;
-	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const int ncb = (nc + 31) / 32, nrb = (nb + 31) / 32, nu = nrb * ncb;
 	const int rl = lane & 31, c4 = 4 * (lane >> 5), clast = (nc - 1) & ~3;
 	uint2 cv[4];
 	v4i bh[KB], bl[KB];
-	auto fetch = [&](int u, uint2(&cx)[4], v4i(&hx)[KB], v4i(&lx)[KB]) {
-		const int rb = u / ncb, cb = u - rb * ncb;
+	auto fetch = [&](int rb, int cb, uint2(&cx)[4], v4i(&hx)[KB], v4i(&lx)[KB]) {
 		const uint16_t *cc = C + (size_t) min(rb * 32 + rl, nb - 1) * ldc;
 #pragma unroll
 		for (int g = 0; g < 4; ++g)
@@ -1466,13 +1465,24 @@ __device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int
 			lx[kb] = *(const v4i *) (gb + 1024);
 		}
 	};
+	// unit u = wave + (NT/64) i as (row block, column block), stepped forward
+	// without a division (wave-uniform)
+	auto advance = [&](int &rb, int &cb) {
+		cb += NT / 64;
+		while (cb >= ncb) {
+			cb -= ncb;
+			++rb;
+		}
+	};
+	int rb = wave / ncb, cb = wave - rb * ncb, rbn = rb, cbn = cb;
+	advance(rbn, cbn);
 	if (wave < nu)
-		fetch(wave, cv, bh, bl);
+		fetch(rb, cb, cv, bh, bl);
 	for (int u = wave; u < nu; u += NT / 64) {
-		const int rb = u / ncb, cb = u - rb * ncb;
 		uint2 cn[4];
 		v4i nh[KB], nl[KB];
-		fetch(min(u + NT / 64, nu - 1), cn, nh, nl);
+		const bool more = u + NT / 64 < nu;   // else the current unit again (loads stay in bounds)
+		fetch(more ? rbn : rb, more ? cbn : cb, cn, nh, nl);
 		v16i acc0 = v16i{0}, acc1 = v16i{0}, acc2 = v16i{0};
 #pragma unroll
 		for (int kb = 0; kb < KB; ++kb) {
@@ -1510,6 +1520,9 @@ __device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int
 			bh[kb] = nh[kb];
 			bl[kb] = nl[kb];
 		}
+		rb = rbn;
+		cb = cbn;
+		advance(rbn, cbn);
 	}
 }
 
