@@ -324,24 +324,33 @@ __global__ __launch_bounds__(256) void trsm_winv_pack_kernel(const uint32_t *__r
 	i8_piece(v, p, wp + (size_t) I * 8192 + (rb * 2 + kb) * 2048 + lane * 16);
 }
 
-// Block rows [i0, i0 + nbr) of X; workgroup j owns columns [64j, 64j + 64),
-// wave (rb, cb) one 32 x 32 block.  T = a B_I - band * X[band rows] (kbn k
-// blocks of packed X from row block kb0), then X_I = W_I T, written to B in
-// place and, packed, to Xp (rkb row blocks per column block).
+// Block rows [i0, i0 + nbr) of X; workgroup j owns columns [64j, 64j + 64).
+// Waves 0-3 (rb, cb) own one 32 x 32 block each; the band product is split
+// over k, waves 4-7 taking the upper half of the k blocks and handing their
+// sums to waves 0-3 through LDS (the product is a chain of dependent loads and
+// MFMAs, and the launch has only nrhs/64 workgroups).  T = a B_I - band *
+// X[band rows] (kbn k blocks of packed X from row block kb0), then X_I = W_I T,
+// written to B in place and, packed, to Xp (rkb row blocks per column block).
 template <int E>
-__global__ __launch_bounds__(256) void trsm_block_solve_kernel(typename Elem<E>::T *__restrict__ B, size_t ldb,
+__global__ __launch_bounds__(512) void trsm_block_solve_kernel(typename Elem<E>::T *__restrict__ B, size_t ldb,
 							       int nrhs, int i0, int nbr, int kbn, int kb0,
 							       const int8_t *__restrict__ band,
 							       const int8_t *__restrict__ winv, int8_t *__restrict__ Xp,
 							       int rkb, uint32_t a, ModP mp)
 {
 	__shared__ uint32_t Ts[64][65];
+	__shared__ int Pk[4][3][16][64];   // upper k half's sums
 	const uint32_t p = (uint32_t) mp.p;
-	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, rb = wave >> 1, cb = wave & 1;
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, half = wave >> 2, rb = (wave & 3) >> 1,
+		  cb = wave & 1;
 	const int cbg = 2 * blockIdx.x + cb, col = 32 * cbg + (lane & 31);
 	// B_I's entries are loaded first: their latency overlaps the band product
 	uint32_t bv[16];
-	if (nbr == 64 && 64 * (int) blockIdx.x + 64 <= nrhs) {
+	if (half) {
+#pragma unroll
+		for (int r = 0; r < 16; ++r)
+			bv[r] = 0u;
+	} else if (nbr == 64 && 64 * (int) blockIdx.x + 64 <= nrhs) {
 #pragma unroll
 		for (int r = 0; r < 16; ++r)
 			bv[r] = Elem<E>::load(B + (size_t) (i0 + 32 * rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * ldb + col, p);
@@ -354,12 +363,13 @@ __global__ __launch_bounds__(256) void trsm_block_solve_kernel(typename Elem<E>:
 	}
 	v16i a0 = v16i{0}, a1 = v16i{0}, a2 = v16i{0};
 	{
+		const int kh = kbn / 2, kend = half ? kbn : kh;
 		const int8_t *ga = band + (size_t) rb * kbn * 2048 + lane * 16;
 		const int8_t *gx = Xp + ((size_t) cbg * rkb + kb0) * 2048 + lane * 16;
-		// kbn is even; four k blocks' fragments are loaded before their
-		// MFMAs so that the loads' latency is paid once per group
-		int kb = 0;
-		for (; kb + 4 <= kbn; kb += 4) {
+		// four k blocks' fragments are loaded before their MFMAs so that the
+		// loads' latency is paid once per group
+		int kb = half ? kh : 0;
+		for (; kb + 4 <= kend; kb += 4) {
 			v4i ah[4], al[4], bh[4], bl[4];
 #pragma unroll
 			for (int q = 0; q < 4; ++q) {
@@ -377,7 +387,7 @@ __global__ __launch_bounds__(256) void trsm_block_solve_kernel(typename Elem<E>:
 				a1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al[q], bh[q], a1, 0, 0, 0);
 			}
 		}
-		for (; kb < kbn; ++kb) {
+		for (; kb < kend; ++kb) {
 			const size_t o = (size_t) kb * 2048;
 			const v4i ah = *(const v4i *) (ga + o), al = *(const v4i *) (ga + o + 1024);
 			const v4i bh = *(const v4i *) (gx + o), bl = *(const v4i *) (gx + o + 1024);
@@ -387,25 +397,39 @@ __global__ __launch_bounds__(256) void trsm_block_solve_kernel(typename Elem<E>:
 			a1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, a1, 0, 0, 0);
 		}
 	}
-	// T = a B_I - band product; rows >= nbr and columns >= nrhs are zero
+	if (half) {
 #pragma unroll
-	for (int r = 0; r < 16; ++r) {
-		const int row = 32 * rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-		uint32_t tv = 0;
-		if (row < nbr && col < nrhs) {
-			const uint32_t d = mp.mod_f64(fma((double) a0[r], 65536.0, fma((double) a1[r], 256.0, (double) a2[r])));
-			const uint32_t ab = a == 1 ? bv[r] : mp.mod_f64((double) a * (double) bv[r]);
-			tv = ab >= d ? ab - d : ab + p - d;
+		for (int r = 0; r < 16; ++r) {
+			Pk[wave & 3][0][r][lane] = a0[r];
+			Pk[wave & 3][1][r][lane] = a1[r];
+			Pk[wave & 3][2][r][lane] = a2[r];
 		}
-		Ts[row][32 * cb + (lane & 31)] = tv;
 	}
 	__syncthreads();
-	// X_I = W_I T: B fragments of T built from LDS, W_I's from winv
+	// T = a B_I - band product; rows >= nbr and columns >= nrhs are zero
+	if (!half) {
+#pragma unroll
+		for (int r = 0; r < 16; ++r) {
+			const int row = 32 * rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+			uint32_t tv = 0;
+			if (row < nbr && col < nrhs) {
+				const double h = (double) (a0[r] + Pk[wave][0][r][lane]);
+				const double m = (double) (a1[r] + Pk[wave][1][r][lane]);
+				const double l = (double) (a2[r] + Pk[wave][2][r][lane]);
+				const uint32_t d = mp.mod_f64(fma(h, 65536.0, fma(m, 256.0, l)));
+				const uint32_t ab = a == 1 ? bv[r] : mp.mod_f64((double) a * (double) bv[r]);
+				tv = ab >= d ? ab - d : ab + p - d;
+			}
+			Ts[row][32 * cb + (lane & 31)] = tv;
+		}
+	}
+	__syncthreads();
+	// X_I = W_I T (waves 0-3): B fragments of T built from LDS, W_I's from winv
 	a0 = v16i{0};
 	a1 = v16i{0};
 	a2 = v16i{0};
 #pragma unroll
-	for (int kb = 0; kb < 2; ++kb) {
+	for (int kb = 0; kb < 2 && !half; ++kb) {
 		uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
 		const int s0 = 32 * kb + 16 * (lane >> 5), cc = 32 * cb + (lane & 31);
 #pragma unroll
@@ -424,17 +448,19 @@ __global__ __launch_bounds__(256) void trsm_block_solve_kernel(typename Elem<E>:
 		a1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, a1, 0, 0, 0);
 	}
 	__syncthreads();
+	if (!half) {
 #pragma unroll
-	for (int r = 0; r < 16; ++r) {
-		const int row = 32 * rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-		const uint32_t x = mp.mod_f64(fma((double) a0[r], 65536.0, fma((double) a1[r], 256.0, (double) a2[r])));
-		if (row < nbr && col < nrhs)
-			Elem<E>::store(B + (size_t) (i0 + row) * ldb + col, x, p);
-		Ts[row][32 * cb + (lane & 31)] = x;
+		for (int r = 0; r < 16; ++r) {
+			const int row = 32 * rb + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+			const uint32_t x = mp.mod_f64(fma((double) a0[r], 65536.0, fma((double) a1[r], 256.0, (double) a2[r])));
+			if (row < nbr && col < nrhs)
+				Elem<E>::store(B + (size_t) (i0 + row) * ldb + col, x, p);
+			Ts[row][32 * cb + (lane & 31)] = x;
+		}
 	}
 	__syncthreads();
 	// X_I packed: piece (column block, row block, lane) of this workgroup
-	{
+	if (tid < 256) {
 		const int cb2 = tid >> 7, kb2 = (tid >> 6) & 1;
 		const int cc = 32 * cb2 + (lane & 31), s0 = 32 * kb2 + 16 * (lane >> 5);
 		uint32_t v[16];
@@ -499,7 +525,7 @@ static int trsm_blocked_i8_t(mlz_ctx *ctx, uint32_t p, int upper, size_t n, size
 	for (size_t s = 0; s < nb64; ++s) {
 		const size_t I = upper ? nb64 - 1 - s : s;
 		const size_t i0 = 64 * I;
-		trsm_block_solve_kernel<E><<<npad / 64, 256, 0, ctx->stream>>>(
+		trsm_block_solve_kernel<E><<<npad / 64, 512, 0, ctx->stream>>>(
 			B, ldb, (int) nrhs, (int) i0, (int) std::min((size_t) 64, n - i0), meta[3 * I], meta[3 * I + 1] / 32,
 			band + off[I], winvp + I * 8192, (int8_t *) wx, (int) (rpad / 32), a, mp);
 	}
