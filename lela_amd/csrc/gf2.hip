@@ -538,6 +538,9 @@ __global__ __launch_bounds__(256) void gf2_inv_kernel(const uint64_t *__restrict
 }
 
 // G[i] = bits J of row i (compressed to k bits), zero on this panel's pivots.
+// J is strictly increasing, so J[k-1] = k-1 means J[s] = s (a full-rank
+// panel, the usual case): the compressed bits are then the row's first k
+// panel bits as they stand.
 __global__ void gf2_gatherG_kernel(const uint64_t *__restrict__ A, size_t lda, int m, int cw, int nw, int panel,
 				   Gf2State st, uint64_t *__restrict__ G)
 {
@@ -550,12 +553,20 @@ __global__ void gf2_gatherG_kernel(const uint64_t *__restrict__ A, size_t lda, i
 		row[w] = cw + w < nw ? A[(size_t) i * lda + cw + w] : 0ull;
 		g[w] = 0;
 	}
-	if (st.is_pivot[i] != panel + 1)
-		for (int s = 0; s < k; ++s) {
-			const int j = st.J[s];
-			if ((row[j >> 6] >> (j & 63)) & 1ull)
-				g[s >> 6] |= 1ull << (s & 63);
+	if (st.is_pivot[i] != panel + 1) {
+		if (k > 0 && st.J[k - 1] == k - 1) {
+			for (int w = 0; w < G2P; ++w) {
+				const int nb = min(64, max(0, k - 64 * w));
+				g[w] = nb == 64 ? row[w] : row[w] & ((1ull << nb) - 1ull);
+			}
+		} else {
+			for (int s = 0; s < k; ++s) {
+				const int j = st.J[s];
+				if ((row[j >> 6] >> (j & 63)) & 1ull)
+					g[s >> 6] |= 1ull << (s & 63);
+			}
 		}
+	}
 	for (int w = 0; w < G2P; ++w)
 		G[(size_t) i * G2P + w] = g[w];
 }
