@@ -236,7 +236,8 @@ __device__ __forceinline__ void i8_epilogue(const v16i (&acc)[3][WTM][WTN], type
 				for (int r = 0; r < 16; ++r) {
 					const double v = fma((double) acc[0][i][j][r], 65536.0,
 							     fma((double) acc[1][i][j][r], 256.0, (double) acc[2][i][j][r]));
-					double t = (double) a * (double) mp.mod_f64(v);
+					// a = 1: v + b c (|.| < 2^49) is exact, one reduction
+					double t = a == 1 ? v : (double) a * (double) mp.mod_f64(v);
 					if (b)
 						t = fma((double) b, (double) cv[r], t);
 					Elem<E>::store(cb + (size_t) (8 * (r >> 2) + (r & 3)) * ldc, mp.mod_f64(t), p);
@@ -247,13 +248,12 @@ __device__ __forceinline__ void i8_epilogue(const v16i (&acc)[3][WTM][WTN], type
 			for (int r = 0; r < 16; ++r) {
 				const int row = row0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
 				if (row < m && col < n) {
-					// 65536*HH + 256*(HL+LH) + LL has |.| < 2^48 and a*d + b*c < 2^33
-					// (p <= 2^16): both are exact doubles, reduced by mod_f64.
+					// 65536*HH + 256*(HL+LH) + LL has |.| < 2^48, a*d + b*c < 2^33 and
+					// (a = 1) v + b*c < 2^49 (p <= 2^16): exact doubles, reduced by mod_f64.
 					const double v = fma((double) acc[0][i][j][r], 65536.0,
 							     fma((double) acc[1][i][j][r], 256.0, (double) acc[2][i][j][r]));
-					const double d = (double) mp.mod_f64(v);
 					typename Elem<E>::T *cp = C + (size_t) row * ldc + col;
-					double t = (double) a * d;
+					double t = a == 1 ? v : (double) a * (double) mp.mod_f64(v);
 					if (b)
 						t = fma((double) b, (double) Elem<E>::load(cp, (uint32_t) mp.p), t);
 					Elem<E>::store(cp, mp.mod_f64(t), (uint32_t) mp.p);
