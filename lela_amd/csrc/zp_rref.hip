@@ -697,30 +697,33 @@ __global__ void rref_gather_kernel(const uint32_t *__restrict__ A, size_t lda, i
 }
 
 // The panel step after the pivot search, fused for p <= 2^16 (INT8 trailing
-// update).  Workgroup j owns P's columns [64j, 64j + 64): it gathers them from
+// update).  Workgroup j owns P's columns [TC j, TC j + TC) (TC = 32 or 64;
+// narrower column slices give the launch more workgroups): it gathers them from
 // the pivot rows (plus the tracker's 1, as rref_gather_kernel), forms
 // W = P_J^-1 * P for them (64 terms below 2^32 each: exact in a double),
 // writes W into the pivot rows (the update leaves those rows unchanged: their
 // multipliers are zero) and packs W as the update's INT8 B fragments
 // (KB = 2).  Workgroup 0 also marks the pivot rows and appends the pivot
 // lists.
+template <int TC>
 __global__ __launch_bounds__(256) void rref_wb_i8_kernel(uint32_t *__restrict__ A, size_t lda, int nc, int c,
 							 int panel, RrefState st, const uint32_t *__restrict__ Pinv,
 							 int8_t *__restrict__ Wp, int coff, int inj, ModP mp)
 {
 	constexpr int W = RREF_W;
-	__shared__ uint32_t Ps[W][W + 1], Pi[W][W + 1];
+	static_assert(TC == 32 || TC == 64, "column slice of 32 or 64");
+	__shared__ uint32_t Ps[W][TC + 1], Pi[W][W + 1];
 	__shared__ int prow[W];
 	const uint32_t p = (uint32_t) mp.p;
 	const int tid = threadIdx.x;
-	const int j0 = blockIdx.x * W;
+	const int j0 = blockIdx.x * TC;
 	const int k = *st.k;
 	const int kb = *st.rank - *st.rank0;
 	if (tid < W)
 		prow[tid] = tid < k ? st.prow[tid] : -1;
 	__syncthreads();
-	for (int e = tid; e < W * W; e += 256) {
-		const int t = e / W, col = e % W, gc = j0 + col;
+	for (int e = tid; e < W * TC; e += 256) {
+		const int t = e / TC, col = e % TC, gc = j0 + col;
 		uint32_t x = 0;
 		if (t < k && gc < nc) {
 			x = A[(size_t) prow[t] * lda + c + gc];
@@ -728,44 +731,46 @@ __global__ __launch_bounds__(256) void rref_wb_i8_kernel(uint32_t *__restrict__ 
 				x = x + 1 == p ? 0u : x + 1;
 		}
 		Ps[t][col] = x;
-		Pi[t][col] = Pinv[e];
 	}
+	for (int e = tid; e < W * W; e += 256)
+		Pi[e / W][e % W] = Pinv[e];
 	__syncthreads();
-	// W on v_mfma_f64_16x16x4f64: wave w computes rows 16w .. 16w+15 as four
-	// 16 x 16 blocks.  A fragment: lane l holds (row l%16, k l/16); B
+	// W on v_mfma_f64_16x16x4f64: wave w computes rows 16w .. 16w+15 as
+	// TC/16 16 x 16 blocks.  A fragment: lane l holds (row l%16, k l/16); B
 	// fragment: (k l/16, column l%16); result register r of lane l: (row
 	// l/16 + 4r, column l%16).
 	{
+		constexpr int NJ = TC / 16;
 		const int lane = tid & 63, w = tid >> 6;
 		typedef double v4d __attribute__((ext_vector_type(4)));
-		v4d acc[4];
+		v4d acc[NJ];
 #pragma unroll
-		for (int j = 0; j < 4; ++j)
+		for (int j = 0; j < NJ; ++j)
 			acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
 #pragma unroll 4
 		for (int s = 0; s < W; s += 4) {
 			const double a = (double) Pi[16 * w + (lane & 15)][s + (lane >> 4)];
 #pragma unroll
-			for (int j = 0; j < 4; ++j)
+			for (int j = 0; j < NJ; ++j)
 				acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, (double) Ps[s + (lane >> 4)][16 * j + (lane & 15)],
 									     acc[j], 0, 0, 0);
 		}
 		__syncthreads();
 #pragma unroll
-		for (int j = 0; j < 4; ++j)
+		for (int j = 0; j < NJ; ++j)
 #pragma unroll
 			for (int r = 0; r < 4; ++r)
 				Ps[16 * w + (lane >> 4) + 4 * r][16 * j + (lane & 15)] = mp.mod_f64(acc[j][r]);
 	}
 	__syncthreads();
-	for (int e = tid; e < W * W; e += 256) {
-		const int t = e / W, cc = e % W, gc = j0 + cc;
+	for (int e = tid; e < W * TC; e += 256) {
+		const int t = e / TC, cc = e % TC, gc = j0 + cc;
 		if (t < k && gc < nc)
 			A[(size_t) prow[t] * lda + c + gc] = Ps[t][cc];
 	}
 	// B fragments: piece (column block cb, k block q, lane) holds column
 	// 32cb + lane%32 and k = 32q + 16(lane/32) .. +15
-	{
+	if (tid < 128 * (TC / 32)) {
 		const int lane = tid & 63, q = (tid >> 6) & 1, cb = tid >> 7;
 		const int cc = 32 * cb + (lane & 31), t = 32 * q + 16 * (lane >> 5);
 		uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
@@ -775,7 +780,7 @@ __global__ __launch_bounds__(256) void rref_wb_i8_kernel(uint32_t *__restrict__ 
 			hw[b >> 2] |= ((l >> 8) & 0xffu) << (8 * (b & 3));
 			lw[b >> 2] |= (l & 0xffu) << (8 * (b & 3));
 		}
-		int8_t *fr = Wp + ((size_t) (2 * blockIdx.x + cb) * 2 + q) * 2 * 1024 + lane * 16;
+		int8_t *fr = Wp + ((size_t) ((TC / 32) * blockIdx.x + cb) * 2 + q) * 2 * 1024 + lane * 16;
 		*(uint4 *) fr = make_uint4(hw[0], hw[1], hw[2], hw[3]);
 		*(uint4 *) (fr + 1024) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
 	}
@@ -1079,7 +1084,7 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 			// W formed, written and packed in one kernel, G gathered and
 			// packed in another, then the INT8 update
 			ProfScope ps(ctx, "rref_update");
-			rref_wb_i8_kernel<<<ceil_div(nc, RREF_W), 256, 0, ctx->stream>>>(X, ldx, nc, (int) c, panel, st, Pinv,
+			rref_wb_i8_kernel<32><<<ceil_div(nc, 32), 256, 0, ctx->stream>>>(X, ldx, nc, (int) c, panel, st, Pinv,
 											 Wp, coff, inj, mp);
 			rref_gpack_i8_kernel<<<ceil_div(m, 64), 256, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, panel, st,
 											 Gp, pp);
