@@ -156,6 +156,10 @@ __global__ void fl_assemble_kernel(const uint32_t *__restrict__ B, const uint32_
 	}
 }
 
+// Row groups of the assembly launch: each workgroup walks m / FL_ASM_ROWS
+// output rows of its 256 columns (fewer, longer workgroups than one per row).
+constexpr size_t FL_ASM_ROWS = 1024;
+
 constexpr size_t FL_TRSM_BLOCK = 512;   // rows per block of the pivot-block solve (multiple of TRSM_LEAF_ROWS)
 static_assert(FL_TRSM_BLOCK % TRSM_LEAF_ROWS == 0, "solve blocks must align with the inverted diagonal blocks");
 
@@ -365,7 +369,7 @@ static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *
 	MLZ_HIP(ctx, hipMemcpyAsync(d_pcol, pcol.data(), (rank + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
 	if (R) {
 		ProfScope ps(ctx, "fl_assemble");
-		fl_assemble_kernel<E><<<dim3(ceil_div(n, 256), (unsigned) std::min<size_t>(m, 8192)), 256, 0, ctx->stream>>>(
+		fl_assemble_kernel<E><<<dim3(ceil_div(n, 256), (unsigned) std::min<size_t>(m, FL_ASM_ROWS)), 256, 0, ctx->stream>>>(
 			dB, dD, (int) nr, d_colmap, d_src, d_pcol, (int) rank, (int) m, (int) n, p, (T *) R, ldr);
 		MLZ_TRY(check_launch(ctx, "fl_assemble_kernel"));
 	}
