@@ -1532,7 +1532,7 @@ __device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int
 }
 
 // Waves per SIMD the batched echelon kernel is compiled for: the INT8 variant
-// with 32-column panels fits two 512-thread workgroups per CU (2 x 44 KB of
+// with 32-column panels fits two 512-thread workgroups per CU (2 x 28 KB of
 // LDS, 128 VGPRs), so one instance's pivot search overlaps the other's update.
 constexpr int br_waves_per_simd(int W, bool I8, int NT)
 {
@@ -1895,7 +1895,7 @@ int batch_rref_wgs_per_cu(int elem, int variant, uint32_t p)
 	(void) elem;
 	(void) p;
 	// INT8 with 32-column panels: two 512-thread workgroups per CU; the others
-	// take a CU each (1024 threads, or 90 KB of LDS for INT8 with 64 columns)
+	// take a CU each (1024 threads, or 256 VGPRs for INT8 with 64 columns)
 	return variant == BR_I8_W32 ? 2 : 1;
 }
 
