@@ -579,11 +579,9 @@ __global__ void gf2_scatter_kernel(uint64_t *__restrict__ A, size_t lda, int nwc
 	const int w = blockIdx.x * blockDim.x + threadIdx.x;
 	if (t < k && w < nwc)
 		A[(size_t) st.prow[t] * lda + cw + w] = W[(size_t) t * nwc + w];
-}
-
-__global__ void gf2_advance_kernel(Gf2State st)
-{
-	*st.rank += *st.k;
+	// the panel's pivots join the rank (no workgroup of this launch reads it)
+	if (blockIdx.x == 0 && t == 0 && threadIdx.x == 0)
+		*st.rank += k;
 }
 
 __global__ void gf2_init_kernel(Gf2State st, int m, int cnt_counters)
@@ -682,7 +680,6 @@ int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size
 		gf2_gatherG_kernel<<<ceil_div(m, 256), 256, 0, ctx->stream>>>(Wm, nw, (int) m, (int) cw, (int) nw, panel, st, G);
 		MLZ_TRY(gf2_gemm_dev(ctx, m, (size_t) nwc * 64, G2_PB, 1, G, G2P, Wb, nwc, 1, Wm + cw, nw));
 		gf2_scatter_kernel<<<dim3(ceil_div(nwc, 256), G2_PB), 256, 0, ctx->stream>>>(Wm, nw, nwc, (int) cw, st, Wb);
-		gf2_advance_kernel<<<1, 1, 0, ctx->stream>>>(st);
 		MLZ_TRY(check_launch(ctx, "gf2_scatter_kernel"));
 		if ((panel & 3) == 3) {
 			int r = 0;

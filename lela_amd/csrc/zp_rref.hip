@@ -803,12 +803,30 @@ __global__ __launch_bounds__(256) void rref_gpack_i8_kernel(const uint32_t *__re
 	const int k = *st.k;
 	const int i = 32 * rb + (lane & 31), s0 = 32 * q + 16 * (lane >> 5);
 	const bool live = i < m && st.is_pivot[i] != panel + 1;
+	// A full 64-column panel of full rank has J[s] = s (J increases, so
+	// J[63] = 63 says it): the 16 multipliers are then consecutive columns
+	// inside the panel, four 16-byte loads when the row stride keeps them
+	// aligned (c and s0 are multiples of 16)
+	uint32_t xv[16];
+	if (k == RREF_W && st.J[RREF_W - 1] == RREF_W - 1 && (lda & 3) == 0) {
+		const uint4 *src = (const uint4 *) (A + (size_t) i * lda + c + s0);
+#pragma unroll
+		for (int qq = 0; qq < 4; ++qq) {
+			const uint4 u = live ? src[qq] : make_uint4(0u, 0u, 0u, 0u);
+			xv[4 * qq] = u.x;
+			xv[4 * qq + 1] = u.y;
+			xv[4 * qq + 2] = u.z;
+			xv[4 * qq + 3] = u.w;
+		}
+	} else {
+#pragma unroll
+		for (int b = 0; b < 16; ++b)
+			xv[b] = (live && s0 + b < k) ? A[(size_t) i * lda + c + st.J[s0 + b]] : 0u;
+	}
 	uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
 #pragma unroll
 	for (int b = 0; b < 16; ++b) {
-		const int s = s0 + b;
-		const uint32_t x = (live && s < k) ? A[(size_t) i * lda + c + st.J[s]] : 0u;
-		const uint32_t l = i8_limbs(x, p);
+		const uint32_t l = i8_limbs(xv[b], p);
 		hw[b >> 2] |= ((l >> 8) & 0xffu) << (8 * (b & 3));
 		lw[b >> 2] |= (l & 0xffu) << (8 * (b & 3));
 	}
