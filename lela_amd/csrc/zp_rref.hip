@@ -1235,7 +1235,7 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 // ===========================================================================
 constexpr int BR_CH = 128;   // rows per pivot-search elimination
 constexpr int BR_RB = 64;    // rows per update block (multipliers in LDS), VALU update
-constexpr int BR_RB8 = 256;  // rows per update block, MFMA update (multipliers as limbs)
+constexpr int BR_RB8 = 128;  // rows per update block, MFMA update (multipliers as limbs)
 constexpr int BR_KC = 8;     // multipliers per register chunk
 constexpr int BR_RQ = 8;     // rows per update unit of one wave
 
