@@ -1550,7 +1550,7 @@ __device__ __forceinline__ void br_update_i8(BrLds<W, FMA, true> &S, int nb, int
 }
 
 // Waves per SIMD the batched echelon kernel is compiled for: the INT8 variant
-// with 32-column panels fits two 512-thread workgroups per CU (2 x 28 KB of
+// with 32-column panels fits two 512-thread workgroups per CU (2 x 20 KB of
 // LDS, 128 VGPRs), so one instance's pivot search overlaps the other's update.
 constexpr int br_waves_per_simd(int W, bool I8, int NT)
 {
