@@ -574,17 +574,68 @@ __global__ __launch_bounds__(TPB) void rref_fast_kernel(const uint32_t *__restri
 	}
 }
 
+// Inverse of P_J (k x k, padded with the identity to RREF_W; entry (r, s) =
+// A[prow[r]][c + J[s]]) by Gauss-Jordan on [P_J | I] held in registers (128
+// columns x 2 groups of 32 rows), by one 1024-thread workgroup.  Run by the
+// tournament's root (or its first workgroup when the fast path found the
+// pivots without P_J^-1), so the panel needs no launch of its own for it.
+template <bool SMALL>
+__device__ void rref_inv_body(const uint32_t *__restrict__ A, size_t lda, int c, const ModP &mp, RrefState st,
+			      uint32_t *__restrict__ Pinv)
+{
+	constexpr int W = RREF_W, NC = 2 * RREF_W, RPG = RREF_W / (1024 / NC);
+	__shared__ int alive[W], outr[W], outc[W], pos[W];
+	__shared__ uint32_t f[W], pivrow[NC];
+	__shared__ int s_min[2];
+	const int tid = threadIdx.x, cc = tid % NC, g = tid / NC;
+	const int k = *st.k;
+	uint32_t v[RPG];
+#pragma unroll
+	for (int i = 0; i < RPG; ++i) {
+		const int r = g * RPG + i;
+		uint32_t x;
+		if (cc < W)
+			x = (r < k && cc < k) ? A[(size_t) st.prow[r] * lda + c + st.J[cc]] : (r == cc ? 1u : 0u);
+		else
+			x = (cc - W == r) ? 1u : 0u;
+		v[i] = x;
+	}
+	for (int r = tid; r < W; r += blockDim.x) {
+		alive[r] = 1;
+		pos[r] = r;
+	}
+	__syncthreads();
+	const int cnt = reg_eliminate<NC, RPG, SMALL>(v, W, k, true, mp, alive, f, pivrow, s_min, outr, outc, st.bad);
+	if (tid == 0 && cnt < k)
+		*st.bad = 1;
+	for (int s = tid; s < cnt; s += blockDim.x)
+		pos[outr[s]] = outc[s];
+	__syncthreads();
+	if (cc >= W) {
+#pragma unroll
+		for (int i = 0; i < RPG; ++i) {
+			const int r = g * RPG + i;
+			Pinv[(size_t) pos[r] * W + (cc - W)] = v[i];
+		}
+	}
+}
+
 template <bool SMALL>
 __global__ __launch_bounds__(1024) void rref_tourney_kernel(const uint32_t *__restrict__ A, size_t lda, int m,
-							   int c, int wc, ModP mp, RrefState st, int nodes0)
+							   int c, int wc, ModP mp, RrefState st, int nodes0,
+							   uint32_t *__restrict__ Pinv)
 {
 	constexpr int RPG = RREF_CHUNK / (1024 / RREF_W);
 	__shared__ int rowid[RREF_CHUNK], alive[RREF_CHUNK];
 	__shared__ uint32_t f[RREF_CHUNK], pivrow[RREF_W];
 	__shared__ int outr[RREF_W], outc[RREF_W];
 	__shared__ int s_min[2], s_go, s_nr, s_n0;
-	if (*st.need_tree == 0)
+	if (*st.need_tree == 0) {
+		// the fast path settled the pivots; P_J^-1 may still be missing
+		if (blockIdx.x == 0 && *st.inv_done == 0)
+			rref_inv_body<SMALL>(A, lda, c, mp, st, Pinv);
 		return;
+	}
 	const int tid = threadIdx.x, cc = tid % RREF_W, g = tid / RREF_W;
 	int node = blockIdx.x, level = 0;
 	uint32_t v[RPG];
@@ -619,6 +670,8 @@ __global__ __launch_bounds__(1024) void rref_tourney_kernel(const uint32_t *__re
 				*st.k = cnt;
 			for (int t = tid; t < RREF_MAXLVL * nodes0; t += blockDim.x)
 				st.counters[t] = 0;
+			__syncthreads();   // the published list is read back by the inversion
+			rref_inv_body<SMALL>(A, lda, c, mp, st, Pinv);
 			return;
 		}
 		int *lr = st.lists_rows + ((size_t) level * nodes0 + node) * RREF_W;
@@ -837,52 +890,6 @@ __global__ __launch_bounds__(256) void rref_gpack_i8_kernel(const uint32_t *__re
 		*st.rank += k;
 }
 
-// Inverse of P_J (k x k, padded with the identity to RREF_W; entry (r, s) =
-// A[prow[r]][c + J[s]]) by Gauss-Jordan on [P_J | I] held in registers (128
-// columns x 2 groups of 32 rows).
-template <bool SMALL>
-__global__ __launch_bounds__(1024) void rref_inv_kernel(const uint32_t *__restrict__ A, size_t lda, int c, ModP mp,
-						       RrefState st, uint32_t *__restrict__ Pinv)
-{
-	constexpr int W = RREF_W, NC = 2 * RREF_W, RPG = RREF_W / (1024 / NC);
-	__shared__ int alive[W], outr[W], outc[W], pos[W];
-	__shared__ uint32_t f[W], pivrow[NC];
-	__shared__ int s_min[2];
-	if (*st.inv_done)
-		return;
-	const int tid = threadIdx.x, cc = tid % NC, g = tid / NC;
-	const int k = *st.k;
-	uint32_t v[RPG];
-#pragma unroll
-	for (int i = 0; i < RPG; ++i) {
-		const int r = g * RPG + i;
-		uint32_t x;
-		if (cc < W)
-			x = (r < k && cc < k) ? A[(size_t) st.prow[r] * lda + c + st.J[cc]] : (r == cc ? 1u : 0u);
-		else
-			x = (cc - W == r) ? 1u : 0u;
-		v[i] = x;
-	}
-	for (int r = tid; r < W; r += blockDim.x) {
-		alive[r] = 1;
-		pos[r] = r;
-	}
-	__syncthreads();
-	const int cnt = reg_eliminate<NC, RPG, SMALL>(v, W, k, true, mp, alive, f, pivrow, s_min, outr, outc, st.bad);
-	if (tid == 0 && cnt < k)
-		*st.bad = 1;
-	for (int s = tid; s < cnt; s += blockDim.x)
-		pos[outr[s]] = outc[s];
-	__syncthreads();
-	if (cc >= W) {
-#pragma unroll
-		for (int i = 0; i < RPG; ++i) {
-			const int r = g * RPG + i;
-			Pinv[(size_t) pos[r] * W + (cc - W)] = v[i];
-		}
-	}
-}
-
 // G[i][s] = A[i][c + J[s]] for s < k, except 0 on this panel's pivot rows.
 __global__ void rref_gatherG_kernel(const uint32_t *__restrict__ A, size_t lda, int m, int c, int panel,
 				    RrefState st, uint32_t *__restrict__ G)
@@ -1081,23 +1088,17 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 					threads == 256 ? FAST(256, true) : threads == 512 ? FAST(512, true) : FAST(1024, true);
 				else
 					threads == 256 ? FAST(256, false) : threads == 512 ? FAST(512, false) : FAST(1024, false);
-				rref_tourney_kernel<true><<<nodes0, 1024, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp, st, nodes0);
+				rref_tourney_kernel<true><<<nodes0, 1024, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp, st, nodes0,
+											   Pinv);
 			} else {
 				rref_fast_kernel<false, 1024, false><<<1, 1024, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp,
 												   st, Pinv, nullptr);
-				rref_tourney_kernel<false><<<nodes0, 1024, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp, st, nodes0);
+				rref_tourney_kernel<false><<<nodes0, 1024, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, mp, st,
+											    nodes0, Pinv);
 			}
 #undef FAST
 		}
-		MLZ_TRY(check_launch(ctx, "rref_tourney_kernel"));
-		{
-			ProfScope ps(ctx, "rref_inv");
-			if (mp.small)
-				rref_inv_kernel<true><<<1, 1024, 0, ctx->stream>>>(X, ldx, (int) c, mp, st, Pinv);
-			else
-				rref_inv_kernel<false><<<1, 1024, 0, ctx->stream>>>(X, ldx, (int) c, mp, st, Pinv);
-		}
-		MLZ_TRY(check_launch(ctx, "rref_inv_kernel"));
+		MLZ_TRY(check_launch(ctx, "rref_tourney_kernel"));   // it also forms P_J^-1 when the fast path did not
 		if (upd_i8) {
 			// W formed, written and packed in one kernel, G gathered and
 			// packed in another, then the INT8 update
