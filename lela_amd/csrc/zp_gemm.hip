@@ -48,7 +48,8 @@ inline bool aligned16(const void *p, size_t row_bytes)
 }
 
 // Eight consecutive elements of the contiguous dimension, one vector load
-// when the whole run is inside the matrix and aligned (VEC), else per element.
+// when the whole run is inside the matrix and aligned (VEC), four 32-bit
+// loads for a 4-byte aligned run of 16-bit entries, else per element.
 template <int E, bool VEC>
 __device__ inline void load8(const typename Elem<E>::T *ptr, int valid, uint32_t p, uint32_t (&out)[8])
 {
@@ -70,6 +71,18 @@ __device__ inline void load8(const typename Elem<E>::T *ptr, int valid, uint32_t
 			out[4] = w1.x; out[5] = w1.y; out[6] = w1.z; out[7] = w1.w;
 			return;
 		}
+	}
+	if (sizeof(T) == 2 && valid >= 8 && ((uintptr_t) ptr & 3) == 0) {
+		// a 4-byte aligned run of 16-bit entries (rows of even length, say):
+		// four 32-bit loads instead of eight 16-bit ones
+		const uint32_t *w = (const uint32_t *) ptr;
+#pragma unroll
+		for (int q = 0; q < 4; ++q) {
+			const uint32_t x = w[q];
+			out[2 * q] = x & 0xffffu;
+			out[2 * q + 1] = x >> 16;
+		}
+		return;
 	}
 #pragma unroll
 	for (int q = 0; q < 8; ++q)
