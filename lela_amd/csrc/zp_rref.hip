@@ -1586,9 +1586,24 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 		WT *Wm = (WT *) t.Wm;
 		uint32_t *Wb = t.Wb;
 		int *badp = bad + w.task;
-		for (int r = wave; r < m; r += NT / 64)
-			for (int c = lane; c < n; c += 64)
-				Wm[(size_t) r * ldw + c] = (WT) Elem<E>::load((const T *) t.A + (size_t) r * t.lda + c, p);
+		// 16-bit storage into the 16-bit working copy: 4-byte aligned rows move
+		// as pairs (rows of the copy are 8-byte aligned)
+		constexpr bool PAIRS = I8 && sizeof(T) == 2;
+		for (int r = wave; r < m; r += NT / 64) {
+			const T *src = (const T *) t.A + (size_t) r * t.lda;
+			WT *dst = Wm + (size_t) r * ldw;
+			if (PAIRS && ((uintptr_t) src & 3) == 0) {
+				for (int c = 2 * lane; c < n; c += 128) {
+					if (c + 1 < n)
+						*(uint32_t *) (dst + c) = *(const uint32_t *) (src + c);
+					else
+						dst[c] = (WT) Elem<E>::load(src + c, p);
+				}
+			} else {
+				for (int c = lane; c < n; c += 64)
+					dst[c] = (WT) Elem<E>::load(src + c, p);
+			}
+		}
 		for (int r = tid; r < m; r += NT)
 			S.is_piv[r] = 0;
 		__syncthreads();
@@ -1850,9 +1865,20 @@ __global__ __launch_bounds__(NT, br_waves_per_simd(W, I8, NT)) void batch_rref_k
 		// ---- standard layout: pivot rows in pivot-column order, then zero rows
 		for (int r = wave; r < m; r += NT / 64) {
 			const int src = r < rank ? t.piv_rows[r] : -1;
-			for (int c = lane; c < n; c += 64) {
-				const uint32_t x = src >= 0 ? Wm[(size_t) src * ldw + c] : 0u;
-				Elem<E>::store((T *) t.A + (size_t) r * t.lda + c, x, p);
+			T *dst = (T *) t.A + (size_t) r * t.lda;
+			if (PAIRS && ((uintptr_t) dst & 3) == 0) {
+				const WT *srow = Wm + (size_t) (src >= 0 ? src : 0) * ldw;
+				for (int c = 2 * lane; c < n; c += 128) {
+					if (c + 1 < n)
+						*(uint32_t *) (dst + c) = src >= 0 ? *(const uint32_t *) (srow + c) : 0u;
+					else
+						Elem<E>::store(dst + c, src >= 0 ? (uint32_t) srow[c] : 0u, p);
+				}
+			} else {
+				for (int c = lane; c < n; c += 64) {
+					const uint32_t x = src >= 0 ? Wm[(size_t) src * ldw + c] : 0u;
+					Elem<E>::store(dst + c, x, p);
+				}
 			}
 		}
 		if (t.pivots)
