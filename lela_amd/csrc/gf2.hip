@@ -197,7 +197,10 @@ __device__ int gf2_reg_eliminate(uint64_t (&v)[RPT][WPR], bool (&alive)[RPT], in
 #pragma unroll
 	for (int w = 0; w < WPR; ++w)
 		live_cols[w] = s_or[w];
-	int cnt = 0;
+	// s_min is two words used in turn by the columns searched (not by column
+	// parity: skipped columns would leave the last pivot row in the word the
+	// next search reads); the one not in use is cleared for the next search.
+	int cnt = 0, ph = 0;
 	for (int col = 0; col < nsearch; ++col) {
 		if (!((live_cols[col >> 6] >> (col & 63)) & 1ull))
 			continue;
@@ -213,11 +216,12 @@ __device__ int gf2_reg_eliminate(uint64_t (&v)[RPT][WPR], bool (&alive)[RPT], in
 				cand = 256 * q + (tid & ~63) + __ffsll((long long) bal) - 1;
 		}
 		if ((tid & 63) == 0 && cand != INT_MAX)
-			atomicMin(&s_min[col & 1], cand);
+			atomicMin(&s_min[ph], cand);
 		__syncthreads();
-		const int pr = s_min[col & 1];
+		const int pr = s_min[ph];
 		if (tid == 0)
-			s_min[(col + 1) & 1] = INT_MAX;
+			s_min[ph ^ 1] = INT_MAX;
+		ph ^= 1;
 		if (pr != INT_MAX) {
 #pragma unroll
 			for (int q = 0; q < RPT; ++q)

@@ -56,22 +56,45 @@ def lowrank_gf2(seed, m, n, r):
     return orc.gf2_gemm(m, n, r, 1, X, Y, 0, Z)
 
 
+def holes_gf2(seed, m, n):
+    """A random matrix with columns j = 1, 3 (mod 5) cleared: every cleared
+    column sits between two pivot columns."""
+    A = synth.gf2_matrix(seed, m, n)
+    keep = np.array([j % 5 not in (1, 3) for j in range(A.shape[1] * 64)], dtype=np.uint64)
+    return A & (keep.reshape(-1, 64) << np.arange(64, dtype=np.uint64)).sum(axis=1, dtype=np.uint64)
+
+
+# "holes": zero columns between pivot columns, through the single-set path
+# (m <= 512) and the tournament (m > 512, rank-deficient panels); a search
+# that skips a zero column must not pick the previous column's pivot row again.
 @pytest.mark.parametrize("m,n,kind", [(10, 70, "rand"), (70, 10, "rand"), (300, 300, "rand"),
                                       (500, 700, "low"), (1500, 400, "low"), (257, 1000, "sparse"),
-                                      (64, 64, "zero"), (2100, 900, "low")])
+                                      (64, 64, "zero"), (2100, 900, "low"), (847, 597, "low260"),
+                                      (300, 200, "holes"), (1100, 600, "holes")])
 def test_gf2_rref_matches_oracle(gpu_ctx, m, n, kind):
     if kind == "rand":
         A = synth.gf2_matrix(7, m, n)
     elif kind == "low":
         A = lowrank_gf2(8, m, n, min(m, n) // 3)
+    elif kind == "low260":
+        A = lowrank_gf2(20, m, n, 260)
     elif kind == "sparse":
         A = synth.gf2_matrix(9, m, n) & synth.gf2_matrix(10, m, n) & synth.gf2_matrix(11, m, n)
+    elif kind == "holes":
+        A = holes_gf2(12, m, n)
     else:
         A = np.zeros((m, (n + 63) // 64), dtype=np.uint64)
     got, rank, piv = gpu_rref(gpu_ctx, m, n, A)
     want, wrank, wpiv = orc.gf2_echelonize(m, n, A, reduced=True)
     assert rank == wrank and list(piv) == list(wpiv)
     assert np.array_equal(got, want)
+
+
+def test_gf2_rref_zero_column_between_pivots(gpu_ctx):
+    A = np.array([[1], [4]], dtype=np.uint64)          # rows e0 and e2 of GF(2)^3
+    got, rank, piv = gpu_rref(gpu_ctx, 2, 3, A)
+    assert rank == 2 and list(piv) == [0, 2]
+    assert np.array_equal(got, A)
 
 
 def _gf2_matvec(W, ncols, x_bits):
