@@ -133,31 +133,51 @@ __global__ void fl_gather_cols_kernel(const uint32_t *__restrict__ B, int r, int
 
 // Output row t: src[t] >= 0 -> X pivot row src[t] (B row), pivot column
 // pcol[t]; src[t] < 0 -> D pivot row -1-src[t] (RREF(D) row); t >= rank -> 0.
-template <int E>
+template <int E, bool VEC>
 __global__ void fl_assemble_kernel(const uint32_t *__restrict__ B, const uint32_t *__restrict__ D, int nr,
 				   const int *__restrict__ colmap, const int *__restrict__ src,
 				   const int *__restrict__ pcol, int rank, int m, int n, uint32_t p,
 				   typename Elem<E>::T *__restrict__ R, size_t ldr)
 {
-	const int j = blockIdx.x * blockDim.x + threadIdx.x;
-	if (j >= n)
+	// four consecutive columns per thread; VEC (R and ldr aligned to four
+	// entries) writes them as one vector store
+	typedef typename Elem<E>::T T;
+	typedef T T4 __attribute__((ext_vector_type(4)));
+	const int j0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+	if (j0 >= n)
 		return;
-	const int cj = colmap[j];
+	int cj[4];
+#pragma unroll
+	for (int q = 0; q < 4; ++q)
+		cj[q] = j0 + q < n ? colmap[j0 + q] : 0;
 	for (int t = blockIdx.y; t < m; t += gridDim.y) {
-		uint32_t v = 0;
+		uint32_t v[4] = {0u, 0u, 0u, 0u};
 		if (t < rank) {
-			const int s = src[t];
-			if (cj >= 0)
-				v = (s >= 0 && j == pcol[t]) ? 1u : 0u;
-			else
-				v = s >= 0 ? B[(size_t) s * nr + (-1 - cj)] : D[(size_t) (-1 - s) * nr + (-1 - cj)];
+			const int s = src[t], pc = pcol[t];
+			const uint32_t *row = s >= 0 ? B + (size_t) s * nr : D + (size_t) (-1 - s) * nr;
+#pragma unroll
+			for (int q = 0; q < 4; ++q)
+				if (j0 + q < n)
+					v[q] = cj[q] >= 0 ? ((s >= 0 && j0 + q == pc) ? 1u : 0u) : row[-1 - cj[q]];
 		}
-		Elem<E>::store(R + (size_t) t * ldr + j, v, p);   // B and D hold residues below p
+		T *out = R + (size_t) t * ldr + j0;
+		if (VEC && j0 + 4 <= n) {
+			T o[4];
+#pragma unroll
+			for (int q = 0; q < 4; ++q)
+				Elem<E>::store(&o[q], v[q], p);   // B and D hold residues below p
+			*(T4 *) out = T4{o[0], o[1], o[2], o[3]};
+		} else {
+#pragma unroll
+			for (int q = 0; q < 4; ++q)
+				if (j0 + q < n)
+					Elem<E>::store(out + q, v[q], p);
+		}
 	}
 }
 
 // Row groups of the assembly launch: each workgroup walks m / FL_ASM_ROWS
-// output rows of its 256 columns (fewer, longer workgroups than one per row).
+// output rows of its 1024 columns (fewer, longer workgroups than one per row).
 constexpr size_t FL_ASM_ROWS = 1024;
 
 constexpr size_t FL_TRSM_BLOCK = 512;   // rows per block of the pivot-block solve (multiple of TRSM_LEAF_ROWS)
@@ -369,8 +389,13 @@ static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *
 	MLZ_HIP(ctx, hipMemcpyAsync(d_pcol, pcol.data(), (rank + 1) * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
 	if (R) {
 		ProfScope ps(ctx, "fl_assemble");
-		fl_assemble_kernel<E><<<dim3(ceil_div(n, 256), (unsigned) std::min<size_t>(m, FL_ASM_ROWS)), 256, 0, ctx->stream>>>(
-			dB, dD, (int) nr, d_colmap, d_src, d_pcol, (int) rank, (int) m, (int) n, p, (T *) R, ldr);
+		const dim3 grid(ceil_div(ceil_div(n, 4), 256), (unsigned) std::min<size_t>(m, FL_ASM_ROWS));
+		if ((uintptr_t) R % (4 * sizeof(T)) == 0 && ldr % 4 == 0)
+			fl_assemble_kernel<E, true><<<grid, 256, 0, ctx->stream>>>(dB, dD, (int) nr, d_colmap, d_src, d_pcol,
+										 (int) rank, (int) m, (int) n, p, (T *) R, ldr);
+		else
+			fl_assemble_kernel<E, false><<<grid, 256, 0, ctx->stream>>>(dB, dD, (int) nr, d_colmap, d_src, d_pcol,
+										  (int) rank, (int) m, (int) n, p, (T *) R, ldr);
 		MLZ_TRY(check_launch(ctx, "fl_assemble_kernel"));
 	}
 	MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
