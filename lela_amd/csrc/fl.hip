@@ -54,11 +54,15 @@ __global__ void fl_heads_kernel(const uint64_t *__restrict__ row_ptr, const uint
 	hval[i] = hv;
 }
 
+// X[0 .. count) = 0, four entries per thread (X is 16-byte aligned).
 __global__ void fl_zero_kernel(uint32_t *__restrict__ X, size_t count)
 {
-	const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < count)
-		X[i] = 0;
+	const size_t i = 4 * ((size_t) blockIdx.x * blockDim.x + threadIdx.x);
+	if (i + 4 <= count)
+		*(uint4 *) (X + i) = make_uint4(0u, 0u, 0u, 0u);
+	else
+		for (size_t q = i; q < count; ++q)
+			X[q] = 0;
 }
 
 // rowmap[i] = index among pivot rows (>= 0) or -(1 + index among the others);
@@ -93,21 +97,31 @@ __global__ void fl_scatter_kernel(const uint64_t *__restrict__ row_ptr, const ui
 
 // last[ri] = largest A-column index with a nonzero entry in pivot row ri (its
 // head at least, since A is upper triangular with the heads on the diagonal).
+// One wave per row (FL_LASTCOL_ROWS rows per workgroup): the lanes stride
+// over the row's entries and a shuffle reduction takes the maximum.
+constexpr int FL_LASTCOL_ROWS = 4;
 template <int E>
-__global__ void fl_lastcol_kernel(const uint64_t *__restrict__ row_ptr, const uint32_t *__restrict__ col,
-				  const typename Elem<E>::T *__restrict__ val, int m, uint32_t p,
-				  const int *__restrict__ rowmap, const int *__restrict__ colmap, int *__restrict__ last)
+__global__ __launch_bounds__(64 * FL_LASTCOL_ROWS) void fl_lastcol_kernel(
+	const uint64_t *__restrict__ row_ptr, const uint32_t *__restrict__ col, const typename Elem<E>::T *__restrict__ val,
+	int m, uint32_t p, const int *__restrict__ rowmap, const int *__restrict__ colmap, int *__restrict__ last)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= m || rowmap[i] < 0)
+	const int i = blockIdx.x * FL_LASTCOL_ROWS + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+	if (i >= m)
+		return;   // i and ri are uniform across the wave
+	const int ri = rowmap[i];
+	if (ri < 0)
 		return;
-	int mx = rowmap[i];
-	for (uint64_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e) {
+	int mx = ri;
+	for (uint64_t e = row_ptr[i] + lane; e < row_ptr[i + 1]; e += 64) {
 		const int cj = colmap[col[e]];
 		if (cj > mx && Elem<E>::load(val + e, p) % p != 0)
 			mx = cj;
 	}
-	last[rowmap[i]] = mx;
+#pragma unroll
+	for (int off = 32; off >= 1; off >>= 1)
+		mx = max(mx, __shfl_xor(mx, off, 64));
+	if (lane == 0)
+		last[ri] = mx;
 }
 
 // B[i][j0 .. j0 + w) = 0 for every row i < r (grid.y strides over the rows).
@@ -264,7 +278,7 @@ static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *
 		uint32_t *bufs[4] = {dA, dB, dC, dD};
 		for (int q = 0; q < 4; ++q)
 			if (cnt[q])
-				fl_zero_kernel<<<ceil_div(cnt[q], 256), 256, 0, ctx->stream>>>(bufs[q], cnt[q]);
+				fl_zero_kernel<<<ceil_div(ceil_div(cnt[q], 4), 256), 256, 0, ctx->stream>>>(bufs[q], cnt[q]);
 		fl_scatter_kernel<E><<<(unsigned) m, 128, 0, ctx->stream>>>(d_rowptr, d_col, (const T *) d_val, (int) m, p,
 									   d_rowmap, d_colmap, (int) r, (int) nr, dA, dB,
 									   dC, dD);
@@ -279,8 +293,8 @@ static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *
 	if (r && nr) {
 		void *wl;
 		MLZ_TRY(ws_get(ctx, WS_FL_5, r * sizeof(int) + 256, &wl));
-		fl_lastcol_kernel<E><<<ceil_div(m, 256), 256, 0, ctx->stream>>>(d_rowptr, d_col, (const T *) d_val, (int) m,
-										 p, d_rowmap, d_colmap, (int *) wl);
+		fl_lastcol_kernel<E><<<ceil_div(m, (size_t) FL_LASTCOL_ROWS), 64 * FL_LASTCOL_ROWS, 0, ctx->stream>>>(
+			d_rowptr, d_col, (const T *) d_val, (int) m, p, d_rowmap, d_colmap, (int *) wl);
 		MLZ_TRY(check_launch(ctx, "fl_lastcol_kernel"));
 		std::vector<int> last(r);
 		MLZ_HIP(ctx, hipMemcpyAsync(last.data(), wl, r * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
