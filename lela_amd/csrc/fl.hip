@@ -134,15 +134,17 @@ __global__ void fl_zero_cols_kernel(uint32_t *__restrict__ B, int r, int nr, int
 		B[(size_t) i * nr + j0 + j] = 0u;
 }
 
-// G[i][s] = B[i][jd[s]]  (r x rd)
+// G[i][s] = B[i][jd[s]]  (r x rd); x covers the columns s, grid.y strides
+// over the rows i.
 __global__ void fl_gather_cols_kernel(const uint32_t *__restrict__ B, int r, int nr, const int *__restrict__ jd,
 				      int rd, uint32_t *__restrict__ G)
 {
-	const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-	if (idx >= (size_t) r * rd)
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= rd)
 		return;
-	const int i = (int) (idx / rd), s = (int) (idx % rd);
-	G[idx] = B[(size_t) i * nr + jd[s]];
+	const int js = jd[s];
+	for (int i = blockIdx.y; i < r; i += gridDim.y)
+		G[(size_t) i * rd + s] = B[(size_t) i * nr + js];
 }
 
 // Output row t: src[t] >= 0 -> X pivot row src[t] (B row), pivot column
@@ -354,8 +356,8 @@ static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *
 		MLZ_TRY(ws_get(ctx, WS_FL_6, rd * sizeof(int) + 256, &wj));
 		std::vector<int> jdi(jd.begin(), jd.begin() + rd);
 		MLZ_HIP(ctx, hipMemcpyAsync(wj, jdi.data(), rd * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-		fl_gather_cols_kernel<<<ceil_div(r * rd, 256), 256, 0, ctx->stream>>>(dB, (int) r, (int) nr, (const int *) wj,
-										     (int) rd, (uint32_t *) wg);
+		fl_gather_cols_kernel<<<dim3(ceil_div(rd, 256), (unsigned) std::min(r, FL_ASM_ROWS)), 256, 0, ctx->stream>>>(
+			dB, (int) r, (int) nr, (const int *) wj, (int) rd, (uint32_t *) wg);
 		MLZ_TRY(check_launch(ctx, "fl_gather_cols_kernel"));
 		// On D's pivot columns the product is B's own column (RREF(D) has the
 		// identity there), so those columns of B become zero.  When they are
