@@ -964,28 +964,73 @@ __global__ void rref_block_trail_kernel(uint32_t *__restrict__ X, int xw, int nb
 	}
 }
 
-template <int E>
+// Working copy W (m x n, row stride n, canonical residues) of A (stride ld)
+// and back.  Four columns per thread, grid.y strides over the rows; VEC (n,
+// ld and both pointers aligned to four entries) moves them with one vector
+// load and one vector store.
+constexpr size_t RREF_COPY_ROWS = 1024;   // grid.y of the copy launches
+
+template <int E, bool VEC>
 __global__ void rref_load_kernel(const typename Elem<E>::T *__restrict__ src, size_t ld, int m, int n,
 				 uint32_t p, uint32_t *__restrict__ dst)
 {
-	size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-	if (idx >= (size_t) m * n)
+	typedef typename Elem<E>::T T;
+	typedef T T4 __attribute__((ext_vector_type(4)));
+	typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+	const int c0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+	if (c0 >= n)
 		return;
-	int r = (int) (idx / n), cc = (int) (idx % n);
-	dst[idx] = Elem<E>::load(src + (size_t) r * ld + cc, p);
+	for (int r = blockIdx.y; r < m; r += gridDim.y) {
+		const T *s = src + (size_t) r * ld + c0;
+		uint32_t *d = dst + (size_t) r * n + c0;
+		if (VEC) {   // n % 4 == 0: the four columns exist
+			const T4 x = *(const T4 *) s;
+			const T t[4] = {x.x, x.y, x.z, x.w};
+			U4 y;
+			y.x = Elem<E>::load(&t[0], p);
+			y.y = Elem<E>::load(&t[1], p);
+			y.z = Elem<E>::load(&t[2], p);
+			y.w = Elem<E>::load(&t[3], p);
+			*(U4 *) d = y;
+		} else {
+#pragma unroll
+			for (int q = 0; q < 4; ++q)
+				if (c0 + q < n)
+					d[q] = Elem<E>::load(s + q, p);
+		}
+	}
 }
 
 // Output row t: pivot row t for t < rank, zero row after.
-template <int E>
+template <int E, bool VEC>
 __global__ void rref_store_kernel(const uint32_t *__restrict__ W, int m, int n, uint32_t p, RrefState st,
 				  typename Elem<E>::T *__restrict__ dst, size_t ld)
 {
-	size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-	if (idx >= (size_t) m * n)
+	typedef typename Elem<E>::T T;
+	typedef T T4 __attribute__((ext_vector_type(4)));
+	typedef uint32_t U4 __attribute__((ext_vector_type(4)));
+	const int c0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+	if (c0 >= n)
 		return;
-	int t = (int) (idx / n), cc = (int) (idx % n);
-	uint32_t v = t < *st.rank ? W[(size_t) st.piv_rows[t] * n + cc] : 0u;
-	Elem<E>::store(dst + (size_t) t * ld + cc, v, p);
+	const int rank = *st.rank;
+	for (int t = blockIdx.y; t < m; t += gridDim.y) {
+		T *o = dst + (size_t) t * ld + c0;
+		const uint32_t *w = t < rank ? W + (size_t) st.piv_rows[t] * n + c0 : nullptr;
+		if (VEC) {
+			const U4 v = w ? *(const U4 *) w : U4{0u, 0u, 0u, 0u};
+			T x[4];
+			Elem<E>::store(&x[0], v.x, p);
+			Elem<E>::store(&x[1], v.y, p);
+			Elem<E>::store(&x[2], v.z, p);
+			Elem<E>::store(&x[3], v.w, p);
+			*(T4 *) o = T4{x[0], x[1], x[2], x[3]};
+		} else {
+#pragma unroll
+			for (int q = 0; q < 4; ++q)
+				if (c0 + q < n)
+					Elem<E>::store(o + q, w ? w[q] : 0u, p);
+		}
+	}
 }
 
 __global__ void rref_init_kernel(RrefState st, int m, int cnt_counters)
@@ -1002,6 +1047,28 @@ __global__ void rref_init_kernel(RrefState st, int m, int cnt_counters)
 		*st.bad = 0;
 		*st.need_tree = 0;
 		*st.inv_done = 0;
+	}
+}
+
+// A -> W (LOAD) or the pivot rows of W -> A, vectorised when n, lda and A
+// are aligned to four entries (W is workspace, aligned).
+template <int E, bool LOAD>
+static void rref_copy_launch(mlz_ctx *ctx, typename Elem<E>::T *A, size_t lda, size_t m, size_t n, uint32_t p,
+			     uint32_t *W, const RrefState &st)
+{
+	typedef typename Elem<E>::T T;
+	const dim3 grid(ceil_div(ceil_div(n, 4), 256), (unsigned) std::min(m, RREF_COPY_ROWS));
+	const bool vec = n % 4 == 0 && lda % 4 == 0 && (uintptr_t) A % (4 * sizeof(T)) == 0;
+	if (LOAD) {
+		if (vec)
+			rref_load_kernel<E, true><<<grid, 256, 0, ctx->stream>>>(A, lda, (int) m, (int) n, p, W);
+		else
+			rref_load_kernel<E, false><<<grid, 256, 0, ctx->stream>>>(A, lda, (int) m, (int) n, p, W);
+	} else {
+		if (vec)
+			rref_store_kernel<E, true><<<grid, 256, 0, ctx->stream>>>(W, (int) m, (int) n, p, st, A, lda);
+		else
+			rref_store_kernel<E, false><<<grid, 256, 0, ctx->stream>>>(W, (int) m, (int) n, p, st, A, lda);
 	}
 }
 
@@ -1056,13 +1123,12 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 		const int cc = RREF_MAXLVL * nodes0;
 		rref_init_kernel<<<ceil_div(std::max(m, (size_t) cc), 256), 256, 0, ctx->stream>>>(st, (int) m, cc);
 		ProfScope ps(ctx, "rref_load");
-		unsigned g = ceil_div(mn, 256);
 		if (elem == MLZ_U16)
-			rref_load_kernel<MLZ_U16><<<g, 256, 0, ctx->stream>>>((const uint16_t *) A, lda, (int) m, (int) n, pp, Wm);
+			rref_copy_launch<MLZ_U16, true>(ctx, (uint16_t *) A, lda, m, n, pp, Wm, st);
 		else if (elem == MLZ_U32)
-			rref_load_kernel<MLZ_U32><<<g, 256, 0, ctx->stream>>>((const uint32_t *) A, lda, (int) m, (int) n, pp, Wm);
+			rref_copy_launch<MLZ_U32, true>(ctx, (uint32_t *) A, lda, m, n, pp, Wm, st);
 		else
-			rref_load_kernel<MLZ_F64_SYM><<<g, 256, 0, ctx->stream>>>((const double *) A, lda, (int) m, (int) n, pp, Wm);
+			rref_copy_launch<MLZ_F64_SYM, true>(ctx, (double *) A, lda, m, n, pp, Wm, st);
 		MLZ_TRY(check_launch(ctx, "rref_load_kernel"));
 	}
 	// trailing updates on INT8 limbs whenever the gemm family allows them
@@ -1189,13 +1255,12 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 		return set_error(ctx, MLZ_E_NOT_INVERTIBLE, "Could not invert pivot-element in the ring");
 	{
 		ProfScope ps(ctx, "rref_store");
-		unsigned g = ceil_div(mn, 256);
 		if (elem == MLZ_U16)
-			rref_store_kernel<MLZ_U16><<<g, 256, 0, ctx->stream>>>(Wm, (int) m, (int) n, pp, st, (uint16_t *) A, lda);
+			rref_copy_launch<MLZ_U16, false>(ctx, (uint16_t *) A, lda, m, n, pp, Wm, st);
 		else if (elem == MLZ_U32)
-			rref_store_kernel<MLZ_U32><<<g, 256, 0, ctx->stream>>>(Wm, (int) m, (int) n, pp, st, (uint32_t *) A, lda);
+			rref_copy_launch<MLZ_U32, false>(ctx, (uint32_t *) A, lda, m, n, pp, Wm, st);
 		else
-			rref_store_kernel<MLZ_F64_SYM><<<g, 256, 0, ctx->stream>>>(Wm, (int) m, (int) n, pp, st, (double *) A, lda);
+			rref_copy_launch<MLZ_F64_SYM, false>(ctx, (double *) A, lda, m, n, pp, Wm, st);
 		MLZ_TRY(check_launch(ctx, "rref_store_kernel"));
 	}
 	if (pivots_out && r > 0) {
