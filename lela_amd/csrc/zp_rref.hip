@@ -925,24 +925,25 @@ __global__ void rref_advance_kernel(RrefState st)
 __global__ void rref_block_load_kernel(const uint32_t *__restrict__ A, size_t lda, int m, int c0, int nbw,
 				       uint32_t *__restrict__ X, int xw, RrefState st)
 {
-	const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-	if (idx == 0)
+	// x covers the columns j, grid.y strides over the rows
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
 		*st.rank0 = *st.rank;
-	if (idx >= (size_t) m * xw)
+	if (j >= xw)
 		return;
-	const int i = (int) (idx / xw), j = (int) (idx % xw);
-	X[idx] = j < nbw ? A[(size_t) i * lda + c0 + j] : 0u;
+	for (int i = blockIdx.y; i < m; i += gridDim.y)
+		X[(size_t) i * xw + j] = j < nbw ? A[(size_t) i * lda + c0 + j] : 0u;
 }
 
 // The block's columns back into A.
 __global__ void rref_block_store_kernel(const uint32_t *__restrict__ X, int xw, int m, int nbw,
 					uint32_t *__restrict__ A, size_t lda, int c0)
 {
-	const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-	if (idx >= (size_t) m * nbw)
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= nbw)
 		return;
-	const int i = (int) (idx / nbw), j = (int) (idx % nbw);
-	A[(size_t) i * lda + c0 + j] = X[(size_t) i * xw + j];
+	for (int i = blockIdx.y; i < m; i += gridDim.y)
+		A[(size_t) i * lda + c0 + j] = X[(size_t) i * xw + j];
 }
 
 // After a column block: TR[t] = the trailing columns (from c0 + nbw on) of
@@ -1224,16 +1225,16 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 			const size_t xw = nt ? 2 * nbw : nbw;
 			{
 				ProfScope ps(ctx, "rref_block");
-				rref_block_load_kernel<<<ceil_div(m * xw, 256), 256, 0, ctx->stream>>>(Wm, n, (int) m, (int) c0,
-													  (int) nbw, X, (int) xw, st);
+				rref_block_load_kernel<<<dim3(ceil_div(xw, 256), (unsigned) std::min(m, RREF_COPY_ROWS)), 256, 0,
+							 ctx->stream>>>(Wm, n, (int) m, (int) c0, (int) nbw, X, (int) xw, st);
 			}
 			MLZ_TRY(check_launch(ctx, "rref_block_load_kernel"));
 			for (size_t c = 0; c < nbw; c += RREF_W)
 				MLZ_TRY(panel_step(X, xw, nt ? xw : nbw, c, (int) c0, nt ? (int) nbw : -1));
 			{
 				ProfScope ps(ctx, "rref_block");
-				rref_block_store_kernel<<<ceil_div(m * nbw, 256), 256, 0, ctx->stream>>>(X, (int) xw, (int) m,
-													    (int) nbw, Wm, n, (int) c0);
+				rref_block_store_kernel<<<dim3(ceil_div(nbw, 256), (unsigned) std::min(m, RREF_COPY_ROWS)), 256, 0,
+							  ctx->stream>>>(X, (int) xw, (int) m, (int) nbw, Wm, n, (int) c0);
 				if (nt)
 					rref_block_trail_kernel<<<dim3(ceil_div(nt, 256), nbw), 256, 0, ctx->stream>>>(
 						X, (int) xw, (int) nbw, Wm, n, (int) c0, (int) nt, st, TR, pp);
