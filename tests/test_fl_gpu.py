@@ -132,29 +132,64 @@ def test_fl_zero_row_ends_scan(gpu_ctx):
     assert rc == 0 and rank == wrank and np.array_equal(got, want)
 
 
+def csr_matvec_mod(csr, shape, v, p):
+    """(X @ v) mod p for X in CSR with entries < p < 2^32 and v < p (exact)."""
+    row_ptr, col, val = csr
+    prod = (val.astype(np.uint64) * v[col.astype(np.int64)].astype(np.uint64)) % np.uint64(p)
+    rows = np.repeat(np.arange(shape[0]), np.diff(row_ptr.astype(np.int64)))
+    out = np.zeros(shape[0], dtype=np.uint64)
+    np.add.at(out, rows, prod)          # at most n terms below 2^32 per row
+    return out % np.uint64(p)
+
+
+def check_rref_all_rows(R, piv, rank, csr, shape, p, seed, chunk=1000):
+    """Every row of R: the standard RREF structure (rows past the rank zero,
+    row t's first nonzero a 1 in column piv[t], the pivot columns the identity
+    on the first rank rows), and X = X[:, piv] R at a random vector over every
+    row of X: X v == X[:, piv] (R v) mod p."""
+    m, n = shape
+    piv = piv.astype(np.int64)
+    assert rank == piv.size and (rank == 0 or np.all(np.diff(piv) > 0))
+    assert not R[rank:].any()
+    rng = np.random.default_rng(seed)
+    v = rng.integers(0, p, size=n, dtype=np.uint64)
+    Rv = np.zeros(rank, dtype=np.uint64)
+    P = np.uint64(p)
+    for s0 in range(0, rank, chunk):
+        blk = R[s0:min(rank, s0 + chunk)]
+        nzfirst = (blk != 0).argmax(axis=1)
+        assert np.array_equal(nzfirst, piv[s0:s0 + blk.shape[0]]), f"row heads near row {s0}"
+        sub = blk[:, piv]
+        assert np.count_nonzero(sub) == blk.shape[0], f"pivot columns near row {s0}"
+        assert np.all(sub[np.arange(blk.shape[0]), np.arange(s0, s0 + blk.shape[0])] == 1)
+        Rv[s0:s0 + blk.shape[0]] = (blk.astype(np.uint64) @ v) % P   # n terms < 2^32: below 2^64
+    lhs = csr_matvec_mod(csr, shape, v, p)
+    # X[:, piv] @ Rv: the entries of X in pivot columns, weighted by Rv at their position
+    pos = np.full(n, -1, dtype=np.int64)
+    pos[piv] = np.arange(rank)
+    row_ptr, col, val = csr
+    w = np.zeros(n, dtype=np.uint64)
+    w[piv] = Rv
+    sel = pos[col.astype(np.int64)] >= 0
+    row_ptr2 = np.zeros(m + 1, dtype=np.uint64)
+    rows = np.repeat(np.arange(m), np.diff(row_ptr.astype(np.int64)))
+    np.add.at(row_ptr2, rows[sel] + 1, 1)
+    csr2 = (np.cumsum(row_ptr2, dtype=np.uint64), col[sel], val[sel])
+    rhs = csr_matvec_mod(csr2, shape, w, p)
+    assert np.array_equal(lhs, rhs), "a row of X is not X[:, piv] R"
+
+
 def test_fl_config4_full(gpu_ctx):
-    """20000 x 30000, p = 65521: rank, RREF structure on sampled rows, and
-    sampled rows of X lie in the row space of R."""
+    """20000 x 30000, p = 65521: rank 20000, the RREF structure of every row
+    of R, and every row of X equals X[:, piv] R (checked at a random vector)."""
     p = 65521
     csr, shape = synth.fl_matrix(synth.seed_for(4, 0), p)
     F = la.Modular(p, "uint32")
     gpu_ctx.F = F
     FL = la.FaugereLachartre(gpu_ctx)
     R = np.zeros(shape, dtype=np.uint32)
+    print("fl config 4", flush=True)
     rank, det = FL.echelonize(R, csr, shape=shape)
-    piv = FL.pivots
     assert rank == 20000
-    assert np.all(np.diff(piv.astype(np.int64)) > 0)
-    assert not R[rank:].any()
-    P = np.uint64(p)
-    row_ptr, col, val = csr
-    for i in range(0, shape[0], 997):
-        assert R[i, piv[i]] == 1 and R[i, :piv[i]].sum() == 0
-        assert np.count_nonzero(R[i, piv]) == 1
-        x = np.zeros(shape[1], dtype=np.uint64)
-        s, e = int(row_ptr[i]), int(row_ptr[i + 1])
-        x[col[s:e].astype(np.int64)] = val[s:e]
-        coef = x[piv]
-        nzc = np.nonzero(coef)[0]
-        recon = (coef[nzc] @ R[nzc].astype(np.uint64)) % P
-        assert np.array_equal(recon, x)
+    print("all-row check", flush=True)
+    check_rref_all_rows(R, FL.pivots, rank, csr, shape, p, seed=11)

@@ -143,17 +143,58 @@ def test_gemm_device_pointers(gpu_ctx):
     assert np.array_equal(got, want)
 
 
+def matvec_mod(M, x, p):
+    """(M @ x) mod p exactly in uint64 for M entries < p and x < p: x is split
+    into limbs narrow enough that no partial sum reaches 2^64."""
+    P = np.uint64(p)
+    bits = max(1, int(p - 1).bit_length())
+    kbits = max(1, int(M.shape[1]).bit_length())
+    lb = max(1, 63 - bits - kbits)
+    Mu = M.astype(np.uint64)
+    out = np.zeros(M.shape[0], dtype=np.uint64)
+    x = x.astype(np.uint64)
+    shift = 0
+    while shift < bits:
+        limb = (x >> np.uint64(shift)) & np.uint64((1 << lb) - 1)
+        part = (Mu @ limb) % P
+        out = (out + part * np.uint64(pow(2, shift, p))) % P
+        shift += lb
+    return out
+
+
 def _freivalds(A, B, C0, Cnew, p, a, b, seed):
     """Checks Cnew == a*A*B + b*C0 (mod p) on random vectors: exact uint64 math."""
     rng = np.random.default_rng(seed)
     P = np.uint64(p)
     for _ in range(2):
         x = rng.integers(0, p, size=B.shape[1], dtype=np.uint64)
-        Bx = (B.astype(np.uint64) @ x) % P
-        ABx = (A.astype(np.uint64) @ Bx) % P
-        lhs = (Cnew.astype(np.uint64) @ x) % P
-        rhs = (np.uint64(a) * ABx + np.uint64(b) * ((C0.astype(np.uint64) @ x) % P)) % P
+        ABx = matvec_mod(A, matvec_mod(B, x, p), p)
+        lhs = matvec_mod(Cnew, x, p)
+        rhs = (np.uint64(a) * ABx + np.uint64(b) * matvec_mod(C0, x, p)) % P
         assert np.array_equal(lhs, rhs)
+
+
+def test_matvec_mod_is_exact():
+    """The limb split of matvec_mod against Python integers, at the largest entries."""
+    p = 4294967291
+    rng = np.random.default_rng(3)
+    M = rng.integers(p - 1000, p, size=(3, 9000), dtype=np.uint64)
+    x = rng.integers(p - 1000, p, size=9000, dtype=np.uint64)
+    want = [sum(int(a) * int(b) for a, b in zip(row, x)) % p for row in M]
+    assert [int(v) for v in matvec_mod(M, x, p)] == want
+
+
+@pytest.mark.parametrize("algo", ["plain", "int8", "fp64"])
+def test_gemm_cfg1_512_uint32(gpu_ctx, algo):
+    """The reference's CPU-runnable case: C <- A*B + C at 512^3 over
+    Modular<uint32>, p = 65521, on its seeded inputs, bit-exact against the
+    oracle's ZpModule<uint32> product (level3-generic.tcc:285-322)."""
+    p = 65521
+    A = synth.zp_matrix(synth.seed_for(1, 0), 512, 512, p)
+    B = synth.zp_matrix(synth.seed_for(1, 1), 512, 512, p)
+    C = synth.zp_matrix(synth.seed_for(1, 2), 512, 512, p)
+    got = run_gemm(gpu_ctx, p, "uint32", algo, 1, A, B, 1, C)
+    assert np.array_equal(got, orc.zp_gemm(p, 1, A, B, 1, C))
 
 
 def test_gemm_int8_full_size_8192(gpu_ctx):
@@ -181,6 +222,27 @@ def test_gemm_fp64_large_prime_2048(gpu_ctx):
     C = C0.copy()
     la.BLAS3.gemm(gpu_ctx, 1, A, B, 1, C, algo=la.ALGO_FP64)
     _freivalds(A, B, C0, C, p, 1, 1, seed=6)
+    Cs = C0[:256].copy()
+    la.BLAS3.gemm(gpu_ctx, 1, A[:256], B, 1, Cs, algo=la.ALGO_PLAIN)
+    assert np.array_equal(Cs, C[:256])
+
+
+@pytest.mark.timeout(600)
+def test_gemm_fp64_full_size_8192(gpu_ctx):
+    """The FP64-carrier leg at full size: 8192^3, p = 67108859, uint32 storage.
+    Freivalds on the whole product, and a 256-row slice from the plain kernel
+    compared for equality."""
+    p, n = 67108859, 8192
+    F = la.Modular(p, "uint32")
+    gpu_ctx.F = F
+    A = synth.zp_matrix(synth.seed_for(2, 3), n, n, p)
+    B = synth.zp_matrix(synth.seed_for(2, 4), n, n, p)
+    C0 = synth.zp_matrix(synth.seed_for(2, 5), n, n, p)
+    C = C0.copy()
+    print("fp64 8192^3 product", flush=True)
+    la.BLAS3.gemm(gpu_ctx, 1, A, B, 1, C, algo=la.ALGO_FP64)
+    print("freivalds", flush=True)
+    _freivalds(A, B, C0, C, p, 1, 1, seed=7)
     Cs = C0[:256].copy()
     la.BLAS3.gemm(gpu_ctx, 1, A[:256], B, 1, Cs, algo=la.ALGO_PLAIN)
     assert np.array_equal(Cs, C[:256])

@@ -117,6 +117,41 @@ def test_gf2_gemm_full_size_16384(gpu_ctx):
         assert np.array_equal(_gf2_matvec(got, n, x), _gf2_matvec(A, n, _gf2_matvec(B, n, x)))
 
 
+def gf2_matvec_packed(W, v_words):
+    """(W @ v) mod 2 for packed rows W and a packed bit vector v: parity of the
+    popcount of each row AND v."""
+    return (np.bitwise_count(W & v_words[None, :]).sum(axis=1, dtype=np.int64) & 1).astype(np.uint8)
+
+
+def check_gf2_rref_all_rows(M, R, piv, rank, n, seed, chunk=2048):
+    """Every row of R in standard RREF (rows past the rank zero, row t's lowest
+    set bit at piv[t], pivot columns the identity on the first rank rows), and
+    every row of M equals M[:, piv] R: M v == M[:, piv] (R v) over GF(2) at
+    random bit vectors v."""
+    piv = piv.astype(np.int64)
+    assert rank == piv.size and (rank == 0 or np.all(np.diff(piv) > 0))
+    assert not R[rank:].any()
+    for s0 in range(0, rank, chunk):
+        bits = bf.gf2_unpack(R[s0:min(rank, s0 + chunk)], n)
+        t = np.arange(s0, s0 + bits.shape[0])
+        assert np.array_equal(bits.argmax(axis=1), piv[t]), f"row heads near row {s0}"
+        sub = bits[:, piv]
+        assert np.count_nonzero(sub) == bits.shape[0] and np.all(sub[np.arange(bits.shape[0]), t] == 1)
+    words = (n + 63) // 64
+    for it in range(2):
+        v = synth.raw_draws(seed, it * words, words)
+        if n % 64:
+            v[-1] &= np.uint64((1 << (n % 64)) - 1)
+        lhs = gf2_matvec_packed(M, v)
+        Rv = gf2_matvec_packed(R[:rank], v)
+        # M[:, piv] @ Rv: pack Rv at the pivot columns and take M's parity against it
+        w_bits = np.zeros((1, n), dtype=np.uint8)
+        w_bits[0, piv] = Rv
+        w = bf.gf2_pack(w_bits)[0]
+        rhs = gf2_matvec_packed(M, w)
+        assert np.array_equal(lhs, rhs), "a row of M is not M[:, piv] R"
+
+
 def test_gf2_rref_rank12000_16384(gpu_ctx):
     n, r = 16384, 12000
     X = synth.gf2_matrix(synth.seed_for(3, 2), n, r)
@@ -124,20 +159,4 @@ def test_gf2_rref_rank12000_16384(gpu_ctx):
     M = gpu_gemm(gpu_ctx, n, n, r, 1, X, Y, 0, np.zeros((n, n // 64), dtype=np.uint64))
     R, rank, piv = gpu_rref(gpu_ctx, n, n, M)
     assert rank == r
-    assert not R[rank:].any()
-    assert np.all(np.diff(piv.astype(np.int64)) > 0)
-    # pivot columns hold the identity on the first rank rows (sampled rows)
-    for i in range(0, rank, 499):
-        bits = bf.gf2_unpack(R[i:i + 1], n)[0]
-        assert bits[piv[i]] == 1 and bits[:piv[i]].sum() == 0
-        assert bits[piv].sum() == 1
-    # sampled rows of M lie in the row space: row = sum of R rows at its pivot bits
-    Rb = None
-    for i in range(0, n, 1777):
-        row = bf.gf2_unpack(M[i:i + 1], n)[0]
-        sel = np.nonzero(row[piv])[0]
-        acc = np.zeros(n // 64, dtype=np.uint64)
-        for t in sel:
-            acc ^= R[t]
-        assert np.array_equal(acc, M[i])
-    del Rb
+    check_gf2_rref_all_rows(M, R, piv, rank, n, seed=13)

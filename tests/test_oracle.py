@@ -226,3 +226,32 @@ def test_inverse():
     for p in (3, 101, 65521, 4294967291):
         for y in (1, 2, p - 1):
             assert (orc.inv(p, y) * y) % p == 1
+
+
+def test_full_size_checks_accept_rref_and_reject_planted_errors():
+    """The all-row checks that the full-size GPU tests apply (Z/p FL and the
+    GF(2) echelon form) accept the oracle's RREF and reject an R with one
+    entry changed."""
+    from lela_amd import dense_to_csr
+    from test_fl_gpu import check_rref_all_rows
+    from test_gf2_gpu import check_gf2_rref_all_rows
+    p = 65521
+    X = f4_matrix(5, 200, 260, p)
+    rc, R, rank, piv, _ = orc.zp_echelonize(p, X, reduced=True)
+    csr = dense_to_csr(X)
+    check_rref_all_rows(R.copy(), piv, rank, csr, X.shape, p, seed=3)
+    free = [j for j in range(X.shape[1]) if j not in set(piv.tolist())]
+    bad = R.copy()
+    bad[7, free[5]] = (int(bad[7, free[5]]) + 1) % p
+    with pytest.raises(AssertionError):
+        check_rref_all_rows(bad, piv, rank, csr, X.shape, p, seed=3)
+    n = 300
+    Xg, Yg = synth.gf2_matrix(1, n, 200), synth.gf2_matrix(2, 200, n)
+    M = orc.gf2_gemm(n, n, 200, 1, Xg, Yg, 0, np.zeros((n, (n + 63) // 64), dtype=np.uint64))
+    Rg, rk, pv = orc.gf2_echelonize(n, n, M)
+    check_gf2_rref_all_rows(M, Rg, pv, rk, n, seed=4)
+    freeg = [j for j in range(n) if j not in set(pv.tolist())]
+    badg = Rg.copy()
+    badg[3, freeg[2] // 64] ^= np.uint64(1 << (freeg[2] % 64))
+    with pytest.raises(AssertionError):
+        check_gf2_rref_all_rows(M, badg, pv, rk, n, seed=4)
