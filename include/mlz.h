@@ -127,6 +127,25 @@ int mlz_zp_gemm(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t n, siz
 		uint64_t a, const void *A, size_t lda, const void *B, size_t ldb,
 		uint64_t b, void *C, size_t ldc, int algo);
 
+/* The same product with no device allocation: for callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation).  No counterpart in LELA.  A, B and C are device
+ * pointers whatever the context's pointer mode; the call queues on the
+ * context's stream and returns.  The INT8 and FP64 families pack their
+ * operands into a static device array of MLZ_NOALLOC_SCRATCH_MB MiB that each
+ * device holds until the process ends (the plain family needs none); a call
+ * whose packed operands exceed it returns MLZ_E_ARG before any launch (INT8:
+ * (m' + n') * k' * 2 bytes with m', n' padded to the 128- or 256-row tile and
+ * k' = min(k, 65504) rounded up to 64; FP64: (m' * 32 + n' * 64) * ceil(k'/4)
+ * bytes, m' = m rounded to 64, n' = n to 128, k' = min(k, the exact FP64
+ * k-chunk, 65552 at p = 67108859)).  Every call on one device shares
+ * that array: calls that overlap on a device (two contexts or threads) race.
+ * No profiling events are recorded for it. */
+#define MLZ_NOALLOC_SCRATCH_MB 64
+int mlz_zp_gemm_noalloc(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t n, size_t k,
+			uint64_t a, const void *A, size_t lda, const void *B, size_t ldb,
+			uint64_t b, void *C, size_t ldc, int algo);
+
 /* B <- a * A^-1 * B with A n x n triangular (upper != 0: upper), B n x nrhs.
  * Replaces BLAS3::trsm (lela/blas/level3.h:120-122) ->
  * _trsm<Ring, GenericModule::Tag>::trsm_impl (lela/blas/level3-generic.tcc:421-498).

@@ -20,6 +20,7 @@ MLZ_E_NOT_INVERTIBLE = 3
 MLZ_E_WRONG_FORM = 4
 MLZ_E_NOMEM = 5
 MLZ_E_UNSUPPORTED = 6
+NOALLOC_SCRATCH_MB = 64
 
 _c_size = ctypes.c_size_t
 _c_u64 = ctypes.c_uint64
@@ -50,6 +51,8 @@ SIGNATURES = {
     "mlz_timer_stop": (_c_int, [_ctx, ctypes.POINTER(ctypes.c_double)]),
     "mlz_zp_gemm": (_c_int, [_ctx, _c_int, _c_u64, _c_size, _c_size, _c_size, _c_u64,
                              _vp, _c_size, _vp, _c_size, _c_u64, _vp, _c_size, _c_int]),
+    "mlz_zp_gemm_noalloc": (_c_int, [_ctx, _c_int, _c_u64, _c_size, _c_size, _c_size, _c_u64,
+                                     _vp, _c_size, _vp, _c_size, _c_u64, _vp, _c_size, _c_int]),
     "mlz_zp_trsm": (_c_int, [_ctx, _c_int, _c_u64, _c_int, _c_int, _c_size, _c_size,
                              _c_u64, _vp, _c_size, _vp, _c_size, ctypes.POINTER(_c_size), _c_int]),
     "mlz_zp_rref": (_c_int, [_ctx, _c_int, _c_u64, _c_size, _c_size, _vp, _c_size,

@@ -283,6 +283,13 @@ struct mlz_ctx {
 	int rref_fast_threads;   // threads of the echelon form's fused pivot-search kernel (256, 512, 1024)
 	uint32_t invtab_p;       // modulus whose inverse table WS_INVTAB holds (0 = none)
 	struct mlz_tuner *tuner;   // autotuner table and cost-model weights (batch.hip), lazily created
+	// No-allocation calls (mlz_zp_gemm_noalloc): while `noalloc` is set, gemm
+	// operand packing takes its buffers from `fixed_ws` (this device's copy of
+	// the library's static scratch array) instead of the workspace slots, and
+	// no profiling events are created.
+	bool noalloc;
+	void *fixed_ws;
+	size_t fixed_ws_bytes;
 };
 
 namespace mlz {
@@ -355,6 +362,9 @@ int zp_rank_update_i8_packed(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, int K
 			     const int8_t *Wp, uint32_t *C, size_t ldc);
 int check_modulus(mlz_ctx *ctx, int elem, uint64_t p);
 void tuner_free(mlz_ctx *ctx);
+// Looks up (once per device, with the device current) the address of the
+// static scratch array that mlz_zp_gemm_noalloc uses; called by mlz_ctx_create.
+int noalloc_scratch_init(mlz_ctx *ctx);
 
 // Host <-> device staging of a strided matrix (rows x cols elements of esize
 // bytes, leading dimension ld elements) into a dense device buffer with

@@ -70,7 +70,7 @@ void prof_begin(mlz_ctx *ctx, const char *name, PendingEv *ev)
 {
 	ev->name = name;
 	ev->e0 = ev->e1 = nullptr;
-	if (!ctx->prof_on)
+	if (!ctx->prof_on || ctx->noalloc)
 		return;
 	ev->e0 = get_event(ctx);
 	ev->e1 = get_event(ctx);
@@ -80,7 +80,7 @@ void prof_begin(mlz_ctx *ctx, const char *name, PendingEv *ev)
 
 void prof_end(mlz_ctx *ctx, PendingEv *ev)
 {
-	if (!ctx->prof_on || !ev->e0 || !ev->e1)
+	if (!ctx->prof_on || ctx->noalloc || !ev->e0 || !ev->e1)
 		return;
 	(void) hipEventRecord(ev->e1, ctx->stream);
 	ctx->pending.push_back(*ev);
@@ -169,6 +169,9 @@ int mlz_ctx_create(mlz_ctx **out, int device, const char *config)
 	ctx->rref_fast_threads = 1024;
 	ctx->invtab_p = 0;
 	ctx->tuner = nullptr;
+	ctx->noalloc = false;
+	ctx->fixed_ws = nullptr;
+	ctx->fixed_ws_bytes = 0;
 	// Default priority only: streams of different priorities are not used.
 	if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
 		delete ctx;
@@ -182,6 +185,9 @@ int mlz_ctx_create(mlz_ctx **out, int device, const char *config)
 	ctx->num_cus = 256;
 	if (hipGetDeviceProperties(&prop, device) == hipSuccess)
 		ctx->num_cus = prop.multiProcessorCount;
+	// A failed lookup leaves fixed_ws null: only the no-allocation entry
+	// point needs it, and it reports the failure (the message is kept).
+	(void) noalloc_scratch_init(ctx);
 	*out = ctx;
 	return MLZ_OK;
 }

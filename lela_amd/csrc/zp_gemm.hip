@@ -24,6 +24,8 @@
 // c <- b*c + a*d mod p, with d = (A*B)_{ij} mod p.
 #include "batch.hpp"
 
+#include <mutex>
+
 namespace mlz {
 
 // ===========================================================================
@@ -455,6 +457,66 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_async_kernel(const 
 	i8_epilogue<E, T, WTM, WTN>(acc, C, ldc, m, n, a, b, mp, tm, tn, wr, wc, lane);
 }
 
+// ===========================================================================
+// No-allocation scratch.  mlz_zp_gemm_noalloc packs its operands into this
+// file-scope device array instead of the context's workspace slots, so the
+// call makes no hipMalloc.  Each device holds its own copy for the life of the
+// process; its address on a device comes from hipGetSymbolAddress with that
+// device current (the array's name in host code is only a host-side stand-in).
+// Every call on a device shares the array: calls that overlap on one device
+// (two contexts, two threads) race on it.
+// ===========================================================================
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t) MLZ_NOALLOC_SCRATCH_MB << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+static bool g_scratch_done[NOALLOC_MAX_DEVICES];
+
+int noalloc_scratch_init(mlz_ctx *ctx)
+{
+	if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES)
+		return set_error(ctx, MLZ_E_UNSUPPORTED, "no-allocation scratch: device %d beyond %d", ctx->device,
+				 NOALLOC_MAX_DEVICES);
+	std::lock_guard<std::mutex> lock(g_scratch_mutex);
+	if (!g_scratch_done[ctx->device]) {
+		void *ptr = nullptr;
+		// The caller (mlz_ctx_create) has made ctx->device current.
+		hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+		if (e != hipSuccess) {
+			(void) hipGetLastError();
+			return set_error(ctx, MLZ_E_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", ctx->device,
+					 hipGetErrorString(e));
+		}
+		g_scratch_ptr[ctx->device] = ptr;
+		g_scratch_done[ctx->device] = true;
+	}
+	ctx->fixed_ws = g_scratch_ptr[ctx->device];
+	ctx->fixed_ws_bytes = NOALLOC_SCRATCH_BYTES;
+	return MLZ_OK;
+}
+
+// Buffers for the packed A and B operands of one gemm launch: the workspace
+// slots, or during a no-allocation call the scratch array (A at its start, B
+// after it on a 256-byte boundary).  Both pack kernels write every byte they
+// hand to the gemm kernel (padding included), so nothing needs clearing.
+static int pack_buffers(mlz_ctx *ctx, size_t abytes, size_t bbytes, void **ap, void **bp)
+{
+	if (!ctx->noalloc) {
+		MLZ_TRY(ws_get(ctx, WS_PACK_A, abytes, ap));
+		return ws_get(ctx, WS_PACK_B, bbytes, bp);
+	}
+	const size_t boff = round_up(abytes, 256);
+	if (!ctx->fixed_ws || boff + bbytes > ctx->fixed_ws_bytes)
+		return set_error(ctx, MLZ_E_ARG,
+				 "no-allocation gemm needs %zu bytes of packed operands, beyond the %zu-byte scratch",
+				 boff + bbytes, ctx->fixed_ws_bytes);
+	*ap = ctx->fixed_ws;
+	*bp = (uint8_t *) ctx->fixed_ws + boff;
+	return MLZ_OK;
+}
+
 // Tile variants of the INT8 kernel (index = ctx option "gemm.i8.variant"):
 //   1: 128 x 128, 4 waves, 64 k per step
 //   2: 128 x 256, 4 waves of 64 x 128, 32 k per step
@@ -482,8 +544,7 @@ static int gemm_i8_launch_t(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t
 		const size_t kpad = round_up(kc, 64);
 		const int KB = (int) (kpad / 32);
 		void *ap = nullptr, *bp = nullptr;
-		MLZ_TRY(ws_get(ctx, WS_PACK_A, mpad * kpad * 2, &ap));
-		MLZ_TRY(ws_get(ctx, WS_PACK_B, npad * kpad * 2, &bp));
+		MLZ_TRY(pack_buffers(ctx, mpad * kpad * 2, npad * kpad * 2, &ap, &bp));
 		{
 			ProfScope ps(ctx, "pack_i8_A");
 			dim3 g(mpad / 64, kpad / 64);
@@ -974,8 +1035,8 @@ static int gemm_f64_launch(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t 
 		const size_t kc = std::min(kchunk, k - k0);
 		const int KB4 = (int) ceil_div(kc, 4);
 		void *ap = nullptr, *bp = nullptr;
-		MLZ_TRY(ws_get(ctx, WS_PACK_A, mpad * KB4 * 4 * sizeof(double), &ap));
-		MLZ_TRY(ws_get(ctx, WS_PACK_B, npad * KB4 * 4 * 2 * sizeof(double), &bp));
+		MLZ_TRY(pack_buffers(ctx, mpad * KB4 * 4 * sizeof(double), npad * KB4 * 4 * 2 * sizeof(double),
+				     &ap, &bp));
 		{
 			ProfScope ps(ctx, "pack_f64_A");
 			int total = (int) (mpad / 16 * KB4 * 64);
@@ -1401,4 +1462,27 @@ extern "C" int mlz_zp_gemm(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, si
 	MLZ_TRY(stage_in(ctx, WS_STAGE_C, C, m, n, ldc, es, &dC));
 	MLZ_TRY(zp_gemm_dev(ctx, elem, p, m, n, k, a, dA, k, dB, n, b, dC, n, algo));
 	return stage_out(ctx, C, dC, m, n, ldc, es);
+}
+
+extern "C" int mlz_zp_gemm_noalloc(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t n, size_t k,
+				   uint64_t a, const void *A, size_t lda, const void *B, size_t ldb, uint64_t b,
+				   void *C, size_t ldc, int algo)
+{
+	if (!ctx)
+		return MLZ_E_ARG;
+	MLZ_TRY(check_modulus(ctx, elem, p));
+	if ((m && k && !A) || (k && n && !B) || (m && n && !C))
+		return set_error(ctx, MLZ_E_ARG, "null matrix pointer");
+	if ((m && k && lda < k) || (k && n && ldb < n) || (m && n && ldc < n))
+		return set_error(ctx, MLZ_E_ARG, "leading dimension smaller than row length");
+	if (algo < MLZ_ALGO_AUTO || algo > MLZ_ALGO_FP64)
+		return set_error(ctx, MLZ_E_ARG, "unknown gemm algo %d", algo);
+	if (!ctx->fixed_ws)
+		return set_error(ctx, MLZ_E_HIP, "no-allocation scratch has no device address on device %d",
+				 ctx->device);
+	MLZ_HIP(ctx, hipSetDevice(ctx->device));
+	ctx->noalloc = true;
+	const int rc = zp_gemm_dev(ctx, elem, p, m, n, k, a, A, lda, B, ldb, b, C, ldc, algo);
+	ctx->noalloc = false;
+	return rc;
 }
