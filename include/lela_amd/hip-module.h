@@ -10,51 +10,102 @@
  * specialisations for that Tag that dispatch on Matrix::StorageType and
  * forward anything they do not handle to Tag::Parent (the precedents are
  * lela/blas/level3-modular.h:29-68 and lela/blas/level3-m4ri.h:89-128).
- * EchelonForm is specialised as a whole class, like the GF2 + M4RI
- * specialisation in lela/solutions/echelon-form-gf2.h:37-182.
+ * EchelonForm and FaugereLachartre are specialised as whole classes, like
+ * the GF2 + M4RI specialisation in lela/solutions/echelon-form-gf2.h:37-182.
  *
- * This header is compiled inside a LELA build (it includes LELA's headers)
- * and links against lela_amd/liblela_amd.so.  Return codes of the C ABI are
- * turned back into LELA's exceptions.
+ * What runs where:
+ *   BLAS3::gemm, BLAS3::trsm on three dense Z/p matrices   -> GPU
+ *   BLAS3::gemm on three little-endian dense GF(2) matrices -> GPU
+ *   EchelonForm::echelonize (A, true) of a dense matrix      -> GPU (mlz_zp_rref / mlz_gf2_rref)
+ *   EchelonForm::echelonize (A, *, METHOD_FAUGERE_LACHARTRE) -> GPU FL (dense A, as LELA's dense
+ *       overload, always the reduced form: lela/solutions/echelon-form.h:133-139)
+ *   FaugereLachartre::echelonize (R, X, rank, det, true)     -> GPU FL (any row type)
+ *   the non-reduced forms and every other storage           -> LELA's own CPU code
+ *
+ * Determinant of FaugereLachartre: LELA multiplies det by the heads of the
+ * scanned rows of X and then by the heads of its (non-reduced) echelon form
+ * of D - C A^-1 B (lela/algorithms/faugere-lachartre.tcc:119, :831-833).  The
+ * GPU call returns the first factor; the second depends on LELA's pivot
+ * order, so the GPU route returns the exact LELA value only when the rank of
+ * X equals the number of its pivot rows (every row scanned is a pivot row,
+ * D - C A^-1 B = 0); otherwise it runs LELA's CPU FaugereLachartre when the
+ * caller asked for det (det_wanted (), the default), or returns the first
+ * factor alone when the caller has said it does not use det.
+ *
+ * The header is compiled inside a LELA build (it includes LELA's headers,
+ * C++98) and links against lela_amd/liblela_amd.so.  Return codes of the C
+ * ABI are turned back into LELA's exceptions.
  */
 #ifndef LELA_AMD_HIP_MODULE_H
 #define LELA_AMD_HIP_MODULE_H
 
+#include <algorithm>
+#include <iostream>      // LELA's bit-iterator.h uses the stream operators of bool without including it
 #include <map>
 #include <vector>
 
+#include "lela/algorithms/elimination.h"
 #include "lela/algorithms/faugere-lachartre.h"   // WrongMatrixForm (faugere-lachartre.tcc:32-45)
 #include "lela/blas/context.h"
 #include "lela/blas/level3-ll.h"
+#include "lela/blas/level3.h"
 #include "lela/matrix/dense.h"
 #include "lela/ring/gf2.h"
 #include "lela/ring/modular.h"
 #include "lela/solutions/echelon-form.h"
 #include "lela/util/error.h"
+#include "lela/vector/traits.h"
 
 #include "../mlz.h"
 
 namespace LELA
 {
 
-/** One HIP context (stream + device workspace) per LELA Context. */
+/** One HIP context (stream + device workspace) shared by a LELA Context and
+ * its copies: LELA's algorithm classes copy the Context they are given
+ * (echelon-form.h:33, faugere-lachartre.h:41-42), and those copies run on
+ * the same thread, so they share one stream. */
 class HIPDevice
 {
-	mlz_ctx *_ctx;
+	struct Shared { mlz_ctx *ctx; long refs; };
+	Shared *_s;
 
-	HIPDevice (const HIPDevice &);
-	HIPDevice &operator = (const HIPDevice &);
-
-public:
-	HIPDevice () : _ctx (0)
+	void release ()
 	{
-		if (mlz_ctx_create (&_ctx, 0, 0) != MLZ_OK)
-			throw LELAError ("HIPModule: no usable MI355X device");
+		if (_s && --_s->refs == 0) {
+			if (_s->ctx)
+				mlz_ctx_destroy (_s->ctx);
+			delete _s;
+		}
+		_s = 0;
 	}
 
-	~HIPDevice () { if (_ctx) mlz_ctx_destroy (_ctx); }
+public:
+	HIPDevice () : _s (new Shared)
+	{
+		_s->ctx = 0;
+		_s->refs = 1;
+		if (mlz_ctx_create (&_s->ctx, 0, 0) != MLZ_OK) {
+			release ();
+			throw LELAError ("HIPModule: no usable MI355X device");
+		}
+	}
 
-	mlz_ctx *ctx () const { return _ctx; }
+	HIPDevice (const HIPDevice &d) : _s (d._s) { ++_s->refs; }
+
+	HIPDevice &operator = (const HIPDevice &d)
+	{
+		if (d._s != _s) {
+			release ();
+			_s = d._s;
+			++_s->refs;
+		}
+		return *this;
+	}
+
+	~HIPDevice () { release (); }
+
+	mlz_ctx *ctx () const { return _s->ctx; }
 
 	/** Maps a return code to LELA's exceptions (idx: trsm diagonal index). */
 	void check (int rc, size_t idx = 0) const
@@ -63,7 +114,7 @@ public:
 			return;
 		if (rc == MLZ_E_NOT_INVERTIBLE)
 			throw DiagonalEntryNotInvertible (idx);
-		throw LELAError (mlz_last_error (_ctx));
+		throw LELAError (mlz_last_error (_s->ctx));
 	}
 };
 
@@ -103,6 +154,26 @@ inline uint64_t hip_scalar (const Modular<Element> &F, const Element &x)
 	if (x < 0)                                  // Modular<double> symmetric range
 		return (uint64_t) (x + F._modulus);
 	return (uint64_t) x;
+}
+
+/** First word and word stride (uint64 words between rows) of a dense 0-1 matrix. */
+template <class Matrix>
+inline uint64 *hip_words (Matrix &A)
+	{ return &*A.rowBegin ()->word_begin (); }
+
+template <class Matrix>
+inline const uint64 *hip_words (const Matrix &A)
+	{ return &*A.rowBegin ()->word_begin (); }
+
+template <class Matrix>
+inline size_t hip_row_words (const Matrix &A)
+{
+	if (A.rowdim () < 2)
+		return (A.coldim () + 63) / 64;
+	typename Matrix::ConstRowIterator r = A.rowBegin ();
+	const uint64 *w0 = &*r->word_begin ();
+	++r;
+	return (size_t) (&*r->word_begin () - w0);
 }
 
 namespace BLAS3
@@ -181,25 +252,28 @@ public:
 				    typename Matrix2::StorageType ()); }
 };
 
-/** Word stride (uint64 words between rows) of a Dense01Matrix. */
-template <class Matrix>
-inline size_t hip_row_words (const Matrix &A)
-{
-	if (A.rowdim () < 2)
-		return (A.coldim () + 63) / 64;
-	typename Matrix::ConstRowIterator r = A.rowBegin ();
-	const uint64 *w0 = &*r->word_begin ();
-	++r;
-	return (size_t) (&*r->word_begin () - w0);
-}
-
 template <>
 class _gemm<GF2, HIPModule<GF2>::Tag>
 {
 	typedef HIPModule<GF2>::Tag Tag;
 
+	template <class Modules, class Matrix1, class Matrix2, class Matrix3>
+	static Matrix3 &gpu (const GF2 &, Modules &M, bool a, const Matrix1 &A, const Matrix2 &B, bool b, Matrix3 &C)
+	{
+		lela_check (A.coldim () == B.rowdim ());
+		lela_check (A.rowdim () == C.rowdim ());
+		lela_check (B.coldim () == C.coldim ());
+		if (C.rowdim () == 0 || C.coldim () == 0)
+			return C;
+		M.dev.check (mlz_gf2_gemm (M.dev.ctx (), C.rowdim (), C.coldim (), A.coldim (), a,
+					   A.coldim () && A.rowdim () ? hip_words (A) : 0, hip_row_words (A),
+					   A.coldim () && B.coldim () ? hip_words (B) : 0, hip_row_words (B),
+					   b, hip_words (C), hip_row_words (C)));
+		return C;
+	}
+
 public:
-	/* Anything but three little-endian Dense01Matrix: the parent module. */
+	/* Anything but three little-endian dense 0-1 matrices: the parent module. */
 	template <class Modules, class Matrix1, class Matrix2, class Matrix3>
 	static Matrix3 &op (const GF2 &F, Modules &M, bool a, const Matrix1 &A, const Matrix2 &B, bool b, Matrix3 &C)
 		{ return _gemm<GF2, Tag::Parent>::op (F, M, a, A, B, b, C); }
@@ -208,107 +282,311 @@ public:
 	static Dense01Matrix<It, CIt, LittleEndian<uint64> > &
 	op (const GF2 &F, Modules &M, bool a, const Dense01Matrix<It, CIt, LittleEndian<uint64> > &A,
 	    const Dense01Matrix<It, CIt, LittleEndian<uint64> > &B, bool b, Dense01Matrix<It, CIt, LittleEndian<uint64> > &C)
-	{
-		lela_check (A.coldim () == B.rowdim ());
-		lela_check (A.rowdim () == C.rowdim ());
-		lela_check (B.coldim () == C.coldim ());
-		if (C.rowdim () == 0 || C.coldim () == 0)
-			return C;
-		M.dev.check (mlz_gf2_gemm (M.dev.ctx (), C.rowdim (), C.coldim (), A.coldim (), a,
-					   A.coldim () ? &*A.rowBegin ()->word_begin () : 0, hip_row_words (A),
-					   A.coldim () ? &*B.rowBegin ()->word_begin () : 0, hip_row_words (B),
-					   b, &*C.rowBegin ()->word_begin (), hip_row_words (C)));
-		return C;
-	}
+		{ return gpu (F, M, a, A, B, b, C); }
+
+#ifndef __LELA_HAVE_M4RI
+	/* DenseMatrix<bool> is a little-endian Dense01Matrix without M4RI
+	 * (dense-zero-one.h:369-394, bit-iterator.h:300-312); the exact-match
+	 * overload keeps it from the generic template above. */
+	template <class Modules>
+	static DenseMatrix<bool> &op (const GF2 &F, Modules &M, bool a, const DenseMatrix<bool> &A,
+				      const DenseMatrix<bool> &B, bool b, DenseMatrix<bool> &C)
+		{ return gpu (F, M, a, A, B, b, C); }
+#endif
 };
 
 } // namespace BLAS3
 
+/** Appends the nonzero entries of a row to (col, val): sparse rows as they
+ * are, dense rows scanned for nonzeros. */
+template <class Element, class Row>
+inline void hip_append_row (const Modular<Element> &, const Row &row, std::vector<uint32_t> &col,
+			    std::vector<Element> &val, VectorRepresentationTypes::Sparse)
+{
+	for (typename Row::const_iterator j = row.begin (); j != row.end (); ++j) {
+		col.push_back ((uint32_t) j->first);
+		val.push_back (j->second);
+	}
+}
+
+template <class Element, class Row>
+inline void hip_append_row (const Modular<Element> &F, const Row &row, std::vector<uint32_t> &col,
+			    std::vector<Element> &val, VectorRepresentationTypes::Dense)
+{
+	uint32_t idx = 0;
+	for (typename Row::const_iterator j = row.begin (); j != row.end (); ++j, ++idx)
+		if (!F.isZero (*j)) {
+			col.push_back (idx);
+			val.push_back (*j);
+		}
+}
+
 /** FaugereLachartre over Z/p on the HIP module
  * (lela/algorithms/faugere-lachartre.h:40-81): the F4 matrix X goes to the
- * device as CSR, R comes back dense and is written into R row by row. */
+ * device as CSR (any row type), R comes back dense and is copied into R.
+ * The non-reduced and only_D forms run LELA's CPU FaugereLachartre. */
 template <class Element>
 class FaugereLachartre<Modular<Element>, HIPModule<Modular<Element> > >
 {
 	Context<Modular<Element>, HIPModule<Modular<Element> > > &ctx;
+	bool _det_wanted;
+
+	template <class Matrix>
+	void cpu (Matrix &R, const Matrix &X, size_t &rank, Element &det, bool reduced, bool only_D)
+	{
+		Context<Modular<Element>, ZpModule<Element> > cctx (ctx.F);
+		FaugereLachartre<Modular<Element>, ZpModule<Element> > FL (cctx);
+		FL.echelonize (R, X, rank, det, reduced, only_D);
+	}
 
 public:
-	FaugereLachartre (Context<Modular<Element>, HIPModule<Modular<Element> > > &_ctx) : ctx (_ctx) {}
+	FaugereLachartre (Context<Modular<Element>, HIPModule<Modular<Element> > > &_ctx)
+		: ctx (_ctx), _det_wanted (true) {}
 
-	/* Only the reduced form (reduced = true, only_D = false) is offered by
-	 * the device path; the other forms throw. */
+	/** Whether the caller uses det.  If not, a matrix whose D part has
+	 * positive rank stays on the GPU and det holds only the X-scan factor. */
+	void det_wanted (bool wanted) { _det_wanted = wanted; }
+
 	template <class Matrix>
 	void echelonize (Matrix &R, const Matrix &X, size_t &rank, Element &det, bool reduced = true, bool only_D = false)
 	{
-		if (!reduced || only_D)
-			throw LELAError ("HIPModule: FaugereLachartre runs the reduced form only");
+		if (!reduced || only_D) {
+			cpu (R, X, rank, det, reduced, only_D);
+			return;
+		}
 
 		std::vector<uint64_t> row_ptr (1, 0);
 		std::vector<uint32_t> col;
 		std::vector<Element> val;
 		for (typename Matrix::ConstRowIterator i = X.rowBegin (); i != X.rowEnd (); ++i) {
-			for (typename Matrix::ConstRow::const_iterator j = i->begin (); j != i->end (); ++j) {
-				col.push_back ((uint32_t) j->first);
-				val.push_back (j->second);
-			}
+			hip_append_row (ctx.F, *i, col, val,
+					typename VectorTraits<Modular<Element>, typename Matrix::ConstRow>::RepresentationType ());
 			row_ptr.push_back (col.size ());
 		}
 
 		DenseMatrix<Element> Rd (X.rowdim (), X.coldim ());
+		std::vector<uint32_t> pivots (std::min (X.rowdim (), X.coldim ()) + 1);
 		uint64_t det_c = 1;
-		size_t bad_row = 0;
+		size_t bad_row = 0, r = 0;
 		const int rc = mlz_zp_fl_echelonize (ctx.M.dev.ctx (), HIPElement<Element>::value (),
 						     (uint64_t) ctx.F._modulus, X.rowdim (), X.coldim (), &row_ptr[0],
 						     col.empty () ? 0 : &col[0], val.empty () ? 0 : &val[0],
-						     &Rd[0][0], Rd.disp (), &rank, 0, &det_c, &bad_row, MLZ_ALGO_AUTO);
-		ctx.F.init (det, (long) det_c);
+						     X.rowdim () && X.coldim () ? &Rd[0][0] : 0, Rd.disp (), &r,
+						     &pivots[0], &det_c, &bad_row, MLZ_ALGO_AUTO);
 		if (rc == MLZ_E_WRONG_FORM)
 			throw WrongMatrixForm (bad_row);
-		if (rc == MLZ_E_NOT_INVERTIBLE)
-			return;                         // LELA leaves R as it was (:790-799)
+		if (rc == MLZ_E_NOT_INVERTIBLE) {        // LELA leaves R as it was (:790-799)
+			rank = r;
+			ctx.F.init (det, (long) det_c);
+			return;
+		}
 		ctx.M.dev.check (rc);
+
+		// The X scan's pivot rows are the first pivots; more pivots mean
+		// D - C A^-1 B had positive rank, and LELA's det then also has
+		// the heads of its echelon form of D.
+		size_t scanned = 0;
+		for (size_t i = 0; i + 1 < row_ptr.size () && row_ptr[i + 1] > row_ptr[i]; ++i)
+			if (i == 0 || col[row_ptr[i]] > col[row_ptr[i - 1]])
+				++scanned;
+		if (r > scanned && _det_wanted) {
+			cpu (R, X, rank, det, reduced, only_D);
+			return;
+		}
+		rank = r;
+		ctx.F.init (det, (long) det_c);
 		BLAS3::copy (ctx, Rd, R);
 	}
 };
 
-/** EchelonForm over Z/p on the HIP module: the reduced form of a dense matrix
- * runs on the GPU (mlz_zp_rref); the non-reduced forms, whose layout depends
- * on LELA's pivot order, run LELA's own Elimination on a ZpModule context. */
+/** EchelonForm over Z/p on the HIP module (lela/solutions/echelon-form.h:31-164).
+ * Dense reduced forms and the Faugere-Lachartre method run on the GPU; the
+ * non-reduced forms, whose layout depends on LELA's pivot order, run LELA's
+ * own Elimination on a ZpModule context (echelon-form.h:113-117,123-131 give
+ * the same result for both methods: tests/test-gauss-jordan.C:96-99). */
 template <class Element>
 class EchelonForm<Modular<Element>, HIPModule<Modular<Element> > >
 {
 	Context<Modular<Element>, HIPModule<Modular<Element> > > &_ctx;
 	Context<Modular<Element>, ZpModule<Element> > _cpu;
-	EchelonForm<Modular<Element>, ZpModule<Element> > _fallback;
 	std::map<const void *, size_t> _rank_table;
+
+	template <class Matrix>
+	size_t cpu_echelonize (Matrix &A, bool reduced)
+	{
+		Elimination<Modular<Element>, ZpModule<Element> > elim (_cpu);
+		typename Elimination<Modular<Element>, ZpModule<Element> >::Permutation P;
+		size_t rank = 0;
+		Element d;
+		if (reduced) {
+			DenseMatrix<Element> L;
+			elim.echelonize_reduced (A, L, P, rank, d, false);
+		} else
+			elim.echelonize (A, P, rank, d, false);
+		return rank;
+	}
+
+	template <class Matrix>
+	size_t fl_echelonize (Matrix &A, bool reduced)
+	{
+		FaugereLachartre<Modular<Element>, HIPModule<Modular<Element> > > FL (_ctx);
+		FL.det_wanted (false);                  // EchelonForm keeps only the rank
+		size_t rank = 0;
+		Element d;
+		FL.echelonize (A, A, rank, d, reduced);
+		return rank;
+	}
 
 public:
 	enum Method { METHOD_UNKNOWN, METHOD_STANDARD_GJ, METHOD_ASYMPTOTICALLY_FAST_GJ, METHOD_FAUGERE_LACHARTRE };
 
 	EchelonForm (Context<Modular<Element>, HIPModule<Modular<Element> > > &ctx)
-		: _ctx (ctx), _cpu (ctx.F), _fallback (_cpu) {}
+		: _ctx (ctx), _cpu (ctx.F) {}
 
-	DenseMatrix<Element> &echelonize (DenseMatrix<Element> &A, bool reduced = false,
-					  Method method = METHOD_ASYMPTOTICALLY_FAST_GJ)
+	/** Any matrix (sparse rows included): LELA's standard method on the CPU,
+	 * or Faugere-Lachartre with the caller's `reduced` (:86-94). */
+	template <class Matrix>
+	Matrix &echelonize (Matrix &A, bool reduced = false, Method method = METHOD_STANDARD_GJ)
 	{
-		if (!reduced || method == METHOD_FAUGERE_LACHARTRE) {
-			_fallback.echelonize (A, reduced,
-					      (typename EchelonForm<Modular<Element>, ZpModule<Element> >::Method) method);
-			_rank_table[&A] = _fallback.rank (A);
-			return A;
-		}
-		size_t rank = 0;
-		std::vector<uint32_t> pivots (std::min (A.rowdim (), A.coldim ()) + 1);
-		if (A.rowdim () && A.coldim ())
-			_ctx.M.dev.check (mlz_zp_rref (_ctx.M.dev.ctx (), HIPElement<Element>::value (),
-						       (uint64_t) _ctx.F._modulus, A.rowdim (), A.coldim (), &A[0][0],
-						       A.disp (), &rank, &pivots[0], MLZ_ALGO_AUTO));
+		size_t rank;
+		if (method == METHOD_STANDARD_GJ)
+			rank = cpu_echelonize (A, reduced);
+		else if (method == METHOD_FAUGERE_LACHARTRE)
+			rank = fl_echelonize (A, reduced);
+		else
+			throw LELAError ("Invalid method for choice of matrix");
 		_rank_table[&A] = rank;
 		return A;
 	}
 
+	/** Dense matrices (:103-154). */
+	DenseMatrix<Element> &echelonize (DenseMatrix<Element> &A, bool reduced = false,
+					  Method method = METHOD_ASYMPTOTICALLY_FAST_GJ)
+	{
+		size_t rank = 0;
+		switch (method) {
+		case METHOD_STANDARD_GJ:
+		case METHOD_ASYMPTOTICALLY_FAST_GJ:
+			if (!reduced) {
+				rank = cpu_echelonize (A, false);
+				break;
+			}
+			if (A.rowdim () && A.coldim ()) {
+				std::vector<uint32_t> pivots (std::min (A.rowdim (), A.coldim ()) + 1);
+				_ctx.M.dev.check (mlz_zp_rref (_ctx.M.dev.ctx (), HIPElement<Element>::value (),
+							       (uint64_t) _ctx.F._modulus, A.rowdim (), A.coldim (),
+							       &A[0][0], A.disp (), &rank, &pivots[0], MLZ_ALGO_AUTO));
+			}
+			break;
+
+		case METHOD_FAUGERE_LACHARTRE:
+			// LELA's dense overload runs the reduced FL whatever `reduced` says (:133-139)
+			rank = fl_echelonize (A, true);
+			break;
+
+		default:
+			throw LELAError ("Invalid method for choice of matrix");
+		}
+		_rank_table[&A] = rank;
+		return A;
+	}
+
+	/** Rank recorded for A by echelonize; 0 for a matrix never passed to it. */
 	template <class Matrix>
-	size_t rank (const Matrix &A) const { return _rank_table.find (&A)->second; }
+	size_t rank (const Matrix &A) const
+	{
+		std::map<const void *, size_t>::const_iterator i = _rank_table.find (&A);
+		return i == _rank_table.end () ? 0 : i->second;
+	}
+};
+
+/** EchelonForm over GF(2) on the HIP module.  The reduced form of a dense
+ * 0-1 matrix runs on the GPU (mlz_gf2_rref); the other forms run LELA's
+ * Elimination / FaugereLachartre on a GenericModule context.  The reference's
+ * own GF(2) specialisation (echelon-form-gf2.h:37-182) exists only with M4RI;
+ * without it EchelonForm<GF2> uses the generic class (echelon-form.h:31-164),
+ * whose dense overload runs GaussJordan (reduced: the same RREF). */
+template <>
+class EchelonForm<GF2, HIPModule<GF2> >
+{
+	Context<GF2, HIPModule<GF2> > &_ctx;
+	Context<GF2, GenericModule<GF2> > _cpu;
+	std::map<const void *, size_t> _rank_table;
+
+	template <class Matrix>
+	size_t cpu_echelonize (Matrix &A, bool reduced)
+	{
+		Elimination<GF2, GenericModule<GF2> > elim (_cpu);
+		Elimination<GF2, GenericModule<GF2> >::Permutation P;
+		size_t rank = 0;
+		bool d;
+		if (reduced) {
+			DenseMatrix<bool> L;
+			elim.echelonize_reduced (A, L, P, rank, d, false);
+		} else
+			elim.echelonize (A, P, rank, d, false);
+		return rank;
+	}
+
+	template <class Matrix>
+	size_t cpu_fl (Matrix &A, bool reduced)
+	{
+		FaugereLachartre<GF2, GenericModule<GF2> > FL (_cpu);
+		size_t rank = 0;
+		bool d;
+		FL.echelonize (A, A, rank, d, reduced);
+		return rank;
+	}
+
+public:
+	enum Method { METHOD_UNKNOWN, METHOD_STANDARD_GJ, METHOD_ASYMPTOTICALLY_FAST_GJ, METHOD_M4RI,
+		      METHOD_FAUGERE_LACHARTRE };
+
+	EchelonForm (Context<GF2, HIPModule<GF2> > &ctx) : _ctx (ctx), _cpu (ctx.F) {}
+
+	template <class Matrix>
+	Matrix &echelonize (Matrix &A, bool reduced = false, Method method = METHOD_STANDARD_GJ)
+	{
+		size_t rank;
+		if (method == METHOD_STANDARD_GJ)
+			rank = cpu_echelonize (A, reduced);
+		else if (method == METHOD_FAUGERE_LACHARTRE)
+			rank = cpu_fl (A, reduced);
+		else
+			throw LELAError ("Invalid method for choice of matrix");
+		_rank_table[&A] = rank;
+		return A;
+	}
+
+#ifndef __LELA_HAVE_M4RI
+	/** Dense 0-1 matrices: the reduced form on the GPU. */
+	DenseMatrix<bool> &echelonize (DenseMatrix<bool> &A, bool reduced = false,
+				       Method method = METHOD_ASYMPTOTICALLY_FAST_GJ)
+	{
+		size_t rank = 0;
+		if (method == METHOD_FAUGERE_LACHARTRE)
+			rank = cpu_fl (A, true);
+		else if (method != METHOD_STANDARD_GJ && method != METHOD_ASYMPTOTICALLY_FAST_GJ &&
+			 method != METHOD_M4RI)
+			throw LELAError ("Invalid method for choice of matrix");
+		else if (!reduced)
+			rank = cpu_echelonize (A, false);
+		else if (A.rowdim () && A.coldim ()) {
+			std::vector<uint32_t> pivots (std::min (A.rowdim (), A.coldim ()) + 1);
+			_ctx.M.dev.check (mlz_gf2_rref (_ctx.M.dev.ctx (), A.rowdim (), A.coldim (), hip_words (A),
+							hip_row_words (A), &rank, &pivots[0]));
+		}
+		_rank_table[&A] = rank;
+		return A;
+	}
+#endif
+
+	template <class Matrix>
+	size_t rank (const Matrix &A) const
+	{
+		std::map<const void *, size_t>::const_iterator i = _rank_table.find (&A);
+		return i == _rank_table.end () ? 0 : i->second;
+	}
 };
 
 } // namespace LELA
