@@ -240,7 +240,7 @@ inline size_t elem_size(int e)
 enum WsSlot {
 	WS_STAGE_A, WS_STAGE_B, WS_STAGE_C,      // host-pointer staging
 	WS_PACK_A, WS_PACK_B,                    // gemm operand packing
-	WS_TRSM_0, WS_TRSM_1, WS_TRSM_2,         // trsm temporaries
+	WS_TRSM_0, WS_TRSM_1, WS_TRSM_2, WS_TRSM_3,   // trsm temporaries
 	WS_RREF_0, WS_RREF_1, WS_RREF_2, WS_RREF_3, WS_RREF_4, WS_RREF_5,
 	WS_FL_0, WS_FL_1, WS_FL_2, WS_FL_3, WS_FL_4, WS_FL_5, WS_FL_6, WS_FL_7,
 	WS_GF2_0, WS_GF2_1, WS_GF2_2,
@@ -328,6 +328,26 @@ struct ProfScope {
 
 int check_launch(mlz_ctx *ctx, const char *what);
 
+// A batch of equally shaped products C_z <- a A_z B_z + b C_z (z < count):
+// operand w (0 = A, 1 = B, 2 = C) of item z sits at element offset
+// (z / inner) * outer[w] + (z % inner) * in[w] from the base pointer, so
+// blocks along a diagonal (two strides) are one batch.  count = 1 is the
+// ordinary single product.
+struct I8Batch {
+	int count, inner;
+	long long outer[3], in[3];
+	__host__ __device__ long long offset(int z, int w) const
+		{ return (long long) (z / inner) * outer[w] + (long long) (z % inner) * in[w]; }
+	static I8Batch single()
+	{
+		I8Batch b;
+		b.count = b.inner = 1;
+		for (int w = 0; w < 3; ++w)
+			b.outer[w] = b.in[w] = 0;
+		return b;
+	}
+};
+
 // Internal device-pointer entry points shared between the kernel files.
 // All pointers are device pointers; work is queued on ctx->stream.
 int zp_gemm_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, size_t k,
@@ -354,6 +374,12 @@ int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64
 int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size_t *rank_out,
 		 uint32_t *pivots_out);
 int pick_gemm_algo(uint64_t p, int elem, size_t m, size_t n, size_t k, int algo);
+// bt.count products of one shape on INT8 MFMA (p <= 2^16), operands at the
+// batch's offsets from A, B and C (element storage E, device pointers).
+// elemA: A's storage, elem's or MLZ_U32 (canonical residues).
+int zp_gemm_i8_batched_dev(mlz_ctx *ctx, int elem, int elemA, uint32_t p, const I8Batch &bt, size_t m, size_t n,
+			   size_t k, uint32_t a, const void *A, size_t lda, const void *B, size_t ldb, uint32_t b,
+			   void *C, size_t ldc);
 int zp_rank_update_i8_dev(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, const uint32_t *G, size_t ldg,
 			  const uint32_t *W, size_t ldw, uint32_t *C, size_t ldc);
 // The same update with G (m x 32*KB) and W (32*KB x n) already packed as INT8

@@ -148,9 +148,12 @@ __device__ __forceinline__ void i8_pack_tile(const typename Elem<E>::T *__restri
 template <int E, bool TRANS, bool VEC>
 __global__ __launch_bounds__(256) void pack_i8_kernel(const typename Elem<E>::T *__restrict__ src,
 						      size_t ld, int idim, int kdim, uint32_t p,
-						      int8_t *__restrict__ dst, int KB)
+						      int8_t *__restrict__ dst, int KB, I8Batch bt, size_t dstride)
 {
 	__shared__ uint32_t tile[64][65];
+	const int z = blockIdx.z;
+	src += bt.offset(z, 0);
+	dst += (size_t) z * dstride;
 	i8_pack_tile<E, TRANS, VEC>(src, ld, idim, kdim, p, dst, KB, blockIdx.x * 64, blockIdx.y * 64, tile);
 }
 
@@ -339,9 +342,13 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_kernel(const int8_t
 							   const int8_t *__restrict__ Bp, int KB,
 							   typename Elem<E>::T *__restrict__ C, size_t ldc,
 							   int m, int n, uint32_t a, uint32_t b, ModP mp,
-							   int tiles_m, int tiles_n)
+							   int tiles_m, int tiles_n, I8Batch bt, size_t sAp, size_t sBp)
 {
 	__shared__ __attribute__((aligned(16))) int8_t lds[2 * I8Tile<WTM, WTN, KK, WM, WN>::STAGE];
+	const int z = blockIdx.y;
+	Ap += (size_t) z * sAp;
+	Bp += (size_t) z * sBp;
+	C += bt.offset(z, 2);
 	int tm, tn;
 	tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
 	i8_tile<E, WTM, WTN, KK, WM, WN>(Ap, Bp, KB, C, ldc, m, n, a, b, mp, tm, tn, lds);
@@ -386,9 +393,15 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_async_kernel(const 
 								 const int8_t *__restrict__ Bp, int KB,
 								 typename Elem<E>::T *__restrict__ C, size_t ldc,
 								 int m, int n, uint32_t a, uint32_t b, ModP mp,
-								 int tiles_m, int tiles_n)
+								 int tiles_m, int tiles_n, I8Batch bt, size_t sAp, size_t sBp)
 {
 	typedef I8Tile<WTM, WTN, KK, WM, WN> T;
+	{
+		const int z = blockIdx.y;
+		Ap += (size_t) z * sAp;
+		Bp += (size_t) z * sBp;
+		C += bt.offset(z, 2);
+	}
 	static_assert(NS == 3 || NS == 4, "three or four stage buffers");
 	__shared__ __attribute__((aligned(16))) int8_t s0[T::STAGE], s1[T::STAGE], s2[T::STAGE],
 		s3[NS > 3 ? T::STAGE : 16];
@@ -530,57 +543,71 @@ static int pack_buffers(mlz_ctx *ctx, size_t abytes, size_t bbytes, void **ap, v
 //  10: as 4, staged by direct loads through 4 LDS buffers
 enum { I8V_AUTO = 0, I8V_COUNT = 11 };
 
-template <int E, int WTM, int WTN, int KK, int OCC, int WM = 2, int WN = 2, int NS = 0>
+// EA: storage of the A operand when it differs from B's and C's (E).
+template <int E, int WTM, int WTN, int KK, int OCC, int WM = 2, int WN = 2, int NS = 0, int EA = E>
 static int gemm_i8_launch_t(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, uint32_t a,
 			    const void *A, size_t lda, const void *B, size_t ldb, uint32_t b, void *C,
-			    size_t ldc)
+			    size_t ldc, const I8Batch &bt = I8Batch::single())
 {
 	typedef typename Elem<E>::T T;
+	typedef typename Elem<EA>::T TA;
 	typedef I8Tile<WTM, WTN, KK, WM, WN> Tile;
 	const ModP mp = make_modp(p);
 	const size_t mpad = round_up(m, Tile::BM), npad = round_up(n, Tile::BN);
+	if (bt.count < 1 || bt.count > 65535)
+		return set_error(ctx, MLZ_E_ARG, "batch of %d products (1 .. 65535)", bt.count);
 	for (size_t k0 = 0; k0 < k; k0 += I8_KMAX) {
 		const size_t kc = std::min((size_t) I8_KMAX, k - k0);
 		const size_t kpad = round_up(kc, 64);
 		const int KB = (int) (kpad / 32);
+		const size_t sAp = mpad * kpad * 2, sBp = npad * kpad * 2;   // bytes per packed item
 		void *ap = nullptr, *bp = nullptr;
-		MLZ_TRY(pack_buffers(ctx, mpad * kpad * 2, npad * kpad * 2, &ap, &bp));
+		MLZ_TRY(pack_buffers(ctx, sAp * bt.count, sBp * bt.count, &ap, &bp));
 		{
 			ProfScope ps(ctx, "pack_i8_A");
-			dim3 g(mpad / 64, kpad / 64);
-			const T *src = (const T *) A + k0;
-			if (aligned16(src, lda * sizeof(T)))
-				pack_i8_kernel<E, false, true><<<g, 256, 0, ctx->stream>>>(src, lda, (int) m, (int) kc, p,
-											   (int8_t *) ap, KB);
+			dim3 g(mpad / 64, kpad / 64, bt.count);
+			const TA *src = (const TA *) A + k0;
+			const bool vec = aligned16(src, lda * sizeof(TA)) &&
+					 (bt.count == 1 || ((bt.outer[0] | bt.in[0]) * (long long) sizeof(TA)) % 16 == 0);
+			if (vec)
+				pack_i8_kernel<EA, false, true><<<g, 256, 0, ctx->stream>>>(src, lda, (int) m, (int) kc, p,
+											    (int8_t *) ap, KB, bt, sAp);
 			else
-				pack_i8_kernel<E, false, false><<<g, 256, 0, ctx->stream>>>(src, lda, (int) m, (int) kc, p,
-											    (int8_t *) ap, KB);
+				pack_i8_kernel<EA, false, false><<<g, 256, 0, ctx->stream>>>(src, lda, (int) m, (int) kc, p,
+											     (int8_t *) ap, KB, bt, sAp);
 		}
 		MLZ_TRY(check_launch(ctx, "pack_i8_kernel(A)"));
 		{
 			ProfScope ps(ctx, "pack_i8_B");
-			dim3 g(npad / 64, kpad / 64);
+			dim3 g(npad / 64, kpad / 64, bt.count);
 			const T *src = (const T *) B + k0 * ldb;
-			if (aligned16(src, ldb * sizeof(T)))
+			// the pack kernel reads operand 0's offsets: hand it B's
+			I8Batch bb = bt;
+			bb.outer[0] = bt.outer[1];
+			bb.in[0] = bt.in[1];
+			const bool vec = aligned16(src, ldb * sizeof(T)) &&
+					 (bt.count == 1 || ((bt.outer[1] | bt.in[1]) * (long long) sizeof(T)) % 16 == 0);
+			if (vec)
 				pack_i8_kernel<E, true, true><<<g, 256, 0, ctx->stream>>>(src, ldb, (int) n, (int) kc, p,
-											  (int8_t *) bp, KB);
+											  (int8_t *) bp, KB, bb, sBp);
 			else
 				pack_i8_kernel<E, true, false><<<g, 256, 0, ctx->stream>>>(src, ldb, (int) n, (int) kc, p,
-											   (int8_t *) bp, KB);
+											   (int8_t *) bp, KB, bb, sBp);
 		}
 		MLZ_TRY(check_launch(ctx, "pack_i8_kernel(B)"));
 		const int tiles_m = (int) (mpad / Tile::BM), tiles_n = (int) (npad / Tile::BN);
 		{
 			ProfScope ps(ctx, "gemm_i8_mfma");
+			dim3 g(tiles_m * tiles_n, bt.count);
 			if constexpr (NS == 0)
-				gemm_i8_kernel<E, WTM, WTN, KK, OCC, WM, WN><<<tiles_m * tiles_n, Tile::THREADS, 0, ctx->stream>>>(
+				gemm_i8_kernel<E, WTM, WTN, KK, OCC, WM, WN><<<g, Tile::THREADS, 0, ctx->stream>>>(
 					(const int8_t *) ap, (const int8_t *) bp, KB, (T *) C, ldc, (int) m, (int) n, a,
-					k0 == 0 ? b : 1u, mp, tiles_m, tiles_n);
+					k0 == 0 ? b : 1u, mp, tiles_m, tiles_n, bt, sAp, sBp);
 			else
 				gemm_i8_async_kernel<E, WTM, WTN, KK, OCC, WM, WN, NS>
-					<<<tiles_m * tiles_n, Tile::THREADS, 0, ctx->stream>>>(
+					<<<g, Tile::THREADS, 0, ctx->stream>>>(
 						(const int8_t *) ap, (const int8_t *) bp, KB, (T *) C, ldc, (int) m, (int) n,
-						a, k0 == 0 ? b : 1u, mp, tiles_m, tiles_n);
+						a, k0 == 0 ? b : 1u, mp, tiles_m, tiles_n, bt, sAp, sBp);
 		}
 		MLZ_TRY(check_launch(ctx, "gemm_i8_kernel"));
 	}
@@ -631,6 +658,35 @@ static int gemm_i8_launch(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k
 	default:
 		return gemm_i8_launch_t<E, 2, 2, 2, 1>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
 	}
+}
+
+// Batches use the 128 x 128 tile through four LDS buffers (variant 10): a
+// batch is many small products, not one large one.  elemA: storage of A
+// (MLZ_U32 or elem).
+int zp_gemm_i8_batched_dev(mlz_ctx *ctx, int elem, int elemA, uint32_t p, const I8Batch &bt, size_t m, size_t n,
+			   size_t k, uint32_t a, const void *A, size_t lda, const void *B, size_t ldb, uint32_t b,
+			   void *C, size_t ldc)
+{
+	if (elemA != elem && elemA != MLZ_U32)
+		return set_error(ctx, MLZ_E_ARG, "batched gemm: A in the storage of B and C, or uint32");
+	if (p > 65536)
+		return set_error(ctx, MLZ_E_UNSUPPORTED, "INT8 limb gemm needs p <= 2^16");
+	if (m == 0 || n == 0 || bt.count == 0)
+		return MLZ_OK;
+	if (k == 0 || m > (size_t) INT32_MAX || n > (size_t) INT32_MAX)
+		return set_error(ctx, MLZ_E_ARG, "batched gemm needs 0 < k and m, n < 2^31");
+	a %= p;
+	b %= p;
+	const bool a32 = elemA == MLZ_U32;
+	if (elem == MLZ_U16)
+		return a32 ? gemm_i8_launch_t<MLZ_U16, 2, 2, 1, 2, 2, 2, 4, MLZ_U32>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C,
+										       ldc, bt)
+			   : gemm_i8_launch_t<MLZ_U16, 2, 2, 1, 2, 2, 2, 4>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc, bt);
+	if (elem == MLZ_U32)
+		return gemm_i8_launch_t<MLZ_U32, 2, 2, 1, 2, 2, 2, 4>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc, bt);
+	return a32 ? gemm_i8_launch_t<MLZ_F64_SYM, 2, 2, 1, 2, 2, 2, 4, MLZ_U32>(ctx, p, m, n, k, a, A, lda, B, ldb, b,
+										   C, ldc, bt)
+		   : gemm_i8_launch_t<MLZ_F64_SYM, 2, 2, 1, 2, 2, 2, 4>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc, bt);
 }
 
 // ===========================================================================
@@ -729,20 +785,20 @@ int zp_rank_update_i8_dev(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k
 		dim3 g(mpad / 64, kpad / 64);
 		if (aligned16(G, ldg * 4))
 			pack_i8_kernel<MLZ_U32, false, true><<<g, 256, 0, ctx->stream>>>(G, ldg, (int) m, (int) k, p,
-											 (int8_t *) gp, KB);
+											 (int8_t *) gp, KB, I8Batch::single(), 0);
 		else
 			pack_i8_kernel<MLZ_U32, false, false><<<g, 256, 0, ctx->stream>>>(G, ldg, (int) m, (int) k, p,
-											  (int8_t *) gp, KB);
+											  (int8_t *) gp, KB, I8Batch::single(), 0);
 	}
 	{
 		ProfScope ps(ctx, "pack_i8_B");
 		dim3 g(npad / 64, kpad / 64);
 		if (aligned16(W, ldw * 4))
 			pack_i8_kernel<MLZ_U32, true, true><<<g, 256, 0, ctx->stream>>>(W, ldw, (int) n, (int) k, p,
-											(int8_t *) wp, KB);
+											(int8_t *) wp, KB, I8Batch::single(), 0);
 		else
 			pack_i8_kernel<MLZ_U32, true, false><<<g, 256, 0, ctx->stream>>>(W, ldw, (int) n, (int) k, p,
-											 (int8_t *) wp, KB);
+											 (int8_t *) wp, KB, I8Batch::single(), 0);
 	}
 	MLZ_TRY(check_launch(ctx, "pack_i8_kernel"));
 	return zp_rank_update_i8_packed(ctx, p, m, n, KB, (const int8_t *) gp, (const int8_t *) wp, C, ldc);

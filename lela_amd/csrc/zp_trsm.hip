@@ -551,13 +551,97 @@ int zp_trsm_blocked_i8(mlz_ctx *ctx, int elem, uint64_t p, int upper, size_t n, 
 
 constexpr size_t TRSM_BLOCK_I8 = 1024;   // 512: same speed at 8192 x 1024; 2048: 20% slower
 
+// ---------------------------------------------------------------------------
+// Inverses of the full TRSM_INV_BLOCK-row diagonal blocks (p <= 2^16).  With
+// them the solve is a chain of n / TRSM_INV_BLOCK gemm pairs instead of
+// n / 64 dependent block solves.  They do not depend on B, so all blocks are
+// inverted together, by doubling from the 64-row inverses of
+// zp_trsm_prepare: a lower block [L11 0; L21 L22] has the inverse
+// [L11^-1 0; -L22^-1 L21 L11^-1 L22^-1] (upper: the off-diagonal block is
+// -U11^-1 U12 U22^-1), so each doubling step is two batched gemms over every
+// pair of diagonal blocks of the current size.  Storage: block I's inverse
+// is the dense S x S array Z + I*S*S (S = TRSM_INV_BLOCK), canonical uint32.
+// ---------------------------------------------------------------------------
+constexpr size_t TRSM_INV_BLOCK = 1024;
+
+// Ad <- the strict triangle of each full diagonal block of A (canonical,
+// zero elsewhere); Z <- the 64-row diagonal inverses of W in place, zero
+// elsewhere.  One thread per entry of the nfull S x S arrays.
+template <int E, bool UPPER>
+__global__ __launch_bounds__(256) void trsm_inv_init_kernel(const typename Elem<E>::T *__restrict__ A, size_t lda,
+							    const uint32_t *__restrict__ W, size_t total,
+							    uint32_t *__restrict__ Ad, uint32_t *__restrict__ Z,
+							    uint32_t p)
+{
+	constexpr size_t S = TRSM_INV_BLOCK;
+	const size_t idx = (size_t) blockIdx.x * 256 + threadIdx.x;
+	if (idx >= total)
+		return;
+	const size_t I = idx / (S * S), rem = idx % (S * S);
+	const int r = (int) (rem / S), c = (int) (rem % S);
+	const size_t g = I * S;
+	const bool tri = UPPER ? c > r : c < r;
+	Ad[idx] = tri ? Elem<E>::load(A + (g + r) * lda + g + c, p) : 0u;
+	const bool diag = (r >> 6) == (c >> 6) && (UPPER ? c >= r : c <= r);
+	Z[idx] = diag ? W[(g + r) * TRSM_LEAF + (c & (TRSM_LEAF - 1))] : 0u;
+}
+
+template <int E>
+static int trsm_block_inverses(mlz_ctx *ctx, uint32_t p, int upper, size_t nfull, const typename Elem<E>::T *A,
+			       size_t lda, const uint32_t *W, uint32_t *Z, uint32_t *Ad, uint32_t *Tb)
+{
+	constexpr size_t S = TRSM_INV_BLOCK;
+	const size_t total = nfull * S * S;
+	{
+		ProfScope ps(ctx, "trsm_inv_init");
+		if (upper)
+			trsm_inv_init_kernel<E, true><<<ceil_div(total, 256), 256, 0, ctx->stream>>>(A, lda, W, total, Ad, Z, p);
+		else
+			trsm_inv_init_kernel<E, false><<<ceil_div(total, 256), 256, 0, ctx->stream>>>(A, lda, W, total, Ad, Z, p);
+	}
+	MLZ_TRY(check_launch(ctx, "trsm_inv_init_kernel"));
+	for (size_t s = TRSM_LEAF; 2 * s <= S; s *= 2) {
+		// pair j of block I: rows and columns o .. o + 2s of Z_I, o = 2sj;
+		// element (o + r, o + c) of block I is at I*S*S + o*(S + 1) + r*S + c
+		const int inner = (int) (S / (2 * s));
+		I8Batch bt;
+		bt.count = (int) nfull * inner;
+		bt.inner = inner;
+		const long long so = (long long) (S * S), si = (long long) (2 * s * (S + 1));
+		const long long to = (long long) (inner * s * s), ti = (long long) (s * s);
+		// T = off-diagonal block of A times one diagonal inverse
+		bt.outer[0] = so; bt.in[0] = si;
+		bt.outer[1] = so; bt.in[1] = si;
+		bt.outer[2] = to; bt.in[2] = ti;
+		const uint32_t *offA = upper ? Ad + s : Ad + s * S;            // U12 / L21
+		const uint32_t *inv_r = upper ? Z + s * S + s : Z;             // U22^-1 / L11^-1
+		MLZ_TRY(zp_gemm_i8_batched_dev(ctx, MLZ_U32, MLZ_U32, p, bt, s, s, s, 1, offA, S, inv_r, S, 0, Tb, s));
+		// off-diagonal block of the inverse = -(other diagonal inverse) * T
+		bt.outer[1] = to; bt.in[1] = ti;
+		bt.outer[2] = so; bt.in[2] = si;
+		const uint32_t *inv_l = upper ? Z : Z + s * S + s;             // U11^-1 / L22^-1
+		uint32_t *out = upper ? Z + s : Z + s * S;
+		MLZ_TRY(zp_gemm_i8_batched_dev(ctx, MLZ_U32, MLZ_U32, p, bt, s, s, s, p - 1, inv_l, S, Tb, s, 0, out, S));
+	}
+	return MLZ_OK;
+}
+
 template <int E>
 static int trsm_rec(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, size_t nrhs, uint32_t a,
 		    const typename Elem<E>::T *A, size_t lda, typename Elem<E>::T *B, size_t ldb,
-		    const uint32_t *W, int algo)
+		    const uint32_t *W, int algo, const uint32_t *Z = nullptr, typename Elem<E>::T *Xt = nullptr)
 {
 	if (n == 0)
 		return MLZ_OK;
+	if (Z && n == TRSM_INV_BLOCK) {
+		// a full diagonal block with its inverse: X = a Z_I B_I through Xt
+		MLZ_TRY(zp_gemm_i8_batched_dev(ctx, E, MLZ_U32, p, I8Batch::single(), n, nrhs, n, a, Z, n, B, ldb, 0, Xt,
+					       nrhs));
+		const size_t es = sizeof(typename Elem<E>::T);
+		MLZ_HIP(ctx, hipMemcpy2DAsync(B, ldb * es, Xt, nrhs * es, nrhs * es, n, hipMemcpyDeviceToDevice,
+					      ctx->stream));
+		return MLZ_OK;
+	}
 	if (n <= (size_t) TRSM_LEAF)
 		return trsm_leaf<E>(ctx, p, n, nrhs, a, W, B, ldb, algo);
 	// diagonal blocks up to TRSM_BLOCK_I8 rows: one launch per 64-row block
@@ -567,17 +651,22 @@ static int trsm_rec(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, siz
 	// Split at a multiple of the leaf size near n/2, so that every leaf is
 	// one of the precomputed diagonal blocks (any split gives the same unique
 	// solution; LELA splits at n/2).
-	size_t l = (n / 2 + TRSM_LEAF - 1) / TRSM_LEAF * TRSM_LEAF;
+	// With block inverses the split is at a multiple of their size, so
+	// that every full block is a leaf; a last partial block (at the end of
+	// the matrix) runs the 64-row chain.
+	const size_t unit_split = Z ? TRSM_INV_BLOCK : (size_t) TRSM_LEAF;
+	size_t l = (n / 2 + unit_split - 1) / unit_split * unit_split;
 	const typename Elem<E>::T *A11 = A, *A22 = A + l * lda + l;
 	typename Elem<E>::T *B1 = B, *B2 = B + l * ldb;
+	const uint32_t *Z2 = Z ? Z + (l / TRSM_INV_BLOCK) * TRSM_INV_BLOCK * TRSM_INV_BLOCK : nullptr;
 	if (!upper) {
-		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, a, A11, lda, B1, ldb, W, algo));
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, a, A11, lda, B1, ldb, W, algo, Z, Xt));
 		MLZ_TRY(zp_gemm_dev(ctx, E, p, n - l, nrhs, l, p - 1, A + l * lda, lda, B1, ldb, a, B2, ldb, algo));
-		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n - l, nrhs, 1, A22, lda, B2, ldb, W + l * TRSM_LEAF, algo));
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n - l, nrhs, 1, A22, lda, B2, ldb, W + l * TRSM_LEAF, algo, Z2, Xt));
 	} else {
-		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n - l, nrhs, a, A22, lda, B2, ldb, W + l * TRSM_LEAF, algo));
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n - l, nrhs, a, A22, lda, B2, ldb, W + l * TRSM_LEAF, algo, Z2, Xt));
 		MLZ_TRY(zp_gemm_dev(ctx, E, p, l, nrhs, n - l, p - 1, A + l, lda, B2, ldb, a, B1, ldb, algo));
-		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, 1, A11, lda, B1, ldb, W, algo));
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, 1, A11, lda, B1, ldb, W, algo, Z, Xt));
 	}
 	return MLZ_OK;
 }
@@ -630,6 +719,30 @@ int zp_trsm_prepare(mlz_ctx *ctx, int elem, uint64_t p, int upper, int unit, siz
 	return MLZ_OK;
 }
 
+// The solve: for p <= 2^16 (INT8 gemms) and at least two full blocks of
+// TRSM_INV_BLOCK rows, their inverses first, then the recursion with them as
+// leaves; otherwise the recursion down to the 64-row chain.
+template <int E>
+static int trsm_apply_t(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, size_t nrhs, uint32_t a,
+			const typename Elem<E>::T *A, size_t lda, typename Elem<E>::T *B, size_t ldb, const uint32_t *W,
+			int algo)
+{
+	constexpr size_t S = TRSM_INV_BLOCK;
+	const size_t nfull = n / S;
+	if (p > 65536 || (algo != MLZ_ALGO_AUTO && algo != MLZ_ALGO_INT8) || nfull < 2)
+		return trsm_rec<E>(ctx, p, upper, unit, n, nrhs, a, A, lda, B, ldb, W, algo);
+	const size_t zbytes = round_up(nfull * S * S * sizeof(uint32_t), 256);
+	const size_t tbytes = round_up(nfull * S * (S / 4) * sizeof(uint32_t), 256);   // largest doubling step
+	const size_t xbytes = round_up(S * nrhs * sizeof(typename Elem<E>::T), 256);
+	void *w = nullptr;
+	MLZ_TRY(ws_get(ctx, WS_TRSM_3, 2 * zbytes + tbytes + xbytes, &w));
+	uint32_t *Z = (uint32_t *) w, *Ad = (uint32_t *) ((char *) w + zbytes);
+	uint32_t *Tb = (uint32_t *) ((char *) w + 2 * zbytes);
+	typename Elem<E>::T *Xt = (typename Elem<E>::T *) ((char *) w + 2 * zbytes + tbytes);
+	MLZ_TRY(trsm_block_inverses<E>(ctx, p, upper, nfull, A, lda, W, Z, Ad, Tb));
+	return trsm_rec<E>(ctx, p, upper, unit, n, nrhs, a, A, lda, B, ldb, W, algo, Z, Xt);
+}
+
 // The solve proper with the block inverses W of zp_trsm_prepare (for a
 // diagonal sub-block starting at row i0 of the prepared matrix, pass
 // W + i0 * TRSM_LEAF; i0 a multiple of TRSM_LEAF).
@@ -643,10 +756,13 @@ int zp_trsm_apply(mlz_ctx *ctx, int elem, uint64_t p, int upper, int unit, size_
 		return zp_scal_dev(ctx, elem, p, n, nrhs, 0, B, ldb);
 	const uint32_t pp = (uint32_t) p, aa = (uint32_t) a;
 	if (elem == MLZ_U16)
-		return trsm_rec<MLZ_U16>(ctx, pp, upper, unit, n, nrhs, aa, (const uint16_t *) A, lda, (uint16_t *) B, ldb, W, algo);
+		return trsm_apply_t<MLZ_U16>(ctx, pp, upper, unit, n, nrhs, aa, (const uint16_t *) A, lda, (uint16_t *) B,
+					     ldb, W, algo);
 	if (elem == MLZ_U32)
-		return trsm_rec<MLZ_U32>(ctx, pp, upper, unit, n, nrhs, aa, (const uint32_t *) A, lda, (uint32_t *) B, ldb, W, algo);
-	return trsm_rec<MLZ_F64_SYM>(ctx, pp, upper, unit, n, nrhs, aa, (const double *) A, lda, (double *) B, ldb, W, algo);
+		return trsm_apply_t<MLZ_U32>(ctx, pp, upper, unit, n, nrhs, aa, (const uint32_t *) A, lda, (uint32_t *) B,
+					     ldb, W, algo);
+	return trsm_apply_t<MLZ_F64_SYM>(ctx, pp, upper, unit, n, nrhs, aa, (const double *) A, lda, (double *) B, ldb,
+					 W, algo);
 }
 
 int zp_trsm_dev(mlz_ctx *ctx, int elem, uint64_t p, int upper, int unit, size_t n, size_t nrhs,

@@ -235,9 +235,9 @@ def test_gemm_fp64_full_size_8192(gpu_ctx):
     p, n = 67108859, 8192
     F = la.Modular(p, "uint32")
     gpu_ctx.F = F
-    A = synth.zp_matrix(synth.seed_for(2, 3), n, n, p)
-    B = synth.zp_matrix(synth.seed_for(2, 4), n, n, p)
-    C0 = synth.zp_matrix(synth.seed_for(2, 5), n, n, p)
+    A = synth.zp_matrix(synth.seed_for(2, 5), n, n, p)
+    B = synth.zp_matrix(synth.seed_for(2, 6), n, n, p)
+    C0 = synth.zp_matrix(synth.seed_for(2, 7), n, n, p)
     C = C0.copy()
     print("fp64 8192^3 product", flush=True)
     la.BLAS3.gemm(gpu_ctx, 1, A, B, 1, C, algo=la.ALGO_FP64)

@@ -102,3 +102,37 @@ def test_trsm_full_size_8192x1024(gpu_ctx):
         lhs = (L.astype(np.uint64) @ Xy) % P
         rhs = (B.astype(np.uint64) @ y) % P
         assert np.array_equal(lhs, rhs)
+
+
+@pytest.mark.parametrize("upper", [False, True])
+@pytest.mark.parametrize("unit", [False, True])
+@pytest.mark.parametrize("storage", ["uint16", "uint32", "double"])
+def test_trsm_block_inverse_path(gpu_ctx, upper, unit, storage):
+    """n = 2 * 1024 + 333, p <= 2^16: the two full 1024-row diagonal blocks
+    are inverted up front (doubling from the 64-row inverses) and solved as
+    gemms, the last partial block by the 64-row chain; bit-exact against
+    the oracle."""
+    p = 65521
+    n, nrhs = 2381, 40
+    print(f"trsm inverse path upper={upper} unit={unit} {storage}", flush=True)
+    A = tri(31 + upper, n, p, upper, unit)
+    B = synth.zp_matrix(33, n, nrhs, p)
+    got = run_trsm(gpu_ctx, p, storage, upper, unit, 5, A, B)
+    rc, _, want = orc.zp_trsm(p, upper, unit, 5, A, B)
+    assert rc == 0 and np.array_equal(got, want)
+
+
+def test_trsm_upper_nonunit_4096(gpu_ctx):
+    """Upper, non-unit diagonal, n = 4096 (four inverted blocks), checked by
+    Freivalds: A X == a B."""
+    from test_gemm_gpu import matvec_mod
+    p, n, nrhs = 65521, 4096, 300
+    A = tri(41, n, p, True)
+    B = synth.zp_matrix(42, n, nrhs, p)
+    X = run_trsm(gpu_ctx, p, "uint32", True, False, 9, A, B)
+    rng = np.random.default_rng(4)
+    for _ in range(2):
+        y = rng.integers(0, p, size=nrhs, dtype=np.uint64)
+        lhs = matvec_mod(A, matvec_mod(X, y, p), p)
+        rhs = (np.uint64(9) * matvec_mod(B, y, p)) % np.uint64(p)
+        assert np.array_equal(lhs, rhs)
