@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip plain/FP64 gemm, FL and other rows")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--only", choices=["fl", "rows", "batched"], default=None,
+    ap.add_argument("--only", choices=["fl", "rows", "trsm", "gf2", "batched"], default=None,
                     help="run one secondary leg alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run one device-to-device copy of BYTES (counter calibration)")
@@ -182,21 +182,28 @@ def bench_batch(la, synth, np):
     return out
 
 
-def bench_rows(la, synth, np):
-    """trsm 8192 x 1024 and the GF(2) 16384^2 gemm and rank-12000 RREF."""
+def bench_trsm(la, synth, np):
+    """trsm lower, unit diagonal, 8192 x 8192 with 1024 right-hand sides."""
     out = {}
     ctx = la.Context(la.Modular(P, "uint16"), device=0)
     L = np.tril(synth.zp_matrix(synth.seed_for(2, 3), N, N, P, np.uint16), -1)
     L[np.arange(N), np.arange(N)] = 1
     dL = la.DeviceMatrix.from_host(ctx, L)
     dB = la.DeviceMatrix.from_host(ctx, synth.zp_matrix(synth.seed_for(2, 4), N, 1024, P, np.uint16))
-    t = time_steps(ctx, lambda: la.BLAS3.trsm(ctx, 1, dL, dB, la.LowerTriangular, True), 3, 1)
+    t = time_steps(ctx, lambda: la.BLAS3.trsm(ctx, 1, dL, dB, la.LowerTriangular, True), 5, 1)
     out["trsm_8192x1024_ms"] = t * 1e3
     out["trsm_modular_MACs_per_s"] = N * (N - 1) / 2 * 1024 / t
     dL.free()
     dB.free()
+    ctx.close()
+    return out
+
+
+def bench_gf2(la, synth, np):
+    """GF(2) 16384^2 gemm and the echelon form of a rank-12000 product."""
+    out = {}
+    ctx = la.Context(la.GF2(), device=0)
     n = 16384
-    ctx.F = la.GF2()
     dA = la.DeviceMatrix.from_host(ctx, synth.gf2_matrix(synth.seed_for(3, 0), n, n))
     dB = la.DeviceMatrix.from_host(ctx, synth.gf2_matrix(synth.seed_for(3, 1), n, n))
     dC = la.DeviceMatrix(ctx, n, n // 64, np.uint64)
@@ -226,6 +233,13 @@ def bench_rows(la, synth, np):
     return out
 
 
+def bench_rows(la, synth, np):
+    """trsm 8192 x 1024 and the GF(2) 16384^2 gemm and rank-12000 RREF."""
+    out = bench_trsm(la, synth, np)
+    out.update(bench_gf2(la, synth, np))
+    return out
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -240,7 +254,8 @@ def main():
     from lela_amd import synth
 
     if args.only:
-        leg = {"fl": bench_fl, "rows": bench_rows, "batched": bench_batch}[args.only]
+        leg = {"fl": bench_fl, "rows": bench_rows, "trsm": bench_trsm, "gf2": bench_gf2,
+               "batched": bench_batch}[args.only]
         print(json.dumps({"leg": args.only, args.only: leg(la, synth, np)}), flush=True)
         return
 

@@ -82,7 +82,8 @@ int mlz_ctx_set_ptr_mode(mlz_ctx *ctx, int mode);
  *                      with two workgroups per CU, 5 = 128x256 and
  *                      6 = 256x128 with 8 waves (two per SIMD); 7, 8 = 5
  *                      staged through 3 / 4 LDS buffers by direct loads,
- *                      9 = 7 with 64 k per step, 10 = 4 through 4 buffers.
+ *                      9 = 7 with 64 k per step, 10 = 4 through 4 buffers,
+ *                      11 = 64x64 (4 waves of 32x32) through 4 buffers.
  *   "rref.search.threads" threads of the echelon form's fused pivot search
  *                      and inverse (one workgroup per panel): 256, 512 or 1024.
  *   "rref.update.blocks" 32-column blocks per wave of the echelon form's

@@ -230,8 +230,8 @@ int mlz_ctx_set_option(mlz_ctx *ctx, const char *key, int64_t value)
 	if (!ctx || !key)
 		return MLZ_E_ARG;
 	if (strcmp(key, "gemm.i8.variant") == 0) {
-		if (value < 0 || value > 10)
-			return set_error(ctx, MLZ_E_ARG, "gemm.i8.variant must be 0..10");
+		if (value < 0 || value > 11)
+			return set_error(ctx, MLZ_E_ARG, "gemm.i8.variant must be 0..11");
 		ctx->i8_variant = (int) value;
 		return MLZ_OK;
 	}

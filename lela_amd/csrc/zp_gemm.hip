@@ -541,16 +541,19 @@ static int pack_buffers(mlz_ctx *ctx, size_t abytes, size_t bbytes, void **ap, v
 //   8: as 5, staged by direct loads through 4 LDS buffers
 //   9: as 5 with 64 k per step, direct loads through 3 LDS buffers
 //  10: as 4, staged by direct loads through 4 LDS buffers
-enum { I8V_AUTO = 0, I8V_COUNT = 11 };
+//  11: 64 x 64, 4 waves of 32 x 32, through 4 LDS buffers, 4 workgroups/CU
+enum { I8V_AUTO = 0, I8V_COUNT = 12 };
 
 // EA: storage of the A operand when it differs from B's and C's (E).
 template <int E, int WTM, int WTN, int KK, int OCC, int WM = 2, int WN = 2, int NS = 0, int EA = E>
 static int gemm_i8_launch_t(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, uint32_t a,
 			    const void *A, size_t lda, const void *B, size_t ldb, uint32_t b, void *C,
-			    size_t ldc, const I8Batch &bt = I8Batch::single())
+			    size_t ldc, const I8Batch &bt = I8Batch::single(), const int8_t *Apre = nullptr)
 {
 	typedef typename Elem<E>::T T;
 	typedef typename Elem<EA>::T TA;
+	if (Apre && k > (size_t) I8_KMAX)
+		return set_error(ctx, MLZ_E_ARG, "a pre-packed A operand needs k <= %d", I8_KMAX);
 	typedef I8Tile<WTM, WTN, KK, WM, WN> Tile;
 	const ModP mp = make_modp(p);
 	const size_t mpad = round_up(m, Tile::BM), npad = round_up(n, Tile::BN);
@@ -562,8 +565,10 @@ static int gemm_i8_launch_t(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t
 		const int KB = (int) (kpad / 32);
 		const size_t sAp = mpad * kpad * 2, sBp = npad * kpad * 2;   // bytes per packed item
 		void *ap = nullptr, *bp = nullptr;
-		MLZ_TRY(pack_buffers(ctx, sAp * bt.count, sBp * bt.count, &ap, &bp));
-		{
+		MLZ_TRY(pack_buffers(ctx, Apre ? 0 : sAp * bt.count, sBp * bt.count, &ap, &bp));
+		if (Apre)
+			ap = (void *) Apre;
+		else {
 			ProfScope ps(ctx, "pack_i8_A");
 			dim3 g(mpad / 64, kpad / 64, bt.count);
 			const TA *src = (const TA *) A + k0;
@@ -625,6 +630,11 @@ static int i8_variant_for(size_t m, size_t n, size_t k)
 	const size_t tiles_wide = ceil_div(m, 128) * ceil_div(n, 256);
 	if (tiles_wide >= 256 && k >= 4096)
 		return 8;
+	// fewer 128 x 128 tiles than CUs: the 64 x 64 tile fills the chip
+	// (1024 x 1024 x 1024: 17.6 against 28.1 us; 2048 x 1024 x 2048: 34 against
+	// 45 us; equal at 4096 x 1024 x 4096)
+	if (ceil_div(m, 128) * ceil_div(n, 128) < 256)
+		return 11;
 	return 10;
 }
 
@@ -655,20 +665,53 @@ static int gemm_i8_launch(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k
 		return gemm_i8_launch_t<E, 2, 2, 2, 2, 2, 4, 3>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
 	case 10:
 		return gemm_i8_launch_t<E, 2, 2, 1, 2, 2, 2, 4>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
+	case 11:
+		return gemm_i8_launch_t<E, 1, 1, 1, 4, 2, 2, 4>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
 	default:
 		return gemm_i8_launch_t<E, 2, 2, 2, 1>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc);
 	}
 }
 
-// Batches use the 128 x 128 tile through four LDS buffers (variant 10): a
-// batch is many small products, not one large one.  elemA: storage of A
-// (MLZ_U32 or elem).
+template <int E, int WTM, int WTN, int KK, int OCC, int WM, int WN, int NS>
+static int gemm_i8_batched_ea(mlz_ctx *ctx, int elemA, uint32_t p, const I8Batch &bt, size_t m, size_t n, size_t k,
+			      uint32_t a, const void *A, size_t lda, const void *B, size_t ldb, uint32_t b, void *C,
+			      size_t ldc, const int8_t *Apre)
+{
+	if (elemA == MLZ_U16)
+		return gemm_i8_launch_t<E, WTM, WTN, KK, OCC, WM, WN, NS, MLZ_U16>(ctx, p, m, n, k, a, A, lda, B, ldb, b,
+										    C, ldc, bt, Apre);
+	if (elemA == MLZ_U32)
+		return gemm_i8_launch_t<E, WTM, WTN, KK, OCC, WM, WN, NS, MLZ_U32>(ctx, p, m, n, k, a, A, lda, B, ldb, b,
+										    C, ldc, bt, Apre);
+	return gemm_i8_launch_t<E, WTM, WTN, KK, OCC, WM, WN, NS, MLZ_F64_SYM>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C,
+										ldc, bt, Apre);
+}
+
+template <int WTM, int WTN, int KK, int OCC, int WM, int WN, int NS>
+static int gemm_i8_batched_e(mlz_ctx *ctx, int elem, int elemA, uint32_t p, const I8Batch &bt, size_t m, size_t n,
+			     size_t k, uint32_t a, const void *A, size_t lda, const void *B, size_t ldb, uint32_t b,
+			     void *C, size_t ldc, const int8_t *Apre)
+{
+	if (elem == MLZ_U16)
+		return gemm_i8_batched_ea<MLZ_U16, WTM, WTN, KK, OCC, WM, WN, NS>(ctx, elemA, p, bt, m, n, k, a, A, lda, B,
+										   ldb, b, C, ldc, Apre);
+	if (elem == MLZ_U32)
+		return gemm_i8_batched_ea<MLZ_U32, WTM, WTN, KK, OCC, WM, WN, NS>(ctx, elemA, p, bt, m, n, k, a, A, lda, B,
+										   ldb, b, C, ldc, Apre);
+	return gemm_i8_batched_ea<MLZ_F64_SYM, WTM, WTN, KK, OCC, WM, WN, NS>(ctx, elemA, p, bt, m, n, k, a, A, lda, B,
+									       ldb, b, C, ldc, Apre);
+}
+
+// Batches use the 128 x 128 tile through four LDS buffers (variant 10), or
+// the 64 x 64 one (variant 11) when the whole batch has fewer than 256
+// tiles of 128 x 128: a batch is many small products, not one large one.
+// elemA: storage of A.
 int zp_gemm_i8_batched_dev(mlz_ctx *ctx, int elem, int elemA, uint32_t p, const I8Batch &bt, size_t m, size_t n,
 			   size_t k, uint32_t a, const void *A, size_t lda, const void *B, size_t ldb, uint32_t b,
-			   void *C, size_t ldc)
+			   void *C, size_t ldc, const int8_t *Apre)
 {
-	if (elemA != elem && elemA != MLZ_U32)
-		return set_error(ctx, MLZ_E_ARG, "batched gemm: A in the storage of B and C, or uint32");
+	if (elem_size(elem) == 0 || elem_size(elemA) == 0)
+		return set_error(ctx, MLZ_E_ARG, "batched gemm: unknown element storage");
 	if (p > 65536)
 		return set_error(ctx, MLZ_E_UNSUPPORTED, "INT8 limb gemm needs p <= 2^16");
 	if (m == 0 || n == 0 || bt.count == 0)
@@ -677,16 +720,43 @@ int zp_gemm_i8_batched_dev(mlz_ctx *ctx, int elem, int elemA, uint32_t p, const 
 		return set_error(ctx, MLZ_E_ARG, "batched gemm needs 0 < k and m, n < 2^31");
 	a %= p;
 	b %= p;
-	const bool a32 = elemA == MLZ_U32;
+	if (Apre && bt.count > 1 && m % 128)
+		return set_error(ctx, MLZ_E_ARG, "a batch of pre-packed A operands needs rows in multiples of 128");
+	// the 64 x 64 tile unless the batch has 256 or more 128 x 128 tiles
+	if ((size_t) bt.count * ceil_div(m, 128) * ceil_div(n, 128) < 256)
+		return gemm_i8_batched_e<1, 1, 1, 4, 2, 2, 4>(ctx, elem, elemA, p, bt, m, n, k, a, A, lda, B, ldb, b, C, ldc,
+							      Apre);
+	return gemm_i8_batched_e<2, 2, 1, 2, 2, 2, 4>(ctx, elem, elemA, p, bt, m, n, k, a, A, lda, B, ldb, b, C, ldc, Apre);
+}
+
+// Packs A (rows x k per item, the A operand layout of the INT8 kernels) for a
+// batch of bt.count items into dst, item z at z * (bytes per item); returns
+// the bytes per item through *item_bytes.  For A operands used by several
+// products (zp_gemm_i8_batched_dev's Apre).
+int zp_pack_i8_a_dev(mlz_ctx *ctx, int elem, uint32_t p, const I8Batch &bt, size_t rows, size_t k, const void *A,
+		     size_t lda, int8_t *dst, size_t *item_bytes)
+{
+	const size_t mpad = round_up(rows, 128), kpad = round_up(k, 64);
+	const size_t bytes = mpad * kpad * 2;
+	if (item_bytes)
+		*item_bytes = bytes;
+	if (!dst)
+		return MLZ_OK;
+	if (k > (size_t) I8_KMAX || p > 65536)
+		return set_error(ctx, MLZ_E_ARG, "INT8 packing needs k <= %d and p <= 2^16", I8_KMAX);
+	const int KB = (int) (kpad / 32);
+	dim3 g(mpad / 64, kpad / 64, bt.count);
+	ProfScope ps(ctx, "pack_i8_A");
+#define PACK(EE, TT) pack_i8_kernel<EE, false, false><<<g, 256, 0, ctx->stream>>>((const TT *) A, lda, (int) rows, \
+									       (int) k, p, dst, KB, bt, bytes)
 	if (elem == MLZ_U16)
-		return a32 ? gemm_i8_launch_t<MLZ_U16, 2, 2, 1, 2, 2, 2, 4, MLZ_U32>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C,
-										       ldc, bt)
-			   : gemm_i8_launch_t<MLZ_U16, 2, 2, 1, 2, 2, 2, 4>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc, bt);
-	if (elem == MLZ_U32)
-		return gemm_i8_launch_t<MLZ_U32, 2, 2, 1, 2, 2, 2, 4>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc, bt);
-	return a32 ? gemm_i8_launch_t<MLZ_F64_SYM, 2, 2, 1, 2, 2, 2, 4, MLZ_U32>(ctx, p, m, n, k, a, A, lda, B, ldb, b,
-										   C, ldc, bt)
-		   : gemm_i8_launch_t<MLZ_F64_SYM, 2, 2, 1, 2, 2, 2, 4>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C, ldc, bt);
+		PACK(MLZ_U16, uint16_t);
+	else if (elem == MLZ_U32)
+		PACK(MLZ_U32, uint32_t);
+	else
+		PACK(MLZ_F64_SYM, double);
+#undef PACK
+	return check_launch(ctx, "pack_i8_kernel(A, batch)");
 }
 
 // ===========================================================================

@@ -59,49 +59,59 @@ __global__ void init_bad_kernel(unsigned long long *bad)
 	bad[1] = 0ull;
 }
 
-// Inverts every diagonal block of TRSM_LEAF rows in one launch (one
-// workgroup per block): X <- A_bb^-1 by substitution on the identity, in LDS.
-// W receives block b's inverse in rows b*L .. b*L+nb-1, L columns each.
-template <int E, bool UPPER, bool UNIT>
-__global__ __launch_bounds__(256) void trsm_diag_block_inv_kernel(const typename Elem<E>::T *__restrict__ A,
-								  size_t lda, int n,
-								  const uint32_t *__restrict__ dinv,
-								  uint32_t *__restrict__ W, ModP mp)
+// Inverts every diagonal block of TRSM_LEAF rows in one launch, one wave per
+// block (W receives block b's inverse in rows b*L .. b*L+nb-1, L columns
+// each).  Lane j computes column j of A_bb^-1, which depends on itself only:
+// x_i = (e_ij - sum_k A_ik x_k) / A_ii over the rows k solved before i, so
+// the substitution needs no barrier; the block's entries are LDS broadcasts.
+// The sum is delayed: raw products for p < 2^26 (64 terms below 2^58),
+// reduced products otherwise (64 terms below 2^38), one reduction per entry.
+template <int E, bool UPPER, bool UNIT, bool SMALL>
+__global__ __launch_bounds__(64) void trsm_diag_inv_col_kernel(const typename Elem<E>::T *__restrict__ A, size_t lda,
+							       int n, const uint32_t *__restrict__ dinv,
+							       uint32_t *__restrict__ W, ModP mp)
 {
 	constexpr int L = TRSM_LEAF;
-	__shared__ uint32_t Ns[L][L + 1];   // -A strict triangle of the block
-	__shared__ uint32_t X[L][L + 1];
+	__shared__ uint32_t As[L][L];
+	__shared__ uint32_t X[L][L];
+	__shared__ uint32_t Ds[L];
 	const uint32_t p = (uint32_t) mp.p;
+	const int j = threadIdx.x;
 	const int b0 = blockIdx.x * L;
 	const int nb = min(L, n - b0);
-	for (int idx = threadIdx.x; idx < L * L; idx += blockDim.x) {
-		int r = idx / L, c = idx % L;
+	// a partial block is completed by the identity (zero off-diagonal
+	// entries, unit diagonal): its first nb rows and columns are unchanged
+	for (int r = 0; r < L; ++r) {
 		uint32_t v = 0;
-		bool tri = UPPER ? (c > r) : (c < r);
-		if (r < nb && c < nb && tri)
-			v = Elem<E>::load(A + (size_t) (b0 + r) * lda + b0 + c, p);
-		Ns[r][c] = v == 0 ? 0u : p - v;
-		X[r][c] = (r == c) ? 1u : 0u;
+		const bool tri = UPPER ? (j > r) : (j < r);
+		if (r < nb && j < nb && tri)
+			v = Elem<E>::load(A + (size_t) (b0 + r) * lda + b0 + j, p);
+		As[r][j] = v;
+		X[r][j] = 0u;
 	}
+	Ds[j] = (UNIT || j >= nb) ? 1u : dinv[b0 + j];
 	__syncthreads();
-	for (int s = 0; s < nb; ++s) {
-		const int i = UPPER ? nb - 1 - s : s;
-		if (!UNIT && threadIdx.x < L)
-			X[i][threadIdx.x] = mp.mul(X[i][threadIdx.x], dinv[b0 + i]);
-		__syncthreads();
-		// rows still to solve: r > i (lower) or r < i (upper)
-		const int cnt = nb - 1 - s;
-		for (int idx = threadIdx.x; idx < cnt * L; idx += blockDim.x) {
-			int rr = idx / L, c = idx % L;
-			int r = UPPER ? rr : i + 1 + rr;
-			X[r][c] = mp.axpy(Ns[r][i], X[i][c], X[r][c]);
+	// Column j lives in LDS (X[k][j]: this lane's own entries, conflict
+	// free across lanes).  As is zero outside the strict triangle and X is
+	// zero where not solved yet, so the inner loop runs over every k with a
+	// fixed trip count and no branch, and its LDS reads are issued in
+	// batches rather than one round trip per term.
+#pragma unroll 1
+	for (int s = 0; s < L; ++s) {
+		const int i = UPPER ? L - 1 - s : s;
+		uint64_t acc = 0;
+#pragma unroll 16
+		for (int k = 0; k < L; ++k) {
+			const uint32_t a = As[i][k], xk = X[k][j];
+			acc += SMALL ? (uint64_t) a * xk : (uint64_t) mp.mul(a, xk);
 		}
-		__syncthreads();
+		uint32_t v = mp.sub(i == j ? 1u : 0u, (uint32_t) mp.reduce(acc));
+		if (!UNIT)
+			v = mp.mul(v, Ds[i]);
+		X[i][j] = v;
 	}
-	for (int idx = threadIdx.x; idx < nb * L; idx += blockDim.x) {
-		int r = idx / L, c = idx % L;
-		W[(size_t) (b0 + r) * L + c] = X[r][c];
-	}
+	for (int r = 0; r < nb; ++r)
+		W[(size_t) (b0 + r) * L + j] = j < nb ? X[r][j] : 0u;
 }
 
 // Leaf solve B_b <- a * W_b * B_b for one diagonal block (nb <= L rows), in
@@ -207,11 +217,16 @@ static int trsm_diag_inverses(mlz_ctx *ctx, uint32_t p, int upper, int unit, siz
 	const ModP mp = make_modp(p);
 	unsigned g = ceil_div(n, TRSM_LEAF);
 	ProfScope ps(ctx, "trsm_diag_block_inv");
-#define INV(U, N) trsm_diag_block_inv_kernel<E, U, N><<<g, 256, 0, ctx->stream>>>((const T *) A, lda, (int) n, dinv, W, mp)
-	if (upper) { if (unit) INV(true, true); else INV(true, false); }
-	else       { if (unit) INV(false, true); else INV(false, false); }
+#define INV(U, N, S) trsm_diag_inv_col_kernel<E, U, N, S><<<g, 64, 0, ctx->stream>>>((const T *) A, lda, (int) n, dinv, W, mp)
+	if (mp.small) {
+		if (upper) { if (unit) INV(true, true, true); else INV(true, false, true); }
+		else       { if (unit) INV(false, true, true); else INV(false, false, true); }
+	} else {
+		if (upper) { if (unit) INV(true, true, false); else INV(true, false, false); }
+		else       { if (unit) INV(false, true, false); else INV(false, false, false); }
+	}
 #undef INV
-	return check_launch(ctx, "trsm_diag_block_inv_kernel");
+	return check_launch(ctx, "trsm_diag_inv_col_kernel");
 }
 
 template <int E>
@@ -563,15 +578,13 @@ constexpr size_t TRSM_BLOCK_I8 = 1024;   // 512: same speed at 8192 x 1024; 2048
 // is the dense S x S array Z + I*S*S (S = TRSM_INV_BLOCK), canonical uint32.
 // ---------------------------------------------------------------------------
 constexpr size_t TRSM_INV_BLOCK = 1024;
+constexpr size_t TRSM_ZP_BYTES = TRSM_INV_BLOCK * TRSM_INV_BLOCK * 2;   // one inverse as INT8 limbs
 
-// Ad <- the strict triangle of each full diagonal block of A (canonical,
-// zero elsewhere); Z <- the 64-row diagonal inverses of W in place, zero
-// elsewhere.  One thread per entry of the nfull S x S arrays.
-template <int E, bool UPPER>
-__global__ __launch_bounds__(256) void trsm_inv_init_kernel(const typename Elem<E>::T *__restrict__ A, size_t lda,
-							    const uint32_t *__restrict__ W, size_t total,
-							    uint32_t *__restrict__ Ad, uint32_t *__restrict__ Z,
-							    uint32_t p)
+// Z <- the 64-row diagonal inverses of W in place, zero elsewhere (one
+// thread per entry of the nfull S x S arrays).
+template <bool UPPER>
+__global__ __launch_bounds__(256) void trsm_inv_init_kernel(const uint32_t *__restrict__ W, size_t total,
+							    uint32_t *__restrict__ Z)
 {
 	constexpr size_t S = TRSM_INV_BLOCK;
 	const size_t idx = (size_t) blockIdx.x * 256 + threadIdx.x;
@@ -579,46 +592,46 @@ __global__ __launch_bounds__(256) void trsm_inv_init_kernel(const typename Elem<
 		return;
 	const size_t I = idx / (S * S), rem = idx % (S * S);
 	const int r = (int) (rem / S), c = (int) (rem % S);
-	const size_t g = I * S;
-	const bool tri = UPPER ? c > r : c < r;
-	Ad[idx] = tri ? Elem<E>::load(A + (g + r) * lda + g + c, p) : 0u;
 	const bool diag = (r >> 6) == (c >> 6) && (UPPER ? c >= r : c <= r);
-	Z[idx] = diag ? W[(g + r) * TRSM_LEAF + (c & (TRSM_LEAF - 1))] : 0u;
+	Z[idx] = diag ? W[(I * S + r) * TRSM_LEAF + (c & (TRSM_LEAF - 1))] : 0u;
 }
 
 template <int E>
 static int trsm_block_inverses(mlz_ctx *ctx, uint32_t p, int upper, size_t nfull, const typename Elem<E>::T *A,
-			       size_t lda, const uint32_t *W, uint32_t *Z, uint32_t *Ad, uint32_t *Tb)
+			       size_t lda, const uint32_t *W, uint32_t *Z, uint32_t *Tb)
 {
 	constexpr size_t S = TRSM_INV_BLOCK;
 	const size_t total = nfull * S * S;
 	{
 		ProfScope ps(ctx, "trsm_inv_init");
 		if (upper)
-			trsm_inv_init_kernel<E, true><<<ceil_div(total, 256), 256, 0, ctx->stream>>>(A, lda, W, total, Ad, Z, p);
+			trsm_inv_init_kernel<true><<<ceil_div(total, 256), 256, 0, ctx->stream>>>(W, total, Z);
 		else
-			trsm_inv_init_kernel<E, false><<<ceil_div(total, 256), 256, 0, ctx->stream>>>(A, lda, W, total, Ad, Z, p);
+			trsm_inv_init_kernel<false><<<ceil_div(total, 256), 256, 0, ctx->stream>>>(W, total, Z);
 	}
 	MLZ_TRY(check_launch(ctx, "trsm_inv_init_kernel"));
 	for (size_t s = TRSM_LEAF; 2 * s <= S; s *= 2) {
-		// pair j of block I: rows and columns o .. o + 2s of Z_I, o = 2sj;
-		// element (o + r, o + c) of block I is at I*S*S + o*(S + 1) + r*S + c
+		// pair j of block I: rows and columns o .. o + 2s of the block, o = 2sj.
+		// Entry (o + r, o + c) of block I is at I*S*S + o*(S + 1) + r*S + c in
+		// Z and at (I*S + o + r)*lda + I*S + o + c in A.
 		const int inner = (int) (S / (2 * s));
 		I8Batch bt;
 		bt.count = (int) nfull * inner;
 		bt.inner = inner;
-		const long long so = (long long) (S * S), si = (long long) (2 * s * (S + 1));
+		const long long zo = (long long) (S * S), zi = (long long) (2 * s * (S + 1));
+		const long long ao = (long long) (S * (lda + 1)), ai = (long long) (2 * s * (lda + 1));
 		const long long to = (long long) (inner * s * s), ti = (long long) (s * s);
-		// T = off-diagonal block of A times one diagonal inverse
-		bt.outer[0] = so; bt.in[0] = si;
-		bt.outer[1] = so; bt.in[1] = si;
+		// T = off-diagonal block of A (U12 / L21) times one diagonal inverse (U22^-1 / L11^-1)
+		bt.outer[0] = ao; bt.in[0] = ai;
+		bt.outer[1] = zo; bt.in[1] = zi;
 		bt.outer[2] = to; bt.in[2] = ti;
-		const uint32_t *offA = upper ? Ad + s : Ad + s * S;            // U12 / L21
-		const uint32_t *inv_r = upper ? Z + s * S + s : Z;             // U22^-1 / L11^-1
-		MLZ_TRY(zp_gemm_i8_batched_dev(ctx, MLZ_U32, MLZ_U32, p, bt, s, s, s, 1, offA, S, inv_r, S, 0, Tb, s));
+		const typename Elem<E>::T *offA = upper ? A + s : A + s * lda;
+		const uint32_t *inv_r = upper ? Z + s * S + s : Z;
+		MLZ_TRY(zp_gemm_i8_batched_dev(ctx, MLZ_U32, E, p, bt, s, s, s, 1, offA, lda, inv_r, S, 0, Tb, s));
 		// off-diagonal block of the inverse = -(other diagonal inverse) * T
+		bt.outer[0] = zo; bt.in[0] = zi;
 		bt.outer[1] = to; bt.in[1] = ti;
-		bt.outer[2] = so; bt.in[2] = si;
+		bt.outer[2] = zo; bt.in[2] = zi;
 		const uint32_t *inv_l = upper ? Z : Z + s * S + s;             // U11^-1 / L22^-1
 		uint32_t *out = upper ? Z + s : Z + s * S;
 		MLZ_TRY(zp_gemm_i8_batched_dev(ctx, MLZ_U32, MLZ_U32, p, bt, s, s, s, p - 1, inv_l, S, Tb, s, 0, out, S));
@@ -629,18 +642,16 @@ static int trsm_block_inverses(mlz_ctx *ctx, uint32_t p, int upper, size_t nfull
 template <int E>
 static int trsm_rec(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, size_t nrhs, uint32_t a,
 		    const typename Elem<E>::T *A, size_t lda, typename Elem<E>::T *B, size_t ldb,
-		    const uint32_t *W, int algo, const uint32_t *Z = nullptr, typename Elem<E>::T *Xt = nullptr)
+		    const uint32_t *W, int algo, const int8_t *Zp = nullptr)
 {
 	if (n == 0)
 		return MLZ_OK;
-	if (Z && n == TRSM_INV_BLOCK) {
-		// a full diagonal block with its inverse: X = a Z_I B_I through Xt
-		MLZ_TRY(zp_gemm_i8_batched_dev(ctx, E, MLZ_U32, p, I8Batch::single(), n, nrhs, n, a, Z, n, B, ldb, 0, Xt,
-					       nrhs));
-		const size_t es = sizeof(typename Elem<E>::T);
-		MLZ_HIP(ctx, hipMemcpy2DAsync(B, ldb * es, Xt, nrhs * es, nrhs * es, n, hipMemcpyDeviceToDevice,
-					      ctx->stream));
-		return MLZ_OK;
+	if (Zp && n == TRSM_INV_BLOCK) {
+		// a full diagonal block with its inverse (packed): X = a Z_I B_I.  The
+		// product is written over B_I, which the gemm reads only through its
+		// packed copy.
+		return zp_gemm_i8_batched_dev(ctx, E, MLZ_U32, p, I8Batch::single(), n, nrhs, n, a, nullptr, n, B, ldb, 0, B,
+					      ldb, Zp);
 	}
 	if (n <= (size_t) TRSM_LEAF)
 		return trsm_leaf<E>(ctx, p, n, nrhs, a, W, B, ldb, algo);
@@ -654,19 +665,19 @@ static int trsm_rec(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, siz
 	// With block inverses the split is at a multiple of their size, so
 	// that every full block is a leaf; a last partial block (at the end of
 	// the matrix) runs the 64-row chain.
-	const size_t unit_split = Z ? TRSM_INV_BLOCK : (size_t) TRSM_LEAF;
+	const size_t unit_split = Zp ? TRSM_INV_BLOCK : (size_t) TRSM_LEAF;
 	size_t l = (n / 2 + unit_split - 1) / unit_split * unit_split;
 	const typename Elem<E>::T *A11 = A, *A22 = A + l * lda + l;
 	typename Elem<E>::T *B1 = B, *B2 = B + l * ldb;
-	const uint32_t *Z2 = Z ? Z + (l / TRSM_INV_BLOCK) * TRSM_INV_BLOCK * TRSM_INV_BLOCK : nullptr;
+	const int8_t *Zp2 = Zp ? Zp + (l / TRSM_INV_BLOCK) * TRSM_ZP_BYTES : nullptr;
 	if (!upper) {
-		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, a, A11, lda, B1, ldb, W, algo, Z, Xt));
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, a, A11, lda, B1, ldb, W, algo, Zp));
 		MLZ_TRY(zp_gemm_dev(ctx, E, p, n - l, nrhs, l, p - 1, A + l * lda, lda, B1, ldb, a, B2, ldb, algo));
-		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n - l, nrhs, 1, A22, lda, B2, ldb, W + l * TRSM_LEAF, algo, Z2, Xt));
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n - l, nrhs, 1, A22, lda, B2, ldb, W + l * TRSM_LEAF, algo, Zp2));
 	} else {
-		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n - l, nrhs, a, A22, lda, B2, ldb, W + l * TRSM_LEAF, algo, Z2, Xt));
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n - l, nrhs, a, A22, lda, B2, ldb, W + l * TRSM_LEAF, algo, Zp2));
 		MLZ_TRY(zp_gemm_dev(ctx, E, p, l, nrhs, n - l, p - 1, A + l, lda, B2, ldb, a, B1, ldb, algo));
-		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, 1, A11, lda, B1, ldb, W, algo, Z, Xt));
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, 1, A11, lda, B1, ldb, W, algo, Zp));
 	}
 	return MLZ_OK;
 }
@@ -733,14 +744,22 @@ static int trsm_apply_t(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n,
 		return trsm_rec<E>(ctx, p, upper, unit, n, nrhs, a, A, lda, B, ldb, W, algo);
 	const size_t zbytes = round_up(nfull * S * S * sizeof(uint32_t), 256);
 	const size_t tbytes = round_up(nfull * S * (S / 4) * sizeof(uint32_t), 256);   // largest doubling step
-	const size_t xbytes = round_up(S * nrhs * sizeof(typename Elem<E>::T), 256);
 	void *w = nullptr;
-	MLZ_TRY(ws_get(ctx, WS_TRSM_3, 2 * zbytes + tbytes + xbytes, &w));
-	uint32_t *Z = (uint32_t *) w, *Ad = (uint32_t *) ((char *) w + zbytes);
-	uint32_t *Tb = (uint32_t *) ((char *) w + 2 * zbytes);
-	typename Elem<E>::T *Xt = (typename Elem<E>::T *) ((char *) w + 2 * zbytes + tbytes);
-	MLZ_TRY(trsm_block_inverses<E>(ctx, p, upper, nfull, A, lda, W, Z, Ad, Tb));
-	return trsm_rec<E>(ctx, p, upper, unit, n, nrhs, a, A, lda, B, ldb, W, algo, Z, Xt);
+	MLZ_TRY(ws_get(ctx, WS_TRSM_3, zbytes + tbytes + nfull * TRSM_ZP_BYTES, &w));
+	uint32_t *Z = (uint32_t *) w;
+	uint32_t *Tb = (uint32_t *) ((char *) w + zbytes);
+	int8_t *Zp = (int8_t *) w + zbytes + tbytes;
+	MLZ_TRY(trsm_block_inverses<E>(ctx, p, upper, nfull, A, lda, W, Z, Tb));
+	// the inverses as INT8 A operands, packed once for the leaves
+	I8Batch bz = I8Batch::single();
+	bz.count = (int) nfull;
+	bz.outer[0] = (long long) (S * S);
+	size_t item = 0;
+	MLZ_TRY(zp_pack_i8_a_dev(ctx, MLZ_U32, p, bz, S, S, nullptr, S, nullptr, &item));
+	if (item != TRSM_ZP_BYTES)
+		return set_error(ctx, MLZ_E_ARG, "packed inverse size %zu, expected %zu", item, TRSM_ZP_BYTES);
+	MLZ_TRY(zp_pack_i8_a_dev(ctx, MLZ_U32, p, bz, S, S, Z, S, Zp, nullptr));
+	return trsm_rec<E>(ctx, p, upper, unit, n, nrhs, a, A, lda, B, ldb, W, algo, Zp);
 }
 
 // The solve proper with the block inverses W of zp_trsm_prepare (for a

@@ -81,7 +81,7 @@ def test_gemm_strided_views(gpu_ctx, algo):
     assert np.array_equal(Cp[mask], outside[mask])
 
 
-@pytest.mark.parametrize("variant", list(range(1, 11)))
+@pytest.mark.parametrize("variant", list(range(1, 12)))
 @pytest.mark.parametrize("storage", ["uint16", "uint32", "double"])
 def test_gemm_int8_tile_variants(gpu_ctx, variant, storage):
     """Every INT8 workgroup-tile variant gives the same exact product.  The k
