@@ -26,6 +26,19 @@ P = 65521
 P_FP64 = 67108859
 INT8_PEAK_TOPS = 5000.0      # MI355X dense INT8 MFMA, 5.0 POP/s
 FP64_PEAK_TFLOPS = 78.6
+# LDS: 256 CUs x 128 bytes per clock at 2.4 GHz; VALU: 256 CUs x 4 SIMDs x 16 lanes at 2.4 GHz
+LDS_PEAK_TBS = 256 * 128 * 2.4e9 / 1e12
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+
+
+def mfma_roofline(modular_macs, seconds, kernel=None):
+    """INT8-MFMA roofline of a Z/p workload: 4 limb products (8 ops) per modular MAC."""
+    achieved = 8.0 * modular_macs / seconds / 1e12
+    out = {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TFLOP/s",
+           "frac": achieved / INT8_PEAK_TOPS, "modular_MACs": modular_macs}
+    if kernel:
+        out["kernel"] = kernel
+    return out
 
 
 def parse():
@@ -36,7 +49,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip plain/FP64 gemm, FL and other rows")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--only", choices=["fl", "rows", "trsm", "gf2", "batched"], default=None,
+    ap.add_argument("--only", choices=["fl", "rows", "trsm", "gf2", "fp64", "batched", "cpu"], default=None,
                     help="run one secondary leg alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run one device-to-device copy of BYTES (counter calibration)")
@@ -81,6 +94,108 @@ def cpu_baseline(seconds):
                       "ZpModule<uint16> dot + generic gemm, -O2, one core"}
 
 
+def host_cores():
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"nproc": avail, "cpu_count": os.cpu_count()}
+
+
+def cpu_baselines(seconds):
+    """The CPU restatements (oracle) of LELA's paths for configs 2b, 3a, 3b, 4
+    and 5, each on a bounded sample of the workload, one core, extrapolated
+    as each entry's `sample` says."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+    from lela_amd import synth
+    lib = orc.load()
+    out = {}
+    # 2b: trsm columns are independent -- solve a block of right-hand sides
+    L = np.tril(synth.zp_matrix(synth.seed_for(2, 3), N, N, P), -1)
+    L[np.arange(N), np.arange(N)] = 1
+    cols = 32
+    Bs = np.ascontiguousarray(synth.zp_matrix(synth.seed_for(2, 4), N, 1024, P)[:, :cols])
+    import ctypes
+    bad = ctypes.c_size_t(0)
+    t0 = time.perf_counter()
+    lib.orc_zp_trsm(P, 0, 1, N, cols, 1, L.ctypes.data, N, Bs.ctypes.data, cols, ctypes.byref(bad))
+    t = time.perf_counter() - t0
+    W = N * (N - 1) / 2 * cols
+    out["2b_trsm_8192x1024"] = {"value": W / t, "unit": "modular MACs/s", "cores": 1, "kind": "port",
+                                "est_full_s": t * 1024 / cols,
+                                "sample": f"{cols} of 1024 right-hand sides ({t:.1f} s), recursive trsm "
+                                          "restatement, -O2"}
+    del L, Bs
+    # 3a: GF(2) gemm rows are independent; the transposition of B is a fixed cost
+    n = 16384
+    A = synth.gf2_matrix(synth.seed_for(3, 0), n, n)
+    B = synth.gf2_matrix(synth.seed_for(3, 1), n, n)
+    C = np.zeros((129, n // 64), dtype=np.uint64)
+
+    def gf2_rows(r):
+        t0 = time.perf_counter()
+        lib.orc_gf2_gemm(r, n, n, 1, A.ctypes.data, n // 64, B.ctypes.data, n // 64, 0, C.ctypes.data, n // 64)
+        return time.perf_counter() - t0
+    t1, t2 = gf2_rows(1), gf2_rows(129)
+    per_row = max(1e-9, (t2 - t1) / 128)
+    est = t1 - per_row + per_row * n
+    out["3a_gf2_gemm_16384"] = {"value": float(n) ** 3 / est, "unit": "bit-MACs/s", "cores": 1, "kind": "port",
+                                "est_full_s": est,
+                                "sample": f"rows 1 and 129 of {n} ({t1:.1f} s, {t2:.1f} s: transposition of B plus "
+                                          f"{per_row * 1e3:.0f} ms per row), word AND/XOR-parity dots, -O2"}
+    del A, B
+    # 3b: the echelon form of a 4096^2 rank-3000 product, scaled by m*r*n to 16384^2 rank 12000
+    m4, r4 = 4096, 3000
+    X = synth.gf2_matrix(synth.seed_for(3, 2), m4, r4)
+    Y = synth.gf2_matrix(synth.seed_for(3, 3), r4, m4)
+    M = orc.gf2_gemm(m4, m4, r4, 1, X, Y, 0, np.zeros((m4, m4 // 64), dtype=np.uint64))
+    rank = ctypes.c_size_t(0)
+    piv = np.zeros(m4, dtype=np.uint32)
+    t0 = time.perf_counter()
+    lib.orc_gf2_echelonize(m4, m4, M.ctypes.data, m4 // 64, 1, ctypes.byref(rank), piv.ctypes.data)
+    t = time.perf_counter() - t0
+    scale = (16384 * 12000 * 16384) / (m4 * r4 * m4)
+    out["3b_gf2_rref_16384_rank12000"] = {"value": 16384 * 12000 * 16384 / (t * scale), "unit": "bit-ops/s",
+                                          "cores": 1, "kind": "port", "est_full_s": t * scale,
+                                          "sample": f"{m4}^2 rank {rank.value} product ({t:.1f} s), scaled by "
+                                                    f"m*r*n ({scale:.0f}x), Gauss-Jordan restatement, -O2"}
+    # 4: FL phases -- the dense Schur product D - C A^-1 B on sampled rows of C,
+    # the other phases at the same rate from their MAC counts
+    rows = 2
+    Cs = synth.zp_matrix(synth.seed_for(4, 1), rows, 15000, P)
+    Bf = synth.zp_matrix(synth.seed_for(4, 2), 15000, 15000, P)
+    Ds = synth.zp_matrix(synth.seed_for(4, 3), rows, 15000, P)
+    t0 = time.perf_counter()
+    lib.orc_zp_gemm_u32(P, rows, 15000, 15000, P - 1, Cs.ctypes.data, 15000, Bf.ctypes.data, 15000, 1,
+                        Ds.ctypes.data, 15000)
+    t = time.perf_counter() - t0
+    rate = rows * 15000 * 15000 / t
+    phases = {"trsm_A^-1B": 2.25e6 * 15000, "schur": 5000.0 * 15000 * 15000,
+              "rref_D": 5000.0 * 5000 * 12500, "back_substitution": 15000.0 * 5000 * 10000}
+    tot = sum(phases.values())
+    out["4_fl_20000x30000"] = {"value": tot / (tot / rate), "unit": "modular MACs/s", "cores": 1, "kind": "port",
+                               "est_full_s": tot / rate, "phase_MACs": phases,
+                               "sample": f"{rows} of 5000 rows of the Schur product ({t:.1f} s, uint32 ZpModule "
+                                         "dots); the other phases extrapolated at that rate from their MAC counts"}
+    del Bf
+    # 5: the first 16 instances of the 4096-instance batch, one at a time
+    raw = synth.batch_instances(synth.seed_for(5, 0), count=16, p=P)
+    t0 = time.perf_counter()
+    for d in raw:
+        if d["op"] == 1:
+            orc.zp_gemm(P, 1, d["A"].astype(np.uint32), d["B"].astype(np.uint32), 1, d["C"].astype(np.uint32))
+        else:
+            orc.zp_echelonize(P, d["A"].astype(np.uint32))
+    t = time.perf_counter() - t0
+    out["5_batch_4096"] = {"value": len(raw) / t, "unit": "instances/s", "cores": 1, "kind": "port",
+                           "est_full_s": t * 4096 / len(raw),
+                           "sample": f"instances 0..15 of the 4096 ({t:.1f} s; gemm via ZpModule dots, echelon "
+                                     "forms via Elimination), one core"}
+    return out
+
+
 def bench_fl(la, synth, np):
     """Config 4: Faugere-Lachartre on the 20000 x 30000 F4-shaped matrix."""
     ctx = la.Context(la.Modular(P, "uint32"), device=0)
@@ -98,6 +213,9 @@ def bench_fl(la, synth, np):
         out[f"{name}_s"] = t
         out["rank"] = int(res["r"][0])
     out["speedup"] = out["plain_s"] / out["mfma_s"]
+    # algorithmic modular MACs: sparse A^-1 B, Schur product, RREF of D, back-substitution
+    W = 2.25e6 * 15000 + 5000.0 * 15000 * 15000 + 5000.0 * 5000 * 12500 + 15000.0 * 5000 * 10000
+    out["roofline"] = mfma_roofline(W, out["mfma_s"])
     for d in (drp, dc, dv, dR):
         d.free()
     ctx.close()
@@ -182,6 +300,29 @@ def bench_batch(la, synth, np):
     return out
 
 
+def bench_fp64(la, synth, np):
+    """The FP64-carrier gemm, 8192^3, p = 67108859, uint32 storage."""
+    ctx = la.Context(la.Modular(P_FP64, "uint32"), device=0)
+    W = float(N) ** 3
+    dA2 = la.DeviceMatrix.from_host(ctx, synth.zp_matrix(synth.seed_for(2, 5), N, N, P_FP64))
+    dB2 = la.DeviceMatrix.from_host(ctx, synth.zp_matrix(synth.seed_for(2, 6), N, N, P_FP64))
+    dC2 = la.DeviceMatrix.from_host(ctx, synth.zp_matrix(synth.seed_for(2, 7), N, N, P_FP64))
+    ctx.prof_reset()
+    ctx.prof_enable(True)
+    t = time_steps(ctx, lambda: la.BLAS3.gemm(ctx, 1, dA2, dB2, 1, dC2, algo=la.ALGO_FP64), 2, 1)
+    ctx.prof_enable(False)
+    cnt, kms = ctx.prof_query("gemm_f64_mfma")
+    fp64_kernel_ms = kms / max(1, cnt)
+    tf = 2 * 2 * W / (fp64_kernel_ms * 1e-3) / 1e12     # 2 limbs x (mul + add)
+    for d in (dA2, dB2, dC2):
+        d.free()
+    ctx.close()
+    return {"ms_per_step": t * 1e3, "value": W / t, "unit": "modular MACs/s",
+            "roofline": {"bound": "mfma", "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": tf / FP64_PEAK_TFLOPS, "kernel": "gemm_f64_kernel",
+                         "kernel_avg_ms": fp64_kernel_ms, "op": "2 FP64 FMAs (4 flops) per modular MAC"}}
+
+
 def bench_trsm(la, synth, np):
     """trsm lower, unit diagonal, 8192 x 8192 with 1024 right-hand sides."""
     out = {}
@@ -193,6 +334,7 @@ def bench_trsm(la, synth, np):
     t = time_steps(ctx, lambda: la.BLAS3.trsm(ctx, 1, dL, dB, la.LowerTriangular, True), 5, 1)
     out["trsm_8192x1024_ms"] = t * 1e3
     out["trsm_modular_MACs_per_s"] = N * (N - 1) / 2 * 1024 / t
+    out["trsm_roofline"] = mfma_roofline(N * (N - 1) / 2 * 1024, t)
     dL.free()
     dB.free()
     ctx.close()
@@ -207,10 +349,25 @@ def bench_gf2(la, synth, np):
     dA = la.DeviceMatrix.from_host(ctx, synth.gf2_matrix(synth.seed_for(3, 0), n, n))
     dB = la.DeviceMatrix.from_host(ctx, synth.gf2_matrix(synth.seed_for(3, 1), n, n))
     dC = la.DeviceMatrix(ctx, n, n // 64, np.uint64)
-    t = time_steps(ctx, lambda: la.BLAS3.gemm(ctx, 1, la.Dense01Matrix(dA, n), la.Dense01Matrix(dB, n), 0,
-                                              la.Dense01Matrix(dC, n)), 3, 1)
+    gf2 = lambda: la.BLAS3.gemm(ctx, 1, la.Dense01Matrix(dA, n), la.Dense01Matrix(dB, n), 0,
+                                la.Dense01Matrix(dC, n))
+    t = time_steps(ctx, gf2, 5, 1)
     out["gf2_gemm_16384_ms"] = t * 1e3
     out["gf2_bit_MACs_per_s"] = n ** 3 / t
+    # Four Russians: 8 table reads of 8 bytes per (C row, 64-bit word of A, C word)
+    lds = n * (n // 64) * (n // 64) * 64 / t / 1e12
+    out["gf2_gemm_roofline"] = {"bound": "lds", "achieved": lds, "peak": LDS_PEAK_TBS, "unit": "TB/s",
+                                "frac": lds / LDS_PEAK_TBS, "kernel": "gf2_gemm_m4rm_kernel",
+                                "bytes": "table reads, 64 B per (C row, A word, C word)"}
+    ctx.set_option("gf2.gemm.kernel", 1)
+    t = time_steps(ctx, gf2, 5, 1)
+    ctx.set_option("gf2.gemm.kernel", 0)
+    out["gf2_gemm_dot_16384_ms"] = t * 1e3
+    # dot form: one AND+XOR of 32-bit halves (2 lane ops) per 64-bit word of a dot product
+    ops = 2.0 * n * n * (n // 64) / t / 1e12
+    out["gf2_gemm_dot_roofline"] = {"bound": "valu", "achieved": ops, "peak": VALU_PEAK_TOPS, "unit": "Tops/s",
+                                    "frac": ops / VALU_PEAK_TOPS, "kernel": "gf2_gemm_dot_kernel",
+                                    "ops": "2 per (C entry, 64-bit word of k)"}
     r = 12000
     dX = la.DeviceMatrix.from_host(ctx, synth.gf2_matrix(synth.seed_for(3, 2), n, r))
     dY = la.DeviceMatrix.from_host(ctx, synth.gf2_matrix(synth.seed_for(3, 3), r, n))
@@ -254,7 +411,11 @@ def main():
     from lela_amd import synth
 
     if args.only:
-        leg = {"fl": bench_fl, "rows": bench_rows, "trsm": bench_trsm, "gf2": bench_gf2,
+        if args.only == "cpu":
+            print(json.dumps({"leg": "cpu", "cpu_baselines": cpu_baselines(args.cpu_seconds),
+                              "host": host_cores()}), flush=True)
+            return
+        leg = {"fl": bench_fl, "rows": bench_rows, "trsm": bench_trsm, "gf2": bench_gf2, "fp64": bench_fp64,
                "batched": bench_batch}[args.only]
         print(json.dumps({"leg": args.only, args.only: leg(la, synth, np)}), flush=True)
         return
@@ -338,26 +499,9 @@ def main():
         out["int8_over_plain"] = value / variants["plain_u16_p65521"]["value"]
         for d in (dA, dB, dC):
             d.free()
-        F2 = la.Modular(P_FP64, "uint32")
-        ctx.F = F2
-        dA2 = la.DeviceMatrix.from_host(ctx, synth.zp_matrix(synth.seed_for(2, 5), N, N, P_FP64))
-        dB2 = la.DeviceMatrix.from_host(ctx, synth.zp_matrix(synth.seed_for(2, 6), N, N, P_FP64))
-        dC2 = la.DeviceMatrix.from_host(ctx, synth.zp_matrix(synth.seed_for(2, 7), N, N, P_FP64))
-        ctx.prof_reset()
-        ctx.prof_enable(True)
-        t = time_steps(ctx, lambda: la.BLAS3.gemm(ctx, 1, dA2, dB2, 1, dC2, algo=la.ALGO_FP64), 2, 1)
-        ctx.prof_enable(False)
-        cnt, kms = ctx.prof_query("gemm_f64_mfma")
-        fp64_kernel_ms = kms / max(1, cnt)
-        tf = 2 * 2 * W / (fp64_kernel_ms * 1e-3) / 1e12     # 2 limbs x (mul + add)
-        variants["fp64_u32_p67108859"] = {
-            "ms_per_step": t * 1e3, "value": W / t, "unit": "modular MACs/s",
-            "roofline": {"bound": "mfma", "achieved": tf, "peak": FP64_PEAK_TFLOPS,
-                         "unit": "TFLOP/s", "frac": tf / FP64_PEAK_TFLOPS}}
-        for d in (dA2, dB2, dC2):
-            d.free()
-        out["variants"] = variants
         ctx.close()
+        variants["fp64_u32_p67108859"] = bench_fp64(la, synth, np)
+        out["variants"] = variants
         out["fl_echelon"] = bench_fl(la, synth, np)
         out["rows"] = bench_rows(la, synth, np)
         out["batched"] = bench_batch(la, synth, np)
@@ -366,8 +510,11 @@ def main():
             d.free()
         ctx.close()
 
+    out["host"] = host_cores()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        if not args.no_variants:
+            out["cpu_baselines"] = cpu_baselines(args.cpu_seconds)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

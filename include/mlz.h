@@ -87,7 +87,9 @@ int mlz_ctx_set_ptr_mode(mlz_ctx *ctx, int mode);
  *   "rref.search.threads" threads of the echelon form's fused pivot search
  *                      and inverse (one workgroup per panel): 256, 512 or 1024.
  *   "rref.update.blocks" 32-column blocks per wave of the echelon form's
- *                      trailing update on INT8 limbs: 1 (default), 2 or 4. */
+ *                      trailing update on INT8 limbs: 1 (default), 2 or 4.
+ *   "gf2.gemm.kernel"  GF(2) gemm: 0 = Four Russians tables in LDS (default),
+ *                      1 = the plain AND/XOR-popcount dot form. */
 int mlz_ctx_set_option(mlz_ctx *ctx, const char *key, int64_t value);
 int mlz_ctx_synchronize(mlz_ctx *ctx);
 /* Message for the last error on this context (static storage, never NULL). */

@@ -243,7 +243,7 @@ enum WsSlot {
 	WS_TRSM_0, WS_TRSM_1, WS_TRSM_2, WS_TRSM_3,   // trsm temporaries
 	WS_RREF_0, WS_RREF_1, WS_RREF_2, WS_RREF_3, WS_RREF_4, WS_RREF_5,
 	WS_FL_0, WS_FL_1, WS_FL_2, WS_FL_3, WS_FL_4, WS_FL_5, WS_FL_6, WS_FL_7,
-	WS_GF2_0, WS_GF2_1, WS_GF2_2,
+	WS_GF2_0, WS_GF2_1, WS_GF2_2, WS_GF2_3,
 	WS_BATCH_0, WS_BATCH_1, WS_BATCH_2, WS_BATCH_3,
 	WS_SMALL,
 	WS_INVTAB,                               // inverses mod p (p <= 2^16), see invtab_p
@@ -282,6 +282,7 @@ struct mlz_ctx {
 	int upd_nblk;        // 32-column blocks per wave of the rank-k update (1, 2, 4)
 	int rref_fast_threads;   // threads of the echelon form's fused pivot-search kernel (256, 512, 1024)
 	uint32_t invtab_p;       // modulus whose inverse table WS_INVTAB holds (0 = none)
+	int gf2_kernel;          // GF(2) gemm: 0 Four Russians (default), 1 the AND/XOR-popcount dot form
 	struct mlz_tuner *tuner;   // autotuner table and cost-model weights (batch.hip), lazily created
 	// No-allocation calls (mlz_zp_gemm_noalloc): while `noalloc` is set, gemm
 	// operand packing takes its buffers from `fixed_ws` (this device's copy of

@@ -168,6 +168,7 @@ int mlz_ctx_create(mlz_ctx **out, int device, const char *config)
 	ctx->upd_nblk = 1;
 	ctx->rref_fast_threads = 1024;
 	ctx->invtab_p = 0;
+	ctx->gf2_kernel = 0;
 	ctx->tuner = nullptr;
 	ctx->noalloc = false;
 	ctx->fixed_ws = nullptr;
@@ -245,6 +246,12 @@ int mlz_ctx_set_option(mlz_ctx *ctx, const char *key, int64_t value)
 		if (value != 1 && value != 2 && value != 4)
 			return set_error(ctx, MLZ_E_ARG, "rref.update.blocks must be 1, 2 or 4");
 		ctx->upd_nblk = (int) value;
+		return MLZ_OK;
+	}
+	if (strcmp(key, "gf2.gemm.kernel") == 0) {
+		if (value != 0 && value != 1)
+			return set_error(ctx, MLZ_E_ARG, "gf2.gemm.kernel must be 0 or 1");
+		ctx->gf2_kernel = (int) value;
 		return MLZ_OK;
 	}
 	return set_error(ctx, MLZ_E_ARG, "unknown option '%s'", key);

@@ -107,11 +107,135 @@ __global__ __launch_bounds__(256) void gf2_gemm_m4rm_kernel(const uint64_t *__re
 	}
 }
 
+// ---------------------------------------------------------------------------
+// The plain form: every entry of C is a dot product over GF(2), the parity
+// of popcount(A_i AND B^T_j) (lela/blas/level1-gf2.tcc:26-52 on the rows of A
+// and of B's transpose, lela/blas/level3-generic.tcc:243-263).
+// B is first transposed in 64 x 64-bit blocks into word-major columns,
+// Bw[w][j] = bits 64w .. 64w+63 of column j (one wave per block: lane t holds
+// row 64w + t, and the ballot of bit j over the lanes is column j's word).
+// The product kernel gives each lane one column of C: a wave covers 64
+// columns and 64 rows, and accumulates acc[r] ^= A[r][w] & Bw[w][col] in
+// 32-bit halves (two ANDs and one three-way XOR per 64-bit word); A's rows
+// are LDS broadcasts.  At the end, bit r of the output word of row r is the
+// parity of acc[r], and the ballot over the wave's lanes is that word.
+// ---------------------------------------------------------------------------
+constexpr int G2D_KC = 16;    // k-words per LDS stage of the dot kernel
+constexpr int G2D_TR = 128;   // rows per workgroup (2 waves of 64)
+constexpr int G2D_TC = 128;   // columns per workgroup (2 waves of 64)
+
+__global__ __launch_bounds__(64) void gf2_transpose64_kernel(const uint64_t *__restrict__ B, size_t ldb, int k_bits,
+							     int nw, uint64_t *__restrict__ Bw, size_t ldw)
+{
+	const int c = blockIdx.x, w = blockIdx.y, t = threadIdx.x;
+	const int row = w * 64 + t;
+	const uint64_t v = (row < k_bits && c < nw) ? B[(size_t) row * ldb + c] : 0ull;
+	uint64_t mine = 0;
+#pragma unroll 8
+	for (int j = 0; j < 64; ++j) {
+		const uint64_t col = __ballot((v >> j) & 1ull);
+		if (t == j)
+			mine = col;
+	}
+	Bw[(size_t) w * ldw + (size_t) c * 64 + t] = mine;
+}
+
+__global__ __launch_bounds__(256) void gf2_gemm_dot_kernel(const uint64_t *__restrict__ A, size_t lda,
+							   const uint64_t *__restrict__ Bw, size_t ldw,
+							   uint64_t *__restrict__ C, size_t ldc, int m, int nbits,
+							   int kw, int a, int b)
+{
+	__shared__ uint4 As[G2D_TR][G2D_KC / 2];
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int wr = wave >> 1, wc = wave & 1;
+	const int r0 = blockIdx.y * G2D_TR, c0 = blockIdx.x * G2D_TC;
+	const int col = c0 + wc * 64 + lane;
+	const int nw = (nbits + 63) / 64;
+	uint32_t acc[64];
+#pragma unroll
+	for (int r = 0; r < 64; ++r)
+		acc[r] = 0u;
+	if (a) {
+		for (int k0 = 0; k0 < kw; k0 += G2D_KC) {
+			// A tile: 128 rows x KC words, thread t loads row t/2, 8 words
+			{
+				const int r = tid >> 1, h = (tid & 1) * (G2D_KC / 2);
+				const int row = r0 + r;
+				uint64_t v[G2D_KC / 2];
+#pragma unroll
+				for (int q = 0; q < G2D_KC / 2; ++q) {
+					const int w = k0 + h + q;
+					v[q] = (row < m && w < kw) ? A[(size_t) row * lda + w] : 0ull;
+				}
+#pragma unroll
+				for (int q = 0; q < G2D_KC / 2; q += 2)
+					As[r][(h + q) / 2] = make_uint4((uint32_t) v[q], (uint32_t) (v[q] >> 32),
+									(uint32_t) v[q + 1], (uint32_t) (v[q + 1] >> 32));
+			}
+			uint2 bv[G2D_KC];
+#pragma unroll
+			for (int q = 0; q < G2D_KC; ++q) {
+				const uint64_t x = (k0 + q < kw) ? Bw[(size_t) (k0 + q) * ldw + col] : 0ull;
+				bv[q] = make_uint2((uint32_t) x, (uint32_t) (x >> 32));
+			}
+			__syncthreads();
+#pragma unroll
+			for (int q = 0; q < G2D_KC; q += 2) {
+#pragma unroll
+				for (int r = 0; r < 64; ++r) {
+					const uint4 av = As[wr * 64 + r][q / 2];
+					acc[r] ^= (av.x & bv[q].x) ^ (av.y & bv[q].y) ^ (av.z & bv[q + 1].x) ^ (av.w & bv[q + 1].y);
+				}
+			}
+			__syncthreads();
+		}
+	}
+	// output word of row r: the ballot of the parities; lane r keeps row r's
+	uint64_t mine = 0;
+#pragma unroll
+	for (int r = 0; r < 64; ++r) {
+		const uint64_t word = __ballot(__popc(acc[r]) & 1u);
+		if (lane == r)
+			mine = word;
+	}
+	const int row = r0 + wr * 64 + lane;
+	const int gw = (c0 >> 6) + wc;
+	if (row < m && gw < nw) {
+		uint64_t *cp = C + (size_t) row * ldc + gw;
+		const uint64_t mask = (gw == nw - 1) ? tail_mask64(nbits - gw * 64) : ~0ull;
+		const uint64_t old = *cp;
+		const uint64_t v = (b ? old : 0ull) ^ mine;
+		*cp = (v & mask) | (old & ~mask);
+	}
+}
+
+static int gf2_gemm_dot(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda,
+			const uint64_t *B, size_t ldb, int b, uint64_t *C, size_t ldc)
+{
+	const size_t nw = (n + 63) / 64, kw = (k + 63) / 64;
+	const size_t ldw = round_up(nw * 64, G2D_TC);
+	void *w = nullptr;
+	if (a && kw) {
+		MLZ_TRY(ws_get(ctx, WS_GF2_3, std::max((size_t) 1, kw) * ldw * sizeof(uint64_t), &w));
+		ProfScope ps(ctx, "gf2_transpose64");
+		gf2_transpose64_kernel<<<dim3((unsigned) (ldw / 64), (unsigned) kw), 64, 0, ctx->stream>>>(
+			B, ldb, (int) k, (int) nw, (uint64_t *) w, ldw);
+		MLZ_TRY(check_launch(ctx, "gf2_transpose64_kernel"));
+	}
+	ProfScope ps(ctx, "gf2_gemm_dot");
+	dim3 g(ceil_div(n, G2D_TC), ceil_div(m, G2D_TR));
+	gf2_gemm_dot_kernel<<<g, 256, 0, ctx->stream>>>(A, lda, (const uint64_t *) w, ldw, C, ldc, (int) m, (int) n,
+							 (int) kw, (a & 1) && kw > 0, b & 1);
+	return check_launch(ctx, "gf2_gemm_dot_kernel");
+}
+
 int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda,
 		 const uint64_t *B, size_t ldb, int b, uint64_t *C, size_t ldc)
 {
 	if (m == 0 || n == 0)
 		return MLZ_OK;
+	if (ctx->gf2_kernel == 1)
+		return gf2_gemm_dot(ctx, m, n, k, a, A, lda, B, ldb, b, C, ldc);
 	const size_t nw = (n + 63) / 64;
 	dim3 g(ceil_div(nw, G2_TW), ceil_div(m, G2_TM));
 	ProfScope ps(ctx, "gf2_gemm_m4rm");

@@ -13,31 +13,38 @@ from lela_amd import synth
 pytestmark = pytest.mark.gpu
 
 
-def gpu_gemm(ctx, m, n, k, a, A, B, b, C):
+def gpu_gemm(ctx, m, n, k, a, A, B, b, C, kernel=0):
+    """kernel: 0 Four Russians tables, 1 the AND/XOR-popcount dot form."""
     ctx.F = la.GF2()
     Cw = C.copy()
-    la.BLAS3.gemm(ctx, a, la.Dense01Matrix(A, k), la.Dense01Matrix(B, n), b, la.Dense01Matrix(Cw, n))
+    ctx.set_option("gf2.gemm.kernel", kernel)
+    try:
+        la.BLAS3.gemm(ctx, a, la.Dense01Matrix(A, k), la.Dense01Matrix(B, n), b, la.Dense01Matrix(Cw, n))
+    finally:
+        ctx.set_option("gf2.gemm.kernel", 0)
     return Cw
 
 
 @pytest.mark.parametrize("m,n,k", [(1, 1, 1), (5, 70, 9), (64, 64, 64), (130, 300, 129),
                                    (1100, 257, 200), (300, 1030, 700), (2049, 64, 63)])
 @pytest.mark.parametrize("a,b", [(1, 1), (1, 0), (0, 1), (0, 0)])
-def test_gf2_gemm_matches_oracle(gpu_ctx, m, n, k, a, b):
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_gf2_gemm_matches_oracle(gpu_ctx, m, n, k, a, b, kernel):
     A, B, C = synth.gf2_matrix(1, m, k), synth.gf2_matrix(2, k, n), synth.gf2_matrix(3, m, n)
-    got = gpu_gemm(gpu_ctx, m, n, k, a, A, B, b, C)
+    got = gpu_gemm(gpu_ctx, m, n, k, a, A, B, b, C, kernel)
     want = orc.gf2_gemm(m, n, k, a, A, B, b, C)
     assert np.array_equal(got, want)
 
 
-def test_gf2_gemm_ignores_padding_bits(gpu_ctx):
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_gf2_gemm_ignores_padding_bits(gpu_ctx, kernel):
     m, n, k = 40, 100, 70
     A, B, C = synth.gf2_matrix(4, m, k), synth.gf2_matrix(5, k, n), synth.gf2_matrix(6, m, n)
     want = orc.gf2_gemm(m, n, k, 1, A, B, 1, C)
     A2, B2 = A.copy(), B.copy()
     A2[:, -1] |= ~np.uint64((1 << (k % 64)) - 1)       # garbage beyond column k
     B2[:, -1] |= ~np.uint64((1 << (n % 64)) - 1)
-    got = gpu_gemm(gpu_ctx, m, n, k, 1, A2, B2, 1, C)
+    got = gpu_gemm(gpu_ctx, m, n, k, 1, A2, B2, 1, C, kernel)
     assert np.array_equal(bf.gf2_unpack(got, n), bf.gf2_unpack(want, n))
     assert np.array_equal(got[:, -1] >> np.uint64(n % 64), C[:, -1] >> np.uint64(n % 64))
 
@@ -105,12 +112,13 @@ def _gf2_matvec(W, ncols, x_bits):
     return out & 1
 
 
-def test_gf2_gemm_full_size_16384(gpu_ctx):
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_gf2_gemm_full_size_16384(gpu_ctx, kernel):
     n = 16384
     A = synth.gf2_matrix(synth.seed_for(3, 0), n, n)
     B = synth.gf2_matrix(synth.seed_for(3, 1), n, n)
     C = np.zeros_like(A)
-    got = gpu_gemm(gpu_ctx, n, n, n, 1, A, B, 0, C)
+    got = gpu_gemm(gpu_ctx, n, n, n, 1, A, B, 0, C, kernel)
     rng = np.random.default_rng(12)
     for _ in range(2):
         x = rng.integers(0, 2, size=n).astype(np.int64)
