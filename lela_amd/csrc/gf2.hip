@@ -5,7 +5,7 @@
 // (lela/matrix/dense-zero-one.h:253-257).
 //
 // gemm (C <- a*A*B + b*C): "Method of Four Russians" multiplication.  A
-// workgroup owns G2_TM rows x G2_TW words of C in registers.  For each 64-row
+// workgroup owns 1024 or 2048 rows x G2_TW words of C in registers.  For each 64-row
 // slab of B it builds, in LDS, eight 256-entry tables T_b[i] = XOR of the
 // rows 8b+j of the slab for the set bits j of i (one XOR per entry, built by
 // doubling), then every C row XORs T_b[byte b of its A word] for b = 0..7.
@@ -25,15 +25,16 @@
 namespace mlz {
 
 constexpr int G2_TW = 4;                    // C words per workgroup tile (256 columns)
-constexpr int G2_RPT = 4;                   // C rows per thread
-constexpr int G2_TM = 256 * G2_RPT;         // C rows per workgroup
 
 __device__ inline uint64_t tail_mask64(int bits)
 {
 	return bits >= 64 ? ~0ull : ((1ull << bits) - 1);
 }
 
-// k_bits: inner dimension in bits; nbits: C columns.
+// k_bits: inner dimension in bits; nbits: C columns.  G2_RPT rows per thread
+// (256 * G2_RPT per workgroup): 8 halves the table builds per row of C,
+// 4 gives twice the workgroups to narrow products.
+template <int G2_RPT>
 __global__ __launch_bounds__(256) void gf2_gemm_m4rm_kernel(const uint64_t *__restrict__ A, size_t lda,
 							    const uint64_t *__restrict__ B, size_t ldb,
 							    uint64_t *__restrict__ C, size_t ldc, int m,
@@ -45,7 +46,7 @@ __global__ __launch_bounds__(256) void gf2_gemm_m4rm_kernel(const uint64_t *__re
 	const int nw = (nbits + 63) / 64;
 	const int kw = (k_bits + 63) / 64;
 	const int w0 = blockIdx.x * G2_TW;
-	const int row0 = blockIdx.y * G2_TM;
+	const int row0 = blockIdx.y * (256 * G2_RPT);
 	uint64_t acc[G2_RPT][G2_TW];
 #pragma unroll
 	for (int q = 0; q < G2_RPT; ++q)
@@ -237,10 +238,16 @@ int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64
 	if (ctx->gf2_kernel == 1)
 		return gf2_gemm_dot(ctx, m, n, k, a, A, lda, B, ldb, b, C, ldc);
 	const size_t nw = (n + 63) / 64;
-	dim3 g(ceil_div(nw, G2_TW), ceil_div(m, G2_TM));
 	ProfScope ps(ctx, "gf2_gemm_m4rm");
-	gf2_gemm_m4rm_kernel<<<g, 256, 0, ctx->stream>>>(A, lda, B, ldb, C, ldc, (int) m, (int) n, (int) k,
-							  (a & 1) && k > 0, b & 1);
+	// 8 rows per thread where that still leaves 512 workgroups (two per CU):
+	// 16384^3 in 4.65 ms against 4.89 ms with 4
+	dim3 g8(ceil_div(nw, G2_TW), ceil_div(m, 256 * 8));
+	if ((size_t) g8.x * g8.y >= 512)
+		gf2_gemm_m4rm_kernel<8><<<g8, 256, 0, ctx->stream>>>(A, lda, B, ldb, C, ldc, (int) m, (int) n, (int) k,
+								     (a & 1) && k > 0, b & 1);
+	else
+		gf2_gemm_m4rm_kernel<4><<<dim3(ceil_div(nw, G2_TW), ceil_div(m, 256 * 4)), 256, 0, ctx->stream>>>(
+			A, lda, B, ldb, C, ldc, (int) m, (int) n, (int) k, (a & 1) && k > 0, b & 1);
 	return check_launch(ctx, "gf2_gemm_m4rm_kernel");
 }
 
