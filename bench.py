@@ -460,11 +460,12 @@ def main():
         s.free()
         d.free()
 
-    traffic = None
+    traffic, traffic_commit = None, None
     tfile = os.path.join(ROOT, "profiles", "gemm_i8_mfma_8192_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        traffic, traffic_commit = tj.get("hbm_bytes_per_launch"), tj.get("measured_on_commit")
 
     out = {
         "metric": "Z/p gemm modular MACs/s @8192^3 p=65521 (split-int8 MFMA) + F-L echelon s",
@@ -485,6 +486,8 @@ def main():
                    "algo": "split-int8 MFMA (2x2 8-bit limbs, int32 accumulate)"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS,
                      "unit": "TFLOP/s", "frac": achieved / INT8_PEAK_TOPS, "traffic": traffic,
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated, measured on commit "
+                                       f"{traffic_commit} (profiles/gemm_i8_mfma_8192_traffic.json)",
                      "kernel": "gemm_i8_async_kernel", "kernel_avg_ms": avg_kernel_ms,
                      "ops_per_launch": 8.0 * W, "op": "int8 multiply-add counted as 2 ops"},
         "timing": {"event_ms_per_step": ev_ms / args.steps,

@@ -368,6 +368,11 @@ constexpr size_t TRSM_LEAF_ROWS = 64;   // block size of zp_trsm_prepare's inver
 // the last column block I's band needs (a banded A skips the zero columns).
 int zp_trsm_blocked_i8(mlz_ctx *ctx, int elem, uint64_t p, int upper, size_t n, size_t nrhs, uint64_t a,
 		       const void *A, size_t lda, void *B, size_t ldb, const uint32_t *W, const int *bend);
+// Upper banded A (canonical uint32, p <= 2^16, n >= 1024): X = A^-1 B with
+// the 1024-row diagonal blocks inverted up front; c1_64 as for
+// zp_trsm_blocked_i8.
+int zp_trsm_banded_inv_i8(mlz_ctx *ctx, uint64_t p, size_t n, size_t nrhs, const uint32_t *A, size_t lda,
+			  uint32_t *B, size_t ldb, const uint32_t *W, const int *c1_64);
 int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A, size_t lda,
 		size_t *rank_out, uint32_t *pivots_out, int algo);
 int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda,

@@ -327,7 +327,13 @@ static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *
 				c1[I] = (int) c;
 			}
 			ProfScope ps(ctx, "fl_solve");
-			MLZ_TRY(zp_trsm_blocked_i8(ctx, MLZ_U32, p, 1, r, nr, 1, dA, r, dB, nr, Winv, c1.data()));
+			// from two 1024-row blocks on: their inverses first, then one band
+			// gemm and one inverse gemm per block (15 dependent steps instead
+			// of 235 at 15000 rows)
+			if (r >= 2048)
+				MLZ_TRY(zp_trsm_banded_inv_i8(ctx, p, r, nr, dA, r, dB, nr, Winv, c1.data()));
+			else
+				MLZ_TRY(zp_trsm_blocked_i8(ctx, MLZ_U32, p, 1, r, nr, 1, dA, r, dB, nr, Winv, c1.data()));
 		}
 		for (size_t I = i8 ? 0 : nb; I-- > 0;) {
 			const size_t i0 = I * FL_TRSM_BLOCK, i1 = std::min(r, i0 + FL_TRSM_BLOCK);

@@ -730,6 +730,57 @@ int zp_trsm_prepare(mlz_ctx *ctx, int elem, uint64_t p, int upper, int unit, siz
 	return MLZ_OK;
 }
 
+// Upper triangular banded A (canonical uint32, p <= 2^16): X = A^-1 B with
+// the full TRSM_INV_BLOCK-row diagonal blocks inverted up front.  The blocks
+// are solved from the bottom: the partial block at the end (if any) by the
+// 64-row chain, then block I as B_I <- B_I - A[I, i1 .. c1) X[i1 .. c1) (one
+// gemm over its band, c1[I] one past its rows' last nonzero column) and
+// X_I = Z_I B_I.  W: the 64-row inverses of zp_trsm_prepare for A;
+// c1_64: band ends of the 64-row blocks (as zp_trsm_blocked_i8 takes them).
+int zp_trsm_banded_inv_i8(mlz_ctx *ctx, uint64_t p, size_t n, size_t nrhs, const uint32_t *A, size_t lda,
+			  uint32_t *B, size_t ldb, const uint32_t *W, const int *c1_64)
+{
+	constexpr size_t S = TRSM_INV_BLOCK;
+	const size_t nfull = n / S, ntail = n - nfull * S;
+	const uint32_t pp = (uint32_t) p;
+	if (p > 65536 || nfull == 0)
+		return set_error(ctx, MLZ_E_ARG, "banded inverse solve needs p <= 2^16 and n >= %zu", S);
+	const size_t zbytes = round_up(nfull * S * S * sizeof(uint32_t), 256);
+	const size_t tbytes = round_up(nfull * S * (S / 4) * sizeof(uint32_t), 256);
+	void *w = nullptr;
+	MLZ_TRY(ws_get(ctx, WS_TRSM_3, zbytes + tbytes + nfull * TRSM_ZP_BYTES, &w));
+	uint32_t *Z = (uint32_t *) w;
+	uint32_t *Tb = (uint32_t *) ((char *) w + zbytes);
+	int8_t *Zp = (int8_t *) w + zbytes + tbytes;
+	MLZ_TRY(trsm_block_inverses<MLZ_U32>(ctx, pp, 1, nfull, A, lda, W, Z, Tb));
+	I8Batch bz = I8Batch::single();
+	bz.count = (int) nfull;
+	bz.outer[0] = (long long) (S * S);
+	MLZ_TRY(zp_pack_i8_a_dev(ctx, MLZ_U32, pp, bz, S, S, Z, S, Zp, nullptr));
+	if (ntail) {
+		// the last rows: their band lies inside the tail block
+		const size_t t0 = nfull * S, nb64 = (n + 63) / 64;
+		std::vector<int> ct(nb64 - t0 / 64);
+		for (size_t I = t0 / 64; I < nb64; ++I)
+			ct[I - t0 / 64] = std::min((int) n, c1_64[I]) - (int) t0;
+		MLZ_TRY(zp_trsm_blocked_i8(ctx, MLZ_U32, p, 1, ntail, nrhs, 1, A + t0 * lda + t0, lda, B + t0 * ldb, ldb,
+					   W + t0 * TRSM_LEAF, ct.data()));
+	}
+	for (size_t I = nfull; I-- > 0;) {
+		const size_t i0 = I * S, i1 = i0 + S;
+		int c1 = (int) i1;
+		for (size_t b = i0 / 64; b < i1 / 64; ++b)
+			c1 = std::max(c1, c1_64[b]);
+		c1 = std::min(c1, (int) n);
+		if ((size_t) c1 > i1)
+			MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, S, nrhs, c1 - i1, p - 1, A + i0 * lda + i1, lda, B + i1 * ldb, ldb,
+					    1, B + i0 * ldb, ldb, MLZ_ALGO_INT8));
+		MLZ_TRY(zp_gemm_i8_batched_dev(ctx, MLZ_U32, MLZ_U32, pp, I8Batch::single(), S, nrhs, S, 1, nullptr, S,
+					       B + i0 * ldb, ldb, 0, B + i0 * ldb, ldb, Zp + I * TRSM_ZP_BYTES));
+	}
+	return MLZ_OK;
+}
+
 // The solve: for p <= 2^16 (INT8 gemms) and at least two full blocks of
 // TRSM_INV_BLOCK rows, their inverses first, then the recursion with them as
 // leaves; otherwise the recursion down to the 64-row chain.

@@ -193,3 +193,14 @@ def test_fl_config4_full(gpu_ctx):
     assert rank == 20000
     print("all-row check", flush=True)
     check_rref_all_rows(R, FL.pivots, rank, csr, shape, p, seed=11)
+
+def test_fl_banded_inverse_solve(gpu_ctx):
+    """A pivot block of 2100 rows (two full 1024-row blocks inverted up
+    front, a 52-row tail on the 64-row chain) with a 600-wide band: the whole
+    reduction bit-exact against the oracle."""
+    p = 65521
+    csr, shape = synth.fl_matrix(synth.seed_for(4, 5), p, na=2100, nc=100, nb=200, band=600, nnz_a=150)
+    X = synth.csr_to_dense(csr, shape)
+    got, rank, det, piv = run_fl(gpu_ctx, p, "uint32", X)
+    rc, want, wrank, _, _ = orc.fl_echelonize(p, X)
+    assert rc == 0 and rank == wrank and np.array_equal(got, want)
