@@ -170,6 +170,32 @@ int mlz_zp_trsm(mlz_ctx *ctx, mlz_elem elem, uint64_t p, int upper, int unit_dia
 int mlz_zp_rref(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t n, void *A, size_t lda,
 		size_t *rank, uint32_t *pivots, int algo);
 
+/* Non-reduced row-echelon forms of A (m x n) in place, with LELA's
+ * transposition pivoting: step i takes the first column (from the previous
+ * pivot's on) in which a row at a position >= i is nonzero, the first such
+ * row as pivot, swaps it with position i and clears the column below
+ * (DensePivotStrategy, lela/algorithms/pivot-strategy.tcc:19-41).
+ *   MLZ_ECH_U:         Elimination::echelonize (A, P, rank, det, compute_L =
+ *                      false) (lela/algorithms/elimination.tcc:47-139): A
+ *                      becomes U, rows past the rank zero.  This is also
+ *                      EchelonForm::echelonize (A, false) for both its methods
+ *                      (echelon-form.h:113-131; tests/test-gauss-jordan.C:96-99).
+ *   MLZ_ECH_TRANSFORM: the same with compute_L = true (the method's default):
+ *                      left of the diagonal, A[t][s] = T[t][s] with T P A = U
+ *                      (T unit lower), as LELA leaves it (:121-122).
+ *   MLZ_ECH_PLUQ:      Elimination::pluq (:266-370): P A Q = L U with L's
+ *                      entries left of the diagonal in the first rank columns
+ *                      and U from the diagonal on, columns in Q's order.
+ * perm_rows[s] (s < rank, may be NULL): the position swapped with s at step s
+ * (s itself when there was no swap), so LELA's P is the list of
+ * (s, perm_rows[s]) with perm_rows[s] != s; perm_cols likewise for pluq's Q.
+ * Both in step order (pluq's own lists are these reversed, :357-358).
+ * det: the product of the pivots (:100).  MLZ_E_NOT_INVERTIBLE: a pivot has
+ * no inverse (composite p; LELA throws LELAError there, :103-104). */
+enum { MLZ_ECH_U = 0, MLZ_ECH_TRANSFORM = 1, MLZ_ECH_PLUQ = 2 };
+int mlz_zp_echelonize(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t n, void *A, size_t lda,
+		      int mode, size_t *rank, uint64_t *det, uint32_t *perm_rows, uint32_t *perm_cols, int algo);
+
 /* Faugere-Lachartre reduction of an F4-shaped matrix X (m x n, CSR: row_ptr
  * [m+1], col [nnz] strictly increasing within each row, val [nnz] in the
  * element storage) to its reduced row-echelon form R (m x n dense, ldr; may be

@@ -24,7 +24,7 @@ from ._lib import (ALGO_AUTO, ALGO_FP64, ALGO_INT8, ALGO_PLAIN, MLZ_F64_SYM, MLZ
 __all__ = [
     "Modular", "Context", "BLAS3", "DeviceMatrix", "LELAError", "DiagonalEntryNotInvertible",
     "WrongMatrixForm", "LowerTriangular", "UpperTriangular", "ALGO_AUTO", "ALGO_PLAIN",
-    "ALGO_INT8", "ALGO_FP64", "EchelonForm", "GF2", "Dense01Matrix", "FaugereLachartre",
+    "ALGO_INT8", "ALGO_FP64", "EchelonForm", "Elimination", "GF2", "Dense01Matrix", "FaugereLachartre",
     "dense_to_csr",
 ]
 
@@ -291,8 +291,11 @@ class EchelonForm:
                 raise LELAError("only the reduced form is computed on the GPU")
             return _gf2_echelonize(self, A)
         if not reduced:
-            raise LELAError("only the reduced form is computed on the GPU "
-                            "(the non-reduced form depends on LELA's pivot order)")
+            # echelon-form.h:113-131: both methods give Elimination's U
+            _, rank, _ = Elimination(self.ctx).echelonize(A, compute_L=False, algo=algo)
+            self._rank_table[id(A)] = rank
+            self.pivots = None
+            return A
         ctx, F = self.ctx, self.ctx.F
         pa, lda, m, n = _operand(ctx, A, F.dtype)
         rank = ctypes.c_size_t(0)
@@ -306,6 +309,46 @@ class EchelonForm:
 
     def rank(self, A):
         return self._rank_table[id(A)]
+
+
+class Elimination:
+    """Elimination<Ring, HIPModule> (lela/algorithms/elimination.h:34-238) for
+    dense Z/p matrices with LELA's DensePivotStrategy.  Permutations are
+    LELA's: lists of transpositions (i, j)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def _run(self, A, mode, algo):
+        ctx, F = self.ctx, self.ctx.F
+        pa, lda, m, n = _operand(ctx, A, F.dtype)
+        rank = ctypes.c_size_t(0)
+        det = ctypes.c_uint64(0)
+        pr = np.zeros(max(1, min(m, n)), dtype=np.uint32)
+        pc = np.zeros(max(1, min(m, n)), dtype=np.uint32)
+        rc = ctx.lib.mlz_zp_echelonize(ctx.handle, F.elem, F.p, m, n, pa, lda, mode, ctypes.byref(rank),
+                                       ctypes.byref(det), pr.ctypes.data, pc.ctypes.data, algo)
+        if rc == _lib.MLZ_E_NOT_INVERTIBLE:
+            raise LELAError("Could not invert pivot-element in the ring")
+        ctx.check(rc, "echelonize")
+        r = rank.value
+        return pr[:r].copy(), pc[:r].copy(), r, int(det.value)
+
+    def echelonize(self, A, compute_L=True, algo=ALGO_AUTO):
+        """A <- its row-echelon form U (elimination.tcc:47-139); with
+        compute_L (LELA's default) the part left of the diagonal holds T with
+        T P A = U.  Returns (P, rank, det)."""
+        pr, _, r, det = self._run(A, _lib.ECH_TRANSFORM if compute_L else _lib.ECH_U, algo)
+        self.steps = pr
+        return [(i, int(j)) for i, j in enumerate(pr) if j != i], r, det
+
+    def pluq(self, A, algo=ALGO_AUTO):
+        """A <- L and U of P A Q = L U (elimination.tcc:266-370).  Returns
+        (P, Q, rank, det) with LELA's lists (reversed step order)."""
+        pr, pc, r, det = self._run(A, _lib.ECH_PLUQ, algo)
+        P = [(i, int(j)) for i, j in enumerate(pr) if j != i][::-1]
+        Q = [(i, int(j)) for i, j in enumerate(pc) if j != i][::-1]
+        return P, Q, r, det
 
 
 class GF2:

@@ -375,6 +375,11 @@ int zp_trsm_banded_inv_i8(mlz_ctx *ctx, uint64_t p, size_t n, size_t nrhs, const
 			  uint32_t *B, size_t ldb, const uint32_t *W, const int *c1_64);
 int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A, size_t lda,
 		size_t *rank_out, uint32_t *pivots_out, int algo);
+// Non-reduced echelon forms with LELA's transposition pivoting (zp_lu.hip).
+int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A, size_t lda, int mode,
+		   size_t *rank_out, uint64_t *det_out, uint32_t *perm_rows, uint32_t *perm_cols, int algo);
+// Table of inverses mod p <= 2^16 in device memory (0 where none exists).
+int zp_inverse_table(mlz_ctx *ctx, const ModP &mp, const uint16_t **out);
 int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda,
 		 const uint64_t *B, size_t ldb, int b, uint64_t *C, size_t ldc);
 int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size_t *rank_out,

@@ -425,6 +425,11 @@ static int get_invtab(mlz_ctx *ctx, const ModP &mp, const uint16_t **out)
 	return MLZ_OK;
 }
 
+int zp_inverse_table(mlz_ctx *ctx, const ModP &mp, const uint16_t **out)
+{
+	return get_invtab(ctx, mp, out);
+}
+
 __device__ inline int nodes_at(int nodes0, int level)
 {
 	int n = nodes0;

@@ -295,6 +295,118 @@ int orc_zp_echelonize(uint32_t p, size_t m, size_t n, uint32_t *A, size_t lda, i
 	return ORC_OK;
 }
 
+/* Elimination::echelonize (lela/algorithms/elimination.tcc:47-139) with the
+ * row transpositions: step i takes the pivot of DensePivotStrategy
+ * (pivot-strategy.tcc:19-41), swaps rows i and pivot_row (:92-97), and for
+ * every row j below with a nonzero a in the pivot column adds -a/x times the
+ * pivot row to it over the whole row (:112-120).  compute_L (the method's
+ * default): A.setEntry (j, i, -a/x) after that axpy (:121-122), so the part
+ * left of each row's pivot accumulates the transform T with T P A = U.
+ * perm_rows[i] = pivot_row of step i (i itself when no swap); det = the
+ * product of the pivots (:100). */
+int orc_zp_elim(uint32_t p, size_t m, size_t n, uint32_t *A, size_t lda, int compute_L, size_t *rank,
+		uint32_t *det, uint32_t *perm_rows)
+{
+	if (p < 2)
+		return ORC_E_ARG;
+	size_t r = 0, i = 0, pivot_col = 0;
+	uint32_t d = 1 % p;
+	for (; i < m && pivot_col < n; ++i, ++pivot_col) {
+		size_t pivot_row = i;
+		if (!dense_pivot(A, lda, m, n, &pivot_row, &pivot_col))
+			break;
+		if (i != pivot_row)
+			swap_rows_u32(A, lda, n, i, pivot_row);
+		if (perm_rows)
+			perm_rows[i] = (uint32_t) pivot_row;
+		const uint32_t x = A[i * lda + pivot_col];
+		d = zp_mul(p, d, x);
+		uint32_t xinv;
+		if (!orc_inv(p, x, &xinv))
+			return ORC_E_NOT_INVERTIBLE;
+		const uint32_t negxinv = zp_neg(p, xinv);
+		const uint32_t *Pi = A + i * lda;
+		for (size_t j = i + 1; j < m; ++j) {
+			uint32_t *Rj = A + j * lda;
+			const uint32_t aj = Rj[pivot_col];
+			if (aj == 0)
+				continue;
+			const uint32_t c = zp_mul(p, aj, negxinv);
+			for (size_t col = 0; col < n; ++col)
+				Rj[col] = zp_axpy(p, c, Pi[col], Rj[col]);
+			if (compute_L)
+				Rj[i] = c;
+		}
+		++r;
+	}
+	if (rank)
+		*rank = r;
+	if (det)
+		*det = d;
+	return ORC_OK;
+}
+
+static void swap_cols_u32(uint32_t *A, size_t lda, size_t m, size_t i, size_t j)
+{
+	for (size_t r = 0; r < m; ++r) {
+		uint32_t t = A[r * lda + i];
+		A[r * lda + i] = A[r * lda + j];
+		A[r * lda + j] = t;
+	}
+}
+
+/* Elimination::pluq (lela/algorithms/elimination.tcc:266-370): step i takes
+ * the pivot of DensePivotStrategy from (i, i), swaps rows i and pivot_row and
+ * columns i and pivot_col (whole rows and columns, :298-309), and for every
+ * row j below with a nonzero a in column i adds -a/x times the pivot row to it
+ * on the columns right of i (:321-331), then stores L's entry a/x in column i
+ * (:333-335).  perm_rows[i] / perm_cols[i] = the row / column swapped with i
+ * at step i, in step order (LELA reverses both lists at the end, :357-358). */
+int orc_zp_pluq(uint32_t p, size_t m, size_t n, uint32_t *A, size_t lda, size_t *rank, uint32_t *det,
+		uint32_t *perm_rows, uint32_t *perm_cols)
+{
+	if (p < 2)
+		return ORC_E_ARG;
+	size_t r = 0;
+	uint32_t d = 1 % p;
+	for (size_t i = 0; i < m && i < n; ++i) {
+		size_t pivot_row = i, pivot_col = i;
+		if (!dense_pivot(A, lda, m, n, &pivot_row, &pivot_col))
+			break;
+		if (i != pivot_row)
+			swap_rows_u32(A, lda, n, i, pivot_row);
+		if (i != pivot_col)
+			swap_cols_u32(A, lda, m, i, pivot_col);
+		if (perm_rows)
+			perm_rows[i] = (uint32_t) pivot_row;
+		if (perm_cols)
+			perm_cols[i] = (uint32_t) pivot_col;
+		const uint32_t x = A[i * lda + i];
+		d = zp_mul(p, d, x);
+		uint32_t xinv;
+		if (!orc_inv(p, x, &xinv))
+			return ORC_E_NOT_INVERTIBLE;
+		const uint32_t negxinv = zp_neg(p, xinv);
+		const uint32_t *Pi = A + i * lda;
+		for (size_t j = i + 1; j < m; ++j) {
+			uint32_t *Rj = A + j * lda;
+			const uint32_t aj = Rj[i];
+			if (aj == 0)
+				continue;
+			const uint32_t c = zp_mul(p, aj, negxinv);
+			for (size_t col = i + 1; col < n; ++col)
+				Rj[col] = zp_axpy(p, c, Pi[col], Rj[col]);
+			Rj[i] = zp_neg(p, c);
+		}
+		++r;
+	}
+	if (rank)
+		*rank = r;
+	if (det)
+		*det = d;
+	return ORC_OK;
+}
+
 /* ------------------------------------------------------------------ */
 /* GF(2)                                                               */
 /* ------------------------------------------------------------------ */

@@ -66,6 +66,13 @@ int orc_gf2_gemm(size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t 
 /* GF(2) Elimination::echelonize[_reduced] with the Dense01 pivot search
  * (lela/algorithms/pivot-strategy.tcc:44-67) and word-XOR axpy
  * (lela/blas/level1-gf2.tcc:282-299). */
+/* Elimination::echelonize with the transposition list and compute_L
+ * (elimination.tcc:47-139); perm_rows[i] = pivot row of step i. */
+int orc_zp_elim(uint32_t p, size_t m, size_t n, uint32_t *A, size_t lda, int compute_L, size_t *rank,
+		uint32_t *det, uint32_t *perm_rows);
+/* Elimination::pluq (elimination.tcc:266-370); per-step row and column swaps. */
+int orc_zp_pluq(uint32_t p, size_t m, size_t n, uint32_t *A, size_t lda, size_t *rank, uint32_t *det,
+		uint32_t *perm_rows, uint32_t *perm_cols);
 int orc_gf2_echelonize(size_t m, size_t n, uint64_t *A, size_t lda_w, int reduced,
 		       size_t *rank, uint32_t *pivots);
 

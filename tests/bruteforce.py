@@ -90,6 +90,85 @@ def is_rref(R, p):
     return pivots
 
 
+def lela_pivot_sequence(A, p):
+    """The pivots of LELA's transposition pivoting (first nonzero, columns left
+    to right), characterised without eliminating A itself: at step i the pivot
+    column is the first column c (after the previous pivot's) for which some
+    row at a position >= i is, restricted to columns 0..c, outside the span of
+    the pivot rows so far restricted the same way, and the pivot row is the
+    first such position; its row then swaps with position i.  Ranks come from
+    rref_mod (a different elimination order).  Returns (steps, order, cols):
+    steps[i] = pivot position of step i, order = final row order, cols =
+    pivot columns."""
+    A = [[int(x) % p for x in row] for row in np.asarray(A)]
+    m = len(A)
+    n = len(A[0]) if m else 0
+    order = list(range(m))
+    steps, cols = [], []
+    i, c = 0, 0
+
+    def rank(rows):
+        return rref_mod(np.array(rows, dtype=object).reshape(len(rows), -1), p)[1] if rows else 0
+
+    while i < m and c < n:
+        found = None
+        for col in range(c, n):
+            base = [A[order[t]][:col + 1] for t in range(i)]
+            rb = rank(base)
+            for pos in range(i, m):
+                if rank(base + [A[order[pos]][:col + 1]]) > rb:
+                    found = (col, pos)
+                    break
+            if found:
+                break
+        if not found:
+            break
+        col, pos = found
+        steps.append(pos)
+        cols.append(col)
+        order[i], order[pos] = order[pos], order[i]
+        i += 1
+        c = col + 1
+    return steps, order, cols
+
+
+def echelon_given_order(A, order, cols, p):
+    """The echelon form U fixed by the row order and pivot columns: U[t] is the
+    unique vector of (PA)[t] + span(U[0..t-1]) that is zero on cols[0..t-1]
+    (a closed form: subtract (PA)[t][J] (U_J)^-1 U with J the earlier pivot
+    columns); rows past the rank are zero."""
+    PA = np.array([[int(x) % p for x in np.asarray(A)[r]] for r in order], dtype=object).reshape(len(order), -1)
+    m, n = PA.shape
+    U = np.zeros((m, n), dtype=object)
+    for t in range(len(cols)):
+        row = PA[t].copy()
+        if t:
+            J = cols[:t]
+            UJ = U[:t][:, J]
+            inv = inverse_mod(UJ, p)
+            coef = (PA[t, J].dot(inv)) % p
+            row = (row - coef.dot(U[:t])) % p
+        U[t] = row
+    return U
+
+
+def inverse_mod(M, p):
+    """Inverse of a square matrix mod prime p (Gauss-Jordan on [M | I])."""
+    M = [[int(x) % p for x in row] for row in np.asarray(M)]
+    n = len(M)
+    aug = [M[i] + [int(i == j) for j in range(n)] for i in range(n)]
+    for c in range(n):
+        piv = next(i for i in range(c, n) if aug[i][c])
+        aug[c], aug[piv] = aug[piv], aug[c]
+        inv = pow(aug[c][c], -1, p)
+        aug[c] = [(x * inv) % p for x in aug[c]]
+        for i in range(n):
+            if i != c and aug[i][c]:
+                f = aug[i][c]
+                aug[i] = [(x - f * y) % p for x, y in zip(aug[i], aug[c])]
+    return np.array([row[n:] for row in aug], dtype=object).reshape(n, n)
+
+
 # ----------------------------------------------------------------- GF(2)
 
 def gf2_unpack(W, ncols):

@@ -32,6 +32,8 @@ def load():
     lib.orc_zp_trsm.argtypes = [u32, i, i, _sz, _sz, u32, _vp, _sz, _vp, _sz, ctypes.POINTER(_sz)]
     lib.orc_zp_echelonize.argtypes = [u32, _sz, _sz, _vp, _sz, i, ctypes.POINTER(_sz), _vp,
                                       ctypes.POINTER(u32)]
+    lib.orc_zp_elim.argtypes = [u32, _sz, _sz, _vp, _sz, i, ctypes.POINTER(_sz), ctypes.POINTER(u32), _vp]
+    lib.orc_zp_pluq.argtypes = [u32, _sz, _sz, _vp, _sz, ctypes.POINTER(_sz), ctypes.POINTER(u32), _vp, _vp]
     lib.orc_gf2_gemm.argtypes = [_sz, _sz, _sz, i, _vp, _sz, _vp, _sz, i, _vp, _sz]
     lib.orc_gf2_echelonize.argtypes = [_sz, _sz, _vp, _sz, i, ctypes.POINTER(_sz), _vp]
     lib.orc_fl_echelonize.argtypes = [u32, _sz, _sz, _vp, _sz, _vp, _sz, ctypes.POINTER(_sz),
@@ -40,7 +42,8 @@ def load():
     lib.orc_mlp_forward.argtypes = [_vp, _vp, _sz, _vp]
     lib.orc_mlp_loss_grad.argtypes = [_vp, _vp, _vp, _vp, _sz, _vp, ctypes.POINTER(ctypes.c_double)]
     lib.orc_mlp_adam.argtypes = [_vp, _vp, _vp, _vp, _sz, i, ctypes.c_double, _vp]
-    for name in ("orc_zp_gemm_u32", "orc_zp_gemm_u16", "orc_zp_trsm", "orc_zp_echelonize",
+    for name in ("orc_zp_gemm_u32", "orc_zp_gemm_u16", "orc_zp_trsm", "orc_zp_echelonize", "orc_zp_elim",
+                 "orc_zp_pluq",
                  "orc_gf2_gemm", "orc_gf2_echelonize", "orc_fl_echelonize", "orc_inv",
                  "orc_mlp_forward", "orc_mlp_loss_grad", "orc_mlp_adam"):
         getattr(lib, name).restype = ctypes.c_int
@@ -87,6 +90,33 @@ def zp_echelonize(p, A, reduced=True):
     rc = lib.orc_zp_echelonize(p, m, n, R.ctypes.data, n, int(reduced), ctypes.byref(rank),
                                piv.ctypes.data, ctypes.byref(det))
     return rc, R, rank.value, piv[:rank.value].copy(), det.value
+
+
+def zp_elim(p, A, compute_L=False):
+    """Returns (rc, A_out, rank, det, perm_rows) of Elimination::echelonize."""
+    lib = load()
+    R = _c(A, np.uint32).copy()
+    m, n = R.shape
+    rank = _sz(0)
+    det = ctypes.c_uint32(0)
+    perm = np.zeros(max(1, min(m, n)), dtype=np.uint32)
+    rc = lib.orc_zp_elim(p, m, n, R.ctypes.data, n, int(compute_L), ctypes.byref(rank), ctypes.byref(det),
+                         perm.ctypes.data)
+    return rc, R, rank.value, det.value, perm[:rank.value].copy()
+
+
+def zp_pluq(p, A):
+    """Returns (rc, A_out, rank, det, perm_rows, perm_cols) of Elimination::pluq (step order)."""
+    lib = load()
+    R = _c(A, np.uint32).copy()
+    m, n = R.shape
+    rank = _sz(0)
+    det = ctypes.c_uint32(0)
+    pr = np.zeros(max(1, min(m, n)), dtype=np.uint32)
+    pc = np.zeros(max(1, min(m, n)), dtype=np.uint32)
+    rc = lib.orc_zp_pluq(p, m, n, R.ctypes.data, n, ctypes.byref(rank), ctypes.byref(det), pr.ctypes.data,
+                         pc.ctypes.data)
+    return rc, R, rank.value, det.value, pr[:rank.value].copy(), pc[:rank.value].copy()
 
 
 def gf2_gemm(m, n, k, a, A, B, b, C):

@@ -255,3 +255,101 @@ def test_full_size_checks_accept_rref_and_reject_planted_errors():
     badg[3, freeg[2] // 64] ^= np.uint64(1 << (freeg[2] % 64))
     with pytest.raises(AssertionError):
         check_gf2_rref_all_rows(M, badg, pv, rk, n, seed=4)
+
+
+def _echelon_inputs():
+    """Dense, rank-deficient, zero-column and sparse small cases."""
+    out = []
+    for seed, (m, n, r), p in [(1, (12, 15, 12), 101), (2, (20, 14, 9), 65521), (3, (9, 30, 6), 7),
+                               (4, (16, 16, 16), 2147483647), (5, (25, 18, 11), 3)]:
+        out.append((p, _lowrank(seed, m, n, r, p)))
+    rng = np.random.default_rng(9)
+    Z = rng.integers(0, 101, size=(14, 20)).astype(np.uint32)
+    Z[:, [0, 3, 7]] = 0                          # zero columns, including the first
+    Z[5] = 0
+    out.append((101, Z))
+    S = np.where(rng.random((18, 22)) < 0.15, rng.integers(1, 65521, size=(18, 22)), 0).astype(np.uint32)
+    out.append((65521, S))                        # sparse: first nonzeros far down the rows
+    return out
+
+
+@pytest.mark.parametrize("case", range(7))
+def test_elim_matches_independent_pivot_rule(case):
+    """orc_zp_elim (Elimination::echelonize) against the independent
+    characterisation of LELA's pivots and the closed form of U given them:
+    the same transposition sequence, rank and pivot columns, U, det, zero
+    rows past the rank."""
+    p, A = _echelon_inputs()[case]
+    rc, U, rank, det, perm = orc.zp_elim(p, A, compute_L=False)
+    assert rc == orc.ORC_OK
+    steps, order, cols = bf.lela_pivot_sequence(A, p)
+    assert rank == len(steps) and list(perm) == steps
+    want = bf.echelon_given_order(A, order, cols, p)
+    assert np.array_equal(U.astype(object), want)
+    d = 1
+    for t, c in enumerate(cols):
+        d = d * int(want[t, c]) % p
+    assert det == d % p
+    # the reduced form of U is the canonical RREF of A
+    R, r2, piv = bf.rref_mod(U, p)
+    R0, r0, piv0 = bf.rref_mod(A, p)
+    assert r2 == r0 == rank and piv == piv0 == cols and np.array_equal(R, R0)
+
+
+@pytest.mark.parametrize("case", range(7))
+def test_elim_compute_L_is_the_transform(case):
+    """compute_L: the strict lower part of the output is T with T P A = U (T
+    unit lower), and the upper part equals the compute_L = false result
+    (tests/test-gauss-jordan.C:96-99 compares the two forms the same way)."""
+    p, A = _echelon_inputs()[case]
+    rc, O, rank, det, perm = orc.zp_elim(p, A, compute_L=True)
+    rc0, U0, rank0, det0, perm0 = orc.zp_elim(p, A, compute_L=False)
+    assert rc == rc0 == orc.ORC_OK and rank == rank0 and det == det0 and list(perm) == list(perm0)
+    m, n = A.shape
+    O = O.astype(object)
+    T = np.zeros((m, m), dtype=object)
+    U = O.copy()
+    for t in range(m):
+        T[t, t] = 1
+        for s_ in range(min(t, n)):
+            T[t, s_] = O[t, s_]
+            U[t, s_] = 0
+    assert np.array_equal(U, U0.astype(object))
+    order = list(range(m))
+    for i, q in enumerate(perm):
+        order[i], order[q] = order[q], order[i]
+    PA = np.asarray(A).astype(object)[order]
+    assert np.array_equal(T.dot(PA) % p, U)
+
+
+@pytest.mark.parametrize("case", range(7))
+def test_pluq_factors(case):
+    """orc_zp_pluq: P A Q = L U with L unit lower (A's part left of the
+    diagonal, first rank columns) and U upper (A's part from the diagonal on,
+    first rank rows); its pivots are Elimination::echelonize's (the same rows,
+    the same columns in the original numbering)."""
+    p, A = _echelon_inputs()[case]
+    rc, O, rank, det, pr, pc = orc.zp_pluq(p, A)
+    assert rc == orc.ORC_OK
+    m, n = A.shape
+    rc2, _, rank2, det2, perm2 = orc.zp_elim(p, A)
+    steps, order, cols = bf.lela_pivot_sequence(A, p)
+    assert rank == rank2 and det == det2 and list(pr) == list(perm2)
+    corder = list(range(n))
+    for i, q in enumerate(pc):
+        corder[i], corder[q] = corder[q], corder[i]
+    assert corder[:rank] == cols
+    rorder = list(range(m))
+    for i, q in enumerate(pr):
+        rorder[i], rorder[q] = rorder[q], rorder[i]
+    O = O.astype(object)
+    L = np.zeros((m, rank), dtype=object)
+    U = np.zeros((rank, n), dtype=object)
+    for t in range(m):
+        for s_ in range(min(t, rank)):
+            L[t, s_] = O[t, s_]
+        if t < rank:
+            L[t, t] = 1
+            U[t, t:] = O[t, t:]
+    PAQ = np.asarray(A).astype(object)[rorder][:, corder]
+    assert np.array_equal(L.dot(U) % p, PAQ % p)
