@@ -30,6 +30,7 @@ struct ModP {
 	double pinv;     // 1/p
 	int small;       // p < 2^26: products are exact in a double
 	int prime;       // p is prime: inverses by Fermat
+	float pinvf;     // 1/p in single precision (mul16)
 
 	__host__ __device__ inline uint64_t reduce(uint64_t x) const
 	{
@@ -53,6 +54,45 @@ struct ModP {
 		double r = fma(-q, (double) p, prod);
 		r = r < 0.0 ? r + (double) p : r;
 		r = r >= (double) p ? r - (double) p : r;
+		return (uint32_t) r;
+	}
+	// (x*y) mod p for x, y < p <= 2^16: the product is below 2^32 (exact in
+	// 24-bit integer multiplies), and the quotient estimate from single
+	// precision has relative error below 2^-22 of a quotient below 2^16, so
+	// it is off by at most one; the remainder, below 2^17 in magnitude, is
+	// exact in 32-bit two's complement.
+	__host__ __device__ inline uint32_t mul16(uint32_t x, uint32_t y) const
+	{
+		const uint32_t pp = (uint32_t) p;
+#if defined(__HIP_DEVICE_COMPILE__)
+		const uint32_t prod = __umul24(x, y);
+		const uint32_t q = (uint32_t) ((float) prod * pinvf);
+		int32_t r = (int32_t) (prod - __umul24(q, pp));
+#else
+		const uint32_t prod = x * y;
+		const uint32_t q = (uint32_t) ((float) prod * pinvf);
+		int32_t r = (int32_t) (prod - q * pp);
+#endif
+		r += r < 0 ? (int32_t) pp : 0;
+		r -= r >= (int32_t) pp ? (int32_t) pp : 0;
+		return (uint32_t) r;
+	}
+	// (f*g + x) mod p for f, x < p and g <= p <= 2^16: the argument is below
+	// p^2, so the same single-precision quotient applies.
+	__host__ __device__ inline uint32_t mad16(uint32_t x, uint32_t f, uint32_t g) const
+	{
+		const uint32_t pp = (uint32_t) p;
+#if defined(__HIP_DEVICE_COMPILE__)
+		const uint32_t v = __umul24(f, g) + x;
+		const uint32_t q = (uint32_t) ((float) v * pinvf);
+		int32_t r = (int32_t) (v - __umul24(q, pp));
+#else
+		const uint32_t v = f * g + x;
+		const uint32_t q = (uint32_t) ((float) v * pinvf);
+		int32_t r = (int32_t) (v - q * pp);
+#endif
+		r += r < 0 ? (int32_t) pp : 0;
+		r -= r >= (int32_t) pp ? (int32_t) pp : 0;
 		return (uint32_t) r;
 	}
 	__host__ __device__ inline uint32_t mul(uint32_t x, uint32_t y) const
@@ -186,6 +226,7 @@ inline ModP make_modp(uint64_t p)
 	m.p = p;
 	m.mu = ~(uint64_t) 0 / p;
 	m.pinv = 1.0 / (double) p;
+	m.pinvf = (float) (1.0 / (double) p);
 	m.small = p < (1ull << 26);
 	m.prime = is_prime_u32(p);
 	return m;

@@ -7,36 +7,36 @@
 // step i takes the first column (from the previous pivot's on) in which some
 // row at a position >= i is nonzero, the first such row as pivot, and swaps
 // it with position i.  Unlike the reduced form, the result depends on this
-// order, so it is reproduced exactly; the arithmetic is blocked:
+// order, so it is reproduced exactly; the arithmetic is blocked.
 //
-// Rows stay where they are in a working copy; perm[pos] names the physical
-// row at each position, and a transposition swaps two entries of perm.  Per
-// panel of up to LU_W columns from column c at position i:
+// The working copy is kept in position order (a transposition swaps two rows
+// of it).  Per panel of up to LU_W columns from column c at position i:
 //  1. one workgroup loads the panel part of the LU_K rows at positions
-//     i .. i+LU_K-1 (the window), augmented with the identity, and runs the
-//     elimination above on them column by column (the first nonzero row of
-//     the window is the first nonzero row overall, because the window holds
-//     the first positions in order).  The identity part becomes T, the
-//     transform of the window rows: the new pivot rows are T_piv X_w.  A
-//     backward pass over the pivot rows turns T_piv into M = U11^-1 T_piv, so
-//     that W = M X_w = P_J^-1 P (P: the pivot rows on entry, J: their
-//     columns) and V = U11^-1.
-//  2. [T_piv; M] X_w is one gemm over the columns from c on; every row below
-//     the new pivots (window or not) becomes X_j - X_j[J] W, one rank-LU_W
-//     update of all rows (zero multipliers elsewhere); the pivot rows receive
-//     T_piv X_w.  This is the unique result of the row operations of the
-//     column-by-column elimination: every row below changes by a combination
-//     of pivot rows that clears the pivot columns, and that combination is
-//     X_j[J] P_J^-1.
-//  3. If a column of the panel has no nonzero in the window (and rows below
+//     i .. i+LU_K-1 (the window) and runs the elimination above on them,
+//     column by column: the first nonzero row of the window is the first
+//     nonzero row overall, because the window holds the first positions in
+//     order.  It returns the pivots, the window's new order, L11 (the
+//     multipliers among the panel's pivot rows) and U11 (the pivot rows on
+//     the pivot columns after elimination).
+//  2. The window rows are put in that order.  With P the panel's pivot rows
+//     as they stood (now at positions i ..), one thread per column computes
+//     U = L11^-1 P (forward substitution: the new pivot rows) and
+//     W = U11^-1 U = P_J^-1 P (back substitution).
+//  3. Every row below the new pivots becomes X_j - X_j[J] W: one rank-LU_W
+//     update of the contiguous rows below.  This is the unique result of the
+//     column-by-column elimination's row operations: each row below changes by
+//     the combination of pivot rows that clears the pivot columns, X_j[J]
+//     P_J^-1.
+//  4. If a column of the panel has no nonzero in the window (and rows below
 //     it exist), the panel ends there; after its update a search over every
 //     remaining row finds the next pivot (first column, then first row), and
 //     its transposition is made before the next panel.
-// compute_L: the working copy carries [A | I_m]; the augmented part of row
-// perm[t] at column perm[s] is T[t][s] of T P A = U, the value LELA leaves in
-// A[t][s] left of the pivots.  pluq: the multipliers of the rows below are
-// X_j[J] V (one more product per panel), the pivot rows' own from the window
-// elimination, stored per physical row; the column order comes from the
+// compute_L: the working copy carries [A | I_m]; with orig[t] the original
+// index of the row at position t, the augmented part of row t at column
+// orig[s] is T[t][s] of T P A = U, the value LELA leaves in A[t][s] left of
+// the pivots.  pluq: the multipliers of the rows below are X_j[J] U11^-1 (one
+// thread per row), the pivot rows' own are L11; they are kept in a buffer
+// whose rows move with the working copy's; the column order comes from the
 // pivot columns.
 #include "common.hpp"
 
@@ -45,279 +45,564 @@
 namespace mlz {
 
 constexpr int LU_W = 64;     // panel width (columns)
-constexpr int LU_K = 128;    // window rows
-constexpr int LU_NT = 1024;  // threads of the window kernel
+constexpr int LU_K = 64;     // window rows (one per lane)
+constexpr int LU_NT = 512;   // threads of the window kernel: 8 waves of 8 panel columns
 constexpr int LU_SEARCH_COLS = 2048;   // columns per launch of the pivot search
+constexpr uint32_t LU_EV_NONE = ~0u;    // a column without a nonzero in the window
+constexpr int LU_SPIN_MAX = 1 << 22;   // polls of a column's flag before the kernel reports an error
 
 // Device outputs of one window kernel.
 struct LuPanel {
 	int rp;            // pivots found
 	int stop;          // 1: a column of the panel needs the rows beyond the window
 	int stop_col;      // that column (absolute)
-	int bad;           // a pivot had no inverse (composite modulus)
+	int bad;           // 1: a pivot had no inverse (composite modulus); 2: a wave stopped waiting
 	int steps[LU_W];   // window position of step s's pivot row (before its swap)
 	int J[LU_W];       // pivot column of step s (absolute)
 	uint32_t x[LU_W];  // pivot value of step s
+	uint32_t dinv[LU_W];   // its inverse
 	int wperm[LU_K];   // window order on exit: original window index at each position
 };
 
 // One panel: window rows i0 .. i0+kw-1 (kw = min(LU_K, m - i0)), columns
-// c .. c+wc-1 (pivots searched in them only).  Outputs: TM rows 0..63 =
-// T_piv, rows 64..127 = M (LU_K columns each, zero past rp), V = U11^-1
-// (LU_W x LU_W), F[orig window index][s] = the multiplier a/x of step s.
+// c .. c+wc-1 (pivots searched in them only).  Outputs, with rp pivots and P
+// the panel's pivot rows as they stood (in step order), LU_W x LU_W each and
+// zero outside the first rp rows:
+//   LiM rows 0 .. 63:   L11^-1, the combination of P giving each new pivot row;
+//   LiM rows 64 .. 127: P_J^-1 (P on the pivot columns J, inverted);
+//   L11: unit lower triangular, the multipliers among the pivot rows (L11[t][s],
+//        s < t: the multiple of new pivot row s subtracted from pivot row t),
+//        so that P_J = L11 U11 with U11 the new pivot rows on J.
+//
+// Lane l holds the row at window position l, wave w the panel columns 8w ..
+// 8w+7.  Column cc: the wave holding it takes the first lane >= cur with a
+// nonzero there (one ballot), the pivot's inverse (from an LDS copy of the
+// table when p <= 2^16), stores the multipliers and the column's event, then
+// sets the column's flag.  Every other wave waits for that flag only (not for
+// each other), exchanges the two rows if the pivot is not at cur (lanes cur
+// and first: the transposition), reads the pivot row from lane cur and
+// updates its columns from cc on.  Nothing in LDS is rewritten within a
+// panel while a wave may still read it: each column has its own event, each
+// step its own multipliers, and the transpositions' effect on the stored
+// multipliers is replayed afterwards.  Then L11^-1 and P_J^-1 = U11^-1 L11^-1
+// come from a blocked substitution over 8 x 8 blocks with all threads: the
+// diagonal blocks are inverted first, then each block row is one product.
+template <bool SMALL, bool TAB>
 __global__ __launch_bounds__(LU_NT) void lu_window_kernel(const uint32_t *__restrict__ X, size_t ldx, int m, int c,
-							  int wc, int i0, const int *__restrict__ perm, ModP mp,
-							  const uint16_t *__restrict__ invtab, LuPanel *__restrict__ out,
-							  uint32_t *__restrict__ TM, uint32_t *__restrict__ V,
-							  uint32_t *__restrict__ F)
+							  int wc, int i0, ModP mp, const uint16_t *__restrict__ invtab,
+							  LuPanel *__restrict__ out, uint32_t *__restrict__ L11,
+							  uint32_t *__restrict__ LiM)
 {
-	__shared__ uint32_t Xl[LU_K][LU_W];
-	__shared__ uint32_t Tl[LU_K][LU_K];
-	__shared__ uint32_t Il[LU_W][LU_W];
-	__shared__ int ord[LU_K];
-	__shared__ uint32_t fv[LU_K];
-	__shared__ uint32_t sinv[LU_W];
-	__shared__ int jl[LU_W];
-	__shared__ int s_first, s_bad;
-	const int tid = threadIdx.x;
+	using FT = typename std::conditional<TAB, uint16_t, uint32_t>::type;
+	constexpr int FP = TAB ? LU_W + 2 : LU_W + 1;       // padded row of Lm (odd word stride)
+	constexpr int XP = LU_W + 4;                        // padded rows of the post-pass matrices
+	constexpr int MAT = LU_W * XP;                      // words per matrix
+	constexpr size_t POST = (size_t) (5 * MAT + 8 * XP + 2 * 512) * 4;
+	constexpr size_t BIG = TAB ? (65536 * 2 > POST ? 65536 * 2 : POST) : POST;
+	__shared__ __align__(16) unsigned char big[BIG];    // the inverse table, then the post-pass
+	__shared__ FT Lm[LU_K * FP];                        // multipliers by position and step
+	__shared__ uint32_t evc[LU_W][2];                   // per column: pivot lane (or none), inverse
+	__shared__ int ready[LU_W];                         // per column: its event is published
+	__shared__ uint32_t sdinv[LU_W], sx[LU_W];
+	__shared__ int ord[LU_K], jl[LU_W], sfirst[LU_W];
+	const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
 	const int kw = min(LU_K, m - i0);
 	const bool covers = i0 + LU_K >= m;
-	for (int idx = tid; idx < LU_K * LU_W; idx += LU_NT) {
-		const int q = idx / LU_W, cc = idx % LU_W;
-		Xl[q][cc] = (q < kw && cc < wc) ? X[(size_t) perm[i0 + q] * ldx + c + cc] : 0u;
+	const uint32_t pp = (uint32_t) mp.p;
+	auto mul = [&](uint32_t a, uint32_t b) -> uint32_t { return TAB ? mp.mul16(a, b) : mp.mulT<SMALL>(a, b); };
+	auto upd = [&](uint32_t a, uint32_t f, uint32_t pv) -> uint32_t {     // a - f pv
+		return TAB ? mp.mad16(a, f, pp - pv) : mp.sub(a, mp.mulT<SMALL>(f, pv));
+	};
+	uint32_t x[8];
+#pragma unroll
+	for (int jj = 0; jj < 8; ++jj) {
+		const int col = 8 * w + jj;
+		x[jj] = (l < kw && col < wc) ? X[(size_t) (i0 + l) * ldx + c + col] : 0u;
 	}
-	for (int idx = tid; idx < LU_K * LU_K; idx += LU_NT) {
-		const int q = idx / LU_K, r = idx % LU_K;
-		Tl[q][r] = (q == r && q < kw) ? 1u : 0u;
-	}
-	if (tid < LU_K)
+	if (tid < LU_K) {
 		ord[tid] = tid;
-	if (tid == 0)
-		s_bad = 0;
-	__syncthreads();
-	int cur = 0, stop = 0, stop_col = 0;
-	for (int cc = 0; cc < wc; ++cc) {
-		if (cur == kw) {             // every window row is a pivot row
-			if (!covers) {
-				stop = 1;
-				stop_col = c + cc;
-			}
-			break;
+		ready[tid] = 0;
+	}
+	if (TAB) {   // the table's first p entries, 16 bytes per load, all loads issued at once
+		constexpr int VPT = 65536 * 2 / 16 / LU_NT;
+		const int nv = ((int) mp.p + 7) / 8;
+		const uint4 *src = reinterpret_cast<const uint4 *>(invtab);
+		uint4 *dst = reinterpret_cast<uint4 *>(big);
+		uint4 tv[VPT];
+#pragma unroll
+		for (int q = 0; q < VPT; ++q)
+			tv[q] = src[min(q * LU_NT + tid, nv - 1)];
+#pragma unroll
+		for (int q = 0; q < VPT; ++q) {
+			const int e = q * LU_NT + tid;
+			if (e < nv)
+				dst[e] = tv[q];
 		}
-		if (tid == 0)
-			s_first = INT_MAX;
-		__syncthreads();
-		if (tid >= cur && tid < kw && Xl[tid][cc] != 0u)
-			atomicMin(&s_first, tid);
-		__syncthreads();
-		const int first = s_first;
-		if (first == INT_MAX) {
-			if (!covers) {       // rows beyond the window may have one
-				stop = 1;
-				stop_col = c + cc;
+	}
+	const uint16_t *itab = reinterpret_cast<const uint16_t *>(big);
+	__syncthreads();
+	int cur = 0, stop = 0, stop_col = 0, bad = 0;
+	bool done = false;
+	// The owner's search of column cc_ at step s: the event, the multipliers
+	// by position (after the exchange of positions s and first) and the step
+	// records, then the column's flag (a wave's LDS operations complete in order).
+	// The owner keeps its own copy (o_e0, o_inv, o_f) and reads nothing back.
+	uint32_t o_e0 = LU_EV_NONE, o_inv = 0u, o_f = 0u;
+	auto publish = [&](int cc_, int s, uint32_t v) {
+		const uint64_t bal = __ballot(v != 0u && l >= s);
+		if (bal == 0) {
+			o_e0 = LU_EV_NONE;
+			if (l == 0)
+				evc[cc_][0] = LU_EV_NONE;
+		} else {
+			const int first = __builtin_ctzll(bal);
+			const uint32_t xp = __builtin_amdgcn_readlane(v, first);
+			const uint32_t vc = __builtin_amdgcn_readlane(v, s);
+			const uint32_t inv = TAB ? (uint32_t) itab[xp] : mp.inv(xp);
+			const uint32_t a = l == first ? vc : v;
+			o_e0 = (uint32_t) first;
+			o_inv = inv;
+			o_f = l > s ? mul(a, inv) : 0u;
+			Lm[l * FP + s] = (FT) o_f;
+			if (l == 0) {
+				evc[cc_][0] = (uint32_t) first;
+				evc[cc_][1] = inv;
+				jl[s] = cc_;
+				sdinv[s] = inv;
+				sfirst[s] = first;
+				sx[s] = xp;
+			}
+		}
+		__builtin_amdgcn_wave_barrier();
+		if (l == 0)
+			__hip_atomic_store(&ready[cc_], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+	};
+	for (int cb = 0; cb < wc && !done; cb += 8) {
+		const int ow = cb >> 3;
+		const bool own = __builtin_amdgcn_readfirstlane(w) == ow;   // a scalar test
+		// The owner of columns cb .. cb+7 runs at a raised wave priority and
+		// one column ahead: after each event it updates the next column
+		// first, publishes it, then updates its other columns.
+		if (own) {
+			__builtin_amdgcn_s_setprio(2);
+			if (cur < kw)
+				publish(cb, cur, x[0]);
+		}
+#pragma unroll
+		for (int j = 0; j < 8; ++j) {
+			const int cc = cb + j;
+			if (cc >= wc) {
+				done = true;
 				break;
 			}
-			continue;            // a zero column: no pivot (pivot-strategy.tcc:31-39)
-		}
-		if (first != cur) {
-			for (int col = tid; col < LU_W + LU_K; col += LU_NT) {
-				if (col < LU_W) {
-					const uint32_t t = Xl[cur][col];
-					Xl[cur][col] = Xl[first][col];
-					Xl[first][col] = t;
-				} else {
-					const int tc = col - LU_W;
-					const uint32_t t = Tl[cur][tc];
-					Tl[cur][tc] = Tl[first][tc];
-					Tl[first][tc] = t;
+			if (cur == kw) {     // every window row is a pivot row
+				if (!covers) {
+					stop = 1;
+					stop_col = c + cc;
+				}
+				done = true;
+				break;
+			}
+			if (!own) {
+				int spins = 0;
+				while (__hip_atomic_load(&ready[cc], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0 &&
+				       spins < LU_SPIN_MAX) {
+					__builtin_amdgcn_s_sleep(1);
+					++spins;
+				}
+				__builtin_amdgcn_wave_barrier();
+				if (spins >= LU_SPIN_MAX) {   // never expected: reported, not waited for
+					bad = 2;
+					done = true;
+					break;
 				}
 			}
-			if (tid == 0) {
-				const int t = ord[cur];
-				ord[cur] = ord[first];
-				ord[first] = t;
-			}
-			__syncthreads();
-		}
-		if (tid == 0) {
-			const uint32_t x = Xl[cur][cc];
-			const uint32_t inv = invtab ? (uint32_t) invtab[x] : mp.inv(x);
-			if (inv == 0u)
-				s_bad = 1;
-			sinv[cur] = inv;
-			jl[cur] = cc;
-			out->steps[cur] = first;
-			out->J[cur] = c + cc;
-			out->x[cur] = x;
-		}
-		__syncthreads();
-		if (tid > cur && tid < kw) {
-			const uint32_t f = mp.mul(Xl[tid][cc], sinv[cur]);
-			fv[tid] = f;
-			F[(size_t) ord[tid] * LU_W + cur] = f;
-			Xl[tid][cc] = 0u;
-		}
-		__syncthreads();
-		// rows below: row_r -= f_r * pivot row on the panel's later columns and on T
-		const int pcols = wc - cc - 1, cols = pcols + LU_K, rows = kw - cur - 1;
-		for (int idx = tid; idx < rows * cols; idx += LU_NT) {
-			const int r = cur + 1 + idx / cols, ci = idx % cols;
-			const uint32_t f = fv[r];
-			if (f == 0u)
-				continue;
-			if (ci < pcols) {
-				const int col = cc + 1 + ci;
-				Xl[r][col] = mp.sub(Xl[r][col], mp.mul(f, Xl[cur][col]));
+			uint32_t e0, inv, f;
+			if (own) {
+				e0 = o_e0;
+				inv = o_inv;
+				f = o_f;
 			} else {
-				const int tc = ci - pcols;
-				Tl[r][tc] = mp.sub(Tl[r][tc], mp.mul(f, Tl[cur][tc]));
+				e0 = __builtin_amdgcn_readfirstlane(evc[cc][0]);
+				inv = __builtin_amdgcn_readfirstlane(evc[cc][1]);
+				f = (uint32_t) Lm[l * FP + cur];
 			}
+			if (e0 == LU_EV_NONE) {
+				if (!covers) {       // rows beyond the window may have one
+					stop = 1;
+					stop_col = c + cc;
+					done = true;
+					break;
+				}
+				// a zero column: no pivot (pivot-strategy.tcc:31-39)
+				if (own && j < 7 && cc + 1 < wc)
+					publish(cc + 1, cur, x[j + 1]);
+				continue;
+			}
+			const int first = (int) e0;
+			if (inv == 0u) {             // not invertible: the modulus is composite
+				bad = 1;
+				done = true;
+				break;
+			}
+			if (first != cur) {          // the transposition (cur, first)
+				const int src = l == cur ? first : (l == first ? cur : l);
+#pragma unroll
+				for (int jj = 0; jj < 8; ++jj)
+					x[jj] = __shfl(x[jj], src, 64);
+				if (tid == 0) {
+					const int t0 = ord[cur];
+					ord[cur] = ord[first];
+					ord[first] = t0;
+				}
+			}
+			const bool ahead = own && j < 7 && cc + 1 < wc && cur + 1 < kw;
+			if (ahead) {
+				x[j + 1] = upd(x[j + 1], f, __builtin_amdgcn_readlane(x[j + 1], cur));
+				publish(cc + 1, cur + 1, x[j + 1]);
+			}
+#pragma unroll
+			for (int jj = 0; jj < 8; ++jj)
+				if (8 * w + jj >= cc && !(ahead && jj == j + 1))   // the pivot row is zero left of cc
+					x[jj] = upd(x[jj], f, __builtin_amdgcn_readlane(x[jj], cur));
+			++cur;
 		}
-		__syncthreads();
-		++cur;
+		if (own)
+			__builtin_amdgcn_s_setprio(0);
 	}
 	const int rp = cur;
-	if (s_bad) {
-		if (tid == 0) {
-			out->bad = 1;
-			out->rp = rp;
+	if (bad == 2)
+		ready[0] = -1;           // a wave gave up waiting
+	__syncthreads();             // the table is no longer read
+	if (ready[0] == -1)
+		bad = 2;
+	// The multipliers of step s were stored by position at step s; the
+	// transpositions of later steps move two of them each.
+	if (tid < rp)
+		for (int t = tid + 1; t < rp; ++t) {
+			const int ft = sfirst[t];
+			if (ft != t) {
+				const FT t0 = Lm[t * FP + tid];
+				Lm[t * FP + tid] = Lm[ft * FP + tid];
+				Lm[ft * FP + tid] = t0;
+			}
 		}
-		return;
-	}
-	for (int idx = tid; idx < LU_W * LU_K; idx += LU_NT) {
-		const int t = idx / LU_K, q = idx % LU_K;
-		TM[idx] = t < rp ? Tl[t][q] : 0u;
-	}
-	for (int idx = tid; idx < LU_W * LU_W; idx += LU_NT) {
-		const int t = idx / LU_W, u = idx % LU_W;
-		Il[t][u] = (t == u && t < rp) ? 1u : 0u;
-	}
+	uint32_t *Xf = reinterpret_cast<uint32_t *>(big);   // the panel after elimination
+	uint32_t *Ls = Xf + MAT, *Us = Ls + MAT, *Li = Us + MAT, *Ms = Li + MAT, *Rt = Ms + MAT;
+	uint32_t *IL = Rt + 8 * XP, *IU = IL + 512;          // inverses of the 8 x 8 diagonal blocks
+#pragma unroll
+	for (int jj = 0; jj < 8; ++jj)
+		Xf[l * XP + 8 * w + jj] = x[jj];
+	if (tid >= rp && tid < LU_W)
+		sdinv[tid] = 1u;
 	__syncthreads();
-	// backward pass over the pivot rows [U | T | I]: U11^-1 from the right
-	for (int t = rp - 1; t >= 0; --t) {
-		const uint32_t inv = sinv[t];
-		for (int col = tid; col < LU_W + LU_K + LU_W; col += LU_NT) {
-			if (col < LU_W)
-				Xl[t][col] = mp.mul(Xl[t][col], inv);
-			else if (col < LU_W + LU_K)
-				Tl[t][col - LU_W] = mp.mul(Tl[t][col - LU_W], inv);
-			else
-				Il[t][col - LU_W - LU_K] = mp.mul(Il[t][col - LU_W - LU_K], inv);
-		}
-		__syncthreads();
-		if (tid < t)
-			fv[tid] = Xl[tid][jl[t]];
-		__syncthreads();
-		const int cols = LU_W + LU_K + LU_W;
-		for (int idx = tid; idx < t * cols; idx += LU_NT) {
-			const int s = idx / cols, col = idx % cols;
-			const uint32_t g = fv[s];
-			if (g == 0u)
-				continue;
-			if (col < LU_W)
-				Xl[s][col] = mp.sub(Xl[s][col], mp.mul(g, Xl[t][col]));
-			else if (col < LU_W + LU_K)
-				Tl[s][col - LU_W] = mp.sub(Tl[s][col - LU_W], mp.mul(g, Tl[t][col - LU_W]));
-			else
-				Il[s][col - LU_W - LU_K] = mp.sub(Il[s][col - LU_W - LU_K], mp.mul(g, Il[t][col - LU_W - LU_K]));
-		}
-		__syncthreads();
-	}
-	for (int idx = tid; idx < LU_W * LU_K; idx += LU_NT) {
-		const int t = idx / LU_K, q = idx % LU_K;
-		TM[LU_W * LU_K + idx] = t < rp ? Tl[t][q] : 0u;
-	}
-	for (int idx = tid; idx < LU_W * LU_W; idx += LU_NT)
-		V[idx] = Il[idx / LU_W][idx % LU_W];
 	if (tid < LU_K)
-		out->wperm[tid] = ord[tid];
+		out->wperm[tid] = tid < kw ? ord[tid] : tid;
+	if (tid < rp) {
+		out->steps[tid] = sfirst[tid];
+		out->J[tid] = c + jl[tid];
+		out->x[tid] = sx[tid];
+		out->dinv[tid] = sdinv[tid];
+	}
 	if (tid == 0) {
 		out->rp = rp;
 		out->stop = stop;
 		out->stop_col = stop_col;
-		out->bad = 0;
+		out->bad = bad;
+	}
+	if (bad)
+		return;
+	// L11 and U11, completed by the identity past rp
+	for (int idx = tid; idx < LU_W * LU_W; idx += LU_NT) {
+		const int t = idx / LU_W, s_ = idx % LU_W;
+		const uint32_t lv = t == s_ ? 1u : ((s_ < t && t < rp) ? (uint32_t) Lm[t * FP + s_] : 0u);
+		Ls[t * XP + s_] = lv;
+		Us[t * XP + s_] = t == s_ ? (t < rp ? Xf[t * XP + jl[t]] : 1u)
+					  : ((t < s_ && s_ < rp) ? Xf[t * XP + jl[s_]] : 0u);
+		L11[idx] = t < rp ? lv : 0u;
+	}
+	__syncthreads();
+	// sums of products: raw products below 2^52 for p < 2^26 (64 terms below
+	// 2^58), reduced products otherwise
+	auto prod = [&](uint32_t a, uint32_t b) -> uint64_t {
+		const uint64_t v = (uint64_t) a * b;
+		return SMALL ? v : mp.reduce(v);
+	};
+	if (tid < 128) {             // column u of the inverse of diagonal block B
+		const int B = (tid & 63) >> 3, u = tid & 7, o = 8 * B;
+		uint32_t y[8];
+		if (tid < 64) {          // unit lower: y_r = -sum_{u <= q < r} L_rq y_q
+#pragma unroll
+			for (int r = 0; r < 8; ++r) {
+				uint64_t acc = 0;
+#pragma unroll
+				for (int q = 0; q < r; ++q)
+					if (q >= u)
+						acc += prod(Ls[(o + r) * XP + o + q], y[q]);
+				y[r] = r < u ? 0u : (r == u ? 1u : mp.neg((uint32_t) mp.reduce(acc)));
+			}
+#pragma unroll
+			for (int r = 0; r < 8; ++r)
+				IL[B * 64 + r * 8 + u] = y[r];
+		} else {                 // upper: y_r = -(sum_{r < q <= u} U_rq y_q) / U_rr, y_u = 1 / U_uu
+#pragma unroll
+			for (int r = 7; r >= 0; --r) {
+				uint64_t acc = 0;
+#pragma unroll
+				for (int q = r + 1; q < 8; ++q)
+					if (q <= u)
+						acc += prod(Us[(o + r) * XP + o + q], y[q]);
+				y[r] = r > u ? 0u
+					     : (r == u ? sdinv[o + r] : mul(mp.neg((uint32_t) mp.reduce(acc)), sdinv[o + r]));
+			}
+#pragma unroll
+			for (int r = 0; r < 8; ++r)
+				IU[B * 64 + r * 8 + u] = y[r];
+		}
+	}
+	__syncthreads();
+	const int r8 = tid >> 6, jc = tid & 63;
+	// L11 Li = I by block rows, top down: Li_B = IL_B (E_B - sum_{C<B} L_BC Li_C)
+	for (int B = 0; B < 8; ++B) {
+		const int t = 8 * B + r8;
+		uint64_t acc = 0;
+		for (int qb = 0; qb < B; ++qb)
+#pragma unroll
+			for (int u = 0; u < 8; ++u)
+				acc += prod(Ls[t * XP + 8 * qb + u], Li[(8 * qb + u) * XP + jc]);
+		Rt[r8 * XP + jc] = mp.sub(t == jc ? 1u : 0u, (uint32_t) mp.reduce(acc));
+		__syncthreads();
+		acc = 0;
+#pragma unroll
+		for (int u = 0; u < 8; ++u)
+			acc += prod(IL[B * 64 + r8 * 8 + u], Rt[u * XP + jc]);
+		Li[t * XP + jc] = (uint32_t) mp.reduce(acc);
+		__syncthreads();
+	}
+	// U11 M = Li by block rows, bottom up: M_B = IU_B (Li_B - sum_{C>B} U_BC M_C)
+	for (int B = 7; B >= 0; --B) {
+		const int t = 8 * B + r8;
+		uint64_t acc = 0;
+		for (int qb = B + 1; qb < 8; ++qb)
+#pragma unroll
+			for (int u = 0; u < 8; ++u)
+				acc += prod(Us[t * XP + 8 * qb + u], Ms[(8 * qb + u) * XP + jc]);
+		Rt[r8 * XP + jc] = mp.sub(Li[t * XP + jc], (uint32_t) mp.reduce(acc));
+		__syncthreads();
+		acc = 0;
+#pragma unroll
+		for (int u = 0; u < 8; ++u)
+			acc += prod(IU[B * 64 + r8 * 8 + u], Rt[u * XP + jc]);
+		Ms[t * XP + jc] = (uint32_t) mp.reduce(acc);
+		__syncthreads();
+	}
+	for (int idx = tid; idx < LU_W * LU_W; idx += LU_NT) {
+		const int t = idx / LU_W, s_ = idx % LU_W;
+		LiM[idx] = t < rp ? Li[t * XP + s_] : 0u;
+		LiM[LU_W * LU_W + idx] = t < rp ? Ms[t * XP + s_] : 0u;
 	}
 }
 
-// Xw[q][col] = X[perm[i0 + q]][c + col] for q < kw, zero rows up to LU_K.
-__global__ void lu_gather_window_kernel(const uint32_t *__restrict__ X, size_t ldx, int kw, int c, int nc, int i0,
-					const int *__restrict__ perm, uint32_t *__restrict__ Xw)
+// Rows i0 .. i0+kw-1 of Y (ld ncols) put in the window's new order, through
+// the temporary T (kw x ncols): first gather, then copy back.
+template <typename TY>
+__global__ void lu_window_gather_kernel(const TY *__restrict__ Y, size_t ld, int ncols, int i0,
+					const LuPanel *__restrict__ st, TY *__restrict__ T)
 {
 	const int col = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
-	if (col >= nc)
-		return;
-	Xw[(size_t) q * nc + col] = q < kw ? X[(size_t) perm[i0 + q] * ldx + c + col] : 0u;
+	if (col < ncols)
+		T[(size_t) q * ncols + col] = Y[(size_t) (i0 + st->wperm[q]) * ld + col];
 }
 
-// perm[i0 + q] <- perm[i0 + wperm[q]] for q < kw (one workgroup).
-__global__ void lu_permute_window_kernel(int *__restrict__ perm, int i0, int kw, const LuPanel *__restrict__ st)
+template <typename TY>
+__global__ void lu_window_put_kernel(TY *__restrict__ Y, size_t ld, int ncols, int i0, const TY *__restrict__ T)
+{
+	const int col = blockIdx.x * blockDim.x + threadIdx.x, q = blockIdx.y;
+	if (col < ncols)
+		Y[(size_t) (i0 + q) * ld + col] = T[(size_t) q * ncols + col];
+}
+
+// orig[i0 + q] <- orig[i0 + wperm[q]] (one workgroup of LU_K threads).
+__global__ void lu_window_orig_kernel(int *__restrict__ orig, int i0, int kw, const LuPanel *__restrict__ st)
 {
 	__shared__ int old[LU_K];
 	const int q = threadIdx.x;
 	if (q < kw)
-		old[q] = perm[i0 + q];
+		old[q] = orig[i0 + q];
 	__syncthreads();
 	if (q < kw)
-		perm[i0 + q] = old[st->wperm[q]];
+		orig[i0 + q] = old[st->wperm[q]];
 }
 
-// G[perm[pos]][t] = X[perm[pos]][J_t] for positions pos >= i1 and t < rp,
-// zero for every other row and for t >= rp (G is m x LU_W).
-__global__ void lu_gather_mult_kernel(const uint32_t *__restrict__ X, size_t ldx, int m, int i1,
-				      const int *__restrict__ perm, const LuPanel *__restrict__ st,
-				      uint32_t *__restrict__ G)
+// pluq: the pivot rows' own multipliers, Lb[i0 + t][i0 + s] = L11[t][s] (s < t < rp).
+__global__ void lu_pivot_mult_kernel(const uint32_t *__restrict__ L11, int i0, const LuPanel *__restrict__ st,
+				     uint32_t *__restrict__ Lb, size_t ldl)
 {
-	const int pos = blockIdx.x * (blockDim.x / LU_W) + threadIdx.x / LU_W, t = threadIdx.x % LU_W;
-	if (pos >= m)
-		return;
-	const int row = perm[pos];
-	const int rp = st->rp;
-	G[(size_t) row * LU_W + t] = (pos >= i1 && t < rp) ? X[(size_t) row * ldx + st->J[t]] : 0u;
+	const int t = blockIdx.x, s_ = threadIdx.x;
+	if (t < st->rp && s_ < t)
+		Lb[(size_t) (i0 + t) * ldl + i0 + s_] = L11[t * LU_W + s_];
 }
 
-// The pivot rows: X[perm[i0 + t]][c + col] = U[t][col] for t < rp.
+// The new pivot rows: X[i0 + t][c + col] = U[t][col] for t < rp.
 __global__ void lu_scatter_pivots_kernel(uint32_t *__restrict__ X, size_t ldx, int c, int nc, int i0,
-					 const int *__restrict__ perm, const LuPanel *__restrict__ st,
-					 const uint32_t *__restrict__ U)
+					 const LuPanel *__restrict__ st, const uint32_t *__restrict__ U)
 {
 	const int col = blockIdx.x * blockDim.x + threadIdx.x, t = blockIdx.y;
 	if (col >= nc || t >= st->rp)
 		return;
-	X[(size_t) perm[i0 + t] * ldx + c + col] = U[(size_t) t * nc + col];
+	X[(size_t) (i0 + t) * ldx + c + col] = U[(size_t) t * nc + col];
 }
 
-// pluq multipliers of this panel's steps i0 .. i0+rp-1: rows below take
-// GV[row][t] (= X_j[J] U11^-1), the pivot rows their window multipliers.
-__global__ void lu_store_mult_kernel(uint32_t *__restrict__ L, size_t ldl, int m, int i0,
-				     const int *__restrict__ perm, const LuPanel *__restrict__ st,
-				     const uint32_t *__restrict__ GV, const uint32_t *__restrict__ F)
+// G[q][t] = X[i1 + q][J_t] for t < rp, zero for t >= rp (G is mb x LU_W).
+// Rows i1 .. i1+mb-1 with i1 >= i0; those at positions below i0 + rp (the new
+// pivot rows) get zeros, so an update from i0 on leaves them unchanged.
+__global__ void lu_gather_mult_kernel(const uint32_t *__restrict__ X, size_t ldx, int mb, int i1, int i0,
+				      const LuPanel *__restrict__ st, uint32_t *__restrict__ G)
 {
-	const int pos = blockIdx.x * (blockDim.x / LU_W) + threadIdx.x / LU_W, t = threadIdx.x % LU_W;
-	const int rp = st->rp;
-	if (pos < i0 || pos >= m || t >= rp)
+	const int q = blockIdx.x * (blockDim.x / LU_W) + threadIdx.x / LU_W, t = threadIdx.x % LU_W;
+	if (q >= mb)
 		return;
-	const int row = perm[pos];
-	uint32_t v;
-	if (pos >= i0 + rp)
-		v = GV[(size_t) row * LU_W + t];
-	else {
-		const int k = pos - i0;           // pivot row k of the panel: its multipliers of steps t < k
-		v = t < k ? F[(size_t) st->wperm[k] * LU_W + t] : 0u;
-		if (t >= k)
-			return;
+	const int rp = st->rp;
+	G[(size_t) q * LU_W + t] = (t < rp && i1 + q >= i0 + rp) ? X[(size_t) (i1 + q) * ldx + st->J[t]] : 0u;
+}
+
+// The panel's update for p <= 2^16, fused (the INT8 rank update follows).
+// Workgroup j owns the columns [c + TC j, c + TC j + TC) of the window rows:
+// it reads them in the window's new order (position q: window row wperm[q]),
+// forms [L11^-1; P_J^-1] P on v_mfma_f64_16x16x4f64 (64 terms below 2^32:
+// exact in a double), writes the new pivot rows and the remaining window rows
+// back in that order, and packs W = P_J^-1 P as the update's INT8 B fragments
+// (KB = 2).  Every read of a slice precedes its writes, and slices are
+// disjoint, so the reordering is done in place.
+template <int TC>
+__global__ __launch_bounds__(256) void lu_wb_i8_kernel(uint32_t *__restrict__ X, size_t ldx, int nc, int c, int i0,
+						       int kw, const LuPanel *__restrict__ st,
+						       const uint32_t *__restrict__ LiM, int8_t *__restrict__ Wp, ModP mp)
+{
+	static_assert(TC == 32, "column slice of 32");
+	__shared__ uint32_t Ps[LU_K][TC + 1], Pm[2 * LU_W][LU_W + 1], Os[2 * LU_W][TC + 1];
+	__shared__ int wq[LU_K];
+	const uint32_t p = (uint32_t) mp.p;
+	const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+	const int j0 = blockIdx.x * TC;
+	const int rp = st->rp;
+	if (tid < LU_K)
+		wq[tid] = st->wperm[tid];
+	__syncthreads();
+	for (int e = tid; e < LU_K * TC; e += 256) {
+		const int q = e / TC, col = e % TC, gc = j0 + col;
+		Ps[q][col] = (q < kw && gc < nc) ? X[(size_t) (i0 + wq[q]) * ldx + c + gc] : 0u;
 	}
-	L[(size_t) row * ldl + i0 + t] = v;
+	for (int e = tid; e < 2 * LU_W * LU_W; e += 256)
+		Pm[e / LU_W][e % LU_W] = LiM[e];
+	__syncthreads();
+	// wave w: output rows 32w .. 32w+31 as two 16-row blocks, TC/16 column
+	// blocks each.  A fragment: lane l holds (row l%16, k l/16); B fragment:
+	// (k l/16, column l%16); result register r: (row l/16 + 4r, column l%16).
+	{
+		constexpr int NJ = TC / 16;
+		typedef double v4d __attribute__((ext_vector_type(4)));
+		v4d acc[2][NJ];
+#pragma unroll
+		for (int b = 0; b < 2; ++b)
+#pragma unroll
+			for (int j = 0; j < NJ; ++j)
+				acc[b][j] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+		for (int s_ = 0; s_ < LU_W; s_ += 4) {
+			const int kk = s_ + (lane >> 4);
+			const double a0 = (double) Pm[32 * w + (lane & 15)][kk];
+			const double a1 = (double) Pm[32 * w + 16 + (lane & 15)][kk];
+#pragma unroll
+			for (int j = 0; j < NJ; ++j) {
+				const double bv = (double) Ps[kk][16 * j + (lane & 15)];
+				acc[0][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc[0][j], 0, 0, 0);
+				acc[1][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc[1][j], 0, 0, 0);
+			}
+		}
+#pragma unroll
+		for (int b = 0; b < 2; ++b)
+#pragma unroll
+			for (int j = 0; j < NJ; ++j)
+#pragma unroll
+				for (int r = 0; r < 4; ++r)
+					Os[32 * w + 16 * b + (lane >> 4) + 4 * r][16 * j + (lane & 15)] = mp.mod_f64(acc[b][j][r]);
+	}
+	__syncthreads();
+	for (int e = tid; e < LU_K * TC; e += 256) {
+		const int q = e / TC, col = e % TC, gc = j0 + col;
+		if (q < kw && gc < nc)
+			X[(size_t) (i0 + q) * ldx + c + gc] = q < rp ? Os[q][col] : Ps[q][col];
+	}
+	// B fragments: piece (column block, k block q, lane) holds column
+	// 32 cb + lane%32 and k = 32q + 16(lane/32) .. +15
+	if (tid < 128) {
+		const int q = tid >> 6, cc = lane & 31, t = 32 * q + 16 * (lane >> 5);
+		uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
+#pragma unroll
+		for (int b = 0; b < 16; ++b) {
+			const uint32_t lb = i8_limbs(Os[LU_W + t + b][cc], p);
+			hw[b >> 2] |= ((lb >> 8) & 0xffu) << (8 * (b & 3));
+			lw[b >> 2] |= (lb & 0xffu) << (8 * (b & 3));
+		}
+		int8_t *fr = Wp + ((size_t) blockIdx.x * 2 + q) * 2 * 1024 + lane * 16;
+		*(uint4 *) fr = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+		*(uint4 *) (fr + 1024) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+	}
+}
+
+// G for the update of rows i0 .. i0+mb-1: G[r][t] = X[i0 + r][J_t] for t < rp,
+// zero on the new pivot rows (r < rp), packed as the update's INT8 A fragments
+// (KB = 2).  Thread: the 16-byte piece (row block, k block, lane) of one
+// workgroup's 64 rows.
+__global__ __launch_bounds__(256) void lu_gpack_i8_kernel(const uint32_t *__restrict__ X, size_t ldx, int mb, int i0,
+							  int c, const LuPanel *__restrict__ st, int8_t *__restrict__ Gp,
+							  uint32_t p)
+{
+	const int tid = threadIdx.x, lane = tid & 63, q = (tid >> 6) & 1, rb = 2 * blockIdx.x + (tid >> 7);
+	const int rp = st->rp;
+	const int r = 32 * rb + (lane & 31), s0 = 32 * q + 16 * (lane >> 5);
+	const bool live = r < mb && r >= rp;
+	const uint32_t *row = X + (size_t) (i0 + r) * ldx;
+	uint32_t xv[16];
+	// a full-rank panel has J_t = c + t: sixteen consecutive entries, four
+	// 16-byte loads when the row stride and c keep them aligned
+	if (rp == LU_W && st->J[LU_W - 1] == c + LU_W - 1 && (ldx & 3) == 0 && (c & 3) == 0) {
+		const uint4 *src = (const uint4 *) (row + c + s0);
+#pragma unroll
+		for (int qq = 0; qq < 4; ++qq) {
+			const uint4 u = live ? src[qq] : make_uint4(0u, 0u, 0u, 0u);
+			xv[4 * qq] = u.x;
+			xv[4 * qq + 1] = u.y;
+			xv[4 * qq + 2] = u.z;
+			xv[4 * qq + 3] = u.w;
+		}
+	} else {
+#pragma unroll
+		for (int b = 0; b < 16; ++b)
+			xv[b] = (live && s0 + b < rp) ? row[st->J[s0 + b]] : 0u;
+	}
+	uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
+#pragma unroll
+	for (int b = 0; b < 16; ++b) {
+		const uint32_t lb = i8_limbs(xv[b], p);
+		hw[b >> 2] |= ((lb >> 8) & 0xffu) << (8 * (b & 3));
+		lw[b >> 2] |= (lb & 0xffu) << (8 * (b & 3));
+	}
+	int8_t *fr = Gp + ((size_t) rb * 2 + q) * 2 * 1024 + lane * 16;
+	*(uint4 *) fr = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+	*(uint4 *) (fr + 1024) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
 }
 
 // First nonzero at positions >= i0, columns [c0, c1): key = col << 32 | pos.
 __global__ void lu_search_kernel(const uint32_t *__restrict__ X, size_t ldx, int m, int i0, int c0, int c1,
-				 const int *__restrict__ perm, unsigned long long *__restrict__ key)
+				 unsigned long long *__restrict__ key)
 {
 	const int col = c0 + blockIdx.x * blockDim.x + threadIdx.x;
 	if (col >= c1)
 		return;
 	for (int pos = i0 + blockIdx.y; pos < m; pos += gridDim.y) {
-		if (X[(size_t) perm[pos] * ldx + col] != 0u) {
+		if (X[(size_t) pos * ldx + col] != 0u) {
 			atomicMin(key, ((unsigned long long) col << 32) | (unsigned) pos);
 			return;
 		}
@@ -326,51 +611,56 @@ __global__ void lu_search_kernel(const uint32_t *__restrict__ X, size_t ldx, int
 
 __global__ void lu_key_init_kernel(unsigned long long *key) { *key = ~0ull; }
 
-__global__ void lu_swap_kernel(int *perm, int a, int b)
+// Swap rows a and b of Y (ncols entries) -- one transposition.
+template <typename TY>
+__global__ void lu_swap_rows_kernel(TY *__restrict__ Y, size_t ld, int ncols, int a, int b)
 {
-	const int t = perm[a];
-	perm[a] = perm[b];
-	perm[b] = t;
+	const int col = blockIdx.x * blockDim.x + threadIdx.x;
+	if (col >= ncols)
+		return;
+	const TY t = Y[(size_t) a * ld + col];
+	Y[(size_t) a * ld + col] = Y[(size_t) b * ld + col];
+	Y[(size_t) b * ld + col] = t;
 }
 
-__global__ void lu_iota_kernel(int *perm, int m)
+__global__ void lu_iota_kernel(int *orig, int m)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < m)
-		perm[i] = i;
+		orig[i] = i;
 }
 
 // Working copy: canonical residues of A in columns 0 .. n-1, for compute_L the
 // identity in columns n .. n+m-1.
 template <int E>
 __global__ void lu_load_kernel(const typename Elem<E>::T *__restrict__ A, size_t lda, int m, int n, int ntot, uint32_t p,
-			       uint32_t *__restrict__ X)
+			       uint32_t *__restrict__ X, size_t ldx)
 {
 	const int col = blockIdx.x * blockDim.x + threadIdx.x, row = blockIdx.y;
 	if (col >= ntot)
 		return;
-	X[(size_t) row * ntot + col] = col < n ? Elem<E>::load(A + (size_t) row * lda + col, p)
+	X[(size_t) row * ldx + col] = col < n ? Elem<E>::load(A + (size_t) row * lda + col, p)
 					       : (col - n == row ? 1u : 0u);
 }
 
-// Output row t (position order).  mode 0: U.  mode 1: U right of the
-// diagonal, T[t][s] = X[perm t][n + perm s] left of it.  mode 2 (pluq): L
-// left of the diagonal (first `rank` columns), U in column order corder.
+// Output row t.  mode U: the working copy.  compute_L: U from the diagonal
+// on, T[t][s] = X[t][n + orig[s]] left of it.  pluq: L left of the diagonal
+// (first `rank` columns), U in the column order corder.
 template <int E>
 __global__ void lu_store_kernel(const uint32_t *__restrict__ X, size_t ldx, int m, int n, int rank, int mode,
-				const int *__restrict__ perm, const int *__restrict__ corder,
+				const int *__restrict__ orig, const int *__restrict__ corder,
 				const uint32_t *__restrict__ L, size_t ldl, uint32_t p, typename Elem<E>::T *__restrict__ A,
 				size_t lda)
 {
 	const int col = blockIdx.x * blockDim.x + threadIdx.x, t = blockIdx.y;
 	if (col >= n)
 		return;
-	const size_t row = (size_t) perm[t];
+	const size_t row = (size_t) t;
 	uint32_t v;
-	if (mode == 0)
+	if (mode == MLZ_ECH_U)
 		v = X[row * ldx + col];
-	else if (mode == 1)
-		v = col < t ? X[row * ldx + n + perm[col]] : X[row * ldx + col];
+	else if (mode == MLZ_ECH_TRANSFORM)
+		v = col < t ? X[row * ldx + n + orig[col]] : X[row * ldx + col];
 	else
 		v = col < t ? (col < rank ? L[row * ldl + col] : 0u) : X[row * ldx + corder[col]];
 	Elem<E>::store(A + (size_t) t * lda + col, v, p);
@@ -391,28 +681,30 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 	const ModP mp = make_modp(p);
 	const uint32_t pp = (uint32_t) p;
 	const size_t ntot = mode == MLZ_ECH_TRANSFORM ? n + m : n;
+	// rows of the working copy start on 128-byte boundaries
+	const size_t ldx = round_up(ntot, 32);
 	const size_t rmax = std::min(m, n);
-	// workspace: X (m x ntot), Xw (LU_K x ntot), UW (128 x ntot), G and GV (m x LU_W),
-	// TM (128 x LU_K), V (LU_W^2), F (LU_K x LU_W), L (m x rmax for pluq), perm, corder, state
+	const bool pluq = mode == MLZ_ECH_PLUQ;
+	// workspace: X (m x ntot); the window temporary (LU_K x max(ntot, rmax)) and
+	// UW (2 LU_W x ntot: the new pivot rows, then W = P_J^-1 P); G (m x LU_W)
+	// and for pluq Lb (m x rmax); L11, LiM (2 LU_W x LU_W), Ui, orig, corder, state
 	void *w0, *w1, *w2, *w3;
-	MLZ_TRY(ws_get(ctx, WS_RREF_0, m * ntot * sizeof(uint32_t), &w0));
-	const size_t sXw = round_up(LU_K * ntot * 4, 256), sUW = round_up(2 * LU_W * ntot * 4, 256);
+	MLZ_TRY(ws_get(ctx, WS_RREF_0, m * ldx * sizeof(uint32_t), &w0));
+	const size_t sXw = round_up(LU_K * std::max(ntot, rmax) * 4, 256), sUW = round_up(2 * LU_W * ntot * 4, 256);
 	MLZ_TRY(ws_get(ctx, WS_RREF_1, sXw + sUW, &w1));
 	const size_t sG = round_up(m * LU_W * 4, 256);
-	MLZ_TRY(ws_get(ctx, WS_RREF_2, 2 * sG + (mode == MLZ_ECH_PLUQ ? m * rmax * 4 : 0), &w2));
-	const size_t small = round_up(2 * LU_W * LU_K * 4, 256) + round_up(LU_W * LU_W * 4, 256) +
-			     round_up(LU_K * LU_W * 4, 256) + round_up(m * 4, 256) + round_up(n * 4, 256) +
-			     round_up(sizeof(LuPanel), 256) + 256;
+	MLZ_TRY(ws_get(ctx, WS_RREF_2, sG + (pluq ? m * rmax * 4 : 0), &w2));
+	const size_t sq = round_up(LU_W * LU_W * 4, 256);
+	const size_t small = 4 * sq + round_up(m * 4, 256) + round_up(n * 4, 256) + round_up(sizeof(LuPanel), 256) + 256;
 	MLZ_TRY(ws_get(ctx, WS_RREF_3, small, &w3));
 	uint32_t *X = (uint32_t *) w0;
 	uint32_t *Xw = (uint32_t *) w1, *UW = (uint32_t *) ((char *) w1 + sXw);
-	uint32_t *G = (uint32_t *) w2, *GV = (uint32_t *) ((char *) w2 + sG);
-	uint32_t *L = (uint32_t *) ((char *) w2 + 2 * sG);
+	uint32_t *G = (uint32_t *) w2, *Lb = (uint32_t *) ((char *) w2 + sG);
 	char *q = (char *) w3;
-	uint32_t *TM = (uint32_t *) q; q += round_up(2 * LU_W * LU_K * 4, 256);
-	uint32_t *V = (uint32_t *) q; q += round_up(LU_W * LU_W * 4, 256);
-	uint32_t *F = (uint32_t *) q; q += round_up(LU_K * LU_W * 4, 256);
-	int *perm = (int *) q; q += round_up(m * 4, 256);
+	uint32_t *L11 = (uint32_t *) q; q += sq;
+	uint32_t *LiM = (uint32_t *) q; q += 2 * sq;
+	uint32_t *Ui = (uint32_t *) q; q += sq;
+	int *orig = (int *) q; q += round_up(m * 4, 256);
 	int *corder = (int *) q; q += round_up(n * 4, 256);
 	LuPanel *st = (LuPanel *) q; q += round_up(sizeof(LuPanel), 256);
 	unsigned long long *key = (unsigned long long *) q;
@@ -422,17 +714,17 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 		MLZ_TRY(zp_inverse_table(ctx, mp, &invtab));
 	{
 		ProfScope ps(ctx, "lu_load");
-		lu_iota_kernel<<<ceil_div(m, 256), 256, 0, ctx->stream>>>(perm, (int) m);
+		lu_iota_kernel<<<ceil_div(m, 256), 256, 0, ctx->stream>>>(orig, (int) m);
 		dim3 g(ceil_div(ntot, 256), (unsigned) m);
 		if (elem == MLZ_U16)
 			lu_load_kernel<MLZ_U16><<<g, 256, 0, ctx->stream>>>((const uint16_t *) A, lda, (int) m, (int) n,
-									     (int) ntot, pp, X);
+									     (int) ntot, pp, X, ldx);
 		else if (elem == MLZ_U32)
 			lu_load_kernel<MLZ_U32><<<g, 256, 0, ctx->stream>>>((const uint32_t *) A, lda, (int) m, (int) n,
-									     (int) ntot, pp, X);
+									     (int) ntot, pp, X, ldx);
 		else
 			lu_load_kernel<MLZ_F64_SYM><<<g, 256, 0, ctx->stream>>>((const double *) A, lda, (int) m, (int) n,
-										 (int) ntot, pp, X);
+										 (int) ntot, pp, X, ldx);
 		MLZ_TRY(check_launch(ctx, "lu_load_kernel"));
 	}
 	const bool upd_i8 = p <= 65536 && (algo == MLZ_ALGO_AUTO || algo == MLZ_ALGO_INT8);
@@ -444,56 +736,141 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 	// A transposition made after a search: the next window then finds its
 	// first pivot at position i, and its step records this position instead.
 	long pending = -1;
-	LuPanel h;
+	// The panel state comes back through pinned memory and an event, so the
+	// host can queue the panel's update behind the window kernel first.
+	struct Res {
+		LuPanel *h = nullptr;
+		hipEvent_t ev = nullptr;
+		~Res()
+		{
+			if (ev)
+				(void) hipEventDestroy(ev);
+			if (h)
+				(void) hipHostFree(h);
+		}
+	} res;
+	MLZ_HIP(ctx, hipHostMalloc((void **) &res.h, sizeof(LuPanel), hipHostMallocDefault));
+	MLZ_HIP(ctx, hipEventCreateWithFlags(&res.ev, hipEventDisableTiming));
+	auto swap_rows = [&](int a, int b) -> int {
+		lu_swap_rows_kernel<uint32_t><<<ceil_div(ntot, 256), 256, 0, ctx->stream>>>(X, ldx, (int) ntot, a, b);
+		lu_swap_rows_kernel<int><<<1, 1, 0, ctx->stream>>>(orig, 1, 1, a, b);
+		if (pluq)
+			lu_swap_rows_kernel<uint32_t><<<ceil_div(rmax, 256), 256, 0, ctx->stream>>>(Lb, rmax, (int) rmax, a, b);
+		return check_launch(ctx, "lu_swap_rows_kernel");
+	};
+	// The update of the panel at (i, c).  With rp < 0 (not known yet) every
+	// size is taken at its maximum: the rows from i on (the pivot rows get
+	// zero multipliers), all LU_W rows of P (LiM is zero past rp).
+	// p <= 2^16 without pluq: the fused kernels and the packed INT8 update
+	const bool fused = upd_i8 && !pluq;
+	int8_t *Gpk = nullptr, *Wpk = nullptr;
+	if (fused) {
+		void *gq, *wq;
+		MLZ_TRY(ws_get(ctx, WS_PACK_A, round_up(m, 64) * LU_W * 2, &gq));
+		MLZ_TRY(ws_get(ctx, WS_PACK_B, round_up(ntot, 64) * LU_W * 2, &wq));
+		Gpk = (int8_t *) gq;
+		Wpk = (int8_t *) wq;
+	}
+	auto update_fused = [&](int kw) -> int {
+		const size_t nc = ntot - c, mb = m - i;
+		ProfScope ps(ctx, "lu_update");
+		lu_window_orig_kernel<<<1, LU_K, 0, ctx->stream>>>(orig, (int) i, kw, st);
+		lu_wb_i8_kernel<32><<<ceil_div(nc, 32), 256, 0, ctx->stream>>>(X, ldx, (int) nc, (int) c, (int) i, kw, st,
+										    LiM, Wpk, mp);
+		lu_gpack_i8_kernel<<<ceil_div(mb, 64), 256, 0, ctx->stream>>>(X, ldx, (int) mb, (int) i, (int) c, st, Gpk,
+										pp);
+		MLZ_TRY(check_launch(ctx, "lu_gpack_i8_kernel"));
+		return zp_rank_update_i8_packed(ctx, pp, mb, nc, 2, Gpk, Wpk, X + i * ldx + c, ldx);
+	};
+	auto update = [&](int kw, int rp) -> int {
+		const size_t nc = ntot - c;
+		const size_t rb = rp < 0 ? i : i + (size_t) rp, mb = m - rb;
+		const size_t kk = rp < 0 ? (size_t) kw : (size_t) rp;
+		ProfScope ps(ctx, "lu_update");
+		// the window rows in their new order (the working copy and, for pluq, the multipliers)
+		lu_window_gather_kernel<uint32_t><<<dim3(ceil_div(ntot, 256), kw), 256, 0, ctx->stream>>>(
+			X, ldx, (int) ntot, (int) i, st, Xw);
+		lu_window_put_kernel<uint32_t><<<dim3(ceil_div(ntot, 256), kw), 256, 0, ctx->stream>>>(
+			X, ldx, (int) ntot, (int) i, Xw);
+		lu_window_orig_kernel<<<1, LU_K, 0, ctx->stream>>>(orig, (int) i, kw, st);
+		if (pluq && i > 0) {
+			lu_window_gather_kernel<uint32_t><<<dim3(ceil_div(rmax, 256), kw), 256, 0, ctx->stream>>>(
+				Lb, rmax, (int) rmax, (int) i, st, Xw);
+			lu_window_put_kernel<uint32_t><<<dim3(ceil_div(rmax, 256), kw), 256, 0, ctx->stream>>>(
+				Lb, rmax, (int) rmax, (int) i, Xw);
+		}
+		MLZ_TRY(check_launch(ctx, "lu_window_put_kernel"));
+		// UW = [L11^-1; P_J^-1] P: the new pivot rows and W
+		MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, 2 * LU_W, nc, kk, 1, LiM, LU_W, X + i * ldx + c, ldx, 0, UW, nc,
+				    algo));
+		if (mb) {
+			lu_gather_mult_kernel<<<ceil_div(mb, 256 / LU_W), 256, 0, ctx->stream>>>(X, ldx, (int) mb, (int) rb,
+												   (int) i, st, G);
+			MLZ_TRY(check_launch(ctx, "lu_gather_mult_kernel"));
+		}
+		if (pluq) {
+			if (mb) {
+				// the rows below: x U11 = X_j[J], U11^-1 = P_J^-1 L11
+				MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, LU_W, LU_W, LU_W, 1, LiM + LU_W * LU_W, LU_W, L11, LU_W, 0,
+						    Ui, LU_W, algo));
+				MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, mb, kk, kk, 1, G, LU_W, Ui, LU_W, 0, Lb + rb * rmax + i,
+						    rmax, algo));
+			}
+			lu_pivot_mult_kernel<<<LU_W, LU_W, 0, ctx->stream>>>(L11, (int) i, st, Lb, rmax);
+			MLZ_TRY(check_launch(ctx, "lu_pivot_mult_kernel"));
+		}
+		if (mb) {
+			uint32_t *Xb = X + rb * ldx + c;
+			const uint32_t *Wp = UW + (size_t) LU_W * nc;
+			if (upd_i8)
+				MLZ_TRY(zp_rank_update_i8_dev(ctx, pp, mb, nc, LU_W, G, LU_W, Wp, nc, Xb, ldx));
+			else
+				MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, mb, nc, LU_W, p - 1, G, LU_W, Wp, nc, 1, Xb, ldx, algo));
+		}
+		lu_scatter_pivots_kernel<<<dim3(ceil_div(nc, 256), LU_W), 256, 0, ctx->stream>>>(X, ldx, (int) c, (int) nc,
+												(int) i, st, UW);
+		return check_launch(ctx, "lu_scatter_pivots_kernel");
+	};
 	while (i < m && c < n) {
 		const int wc = (int) std::min((size_t) LU_W, n - c);
 		const int kw = (int) std::min((size_t) LU_K, m - i);
 		{
 			ProfScope ps(ctx, "lu_window");
-			lu_window_kernel<<<1, LU_NT, 0, ctx->stream>>>(X, ntot, (int) m, (int) c, wc, (int) i, perm, mp, invtab,
-									st, TM, V, F);
+			if (invtab)
+				lu_window_kernel<true, true><<<1, LU_NT, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, (int) i,
+											    mp, invtab, st, L11, LiM);
+			else if (mp.small)
+				lu_window_kernel<true, false><<<1, LU_NT, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, (int) i,
+											     mp, invtab, st, L11, LiM);
+			else
+				lu_window_kernel<false, false><<<1, LU_NT, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc,
+											      (int) i, mp, invtab, st, L11, LiM);
 		}
 		MLZ_TRY(check_launch(ctx, "lu_window_kernel"));
-		MLZ_HIP(ctx, hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
-		MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
-		if (h.bad)
+		MLZ_HIP(ctx, hipMemcpyAsync(res.h, st, sizeof(LuPanel), hipMemcpyDeviceToHost, ctx->stream));
+		MLZ_HIP(ctx, hipEventRecord(res.ev, ctx->stream));
+		// pluq's sizes depend on rp; otherwise the update is queued at once
+		const bool early = !pluq;
+		if (early)
+			MLZ_TRY(fused ? update_fused(kw) : update(kw, -1));
+		MLZ_HIP(ctx, hipEventSynchronize(res.ev));
+		const LuPanel &h = *res.h;
+		if (h.bad) {
+			MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+			if (h.bad == 2)
+				return set_error(ctx, MLZ_E_HIP, "lu_window_kernel: a wave stopped waiting for a column");
 			return set_error(ctx, MLZ_E_NOT_INVERTIBLE, "Could not invert pivot-element in the ring");
+		}
 		const int rp = h.rp;
 		for (int s = 0; s < rp; ++s) {
 			steps.push_back(s == 0 && pending >= 0 ? (int) pending : (int) i + h.steps[s]);
 			pcols.push_back(h.J[s]);
 			det = mp.mul((uint32_t) det, h.x[s]);
 		}
-		if (rp > 0)
-			pending = -1;
 		if (rp > 0) {
-			const size_t nc = ntot - c;
-			ProfScope ps(ctx, "lu_update");
-			lu_gather_window_kernel<<<dim3(ceil_div(nc, 256), LU_K), 256, 0, ctx->stream>>>(X, ntot, kw, (int) c,
-												  (int) nc, (int) i, perm, Xw);
-			MLZ_TRY(check_launch(ctx, "lu_gather_window_kernel"));
-			// [T_piv; M] X_w: the new pivot rows and W
-			MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, 2 * LU_W, nc, LU_K, 1, TM, LU_K, Xw, nc, 0, UW, nc, algo));
-			lu_permute_window_kernel<<<1, LU_K, 0, ctx->stream>>>(perm, (int) i, kw, st);
-			lu_gather_mult_kernel<<<ceil_div(m, 256 / LU_W), 256, 0, ctx->stream>>>(X, ntot, (int) m, (int) i + rp,
-												  perm, st, G);
-			MLZ_TRY(check_launch(ctx, "lu_gather_mult_kernel"));
-			if (mode == MLZ_ECH_PLUQ) {
-				// multipliers X_j[J] U11^-1 (before X changes)
-				MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, m, LU_W, LU_W, 1, G, LU_W, V, LU_W, 0, GV, LU_W, algo));
-				lu_store_mult_kernel<<<ceil_div(m, 256 / LU_W), 256, 0, ctx->stream>>>(L, rmax, (int) m, (int) i,
-													 perm, st, GV, F);
-				MLZ_TRY(check_launch(ctx, "lu_store_mult_kernel"));
-			}
-			const uint32_t *Wp = UW + (size_t) LU_W * nc;
-			if (upd_i8)
-				MLZ_TRY(zp_rank_update_i8_dev(ctx, pp, m, nc, LU_W, G, LU_W, Wp, nc, X + c, ntot));
-			else
-				MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, m, nc, LU_W, p - 1, G, LU_W, Wp, nc, 1, X + c, ntot, algo));
-			lu_scatter_pivots_kernel<<<dim3(ceil_div(nc, 256), LU_W), 256, 0, ctx->stream>>>(X, ntot, (int) c,
-													(int) nc, (int) i, perm,
-													st, UW);
-			MLZ_TRY(check_launch(ctx, "lu_scatter_pivots_kernel"));
+			pending = -1;
+			if (!early)
+				MLZ_TRY(update(kw, rp));
 		}
 		i += rp;
 		if (!h.stop) {
@@ -509,7 +886,7 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 			unsigned long long k = ~0ull;
 			lu_key_init_kernel<<<1, 1, 0, ctx->stream>>>(key);
 			lu_search_kernel<<<dim3(ceil_div(c1 - c, 256), (unsigned) std::min<size_t>(m - i, 64)), 256, 0,
-					   ctx->stream>>>(X, ntot, (int) m, (int) i, (int) c, (int) c1, perm, key);
+					   ctx->stream>>>(X, ldx, (int) m, (int) i, (int) c, (int) c1, key);
 			MLZ_TRY(check_launch(ctx, "lu_search_kernel"));
 			MLZ_HIP(ctx, hipMemcpyAsync(&k, key, sizeof k, hipMemcpyDeviceToHost, ctx->stream));
 			MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -521,8 +898,7 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 			c = (size_t) (k >> 32);
 			const size_t pos = (size_t) (k & 0xffffffffull);
 			if (pos != i) {
-				lu_swap_kernel<<<1, 1, 0, ctx->stream>>>(perm, (int) i, (int) pos);
-				MLZ_TRY(check_launch(ctx, "lu_swap_kernel"));
+				MLZ_TRY(swap_rows((int) i, (int) pos));
 				pending = (long) pos;
 			}
 			break;
@@ -530,17 +906,16 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 		if (!found)
 			break;
 	}
-	const std::vector<int> &st_final = steps, &pc_final = pcols;
-	const size_t rank = st_final.size();
+	const size_t rank = steps.size();
 	if (perm_rows)
 		for (size_t s = 0; s < rank; ++s)
-			perm_rows[s] = (uint32_t) st_final[s];
+			perm_rows[s] = (uint32_t) steps[s];
 	// pluq's column order: the pivot columns swapped into place step by step
 	std::vector<int> co(n), where(n);
 	for (size_t j = 0; j < n; ++j)
 		co[j] = where[j] = (int) j;
 	for (size_t s = 0; s < rank; ++s) {
-		const int cpos = where[pc_final[s]];
+		const int cpos = where[pcols[s]];
 		if (perm_cols)
 			perm_cols[s] = (uint32_t) cpos;
 		const int a = co[s], b = co[cpos];
@@ -549,13 +924,13 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 		where[b] = (int) s;
 		where[a] = cpos;
 	}
-	if (mode == MLZ_ECH_PLUQ)
+	if (pluq)
 		MLZ_HIP(ctx, hipMemcpyAsync(corder, co.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
 	{
 		ProfScope ps(ctx, "lu_store");
 		dim3 g(ceil_div(n, 256), (unsigned) m);
-#define STORE(EE, TT) lu_store_kernel<EE><<<g, 256, 0, ctx->stream>>>(X, ntot, (int) m, (int) n, (int) rank, mode, \
-								     perm, corder, L, rmax, pp, (TT *) A, lda)
+#define STORE(EE, TT) lu_store_kernel<EE><<<g, 256, 0, ctx->stream>>>(X, ldx, (int) m, (int) n, (int) rank, mode, \
+								     orig, corder, Lb, rmax, pp, (TT *) A, lda)
 		if (elem == MLZ_U16)
 			STORE(MLZ_U16, uint16_t);
 		else if (elem == MLZ_U32)
