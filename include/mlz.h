@@ -88,6 +88,10 @@ int mlz_ctx_set_ptr_mode(mlz_ctx *ctx, int mode);
  *                      and inverse (one workgroup per panel): 256, 512 or 1024.
  *   "rref.update.blocks" 32-column blocks per wave of the echelon form's
  *                      trailing update on INT8 limbs: 1 (default), 2 or 4.
+ *   "update.i8.lds"    that update's full 32x32 tiles staged in LDS by direct
+ *                      loads, 1, 2 or 4 tiles per wave; 0 = registers only;
+ *                      -1 (default) = 2 when rows start on 128-byte
+ *                      boundaries, else 1.
  *   "gf2.gemm.kernel"  GF(2) gemm: 0 = Four Russians tables in LDS (default),
  *                      1 = the plain AND/XOR-popcount dot form. */
 int mlz_ctx_set_option(mlz_ctx *ctx, const char *key, int64_t value);
@@ -200,8 +204,10 @@ int mlz_zp_echelonize(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t 
  * [m+1], col [nnz] strictly increasing within each row, val [nnz] in the
  * element storage) to its reduced row-echelon form R (m x n dense, ldr; may be
  * NULL to compute only rank/pivots).  pivots receives rank column indices
- * (array of min(m, n) entries, may be NULL).  det receives the product of the
- * heads of the scanned rows of X (lela/algorithms/faugere-lachartre.tcc:119).
+ * (array of min(m, n) entries, may be NULL).  det (may be NULL, which skips
+ * its work) receives LELA's determinant output: the product of the heads of the
+ * scanned rows of X (lela/algorithms/faugere-lachartre.tcc:119) and of the rows
+ * of the non-reduced echelon form of D - C A^-1 B (:822, :833).
  * Replaces FaugereLachartre<Modular<T>>::echelonize(R, X, rank, det, reduced=true)
  * (lela/algorithms/faugere-lachartre.tcc:735-917).
  * MLZ_E_WRONG_FORM: a head decreases at row *bad_row (WrongMatrixForm, :80-81).

@@ -307,6 +307,7 @@ struct PendingEv {
 struct mlz_ctx {
 	int device;
 	hipStream_t stream;
+	hipStream_t aux_stream;  // second stream (default priority) for overlapped work; created on first use
 	int ptr_mode;
 	std::string last_error;
 	void *ws[mlz::WS_NUM_SLOTS];
@@ -321,6 +322,8 @@ struct mlz_ctx {
 	size_t host_pinned_bytes;
 	int i8_variant;      // INT8 gemm tile variant, 0 = chosen by shape
 	int upd_nblk;        // 32-column blocks per wave of the rank-k update (1, 2, 4)
+	int upd_lds;         // INT8 rank-k update: full tiles staged in LDS, this many per wave (1, 2, 4); 0: registers
+	                     // only; -1: chosen by the row alignment
 	int rref_fast_threads;   // threads of the echelon form's fused pivot-search kernel (256, 512, 1024)
 	uint32_t invtab_p;       // modulus whose inverse table WS_INVTAB holds (0 = none)
 	int gf2_kernel;          // GF(2) gemm: 0 Four Russians (default), 1 the AND/XOR-popcount dot form
@@ -417,8 +420,10 @@ int zp_trsm_banded_inv_i8(mlz_ctx *ctx, uint64_t p, size_t n, size_t nrhs, const
 int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A, size_t lda,
 		size_t *rank_out, uint32_t *pivots_out, int algo);
 // Non-reduced echelon forms with LELA's transposition pivoting (zp_lu.hip).
+// store = false: A is only read (rank, det and the lists are still returned).
 int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A, size_t lda, int mode,
-		   size_t *rank_out, uint64_t *det_out, uint32_t *perm_rows, uint32_t *perm_cols, int algo);
+		   size_t *rank_out, uint64_t *det_out, uint32_t *perm_rows, uint32_t *perm_cols, int algo,
+		   bool store = true);
 // Table of inverses mod p <= 2^16 in device memory (0 where none exists).
 int zp_inverse_table(mlz_ctx *ctx, const ModP &mp, const uint16_t **out);
 int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda,
@@ -440,7 +445,9 @@ int zp_rank_update_i8_dev(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k
 // The same update with G (m x 32*KB) and W (32*KB x n) already packed as INT8
 // limb fragments ([32-index block][k block][limb][64 lanes][16 bytes]).
 int zp_rank_update_i8_packed(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, int KB, const int8_t *Gp,
-			     const int8_t *Wp, uint32_t *C, size_t ldc);
+			     const int8_t *Wp, uint32_t *C, size_t ldc, hipStream_t stream = nullptr);
+// The context's second stream, created on first use (default priority).
+int aux_stream(mlz_ctx *ctx, hipStream_t *out);
 int check_modulus(mlz_ctx *ctx, int elem, uint64_t p);
 void tuner_free(mlz_ctx *ctx);
 // Looks up (once per device, with the device current) the address of the

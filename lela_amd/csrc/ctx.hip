@@ -70,7 +70,7 @@ void prof_begin(mlz_ctx *ctx, const char *name, PendingEv *ev)
 {
 	ev->name = name;
 	ev->e0 = ev->e1 = nullptr;
-	if (!ctx->prof_on || ctx->noalloc)
+	if (!ctx->prof_on || ctx->noalloc || !name)
 		return;
 	ev->e0 = get_event(ctx);
 	ev->e1 = get_event(ctx);
@@ -131,6 +131,20 @@ int stage_out(mlz_ctx *ctx, void *host, const void *dev, size_t rows, size_t col
 	return MLZ_OK;
 }
 
+int aux_stream(mlz_ctx *ctx, hipStream_t *out)
+{
+	if (!ctx->aux_stream) {
+		// Default priority only: streams of different priorities are not used.
+		const hipError_t e = hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking);
+		if (e != hipSuccess) {
+			ctx->aux_stream = nullptr;
+			return set_error(ctx, MLZ_E_HIP, "hipStreamCreateWithFlags: %s", hipGetErrorString(e));
+		}
+	}
+	*out = ctx->aux_stream;
+	return MLZ_OK;
+}
+
 }  // namespace mlz
 
 using namespace mlz;
@@ -166,6 +180,7 @@ int mlz_ctx_create(mlz_ctx **out, int device, const char *config)
 	ctx->host_pinned_bytes = 0;
 	ctx->i8_variant = 0;
 	ctx->upd_nblk = 1;
+	ctx->upd_lds = -1;
 	ctx->rref_fast_threads = 1024;
 	ctx->invtab_p = 0;
 	ctx->gf2_kernel = 0;
@@ -173,6 +188,7 @@ int mlz_ctx_create(mlz_ctx **out, int device, const char *config)
 	ctx->noalloc = false;
 	ctx->fixed_ws = nullptr;
 	ctx->fixed_ws_bytes = 0;
+	ctx->aux_stream = nullptr;
 	// Default priority only: streams of different priorities are not used.
 	if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
 		delete ctx;
@@ -213,6 +229,8 @@ int mlz_ctx_destroy(mlz_ctx *ctx)
 	(void) hipEventDestroy(ctx->timer1);
 	if (ctx->host_pinned)
 		(void) hipHostFree(ctx->host_pinned);
+	if (ctx->aux_stream)
+		(void) hipStreamDestroy(ctx->aux_stream);
 	(void) hipStreamDestroy(ctx->stream);
 	delete ctx;
 	return MLZ_OK;
@@ -246,6 +264,12 @@ int mlz_ctx_set_option(mlz_ctx *ctx, const char *key, int64_t value)
 		if (value != 1 && value != 2 && value != 4)
 			return set_error(ctx, MLZ_E_ARG, "rref.update.blocks must be 1, 2 or 4");
 		ctx->upd_nblk = (int) value;
+		return MLZ_OK;
+	}
+	if (strcmp(key, "update.i8.lds") == 0) {
+		if (value != -1 && value != 0 && value != 1 && value != 2 && value != 4)
+			return set_error(ctx, MLZ_E_ARG, "update.i8.lds must be -1, 0, 1, 2 or 4");
+		ctx->upd_lds = (int) value;
 		return MLZ_OK;
 	}
 	if (strcmp(key, "gf2.gemm.kernel") == 0) {

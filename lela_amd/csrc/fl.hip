@@ -18,12 +18,16 @@
 //   4. D <- D - C B  (:805, Schur complement; zp_gemm.hip)
 //   5. D <- RREF(D)  (:822-852; zp_rref.hip) -- LELA takes the non-reduced
 //      echelon form and reduces it with D1^-1 D2 (:861); RREF(D) is that
-//      result directly.
+//      result directly.  When det is asked for, the non-reduced form's heads
+//      (setup_splicer on D, :833) come from zp_lu.hip's elimination of D with
+//      LELA's transposition pivoting: for a dense D, EchelonForm::echelonize
+//      runs GaussJordan::echelonize and move_L (solutions/echelon-form.h:
+//      124-131), whose result LELA's own test asserts equal to
+//      Elimination::echelonize's (tests/test-gauss-jordan.C:92-99).
 //   6. B <- B - B[:, J_D] RREF(D)  (:883-892, B2 - B1 D1^-1 D2)
 //   7. assembly of the standard-layout RREF (MatrixGrid3, :901; :363-504).
-// det: the factor from the X scan (:119) is returned; the factor LELA adds
-// from the D scan (:833) depends on its non-canonical echelon form of D and
-// is not reproduced.
+// det (when asked for): the product of the heads of the X scan (:119) and of
+// the rows of D's non-reduced echelon form (:833).
 #include "common.hpp"
 
 #include <algorithm>
@@ -207,7 +211,7 @@ struct FlResult {
 
 template <int E>
 static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *d_rowptr, const uint32_t *d_col,
-		  const void *d_val, void *R, size_t ldr, uint32_t *pivots_out, FlResult *res, int algo)
+		  const void *d_val, void *R, size_t ldr, uint32_t *pivots_out, FlResult *res, int algo, bool want_det)
 {
 	typedef typename Elem<E>::T T;
 	const ModP mp = make_modp(p);
@@ -350,6 +354,13 @@ static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *
 	// 4. D <- D - C B
 	if (mr && nr && r)
 		MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, mr, nr, r, p - 1, dC, r, dB, nr, 1, dD, nr, algo));
+	// det's factor from D: the heads of its non-reduced echelon form (D is only read)
+	if (want_det && mr && nr) {
+		size_t rl = 0;
+		uint64_t dl = 1;
+		MLZ_TRY(zp_echelon_dev(ctx, MLZ_U32, p, mr, nr, dD, nr, MLZ_ECH_U, &rl, &dl, nullptr, nullptr, algo, false));
+		res->det = mp.mul((uint32_t) res->det, (uint32_t) dl);
+	}
 	// 5. RREF(D)
 	size_t rd = 0;
 	std::vector<uint32_t> jd(std::min(mr, nr) + 1);
@@ -492,11 +503,11 @@ extern "C" int mlz_zp_fl_echelonize(mlz_ctx *ctx, mlz_elem elem, uint64_t p, siz
 	const uint32_t pp = (uint32_t) p;
 	const size_t ldr_dev = ctx->ptr_mode == MLZ_PTR_DEVICE ? ldr : n;
 	if (elem == MLZ_U16)
-		rc = fl_run<MLZ_U16>(ctx, pp, m, n, d_rowptr, d_col, d_val, dR, ldr_dev, pivots, &res, algo);
+		rc = fl_run<MLZ_U16>(ctx, pp, m, n, d_rowptr, d_col, d_val, dR, ldr_dev, pivots, &res, algo, det != nullptr);
 	else if (elem == MLZ_U32)
-		rc = fl_run<MLZ_U32>(ctx, pp, m, n, d_rowptr, d_col, d_val, dR, ldr_dev, pivots, &res, algo);
+		rc = fl_run<MLZ_U32>(ctx, pp, m, n, d_rowptr, d_col, d_val, dR, ldr_dev, pivots, &res, algo, det != nullptr);
 	else
-		rc = fl_run<MLZ_F64_SYM>(ctx, pp, m, n, d_rowptr, d_col, d_val, dR, ldr_dev, pivots, &res, algo);
+		rc = fl_run<MLZ_F64_SYM>(ctx, pp, m, n, d_rowptr, d_col, d_val, dR, ldr_dev, pivots, &res, algo, det != nullptr);
 	if (rank) *rank = res.rank;
 	if (det) *det = res.det;
 	if (bad_row) *bad_row = res.bad_row;

@@ -836,6 +836,77 @@ __global__ __launch_bounds__(256) void rank_update_i8_kernel(const int8_t *__res
 	}
 }
 
+// The same update for full 32 x 32 tiles of a 16-byte aligned C: each wave
+// takes TPW tiles side by side (one 32-row strip), loads them straight into
+// LDS first (global_load_lds_dwordx4, four loads of eight rows per tile; no
+// registers hold them, so the loads of all TPW tiles are in flight while the
+// MFMAs run), then per tile reads C in the accumulator layout, writes the
+// results back to LDS and stores them as 16-byte rows.  The G fragments are
+// loaded once per strip.  A wave touches only its own LDS tiles and its LDS
+// operations complete in order: no barriers.
+template <int TPW>
+__global__ __launch_bounds__(256) void rank_update_i8_lds_kernel(const int8_t *__restrict__ Gp,
+								 const int8_t *__restrict__ Wp, int KB,
+								 uint32_t *__restrict__ C, size_t ldc, int m, int sns,
+								 ModP mp)
+{
+	__shared__ __attribute__((aligned(16))) uint32_t tile[4 * TPW * 32 * 32];
+	const int lane = threadIdx.x & 63;
+	const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	const int wid = blockIdx.x * 4 + wv;
+	const int rb = wid / sns, sn = wid - rb * sns;
+	if (rb * 32 >= m)
+		return;
+	uint32_t *T = tile + wv * (TPW * 1024);
+	uint32_t *Cw = C + (size_t) (rb * 32) * ldc + sn * (32 * TPW);
+	const int rows = min(32, m - rb * 32);     // a last partial row block: rows past m are
+						   // loaded from the last row and not stored
+	// tile j, load q: rows 8q .. 8q+7; lane l: row 8q + l/8, columns
+	// 32j + 4(l%8) .. +3, landing at T + 1024j + 256q + 4l (row-major tiles)
+#pragma unroll
+	for (int j = 0; j < TPW; ++j)
+#pragma unroll
+		for (int q = 0; q < 4; ++q)
+			__builtin_amdgcn_global_load_lds((const MLZ_AS_GLOBAL void *) (Cw + (size_t) min(8 * q + (lane >> 3), rows - 1) *
+										  ldc + 32 * j + 4 * (lane & 7)),
+							 (MLZ_AS_LDS void *) (T + 1024 * j + 256 * q), 16, 0, 0);
+	const size_t panel = (size_t) KB * 2 * I8_FRAG;
+	const int8_t *ga = Gp + (size_t) rb * panel + lane * 16;
+	const uint32_t p = (uint32_t) mp.p;
+	const int col = lane & 31;
+	for (int j = 0; j < TPW; ++j) {
+		const int8_t *gb = Wp + (size_t) (sn * TPW + j) * panel + lane * 16;
+		v16i acc0 = v16i{0}, acc1 = v16i{0}, acc2 = v16i{0};
+		for (int kb = 0; kb < KB; ++kb) {
+			const v4i ah = *(const v4i *) (ga + (size_t) kb * 2 * I8_FRAG);
+			const v4i al = *(const v4i *) (ga + (size_t) kb * 2 * I8_FRAG + I8_FRAG);
+			const v4i bh = *(const v4i *) (gb + (size_t) kb * 2 * I8_FRAG);
+			const v4i bl = *(const v4i *) (gb + (size_t) kb * 2 * I8_FRAG + I8_FRAG);
+			acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bh, acc0, 0, 0, 0);
+			acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bl, acc1, 0, 0, 0);
+			acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl, acc2, 0, 0, 0);
+			acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, acc1, 0, 0, 0);
+		}
+		// all of this wave's loads are done, the tiles among them
+		wait_vmcnt<0>();
+		asm volatile("" ::: "memory");
+		uint32_t *Tj = T + 1024 * j;
+#pragma unroll
+		for (int r = 0; r < 16; ++r) {
+			const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+			const double v = fma((double) acc0[r], 65536.0, fma((double) acc1[r], 256.0, (double) acc2[r]));
+			const uint32_t d = mp.mod_f64(v);
+			const uint32_t c = Tj[row * 32 + col];
+			Tj[row * 32 + col] = c >= d ? c - d : c + p - d;
+		}
+#pragma unroll
+		for (int q = 0; q < 4; ++q)
+			if (8 * q + (lane >> 3) < rows)
+				*(uint4 *) (Cw + (size_t) (8 * q + (lane >> 3)) * ldc + 32 * j + 4 * (lane & 7)) =
+					*(const uint4 *) (Tj + 256 * q + 4 * lane);
+	}
+}
+
 // C (m x n, canonical uint32, leading dimension ldc) <- C - G*W mod p with G
 // m x k (ldg) and W k x n (ldw), p <= 2^16, k <= I8_KMAX.
 int zp_rank_update_i8_dev(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, const uint32_t *G, size_t ldg,
@@ -874,29 +945,58 @@ int zp_rank_update_i8_dev(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k
 	return zp_rank_update_i8_packed(ctx, p, m, n, KB, (const int8_t *) gp, (const int8_t *) wp, C, ldc);
 }
 
+static int rank_update_i8_reg(mlz_ctx *ctx, const ModP &mp, size_t m, size_t n, int KB, const int8_t *gp,
+			      const int8_t *wp, uint32_t *C, size_t ldc, hipStream_t stream);
+
 int zp_rank_update_i8_packed(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, int KB, const int8_t *gp,
-			     const int8_t *wp, uint32_t *C, size_t ldc)
+			     const int8_t *wp, uint32_t *C, size_t ldc, hipStream_t stream)
 {
+	if (!stream)
+		stream = ctx->stream;
 	if (m == 0 || n == 0)
 		return MLZ_OK;
 	const ModP mp = make_modp(p);
+	ProfScope ps(ctx, stream == ctx->stream ? "rank_update_i8" : nullptr);
+	// full tiles of an aligned C on the LDS-staged kernel, the edge strips below
+	// tiles per wave: two when the rows start on 128-byte boundaries, else one
+	// (measured); the context option fixes it
+	const int tpw = ctx->upd_lds >= 0 ? ctx->upd_lds : ((ldc & 31) == 0 ? 2 : 1);
+	const size_t rbs = ceil_div(m, 32), sns = tpw > 0 ? n / (32 * (size_t) tpw) : 0, cbs = sns * (size_t) tpw;
+	if (tpw > 0 && sns && (ldc & 3) == 0 && ((uintptr_t) C & 15) == 0) {
+		const unsigned grid = ceil_div(rbs * sns, 4);
+		if (tpw == 1)
+			rank_update_i8_lds_kernel<1><<<grid, 256, 0, stream>>>(gp, wp, KB, C, ldc, (int) m, (int) sns, mp);
+		else if (tpw == 2)
+			rank_update_i8_lds_kernel<2><<<grid, 256, 0, stream>>>(gp, wp, KB, C, ldc, (int) m, (int) sns, mp);
+		else
+			rank_update_i8_lds_kernel<4><<<grid, 256, 0, stream>>>(gp, wp, KB, C, ldc, (int) m, (int) sns, mp);
+		MLZ_TRY(check_launch(ctx, "rank_update_i8_lds_kernel"));
+		const size_t panel = (size_t) KB * 2 * I8_FRAG;
+		if (n > cbs * 32)        // the last columns
+			MLZ_TRY(rank_update_i8_reg(ctx, mp, m, n - cbs * 32, KB, gp, wp + cbs * panel, C + cbs * 32, ldc, stream));
+		return MLZ_OK;
+	}
+	return rank_update_i8_reg(ctx, mp, m, n, KB, gp, wp, C, ldc, stream);
+}
+
+// The register-staged kernel over all of C.
+static int rank_update_i8_reg(mlz_ctx *ctx, const ModP &mp, size_t m, size_t n, int KB, const int8_t *gp,
+			      const int8_t *wp, uint32_t *C, size_t ldc, hipStream_t stream)
+{
 	const size_t mpad = round_up(m, 64);
-	// strip width (32-column blocks per wave): wider strips reuse each row
-	// block's fragments more, narrower ones leave room for more waves
 	const int nblk = ctx->upd_nblk == 2 || ctx->upd_nblk == 4 ? ctx->upd_nblk : 1;
 	const int strips_n = (int) ceil_div(n, 32 * nblk);
 	const size_t waves = (mpad / 32) * (size_t) strips_n;
 	const unsigned grid = ceil_div(waves, 4);
-	ProfScope ps(ctx, "rank_update_i8");
 	if (nblk == 1)
-		rank_update_i8_kernel<1><<<grid, 256, 0, ctx->stream>>>((const int8_t *) gp, (const int8_t *) wp, KB, C, ldc,
-									(int) m, (int) n, mp, strips_n);
+		rank_update_i8_kernel<1><<<grid, 256, 0, stream>>>((const int8_t *) gp, (const int8_t *) wp, KB, C, ldc,
+								   (int) m, (int) n, mp, strips_n);
 	else if (nblk == 4)
-		rank_update_i8_kernel<4><<<grid, 256, 0, ctx->stream>>>((const int8_t *) gp, (const int8_t *) wp, KB, C, ldc,
-									(int) m, (int) n, mp, strips_n);
+		rank_update_i8_kernel<4><<<grid, 256, 0, stream>>>((const int8_t *) gp, (const int8_t *) wp, KB, C, ldc,
+								   (int) m, (int) n, mp, strips_n);
 	else
-		rank_update_i8_kernel<2><<<grid, 256, 0, ctx->stream>>>((const int8_t *) gp, (const int8_t *) wp, KB, C, ldc,
-									(int) m, (int) n, mp, strips_n);
+		rank_update_i8_kernel<2><<<grid, 256, 0, stream>>>((const int8_t *) gp, (const int8_t *) wp, KB, C, ldc,
+								   (int) m, (int) n, mp, strips_n);
 	return check_launch(ctx, "rank_update_i8_kernel");
 }
 

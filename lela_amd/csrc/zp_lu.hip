@@ -48,6 +48,7 @@ constexpr int LU_W = 64;     // panel width (columns)
 constexpr int LU_K = 64;     // window rows (one per lane)
 constexpr int LU_NT = 512;   // threads of the window kernel: 8 waves of 8 panel columns
 constexpr int LU_SEARCH_COLS = 2048;   // columns per launch of the pivot search
+constexpr int LU_NARROW = 128;         // columns of a panel's update done before the next panel (multiple of 32)
 constexpr uint32_t LU_EV_NONE = ~0u;    // a column without a nonzero in the window
 constexpr int LU_SPIN_MAX = 1 << 22;   // polls of a column's flag before the kernel reports an error
 
@@ -467,7 +468,8 @@ __global__ void lu_gather_mult_kernel(const uint32_t *__restrict__ X, size_t ldx
 }
 
 // The panel's update for p <= 2^16, fused (the INT8 rank update follows).
-// Workgroup j owns the columns [c + TC j, c + TC j + TC) of the window rows:
+// Workgroup j owns the columns [c + TC b, c + TC b + TC), b = cb0 + j, of the
+// window rows:
 // it reads them in the window's new order (position q: window row wperm[q]),
 // forms [L11^-1; P_J^-1] P on v_mfma_f64_16x16x4f64 (64 terms below 2^32:
 // exact in a double), writes the new pivot rows and the remaining window rows
@@ -477,14 +479,16 @@ __global__ void lu_gather_mult_kernel(const uint32_t *__restrict__ X, size_t ldx
 template <int TC>
 __global__ __launch_bounds__(256) void lu_wb_i8_kernel(uint32_t *__restrict__ X, size_t ldx, int nc, int c, int i0,
 						       int kw, const LuPanel *__restrict__ st,
-						       const uint32_t *__restrict__ LiM, int8_t *__restrict__ Wp, ModP mp)
+						       const uint32_t *__restrict__ LiM, int8_t *__restrict__ Wp, ModP mp,
+						       int cb0)
 {
 	static_assert(TC == 32, "column slice of 32");
 	__shared__ uint32_t Ps[LU_K][TC + 1], Pm[2 * LU_W][LU_W + 1], Os[2 * LU_W][TC + 1];
 	__shared__ int wq[LU_K];
 	const uint32_t p = (uint32_t) mp.p;
 	const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-	const int j0 = blockIdx.x * TC;
+	const int cbk = cb0 + (int) blockIdx.x;     // the slice's block index (W fragments too)
+	const int j0 = cbk * TC;
 	const int rp = st->rp;
 	if (tid < LU_K)
 		wq[tid] = st->wperm[tid];
@@ -545,7 +549,7 @@ __global__ __launch_bounds__(256) void lu_wb_i8_kernel(uint32_t *__restrict__ X,
 			hw[b >> 2] |= ((lb >> 8) & 0xffu) << (8 * (b & 3));
 			lw[b >> 2] |= (lb & 0xffu) << (8 * (b & 3));
 		}
-		int8_t *fr = Wp + ((size_t) blockIdx.x * 2 + q) * 2 * 1024 + lane * 16;
+		int8_t *fr = Wp + ((size_t) cbk * 2 + q) * 2 * 1024 + lane * 16;
 		*(uint4 *) fr = make_uint4(hw[0], hw[1], hw[2], hw[3]);
 		*(uint4 *) (fr + 1024) = make_uint4(lw[0], lw[1], lw[2], lw[3]);
 	}
@@ -667,7 +671,7 @@ __global__ void lu_store_kernel(const uint32_t *__restrict__ X, size_t ldx, int 
 }
 
 int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A, size_t lda, int mode,
-		   size_t *rank_out, uint64_t *det_out, uint32_t *perm_rows, uint32_t *perm_cols, int algo)
+		   size_t *rank_out, uint64_t *det_out, uint32_t *perm_rows, uint32_t *perm_cols, int algo, bool store)
 {
 	if (m == 0 || n == 0) {
 		if (rank_out)
@@ -695,19 +699,27 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 	const size_t sG = round_up(m * LU_W * 4, 256);
 	MLZ_TRY(ws_get(ctx, WS_RREF_2, sG + (pluq ? m * rmax * 4 : 0), &w2));
 	const size_t sq = round_up(LU_W * LU_W * 4, 256);
-	const size_t small = 4 * sq + round_up(m * 4, 256) + round_up(n * 4, 256) + round_up(sizeof(LuPanel), 256) + 256;
+	const size_t small = 6 * sq + round_up(m * 4, 256) + round_up(n * 4, 256) + 2 * round_up(sizeof(LuPanel), 256) + 256;
 	MLZ_TRY(ws_get(ctx, WS_RREF_3, small, &w3));
 	uint32_t *X = (uint32_t *) w0;
 	uint32_t *Xw = (uint32_t *) w1, *UW = (uint32_t *) ((char *) w1 + sXw);
 	uint32_t *G = (uint32_t *) w2, *Lb = (uint32_t *) ((char *) w2 + sG);
 	char *q = (char *) w3;
+	// the panel state and LiM alternate between two copies (a panel's update
+	// may still read them while the next panel's kernel runs)
 	uint32_t *L11 = (uint32_t *) q; q += sq;
-	uint32_t *LiM = (uint32_t *) q; q += 2 * sq;
+	uint32_t *LiMb[2];
+	LiMb[0] = (uint32_t *) q; q += 2 * sq;
+	LiMb[1] = (uint32_t *) q; q += 2 * sq;
 	uint32_t *Ui = (uint32_t *) q; q += sq;
 	int *orig = (int *) q; q += round_up(m * 4, 256);
 	int *corder = (int *) q; q += round_up(n * 4, 256);
-	LuPanel *st = (LuPanel *) q; q += round_up(sizeof(LuPanel), 256);
+	LuPanel *stb[2];
+	stb[0] = (LuPanel *) q; q += round_up(sizeof(LuPanel), 256);
+	stb[1] = (LuPanel *) q; q += round_up(sizeof(LuPanel), 256);
 	unsigned long long *key = (unsigned long long *) q;
+	LuPanel *st = stb[0];
+	uint32_t *LiM = LiMb[0];
 
 	const uint16_t *invtab = nullptr;
 	if (p <= 65536)
@@ -761,26 +773,83 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 	// The update of the panel at (i, c).  With rp < 0 (not known yet) every
 	// size is taken at its maximum: the rows from i on (the pivot rows get
 	// zero multipliers), all LU_W rows of P (LiM is zero past rp).
-	// p <= 2^16 without pluq: the fused kernels and the packed INT8 update
+	// p <= 2^16 without pluq: the fused kernels and the packed INT8 update,
+	// split in two.  The narrow part (the first LU_NARROW columns from c, which
+	// hold the next panel) runs on the context's stream; the wide part (the
+	// rest) on the second stream, so that it overlaps the next panel's kernel.
+	// The next narrow part waits for this wide part (their columns meet); the
+	// packed operands alternate between two copies.
 	const bool fused = upd_i8 && !pluq;
-	int8_t *Gpk = nullptr, *Wpk = nullptr;
+	int8_t *Gpk[2] = {nullptr, nullptr}, *Wpk[2] = {nullptr, nullptr};
+	struct Overlap {
+		hipStream_t s1 = nullptr;
+		hipEvent_t nev[2] = {nullptr, nullptr}, uev[2] = {nullptr, nullptr};
+		~Overlap()
+		{
+			if (s1)
+				(void) hipStreamSynchronize(s1);
+			for (int b = 0; b < 2; ++b) {
+				if (nev[b])
+					(void) hipEventDestroy(nev[b]);
+				if (uev[b])
+					(void) hipEventDestroy(uev[b]);
+			}
+		}
+	} ov;
+	bool wide_live = false;      // a wide part is queued on the second stream
+	int wide_b = 0;              // ... with this copy
 	if (fused) {
 		void *gq, *wq;
-		MLZ_TRY(ws_get(ctx, WS_PACK_A, round_up(m, 64) * LU_W * 2, &gq));
-		MLZ_TRY(ws_get(ctx, WS_PACK_B, round_up(ntot, 64) * LU_W * 2, &wq));
-		Gpk = (int8_t *) gq;
-		Wpk = (int8_t *) wq;
+		const size_t gb = round_up(round_up(m, 64) * LU_W * 2, 256), wb = round_up(round_up(ntot, 64) * LU_W * 2, 256);
+		MLZ_TRY(ws_get(ctx, WS_PACK_A, 2 * gb, &gq));
+		MLZ_TRY(ws_get(ctx, WS_PACK_B, 2 * wb, &wq));
+		Gpk[0] = (int8_t *) gq;
+		Gpk[1] = (int8_t *) gq + gb;
+		Wpk[0] = (int8_t *) wq;
+		Wpk[1] = (int8_t *) wq + wb;
+		hipStream_t s1;
+		MLZ_TRY(aux_stream(ctx, &s1));
+		ov.s1 = s1;
+		for (int b = 0; b < 2; ++b) {
+			MLZ_HIP(ctx, hipEventCreateWithFlags(&ov.nev[b], hipEventDisableTiming));
+			MLZ_HIP(ctx, hipEventCreateWithFlags(&ov.uev[b], hipEventDisableTiming));
+		}
 	}
-	auto update_fused = [&](int kw) -> int {
+	// the context's stream waits for the queued wide part
+	auto join_wide = [&]() -> int {
+		if (wide_live)
+			MLZ_HIP(ctx, hipStreamWaitEvent(ctx->stream, ov.uev[wide_b], 0));
+		wide_live = false;
+		return MLZ_OK;
+	};
+	auto update_fused = [&](int kw, int b) -> int {
 		const size_t nc = ntot - c, mb = m - i;
-		ProfScope ps(ctx, "lu_update");
-		lu_window_orig_kernel<<<1, LU_K, 0, ctx->stream>>>(orig, (int) i, kw, st);
-		lu_wb_i8_kernel<32><<<ceil_div(nc, 32), 256, 0, ctx->stream>>>(X, ldx, (int) nc, (int) c, (int) i, kw, st,
-										    LiM, Wpk, mp);
-		lu_gpack_i8_kernel<<<ceil_div(mb, 64), 256, 0, ctx->stream>>>(X, ldx, (int) mb, (int) i, (int) c, st, Gpk,
-										pp);
-		MLZ_TRY(check_launch(ctx, "lu_gpack_i8_kernel"));
-		return zp_rank_update_i8_packed(ctx, pp, mb, nc, 2, Gpk, Wpk, X + i * ldx + c, ldx);
+		const size_t nn = std::min(nc, (size_t) LU_NARROW);
+		MLZ_TRY(join_wide());
+		{
+			ProfScope ps(ctx, "lu_update");
+			lu_window_orig_kernel<<<1, LU_K, 0, ctx->stream>>>(orig, (int) i, kw, st);
+			lu_wb_i8_kernel<32><<<ceil_div(nn, 32), 256, 0, ctx->stream>>>(X, ldx, (int) nc, (int) c, (int) i, kw,
+											    st, LiM, Wpk[b], mp, 0);
+			lu_gpack_i8_kernel<<<ceil_div(mb, 64), 256, 0, ctx->stream>>>(X, ldx, (int) mb, (int) i, (int) c, st,
+											Gpk[b], pp);
+			MLZ_TRY(check_launch(ctx, "lu_gpack_i8_kernel"));
+			MLZ_TRY(zp_rank_update_i8_packed(ctx, pp, mb, nn, 2, Gpk[b], Wpk[b], X + i * ldx + c, ldx));
+		}
+		if (nc > nn) {
+			const int nbn = (int) (nn / 32);
+			MLZ_HIP(ctx, hipEventRecord(ov.nev[b], ctx->stream));
+			MLZ_HIP(ctx, hipStreamWaitEvent(ov.s1, ov.nev[b], 0));
+			lu_wb_i8_kernel<32><<<ceil_div(nc, 32) - nbn, 256, 0, ov.s1>>>(X, ldx, (int) nc, (int) c, (int) i, kw,
+											    st, LiM, Wpk[b], mp, nbn);
+			MLZ_TRY(check_launch(ctx, "lu_wb_i8_kernel"));
+			MLZ_TRY(zp_rank_update_i8_packed(ctx, pp, mb, nc - nn, 2, Gpk[b], Wpk[b] + (size_t) nbn * 4096,
+							 X + i * ldx + c + nn, ldx, ov.s1));
+			MLZ_HIP(ctx, hipEventRecord(ov.uev[b], ov.s1));
+			wide_live = true;
+			wide_b = b;
+		}
+		return MLZ_OK;
 	};
 	auto update = [&](int kw, int rp) -> int {
 		const size_t nc = ntot - c;
@@ -831,9 +900,14 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 												(int) i, st, UW);
 		return check_launch(ctx, "lu_scatter_pivots_kernel");
 	};
+	int panel = 0;
 	while (i < m && c < n) {
 		const int wc = (int) std::min((size_t) LU_W, n - c);
 		const int kw = (int) std::min((size_t) LU_K, m - i);
+		const int pb = fused ? (panel & 1) : 0;
+		++panel;
+		st = stb[pb];
+		LiM = LiMb[pb];
 		{
 			ProfScope ps(ctx, "lu_window");
 			if (invtab)
@@ -852,7 +926,7 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 		// pluq's sizes depend on rp; otherwise the update is queued at once
 		const bool early = !pluq;
 		if (early)
-			MLZ_TRY(fused ? update_fused(kw) : update(kw, -1));
+			MLZ_TRY(fused ? update_fused(kw, pb) : update(kw, -1));
 		MLZ_HIP(ctx, hipEventSynchronize(res.ev));
 		const LuPanel &h = *res.h;
 		if (h.bad) {
@@ -880,6 +954,7 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 		// the window ran out: the next pivot (first column, then first
 		// position) among all remaining rows, searched in column chunks
 		c = h.stop_col;
+		MLZ_TRY(join_wide());
 		bool found = false;
 		while (i < m && c < n) {
 			const size_t c1 = std::min(n, c + LU_SEARCH_COLS);
@@ -906,6 +981,7 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 		if (!found)
 			break;
 	}
+	MLZ_TRY(join_wide());
 	const size_t rank = steps.size();
 	if (perm_rows)
 		for (size_t s = 0; s < rank; ++s)
@@ -924,9 +1000,9 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 		where[b] = (int) s;
 		where[a] = cpos;
 	}
-	if (pluq)
+	if (pluq && store)
 		MLZ_HIP(ctx, hipMemcpyAsync(corder, co.data(), n * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-	{
+	if (store) {
 		ProfScope ps(ctx, "lu_store");
 		dim3 g(ceil_div(n, 256), (unsigned) m);
 #define STORE(EE, TT) lu_store_kernel<EE><<<g, 256, 0, ctx->stream>>>(X, ldx, (int) m, (int) n, (int) rank, mode, \

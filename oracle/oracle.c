@@ -600,16 +600,22 @@ int orc_fl_echelonize(uint32_t p, size_t m, size_t n, const uint32_t *X, size_t 
 			goto out;
 		/* :805 D <- D - C B */
 		orc_zp_gemm_u32(p, mr, nc, r, p - 1, Cm, r, Bm, nc, 1, Dm, nc);
-		/* :822 EF.echelonize(D): GaussJordan non-reduced, equal to Elimination::echelonize
-		 * (asserted at tests/test-gauss-jordan.C:96-99). */
+		/* :822 EF.echelonize(D): for a dense matrix GaussJordan::echelonize and
+		 * move_L (solutions/echelon-form.h:124-131), whose result LELA's test
+		 * asserts equal to Elimination::echelonize's (tests/test-gauss-jordan.C:
+		 * 92-99): orc_zp_elim, LELA's transposition pivoting. */
 		size_t rd = 0;
+		uint32_t ddet = 1 % p;
 		dpiv = (uint32_t *) malloc((nc + 1) * 4);
-		rc = orc_zp_echelonize(p, mr, nc, Dm, nc, 0, &rd, dpiv, NULL);
+		rc = orc_zp_elim(p, mr, nc, Dm, nc, 0, &rd, &ddet, NULL);
 		if (rc)
 			goto out;
-		/* :833 setup_splicer(D): heads of the echelon rows enter det. */
-		for (size_t i = 0; i < rd; ++i)
-			d = zp_mul(p, d, Dm[i * nc + dpiv[i]]);
+		/* :833 setup_splicer(D): the scan of the echelon rows; every head enters det. */
+		for (size_t i = 0; i < rd; ++i) {
+			long h = row_head(Dm + i * nc, nc);
+			dpiv[i] = (uint32_t) h;
+			d = zp_mul(p, d, Dm[i * nc + (size_t) h]);
+		}
 		/* :852-861 D1^-1 D2 over D's pivot / non-pivot columns. */
 		unsigned char *dcol = (unsigned char *) calloc(nc + 1, 1);
 		for (size_t i = 0; i < rd; ++i) dcol[dpiv[i]] = 1;

@@ -11,8 +11,7 @@
  * lela/vector/bit-iterator.h:249-311, lela/matrix/dense-zero-one.h:253-257).
  *
  * Pinning (what checks each function beyond retyping its formula) is listed
- * per function in tests/test_oracle.py; nothing here is "parity unpinned"
- * except orc_fl_echelonize's det output for the D block (see its comment).
+ * per function in tests/test_oracle.py; nothing here is "parity unpinned".
  */
 #ifndef ORACLE_H
 #define ORACLE_H
@@ -80,8 +79,8 @@ int orc_gf2_echelonize(size_t m, size_t n, uint64_t *A, size_t lda_w, int reduce
  * (lela/algorithms/faugere-lachartre.tcc:47-146,735-917) on dense storage.
  * R (m x n, ldr) receives the reduced row-echelon form.  On WrongMatrixForm
  * returns ORC_E_WRONG_FORM with *bad_row set.  det follows :119 for the X scan
- * and the non-reduced Elimination form of D for the D scan (the D part of det
- * is parity unpinned: it depends on LELA's non-canonical echelon form). */
+ * and :833 for the scan of D's non-reduced echelon form, which is
+ * orc_zp_elim's (LELA's transposition pivoting). */
 int orc_fl_echelonize(uint32_t p, size_t m, size_t n, const uint32_t *X, size_t ldx,
 		      uint32_t *R, size_t ldr, size_t *rank, uint32_t *det, size_t *bad_row);
 

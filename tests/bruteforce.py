@@ -216,3 +216,42 @@ def gf2_rref_bits(A):
         if r == m:
             break
     return M, r, pivots
+
+
+def fl_det(X, p):
+    """FaugereLachartre's det output on an F4-shaped matrix from first
+    principles: the heads of the rows scanned in order (a head that repeats
+    marks a C|D row), then D' = D - C A^-1 B with A^-1 by Gauss-Jordan, and
+    the heads of D' in the echelon form fixed by lela_pivot_sequence (each
+    head is U[t][cols[t]] of echelon_given_order).  Returns (det, rank of D')."""
+    X = np.asarray(X)
+    m, n = X.shape
+    d = 1 % p
+    prow, pcol, last = [], [], -1
+    for i in range(m):
+        nz = np.nonzero(X[i] % p)[0]
+        if nz.size == 0:
+            break
+        h = int(nz[0])
+        assert h >= last
+        if h > last:
+            prow.append(i)
+            pcol.append(h)
+            last = h
+        d = d * int(X[i, h]) % p
+    rrows = [i for i in range(m) if i not in prow]
+    rcols = [c for c in range(n) if c not in pcol]
+    if not rrows or not rcols:
+        return d, 0
+    Xo = X.astype(object) % p
+    D = Xo[np.ix_(rrows, rcols)]
+    if prow:
+        A = Xo[np.ix_(prow, pcol)]
+        B = Xo[np.ix_(prow, rcols)]
+        C = Xo[np.ix_(rrows, pcol)]
+        D = (D - matmul_mod(C, matmul_mod(inverse_mod(A, p), B, p), p)) % p
+    steps, order, cols = lela_pivot_sequence(D, p)
+    U = echelon_given_order(D, order, cols, p)
+    for t, c in enumerate(cols):
+        d = d * int(U[t, c]) % p
+    return d, len(cols)

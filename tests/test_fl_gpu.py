@@ -1,5 +1,6 @@
 """GPU parity of mlz_zp_fl_echelonize against the oracle's restatement of
-FaugereLachartre::echelonize (bit-exact RREF and rank; det of the X scan),
+FaugereLachartre::echelonize (bit-exact RREF, rank and det, whose D factor
+comes from the heads of D's non-reduced echelon form),
 the reference test's own assertion that FL equals Elimination's reduced form
 (test-faugere-lachartre.C:227-243), WrongMatrixForm, and the 20000 x 30000
 workload through size-independent checks."""
@@ -39,21 +40,21 @@ def run_fl(ctx, p, storage, X, algo=la.ALGO_AUTO):
 def test_fl_matches_oracle(gpu_ctx, p, m, n, seed):
     X = f4_matrix(seed, m, n, p)
     got, rank, det, piv = run_fl(gpu_ctx, p, "uint32", X)
-    rc, want, wrank, _, _ = orc.fl_echelonize(p, X)
+    rc, want, wrank, wdet, _ = orc.fl_echelonize(p, X)
     assert rc == 0
     assert rank == wrank and np.array_equal(got, want)
     # the reference test's check: FL result == Elimination's reduced form
     rc2, E, erank, epiv, _ = orc.zp_echelonize(p, X, reduced=True)
     assert np.array_equal(got, E) and list(piv) == list(epiv)
-    assert det == x_scan_det(X, p)
+    assert det == wdet
 
 
 @pytest.mark.parametrize("storage,p", [("uint16", 65521), ("double", 67108859)])
 def test_fl_storages(gpu_ctx, storage, p):
     X = f4_matrix(7, 150, 220, p)
-    got, rank, _, _ = run_fl(gpu_ctx, p, storage, X)
-    rc, want, wrank, _, _ = orc.fl_echelonize(p, X)
-    assert rank == wrank and np.array_equal(got, want)
+    got, rank, det, _ = run_fl(gpu_ctx, p, storage, X)
+    rc, want, wrank, wdet, _ = orc.fl_echelonize(p, X)
+    assert rank == wrank and np.array_equal(got, want) and det == wdet
 
 
 @pytest.mark.parametrize("algo", [la.ALGO_PLAIN, la.ALGO_INT8])
@@ -63,10 +64,10 @@ def test_fl_small_config4_shape(gpu_ctx, algo):
     csr, shape = synth.fl_matrix(11, p, na=750, nc=250, nb=750, band=60, nnz_a=15)
     X = synth.csr_to_dense(csr, shape)
     got, rank, det, _ = run_fl(gpu_ctx, p, "uint32", X, algo=algo)
-    rc, want, wrank, _, _ = orc.fl_echelonize(p, X)
+    rc, want, wrank, wdet, _ = orc.fl_echelonize(p, X)
     assert rc == 0 and rank == wrank == 1000
     assert np.array_equal(got, want)
-    assert det == x_scan_det(X, p)
+    assert det == wdet
 
 
 @pytest.mark.parametrize("algo", [la.ALGO_AUTO, la.ALGO_PLAIN])
@@ -76,9 +77,9 @@ def test_fl_multiblock_banded_pivot_block(gpu_ctx, algo):
     csr, shape = synth.fl_matrix(13, p, na=1000, nc=300, nb=1000, band=90, nnz_a=20)
     X = synth.csr_to_dense(csr, shape)
     got, rank, det, _ = run_fl(gpu_ctx, p, "uint32", X, algo=algo)
-    rc, want, wrank, _, _ = orc.fl_echelonize(p, X)
+    rc, want, wrank, wdet, _ = orc.fl_echelonize(p, X)
     assert rc == 0 and rank == wrank
-    assert np.array_equal(got, want)
+    assert np.array_equal(got, want) and det == wdet
 
 
 def test_fl_multiblock_dense_pivot_block(gpu_ctx):
@@ -86,9 +87,9 @@ def test_fl_multiblock_dense_pivot_block(gpu_ctx):
     p = 65521
     X = f4_matrix(14, 1100, 1300, p)
     got, rank, det, _ = run_fl(gpu_ctx, p, "uint32", X)
-    rc, want, wrank, _, _ = orc.fl_echelonize(p, X)
+    rc, want, wrank, wdet, _ = orc.fl_echelonize(p, X)
     assert rc == 0 and rank == wrank and rank > 600
-    assert np.array_equal(got, want)
+    assert np.array_equal(got, want) and det == wdet
 
 
 @pytest.mark.parametrize("which", ["first", "middle"])
@@ -103,9 +104,9 @@ def test_fl_zero_columns_in_d(gpu_ctx, which):
     cols = free[:3] if which == "first" else free[len(free) // 3: len(free) // 3 + 2]
     X[:, cols] = 0
     got, rank, det, piv = run_fl(gpu_ctx, p, "uint32", X)
-    rc, want, wrank, _, _ = orc.fl_echelonize(p, X)
+    rc, want, wrank, wdet, _ = orc.fl_echelonize(p, X)
     assert rc == 0 and rank == wrank
-    assert np.array_equal(got, want)
+    assert np.array_equal(got, want) and det == wdet
 
 
 def test_fl_wrong_matrix_form(gpu_ctx):
@@ -127,9 +128,9 @@ def test_fl_zero_row_ends_scan(gpu_ctx):
     X[10] = 0
     X[25, :] = 0
     X[25, 3] = 5                   # a small head after the zero row: not WrongMatrixForm
-    got, rank, _, _ = run_fl(gpu_ctx, p, "uint32", X)
-    rc, want, wrank, _, _ = orc.fl_echelonize(p, X)
-    assert rc == 0 and rank == wrank and np.array_equal(got, want)
+    got, rank, det, _ = run_fl(gpu_ctx, p, "uint32", X)
+    rc, want, wrank, wdet, _ = orc.fl_echelonize(p, X)
+    assert rc == 0 and rank == wrank and np.array_equal(got, want) and det == wdet
 
 
 def csr_matvec_mod(csr, shape, v, p):
@@ -202,5 +203,5 @@ def test_fl_banded_inverse_solve(gpu_ctx):
     csr, shape = synth.fl_matrix(synth.seed_for(4, 5), p, na=2100, nc=100, nb=200, band=600, nnz_a=150)
     X = synth.csr_to_dense(csr, shape)
     got, rank, det, piv = run_fl(gpu_ctx, p, "uint32", X)
-    rc, want, wrank, _, _ = orc.fl_echelonize(p, X)
-    assert rc == 0 and rank == wrank and np.array_equal(got, want)
+    rc, want, wrank, wdet, _ = orc.fl_echelonize(p, X)
+    assert rc == 0 and rank == wrank and np.array_equal(got, want) and det == wdet

@@ -209,6 +209,19 @@ def test_fl_det_when_all_rows_are_pivots():
     assert rc == orc.ORC_OK and rank == 6 and det == (3 * 5 * 7 * 11 * 13 * 17) % p
 
 
+@pytest.mark.parametrize("p", [3, 101])
+@pytest.mark.parametrize("m,n,seed", [(16, 20, 1), (24, 30, 2), (20, 20, 3), (30, 24, 5)])
+def test_fl_det_from_first_principles(p, m, n, seed):
+    # The D factor (:833) depends on the pivoting of D's non-reduced echelon
+    # form; bruteforce.fl_det builds D - C A^-1 B by Gauss-Jordan and takes
+    # the heads from the independent characterisation of LELA's pivots.
+    X = f4_matrix(seed, m, n, p)
+    rc, R, rank, det, _ = orc.fl_echelonize(p, X)
+    assert rc == orc.ORC_OK
+    want, rd = bf.fl_det(X, p)
+    assert det == want
+
+
 def test_fl_wrong_matrix_form():
     p = 101
     X = np.zeros((4, 6), dtype=np.uint32)

@@ -129,6 +129,29 @@ def test_rref_tuning_options(gpu_ctx, threads, blocks):
         gpu_ctx.set_option("rref.update.blocks", 1)
 
 
+@pytest.mark.parametrize("lds", [0, 1, 2, 4])
+def test_update_staging_options(gpu_ctx, lds):
+    """The INT8 trailing update with its tiles in registers or staged in LDS
+    (1, 2, 4 per wave; partial row blocks, ragged column tails, a row stride
+    that is not a multiple of 4) changes speed only: the reduced form and the
+    non-reduced LU form against the oracle."""
+    p = 65521
+    gpu_ctx.set_option("update.i8.lds", lds)
+    try:
+        for A in (synth.zp_matrix(41, 333, 520, p), lowrank(42, 300, 416, 90, p), synth.zp_matrix(43, 200, 203, p)):
+            got, rank, piv = run_rref(gpu_ctx, p, "uint32", A)
+            rc, want, wrank, wpiv, _ = orc.zp_echelonize(p, A, reduced=True)
+            assert rank == wrank and np.array_equal(got, want) and list(piv) == list(wpiv)
+            F = la.Modular(p, "uint32")
+            gpu_ctx.F = F
+            X = F.from_canonical(A)
+            P, lrank, det = la.Elimination(gpu_ctx).echelonize(X, compute_L=False)
+            rc, U, erank, edet, _ = orc.zp_elim(p, A, compute_L=False)
+            assert lrank == erank and det == edet and np.array_equal(F.to_canonical(X), U)
+    finally:
+        gpu_ctx.set_option("update.i8.lds", -1)
+
+
 def test_rref_composite_modulus(gpu_ctx):
     """p = 15: a pivot without inverse is reported (LELA's check,
     elimination.tcc:217-220); with invertible pivots the result is the
