@@ -421,9 +421,10 @@ int zp_rref_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A,
 		size_t *rank_out, uint32_t *pivots_out, int algo);
 // Non-reduced echelon forms with LELA's transposition pivoting (zp_lu.hip).
 // store = false: A is only read (rank, det and the lists are still returned).
+// pivcols (host, may be null): the pivot column of each step.
 int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A, size_t lda, int mode,
 		   size_t *rank_out, uint64_t *det_out, uint32_t *perm_rows, uint32_t *perm_cols, int algo,
-		   bool store = true);
+		   bool store = true, uint32_t *pivcols = nullptr);
 // Table of inverses mod p <= 2^16 in device memory (0 where none exists).
 int zp_inverse_table(mlz_ctx *ctx, const ModP &mp, const uint16_t **out);
 int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda,

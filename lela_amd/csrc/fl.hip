@@ -209,6 +209,64 @@ struct FlResult {
 	size_t bad_row;
 };
 
+// D's echelon rows 0 .. rd-1 split by a column map: cmap[j] = s >= 0 puts
+// column j at D1[:, s] (a pivot column), -1 - f at D2[:, f].
+__global__ void fl_split_d_kernel(const uint32_t *__restrict__ D, size_t ldd, int nr, const int *__restrict__ cmap,
+				  uint32_t *__restrict__ D1, size_t ld1, uint32_t *__restrict__ D2, size_t ld2)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x, t = blockIdx.y;
+	if (j >= nr)
+		return;
+	const uint32_t v = D[(size_t) t * ldd + j];
+	const int s = cmap[j];
+	if (s >= 0)
+		D1[(size_t) t * ld1 + s] = v;
+	else
+		D2[(size_t) t * ld2 + (-1 - s)] = v;
+}
+
+// Row t of the reduced form: the unit vector on pivot column t, D1^-1 D2 on the others.
+__global__ void fl_join_d_kernel(uint32_t *__restrict__ D, size_t ldd, int nr, const int *__restrict__ cmap,
+				 const uint32_t *__restrict__ D2, size_t ld2)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x, t = blockIdx.y;
+	if (j >= nr)
+		return;
+	const int s = cmap[j];
+	D[(size_t) t * ldd + j] = s >= 0 ? (s == t ? 1u : 0u) : D2[(size_t) t * ld2 + (-1 - s)];
+}
+
+// D (rd echelon rows of nr columns, pivot columns J) <- its reduced form:
+// D2 <- D1^-1 D2 with D1 = D[:, J] upper triangular (:852-861).  The
+// non-reduced rows past rd are zero already.
+static int fl_reduce_d(mlz_ctx *ctx, uint32_t p, size_t rd, size_t nr, uint32_t *D, const uint32_t *J, int algo)
+{
+	const size_t nf = nr - rd;
+	std::vector<int> cmap(nr);
+	for (size_t j = 0, f = 0, s = 0; j < nr; ++j) {
+		if (s < rd && J[s] == j)
+			cmap[j] = (int) s++;
+		else
+			cmap[j] = -1 - (int) f++;
+	}
+	void *wa, *wm;
+	MLZ_TRY(ws_get(ctx, WS_FL_3, (rd * rd + rd * nf) * 4 + 512, &wa));
+	MLZ_TRY(ws_get(ctx, WS_FL_6, nr * sizeof(int) + 256, &wm));
+	uint32_t *D1 = (uint32_t *) wa, *D2 = D1 + round_up(rd * rd, 64);
+	MLZ_HIP(ctx, hipMemcpyAsync(wm, cmap.data(), nr * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+	dim3 g(ceil_div(nr, 256), (unsigned) rd);
+	fl_split_d_kernel<<<g, 256, 0, ctx->stream>>>(D, nr, (int) nr, (const int *) wm, D1, rd, D2, nf ? nf : 1);
+	MLZ_TRY(check_launch(ctx, "fl_split_d_kernel"));
+	if (nf) {
+		size_t bad = 0;
+		MLZ_TRY(zp_trsm_dev(ctx, MLZ_U32, p, 1, 0, rd, nf, 1, D1, rd, D2, nf, &bad, algo));
+	}
+	fl_join_d_kernel<<<g, 256, 0, ctx->stream>>>(D, nr, (int) nr, (const int *) wm, D2, nf ? nf : 1);
+	MLZ_TRY(check_launch(ctx, "fl_join_d_kernel"));
+	MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));   // cmap leaves scope
+	return MLZ_OK;
+}
+
 template <int E>
 static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *d_rowptr, const uint32_t *d_col,
 		  const void *d_val, void *R, size_t ldr, uint32_t *pivots_out, FlResult *res, int algo, bool want_det)
@@ -354,18 +412,20 @@ static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *
 	// 4. D <- D - C B
 	if (mr && nr && r)
 		MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, mr, nr, r, p - 1, dC, r, dB, nr, 1, dD, nr, algo));
-	// det's factor from D: the heads of its non-reduced echelon form (D is only read)
-	if (want_det && mr && nr) {
-		size_t rl = 0;
-		uint64_t dl = 1;
-		MLZ_TRY(zp_echelon_dev(ctx, MLZ_U32, p, mr, nr, dD, nr, MLZ_ECH_U, &rl, &dl, nullptr, nullptr, algo, false));
-		res->det = mp.mul((uint32_t) res->det, (uint32_t) dl);
-	}
-	// 5. RREF(D)
+	// 5. RREF(D).  With det: LELA's way, the non-reduced echelon form (its
+	// heads enter det, :833) reduced by D1^-1 D2 (:852-861); else directly.
 	size_t rd = 0;
 	std::vector<uint32_t> jd(std::min(mr, nr) + 1);
-	if (mr && nr)
+	if (mr && nr && want_det) {
+		uint64_t dl = 1;
+		MLZ_TRY(zp_echelon_dev(ctx, MLZ_U32, p, mr, nr, dD, nr, MLZ_ECH_U, &rd, &dl, nullptr, nullptr, algo, true,
+				       jd.data()));
+		res->det = mp.mul((uint32_t) res->det, (uint32_t) dl);
+		if (rd)
+			MLZ_TRY(fl_reduce_d(ctx, p, rd, nr, dD, jd.data(), algo));
+	} else if (mr && nr) {
 		MLZ_TRY(zp_rref_dev(ctx, MLZ_U32, p, mr, nr, dD, nr, &rd, jd.data(), algo));
+	}
 	// 6. B <- B - B[:, J_D] RREF(D)[0:rd]
 	if (r && rd) {
 		void *wg, *wj;

@@ -671,7 +671,8 @@ __global__ void lu_store_kernel(const uint32_t *__restrict__ X, size_t ldx, int 
 }
 
 int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A, size_t lda, int mode,
-		   size_t *rank_out, uint64_t *det_out, uint32_t *perm_rows, uint32_t *perm_cols, int algo, bool store)
+		   size_t *rank_out, uint64_t *det_out, uint32_t *perm_rows, uint32_t *perm_cols, int algo, bool store,
+		   uint32_t *pivcols)
 {
 	if (m == 0 || n == 0) {
 		if (rank_out)
@@ -986,6 +987,9 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 	if (perm_rows)
 		for (size_t s = 0; s < rank; ++s)
 			perm_rows[s] = (uint32_t) steps[s];
+	if (pivcols)
+		for (size_t s = 0; s < rank; ++s)
+			pivcols[s] = (uint32_t) pcols[s];
 	// pluq's column order: the pivot columns swapped into place step by step
 	std::vector<int> co(n), where(n);
 	for (size_t j = 0; j < n; ++j)
