@@ -16,21 +16,22 @@
  * What runs where:
  *   BLAS3::gemm, BLAS3::trsm on three dense Z/p matrices   -> GPU
  *   BLAS3::gemm on three little-endian dense GF(2) matrices -> GPU
- *   EchelonForm::echelonize (A, true) of a dense matrix      -> GPU (mlz_zp_rref / mlz_gf2_rref)
+ *   EchelonForm::echelonize (A, reduced) of a dense matrix   -> GPU (mlz_zp_rref / mlz_gf2_rref for
+ *       the reduced form, mlz_zp_echelonize for the non-reduced one over Z/p)
  *   EchelonForm::echelonize (A, *, METHOD_FAUGERE_LACHARTRE) -> GPU FL (dense A, as LELA's dense
  *       overload, always the reduced form: lela/solutions/echelon-form.h:133-139)
  *   FaugereLachartre::echelonize (R, X, rank, det, true)     -> GPU FL (any row type)
- *   the non-reduced forms and every other storage           -> LELA's own CPU code
+ *   Elimination::echelonize (A, P, rank, det, compute_L) and
+ *   Elimination::pluq (A, P, Q, rank, det) of a dense Z/p matrix with the
+ *       default pivot strategy                               -> GPU (mlz_zp_echelonize)
+ *   the non-reduced forms over GF(2), explicit pivot
+ *   strategies and every other storage                      -> LELA's own CPU code
  *
  * Determinant of FaugereLachartre: LELA multiplies det by the heads of the
- * scanned rows of X and then by the heads of its (non-reduced) echelon form
- * of D - C A^-1 B (lela/algorithms/faugere-lachartre.tcc:119, :831-833).  The
- * GPU call returns the first factor; the second depends on LELA's pivot
- * order, so the GPU route returns the exact LELA value only when the rank of
- * X equals the number of its pivot rows (every row scanned is a pivot row,
- * D - C A^-1 B = 0); otherwise it runs LELA's CPU FaugereLachartre when the
- * caller asked for det (det_wanted (), the default), or returns the first
- * factor alone when the caller has said it does not use det.
+ * scanned rows of X and by the heads of its (non-reduced) echelon form of
+ * D - C A^-1 B (lela/algorithms/faugere-lachartre.tcc:119, :831-833); the GPU
+ * call returns that product.  A caller that does not use det can say so
+ * (det_wanted (false)): the GPU then skips D's non-reduced form and det is 1.
  *
  * The header is compiled inside a LELA build (it includes LELA's headers,
  * C++98) and links against lela_amd/liblela_amd.so.  Return codes of the C
@@ -321,6 +322,103 @@ inline void hip_append_row (const Modular<Element> &F, const Row &row, std::vect
 		}
 }
 
+/** Elimination over Z/p on the HIP module (lela/algorithms/elimination.h:34-238).
+ * echelonize (with and without compute_L) and pluq of a dense matrix with the
+ * default pivot strategy (DensePivotStrategy, pivot-strategy.tcc:19-41) run
+ * on the GPU through mlz_zp_echelonize, which reproduces the transpositions;
+ * explicit pivot strategies, echelonize_reduced, move_L and other storage run
+ * LELA's Elimination on a ZpModule context. */
+template <class Element>
+class Elimination<Modular<Element>, HIPModule<Modular<Element> > >
+{
+public:
+	typedef std::pair<uint32, uint32> Transposition;
+	typedef std::vector<Transposition> Permutation;
+
+private:
+	Context<Modular<Element>, HIPModule<Modular<Element> > > &ctx;
+	Context<Modular<Element>, ZpModule<Element> > _cpu_ctx;
+	Elimination<Modular<Element>, ZpModule<Element> > _cpu;
+
+	size_t gpu (DenseMatrix<Element> &A, int mode, Permutation &P, Permutation *Q, Element &det) const
+	{
+		const size_t r = std::min (A.rowdim (), A.coldim ());
+		std::vector<uint32_t> pr (r + 1), pc (r + 1);
+		size_t rank = 0;
+		uint64_t d = 1;
+		P.clear ();
+		if (Q)
+			Q->clear ();
+		if (A.rowdim () && A.coldim ()) {
+			const int rc = mlz_zp_echelonize (ctx.M.dev.ctx (), HIPElement<Element>::value (),
+							  (uint64_t) ctx.F._modulus, A.rowdim (), A.coldim (), &A[0][0],
+							  A.disp (), mode, &rank, &d, &pr[0], &pc[0], MLZ_ALGO_AUTO);
+			if (rc == MLZ_E_NOT_INVERTIBLE)
+				throw LELAError ("Could not invert pivot-element in the ring");   // elimination.tcc:103-104
+			ctx.M.dev.check (rc);
+		}
+		for (size_t s = 0; s < rank; ++s) {
+			if (pr[s] != s)
+				P.push_back (Transposition ((uint32) s, (uint32) pr[s]));
+			if (Q && pc[s] != s)
+				Q->push_back (Transposition ((uint32) s, (uint32) pc[s]));
+		}
+		ctx.F.init (det, (long) d);
+		return rank;
+	}
+
+public:
+	Elimination (Context<Modular<Element>, HIPModule<Modular<Element> > > &_ctx)
+		: ctx (_ctx), _cpu_ctx (_ctx.F), _cpu (_cpu_ctx) {}
+
+	/** Dense: the GPU (elimination.tcc:47-139). */
+	DenseMatrix<Element> &echelonize (DenseMatrix<Element> &A, Permutation &P, size_t &rank, Element &det,
+					  bool compute_L = true) const
+	{
+		rank = gpu (A, compute_L ? MLZ_ECH_TRANSFORM : MLZ_ECH_U, P, 0, det);
+		return A;
+	}
+
+	template <class Matrix>
+	Matrix &echelonize (Matrix &A, Permutation &P, size_t &rank, Element &det, bool compute_L = true) const
+		{ return _cpu.echelonize (A, P, rank, det, compute_L); }
+
+	template <class Matrix, class PivotStrategy>
+	Matrix &echelonize (Matrix &A, Permutation &P, size_t &rank, Element &det, PivotStrategy PS, bool compute_L) const
+		{ return _cpu.echelonize (A, P, rank, det, PS, compute_L); }
+
+	template <class Matrix1, class Matrix2>
+	Matrix1 &echelonize_reduced (Matrix1 &A, Matrix2 &L, Permutation &P, size_t &rank, Element &det,
+				     bool compute_L = false) const
+		{ return _cpu.echelonize_reduced (A, L, P, rank, det, compute_L); }
+
+	template <class Matrix1, class Matrix2, class PivotStrategy>
+	Matrix1 &echelonize_reduced (Matrix1 &A, Matrix2 &L, Permutation &P, size_t &rank, Element &det,
+				     PivotStrategy PS, bool compute_L) const
+		{ return _cpu.echelonize_reduced (A, L, P, rank, det, PS, compute_L); }
+
+	/** Dense: the GPU (elimination.tcc:266-370; P and Q in LELA's reversed order, :357-358). */
+	DenseMatrix<Element> &pluq (DenseMatrix<Element> &A, Permutation &P, Permutation &Q, size_t &rank,
+				    Element &det) const
+	{
+		rank = gpu (A, MLZ_ECH_PLUQ, P, &Q, det);
+		std::reverse (P.begin (), P.end ());
+		std::reverse (Q.begin (), Q.end ());
+		return A;
+	}
+
+	template <class Matrix>
+	Matrix &pluq (Matrix &A, Permutation &P, Permutation &Q, size_t &rank, Element &det) const
+		{ return _cpu.pluq (A, P, Q, rank, det); }
+
+	template <class Matrix, class PivotStrategy>
+	Matrix &pluq (Matrix &A, Permutation &P, Permutation &Q, size_t &rank, Element &det, PivotStrategy PS) const
+		{ return _cpu.pluq (A, P, Q, rank, det, PS); }
+
+	template <class Matrix1, class Matrix2>
+	void move_L (Matrix1 &L, Matrix2 &A) const { _cpu.move_L (L, A); }
+};
+
 /** FaugereLachartre over Z/p on the HIP module
  * (lela/algorithms/faugere-lachartre.h:40-81): the F4 matrix X goes to the
  * device as CSR (any row type), R comes back dense and is copied into R.
@@ -343,8 +441,8 @@ public:
 	FaugereLachartre (Context<Modular<Element>, HIPModule<Modular<Element> > > &_ctx)
 		: ctx (_ctx), _det_wanted (true) {}
 
-	/** Whether the caller uses det.  If not, a matrix whose D part has
-	 * positive rank stays on the GPU and det holds only the X-scan factor. */
+	/** Whether the caller uses det.  If not, the GPU skips the non-reduced
+	 * echelon form of D that det needs, and det is set to 1. */
 	void det_wanted (bool wanted) { _det_wanted = wanted; }
 
 	template <class Matrix>
@@ -372,7 +470,7 @@ public:
 						     (uint64_t) ctx.F._modulus, X.rowdim (), X.coldim (), &row_ptr[0],
 						     col.empty () ? 0 : &col[0], val.empty () ? 0 : &val[0],
 						     X.rowdim () && X.coldim () ? &Rd[0][0] : 0, Rd.disp (), &r,
-						     &pivots[0], &det_c, &bad_row, MLZ_ALGO_AUTO);
+						     &pivots[0], _det_wanted ? &det_c : 0, &bad_row, MLZ_ALGO_AUTO);
 		if (rc == MLZ_E_WRONG_FORM)
 			throw WrongMatrixForm (bad_row);
 		if (rc == MLZ_E_NOT_INVERTIBLE) {        // LELA leaves R as it was (:790-799)
@@ -381,18 +479,6 @@ public:
 			return;
 		}
 		ctx.M.dev.check (rc);
-
-		// The X scan's pivot rows are the first pivots; more pivots mean
-		// D - C A^-1 B had positive rank, and LELA's det then also has
-		// the heads of its echelon form of D.
-		size_t scanned = 0;
-		for (size_t i = 0; i + 1 < row_ptr.size () && row_ptr[i + 1] > row_ptr[i]; ++i)
-			if (i == 0 || col[row_ptr[i]] > col[row_ptr[i - 1]])
-				++scanned;
-		if (r > scanned && _det_wanted) {
-			cpu (R, X, rank, det, reduced, only_D);
-			return;
-		}
 		rank = r;
 		ctx.F.init (det, (long) det_c);
 		BLAS3::copy (ctx, Rd, R);
@@ -400,10 +486,10 @@ public:
 };
 
 /** EchelonForm over Z/p on the HIP module (lela/solutions/echelon-form.h:31-164).
- * Dense reduced forms and the Faugere-Lachartre method run on the GPU; the
- * non-reduced forms, whose layout depends on LELA's pivot order, run LELA's
- * own Elimination on a ZpModule context (echelon-form.h:113-117,123-131 give
- * the same result for both methods: tests/test-gauss-jordan.C:96-99). */
+ * Dense matrices and the Faugere-Lachartre method run on the GPU; other
+ * storage runs LELA's own Elimination on a ZpModule context.  The dense
+ * non-reduced form is Elimination's with transposition pivoting, which both
+ * methods give (echelon-form.h:113-131, tests/test-gauss-jordan.C:92-99). */
 template <class Element>
 class EchelonForm<Modular<Element>, HIPModule<Modular<Element> > >
 {
@@ -468,7 +554,14 @@ public:
 		case METHOD_STANDARD_GJ:
 		case METHOD_ASYMPTOTICALLY_FAST_GJ:
 			if (!reduced) {
-				rank = cpu_echelonize (A, false);
+				// Elimination::echelonize (:78); GaussJordan + move_L gives the
+				// same form (:128-131, tests/test-gauss-jordan.C:92-99)
+				uint64_t d = 1;
+				if (A.rowdim () && A.coldim ())
+					_ctx.M.dev.check (mlz_zp_echelonize (_ctx.M.dev.ctx (), HIPElement<Element>::value (),
+									     (uint64_t) _ctx.F._modulus, A.rowdim (),
+									     A.coldim (), &A[0][0], A.disp (), MLZ_ECH_U,
+									     &rank, &d, 0, 0, MLZ_ALGO_AUTO));
 				break;
 			}
 			if (A.rowdim () && A.coldim ()) {

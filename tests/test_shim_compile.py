@@ -3,8 +3,10 @@ LELA translation unit.
 
 A translation unit that instantiates every hook -- gemm and trsm for
 Modular<uint16/uint32/double>, GF(2) gemm, EchelonForm (reduced, non-reduced,
-Faugere-Lachartre method, rank) for the three Z/p storages and GF(2), and
-FaugereLachartre on sparse and dense rows -- goes through
+Faugere-Lachartre method, rank) for the three Z/p storages and GF(2),
+FaugereLachartre on sparse and dense rows, and Elimination (echelonize with
+and without L, pluq, the reduced form, move_L) on dense and sparse rows --
+goes through
 `g++ -std=gnu++98 -fsyntax-only` against LELA's own headers.  This is a
 syntax-only check: no object code is produced, nothing is linked or run, and
 no result of LELA is taken from it.  LELA's headers include the
@@ -68,6 +70,17 @@ void zp_hooks (const Modular<E> &F)
 	FL.echelonize (Rd, Xd, rank, det, true);
 	FL.det_wanted (false);
 	FL.echelonize (Rd, Xd, rank, det);
+	typedef Elimination<Modular<E>, HIPModule<Modular<E> > > Elim_t;
+	Elim_t elim (ctx);
+	typename Elim_t::Permutation P, Q;
+	elim.echelonize (A, P, rank, det, true);
+	elim.echelonize (A, P, rank, det, false);
+	elim.pluq (A, P, Q, rank, det);
+	elim.echelonize (X, P, rank, det, false);
+	elim.pluq (X, P, Q, rank, det);
+	DenseMatrix<E> L (8, 8);
+	elim.echelonize_reduced (A, L, P, rank, det, true);
+	elim.move_L (L, A);
 	(void) r;
 }
 
