@@ -15,7 +15,7 @@
  *
  * What runs where:
  *   BLAS3::gemm, BLAS3::trsm on three dense Z/p matrices   -> GPU
- *   BLAS3::gemm on three little-endian dense GF(2) matrices -> GPU
+ *   BLAS3::gemm, BLAS3::trsm on little-endian dense GF(2) matrices -> GPU
  *   EchelonForm::echelonize (A, reduced) of a dense matrix   -> GPU (mlz_zp_rref / mlz_gf2_rref for
  *       the reduced form, mlz_zp_echelonize for the non-reduced one over Z/p)
  *   EchelonForm::echelonize (A, *, METHOD_FAUGERE_LACHARTRE) -> GPU FL (dense A, as LELA's dense
@@ -293,6 +293,47 @@ public:
 	static DenseMatrix<bool> &op (const GF2 &F, Modules &M, bool a, const DenseMatrix<bool> &A,
 				      const DenseMatrix<bool> &B, bool b, DenseMatrix<bool> &C)
 		{ return gpu (F, M, a, A, B, b, C); }
+#endif
+};
+
+template <>
+class _trsm<GF2, HIPModule<GF2>::Tag>
+{
+	typedef HIPModule<GF2>::Tag Tag;
+
+	template <class Modules, class Matrix1, class Matrix2>
+	static Matrix2 &gpu (const GF2 &, Modules &M, bool a, const Matrix1 &A, Matrix2 &B, TriangularMatrixType type,
+			     bool diagIsOne)
+	{
+		lela_check (A.coldim () == B.rowdim ());
+		lela_check (A.rowdim () == B.rowdim ());
+		if (A.rowdim () == 0 || B.coldim () == 0)
+			return B;
+		size_t bad = 0;
+		const int rc = mlz_gf2_trsm (M.dev.ctx (), type == UpperTriangular, diagIsOne, A.rowdim (), B.coldim (), a,
+					     hip_words (A), hip_row_words (A), hip_words (B), hip_row_words (B), &bad);
+		M.dev.check (rc, bad);
+		return B;
+	}
+
+public:
+	/* Anything but two little-endian dense 0-1 matrices: the parent module. */
+	template <class Modules, class Matrix1, class Matrix2>
+	static Matrix2 &op (const GF2 &F, Modules &M, bool a, const Matrix1 &A, Matrix2 &B, TriangularMatrixType type,
+			    bool diagIsOne)
+		{ return _trsm<GF2, Tag::Parent>::op (F, M, a, A, B, type, diagIsOne); }
+
+	template <class Modules, class It, class CIt>
+	static Dense01Matrix<It, CIt, LittleEndian<uint64> > &
+	op (const GF2 &F, Modules &M, bool a, const Dense01Matrix<It, CIt, LittleEndian<uint64> > &A,
+	    Dense01Matrix<It, CIt, LittleEndian<uint64> > &B, TriangularMatrixType type, bool diagIsOne)
+		{ return gpu (F, M, a, A, B, type, diagIsOne); }
+
+#ifndef __LELA_HAVE_M4RI
+	template <class Modules>
+	static DenseMatrix<bool> &op (const GF2 &F, Modules &M, bool a, const DenseMatrix<bool> &A, DenseMatrix<bool> &B,
+				      TriangularMatrixType type, bool diagIsOne)
+		{ return gpu (F, M, a, A, B, type, diagIsOne); }
 #endif
 };
 

@@ -230,6 +230,18 @@ int mlz_zp_fl_echelonize(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size
 int mlz_gf2_gemm(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda_w,
 		 const uint64_t *B, size_t ldb_w, int b, uint64_t *C, size_t ldc_w);
 
+/* B <- a*A^-1*B over GF(2) (a in {0, 1}); A n x n triangular (upper != 0:
+ * upper), B n x nrhs bits, leading dimensions in words; bits of B beyond
+ * column nrhs keep their values.  unit_diag != 0 takes A's diagonal as ones
+ * (what M4RI's mzd_trsm_upper_left / _lower_left do); otherwise a zero on the
+ * diagonal returns MLZ_E_NOT_INVERTIBLE with *bad_diag = the index LELA's
+ * recursion reports (smallest for lower, largest for upper) and B unchanged.
+ * Replaces BLAS3::trsm for GF2: _trsm<GF2, GenericModule::Tag>
+ * (lela/blas/level3-generic.tcc:421-498) and _trsm<GF2, M4RIModule::Tag>
+ * (lela/blas/level3-m4ri.tcc:88-109). */
+int mlz_gf2_trsm(mlz_ctx *ctx, int upper, int unit_diag, size_t n, size_t nrhs, int a, const uint64_t *A,
+		 size_t lda_w, uint64_t *B, size_t ldb_w, size_t *bad_diag);
+
 /* Reduced row-echelon form over GF(2) in place (standard layout as
  * mlz_zp_rref; bits beyond column n are written as zero).
  * Replaces EchelonForm<GF2>::echelonize(A, reduced=true)

@@ -263,6 +263,8 @@ class BLAS3:
     def trsm(ctx, a, A, B, type_, diagIsOne, algo=ALGO_AUTO):
         """B <- a*A^-1*B, A triangular (lela/blas/level3.h:120-122)."""
         F = ctx.F
+        if isinstance(F, GF2):
+            return _gf2_trsm(ctx, a, A, B, type_, diagIsOne)
         pa, lda, n, n2 = _operand(ctx, A, F.dtype)
         pb, ldb, nb, nrhs = _operand(ctx, B, F.dtype)
         if n != n2 or nb != n:
@@ -392,6 +394,18 @@ def _gf2_gemm(ctx, a, A, B, b, C):
     ctx.check(ctx.lib.mlz_gf2_gemm(ctx.handle, m, n, k, int(a) & 1, pa, lda, pb, ldb, int(b) & 1,
                                    pc, ldc), "gf2 gemm")
     return C
+
+
+def _gf2_trsm(ctx, a, A, B, type_, diagIsOne):
+    pa, lda, n, n2 = _gf2_operand(ctx, A)
+    pb, ldb, nb, nrhs = _gf2_operand(ctx, B)
+    if n != n2 or nb != n:
+        raise LELAError("trsm: dimension mismatch")
+    bad = ctypes.c_size_t(0)
+    rc = ctx.lib.mlz_gf2_trsm(ctx.handle, int(type_ == UpperTriangular), int(bool(diagIsOne)), n, nrhs,
+                              int(a) & 1, pa, lda, pb, ldb, ctypes.byref(bad))
+    ctx.check(rc, "gf2 trsm", idx=bad.value if rc == _lib.MLZ_E_NOT_INVERTIBLE else None)
+    return B
 
 
 def _gf2_echelonize(EF, A):

@@ -429,6 +429,8 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 int zp_inverse_table(mlz_ctx *ctx, const ModP &mp, const uint16_t **out);
 int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda,
 		 const uint64_t *B, size_t ldb, int b, uint64_t *C, size_t ldc);
+int gf2_trsm_dev(mlz_ctx *ctx, int upper, int unit, size_t n, size_t nrhs, int a, const uint64_t *A, size_t lda,
+		 uint64_t *B, size_t ldb, size_t *bad_diag);
 int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size_t *rank_out,
 		 uint32_t *pivots_out);
 int pick_gemm_algo(uint64_t p, int elem, size_t m, size_t n, size_t k, int algo);

@@ -844,6 +844,150 @@ int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size
 	return MLZ_OK;
 }
 
+// ---------------------------------------------------------------------------
+// trsm over GF(2): B <- a A^-1 B, A triangular (lela/blas/level3-generic.tcc:
+// 421-498; the M4RI module's mzd_trsm_*_left, level3-m4ri.tcc:88-109, takes
+// the diagonal as ones).  The 64 x 64 diagonal blocks are inverted up front;
+// the recursion splits at multiples of 64 and updates with the GF(2) gemm;
+// a leaf applies its block's inverse.
+// ---------------------------------------------------------------------------
+
+// Inverse of diagonal block b (rows and columns 64b .. 64b+63; a partial last
+// block is completed by the identity), one wave per block: lane i holds row i
+// of the block (its word b) and of the inverse.  Lower: x_i = e_i + sum over
+// j < i with A_ij = 1 of x_j, so column j is added to the rows below it once
+// x_j is final (j ascending); upper likewise with j descending.  Without
+// unit_diag a zero diagonal bit is recorded (bad[0] = the smallest index,
+// bad[1] = the largest).
+__global__ __launch_bounds__(64) void gf2_trsm_inv_kernel(const uint64_t *__restrict__ A, size_t lda, int n, int upper,
+							   int unit, uint64_t *__restrict__ W,
+							   unsigned long long *__restrict__ bad)
+{
+	const int b = blockIdx.x, i = threadIdx.x;
+	const int r0 = 64 * b, rows = min(64, n - r0);
+	uint64_t a = 0;
+	if (i < rows) {
+		a = A[(size_t) (r0 + i) * lda + b] & tail_mask64(rows);   // columns of the block inside A
+		if (!unit && !((a >> i) & 1ull)) {
+			atomicMin(&bad[0], (unsigned long long) (r0 + i));
+			atomicMax(&bad[1], (unsigned long long) (r0 + i));
+		}
+	}
+	// the strict triangle only
+	a &= upper ? (i == 63 ? 0ull : ~0ull << (i + 1)) : ((1ull << i) - 1ull);
+	uint64_t x = 1ull << i;
+	for (int s = 0; s < 64; ++s) {
+		const int j = upper ? 63 - s : s;
+		const uint64_t xj = __shfl(x, j, 64);
+		if ((a >> j) & 1ull)
+			x ^= xj;
+	}
+	W[(size_t) b * 64 + i] = x;
+}
+
+// The block's rows of B (Bb: its first row, `rows` rows) <- X_b times them
+// (X_b = W's block b): output (i, w) is the XOR of the rows j whose bit is set
+// in X_b[i].  Workgroup: G2T_CW words of the rows, staged in LDS.  Bits past
+// nrhs in the last word keep their values.
+constexpr int G2T_CW = 32;
+__global__ __launch_bounds__(256) void gf2_trsm_leaf_kernel(const uint64_t *__restrict__ W, int b, int rows,
+							    uint64_t *__restrict__ Bb, size_t ldb, int nrhs)
+{
+	__shared__ uint64_t Bt[64][G2T_CW + 1];
+	__shared__ uint64_t Xs[64];
+	const int tid = threadIdx.x;
+	const int nw = (nrhs + 63) / 64, w0 = blockIdx.x * G2T_CW;
+	if (tid < 64)
+		Xs[tid] = W[(size_t) b * 64 + tid];
+	for (int e = tid; e < 64 * G2T_CW; e += 256) {
+		const int r = e / G2T_CW, w = e % G2T_CW;
+		Bt[r][w] = (r < rows && w0 + w < nw) ? Bb[(size_t) r * ldb + w0 + w] : 0ull;
+	}
+	__syncthreads();
+	for (int e = tid; e < 64 * G2T_CW; e += 256) {
+		const int r = e / G2T_CW, w = e % G2T_CW, gw = w0 + w;
+		if (r >= rows || gw >= nw)
+			continue;
+		uint64_t acc = 0, sel = Xs[r];
+		while (sel) {
+			const int j = __builtin_ctzll(sel);
+			acc ^= Bt[j][w];
+			sel &= sel - 1;
+		}
+		const uint64_t mask = gw == nw - 1 ? tail_mask64(nrhs - gw * 64) : ~0ull;
+		Bb[(size_t) r * ldb + gw] = (acc & mask) | (Bt[r][w] & ~mask);
+	}
+}
+
+static int gf2_trsm_rec(mlz_ctx *ctx, int upper, size_t n, size_t nrhs, const uint64_t *A, size_t lda, uint64_t *B,
+			size_t ldb, const uint64_t *W, size_t b0)
+{
+	if (n <= 64) {
+		gf2_trsm_leaf_kernel<<<ceil_div(ceil_div(nrhs, 64), G2T_CW), 256, 0, ctx->stream>>>(W, (int) b0, (int) n, B,
+												   ldb, (int) nrhs);
+		return check_launch(ctx, "gf2_trsm_leaf_kernel");
+	}
+	// split at a multiple of 64, as LELA's aligned case (level3-generic.tcc:474)
+	const size_t l = std::max((size_t) 64, (n / 128) * 64), lw = l / 64;
+	uint64_t *B2 = B + l * ldb;
+	if (!upper) {
+		MLZ_TRY(gf2_trsm_rec(ctx, upper, l, nrhs, A, lda, B, ldb, W, b0));
+		MLZ_TRY(gf2_gemm_dev(ctx, n - l, nrhs, l, 1, A + l * lda, lda, B, ldb, 1, B2, ldb));   // B2 += A21 B1
+		return gf2_trsm_rec(ctx, upper, n - l, nrhs, A + l * lda + lw, lda, B2, ldb, W, b0 + lw);
+	}
+	MLZ_TRY(gf2_trsm_rec(ctx, upper, n - l, nrhs, A + l * lda + lw, lda, B2, ldb, W, b0 + lw));
+	MLZ_TRY(gf2_gemm_dev(ctx, l, nrhs, n - l, 1, A + lw, lda, B2, ldb, 1, B, ldb));         // B1 += A12 B2
+	return gf2_trsm_rec(ctx, upper, l, nrhs, A, lda, B, ldb, W, b0);
+}
+
+// B (rows x nbits) <- 0, bits past nbits in the last word kept.
+__global__ void gf2_clear_kernel(uint64_t *__restrict__ B, size_t ldb, int nbits)
+{
+	const int nw = (nbits + 63) / 64, w = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y;
+	if (w >= nw)
+		return;
+	const uint64_t mask = w == nw - 1 ? tail_mask64(nbits - w * 64) : ~0ull;
+	B[(size_t) r * ldb + w] &= ~mask;
+}
+
+__global__ void gf2_trsm_bad_init_kernel(unsigned long long *bad)
+{
+	bad[0] = ~0ull;
+	bad[1] = 0ull;
+}
+
+int gf2_trsm_dev(mlz_ctx *ctx, int upper, int unit, size_t n, size_t nrhs, int a, const uint64_t *A, size_t lda,
+		 uint64_t *B, size_t ldb, size_t *bad_diag)
+{
+	if (n == 0 || nrhs == 0)
+		return MLZ_OK;
+	if (!(a & 1)) {              // a = 0: B <- 0 (level3-generic.tcc:434-435)
+		gf2_clear_kernel<<<dim3(ceil_div(ceil_div(nrhs, 64), 256), (unsigned) n), 256, 0, ctx->stream>>>(B, ldb,
+													     (int) nrhs);
+		return check_launch(ctx, "gf2_clear_kernel");
+	}
+	const size_t nb = ceil_div(n, 64);
+	void *w;
+	MLZ_TRY(ws_get(ctx, WS_TRSM_0, nb * 64 * 8 + 256, &w));
+	uint64_t *W = (uint64_t *) w;
+	unsigned long long *bad = (unsigned long long *) (W + nb * 64);
+	gf2_trsm_bad_init_kernel<<<1, 1, 0, ctx->stream>>>(bad);
+	gf2_trsm_inv_kernel<<<(unsigned) nb, 64, 0, ctx->stream>>>(A, lda, (int) n, upper, unit, W, bad);
+	MLZ_TRY(check_launch(ctx, "gf2_trsm_inv_kernel"));
+	if (!unit) {
+		unsigned long long hb[2];
+		MLZ_HIP(ctx, hipMemcpyAsync(hb, bad, sizeof hb, hipMemcpyDeviceToHost, ctx->stream));
+		MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+		if (hb[0] != ~0ull) {
+			// the recursion meets lower blocks top-down and upper ones bottom-up
+			if (bad_diag)
+				*bad_diag = (size_t) (upper ? hb[1] : hb[0]);
+			return set_error(ctx, MLZ_E_NOT_INVERTIBLE, "Diagonal entry not invertible");
+		}
+	}
+	return gf2_trsm_rec(ctx, upper, n, nrhs, A, lda, B, ldb, W, 0);
+}
+
 }  // namespace mlz
 
 using namespace mlz;
@@ -896,4 +1040,29 @@ extern "C" int mlz_gf2_rref(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_
 	MLZ_TRY(stage_in(ctx, WS_STAGE_A, A, m, nw, lda_w, 8, &dA));
 	MLZ_TRY(gf2_rref_dev(ctx, m, n, (uint64_t *) dA, nw, rank, pivots));
 	return stage_out(ctx, A, dA, m, nw, lda_w, 8);
+}
+
+extern "C" int mlz_gf2_trsm(mlz_ctx *ctx, int upper, int unit_diag, size_t n, size_t nrhs, int a, const uint64_t *A,
+			    size_t lda_w, uint64_t *B, size_t ldb_w, size_t *bad_diag)
+{
+	if (!ctx)
+		return MLZ_E_ARG;
+	const size_t nw = (n + 63) / 64, rw = (nrhs + 63) / 64;
+	if ((n && (!A || lda_w < nw)) || (n && nrhs && (!B || ldb_w < rw)))
+		return set_error(ctx, MLZ_E_ARG, "bad GF(2) matrix pointer or leading dimension");
+	if (n > (size_t) INT32_MAX / 2 || nrhs > (size_t) INT32_MAX / 2)
+		return set_error(ctx, MLZ_E_ARG, "dimension too large");
+	MLZ_HIP(ctx, hipSetDevice(ctx->device));
+	if (ctx->ptr_mode == MLZ_PTR_DEVICE)
+		return gf2_trsm_dev(ctx, upper, unit_diag, n, nrhs, a, A, lda_w, B, ldb_w, bad_diag);
+	if (n == 0 || nrhs == 0)
+		return MLZ_OK;
+	void *dA = nullptr, *dB = nullptr;
+	MLZ_TRY(stage_in(ctx, WS_STAGE_A, A, n, nw, lda_w, 8, &dA));
+	MLZ_TRY(stage_in(ctx, WS_STAGE_B, B, n, rw, ldb_w, 8, &dB));
+	const int rc = gf2_trsm_dev(ctx, upper, unit_diag, n, nrhs, a, (const uint64_t *) dA, nw, (uint64_t *) dB, rw,
+				    bad_diag);
+	if (rc != MLZ_OK)
+		return rc;
+	return stage_out(ctx, B, dB, n, rw, ldb_w, 8);
 }

@@ -463,6 +463,92 @@ int orc_gf2_gemm(size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t 
 	return ORC_OK;
 }
 
+/* BLAS3::trsm over GF(2) on unpacked bits: the generic recursion
+ * (lela/blas/level3-generic.tcc:421-498) with Dense01Matrix's alignment of 64
+ * columns: l = n/2 below 128 columns (:449-450), else a multiple of 64
+ * (:472-474).  Over GF(2) -1 = 1, so gemm(-1, A21, B1, a, B2) is B2 ^= A21 B1
+ * once a = 1. */
+static int gf2_trsm_rec(int upper, int unit, size_t n, size_t nrhs, int a, const unsigned char *A, size_t lda,
+			unsigned char *B, size_t ldb, size_t *bad)
+{
+	if (n == 0)
+		return ORC_OK;
+	if (!a) {                                   /* :434-435 */
+		for (size_t i = 0; i < n; ++i)
+			memset(B + i * ldb, 0, nrhs);
+		return ORC_OK;
+	}
+	if (n == 1) {                               /* :436-448: the only invertible value is 1 */
+		if (!unit && !A[0]) {
+			*bad = 0;
+			return ORC_E_NOT_INVERTIBLE;
+		}
+		return ORC_OK;
+	}
+	size_t l = n < 128 ? n / 2 : (n / 128) * 64;
+	unsigned char *B1 = B, *B2 = B + l * ldb;
+	int rc;
+	if (!upper) {
+		rc = gf2_trsm_rec(upper, unit, l, nrhs, a, A, lda, B1, ldb, bad);
+		if (rc)
+			return rc;
+		for (size_t i = l; i < n; ++i)
+			for (size_t j = 0; j < l; ++j)
+				if (A[i * lda + j])
+					for (size_t c = 0; c < nrhs; ++c)
+						B[i * ldb + c] ^= B1[j * ldb + c];
+		rc = gf2_trsm_rec(upper, unit, n - l, nrhs, 1, A + l * lda + l, lda, B2, ldb, bad);
+		if (rc) {
+			*bad += l;
+			return rc;
+		}
+	} else {
+		rc = gf2_trsm_rec(upper, unit, n - l, nrhs, a, A + l * lda + l, lda, B2, ldb, bad);
+		if (rc) {
+			*bad += l;
+			return rc;
+		}
+		for (size_t i = 0; i < l; ++i)
+			for (size_t j = l; j < n; ++j)
+				if (A[i * lda + j])
+					for (size_t c = 0; c < nrhs; ++c)
+						B[i * ldb + c] ^= B[j * ldb + c];
+		rc = gf2_trsm_rec(upper, unit, l, nrhs, 1, A, lda, B1, ldb, bad);
+		if (rc)
+			return rc;
+	}
+	return ORC_OK;
+}
+
+int orc_gf2_trsm(int upper, int unit_diag, size_t n, size_t nrhs, int a, const uint64_t *A, size_t lda_w,
+		 uint64_t *B, size_t ldb_w, size_t *bad_diag)
+{
+	unsigned char *Au = (unsigned char *) calloc(n * n + 1, 1), *Bu = (unsigned char *) calloc(n * nrhs + 1, 1);
+	if (!Au || !Bu) {
+		free(Au);
+		free(Bu);
+		return ORC_E_ARG;
+	}
+	for (size_t i = 0; i < n; ++i) {
+		for (size_t j = 0; j < n; ++j)
+			Au[i * n + j] = (unsigned char) gf2_get(A + i * lda_w, j);
+		for (size_t c = 0; c < nrhs; ++c)
+			Bu[i * nrhs + c] = (unsigned char) gf2_get(B + i * ldb_w, c);
+	}
+	size_t bad = 0;
+	int rc = gf2_trsm_rec(upper, unit_diag, n, nrhs, a & 1, Au, n, Bu, nrhs, &bad);
+	if (bad_diag)
+		*bad_diag = bad;
+	if (rc == ORC_OK)
+		for (size_t i = 0; i < n; ++i)
+			for (size_t c = 0; c < nrhs; ++c)
+				if ((unsigned char) gf2_get(B + i * ldb_w, c) != Bu[i * nrhs + c])
+					gf2_flip(B + i * ldb_w, c);
+	free(Au);
+	free(Bu);
+	return rc;
+}
+
 int orc_gf2_echelonize(size_t m, size_t n, uint64_t *A, size_t lda_w, int reduced,
 		       size_t *rank, uint32_t *pivots)
 {

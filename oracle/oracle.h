@@ -69,6 +69,13 @@ int orc_gf2_gemm(size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t 
  * (elimination.tcc:47-139); perm_rows[i] = pivot row of step i. */
 int orc_zp_elim(uint32_t p, size_t m, size_t n, uint32_t *A, size_t lda, int compute_L, size_t *rank,
 		uint32_t *det, uint32_t *perm_rows);
+/* BLAS3::trsm over GF(2), B <- a A^-1 B (lela/blas/level3-generic.tcc:421-498
+ * with Dense01Matrix's 64-column alignment).  unit_diag takes the diagonal as
+ * ones; otherwise a zero diagonal returns ORC_E_NOT_INVERTIBLE with *bad_diag
+ * the index the recursion reports, and B unchanged. */
+int orc_gf2_trsm(int upper, int unit_diag, size_t n, size_t nrhs, int a, const uint64_t *A, size_t lda_w,
+		 uint64_t *B, size_t ldb_w, size_t *bad_diag);
+
 /* Elimination::pluq (elimination.tcc:266-370); per-step row and column swaps. */
 int orc_zp_pluq(uint32_t p, size_t m, size_t n, uint32_t *A, size_t lda, size_t *rank, uint32_t *det,
 		uint32_t *perm_rows, uint32_t *perm_cols);

@@ -36,6 +36,7 @@ def load():
     lib.orc_zp_pluq.argtypes = [u32, _sz, _sz, _vp, _sz, ctypes.POINTER(_sz), ctypes.POINTER(u32), _vp, _vp]
     lib.orc_gf2_gemm.argtypes = [_sz, _sz, _sz, i, _vp, _sz, _vp, _sz, i, _vp, _sz]
     lib.orc_gf2_echelonize.argtypes = [_sz, _sz, _vp, _sz, i, ctypes.POINTER(_sz), _vp]
+    lib.orc_gf2_trsm.argtypes = [i, i, _sz, _sz, i, _vp, _sz, _vp, _sz, ctypes.POINTER(_sz)]
     lib.orc_fl_echelonize.argtypes = [u32, _sz, _sz, _vp, _sz, _vp, _sz, ctypes.POINTER(_sz),
                                       ctypes.POINTER(u32), ctypes.POINTER(_sz)]
     lib.orc_inv.argtypes = [u32, u32, ctypes.POINTER(u32)]
@@ -44,7 +45,7 @@ def load():
     lib.orc_mlp_adam.argtypes = [_vp, _vp, _vp, _vp, _sz, i, ctypes.c_double, _vp]
     for name in ("orc_zp_gemm_u32", "orc_zp_gemm_u16", "orc_zp_trsm", "orc_zp_echelonize", "orc_zp_elim",
                  "orc_zp_pluq",
-                 "orc_gf2_gemm", "orc_gf2_echelonize", "orc_fl_echelonize", "orc_inv",
+                 "orc_gf2_gemm", "orc_gf2_echelonize", "orc_gf2_trsm", "orc_fl_echelonize", "orc_inv",
                  "orc_mlp_forward", "orc_mlp_loss_grad", "orc_mlp_adam"):
         getattr(lib, name).restype = ctypes.c_int
     _lib = lib
@@ -126,6 +127,16 @@ def gf2_gemm(m, n, k, a, A, B, b, C):
                           int(b), C.ctypes.data, C.shape[1])
     assert rc == ORC_OK
     return C
+
+
+def gf2_trsm(upper, unit, n, nrhs, a, A, B):
+    """Returns (rc, bad_index, X) for X = a*A^-1*B over GF(2) (word rows)."""
+    lib = load()
+    A, B = _c(A, np.uint64), _c(B, np.uint64).copy()
+    bad = _sz(0)
+    rc = lib.orc_gf2_trsm(int(upper), int(unit), n, nrhs, int(a), A.ctypes.data, A.shape[1], B.ctypes.data,
+                          B.shape[1], ctypes.byref(bad))
+    return rc, bad.value, B
 
 
 def gf2_echelonize(m, n, A, reduced=True):

@@ -168,3 +168,66 @@ def test_gf2_rref_rank12000_16384(gpu_ctx):
     R, rank, piv = gpu_rref(gpu_ctx, n, n, M)
     assert rank == r
     check_gf2_rref_all_rows(M, R, piv, rank, n, seed=13)
+
+
+def gpu_trsm(ctx, upper, unit, a, A, n, B, nrhs):
+    ctx.F = la.GF2()
+    Bm = la.Dense01Matrix(B.copy(), nrhs)
+    la.BLAS3.trsm(ctx, a, la.Dense01Matrix(A, n), Bm, la.UpperTriangular if upper else la.LowerTriangular, unit)
+    return Bm.words
+
+
+@pytest.mark.parametrize("upper", [False, True])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (2, 70), (63, 64), (64, 1), (65, 70), (127, 64), (128, 130), (129, 3),
+                                    (200, 70), (700, 300), (1100, 2100)])
+def test_gf2_trsm_matches_oracle(gpu_ctx, upper, n, nrhs):
+    """GF(2) trsm (level3-generic.tcc:421-498) bit for bit against the oracle,
+    padding bits of B kept, leaf and recursion sizes around 64 and 128."""
+    from test_oracle import gf2_triangular
+    _, A = gf2_triangular(400 + n, n, upper)
+    B = synth.gf2_matrix(401 + n, n, nrhs)
+    B[:, -1] |= np.uint64(0xF000000000000000) if nrhs % 64 and nrhs % 64 < 60 else np.uint64(0)
+    rc, _, want = orc.gf2_trsm(upper, False, n, nrhs, 1, A, B)
+    assert rc == orc.ORC_OK
+    got = gpu_trsm(gpu_ctx, upper, False, 1, A, n, B, nrhs)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("upper", [False, True])
+def test_gf2_trsm_diagonal_and_scalar(gpu_ctx, upper):
+    """A zero on the diagonal: DiagonalEntryNotInvertible at the index LELA's
+    recursion reports, B unchanged; unit_diag takes the diagonal as ones;
+    a = 0 clears B (bits past nrhs kept)."""
+    from test_oracle import gf2_triangular
+    n, nrhs = 300, 70
+    _, A = gf2_triangular(410, n, upper, diag_zeros=(5, 70, 130, 257))
+    B = synth.gf2_matrix(411, n, nrhs)
+    with pytest.raises(la.DiagonalEntryNotInvertible) as ei:
+        gpu_trsm(gpu_ctx, upper, False, 1, A, n, B, nrhs)
+    assert ei.value.idx == (257 if upper else 5)
+    rc, _, want = orc.gf2_trsm(upper, True, n, nrhs, 1, A, B)
+    assert rc == orc.ORC_OK
+    assert np.array_equal(gpu_trsm(gpu_ctx, upper, True, 1, A, n, B, nrhs), want)
+    rc, _, want = orc.gf2_trsm(upper, False, n, nrhs, 0, A, B)
+    assert np.array_equal(gpu_trsm(gpu_ctx, upper, False, 0, A, n, B, nrhs), want)
+
+
+@pytest.mark.parametrize("upper", [False, True])
+def test_gf2_trsm_8192(gpu_ctx, upper):
+    """8192 x 8192 triangular, 1024 right-hand sides: A X = B at random bit
+    vectors over every row (A (X v) == B v)."""
+    n, nrhs = 8192, 1024
+    rng = np.random.default_rng(12 + upper)
+    T = synth.gf2_matrix(420 + upper, n, n)
+    bits = bf.gf2_unpack(T, n)
+    bits = np.triu(bits) if upper else np.tril(bits)
+    bits[np.arange(n), np.arange(n)] = 1
+    A = bf.gf2_pack(bits)
+    B = synth.gf2_matrix(422, n, nrhs)
+    print("gf2 trsm 8192", upper, flush=True)
+    X = gpu_trsm(gpu_ctx, upper, False, 1, A, n, B, nrhs)
+    for _ in range(3):
+        v = rng.integers(0, 2 ** 63, size=nrhs // 64, dtype=np.uint64)
+        xv = gf2_matvec_packed(X, v)                  # X v, n bits
+        lhs = (bits.astype(np.int32) @ xv.astype(np.int32)) & 1
+        assert np.array_equal(lhs.astype(np.uint8), gf2_matvec_packed(B, v))

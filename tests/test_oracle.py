@@ -158,6 +158,57 @@ def test_gf2_gemm(m, n, k, a, b):
     assert np.array_equal(bf.gf2_unpack(got, n), want)
 
 
+def gf2_triangular(seed, n, upper, diag_zeros=()):
+    """n x n GF(2) triangular bits (word rows) with ones on the diagonal
+    except at diag_zeros."""
+    rng = np.random.default_rng(seed)
+    T = rng.integers(0, 2, size=(n, n)).astype(np.uint8)
+    T = np.triu(T) if upper else np.tril(T)
+    T[np.arange(n), np.arange(n)] = 1
+    for i in diag_zeros:
+        T[i, i] = 0
+    return T, bf.gf2_pack(T)
+
+
+@pytest.mark.parametrize("upper", [False, True])
+@pytest.mark.parametrize("n,nrhs", [(1, 1), (2, 70), (63, 64), (64, 1), (65, 70), (127, 64), (128, 130), (129, 3),
+                                    (200, 70)])
+def test_gf2_trsm_solves(upper, n, nrhs):
+    # A X = B over GF(2) checked with numpy's product of the unpacked bits
+    T, A = gf2_triangular(300 + n, n, upper)
+    B = synth.gf2_matrix(301 + n, n, nrhs)
+    rc, bad, X = orc.gf2_trsm(upper, False, n, nrhs, 1, A, B)
+    assert rc == orc.ORC_OK
+    assert np.array_equal(bf.gf2_matmul_bits(T, bf.gf2_unpack(X, nrhs)), bf.gf2_unpack(B, nrhs))
+    # bits past nrhs keep their values
+    tail = nrhs % 64
+    if tail:
+        m = np.uint64((1 << 64) - (1 << tail))
+        assert np.array_equal(X[:, -1] & m, B[:, -1] & m)
+
+
+@pytest.mark.parametrize("upper", [False, True])
+def test_gf2_trsm_diagonal(upper):
+    # A zero diagonal: the recursion meets lower blocks top-down (the smallest
+    # such index is reported) and upper ones bottom-up (the largest); with
+    # unit_diag the diagonal is taken as ones.  a = 0 clears B.
+    n, nrhs = 200, 70
+    T, A = gf2_triangular(310, n, upper, diag_zeros=(5, 70, 130))
+    B = synth.gf2_matrix(311, n, nrhs)
+    rc, bad, X = orc.gf2_trsm(upper, False, n, nrhs, 1, A, B)
+    assert rc == orc.ORC_E_NOT_INVERTIBLE and bad == (130 if upper else 5)
+    assert np.array_equal(X, B)
+    rc, bad, X = orc.gf2_trsm(upper, True, n, nrhs, 1, A, B)
+    T1 = T.copy()
+    T1[np.arange(n), np.arange(n)] = 1
+    assert rc == orc.ORC_OK
+    assert np.array_equal(bf.gf2_matmul_bits(T1, bf.gf2_unpack(X, nrhs)), bf.gf2_unpack(B, nrhs))
+    rc, bad, X = orc.gf2_trsm(upper, False, n, nrhs, 0, A, B)
+    assert rc == orc.ORC_OK and not bf.gf2_unpack(X, nrhs).any()
+    m = np.uint64((1 << 64) - (1 << (nrhs % 64)))
+    assert np.array_equal(X[:, -1] & m, B[:, -1] & m)
+
+
 @pytest.mark.parametrize("m,n", [(10, 70), (70, 10), (64, 64), (100, 130)])
 def test_gf2_echelonize(m, n):
     A = synth.gf2_matrix(80 + m, m, n)

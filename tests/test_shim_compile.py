@@ -2,7 +2,7 @@
 LELA translation unit.
 
 A translation unit that instantiates every hook -- gemm and trsm for
-Modular<uint16/uint32/double>, GF(2) gemm, EchelonForm (reduced, non-reduced,
+Modular<uint16/uint32/double>, GF(2) gemm and trsm, EchelonForm (reduced, non-reduced,
 Faugere-Lachartre method, rank) for the three Z/p storages and GF(2),
 FaugereLachartre on sparse and dense rows, and Elimination (echelonize with
 and without L, pluq, the reduced form, move_L) on dense and sparse rows --
@@ -90,6 +90,8 @@ void gf2_hooks ()
 	Context<GF2, HIPModule<GF2> > ctx (F);
 	DenseMatrix<bool> A (64, 64), B (64, 64), C (64, 64);
 	BLAS3::gemm (ctx, true, A, B, false, C);
+	BLAS3::trsm (ctx, true, A, B, LowerTriangular, true);
+	BLAS3::trsm (ctx, true, A, B, UpperTriangular, false);
 	typedef EchelonForm<GF2, HIPModule<GF2> > EF_t;
 	EF_t EF (ctx);
 	EF.echelonize (A, true);
