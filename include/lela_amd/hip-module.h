@@ -20,7 +20,8 @@
  *       the reduced form, mlz_zp_echelonize for the non-reduced one over Z/p)
  *   EchelonForm::echelonize (A, *, METHOD_FAUGERE_LACHARTRE) -> GPU FL (dense A, as LELA's dense
  *       overload, always the reduced form: lela/solutions/echelon-form.h:133-139)
- *   FaugereLachartre::echelonize (R, X, rank, det, true)     -> GPU FL (any row type)
+ *   FaugereLachartre::echelonize (R, X, rank, det, true)     -> GPU FL (Z/p: any row type; GF(2):
+ *       hybrid rows of little-endian words and dense 0-1 matrices)
  *   Elimination::echelonize (A, P, rank, det, compute_L) and
  *   Elimination::pluq (A, P, Q, rank, det) of a dense Z/p matrix with the
  *       default pivot strategy                               -> GPU (mlz_zp_echelonize)
@@ -634,6 +635,126 @@ public:
 	}
 };
 
+/** The nonzero words of a row of a 0-1 matrix, with their word indices, for
+ * the hybrid-row ABI: rows that are hybrid vectors with little-endian 64-bit
+ * words (lela/vector/hybrid.h:30-33) go as they are; false for any other row
+ * (LELA's CPU code then runs). */
+template <class Row>
+inline bool hip_hybrid_words (const Row &row, std::vector<uint32_t> &blk, std::vector<uint64_t> &words,
+			      LittleEndian<uint64>)
+{
+	for (typename Row::const_iterator j = row.begin (); j != row.end (); ++j) {
+		blk.push_back ((uint32_t) j->first);
+		words.push_back ((uint64_t) j->second);
+	}
+	return true;
+}
+
+template <class Row, class Endianness>
+inline bool hip_hybrid_words (const Row &, std::vector<uint32_t> &, std::vector<uint64_t> &, Endianness)
+	{ return false; }
+
+template <class Row>
+inline bool hip_hybrid_row (const Row &row, std::vector<uint32_t> &blk, std::vector<uint64_t> &words,
+			    VectorRepresentationTypes::Hybrid01)
+	{ return hip_hybrid_words (row, blk, words, typename Row::Endianness ()); }
+
+template <class Row>
+inline bool hip_hybrid_row (const Row &, std::vector<uint32_t> &, std::vector<uint64_t> &,
+			    VectorRepresentationTypes::Generic)
+	{ return false; }
+
+/** FaugereLachartre over GF(2) on the HIP module
+ * (lela/algorithms/faugere-lachartre.h:40-81): X with hybrid rows of
+ * little-endian words (the reference test's SparseMatrix<bool, HybridVector<..>>,
+ * tests/test-faugere-lachartre.C:278-285) or a dense 0-1 matrix goes to the
+ * GPU (mlz_gf2_fl_echelonize) and R comes back dense; other rows, the
+ * non-reduced form and only_D run LELA's CPU FaugereLachartre.  Every head is
+ * 1 over GF(2), so det is true. */
+template <>
+class FaugereLachartre<GF2, HIPModule<GF2> >
+{
+	Context<GF2, HIPModule<GF2> > &ctx;
+
+	template <class Matrix>
+	void cpu (Matrix &R, const Matrix &X, size_t &rank, bool &det, bool reduced, bool only_D)
+	{
+		Context<GF2, GenericModule<GF2> > cctx (ctx.F);
+		FaugereLachartre<GF2, GenericModule<GF2> > FL (cctx);
+		FL.echelonize (R, X, rank, det, reduced, only_D);
+	}
+
+	/* true when the GPU ran: rank, det and Rd (m x n) filled */
+	bool gpu (const std::vector<uint64_t> &row_ptr, const std::vector<uint32_t> &blk,
+		  const std::vector<uint64_t> &words, size_t m, size_t n, DenseMatrix<bool> &Rd, size_t &rank, bool &det)
+	{
+		size_t bad_row = 0, r = 0;
+		const int rc = mlz_gf2_fl_echelonize (ctx.M.dev.ctx (), m, n, &row_ptr[0], blk.empty () ? 0 : &blk[0],
+						      words.empty () ? 0 : &words[0], m && n ? hip_words (Rd) : 0,
+						      hip_row_words (Rd), &r, 0, &bad_row);
+		if (rc == MLZ_E_WRONG_FORM)
+			throw WrongMatrixForm (bad_row);
+		ctx.M.dev.check (rc);
+		rank = r;
+		det = true;
+		return true;
+	}
+
+public:
+	FaugereLachartre (Context<GF2, HIPModule<GF2> > &_ctx) : ctx (_ctx) {}
+
+	template <class Matrix>
+	void echelonize (Matrix &R, const Matrix &X, size_t &rank, bool &det, bool reduced = true, bool only_D = false)
+	{
+		std::vector<uint64_t> row_ptr (1, 0);
+		std::vector<uint32_t> blk;
+		std::vector<uint64_t> words;
+		bool ok = reduced && !only_D;
+		for (typename Matrix::ConstRowIterator i = X.rowBegin (); ok && i != X.rowEnd (); ++i) {
+			ok = hip_hybrid_row (*i, blk, words,
+					     typename VectorTraits<GF2, typename Matrix::ConstRow>::RepresentationType ());
+			row_ptr.push_back (blk.size ());
+		}
+#ifndef __LELA_HAVE_M4RI
+		if (ok) {
+			DenseMatrix<bool> Rd (X.rowdim (), X.coldim ());
+			gpu (row_ptr, blk, words, X.rowdim (), X.coldim (), Rd, rank, det);
+			BLAS3::copy (ctx, Rd, R);
+			return;
+		}
+#endif
+		cpu (R, X, rank, det, reduced, only_D);
+	}
+
+#ifndef __LELA_HAVE_M4RI
+	/** Dense 0-1 matrices (little-endian words without M4RI): the nonzero words as hybrid rows. */
+	void echelonize (DenseMatrix<bool> &R, const DenseMatrix<bool> &X, size_t &rank, bool &det, bool reduced = true,
+			 bool only_D = false)
+	{
+		if (!reduced || only_D) {
+			cpu (R, X, rank, det, reduced, only_D);
+			return;
+		}
+		const size_t m = X.rowdim (), n = X.coldim (), nw = (n + 63) / 64, ld = m && n ? hip_row_words (X) : 0;
+		std::vector<uint64_t> row_ptr (1, 0);
+		std::vector<uint32_t> blk;
+		std::vector<uint64_t> words;
+		const uint64 *W = m && n ? hip_words (X) : 0;
+		for (size_t i = 0; i < m; ++i) {
+			for (size_t w = 0; w < nw; ++w)
+				if (W[i * ld + w]) {
+					blk.push_back ((uint32_t) w);
+					words.push_back (W[i * ld + w]);
+				}
+			row_ptr.push_back (blk.size ());
+		}
+		DenseMatrix<bool> Rd (m, n);
+		gpu (row_ptr, blk, words, m, n, Rd, rank, det);
+		BLAS3::copy (ctx, Rd, R);
+	}
+#endif
+};
+
 /** EchelonForm over GF(2) on the HIP module.  The reduced form of a dense
  * 0-1 matrix runs on the GPU (mlz_gf2_rref); the other forms run LELA's
  * Elimination / FaugereLachartre on a GenericModule context.  The reference's
@@ -665,7 +786,7 @@ class EchelonForm<GF2, HIPModule<GF2> >
 	template <class Matrix>
 	size_t cpu_fl (Matrix &A, bool reduced)
 	{
-		FaugereLachartre<GF2, GenericModule<GF2> > FL (_cpu);
+		FaugereLachartre<GF2, HIPModule<GF2> > FL (_ctx);   // the GPU where the rows allow it
 		size_t rank = 0;
 		bool d;
 		FL.echelonize (A, A, rank, d, reduced);

@@ -230,6 +230,22 @@ int mlz_zp_fl_echelonize(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size
 int mlz_gf2_gemm(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda_w,
 		 const uint64_t *B, size_t ldb_w, int b, uint64_t *C, size_t ldc_w);
 
+/* Faugere-Lachartre reduction over GF(2) of an F4-shaped matrix X (m x n
+ * bits) given as hybrid rows (lela/vector/hybrid.h:30-33): row i holds the
+ * pairs (blk[e], words[e]) for e in [row_ptr[i], row_ptr[i+1]), word indices
+ * strictly increasing (LELA's are uint16, these uint32), bit j of a word =
+ * column 64*blk + j.  R (m x n bits, ldr_w words; may be NULL for rank and
+ * pivots only) receives the reduced row-echelon form; bits of R past column n
+ * keep their values.  pivots: rank column indices (min(m, n) entries, may be
+ * NULL).  Every head is 1 over GF(2), so LELA's det is 1 and is not returned.
+ * Replaces FaugereLachartre<GF2>::echelonize(R, X, rank, det, reduced=true)
+ * (lela/algorithms/faugere-lachartre.tcc:735-917) on
+ * SparseMatrix<bool, HybridVector<..>> (tests/test-faugere-lachartre.C:278-285).
+ * MLZ_E_WRONG_FORM: a head decreases at row *bad_row (WrongMatrixForm, :80-81). */
+int mlz_gf2_fl_echelonize(mlz_ctx *ctx, size_t m, size_t n, const uint64_t *row_ptr, const uint32_t *blk,
+			  const uint64_t *words, uint64_t *R, size_t ldr_w, size_t *rank, uint32_t *pivots,
+			  size_t *bad_row);
+
 /* B <- a*A^-1*B over GF(2) (a in {0, 1}); A n x n triangular (upper != 0:
  * upper), B n x nrhs bits, leading dimensions in words; bits of B beyond
  * column nrhs keep their values.  unit_diag != 0 takes A's diagonal as ones

@@ -25,7 +25,7 @@ __all__ = [
     "Modular", "Context", "BLAS3", "DeviceMatrix", "LELAError", "DiagonalEntryNotInvertible",
     "WrongMatrixForm", "LowerTriangular", "UpperTriangular", "ALGO_AUTO", "ALGO_PLAIN",
     "ALGO_INT8", "ALGO_FP64", "EchelonForm", "Elimination", "GF2", "Dense01Matrix", "FaugereLachartre",
-    "dense_to_csr",
+    "dense_to_csr", "dense01_to_hybrid",
 ]
 
 LowerTriangular = "lower"
@@ -420,6 +420,16 @@ def _gf2_echelonize(EF, A):
     return A
 
 
+def dense01_to_hybrid(X):
+    """Hybrid rows (row_ptr uint64, word index uint32, word uint64) of a
+    Dense01Matrix: the nonzero words of each row (lela/vector/hybrid.h:30-33)."""
+    W = np.asarray(X.words, dtype=np.uint64)
+    rows, blks = np.nonzero(W)
+    row_ptr = np.zeros(W.shape[0] + 1, dtype=np.uint64)
+    np.add.at(row_ptr, rows + 1, 1)
+    return np.cumsum(row_ptr, dtype=np.uint64), blks.astype(np.uint32), W[rows, blks]
+
+
 def dense_to_csr(X):
     """Row-compressed form (row_ptr uint64, col uint32, values) of a dense
     matrix, dropping zeros -- LELA's sparse rows store only nonzeros
@@ -433,7 +443,8 @@ def dense_to_csr(X):
 
 
 class FaugereLachartre:
-    """FaugereLachartre<Modular<T>, HIPModule> (lela/algorithms/faugere-lachartre.h:40-81)."""
+    """FaugereLachartre<Ring, HIPModule> (lela/algorithms/faugere-lachartre.h:40-81)
+    over Modular<T> (CSR or dense rows) and GF2 (hybrid rows)."""
 
     def __init__(self, ctx):
         self.ctx = ctx
@@ -450,6 +461,8 @@ class FaugereLachartre:
         if not reduced:
             raise LELAError("only the reduced form is computed on the GPU")
         ctx, F = self.ctx, self.ctx.F
+        if isinstance(F, GF2):
+            return self._gf2(R, X, shape)
         if isinstance(X, tuple):
             row_ptr, col, val = X
             m, n = shape
@@ -476,6 +489,38 @@ class FaugereLachartre:
             if ctx._ptr_mode != PTR_HOST:
                 ctx.set_ptr_mode(PTR_HOST)
         return self._call(m, n, row_ptr.ctypes.data, col.ctypes.data, val.ctypes.data, pr, ldr, algo)
+
+    def _gf2(self, R, X, shape):
+        """FaugereLachartre<GF2> on hybrid rows: X is a Dense01Matrix or a
+        triple (row_ptr, blk, words) with `shape` (lela/vector/hybrid.h:30-33);
+        R a Dense01Matrix or None.  Returns (rank, 1): every head is 1."""
+        ctx = self.ctx
+        if isinstance(X, Dense01Matrix):
+            m, n = X.rowdim(), X.coldim()
+            row_ptr, blk, words = dense01_to_hybrid(X)
+        else:
+            row_ptr, blk, words = X
+            m, n = shape
+        row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint64)
+        blk = np.ascontiguousarray(blk, dtype=np.uint32)
+        words = np.ascontiguousarray(words, dtype=np.uint64)
+        if ctx._ptr_mode != PTR_HOST:
+            ctx.set_ptr_mode(PTR_HOST)
+        if R is not None:
+            pr, ldr, rm, rn = _gf2_operand(ctx, R)
+            if (rm, rn) != (m, n):
+                raise LELAError("R must have the shape of X")
+        else:
+            pr, ldr = None, (n + 63) // 64
+        rank = ctypes.c_size_t(0)
+        bad = ctypes.c_size_t(0)
+        piv = np.zeros(max(1, min(m, n)), dtype=np.uint32)
+        rc = ctx.lib.mlz_gf2_fl_echelonize(ctx.handle, m, n, row_ptr.ctypes.data, blk.ctypes.data,
+                                           words.ctypes.data, pr, ldr, ctypes.byref(rank), piv.ctypes.data,
+                                           ctypes.byref(bad))
+        ctx.check(rc, "gf2 fl_echelonize", idx=bad.value if rc == _lib.MLZ_E_WRONG_FORM else None)
+        self.pivots = piv[:rank.value].copy()
+        return rank.value, 1
 
     def _call(self, m, n, row_ptr, col, val, pr, ldr, algo):
         ctx, F = self.ctx, self.ctx.F

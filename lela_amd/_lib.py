@@ -60,6 +60,8 @@ SIGNATURES = {
     "mlz_gf2_gemm": (_c_int, [_ctx, _c_size, _c_size, _c_size, _c_int, _vp, _c_size, _vp, _c_size,
                               _c_int, _vp, _c_size]),
     "mlz_gf2_rref": (_c_int, [_ctx, _c_size, _c_size, _vp, _c_size, ctypes.POINTER(_c_size), _vp]),
+    "mlz_gf2_fl_echelonize": (_c_int, [_ctx, _c_size, _c_size, _vp, _vp, _vp, _vp, _c_size, ctypes.POINTER(_c_size),
+                                       _vp, ctypes.POINTER(_c_size)]),
     "mlz_gf2_trsm": (_c_int, [_ctx, _c_int, _c_int, _c_size, _c_size, _c_int, _vp, _c_size, _vp, _c_size,
                               ctypes.POINTER(_c_size)]),
     "mlz_zp_echelonize": (_c_int, [_ctx, _c_int, _c_u64, _c_size, _c_size, _vp, _c_size, _c_int,

@@ -38,6 +38,7 @@ CONFIG = """#define __LELA_INT8 char
 TU = r"""
 #include "lela_amd/hip-module.h"
 #include "lela/matrix/sparse.h"
+#include "lela/vector/hybrid.h"
 using namespace LELA;
 
 template <class E>
@@ -98,6 +99,17 @@ void gf2_hooks ()
 	EF.echelonize (A, false);
 	EF.echelonize (A, true, EF_t::METHOD_FAUGERE_LACHARTRE);
 	size_t r = EF.rank (A);
+	FaugereLachartre<GF2, HIPModule<GF2> > FL (ctx);
+	size_t rank;
+	bool det;
+	FL.echelonize (C, A, rank, det, true);
+	typedef SparseMatrix<bool, HybridVector<DefaultEndianness<uint64>, uint16, uint64> > Hyb;
+	Hyb X (64, 64), R (64, 64);
+	FL.echelonize (R, X, rank, det, true);
+	FL.echelonize (R, X, rank, det, false);
+	SparseMatrix<bool> S (64, 64), T (64, 64);
+	FL.echelonize (T, S, rank, det, true);
+	EF.echelonize (X, true, EF_t::METHOD_FAUGERE_LACHARTRE);
 	(void) r;
 }
 
