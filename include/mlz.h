@@ -200,6 +200,21 @@ enum { MLZ_ECH_U = 0, MLZ_ECH_TRANSFORM = 1, MLZ_ECH_PLUQ = 2 };
 int mlz_zp_echelonize(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t n, void *A, size_t lda,
 		      int mode, size_t *rank, uint64_t *det, uint32_t *perm_rows, uint32_t *perm_cols, int algo);
 
+/* The same three forms with SparsePartialPivotStrategy, the pivot rule LELA's
+ * Elimination uses for matrices with sparse, sparse 0-1 and hybrid 0-1 rows
+ * (DefaultPivotStrategy, lela/algorithms/pivot-strategy.h:150-160;
+ * pivot-strategy.tcc:69-188): at step i, among the rows at positions >= i,
+ * those whose first nonzero at or after the search column (the previous
+ * pivot's column + 1; i for pluq) is leftmost, and of those the one with the
+ * fewest nonzeros from that column on -- count_words != 0: the fewest nonzero
+ * 64-column words from the pivot's word on, the hybrid-row count -- the first
+ * such row on ties.  Each step then eliminates as mlz_zp_echelonize's modes
+ * do (:112-123, :330-350).  A is dense here (LELA's sparse rows unpacked;
+ * GF(2) rows as Z/2 entries); outputs as mlz_zp_echelonize.  m, n < 2^21. */
+int mlz_zp_echelonize_sparse(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t n, void *A, size_t lda,
+			     int mode, int count_words, size_t *rank, uint64_t *det, uint32_t *perm_rows,
+			     uint32_t *perm_cols);
+
 /* Faugere-Lachartre reduction of an F4-shaped matrix X (m x n, CSR: row_ptr
  * [m+1], col [nnz] strictly increasing within each row, val [nnz] in the
  * element storage) to its reduced row-echelon form R (m x n dense, ldr; may be

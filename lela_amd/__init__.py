@@ -25,7 +25,7 @@ __all__ = [
     "Modular", "Context", "BLAS3", "DeviceMatrix", "LELAError", "DiagonalEntryNotInvertible",
     "WrongMatrixForm", "LowerTriangular", "UpperTriangular", "ALGO_AUTO", "ALGO_PLAIN",
     "ALGO_INT8", "ALGO_FP64", "EchelonForm", "Elimination", "GF2", "Dense01Matrix", "FaugereLachartre",
-    "dense_to_csr", "dense01_to_hybrid",
+    "dense_to_csr", "dense01_to_hybrid", "DensePivotStrategy", "SparsePartialPivotStrategy",
 ]
 
 LowerTriangular = "lower"
@@ -313,41 +313,64 @@ class EchelonForm:
         return self._rank_table[id(A)]
 
 
+class DensePivotStrategy:
+    """LELA's pivot rule for dense rows: the first nonzero column, the first
+    row in it (lela/algorithms/pivot-strategy.h:57-91)."""
+
+
+class SparsePartialPivotStrategy:
+    """LELA's pivot rule for sparse rows: the leftmost first nonzero, then the
+    fewest nonzeros from there on, the first row on ties; words=True counts
+    nonzero 64-column words as for GF(2) hybrid rows
+    (lela/algorithms/pivot-strategy.h:93-131, pivot-strategy.tcc:69-188)."""
+
+    def __init__(self, words=False):
+        self.words = bool(words)
+
+
 class Elimination:
     """Elimination<Ring, HIPModule> (lela/algorithms/elimination.h:34-238) for
-    dense Z/p matrices with LELA's DensePivotStrategy.  Permutations are
-    LELA's: lists of transpositions (i, j)."""
+    Z/p matrices, dense storage.  PS is the pivot strategy LELA would take
+    from the row type (DefaultPivotStrategy, pivot-strategy.h:138-160):
+    DensePivotStrategy() for dense rows (the default here),
+    SparsePartialPivotStrategy() for sparse rows.  Permutations are LELA's:
+    lists of transpositions (i, j)."""
 
     def __init__(self, ctx):
         self.ctx = ctx
 
-    def _run(self, A, mode, algo):
+    def _run(self, A, mode, algo, PS=None):
         ctx, F = self.ctx, self.ctx.F
         pa, lda, m, n = _operand(ctx, A, F.dtype)
         rank = ctypes.c_size_t(0)
         det = ctypes.c_uint64(0)
         pr = np.zeros(max(1, min(m, n)), dtype=np.uint32)
         pc = np.zeros(max(1, min(m, n)), dtype=np.uint32)
-        rc = ctx.lib.mlz_zp_echelonize(ctx.handle, F.elem, F.p, m, n, pa, lda, mode, ctypes.byref(rank),
-                                       ctypes.byref(det), pr.ctypes.data, pc.ctypes.data, algo)
+        if isinstance(PS, SparsePartialPivotStrategy):
+            rc = ctx.lib.mlz_zp_echelonize_sparse(ctx.handle, F.elem, F.p, m, n, pa, lda, mode, int(PS.words),
+                                                  ctypes.byref(rank), ctypes.byref(det), pr.ctypes.data,
+                                                  pc.ctypes.data)
+        else:
+            rc = ctx.lib.mlz_zp_echelonize(ctx.handle, F.elem, F.p, m, n, pa, lda, mode, ctypes.byref(rank),
+                                           ctypes.byref(det), pr.ctypes.data, pc.ctypes.data, algo)
         if rc == _lib.MLZ_E_NOT_INVERTIBLE:
             raise LELAError("Could not invert pivot-element in the ring")
         ctx.check(rc, "echelonize")
         r = rank.value
         return pr[:r].copy(), pc[:r].copy(), r, int(det.value)
 
-    def echelonize(self, A, compute_L=True, algo=ALGO_AUTO):
+    def echelonize(self, A, compute_L=True, algo=ALGO_AUTO, PS=None):
         """A <- its row-echelon form U (elimination.tcc:47-139); with
         compute_L (LELA's default) the part left of the diagonal holds T with
         T P A = U.  Returns (P, rank, det)."""
-        pr, _, r, det = self._run(A, _lib.ECH_TRANSFORM if compute_L else _lib.ECH_U, algo)
+        pr, _, r, det = self._run(A, _lib.ECH_TRANSFORM if compute_L else _lib.ECH_U, algo, PS)
         self.steps = pr
         return [(i, int(j)) for i, j in enumerate(pr) if j != i], r, det
 
-    def pluq(self, A, algo=ALGO_AUTO):
+    def pluq(self, A, algo=ALGO_AUTO, PS=None):
         """A <- L and U of P A Q = L U (elimination.tcc:266-370).  Returns
         (P, Q, rank, det) with LELA's lists (reversed step order)."""
-        pr, pc, r, det = self._run(A, _lib.ECH_PLUQ, algo)
+        pr, pc, r, det = self._run(A, _lib.ECH_PLUQ, algo, PS)
         P = [(i, int(j)) for i, j in enumerate(pr) if j != i][::-1]
         Q = [(i, int(j)) for i, j in enumerate(pc) if j != i][::-1]
         return P, Q, r, det

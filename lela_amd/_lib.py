@@ -20,6 +20,10 @@ MLZ_E_NOT_INVERTIBLE = 3
 MLZ_E_WRONG_FORM = 4
 MLZ_E_NOMEM = 5
 MLZ_E_UNSUPPORTED = 6
+MLZ_E_FORMAT = 7
+MLZ_E_INPUT = 8
+MLZ_E_FILE = 9
+MLZ_IO_GF2 = 4
 NOALLOC_SCRATCH_MB = 64
 
 _c_size = ctypes.c_size_t
@@ -66,6 +70,8 @@ SIGNATURES = {
                               ctypes.POINTER(_c_size)]),
     "mlz_zp_echelonize": (_c_int, [_ctx, _c_int, _c_u64, _c_size, _c_size, _vp, _c_size, _c_int,
                                    ctypes.POINTER(_c_size), ctypes.POINTER(_c_u64), _vp, _vp, _c_int]),
+    "mlz_zp_echelonize_sparse": (_c_int, [_ctx, _c_int, _c_u64, _c_size, _c_size, _vp, _c_size, _c_int, _c_int,
+                                          ctypes.POINTER(_c_size), ctypes.POINTER(_c_u64), _vp, _vp]),
     "mlz_zp_fl_echelonize": (_c_int, [_ctx, _c_int, _c_u64, _c_size, _c_size, _vp, _vp, _vp, _vp,
                                       _c_size, ctypes.POINTER(_c_size), _vp, ctypes.POINTER(_c_u64),
                                       ctypes.POINTER(_c_size), _c_int]),
@@ -84,6 +90,15 @@ SIGNATURES = {
     "mlz_model_set": (_c_int, [_ctx, _c_int, _vp]),
     "mlz_model_predict": (_c_int, [_ctx, _c_int, _vp, _c_size, _vp]),
     "mlz_model_grad": (_c_int, [_ctx, _vp, _vp, _vp, _vp, _c_size, _vp, ctypes.POINTER(ctypes.c_float)]),
+    # include/mlz_io.h: matrix files (host code)
+    "mlz_matrix_detect_format": (_c_int, [ctypes.c_char_p, ctypes.POINTER(_c_int)]),
+    "mlz_matrix_read": (_c_int, [ctypes.c_char_p, _c_int, _c_int, _c_u64, _c_size, _c_size, ctypes.POINTER(_vp)]),
+    "mlz_matrix_shape": (_c_int, [_vp, ctypes.POINTER(_c_size), ctypes.POINTER(_c_size), ctypes.POINTER(_c_size)]),
+    "mlz_matrix_get_csr": (_c_int, [_vp, _vp, _vp, _vp]),
+    "mlz_matrix_release": (_c_int, [_vp]),
+    "mlz_matrix_write_csr": (_c_int, [ctypes.c_char_p, _c_int, _c_int, _c_u64, _c_size, _c_size, _vp, _vp, _vp]),
+    "mlz_matrix_write_dense": (_c_int, [ctypes.c_char_p, _c_int, _c_int, _c_u64, _c_size, _c_size, _vp, _c_size]),
+    "mlz_io_last_error": (ctypes.c_char_p, []),
 }
 
 TASK_GEMM, TASK_RREF = 1, 2

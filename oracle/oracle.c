@@ -407,6 +407,107 @@ int orc_zp_pluq(uint32_t p, size_t m, size_t n, uint32_t *A, size_t lda, size_t 
 	return ORC_OK;
 }
 
+/* SparsePartialPivotStrategy::getPivot (lela/algorithms/pivot-strategy.tcc:69-99
+ * for sparse rows, :132-183 for hybrid 0-1 rows) on dense storage: scanning the
+ * rows from *row down, the row whose first nonzero at or after start = *col is
+ * leftmost, and of those the one with the fewest nonzeros in [start, n) --
+ * words != 0: the fewest nonzero 64-column words from the pivot's word on
+ * (the hybrid row's stored blocks, i->end () - block) -- the first on ties
+ * (strict <).  Returns 0 when no row has a nonzero at or after start. */
+static int sparse_pivot(const uint32_t *A, size_t lda, size_t m, size_t n, int words, size_t *row, size_t *col)
+{
+	const size_t start = *col;
+	size_t best_col = n, best_cnt = (size_t) -1, best_row = 0;
+	for (size_t r = *row; r < m; ++r) {
+		const uint32_t *R = A + r * lda;
+		size_t lead = start;
+		while (lead < n && R[lead] == 0)
+			++lead;
+		if (lead == n)
+			continue;
+		size_t cnt = 0;
+		if (!words) {
+			for (size_t c = lead; c < n; ++c)
+				cnt += R[c] != 0;
+		} else {
+			for (size_t w = lead / 64; w * 64 < n; ++w) {
+				int any = 0;
+				for (size_t c = w * 64; c < n && c < w * 64 + 64; ++c)
+					any |= R[c] != 0;
+				cnt += any;
+			}
+		}
+		if (lead < best_col) {
+			best_col = lead;
+			best_cnt = cnt;
+			best_row = r;
+		} else if (lead == best_col && cnt < best_cnt) {
+			best_cnt = cnt;
+			best_row = r;
+		}
+	}
+	if (best_cnt == (size_t) -1)
+		return 0;
+	*row = best_row;
+	*col = best_col;
+	return 1;
+}
+
+int orc_zp_elim_sparse(uint32_t p, size_t m, size_t n, uint32_t *A, size_t lda, int mode, int words,
+		       size_t *rank, uint32_t *det, uint32_t *perm_rows, uint32_t *perm_cols)
+{
+	if (p < 2 || mode < 0 || mode > 2)
+		return ORC_E_ARG;
+	const int pluq = mode == 2;
+	size_t r = 0;
+	uint32_t d = 1 % p;
+	size_t i = 0, pivot_col = 0;
+	/* elimination.tcc:73-131 (echelonize), :296-355 (pluq) */
+	for (; i < m && (pluq ? i : pivot_col) < n; ++i, ++pivot_col) {
+		size_t pivot_row = i;
+		if (pluq)
+			pivot_col = i;
+		if (!sparse_pivot(A, lda, m, n, words, &pivot_row, &pivot_col))
+			break;
+		if (i != pivot_row)
+			swap_rows_u32(A, lda, n, i, pivot_row);
+		if (pluq && i != pivot_col)
+			swap_cols_u32(A, lda, m, i, pivot_col);
+		if (perm_rows)
+			perm_rows[i] = (uint32_t) pivot_row;
+		if (perm_cols)
+			perm_cols[i] = (uint32_t) (pluq ? pivot_col : i);
+		const size_t pc = pluq ? i : pivot_col;
+		const uint32_t x = A[i * lda + pc];
+		d = zp_mul(p, d, x);
+		uint32_t xinv;
+		if (!orc_inv(p, x, &xinv))
+			return ORC_E_NOT_INVERTIBLE;
+		const uint32_t negxinv = zp_neg(p, xinv);
+		const uint32_t *Pi = A + i * lda;
+		for (size_t j = i + 1; j < m; ++j) {
+			uint32_t *Rj = A + j * lda;
+			const uint32_t aj = Rj[pc];
+			if (aj == 0)
+				continue;
+			const uint32_t c = zp_mul(p, aj, negxinv);
+			const size_t c0 = mode == 0 ? pc : mode == 1 ? 0 : i + 1;
+			for (size_t col = c0; col < n; ++col)
+				Rj[col] = zp_axpy(p, c, Pi[col], Rj[col]);
+			if (mode == 1)
+				Rj[i] = c;
+			else if (mode == 2)
+				Rj[i] = zp_neg(p, c);
+		}
+		++r;
+	}
+	if (rank)
+		*rank = r;
+	if (det)
+		*det = d;
+	return ORC_OK;
+}
+
 /* ------------------------------------------------------------------ */
 /* GF(2)                                                               */
 /* ------------------------------------------------------------------ */

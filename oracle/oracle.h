@@ -79,6 +79,13 @@ int orc_gf2_trsm(int upper, int unit_diag, size_t n, size_t nrhs, int a, const u
 /* Elimination::pluq (elimination.tcc:266-370); per-step row and column swaps. */
 int orc_zp_pluq(uint32_t p, size_t m, size_t n, uint32_t *A, size_t lda, size_t *rank, uint32_t *det,
 		uint32_t *perm_rows, uint32_t *perm_cols);
+/* Elimination::echelonize (mode 0: compute_L = false, 1: compute_L = true)
+ * and Elimination::pluq (mode 2) with SparsePartialPivotStrategy
+ * (lela/algorithms/pivot-strategy.tcc:69-188; words != 0 counts nonzero
+ * 64-column words as for hybrid rows).  perm_rows / perm_cols as orc_zp_elim
+ * and orc_zp_pluq. */
+int orc_zp_elim_sparse(uint32_t p, size_t m, size_t n, uint32_t *A, size_t lda, int mode, int words,
+		       size_t *rank, uint32_t *det, uint32_t *perm_rows, uint32_t *perm_cols);
 int orc_gf2_echelonize(size_t m, size_t n, uint64_t *A, size_t lda_w, int reduced,
 		       size_t *rank, uint32_t *pivots);
 

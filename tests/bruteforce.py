@@ -255,3 +255,69 @@ def fl_det(X, p):
     for t, c in enumerate(cols):
         d = d * int(U[t, c]) % p
     return d, len(cols)
+
+
+def sparse_rule_replay(A, p, mode, words, perm_rows, perm_cols):
+    """Checks a pivot sequence against the definition of LELA's
+    SparsePartialPivotStrategy and replays the elimination it implies.
+
+    At step i every row at a position >= i is described by (first nonzero at
+    or after the search column, its count, its position), where the count is
+    the number of nonzero entries from the search column on, or with `words`
+    the number of nonzero 64-column words from the first nonzero's word on;
+    the chosen row must carry the smallest such triple.  The elimination is
+    then the textbook one (swap, clear the column below; mode 1 keeps the
+    multipliers left of the diagonal, mode 2 also swaps the pivot column into
+    place and stores a/x), over Python integers.  Returns the final matrix
+    as an object array and det, or raises AssertionError."""
+    M = [[int(x) % p for x in row] for row in np.asarray(A)]
+    m = len(M)
+    n = len(M[0]) if m else 0
+    start = 0
+    det = 1 % p
+    for i, pr in enumerate(perm_rows):
+        if mode == 2:
+            start = i
+        cands = []
+        for r in range(i, m):
+            nz = [c for c in range(start, n) if M[r][c] % p]
+            if not nz:
+                continue
+            lead = nz[0]
+            if words:
+                cnt = len({c // 64 for c in range(lead, n) if M[r][c] % p})
+            else:
+                cnt = len(nz)
+            cands.append((lead, cnt, r))
+        assert cands, f"step {i}: no row left but a pivot was chosen"
+        lead, _, best = min(cands)
+        assert int(pr) == best, f"step {i}: chose row {pr}, the rule gives {best}"
+        M[i], M[best] = M[best], M[i]
+        if mode == 2:
+            assert int(perm_cols[i]) == lead, f"step {i}: column {perm_cols[i]}, the rule gives {lead}"
+            for r in range(m):
+                M[r][i], M[r][lead] = M[r][lead], M[r][i]
+        pc = i if mode == 2 else lead
+        x = M[i][pc]
+        det = det * x % p
+        xinv = pow(x, -1, p)
+        for j in range(i + 1, m):
+            a = M[j][pc]
+            if a == 0:
+                continue
+            c = (-a * xinv) % p
+            c0 = pc if mode == 0 else 0 if mode == 1 else i + 1
+            for col in range(c0, n):
+                M[j][col] = (M[j][col] + c * M[i][col]) % p
+            if mode == 1:
+                M[j][i] = c
+            elif mode == 2:
+                M[j][i] = (-c) % p
+        start = pc + 1
+    # the sequence must be complete: no row left with a nonzero at or after start
+    k = len(perm_rows)
+    if mode == 2:
+        start = k
+    for r in range(k, m):
+        assert not any(M[r][c] % p for c in range(start, n)), "elimination stopped early"
+    return np.array(M, dtype=object).reshape(m, n), det

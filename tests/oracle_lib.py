@@ -34,6 +34,8 @@ def load():
                                       ctypes.POINTER(u32)]
     lib.orc_zp_elim.argtypes = [u32, _sz, _sz, _vp, _sz, i, ctypes.POINTER(_sz), ctypes.POINTER(u32), _vp]
     lib.orc_zp_pluq.argtypes = [u32, _sz, _sz, _vp, _sz, ctypes.POINTER(_sz), ctypes.POINTER(u32), _vp, _vp]
+    lib.orc_zp_elim_sparse.argtypes = [u32, _sz, _sz, _vp, _sz, i, i, ctypes.POINTER(_sz), ctypes.POINTER(u32),
+                                       _vp, _vp]
     lib.orc_gf2_gemm.argtypes = [_sz, _sz, _sz, i, _vp, _sz, _vp, _sz, i, _vp, _sz]
     lib.orc_gf2_echelonize.argtypes = [_sz, _sz, _vp, _sz, i, ctypes.POINTER(_sz), _vp]
     lib.orc_gf2_trsm.argtypes = [i, i, _sz, _sz, i, _vp, _sz, _vp, _sz, ctypes.POINTER(_sz)]
@@ -44,7 +46,7 @@ def load():
     lib.orc_mlp_loss_grad.argtypes = [_vp, _vp, _vp, _vp, _sz, _vp, ctypes.POINTER(ctypes.c_double)]
     lib.orc_mlp_adam.argtypes = [_vp, _vp, _vp, _vp, _sz, i, ctypes.c_double, _vp]
     for name in ("orc_zp_gemm_u32", "orc_zp_gemm_u16", "orc_zp_trsm", "orc_zp_echelonize", "orc_zp_elim",
-                 "orc_zp_pluq",
+                 "orc_zp_pluq", "orc_zp_elim_sparse",
                  "orc_gf2_gemm", "orc_gf2_echelonize", "orc_gf2_trsm", "orc_fl_echelonize", "orc_inv",
                  "orc_mlp_forward", "orc_mlp_loss_grad", "orc_mlp_adam"):
         getattr(lib, name).restype = ctypes.c_int
@@ -117,6 +119,22 @@ def zp_pluq(p, A):
     pc = np.zeros(max(1, min(m, n)), dtype=np.uint32)
     rc = lib.orc_zp_pluq(p, m, n, R.ctypes.data, n, ctypes.byref(rank), ctypes.byref(det), pr.ctypes.data,
                          pc.ctypes.data)
+    return rc, R, rank.value, det.value, pr[:rank.value].copy(), pc[:rank.value].copy()
+
+
+def zp_elim_sparse(p, A, mode, words=False):
+    """Returns (rc, A_out, rank, det, perm_rows, perm_cols) of Elimination with
+    SparsePartialPivotStrategy: mode 0 echelonize, 1 echelonize with
+    compute_L, 2 pluq (step order)."""
+    lib = load()
+    R = _c(A, np.uint32).copy()
+    m, n = R.shape
+    rank = _sz(0)
+    det = ctypes.c_uint32(0)
+    pr = np.zeros(max(1, min(m, n)), dtype=np.uint32)
+    pc = np.zeros(max(1, min(m, n)), dtype=np.uint32)
+    rc = lib.orc_zp_elim_sparse(p, m, n, R.ctypes.data, n, int(mode), int(words), ctypes.byref(rank),
+                                ctypes.byref(det), pr.ctypes.data, pc.ctypes.data)
     return rc, R, rank.value, det.value, pr[:rank.value].copy(), pc[:rank.value].copy()
 
 

@@ -417,3 +417,99 @@ def test_pluq_factors(case):
             U[t, t:] = O[t, t:]
     PAQ = np.asarray(A).astype(object)[rorder][:, corder]
     assert np.array_equal(L.dot(U) % p, PAQ % p)
+
+
+def _sparse_pivot_inputs():
+    """Cases where the sparse rule and the dense one part ways: rows sharing a
+    first column with different counts, ties, zero rows and columns, words."""
+    out = []
+    # row 0 and row 2 start in column 0; row 2 has fewer nonzeros; rows 1, 3 tie
+    out.append((101, np.array([[1, 2, 3, 4, 5], [0, 1, 1, 0, 0], [7, 0, 0, 0, 1], [0, 1, 0, 1, 0]],
+                              dtype=np.uint32), False))
+    rng = np.random.default_rng(21)
+    for seed, (m, n, dens, p) in enumerate([(14, 20, 0.2, 101), (20, 14, 0.3, 65521), (25, 30, 0.1, 7),
+                                            (16, 16, 0.5, 2147483647), (30, 12, 0.25, 3), (12, 40, 0.15, 2)]):
+        r = np.random.default_rng(100 + seed)
+        S = np.where(r.random((m, n)) < dens, r.integers(1, p, size=(m, n)), 0).astype(np.uint32)
+        if seed % 2 == 0:
+            S[3] = 0
+            S[:, 1] = 0
+        out.append((p, S, False))
+    # GF(2) rows over 200 columns: counts of 64-column words
+    for seed in range(3):
+        r = np.random.default_rng(200 + seed)
+        B = (r.random((24, 200)) < 0.04).astype(np.uint32)
+        B[r.integers(0, 24, 6), r.integers(0, 200, 6)] = 1
+        out.append((2, B, True))
+    return out
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("case", range(10))
+def test_elim_sparse_follows_the_rule(case, mode):
+    """orc_zp_elim_sparse: every pivot satisfies SparsePartialPivotStrategy's
+    definition against the matrix as it stands at that step, and the result
+    is the textbook elimination with those pivots (bruteforce.sparse_rule_replay);
+    the forms then satisfy the factor identities of the dense rule's forms."""
+    p, A, words = _sparse_pivot_inputs()[case]
+    rc, O, rank, det, pr, pc = orc.zp_elim_sparse(p, A, mode, words)
+    assert rc == orc.ORC_OK
+    want, wdet = bf.sparse_rule_replay(A, p, mode, words, pr, pc)
+    assert np.array_equal(O.astype(object), want) and det == wdet
+    m, n = A.shape
+    R0, r0, piv0 = bf.rref_mod(A, p)
+    assert rank == r0
+    if mode == 0:
+        # an echelon form of A: same row space, so the same RREF
+        R, r2, piv = bf.rref_mod(O, p)
+        assert r2 == rank and piv == piv0 and np.array_equal(R, R0)
+        for t in range(rank, m):
+            assert not O[t].any()
+    if mode == 1:
+        rc0, U0, rank0, det0, pr0, _ = orc.zp_elim_sparse(p, A, 0, words)
+        assert list(pr0) == list(pr) and det0 == det
+        T = np.zeros((m, m), dtype=object)
+        U = O.astype(object).copy()
+        for t in range(m):
+            T[t, t] = 1
+            for s_ in range(min(t, n)):
+                T[t, s_] = U[t, s_]
+                U[t, s_] = 0
+        assert np.array_equal(U, U0.astype(object))
+        order = list(range(m))
+        for i, q in enumerate(pr):
+            order[i], order[q] = order[q], order[i]
+        assert np.array_equal(T.dot(np.asarray(A).astype(object)[order]) % p, U)
+    if mode == 2:
+        rorder, corder = list(range(m)), list(range(n))
+        for i, q in enumerate(pr):
+            rorder[i], rorder[q] = rorder[q], rorder[i]
+        for i, q in enumerate(pc):
+            corder[i], corder[q] = corder[q], corder[i]
+        Oo = O.astype(object)
+        L = np.zeros((m, rank), dtype=object)
+        U = np.zeros((rank, n), dtype=object)
+        for t in range(m):
+            for s_ in range(min(t, rank)):
+                L[t, s_] = Oo[t, s_]
+            if t < rank:
+                L[t, t] = 1
+                U[t, t:] = Oo[t, t:]
+        assert np.array_equal(L.dot(U) % p, np.asarray(A).astype(object)[rorder][:, corder] % p)
+
+
+def test_elim_sparse_hand_case():
+    """Worked by hand: rows 0 and 2 start in column 0, row 2 has 2 nonzeros
+    against 5, so the sparse rule takes row 2 (the dense rule takes row 0)."""
+    p = 101
+    A = np.array([[1, 2, 3, 4, 5], [0, 1, 1, 0, 0], [7, 0, 0, 0, 1], [0, 1, 0, 1, 0]], dtype=np.uint32)
+    rc, U, rank, det, pr, _ = orc.zp_elim_sparse(p, A, 0)
+    assert rc == 0 and pr[0] == 2
+    # step 1 from column 1: rows 1 (1 1 0 0 -> two entries), 2 (the old row 0,
+    # 2 3 4 5 - 1/7 -> five), 3 (1 0 1 0 -> two): 1 and 3 tie, row 1 is first
+    assert pr[1] == 1
+    rc_d, _, _, _, perm_d = orc.zp_elim(p, A)
+    assert perm_d[0] == 0
+    # the sparse-rule replay fails a sequence with the dense rule's first pivot
+    with pytest.raises(AssertionError):
+        bf.sparse_rule_replay(A, p, 0, False, [0] + list(pr[1:]), None)
