@@ -23,10 +23,12 @@
  *   FaugereLachartre::echelonize (R, X, rank, det, true)     -> GPU FL (Z/p: any row type; GF(2):
  *       hybrid rows of little-endian words and dense 0-1 matrices)
  *   Elimination::echelonize (A, P, rank, det, compute_L) and
- *   Elimination::pluq (A, P, Q, rank, det) of a dense Z/p matrix with the
- *       default pivot strategy                               -> GPU (mlz_zp_echelonize)
- *   the non-reduced forms over GF(2), explicit pivot
- *   strategies and every other storage                      -> LELA's own CPU code
+ *   Elimination::pluq (A, P, Q, rank, det) over Z/p with the
+ *       default pivot strategy: dense rows                  -> GPU (mlz_zp_echelonize)
+ *                               sparse rows                 -> GPU (mlz_zp_echelonize_sparse)
+ *   EchelonForm::echelonize (A, reduced) of sparse rows      -> GPU (the reduced form, or Elimination's
+ *       non-reduced form with the sparse pivot rule; GF(2): sparse and hybrid 0-1 rows over Z/2)
+ *   other explicit pivot strategies and every other storage -> LELA's own CPU code
  *
  * Determinant of FaugereLachartre: LELA multiplies det by the heads of the
  * scanned rows of X and by the heads of its (non-reduced) echelon form of
@@ -365,10 +367,13 @@ inline void hip_append_row (const Modular<Element> &F, const Row &row, std::vect
 }
 
 /** Elimination over Z/p on the HIP module (lela/algorithms/elimination.h:34-238).
- * echelonize (with and without compute_L) and pluq of a dense matrix with the
- * default pivot strategy (DensePivotStrategy, pivot-strategy.tcc:19-41) run
- * on the GPU through mlz_zp_echelonize, which reproduces the transpositions;
- * explicit pivot strategies, echelonize_reduced, move_L and other storage run
+ * echelonize (with and without compute_L) and pluq run on the GPU with the
+ * pivot strategy LELA takes from the row type (DefaultPivotStrategy,
+ * pivot-strategy.h:138-160): dense matrices with DensePivotStrategy
+ * (mlz_zp_echelonize), matrices with sparse rows with
+ * SparsePartialPivotStrategy (mlz_zp_echelonize_sparse, also when that
+ * strategy is passed explicitly); both reproduce the transpositions.  Other
+ * explicit strategies, echelonize_reduced, move_L and other row types run
  * LELA's Elimination on a ZpModule context. */
 template <class Element>
 class Elimination<Modular<Element>, HIPModule<Modular<Element> > >
@@ -409,6 +414,75 @@ private:
 		return rank;
 	}
 
+	/** Sparse rows: unpacked into a dense array, reduced on the GPU with the
+	 * sparse pivot rule, packed back without zeros (LELA's sparse axpy drops
+	 * them, level1-generic.tcc:222-254). */
+	template <class Matrix>
+	size_t gpu_sparse (Matrix &A, int mode, Permutation &P, Permutation *Q, Element &det) const
+	{
+		const size_t m = A.rowdim (), n = A.coldim (), r = std::min (m, n);
+		std::vector<Element> D (m * n, ctx.F.zero ());
+		size_t i = 0;
+		for (typename Matrix::RowIterator row = A.rowBegin (); row != A.rowEnd (); ++row, ++i)
+			for (typename Matrix::Row::const_iterator j = row->begin (); j != row->end (); ++j)
+				D[i * n + j->first] = j->second;
+		std::vector<uint32_t> pr (r + 1), pc (r + 1);
+		size_t rank = 0;
+		uint64_t d = 1;
+		P.clear ();
+		if (Q)
+			Q->clear ();
+		if (m && n) {
+			const int rc = mlz_zp_echelonize_sparse (ctx.M.dev.ctx (), HIPElement<Element>::value (),
+								 (uint64_t) ctx.F._modulus, m, n, &D[0], n, mode, 0,
+								 &rank, &d, &pr[0], &pc[0]);
+			if (rc == MLZ_E_NOT_INVERTIBLE)
+				throw LELAError ("Could not invert pivot-element in the ring");   // elimination.tcc:103-104
+			ctx.M.dev.check (rc);
+		}
+		i = 0;
+		for (typename Matrix::RowIterator row = A.rowBegin (); row != A.rowEnd (); ++row, ++i) {
+			row->clear ();
+			for (size_t j = 0; j < n; ++j)
+				if (!ctx.F.isZero (D[i * n + j]))
+					row->push_back (typename Matrix::Row::value_type (j, D[i * n + j]));
+		}
+		for (size_t s = 0; s < rank; ++s) {
+			if (pr[s] != s)
+				P.push_back (Transposition ((uint32) s, (uint32) pr[s]));
+			if (Q && pc[s] != s)
+				Q->push_back (Transposition ((uint32) s, (uint32) pc[s]));
+		}
+		ctx.F.init (det, (long) d);
+		return rank;
+	}
+
+	template <class Matrix>
+	Matrix &echelonize_rows (Matrix &A, Permutation &P, size_t &rank, Element &det, bool compute_L,
+				 VectorRepresentationTypes::Sparse) const
+	{
+		rank = gpu_sparse (A, compute_L ? MLZ_ECH_TRANSFORM : MLZ_ECH_U, P, 0, det);
+		return A;
+	}
+
+	template <class Matrix, class Rep>
+	Matrix &echelonize_rows (Matrix &A, Permutation &P, size_t &rank, Element &det, bool compute_L, Rep) const
+		{ return _cpu.echelonize (A, P, rank, det, compute_L); }
+
+	template <class Matrix>
+	Matrix &pluq_rows (Matrix &A, Permutation &P, Permutation &Q, size_t &rank, Element &det,
+			   VectorRepresentationTypes::Sparse) const
+	{
+		rank = gpu_sparse (A, MLZ_ECH_PLUQ, P, &Q, det);
+		std::reverse (P.begin (), P.end ());
+		std::reverse (Q.begin (), Q.end ());
+		return A;
+	}
+
+	template <class Matrix, class Rep>
+	Matrix &pluq_rows (Matrix &A, Permutation &P, Permutation &Q, size_t &rank, Element &det, Rep) const
+		{ return _cpu.pluq (A, P, Q, rank, det); }
+
 public:
 	Elimination (Context<Modular<Element>, HIPModule<Modular<Element> > > &_ctx)
 		: ctx (_ctx), _cpu_ctx (_ctx.F), _cpu (_cpu_ctx) {}
@@ -421,9 +495,21 @@ public:
 		return A;
 	}
 
+	/** Sparse rows: the GPU with the sparse pivot rule; other rows: the CPU. */
 	template <class Matrix>
 	Matrix &echelonize (Matrix &A, Permutation &P, size_t &rank, Element &det, bool compute_L = true) const
-		{ return _cpu.echelonize (A, P, rank, det, compute_L); }
+	{
+		return echelonize_rows (A, P, rank, det, compute_L,
+					typename VectorTraits<Modular<Element>, typename Matrix::Row>::RepresentationType ());
+	}
+
+	template <class Matrix, class Modules>
+	Matrix &echelonize (Matrix &A, Permutation &P, size_t &rank, Element &det,
+			    SparsePartialPivotStrategy<Modular<Element>, Modules>, bool compute_L) const
+	{
+		return echelonize_rows (A, P, rank, det, compute_L,
+					typename VectorTraits<Modular<Element>, typename Matrix::Row>::RepresentationType ());
+	}
 
 	template <class Matrix, class PivotStrategy>
 	Matrix &echelonize (Matrix &A, Permutation &P, size_t &rank, Element &det, PivotStrategy PS, bool compute_L) const
@@ -449,9 +535,13 @@ public:
 		return A;
 	}
 
+	/** Sparse rows: the GPU with the sparse pivot rule; other rows: the CPU. */
 	template <class Matrix>
 	Matrix &pluq (Matrix &A, Permutation &P, Permutation &Q, size_t &rank, Element &det) const
-		{ return _cpu.pluq (A, P, Q, rank, det); }
+	{
+		return pluq_rows (A, P, Q, rank, det,
+				  typename VectorTraits<Modular<Element>, typename Matrix::Row>::RepresentationType ());
+	}
 
 	template <class Matrix, class PivotStrategy>
 	Matrix &pluq (Matrix &A, Permutation &P, Permutation &Q, size_t &rank, Element &det, PivotStrategy PS) const
@@ -528,8 +618,8 @@ public:
 };
 
 /** EchelonForm over Z/p on the HIP module (lela/solutions/echelon-form.h:31-164).
- * Dense matrices and the Faugere-Lachartre method run on the GPU; other
- * storage runs LELA's own Elimination on a ZpModule context.  The dense
+ * Dense matrices, sparse rows and the Faugere-Lachartre method run on the
+ * GPU; other storage runs LELA's own Elimination on a ZpModule context.  The dense
  * non-reduced form is Elimination's with transposition pivoting, which both
  * methods give (echelon-form.h:113-131, tests/test-gauss-jordan.C:92-99). */
 template <class Element>
@@ -554,6 +644,45 @@ class EchelonForm<Modular<Element>, HIPModule<Modular<Element> > >
 		return rank;
 	}
 
+	/** Sparse rows, standard method (:86-91): the non-reduced form is
+	 * Elimination's with the sparse pivot rule, on the GPU; the reduced form
+	 * is unique, so the dense reduced kernel gives it. */
+	template <class Matrix>
+	size_t std_echelonize (Matrix &A, bool reduced, VectorRepresentationTypes::Sparse)
+	{
+		size_t rank = 0;
+		Element d;
+		if (!reduced) {
+			Elimination<Modular<Element>, HIPModule<Modular<Element> > > elim (_ctx);
+			typename Elimination<Modular<Element>, HIPModule<Modular<Element> > >::Permutation P;
+			elim.echelonize (A, P, rank, d, false);
+			return rank;
+		}
+		const size_t m = A.rowdim (), n = A.coldim ();
+		if (!m || !n)
+			return 0;
+		std::vector<Element> D (m * n, _ctx.F.zero ());
+		size_t i = 0;
+		for (typename Matrix::RowIterator row = A.rowBegin (); row != A.rowEnd (); ++row, ++i)
+			for (typename Matrix::Row::const_iterator j = row->begin (); j != row->end (); ++j)
+				D[i * n + j->first] = j->second;
+		std::vector<uint32_t> pivots (std::min (m, n) + 1);
+		_ctx.M.dev.check (mlz_zp_rref (_ctx.M.dev.ctx (), HIPElement<Element>::value (), (uint64_t) _ctx.F._modulus,
+					       m, n, &D[0], n, &rank, &pivots[0], MLZ_ALGO_AUTO));
+		i = 0;
+		for (typename Matrix::RowIterator row = A.rowBegin (); row != A.rowEnd (); ++row, ++i) {
+			row->clear ();
+			for (size_t j = 0; j < n; ++j)
+				if (!_ctx.F.isZero (D[i * n + j]))
+					row->push_back (typename Matrix::Row::value_type (j, D[i * n + j]));
+		}
+		return rank;
+	}
+
+	template <class Matrix, class Rep>
+	size_t std_echelonize (Matrix &A, bool reduced, Rep)
+		{ return cpu_echelonize (A, reduced); }
+
 	template <class Matrix>
 	size_t fl_echelonize (Matrix &A, bool reduced)
 	{
@@ -571,14 +700,16 @@ public:
 	EchelonForm (Context<Modular<Element>, HIPModule<Modular<Element> > > &ctx)
 		: _ctx (ctx), _cpu (ctx.F) {}
 
-	/** Any matrix (sparse rows included): LELA's standard method on the CPU,
-	 * or Faugere-Lachartre with the caller's `reduced` (:86-94). */
+	/** Any matrix: the standard method (sparse rows on the GPU, other rows
+	 * LELA's CPU Elimination), or Faugere-Lachartre with the caller's
+	 * `reduced` (:86-94). */
 	template <class Matrix>
 	Matrix &echelonize (Matrix &A, bool reduced = false, Method method = METHOD_STANDARD_GJ)
 	{
 		size_t rank;
 		if (method == METHOD_STANDARD_GJ)
-			rank = cpu_echelonize (A, reduced);
+			rank = std_echelonize (A, reduced,
+					       typename VectorTraits<Modular<Element>, typename Matrix::Row>::RepresentationType ());
 		else if (method == METHOD_FAUGERE_LACHARTRE)
 			rank = fl_echelonize (A, reduced);
 		else
@@ -755,9 +886,12 @@ public:
 #endif
 };
 
-/** EchelonForm over GF(2) on the HIP module.  The reduced form of a dense
- * 0-1 matrix runs on the GPU (mlz_gf2_rref); the other forms run LELA's
- * Elimination / FaugereLachartre on a GenericModule context.  The reference's
+/** EchelonForm over GF(2) on the HIP module.  On the GPU: dense 0-1
+ * matrices (reduced: mlz_gf2_rref; non-reduced: Elimination's form over Z/2),
+ * sparse and hybrid 0-1 rows with the standard method (reduced: mlz_gf2_rref;
+ * non-reduced: the sparse pivot rule over Z/2) and the Faugere-Lachartre
+ * method on hybrid and dense rows; other rows run LELA's Elimination /
+ * FaugereLachartre on a GenericModule context.  The reference's
  * own GF(2) specialisation (echelon-form-gf2.h:37-182) exists only with M4RI;
  * without it EchelonForm<GF2> uses the generic class (echelon-form.h:31-164),
  * whose dense overload runs GaussJordan (reduced: the same RREF). */
@@ -783,6 +917,107 @@ class EchelonForm<GF2, HIPModule<GF2> >
 		return rank;
 	}
 
+	/** Rows of 0-1 matrices as one 16-bit Z/2 entry per column (the GPU's
+	 * non-reduced forms over GF(2) run over Z/2: the same eliminations) and
+	 * back; false for rows the ABI does not take (LELA's CPU code then runs). */
+	template <class Row>
+	static bool unpack_row (const Row &row, uint16_t *D, size_t, VectorRepresentationTypes::Sparse01)
+	{
+		for (typename Row::const_iterator j = row.begin (); j != row.end (); ++j)
+			D[*j] = 1;
+		return true;
+	}
+
+	template <class Row>
+	static bool unpack_row (const Row &row, uint16_t *D, size_t n, VectorRepresentationTypes::Hybrid01)
+	{
+		std::vector<uint32_t> blk;
+		std::vector<uint64_t> words;
+		if (!hip_hybrid_row (row, blk, words, VectorRepresentationTypes::Hybrid01 ()))
+			return false;
+		for (size_t e = 0; e < blk.size (); ++e)
+			for (size_t b = 0; b < 64 && 64 * (size_t) blk[e] + b < n; ++b)
+				if ((words[e] >> b) & 1)
+					D[64 * (size_t) blk[e] + b] = 1;
+		return true;
+	}
+
+	template <class Row>
+	static void pack_row (Row &row, const uint16_t *D, size_t n, VectorRepresentationTypes::Sparse01)
+	{
+		row.clear ();
+		for (size_t j = 0; j < n; ++j)
+			if (D[j])
+				row.push_back (j);
+	}
+
+	template <class Row>
+	static void pack_row (Row &row, const uint16_t *D, size_t n, VectorRepresentationTypes::Hybrid01)
+	{
+		row.clear ();
+		for (size_t w = 0; 64 * w < n; ++w) {
+			uint64_t x = 0;
+			for (size_t b = 0; b < 64 && 64 * w + b < n; ++b)
+				x |= (uint64_t) (D[64 * w + b] & 1) << b;
+			if (x)
+				row.push_back (typename Row::value_type (w, x));
+		}
+	}
+
+	/** Sparse and hybrid 0-1 rows, standard method (echelon-form.h:86-91):
+	 * the reduced form is unique (mlz_gf2_rref on the bits); the non-reduced
+	 * form is Elimination's with SparsePartialPivotStrategy, which counts
+	 * entries for sparse rows and words for hybrid rows
+	 * (pivot-strategy.tcc:101-188). */
+	template <class Matrix, class Rep>
+	size_t std_sparse (Matrix &A, bool reduced, Rep rep)
+	{
+		const size_t m = A.rowdim (), n = A.coldim ();
+		if (!m || !n)
+			return 0;
+		std::vector<uint16_t> D (m * n, 0);
+		size_t i = 0;
+		for (typename Matrix::RowIterator row = A.rowBegin (); row != A.rowEnd (); ++row, ++i)
+			if (!unpack_row (*row, &D[i * n], n, rep))
+				return cpu_echelonize (A, reduced);
+		size_t rank = 0;
+		if (reduced) {
+			const size_t nw = (n + 63) / 64;
+			std::vector<uint64_t> W (m * nw, 0);
+			for (size_t r = 0; r < m; ++r)
+				for (size_t j = 0; j < n; ++j)
+					W[r * nw + j / 64] |= (uint64_t) D[r * n + j] << (j % 64);
+			std::vector<uint32_t> pivots (std::min (m, n) + 1);
+			_ctx.M.dev.check (mlz_gf2_rref (_ctx.M.dev.ctx (), m, n, &W[0], nw, &rank, &pivots[0]));
+			for (size_t r = 0; r < m; ++r)
+				for (size_t j = 0; j < n; ++j)
+					D[r * n + j] = (uint16_t) ((W[r * nw + j / 64] >> (j % 64)) & 1);
+		} else {
+			uint64_t d = 1;
+			_ctx.M.dev.check (mlz_zp_echelonize_sparse (_ctx.M.dev.ctx (), MLZ_U16, 2, m, n, &D[0], n, MLZ_ECH_U,
+								    hip_counts_words (rep), &rank, &d, 0, 0));
+		}
+		i = 0;
+		for (typename Matrix::RowIterator row = A.rowBegin (); row != A.rowEnd (); ++row, ++i)
+			pack_row (*row, &D[i * n], n, rep);
+		return rank;
+	}
+
+	static int hip_counts_words (VectorRepresentationTypes::Sparse01) { return 0; }
+	static int hip_counts_words (VectorRepresentationTypes::Hybrid01) { return 1; }
+
+	template <class Matrix>
+	size_t std_echelonize (Matrix &A, bool reduced, VectorRepresentationTypes::Sparse01 r)
+		{ return std_sparse (A, reduced, r); }
+
+	template <class Matrix>
+	size_t std_echelonize (Matrix &A, bool reduced, VectorRepresentationTypes::Hybrid01 r)
+		{ return std_sparse (A, reduced, r); }
+
+	template <class Matrix, class Rep>
+	size_t std_echelonize (Matrix &A, bool reduced, Rep)
+		{ return cpu_echelonize (A, reduced); }
+
 	template <class Matrix>
 	size_t cpu_fl (Matrix &A, bool reduced)
 	{
@@ -804,7 +1039,7 @@ public:
 	{
 		size_t rank;
 		if (method == METHOD_STANDARD_GJ)
-			rank = cpu_echelonize (A, reduced);
+			rank = std_echelonize (A, reduced, typename VectorTraits<GF2, typename Matrix::Row>::RepresentationType ());
 		else if (method == METHOD_FAUGERE_LACHARTRE)
 			rank = cpu_fl (A, reduced);
 		else
@@ -824,9 +1059,27 @@ public:
 		else if (method != METHOD_STANDARD_GJ && method != METHOD_ASYMPTOTICALLY_FAST_GJ &&
 			 method != METHOD_M4RI)
 			throw LELAError ("Invalid method for choice of matrix");
-		else if (!reduced)
-			rank = cpu_echelonize (A, false);
-		else if (A.rowdim () && A.coldim ()) {
+		else if (!reduced) {
+			// Elimination::echelonize with DensePivotStrategy (echelon-form.h:113-131) over Z/2
+			const size_t m = A.rowdim (), n = A.coldim (), nw = hip_row_words (A);
+			if (m && n) {
+				const uint64_t *W = hip_words (A);
+				std::vector<uint16_t> D (m * n);
+				for (size_t r = 0; r < m; ++r)
+					for (size_t j = 0; j < n; ++j)
+						D[r * n + j] = (uint16_t) ((W[r * nw + j / 64] >> (j % 64)) & 1);
+				uint64_t d = 1;
+				_ctx.M.dev.check (mlz_zp_echelonize (_ctx.M.dev.ctx (), MLZ_U16, 2, m, n, &D[0], n, MLZ_ECH_U,
+								     &rank, &d, 0, 0, MLZ_ALGO_AUTO));
+				uint64_t *Wo = hip_words (A);
+				for (size_t r = 0; r < m; ++r)
+					for (size_t j = 0; j < n; ++j) {
+						const uint64_t bit = 1ull << (j % 64);
+						Wo[r * nw + j / 64] = D[r * n + j] ? (Wo[r * nw + j / 64] | bit)
+										   : (Wo[r * nw + j / 64] & ~bit);
+					}
+			}
+		} else if (A.rowdim () && A.coldim ()) {
 			std::vector<uint32_t> pivots (std::min (A.rowdim (), A.coldim ()) + 1);
 			_ctx.M.dev.check (mlz_gf2_rref (_ctx.M.dev.ctx (), A.rowdim (), A.coldim (), hip_words (A),
 							hip_row_words (A), &rank, &pivots[0]));

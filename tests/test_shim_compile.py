@@ -5,7 +5,7 @@ A translation unit that instantiates every hook -- gemm and trsm for
 Modular<uint16/uint32/double>, GF(2) gemm and trsm, EchelonForm (reduced, non-reduced,
 Faugere-Lachartre method, rank) for the three Z/p storages and GF(2),
 FaugereLachartre on sparse and dense rows, and Elimination (echelonize with
-and without L, pluq, the reduced form, move_L) on dense and sparse rows --
+and without L, with the sparse pivot strategy, pluq, the reduced form, move_L) on dense and sparse rows, GF(2) EchelonForm on dense, sparse and hybrid rows --
 goes through
 `g++ -std=gnu++98 -fsyntax-only` against LELA's own headers.  This is a
 syntax-only check: no object code is produced, nothing is linked or run, and
@@ -78,7 +78,10 @@ void zp_hooks (const Modular<E> &F)
 	elim.echelonize (A, P, rank, det, false);
 	elim.pluq (A, P, Q, rank, det);
 	elim.echelonize (X, P, rank, det, false);
+	elim.echelonize (X, P, rank, det, true);
+	elim.echelonize (X, P, rank, det, SparsePartialPivotStrategy<Modular<E>, HIPModule<Modular<E> > > (ctx), true);
 	elim.pluq (X, P, Q, rank, det);
+	EF.echelonize (X, false);
 	DenseMatrix<E> L (8, 8);
 	elim.echelonize_reduced (A, L, P, rank, det, true);
 	elim.move_L (L, A);
@@ -110,6 +113,10 @@ void gf2_hooks ()
 	SparseMatrix<bool> S (64, 64), T (64, 64);
 	FL.echelonize (T, S, rank, det, true);
 	EF.echelonize (X, true, EF_t::METHOD_FAUGERE_LACHARTRE);
+	EF.echelonize (X, true);
+	EF.echelonize (X, false);
+	EF.echelonize (S, true);
+	EF.echelonize (S, false);
 	(void) r;
 }
 
