@@ -141,9 +141,73 @@ __global__ __launch_bounds__(LU_NT) void lu_window_kernel(const uint32_t *__rest
 		}
 	}
 	const uint16_t *itab = reinterpret_cast<const uint16_t *>(big);
+	__shared__ int sfail;
+	if (tid == 0)
+		sfail = 0;
 	__syncthreads();
 	int cur = 0, stop = 0, stop_col = 0, bad = 0;
-	bool done = false;
+	bool done = false, fast = false;
+	// Fast path for a full window (LU_K rows, LU_W columns): no transposition
+	// happens while every step's diagonal entry is nonzero -- row s is then
+	// the first nonzero row of column s among positions >= s, and column s the
+	// first column with a nonzero -- so the panel is eliminated in 8-column
+	// blocks: the wave owning block k eliminates its columns in registers
+	// (cross-lane reads, no barrier between its steps), and after one barrier
+	// the waves to its right apply the block's multipliers.  A zero diagonal
+	// entry (or a pivot without an inverse) sends the panel to the
+	// column-by-column search below, from the original rows.
+	if (kw == LU_K && wc == LU_W) {
+		for (int k = 0; k < 8; ++k) {
+			if (__builtin_amdgcn_readfirstlane(w) == k) {
+#pragma unroll
+				for (int j = 0; j < 8; ++j) {
+					const int s = 8 * k + j;
+					const uint32_t xs = __builtin_amdgcn_readlane(x[j], s);
+					const uint32_t inv = xs == 0u ? 0u : (TAB ? (uint32_t) itab[xs] : mp.inv(xs));
+					if (inv == 0u) {
+						if (l == 0)
+							sfail = 1;
+						break;
+					}
+					const uint32_t f = l > s ? mul(x[j], inv) : 0u;
+					Lm[l * FP + s] = (FT) f;
+#pragma unroll
+					for (int jj = 0; jj < 8; ++jj)
+						if (jj > j)
+							x[jj] = upd(x[jj], f, __builtin_amdgcn_readlane(x[jj], s));
+					x[j] = l > s ? 0u : x[j];
+					if (l == 0) {
+						jl[s] = s;
+						sdinv[s] = inv;
+						sfirst[s] = s;
+						sx[s] = xs;
+					}
+				}
+			}
+			__syncthreads();
+			if (sfail)
+				break;
+			if (w > k) {
+#pragma unroll
+				for (int j = 0; j < 8; ++j) {
+					const int s = 8 * k + j;
+					const uint32_t f = (uint32_t) Lm[l * FP + s];
+#pragma unroll
+					for (int jj = 0; jj < 8; ++jj)
+						x[jj] = upd(x[jj], f, __builtin_amdgcn_readlane(x[jj], s));
+				}
+			}
+		}
+		if (sfail) {             // start over from the window as loaded
+#pragma unroll
+			for (int jj = 0; jj < 8; ++jj)
+				x[jj] = X[(size_t) (i0 + l) * ldx + c + 8 * w + jj];
+			__syncthreads();     // every wave is past its reads of the fast path's multipliers
+		} else {
+			cur = LU_K;
+			done = fast = true;
+		}
+	}
 	// The owner's search of column cc_ at step s: the event, the multipliers
 	// by position (after the exchange of positions s and first) and the step
 	// records, then the column's flag (a wave's LDS operations complete in order).
@@ -278,8 +342,8 @@ __global__ __launch_bounds__(LU_NT) void lu_window_kernel(const uint32_t *__rest
 	if (ready[0] == -1)
 		bad = 2;
 	// The multipliers of step s were stored by position at step s; the
-	// transpositions of later steps move two of them each.
-	if (tid < rp)
+	// transpositions of later steps move two of them each (the fast path makes none).
+	if (!fast && tid < rp)
 		for (int t = tid + 1; t < rp; ++t) {
 			const int ft = sfirst[t];
 			if (ft != t) {
