@@ -296,7 +296,53 @@ def bench_batch(la, synth, np):
     out["cu_us"]["best"] = float(np.nansum(np.nanmin(cost, axis=1)))
     out["model_gain_over_table"] = out["cu_us"]["table"] / out["cu_us"]["model"]
     ev.free()
+    out["mixed_density"] = bench_mixed_density(ctx, F, la, synth, np, lb)
     ctx.close()
+    return out
+
+
+def bench_mixed_density(ctx, F, la, synth, np, lb):
+    """Where the choice matters: 2048 gemm instances (m, n, k in [64, 1024])
+    whose A holds 0.05% to 20% nonzeros, log-uniform.  Below about 0.5% the
+    sparse-A variant is the fastest, above it INT8 MFMA; the table keys on
+    shape only, the model also sees the density.  Both are fitted on a
+    separate 1024-instance batch; then each policy's plan runs as one captured
+    graph (wall time) and the measured cost of its choices is summed."""
+    def workload(seed, count):
+        raw = synth.gemm_density_instances(seed, count, p=P)
+        return lb.Workload(ctx, F, [lb.Instance(1, d["A"], d["B"], d["C"], density=d["density"]) for d in raw])
+
+    train = workload(synth.seed_for(5, 3), 1024)
+    lb.fit(ctx, F, train.tasks, reps=3, steps=3000)
+    train.free()
+    ev = workload(synth.seed_for(5, 4), 2048)
+    out = {"instances": ev.n, "density": [0.0005, 0.2]}
+    chosen = {}
+    for name, policy in (("model", lb.POLICY_MODEL), ("table", lb.POLICY_TABLE)):
+        ev.set_variants([-1] * ev.n)
+        plan = lb.Plan(ctx, F, ev.tasks, policy, graph=True)
+        chosen[name] = plan.variants()
+        ts = []
+        for r in range(6):
+            ev.upload()
+            ctx.synchronize()
+            ctx.timer_start()
+            plan.launch()
+            ms = ctx.timer_stop()
+            plan.finish()
+            if r:
+                ts.append(ms)
+        plan.close()
+        out[f"{name}_ms"] = float(np.median(ts))
+    ev.upload()
+    cost = lb.tune(ctx, F, ev.tasks, reps=2)
+    idx = np.arange(ev.n)
+    out["cu_us"] = {name: float(np.nansum(cost[idx, v])) for name, v in chosen.items()}
+    out["cu_us"]["best"] = float(np.nansum(np.nanmin(cost, axis=1)))
+    out["model_gain_over_table"] = out["cu_us"]["table"] / out["cu_us"]["model"]
+    out["model_wall_gain_over_table"] = out["table_ms"] / out["model_ms"]
+    out["sparse_a_chosen"] = {name: int(np.sum(np.asarray(v) == lb.GEMM_SPARSE)) for name, v in chosen.items()}
+    ev.free()
     return out
 
 

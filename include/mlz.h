@@ -361,7 +361,8 @@ int mlz_tune_table_set(mlz_ctx *ctx, int op, const int8_t *table);
  * Weights: W1[5][32] b1[32] W2[32][32] b2[32] W3[32][4] b3[4] (1380 floats). */
 #define MLZ_MODEL_PARAMS 1380
 /* Features of a task: (log2 m - 7.5)/1.5, (log2 n - 7.5)/1.5, (log2 k' - 7.5)/1.5
- * with k' = k for gemm and min(m, n) for rref, 2*density - 1, (log2 p - 16)/8. */
+ * with k' = k for gemm and min(m, n) for rref, (log2 max(density, 2^-16) + 8)/8,
+ * (log2 p - 16)/8. */
 int mlz_task_features(const mlz_task *task, uint64_t p, float *x5);
 /* Trains the op's model on the GPU (full-batch Adam, `steps` steps, learning
  * rate lr, weights initialised from seed) on x [n][5], y [n][MLZ_MAX_VARIANTS]

@@ -139,7 +139,8 @@ void task_features(const mlz_task &t, uint64_t p, float *x)
 	x[0] = (float) ((std::log2((double) t.m) - 7.5) / 1.5);
 	x[1] = (float) ((std::log2((double) t.n) - 7.5) / 1.5);
 	x[2] = (float) ((std::log2(kk) - 7.5) / 1.5);
-	x[3] = 2.0f * t.density - 1.0f;
+	// log2 of the density: the sparse-A gemm wins below about 2^-8 of nonzeros
+	x[3] = (float) ((std::log2(std::max((double) t.density, 1.0 / 65536.0)) + 8.0) / 8.0);
 	x[4] = (float) ((std::log2((double) p) - 16.0) / 8.0);
 }
 

@@ -163,3 +163,19 @@ def batch_instances(seed, count=4096, p=65521, lo=32, hi=1024, dtype=np.uint16, 
         else:
             out.append(dict(op=2, m=m, n=n, k=min(m, n), density=d, A=mat(m, n)))
     return out
+
+
+def gemm_density_instances(seed, count, p=65521, lo=64, hi=1024, dmin=0.0005, dmax=0.2, dtype=np.uint16):
+    """Gemm instances (as dicts, the form of batch_instances) whose A has a
+    density log-uniform in [dmin, dmax]; B and C dense uniform residues,
+    m, n, k uniform in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        m, n, k = (int(v) for v in rng.integers(lo, hi + 1, size=3))
+        d = float(np.exp(rng.uniform(np.log(dmin), np.log(dmax))))
+        A = np.where(rng.random((m, k)) < d, rng.integers(1, p, size=(m, k), dtype=np.int64), 0).astype(dtype)
+        out.append(dict(op=1, m=m, n=n, k=k, density=d, A=A,
+                        B=rng.integers(0, p, size=(k, n), dtype=np.int64).astype(dtype),
+                        C=rng.integers(0, p, size=(m, n), dtype=np.int64).astype(dtype)))
+    return out
