@@ -213,6 +213,14 @@ def bench_fl(la, synth, np):
         out[f"{name}_s"] = t
         out["rank"] = int(res["r"][0])
     out["speedup"] = out["plain_s"] / out["mfma_s"]
+    # the compact result (pivots + the rank x (n - rank) block on the
+    # non-pivot columns, what LELA's sparse R holds) instead of a dense R
+    dNP = la.DeviceMatrix(ctx, shape[0], shape[1] - min(shape), np.uint32)
+    res = {}
+    t = time_steps(ctx, lambda: res.update(r=FL.echelonize_compact((drp, dc, dv), shape=shape, NP=dNP)), 2, 1)
+    out["compact_s"] = t
+    out["compact_rank"] = int(res["r"][0])
+    dNP.free()
     # algorithmic modular MACs: sparse A^-1 B, Schur product, RREF of D, back-substitution
     W = 2.25e6 * 15000 + 5000.0 * 15000 * 15000 + 5000.0 * 5000 * 12500 + 15000.0 * 5000 * 10000
     out["roofline"] = mfma_roofline(W, out["mfma_s"])

@@ -233,6 +233,21 @@ int mlz_zp_fl_echelonize(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size
 			 size_t ldr, size_t *rank, uint32_t *pivots, uint64_t *det, size_t *bad_row,
 			 int algo);
 
+/* The same reduction with a compact result: pivots (rank column indices)
+ * and NP, the rank rows of R on the n - rank non-pivot columns in increasing
+ * column order, row-major with row stride n - rank.  R is the identity on the
+ * pivot columns, so NP and the pivots determine it; NP is the block LELA's
+ * splicer assembles R from (lela/algorithms/faugere-lachartre.tcc:363-504,
+ * MatrixGrid3) and, for sparse rows, the nonzeros R holds.  At config 4
+ * (20000 x 30000, rank 20000) it is a third of the dense R.  np_capacity
+ * counts elements; a rank * (n - rank) above it returns MLZ_E_ARG with *rank
+ * set (min(m, n) * n always suffices).  Other arguments, outputs and errors
+ * as mlz_zp_fl_echelonize. */
+int mlz_zp_fl_echelonize_compact(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t n,
+				 const uint64_t *row_ptr, const uint32_t *col, const void *val, void *NP,
+				 size_t np_capacity, size_t *rank, uint32_t *pivots, uint64_t *det,
+				 size_t *bad_row, int algo);
+
 /* ---- GF(2) ------------------------------------------------------------- */
 
 /* C <- a*A*B + b*C over GF(2) (a, b in {0, 1}); A m x k, B k x n, C m x n bits,

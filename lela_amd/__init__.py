@@ -513,6 +513,48 @@ class FaugereLachartre:
                 ctx.set_ptr_mode(PTR_HOST)
         return self._call(m, n, row_ptr.ctypes.data, col.ctypes.data, val.ctypes.data, pr, ldr, algo)
 
+    def echelonize_compact(self, X, algo=ALGO_AUTO, shape=None, NP=None):
+        """The reduced form of X in compact form (mlz_zp_fl_echelonize_compact):
+        returns (rank, det, pivots, NP) where NP (rank x (n - rank)) holds the
+        rows of R on the non-pivot columns in increasing order -- R is the
+        identity on the pivot columns.  X as for echelonize (host arrays), or
+        device CSR triples with NP a DeviceMatrix of at least rank x (n - rank)
+        entries (then NP's buffer is returned as filled, row stride n - rank)."""
+        ctx, F = self.ctx, self.ctx.F
+        if isinstance(X, tuple):
+            row_ptr, col, val = X
+            m, n = shape
+        else:
+            m, n = X.shape
+            row_ptr, col, val = dense_to_csr(X)
+        rank = ctypes.c_size_t(0)
+        det = ctypes.c_uint64(0)
+        bad = ctypes.c_size_t(0)
+        piv = np.zeros(max(1, min(m, n)), dtype=np.uint32)
+        if isinstance(row_ptr, DeviceMatrix):
+            if ctx._ptr_mode != PTR_DEVICE:
+                ctx.set_ptr_mode(PTR_DEVICE)
+            args = (row_ptr.ptr, col.ptr, val.ptr, NP.ptr, NP.rows * NP.cols)
+            out = NP
+        else:
+            if ctx._ptr_mode != PTR_HOST:
+                ctx.set_ptr_mode(PTR_HOST)
+            row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint64)
+            col = np.ascontiguousarray(col, dtype=np.uint32)
+            val = np.ascontiguousarray(val, dtype=F.dtype)
+            cap = max(1, min(m, n) * n)        # rank * (n - rank) never exceeds it
+            out = np.zeros(cap, dtype=F.dtype)
+            args = (row_ptr.ctypes.data, col.ctypes.data, val.ctypes.data, out.ctypes.data, cap)
+        rc = ctx.lib.mlz_zp_fl_echelonize_compact(ctx.handle, F.elem, F.p, m, n, args[0], args[1], args[2],
+                                                  args[3], args[4], ctypes.byref(rank), piv.ctypes.data,
+                                                  ctypes.byref(det), ctypes.byref(bad), algo)
+        ctx.check(rc, "fl_echelonize_compact", idx=bad.value if rc == _lib.MLZ_E_WRONG_FORM else None)
+        r = rank.value
+        self.pivots = piv[:r].copy()
+        if isinstance(out, np.ndarray):
+            out = out[:r * (n - r)].reshape(r, n - r).copy()
+        return r, det.value, self.pivots, out
+
     def _gf2(self, R, X, shape):
         """FaugereLachartre<GF2> on hybrid rows: X is a Dense01Matrix or a
         triple (row_ptr, blk, words) with `shape` (lela/vector/hybrid.h:30-33);

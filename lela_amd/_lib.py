@@ -75,6 +75,9 @@ SIGNATURES = {
     "mlz_zp_fl_echelonize": (_c_int, [_ctx, _c_int, _c_u64, _c_size, _c_size, _vp, _vp, _vp, _vp,
                                       _c_size, ctypes.POINTER(_c_size), _vp, ctypes.POINTER(_c_u64),
                                       ctypes.POINTER(_c_size), _c_int]),
+    "mlz_zp_fl_echelonize_compact": (_c_int, [_ctx, _c_int, _c_u64, _c_size, _c_size, _vp, _vp, _vp, _vp,
+                                              _c_size, ctypes.POINTER(_c_size), _vp, ctypes.POINTER(_c_u64),
+                                              ctypes.POINTER(_c_size), _c_int]),
     "mlz_batch_plan": (_c_int, [_ctx, _c_int, _c_u64, _vp, _c_size, _c_int, _c_int, ctypes.POINTER(_vp)]),
     "mlz_batch_launch": (_c_int, [_ctx, _vp]),
     "mlz_batch_finish": (_c_int, [_ctx, _vp, _vp, _vp, ctypes.POINTER(_c_size)]),

@@ -196,6 +196,30 @@ __global__ void fl_assemble_kernel(const uint32_t *__restrict__ B, const uint32_
 	}
 }
 
+// The compact result: row t of R on the non-pivot columns (B/D columns
+// bdcol[0 .. nf)), rows 0 .. rank-1, four columns per thread.
+template <int E>
+__global__ void fl_compact_kernel(const uint32_t *__restrict__ B, const uint32_t *__restrict__ D, int nr,
+				  const int *__restrict__ bdcol, int nf, const int *__restrict__ src, int rank, uint32_t p,
+				  typename Elem<E>::T *__restrict__ NP, size_t ldnp)
+{
+	const int f0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+	if (f0 >= nf)
+		return;
+	int cb[4];
+#pragma unroll
+	for (int q = 0; q < 4; ++q)
+		cb[q] = f0 + q < nf ? bdcol[f0 + q] : 0;
+	for (int t = blockIdx.y; t < rank; t += gridDim.y) {
+		const int s = src[t];
+		const uint32_t *row = s >= 0 ? B + (size_t) s * nr : D + (size_t) (-1 - s) * nr;
+#pragma unroll
+		for (int q = 0; q < 4; ++q)
+			if (f0 + q < nf)
+				Elem<E>::store(NP + (size_t) t * ldnp + f0 + q, row[cb[q]], p);
+	}
+}
+
 // Row groups of the assembly launch: each workgroup walks m / FL_ASM_ROWS
 // output rows of its 1024 columns (fewer, longer workgroups than one per row).
 constexpr size_t FL_ASM_ROWS = 1024;
@@ -207,6 +231,7 @@ struct FlResult {
 	size_t rank;
 	uint64_t det;
 	size_t bad_row;
+	size_t nf;          // n - rank: columns of the compact block
 };
 
 // D's echelon rows 0 .. rd-1 split by a column map: cmap[j] = s >= 0 puts
@@ -269,7 +294,8 @@ static int fl_reduce_d(mlz_ctx *ctx, uint32_t p, size_t rd, size_t nr, uint32_t 
 
 template <int E>
 static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *d_rowptr, const uint32_t *d_col,
-		  const void *d_val, void *R, size_t ldr, uint32_t *pivots_out, FlResult *res, int algo, bool want_det)
+		  const void *d_val, void *R, size_t ldr, uint32_t *pivots_out, FlResult *res, int algo, bool want_det,
+		  void *NP = nullptr, size_t np_cap = 0, void **np_dev = nullptr)
 {
 	typedef typename Elem<E>::T T;
 	const ModP mp = make_modp(p);
@@ -491,6 +517,38 @@ static int fl_run(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, const uint64_t *
 										  (int) rank, (int) m, (int) n, p, (T *) R, ldr);
 		MLZ_TRY(check_launch(ctx, "fl_assemble_kernel"));
 	}
+	res->nf = n - rank;
+	if (NP || np_dev) {
+		// the non-pivot columns in B/D's numbering: all but D's pivot columns
+		const size_t nf = n - rank;
+		if (rank * nf > np_cap) {
+			MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
+			res->rank = rank;
+			return set_error(ctx, MLZ_E_ARG, "the compact block needs %zu x %zu entries, %zu given", rank, nf,
+					 np_cap);
+		}
+		std::vector<int> bdcol;
+		bdcol.reserve(nf + 1);
+		for (size_t b = 0, q = 0; b < nr; ++b) {
+			if (q < rd && jd[q] == b)
+				++q;
+			else
+				bdcol.push_back((int) b);
+		}
+		if (np_dev)
+			MLZ_TRY(ws_get(ctx, WS_BATCH_1, rank * nf * sizeof(T) + 256, np_dev));
+		T *dst = np_dev ? (T *) *np_dev : (T *) NP;
+		void *wb;
+		MLZ_TRY(ws_get(ctx, WS_FL_6, (nf + 1) * sizeof(int), &wb));
+		MLZ_HIP(ctx, hipMemcpyAsync(wb, bdcol.data(), nf * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+		if (nf && rank) {
+			ProfScope ps(ctx, "fl_compact");
+			const dim3 grid(ceil_div(ceil_div(nf, 4), 256), (unsigned) std::min<size_t>(rank, FL_ASM_ROWS));
+			fl_compact_kernel<E><<<grid, 256, 0, ctx->stream>>>(dB, dD, (int) nr, (const int *) wb, (int) nf, d_src,
+									   (int) rank, p, dst, nf);
+			MLZ_TRY(check_launch(ctx, "fl_compact_kernel"));
+		}
+	}
 	MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
 	if (pivots_out)
 		for (size_t t = 0; t < rank; ++t)
@@ -518,7 +576,7 @@ extern "C" int mlz_zp_fl_echelonize(mlz_ctx *ctx, mlz_elem elem, uint64_t p, siz
 	if (m > (size_t) INT32_MAX / 2 || n > (size_t) INT32_MAX / 2)
 		return set_error(ctx, MLZ_E_ARG, "dimension too large");
 	MLZ_HIP(ctx, hipSetDevice(ctx->device));
-	FlResult res = {0, 1 % p, 0};
+	FlResult res = {0, 1 % p, 0, 0};
 	int rc;
 	const size_t es = elem_size(elem);
 	if (m == 0 || n == 0) {
@@ -575,5 +633,81 @@ extern "C" int mlz_zp_fl_echelonize(mlz_ctx *ctx, mlz_elem elem, uint64_t p, siz
 		return rc;
 	if (ctx->ptr_mode != MLZ_PTR_DEVICE && R)
 		return stage_out(ctx, R, dR, m, n, ldr, es);
+	return MLZ_OK;
+}
+
+extern "C" int mlz_zp_fl_echelonize_compact(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size_t n,
+					    const uint64_t *row_ptr, const uint32_t *col, const void *val, void *NP,
+					    size_t np_capacity, size_t *rank, uint32_t *pivots, uint64_t *det,
+					    size_t *bad_row, int algo)
+{
+	if (!ctx)
+		return MLZ_E_ARG;
+	if (p < 2 || (elem == MLZ_U16 && p > 65535) || (elem == MLZ_U32 && p > 0xffffffffull) ||
+	    (elem == MLZ_F64_SYM && p >= 67108864ull) || elem_size(elem) == 0)
+		return set_error(ctx, MLZ_E_ARG, "bad modulus or element type");
+	if (!row_ptr || !NP)
+		return set_error(ctx, MLZ_E_ARG, "bad CSR or output pointer");
+	if (m > (size_t) INT32_MAX / 2 || n > (size_t) INT32_MAX / 2)
+		return set_error(ctx, MLZ_E_ARG, "dimension too large");
+	MLZ_HIP(ctx, hipSetDevice(ctx->device));
+	FlResult res = {0, 1 % p, 0, 0};
+	if (m == 0 || n == 0) {
+		if (rank) *rank = 0;
+		if (det) *det = 1 % p;
+		return MLZ_OK;
+	}
+	const size_t es = elem_size(elem);
+	const uint64_t *d_rowptr;
+	const uint32_t *d_col;
+	const void *d_val;
+	const bool dev = ctx->ptr_mode == MLZ_PTR_DEVICE;
+	if (dev) {
+		d_rowptr = row_ptr;
+		d_col = col;
+		d_val = val;
+	} else {
+		const uint64_t nnz = row_ptr[m];
+		for (size_t i = 0; i < m; ++i) {
+			if (row_ptr[i + 1] < row_ptr[i])
+				return set_error(ctx, MLZ_E_ARG, "row_ptr not monotone at row %zu", i);
+			for (uint64_t e = row_ptr[i]; e < row_ptr[i + 1]; ++e)
+				if (col[e] >= n || (e > row_ptr[i] && col[e] <= col[e - 1]))
+					return set_error(ctx, MLZ_E_ARG, "row %zu: columns not sorted or out of range", i);
+		}
+		void *a, *b, *c;
+		MLZ_TRY(ws_get(ctx, WS_STAGE_A, (m + 1) * 8, &a));
+		MLZ_TRY(ws_get(ctx, WS_STAGE_B, nnz * 4 + 8, &b));
+		MLZ_TRY(ws_get(ctx, WS_STAGE_C, nnz * es + 8, &c));
+		MLZ_HIP(ctx, hipMemcpyAsync(a, row_ptr, (m + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
+		if (nnz) {
+			MLZ_HIP(ctx, hipMemcpyAsync(b, col, nnz * 4, hipMemcpyHostToDevice, ctx->stream));
+			MLZ_HIP(ctx, hipMemcpyAsync(c, val, nnz * es, hipMemcpyHostToDevice, ctx->stream));
+		}
+		d_rowptr = (const uint64_t *) a;
+		d_col = (const uint32_t *) b;
+		d_val = c;
+	}
+	const uint32_t pp = (uint32_t) p;
+	void *npd = nullptr;
+	void **np_dev = dev ? nullptr : &npd;
+	void *np = dev ? NP : nullptr;
+	int rc;
+	if (elem == MLZ_U16)
+		rc = fl_run<MLZ_U16>(ctx, pp, m, n, d_rowptr, d_col, d_val, nullptr, 0, pivots, &res, algo, det != nullptr,
+				     np, np_capacity, np_dev);
+	else if (elem == MLZ_U32)
+		rc = fl_run<MLZ_U32>(ctx, pp, m, n, d_rowptr, d_col, d_val, nullptr, 0, pivots, &res, algo, det != nullptr,
+				     np, np_capacity, np_dev);
+	else
+		rc = fl_run<MLZ_F64_SYM>(ctx, pp, m, n, d_rowptr, d_col, d_val, nullptr, 0, pivots, &res, algo,
+					 det != nullptr, np, np_capacity, np_dev);
+	if (rank) *rank = res.rank;
+	if (det) *det = res.det;
+	if (bad_row) *bad_row = res.bad_row;
+	if (rc != MLZ_OK)
+		return rc;
+	if (!dev && res.rank * res.nf)
+		MLZ_HIP(ctx, hipMemcpy(NP, npd, res.rank * res.nf * es, hipMemcpyDeviceToHost));
 	return MLZ_OK;
 }

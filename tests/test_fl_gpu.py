@@ -4,6 +4,8 @@ comes from the heads of D's non-reduced echelon form),
 the reference test's own assertion that FL equals Elimination's reduced form
 (test-faugere-lachartre.C:227-243), WrongMatrixForm, and the 20000 x 30000
 workload through size-independent checks."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -205,3 +207,41 @@ def test_fl_banded_inverse_solve(gpu_ctx):
     got, rank, det, piv = run_fl(gpu_ctx, p, "uint32", X)
     rc, want, wrank, wdet, _ = orc.fl_echelonize(p, X)
     assert rc == 0 and rank == wrank and np.array_equal(got, want) and det == wdet
+
+
+@pytest.mark.parametrize("storage,p", [("uint32", 65521), ("uint16", 65521), ("double", 67108859),
+                                       ("uint32", 4294967291)])
+def test_fl_compact_output(gpu_ctx, storage, p):
+    """mlz_zp_fl_echelonize_compact: the pivots, det and the rows of R on the
+    non-pivot columns equal the dense reduced form's (the oracle's), for
+    shapes with and without a D part and with zero columns."""
+    F = la.Modular(p, storage)
+    gpu_ctx.F = F
+    for seed, kw in enumerate([dict(na=150, nc=60, nb=120, band=40, nnz_a=8, density=0.05),
+                               dict(na=300, nc=0, nb=80, band=30, nnz_a=5, density=0.1),
+                               dict(na=64, nc=200, nb=300, band=20, nnz_a=4, density=0.03)]):
+        csr, shape = synth.fl_matrix(40 + seed, p, **kw)
+        X = synth.csr_to_dense(csr, shape)
+        rc, want, wrank, wdet, _ = orc.fl_echelonize(p, X.astype(np.uint32))
+        assert rc == 0
+        FL = la.FaugereLachartre(gpu_ctx)
+        rank, det, piv, NP = FL.echelonize_compact((csr[0], csr[1], F.from_canonical(csr[2])), shape=shape)
+        free = np.setdiff1d(np.arange(shape[1]), piv)
+        assert rank == wrank and det == wdet
+        assert np.array_equal(want[np.arange(rank)[:, None], piv[None, :]], np.eye(rank, dtype=np.uint32))
+        assert np.array_equal(F.to_canonical(NP), want[:rank][:, free]), (seed, storage)
+
+
+def test_fl_compact_capacity(gpu_ctx):
+    p = 65521
+    F = la.Modular(p, "uint32")
+    gpu_ctx.F = F
+    csr, shape = synth.fl_matrix(7, p, na=100, nc=30, nb=60, band=20, nnz_a=5, density=0.05)
+    m, n = shape
+    NP = np.zeros(10, dtype=np.uint32)
+    rank = ctypes.c_size_t(0)
+    rc = gpu_ctx.lib.mlz_zp_fl_echelonize_compact(
+        gpu_ctx.handle, F.elem, p, m, n, np.ascontiguousarray(csr[0]).ctypes.data,
+        np.ascontiguousarray(csr[1]).ctypes.data, np.ascontiguousarray(csr[2], dtype=np.uint32).ctypes.data,
+        NP.ctypes.data, NP.size, ctypes.byref(rank), None, None, None, 0)
+    assert rc == 1 and rank.value * (n - rank.value) > NP.size and "compact block" in gpu_ctx.last_error()
