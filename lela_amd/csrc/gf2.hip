@@ -391,6 +391,188 @@ __device__ int gf2_reg_eliminate(uint64_t (&v)[RPT][WPR], bool (&alive)[RPT], in
 	return cnt;
 }
 
+constexpr int G2_IW = G2_CHUNK / 64;   // identity words per row (fast path)
+constexpr int G2_SCAN = 8;            // 256-row groups per pass of the live-row scan
+
+__device__ inline uint64_t g2_readlane64(uint64_t x, int l)
+{
+	const uint32_t lo = (uint32_t) __builtin_amdgcn_readlane((int) (uint32_t) x, l);
+	const uint32_t hi = (uint32_t) __builtin_amdgcn_readlane((int) (uint32_t) (x >> 32), l);
+	return ((uint64_t) hi << 32) | lo;
+}
+
+// Gauss-Jordan on the fast path's G2_CHUNK rows, eight columns at a time
+// (the method of four Russians applied to elimination).  Per strip of eight
+// columns: one wave eliminates the rows' 8-bit strip patterns alone (lane l
+// holds rows 8l .. 8l+7 as the bytes of one word; a row operation on all
+// eight is one multiply and XOR), choosing as gf2_reg_eliminate does the
+// lowest live row with the column's bit, and records for every row the
+// combination of the strip's pivot rows -- as they stood at the strip's
+// start -- that its operations amount to; the 2^k combinations of the k
+// pivot rows (panel and carried words) are tabulated in LDS and every row
+// XORs its entry once.  Four barriers per strip instead of two per column.
+// The result is gf2_reg_eliminate's with reduced = true: every row is
+// cleared in every pivot column, and the same pivot rows are chosen.
+__device__ int gf2_strip_eliminate(uint64_t (&v)[G2_SRPT][G2P], bool (&alive)[G2_SRPT], int nrows, int nsearch,
+				   int *out_rows, int *out_cols, uint64_t (&x)[G2_SRPT][G2_IW])
+{
+	constexpr int RW = G2P + G2_IW;   // words of a row: panel, then carried
+	static_assert(G2_CHUNK == 512 && G2_SRPT == 2, "one wave holds the strip patterns of 512 rows");
+	__shared__ __align__(8) uint8_t sb[G2_CHUNK], sa[G2_CHUNK], sc[G2_CHUNK], sp[G2_CHUNK];
+	__shared__ uint64_t prow[8][RW];
+	__shared__ uint64_t T[2][16][RW];   // combinations of pivots 0-3 and of 4-7
+	__shared__ int sk;
+	__shared__ unsigned long long s_or[G2P];
+	const int tid = threadIdx.x, lane = tid & 63;
+	if (tid < G2P)
+		s_or[tid] = 0ull;
+	__syncthreads();
+	{
+		uint64_t orv[G2P] = {0ull, 0ull, 0ull, 0ull};
+#pragma unroll
+		for (int q = 0; q < G2_SRPT; ++q)
+			if (tid + 256 * q < nrows && alive[q])
+#pragma unroll
+				for (int w = 0; w < G2P; ++w)
+					orv[w] |= v[q][w];
+#pragma unroll
+		for (int w = 0; w < G2P; ++w) {
+			uint32_t lo = (uint32_t) orv[w], hi = (uint32_t) (orv[w] >> 32);
+#pragma unroll
+			for (int off = 32; off >= 1; off >>= 1) {
+				lo |= (uint32_t) __shfl_xor((int) lo, off, 64);
+				hi |= (uint32_t) __shfl_xor((int) hi, off, 64);
+			}
+			if (lane == 0 && (lo | hi))
+				atomicOr(&s_or[w], ((unsigned long long) hi << 32) | lo);
+		}
+	}
+	__syncthreads();
+	uint64_t live_cols[G2P];
+#pragma unroll
+	for (int w = 0; w < G2P; ++w)
+		live_cols[w] = s_or[w];
+	int cnt = 0;
+	for (int c0 = 0; c0 < nsearch; c0 += 8) {
+		const int w = c0 >> 6, sh = c0 & 63;
+		const int nbits = min(8, nsearch - c0);
+		const uint64_t smask = (1ull << nbits) - 1;
+		uint64_t lcw = 0;
+#pragma unroll
+		for (int ww = 0; ww < G2P; ++ww)
+			lcw = ww == w ? live_cols[ww] : lcw;
+		if (((lcw >> sh) & smask) == 0)
+			continue;                       // no live bit: no pivot in the strip
+		// 1. the rows' strip patterns and live flags
+#pragma unroll
+		for (int q = 0; q < G2_SRPT; ++q) {
+			const int r = tid + 256 * q;
+			uint64_t vw = 0;
+#pragma unroll
+			for (int ww = 0; ww < G2P; ++ww)
+				vw = ww == w ? v[q][ww] : vw;
+			sb[r] = r < nrows ? (uint8_t) ((vw >> sh) & smask) : 0;
+			sa[r] = (r < nrows && alive[q]) ? 1 : 0;
+		}
+		__syncthreads();
+		// 2. one wave eliminates the patterns; each word of eight rows is
+		// handled as two 32-bit halves (four rows each)
+		if (tid < 64) {
+			const uint2 b2 = reinterpret_cast<const uint2 *>(sb)[lane];
+			const uint2 l2 = reinterpret_cast<const uint2 *>(sa)[lane];
+			uint32_t Bl = b2.x, Bh = b2.y, Ll = l2.x, Lh = l2.y, Cl = 0, Ch = 0, Pl = 0, Ph = 0;
+			int k = 0;
+			for (int j = 0; j < nbits; ++j) {
+				const uint32_t ml = (Bl >> j) & 0x01010101u, mh = (Bh >> j) & 0x01010101u;
+				const uint32_t al = ml & Ll, ah = mh & Lh;
+				const uint64_t bal = __ballot((al | ah) != 0);
+				if (bal == 0)
+					continue;
+				const int pl = __ffsll((long long) bal) - 1;
+				// this lane's lowest candidate (8 * row in lane), its pattern and combination
+				const int mine = al ? __ffs(al) - 1 : 32 + __ffs(ah) - 1;
+				const uint32_t mb = mine < 32 ? (Bl >> mine) & 0xffu : (Bh >> (mine - 32)) & 0xffu;
+				const uint32_t mc = mine < 32 ? (Cl >> mine) & 0xffu : (Ch >> (mine - 32)) & 0xffu;
+				const int pb = __builtin_amdgcn_readlane(mine, pl);
+				const uint32_t pbyte = (uint32_t) __builtin_amdgcn_readlane((int) mb, pl);
+				const uint32_t pc = (uint32_t) __builtin_amdgcn_readlane((int) mc, pl) ^ (1u << k);
+				uint32_t xl = ml, xh = mh;
+				if (lane == pl) {
+					if (pb < 32) {
+						xl &= ~(0xffu << pb);
+						Ll &= ~(0xffu << pb);
+						Pl |= (uint32_t) (k + 1) << pb;
+					} else {
+						xh &= ~(0xffu << (pb - 32));
+						Lh &= ~(0xffu << (pb - 32));
+						Ph |= (uint32_t) (k + 1) << (pb - 32);
+					}
+				}
+				Bl ^= xl * pbyte;       // one byte per row: no carries
+				Bh ^= xh * pbyte;
+				Cl ^= xl * pc;
+				Ch ^= xh * pc;
+				if (lane == 0) {
+					out_rows[cnt + k] = 8 * pl + (pb >> 3);
+					out_cols[cnt + k] = c0 + j;
+				}
+				++k;
+			}
+			reinterpret_cast<uint2 *>(sc)[lane] = make_uint2(Cl, Ch);
+			reinterpret_cast<uint2 *>(sp)[lane] = make_uint2(Pl, Ph);   // pivot slot + 1, 0 for other rows
+			if (lane == 0)
+				sk = k;
+		}
+		__syncthreads();
+		const int k = sk;
+		// 3. the strip's pivot rows as they stood, then their combinations
+#pragma unroll
+		for (int q = 0; q < G2_SRPT; ++q) {
+			const int r = tid + 256 * q;
+			const int t = (int) sp[r] - 1;
+			if (t >= 0) {
+#pragma unroll
+				for (int ww = 0; ww < G2P; ++ww)
+					prow[t][ww] = v[q][ww];
+#pragma unroll
+				for (int ww = 0; ww < G2_IW; ++ww)
+					prow[t][G2P + ww] = x[q][ww];
+				alive[q] = false;
+			}
+		}
+		__syncthreads();
+		// two tables of 16: a combination is the XOR of one entry of each
+		for (int idx = tid; idx < 32 * RW; idx += blockDim.x) {
+			const int hb = idx / (16 * RW), e = (idx / RW) % 16, ww = idx % RW;
+			uint64_t acc = 0ull;
+#pragma unroll
+			for (int t = 0; t < 4; ++t)
+				if (((e >> t) & 1) && 4 * hb + t < k)
+					acc ^= prow[4 * hb + t][ww];
+			T[hb][e][ww] = acc;
+		}
+		__syncthreads();
+		// 4. every row applies its combination
+#pragma unroll
+		for (int q = 0; q < G2_SRPT; ++q) {
+			const int r = tid + 256 * q;
+			const int cmb = sc[r];
+			if (cmb) {
+				const int lo = cmb & 15, hi = cmb >> 4;
+#pragma unroll
+				for (int ww = 0; ww < G2P; ++ww)
+					v[q][ww] ^= T[0][lo][ww] ^ T[1][hi][ww];
+#pragma unroll
+				for (int ww = 0; ww < G2_IW; ++ww)
+					x[q][ww] ^= T[0][lo][G2P + ww] ^ T[1][hi][G2P + ww];
+			}
+		}
+		cnt += k;
+	}
+	__syncthreads();
+	return cnt;
+}
+
 __device__ inline int g2_nodes_at(int nodes0, int level)
 {
 	int n = nodes0;
@@ -405,37 +587,51 @@ __device__ inline int g2_nodes_at(int nodes0, int level)
 // elimination: pivot rows only ever receive other pivot rows, so the
 // identity part of the pivot row of column J[t] has bits on pivot rows only,
 // and bit s of it is entry (t, pos of row s) of P_J^-1.
-constexpr int G2_IW = G2_CHUNK / 64;   // identity words per row
 __global__ __launch_bounds__(256) void gf2_fast_kernel(const uint64_t *__restrict__ A, size_t lda, int m, int cw,
 						       int nw, int wb, Gf2State st, uint64_t *__restrict__ Pinv)
 {
 	__shared__ int rowid[G2_CHUNK], outr[G2_PB], outc[G2_PB], pos[G2_CHUNK];
-	__shared__ uint64_t pivot[G2P + G2_IW];
-	__shared__ int s_min[2], s_cnt, s_wave[4];
+	__shared__ int s_cnt, s_scan[G2_SCAN][4];
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	if (tid == 0)
 		s_cnt = 0;
 	__syncthreads();
+	// the first G2_CHUNK live rows, G2_SCAN groups of 256 rows per pass (their
+	// flags loaded together, one prefix count per pass)
 	int scanned_all = 1;
-	for (int base = 0; base < m; base += 256) {
-		const int row = base + tid;
-		const bool live = row < m && st.is_pivot[row] == 0;
-		const unsigned long long bal = __ballot(live);
-		if (lane == 0)
-			s_wave[wave] = __popcll(bal);
+	for (int base = 0; base < m; base += 256 * G2_SCAN) {
+		bool live[G2_SCAN];
+#pragma unroll
+		for (int u = 0; u < G2_SCAN; ++u) {
+			const int row = base + 256 * u + tid;
+			live[u] = row < m && st.is_pivot[row] == 0;
+		}
+		unsigned long long bal[G2_SCAN];
+#pragma unroll
+		for (int u = 0; u < G2_SCAN; ++u) {
+			bal[u] = __ballot(live[u]);
+			if (lane == 0)
+				s_scan[u][wave] = __popcll(bal[u]);
+		}
 		__syncthreads();
 		int before = s_cnt;
-		for (int w = 0; w < wave; ++w)
-			before += s_wave[w];
-		const int slot = before + __popcll(bal & ((1ull << lane) - 1));
-		if (live && slot < G2_CHUNK)
-			rowid[slot] = row;
+#pragma unroll
+		for (int u = 0; u < G2_SCAN; ++u) {
+			int b = before;
+			for (int w = 0; w < wave; ++w)
+				b += s_scan[u][w];
+			const int slot = b + __popcll(bal[u] & ((1ull << lane) - 1));
+			if (live[u] && slot < G2_CHUNK)
+				rowid[slot] = base + 256 * u + tid;
+			before += s_scan[u][0] + s_scan[u][1] + s_scan[u][2] + s_scan[u][3];
+		}
 		__syncthreads();
 		if (tid == 0)
-			s_cnt += s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+			s_cnt = before;
 		__syncthreads();
 		if (s_cnt >= G2_CHUNK) {
-			scanned_all = (s_cnt == G2_CHUNK && base + 256 >= m);
+			// every row was seen and exactly G2_CHUNK are live
+			scanned_all = (s_cnt == G2_CHUNK && base + 256 * G2_SCAN >= m);
 			break;
 		}
 	}
@@ -453,7 +649,7 @@ __global__ __launch_bounds__(256) void gf2_fast_kernel(const uint64_t *__restric
 		for (int w = 0; w < G2_IW; ++w)
 			xi[q][w] = (r >> 6) == w ? 1ull << (r & 63) : 0ull;
 	}
-	const int cnt = gf2_reg_eliminate<G2_SRPT, G2P, G2_IW>(v, alive, nr, wb, true, pivot, s_min, outr, outc, xi);
+	const int cnt = gf2_strip_eliminate(v, alive, nr, wb, outr, outc, xi);
 	const bool full = cnt == wb;
 	bool all_zero = false;
 	if (cnt == 0 && !scanned_all) {
@@ -463,11 +659,21 @@ __global__ __launch_bounds__(256) void gf2_fast_kernel(const uint64_t *__restric
 		if (tid == 0)
 			s_any = 0;
 		__syncthreads();
+		// G2_SCAN rows per thread per pass, their loads issued together
 		int any = 0;
-		for (int row = tid; row < m && !any; row += blockDim.x)
-			if (st.is_pivot[row] == 0)
-				for (int w = 0; w < G2P && cw + w < nw; ++w)
-					any |= A[(size_t) row * lda + cw + w] != 0ull;
+		for (int base = 0; base < m && !any; base += 256 * G2_SCAN) {
+			uint64_t acc = 0ull;
+#pragma unroll
+			for (int u = 0; u < G2_SCAN; ++u) {
+				const int row = base + 256 * u + tid;
+				if (row < m && st.is_pivot[row] == 0)
+#pragma unroll
+					for (int w = 0; w < G2P; ++w)
+						if (cw + w < nw)
+							acc |= A[(size_t) row * lda + cw + w];
+			}
+			any |= acc != 0ull;
+		}
 		if (any)
 			s_any = 1;
 		__syncthreads();
