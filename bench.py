@@ -49,7 +49,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip plain/FP64 gemm, FL and other rows")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--only", choices=["fl", "rows", "trsm", "gf2", "fp64", "batched", "cpu"], default=None,
+    ap.add_argument("--only", choices=["fl", "flmfma", "rows", "trsm", "gf2", "fp64", "batched", "cpu"],
+                    default=None,
                     help="run one secondary leg alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run one device-to-device copy of BYTES (counter calibration)")
@@ -197,7 +198,7 @@ def cpu_baselines(seconds):
 
 
 def bench_fl(la, synth, np):
-    """Config 4: Faugere-Lachartre on the 20000 x 30000 F4-shaped matrix."""
+    """Faugere-Lachartre on the 20000 x 30000 F4-shaped matrix."""
     ctx = la.Context(la.Modular(P, "uint32"), device=0)
     csr, shape = synth.fl_matrix(synth.seed_for(4, 0), P)
     drp = la.DeviceMatrix.from_host(ctx, csr[0].reshape(-1, 1))
@@ -230,8 +231,25 @@ def bench_fl(la, synth, np):
     return out
 
 
+def bench_fl_mfma(la, synth, np):
+    """The 20000 x 30000 Faugere-Lachartre reduction on the MFMA path only
+    (dense R, det), three calls: the leg a kernel trace of the pipeline is
+    taken on."""
+    ctx = la.Context(la.Modular(P, "uint32"), device=0)
+    csr, shape = synth.fl_matrix(synth.seed_for(4, 0), P)
+    drp, dc, dv = (la.DeviceMatrix.from_host(ctx, a.reshape(-1, 1)) for a in csr)
+    dR = la.DeviceMatrix(ctx, shape[0], shape[1], np.uint32)
+    FL = la.FaugereLachartre(ctx)
+    res = {}
+    t = time_steps(ctx, lambda: res.update(r=FL.echelonize(dR, (drp, dc, dv), shape=shape)), 2, 1)
+    for d in (drp, dc, dv, dR):
+        d.free()
+    ctx.close()
+    return {"shape": list(shape), "mfma_s": t, "rank": int(res["r"][0]), "det": int(res["r"][1])}
+
+
 def bench_batch(la, synth, np):
-    """Config 5: 4096 uneven instances (m, n, k in [32, 1024], half sparse,
+    """Batches of 4096 uneven instances (m, n, k in [32, 1024], half sparse,
     gemm and echelon forms mixed), resident in HBM, one captured graph per
     policy.  The autotuner table and the cost model are fitted on a separate
     1024-instance training batch (outside the timed region)."""
@@ -469,8 +487,8 @@ def main():
             print(json.dumps({"leg": "cpu", "cpu_baselines": cpu_baselines(args.cpu_seconds),
                               "host": host_cores()}), flush=True)
             return
-        leg = {"fl": bench_fl, "rows": bench_rows, "trsm": bench_trsm, "gf2": bench_gf2, "fp64": bench_fp64,
-               "batched": bench_batch}[args.only]
+        leg = {"fl": bench_fl, "flmfma": bench_fl_mfma, "rows": bench_rows, "trsm": bench_trsm, "gf2": bench_gf2,
+               "fp64": bench_fp64, "batched": bench_batch}[args.only]
         print(json.dumps({"leg": args.only, args.only: leg(la, synth, np)}), flush=True)
         return
 

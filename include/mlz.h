@@ -238,8 +238,8 @@ int mlz_zp_fl_echelonize(mlz_ctx *ctx, mlz_elem elem, uint64_t p, size_t m, size
  * column order, row-major with row stride n - rank.  R is the identity on the
  * pivot columns, so NP and the pivots determine it; NP is the block LELA's
  * splicer assembles R from (lela/algorithms/faugere-lachartre.tcc:363-504,
- * MatrixGrid3) and, for sparse rows, the nonzeros R holds.  At config 4
- * (20000 x 30000, rank 20000) it is a third of the dense R.  np_capacity
+ * MatrixGrid3) and, for sparse rows, the nonzeros R holds.  On the
+ * 20000 x 30000 F4-shaped matrix (rank 20000) it is a third of the dense R.  np_capacity
  * counts elements; a rank * (n - rank) above it returns MLZ_E_ARG with *rank
  * set (min(m, n) * n always suffices).  Other arguments, outputs and errors
  * as mlz_zp_fl_echelonize. */

@@ -191,7 +191,7 @@ def test_fl_config4_full(gpu_ctx):
     gpu_ctx.F = F
     FL = la.FaugereLachartre(gpu_ctx)
     R = np.zeros(shape, dtype=np.uint32)
-    print("fl config 4", flush=True)
+    print("fl 20000 x 30000", flush=True)
     rank, det = FL.echelonize(R, csr, shape=shape)
     assert rank == 20000
     print("all-row check", flush=True)
