@@ -442,7 +442,7 @@ int zp_gemm_i8_batched_dev(mlz_ctx *ctx, int elem, int elemA, uint32_t p, const 
 			   size_t k, uint32_t a, const void *A, size_t lda, const void *B, size_t ldb, uint32_t b,
 			   void *C, size_t ldc, const int8_t *Apre = nullptr);
 int zp_pack_i8_a_dev(mlz_ctx *ctx, int elem, uint32_t p, const I8Batch &bt, size_t rows, size_t k, const void *A,
-		     size_t lda, int8_t *dst, size_t *item_bytes);
+		     size_t lda, int8_t *dst, size_t *item_bytes, hipStream_t stream = nullptr);
 int zp_rank_update_i8_dev(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, const uint32_t *G, size_t ldg,
 			  const uint32_t *W, size_t ldw, uint32_t *C, size_t ldc);
 // The same update with G (m x 32*KB) and W (32*KB x n) already packed as INT8

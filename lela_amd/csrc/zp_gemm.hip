@@ -734,7 +734,7 @@ int zp_gemm_i8_batched_dev(mlz_ctx *ctx, int elem, int elemA, uint32_t p, const 
 // the bytes per item through *item_bytes.  For A operands used by several
 // products (zp_gemm_i8_batched_dev's Apre).
 int zp_pack_i8_a_dev(mlz_ctx *ctx, int elem, uint32_t p, const I8Batch &bt, size_t rows, size_t k, const void *A,
-		     size_t lda, int8_t *dst, size_t *item_bytes)
+		     size_t lda, int8_t *dst, size_t *item_bytes, hipStream_t stream)
 {
 	const size_t mpad = round_up(rows, 128), kpad = round_up(k, 64);
 	const size_t bytes = mpad * kpad * 2;
@@ -744,17 +744,23 @@ int zp_pack_i8_a_dev(mlz_ctx *ctx, int elem, uint32_t p, const I8Batch &bt, size
 		return MLZ_OK;
 	if (k > (size_t) I8_KMAX || p > 65536)
 		return set_error(ctx, MLZ_E_ARG, "INT8 packing needs k <= %d and p <= 2^16", I8_KMAX);
+	if (!stream)
+		stream = ctx->stream;
 	const int KB = (int) (kpad / 32);
 	dim3 g(mpad / 64, kpad / 64, bt.count);
+	const size_t es = elem_size(elem);
+	// 16-byte loads when every row of every item starts aligned
+	const bool vec = aligned16(A, lda * es) && (bt.count == 1 || ((bt.outer[0] | bt.in[0]) * (long long) es) % 16 == 0);
 	ProfScope ps(ctx, "pack_i8_A");
-#define PACK(EE, TT) pack_i8_kernel<EE, false, false><<<g, 256, 0, ctx->stream>>>((const TT *) A, lda, (int) rows, \
-									       (int) k, p, dst, KB, bt, bytes)
-	if (elem == MLZ_U16)
-		PACK(MLZ_U16, uint16_t);
-	else if (elem == MLZ_U32)
-		PACK(MLZ_U32, uint32_t);
-	else
-		PACK(MLZ_F64_SYM, double);
+#define PACK(EE, TT, V) pack_i8_kernel<EE, false, V><<<g, 256, 0, stream>>>((const TT *) A, lda, (int) rows, \
+									   (int) k, p, dst, KB, bt, bytes)
+	if (elem == MLZ_U16) {
+		if (vec) PACK(MLZ_U16, uint16_t, true); else PACK(MLZ_U16, uint16_t, false);
+	} else if (elem == MLZ_U32) {
+		if (vec) PACK(MLZ_U32, uint32_t, true); else PACK(MLZ_U32, uint32_t, false);
+	} else {
+		if (vec) PACK(MLZ_F64_SYM, double, true); else PACK(MLZ_F64_SYM, double, false);
+	}
 #undef PACK
 	return check_launch(ctx, "pack_i8_kernel(A, batch)");
 }

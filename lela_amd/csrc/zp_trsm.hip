@@ -15,6 +15,8 @@
 // (level3-generic.tcc:441-442,461-467,485-491).  B is left untouched then.
 #include "common.hpp"
 
+#include <vector>
+
 namespace mlz {
 
 constexpr int TRSM_LEAF = (int) TRSM_LEAF_ROWS;
@@ -63,16 +65,22 @@ __global__ void init_bad_kernel(unsigned long long *bad)
 // block (W receives block b's inverse in rows b*L .. b*L+nb-1, L columns
 // each).  Lane j computes column j of A_bb^-1, which depends on itself only:
 // x_i = (e_ij - sum_k A_ik x_k) / A_ii over the rows k solved before i, so
-// the substitution needs no barrier; the block's entries are LDS broadcasts.
-// The sum is delayed: raw products for p < 2^26 (64 terms below 2^58),
-// reduced products otherwise (64 terms below 2^38), one reduction per entry.
+// the substitution needs no barrier.  Column j lives in LDS (X[k][j]: the
+// lane's own entries, conflict free across lanes); row i of the block is
+// read as 16-byte broadcasts over the solved part only (4-aligned groups:
+// As is zero on and beyond the diagonal, X zero where unsolved), and four
+// independent partial sums keep the products from forming one dependent
+// chain.  The sum is delayed, one reduction per entry: raw 64-bit products
+// for p < 2^26 (64 terms below 2^58), reduced products otherwise (64 terms
+// below 2^38).  (Entries as doubles with FP64 fmas for p <= 2^16 measured
+// slower, 62 against 35 us for 128 blocks: the step is latency-bound.)
 template <int E, bool UPPER, bool UNIT, bool SMALL>
 __global__ __launch_bounds__(64) void trsm_diag_inv_col_kernel(const typename Elem<E>::T *__restrict__ A, size_t lda,
 							       int n, const uint32_t *__restrict__ dinv,
 							       uint32_t *__restrict__ W, ModP mp)
 {
 	constexpr int L = TRSM_LEAF;
-	__shared__ uint32_t As[L][L];
+	__shared__ __attribute__((aligned(16))) uint32_t As[L][L];
 	__shared__ uint32_t X[L][L];
 	__shared__ uint32_t Ds[L];
 	const uint32_t p = (uint32_t) mp.p;
@@ -81,37 +89,139 @@ __global__ __launch_bounds__(64) void trsm_diag_inv_col_kernel(const typename El
 	const int nb = min(L, n - b0);
 	// a partial block is completed by the identity (zero off-diagonal
 	// entries, unit diagonal): its first nb rows and columns are unchanged
+	// (all loads issued before the first store: one memory latency, not 64)
+	uint32_t v[L];
+#pragma unroll
 	for (int r = 0; r < L; ++r) {
-		uint32_t v = 0;
 		const bool tri = UPPER ? (j > r) : (j < r);
-		if (r < nb && j < nb && tri)
-			v = Elem<E>::load(A + (size_t) (b0 + r) * lda + b0 + j, p);
-		As[r][j] = v;
+		v[r] = (r < nb && j < nb && tri) ? Elem<E>::load(A + (size_t) (b0 + r) * lda + b0 + j, p) : 0u;
+	}
+#pragma unroll
+	for (int r = 0; r < L; ++r) {
+		As[r][j] = v[r];
 		X[r][j] = 0u;
 	}
 	Ds[j] = (UNIT || j >= nb) ? 1u : dinv[b0 + j];
 	__syncthreads();
-	// Column j lives in LDS (X[k][j]: this lane's own entries, conflict
-	// free across lanes).  As is zero outside the strict triangle and X is
-	// zero where not solved yet, so the inner loop runs over every k with a
-	// fixed trip count and no branch, and its LDS reads are issued in
-	// batches rather than one round trip per term.
 #pragma unroll 1
 	for (int s = 0; s < L; ++s) {
 		const int i = UPPER ? L - 1 - s : s;
-		uint64_t acc = 0;
-#pragma unroll 16
-		for (int k = 0; k < L; ++k) {
-			const uint32_t a = As[i][k], xk = X[k][j];
-			acc += SMALL ? (uint64_t) a * xk : (uint64_t) mp.mul(a, xk);
+		const int k0 = UPPER ? ((i + 1) & ~3) : 0, k1 = UPPER ? L : ((i + 3) & ~3);
+		uint64_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+#pragma unroll 4
+		for (int k = k0; k < k1; k += 4) {
+			const uint4 a4 = *(const uint4 *) &As[i][k];
+			const uint32_t x0 = X[k][j], x1 = X[k + 1][j], x2 = X[k + 2][j], x3 = X[k + 3][j];
+			if (SMALL) {
+				acc0 += (uint64_t) a4.x * x0;
+				acc1 += (uint64_t) a4.y * x1;
+				acc2 += (uint64_t) a4.z * x2;
+				acc3 += (uint64_t) a4.w * x3;
+			} else {
+				acc0 += mp.mul(a4.x, x0);
+				acc1 += mp.mul(a4.y, x1);
+				acc2 += mp.mul(a4.z, x2);
+				acc3 += mp.mul(a4.w, x3);
+			}
 		}
-		uint32_t v = mp.sub(i == j ? 1u : 0u, (uint32_t) mp.reduce(acc));
+		uint32_t x = mp.sub(i == j ? 1u : 0u, (uint32_t) mp.reduce((acc0 + acc1) + (acc2 + acc3)));
 		if (!UNIT)
-			v = mp.mul(v, Ds[i]);
-		X[i][j] = v;
+			x = mp.mul(x, Ds[i]);
+		X[i][j] = x;
 	}
 	for (int r = 0; r < nb; ++r)
 		W[(size_t) (b0 + r) * L + j] = j < nb ? X[r][j] : 0u;
+}
+
+// One product of the doubling steps of trsm_block_inverses:
+// C_z = a * A_z * B_z for the s x s blocks of batch member z (s a multiple of
+// 64), A in the caller's storage, B and C canonical uint32 (p <= 2^16), on
+// INT8 MFMA with the limb split done in LDS, so that no separate packing
+// launch is needed.  A workgroup computes one 64 x 64 tile of C (4 waves,
+// 32 x 32 each) over k-chunks of 64; each thread turns four k-consecutive
+// entries of A and of B per step into one 32-bit word per limb, in the
+// fragment layout of trsm_leaf_i8_kernel.  k <= 512 keeps every limb sum far
+// below 2^31 and the combined value an exact double.
+template <int EA>
+__global__ __launch_bounds__(256) void trsm_dbl_kernel(const typename Elem<EA>::T *__restrict__ A, size_t lda,
+						       const uint32_t *__restrict__ B, size_t ldb,
+						       uint32_t *__restrict__ C, size_t ldc, int s, uint32_t a,
+						       I8Batch bt, ModP mp)
+{
+	__shared__ __attribute__((aligned(16))) int8_t Af[2][2][2][64][16];   // [row block][k block][limb]
+	__shared__ __attribute__((aligned(16))) int8_t Bf[2][2][2][64][16];   // [column block][k block][limb]
+	const uint32_t p = (uint32_t) mp.p;
+	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+	const int nt = s >> 6, z = blockIdx.y;
+	const int r0 = (blockIdx.x / nt) * 64, c0 = (blockIdx.x % nt) * 64;
+	const typename Elem<EA>::T *Az = A + bt.offset(z, 0) + (size_t) r0 * lda;
+	const uint32_t *Bz = B + bt.offset(z, 1) + c0;
+	uint32_t *Cz = C + bt.offset(z, 2) + (size_t) r0 * ldc + c0;
+	const int rb = wave >> 1, cb = wave & 1;
+	v16i acc0 = v16i{0}, acc1 = v16i{0}, acc2 = v16i{0};
+	// thread tid handles A groups (row g >> 4, k 4 (g & 15) ..) and B groups
+	// (column g & 63, k 4 (g >> 6) ..) of the chunk, g = tid + 256 q; the next
+	// chunk's entries are loaded into registers while this one is multiplied
+	// (two chunks ahead, in two LDS buffers, measured slower: 120 against 90
+	// us for the eight launches of an 8192-row solve)
+	uint32_t ra[4][4], rbv[4][4];
+	auto load = [&](int k0) {
+#pragma unroll
+		for (int q = 0; q < 4; ++q) {
+			const int g = tid + 256 * q;
+			const int r = g >> 4, ka = 4 * (g & 15), c = g & 63, kb = 4 * (g >> 6);
+#pragma unroll
+			for (int t = 0; t < 4; ++t) {
+				ra[q][t] = Elem<EA>::load(Az + (size_t) r * lda + k0 + ka + t, p);
+				rbv[q][t] = Bz[(size_t) (k0 + kb + t) * ldb + c];
+			}
+		}
+	};
+	auto put = [&]() {
+#pragma unroll
+		for (int q = 0; q < 4; ++q) {
+			const int g = tid + 256 * q;
+			const int r = g >> 4, ka = 4 * (g & 15), c = g & 63, kb = 4 * (g >> 6);
+			uint32_t ahi = 0, alo = 0, bhi = 0, blo = 0;
+#pragma unroll
+			for (int t = 0; t < 4; ++t) {
+				const uint32_t la = i8_limbs(ra[q][t], p), lb = i8_limbs(rbv[q][t], p);
+				ahi |= (la >> 8) << (8 * t);
+				alo |= (la & 0xffu) << (8 * t);
+				bhi |= (lb >> 8) << (8 * t);
+				blo |= (lb & 0xffu) << (8 * t);
+			}
+			const int lna = (r & 31) + 32 * ((ka & 31) >> 4), lnb = (c & 31) + 32 * ((kb & 31) >> 4);
+			*(uint32_t *) &Af[r >> 5][ka >> 5][0][lna][ka & 15] = ahi;
+			*(uint32_t *) &Af[r >> 5][ka >> 5][1][lna][ka & 15] = alo;
+			*(uint32_t *) &Bf[c >> 5][kb >> 5][0][lnb][kb & 15] = bhi;
+			*(uint32_t *) &Bf[c >> 5][kb >> 5][1][lnb][kb & 15] = blo;
+		}
+	};
+	load(0);
+	for (int k0 = 0; k0 < s; k0 += 64) {
+		put();
+		__syncthreads();
+		if (k0 + 64 < s)
+			load(k0 + 64);
+#pragma unroll
+		for (int kb = 0; kb < 2; ++kb) {
+			const v4i ah = *(const v4i *) &Af[rb][kb][0][lane][0], al = *(const v4i *) &Af[rb][kb][1][lane][0];
+			const v4i bh = *(const v4i *) &Bf[cb][kb][0][lane][0], bl = *(const v4i *) &Bf[cb][kb][1][lane][0];
+			acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bh, acc0, 0, 0, 0);
+			acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ah, bl, acc1, 0, 0, 0);
+			acc2 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bl, acc2, 0, 0, 0);
+			acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(al, bh, acc1, 0, 0, 0);
+		}
+		__syncthreads();
+	}
+	const int col = cb * 32 + (lane & 31);
+#pragma unroll
+	for (int r = 0; r < 16; ++r) {
+		const int row = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+		const double v = fma((double) acc0[r], 65536.0, fma((double) acc1[r], 256.0, (double) acc2[r]));
+		Cz[(size_t) row * ldc + col] = mp.mod_f64((double) a * (double) mp.mod_f64(v));
+	}
 }
 
 // Leaf solve B_b <- a * W_b * B_b for one diagonal block (nb <= L rows), in
@@ -602,6 +712,7 @@ static int trsm_block_inverses(mlz_ctx *ctx, uint32_t p, int upper, size_t nfull
 {
 	constexpr size_t S = TRSM_INV_BLOCK;
 	const size_t total = nfull * S * S;
+	const ModP mp = make_modp(p);
 	{
 		ProfScope ps(ctx, "trsm_inv_init");
 		if (upper)
@@ -627,22 +738,52 @@ static int trsm_block_inverses(mlz_ctx *ctx, uint32_t p, int upper, size_t nfull
 		bt.outer[2] = to; bt.in[2] = ti;
 		const typename Elem<E>::T *offA = upper ? A + s : A + s * lda;
 		const uint32_t *inv_r = upper ? Z + s * S + s : Z;
-		MLZ_TRY(zp_gemm_i8_batched_dev(ctx, MLZ_U32, E, p, bt, s, s, s, 1, offA, lda, inv_r, S, 0, Tb, s));
+		const dim3 grid((unsigned) ((s / 64) * (s / 64)), (unsigned) bt.count);
+		trsm_dbl_kernel<E><<<grid, 256, 0, ctx->stream>>>(offA, lda, inv_r, S, Tb, s, (int) s, 1u, bt, mp);
+		MLZ_TRY(check_launch(ctx, "trsm_dbl_kernel"));
 		// off-diagonal block of the inverse = -(other diagonal inverse) * T
 		bt.outer[0] = zo; bt.in[0] = zi;
 		bt.outer[1] = to; bt.in[1] = ti;
 		bt.outer[2] = zo; bt.in[2] = zi;
 		const uint32_t *inv_l = upper ? Z : Z + s * S + s;             // U11^-1 / L22^-1
 		uint32_t *out = upper ? Z + s : Z + s * S;
-		MLZ_TRY(zp_gemm_i8_batched_dev(ctx, MLZ_U32, MLZ_U32, p, bt, s, s, s, p - 1, inv_l, S, Tb, s, 0, out, S));
+		trsm_dbl_kernel<MLZ_U32><<<grid, 256, 0, ctx->stream>>>(inv_l, S, Tb, s, out, S, (int) s, p - 1, bt, mp);
+		MLZ_TRY(check_launch(ctx, "trsm_dbl_kernel"));
 	}
 	return MLZ_OK;
 }
 
+// The off-diagonal blocks of A that trsm_rec's updates read when it runs with
+// the TRSM_INV_BLOCK inverses (Zp), in the order it reads them: rows r0 ..,
+// columns c0 .. of A, and the offset of the block's packed copy.
+struct TrsmUpd {
+	size_t r0, c0, rows, cols, off;
+};
+
+static void trsm_updates(size_t n, size_t base, int upper, std::vector<TrsmUpd> &out)
+{
+	// the cases in which trsm_rec does not split (p <= 2^16 here)
+	if (n == TRSM_INV_BLOCK || n <= TRSM_BLOCK_I8)
+		return;
+	const size_t l = (n / 2 + TRSM_INV_BLOCK - 1) / TRSM_INV_BLOCK * TRSM_INV_BLOCK;
+	if (!upper) {
+		trsm_updates(l, base, upper, out);
+		out.push_back({base + l, base, n - l, l, 0});
+		trsm_updates(n - l, base + l, upper, out);
+	} else {
+		trsm_updates(n - l, base + l, upper, out);
+		out.push_back({base, base + l, l, n - l, 0});
+		trsm_updates(l, base, upper, out);
+	}
+}
+
+// Upk: the packed update blocks in trsm_updates' order (with Zp), *ui the
+// next one to use.
 template <int E>
 static int trsm_rec(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, size_t nrhs, uint32_t a,
 		    const typename Elem<E>::T *A, size_t lda, typename Elem<E>::T *B, size_t ldb,
-		    const uint32_t *W, int algo, const int8_t *Zp = nullptr)
+		    const uint32_t *W, int algo, const int8_t *Zp = nullptr, const int8_t *const *Upk = nullptr,
+		    size_t *ui = nullptr)
 {
 	if (n == 0)
 		return MLZ_OK;
@@ -670,14 +811,24 @@ static int trsm_rec(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, siz
 	const typename Elem<E>::T *A11 = A, *A22 = A + l * lda + l;
 	typename Elem<E>::T *B1 = B, *B2 = B + l * ldb;
 	const int8_t *Zp2 = Zp ? Zp + (l / TRSM_INV_BLOCK) * TRSM_ZP_BYTES : nullptr;
+	// the update C <- -Aoff X + a C, its A block packed ahead when Upk is given
+	auto update = [&](size_t rows, size_t cols, const typename Elem<E>::T *Aoff, const typename Elem<E>::T *X,
+			  typename Elem<E>::T *Cu) -> int {
+		if (Upk)
+			return zp_gemm_i8_batched_dev(ctx, E, E, p, I8Batch::single(), rows, nrhs, cols, p - 1, nullptr, lda,
+						      X, ldb, a, Cu, ldb, Upk[(*ui)++]);
+		return zp_gemm_dev(ctx, E, p, rows, nrhs, cols, p - 1, Aoff, lda, X, ldb, a, Cu, ldb, algo);
+	};
 	if (!upper) {
-		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, a, A11, lda, B1, ldb, W, algo, Zp));
-		MLZ_TRY(zp_gemm_dev(ctx, E, p, n - l, nrhs, l, p - 1, A + l * lda, lda, B1, ldb, a, B2, ldb, algo));
-		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n - l, nrhs, 1, A22, lda, B2, ldb, W + l * TRSM_LEAF, algo, Zp2));
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, a, A11, lda, B1, ldb, W, algo, Zp, Upk, ui));
+		MLZ_TRY(update(n - l, l, A + l * lda, B1, B2));
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n - l, nrhs, 1, A22, lda, B2, ldb, W + l * TRSM_LEAF, algo, Zp2,
+				    Upk, ui));
 	} else {
-		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n - l, nrhs, a, A22, lda, B2, ldb, W + l * TRSM_LEAF, algo, Zp2));
-		MLZ_TRY(zp_gemm_dev(ctx, E, p, l, nrhs, n - l, p - 1, A + l, lda, B2, ldb, a, B1, ldb, algo));
-		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, 1, A11, lda, B1, ldb, W, algo, Zp));
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n - l, nrhs, a, A22, lda, B2, ldb, W + l * TRSM_LEAF, algo, Zp2,
+				    Upk, ui));
+		MLZ_TRY(update(l, n - l, A + l, B2, B1));
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, 1, A11, lda, B1, ldb, W, algo, Zp, Upk, ui));
 	}
 	return MLZ_OK;
 }
@@ -795,12 +946,54 @@ static int trsm_apply_t(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n,
 		return trsm_rec<E>(ctx, p, upper, unit, n, nrhs, a, A, lda, B, ldb, W, algo);
 	const size_t zbytes = round_up(nfull * S * S * sizeof(uint32_t), 256);
 	const size_t tbytes = round_up(nfull * S * (S / 4) * sizeof(uint32_t), 256);   // largest doubling step
+	// the updates' A blocks, packed on the second stream while the inverses
+	// are computed (neither depends on B)
+	std::vector<TrsmUpd> upd;
+	trsm_updates(n, 0, upper, upd);
+	size_t ubytes = 0;
+	for (TrsmUpd &u : upd) {
+		size_t item = 0;
+		MLZ_TRY(zp_pack_i8_a_dev(ctx, E, p, I8Batch::single(), u.rows, u.cols, nullptr, lda, nullptr, &item));
+		u.off = ubytes;
+		ubytes += round_up(item, 256);
+	}
 	void *w = nullptr;
-	MLZ_TRY(ws_get(ctx, WS_TRSM_3, zbytes + tbytes + nfull * TRSM_ZP_BYTES, &w));
+	MLZ_TRY(ws_get(ctx, WS_TRSM_3, zbytes + tbytes + nfull * TRSM_ZP_BYTES + ubytes, &w));
 	uint32_t *Z = (uint32_t *) w;
 	uint32_t *Tb = (uint32_t *) ((char *) w + zbytes);
 	int8_t *Zp = (int8_t *) w + zbytes + tbytes;
+	int8_t *Up = Zp + nfull * TRSM_ZP_BYTES;
+	std::vector<const int8_t *> upk(upd.size());
+	struct Events {
+		hipEvent_t e[2] = {nullptr, nullptr};
+		~Events()
+		{
+			for (hipEvent_t x : e)
+				if (x)
+					(void) hipEventDestroy(x);
+		}
+	} ev;
+	if (!upd.empty()) {
+		for (int q = 0; q < 2; ++q)
+			MLZ_HIP(ctx, hipEventCreateWithFlags(&ev.e[q], hipEventDisableTiming));
+		// A may have been written by work queued before this call
+		MLZ_HIP(ctx, hipEventRecord(ev.e[0], ctx->stream));
+	}
+	// the inverses are queued first, so that the context's stream has work
+	// while the packing launches are issued
 	MLZ_TRY(trsm_block_inverses<E>(ctx, p, upper, nfull, A, lda, W, Z, Tb));
+	if (!upd.empty()) {
+		hipStream_t s1;
+		MLZ_TRY(aux_stream(ctx, &s1));
+		MLZ_HIP(ctx, hipStreamWaitEvent(s1, ev.e[0], 0));
+		for (size_t q = 0; q < upd.size(); ++q) {
+			const TrsmUpd &u = upd[q];
+			upk[q] = Up + u.off;
+			MLZ_TRY(zp_pack_i8_a_dev(ctx, E, p, I8Batch::single(), u.rows, u.cols, A + u.r0 * lda + u.c0, lda,
+						 Up + u.off, nullptr, s1));
+		}
+		MLZ_HIP(ctx, hipEventRecord(ev.e[1], s1));
+	}
 	// the inverses as INT8 A operands, packed once for the leaves
 	I8Batch bz = I8Batch::single();
 	bz.count = (int) nfull;
@@ -810,7 +1003,14 @@ static int trsm_apply_t(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n,
 	if (item != TRSM_ZP_BYTES)
 		return set_error(ctx, MLZ_E_ARG, "packed inverse size %zu, expected %zu", item, TRSM_ZP_BYTES);
 	MLZ_TRY(zp_pack_i8_a_dev(ctx, MLZ_U32, p, bz, S, S, Z, S, Zp, nullptr));
-	return trsm_rec<E>(ctx, p, upper, unit, n, nrhs, a, A, lda, B, ldb, W, algo, Zp);
+	if (upd.empty())
+		return trsm_rec<E>(ctx, p, upper, unit, n, nrhs, a, A, lda, B, ldb, W, algo, Zp);
+	MLZ_HIP(ctx, hipStreamWaitEvent(ctx->stream, ev.e[1], 0));
+	size_t ui = 0;
+	MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n, nrhs, a, A, lda, B, ldb, W, algo, Zp, upk.data(), &ui));
+	if (ui != upd.size())
+		return set_error(ctx, MLZ_E_ARG, "trsm used %zu of %zu packed update blocks", ui, upd.size());
+	return MLZ_OK;
 }
 
 // The solve proper with the block inverses W of zp_trsm_prepare (for a

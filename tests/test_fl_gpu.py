@@ -60,8 +60,8 @@ def test_fl_storages(gpu_ctx, storage, p):
 
 
 @pytest.mark.parametrize("algo", [la.ALGO_PLAIN, la.ALGO_INT8])
-def test_fl_small_config4_shape(gpu_ctx, algo):
-    """The config-4 generator at 1/20 scale, against the oracle."""
+def test_fl_small_f4_shape(gpu_ctx, algo):
+    """The 20000 x 30000 F4-shaped generator at 1/20 scale, against the oracle."""
     p = 65521
     csr, shape = synth.fl_matrix(11, p, na=750, nc=250, nb=750, band=60, nnz_a=15)
     X = synth.csr_to_dense(csr, shape)
@@ -182,7 +182,7 @@ def check_rref_all_rows(R, piv, rank, csr, shape, p, seed, chunk=1000):
     assert np.array_equal(lhs, rhs), "a row of X is not X[:, piv] R"
 
 
-def test_fl_config4_full(gpu_ctx):
+def test_fl_20000x30000_full(gpu_ctx):
     """20000 x 30000, p = 65521: rank 20000, the RREF structure of every row
     of R, and every row of X equals X[:, piv] R (checked at a random vector)."""
     p = 65521

@@ -185,7 +185,7 @@ def test_matvec_mod_is_exact():
 
 
 @pytest.mark.parametrize("algo", ["plain", "int8", "fp64"])
-def test_gemm_cfg1_512_uint32(gpu_ctx, algo):
+def test_gemm_512_cubed_uint32(gpu_ctx, algo):
     """The reference's CPU-runnable case: C <- A*B + C at 512^3 over
     Modular<uint32>, p = 65521, on its seeded inputs, bit-exact against the
     oracle's ZpModule<uint32> product (level3-generic.tcc:285-322)."""
