@@ -38,8 +38,11 @@ template <int G2_RPT>
 __global__ __launch_bounds__(256) void gf2_gemm_m4rm_kernel(const uint64_t *__restrict__ A, size_t lda,
 							    const uint64_t *__restrict__ B, size_t ldb,
 							    uint64_t *__restrict__ C, size_t ldc, int m,
-							    int nbits, int k_bits, int a, int b)
+							    int nbits, int k_bits, int a, int b, const int *__restrict__ kz,
+							    const int *__restrict__ is_pivot, int rest_w, int *__restrict__ rest)
 {
+	if (kz && *kz == 0)
+		return;                 // an echelon-form panel without pivots: nothing to add
 	__shared__ uint64_t T[8][256][G2_TW];
 	__shared__ uint64_t Bs[64][G2_TW];
 	const int tid = threadIdx.x;
@@ -89,11 +92,13 @@ __global__ __launch_bounds__(256) void gf2_gemm_m4rm_kernel(const uint64_t *__re
 			__syncthreads();
 		}
 	}
+	uint64_t live_or = 0ull;
 #pragma unroll
 	for (int q = 0; q < G2_RPT; ++q) {
 		const int row = row0 + tid + 256 * q;
 		if (row >= m)
 			continue;
+		const bool live = rest && is_pivot[row] == 0;
 #pragma unroll
 		for (int w = 0; w < G2_TW; ++w) {
 			const int gw = w0 + w;
@@ -104,8 +109,13 @@ __global__ __launch_bounds__(256) void gf2_gemm_m4rm_kernel(const uint64_t *__re
 			const uint64_t old = *cp;
 			const uint64_t v = (b ? old : 0ull) ^ acc[q][w];
 			*cp = (v & mask) | (old & ~mask);
+			if (live && gw >= rest_w)
+				live_or |= v & mask;
 		}
 	}
+	// echelon-form update: does a live row keep a bit right of the panel?
+	if (rest && __ballot(live_or != 0ull) != 0ull && (threadIdx.x & 63) == 0)
+		atomicOr(rest, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -230,13 +240,16 @@ static int gf2_gemm_dot(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const
 	return check_launch(ctx, "gf2_gemm_dot_kernel");
 }
 
-int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda,
-		 const uint64_t *B, size_t ldb, int b, uint64_t *C, size_t ldc)
+// The method of four Russians; with kz, the launch does nothing when *kz is 0
+// (read on the device, so that the host need not wait for it).
+// With rest, a live row (is_pivot[row] == 0) with a bit in word rest_w or
+// later of its result sets *rest.
+static int gf2_gemm_m4rm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda,
+			     const uint64_t *B, size_t ldb, int b, uint64_t *C, size_t ldc, const int *kz,
+			     const int *is_pivot = nullptr, int rest_w = 0, int *rest = nullptr)
 {
 	if (m == 0 || n == 0)
 		return MLZ_OK;
-	if (ctx->gf2_kernel == 1)
-		return gf2_gemm_dot(ctx, m, n, k, a, A, lda, B, ldb, b, C, ldc);
 	const size_t nw = (n + 63) / 64;
 	ProfScope ps(ctx, "gf2_gemm_m4rm");
 	// 8 rows per thread where that still leaves 512 workgroups (two per CU):
@@ -244,11 +257,22 @@ int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64
 	dim3 g8(ceil_div(nw, G2_TW), ceil_div(m, 256 * 8));
 	if ((size_t) g8.x * g8.y >= 512)
 		gf2_gemm_m4rm_kernel<8><<<g8, 256, 0, ctx->stream>>>(A, lda, B, ldb, C, ldc, (int) m, (int) n, (int) k,
-								     (a & 1) && k > 0, b & 1);
+								     (a & 1) && k > 0, b & 1, kz, is_pivot, rest_w, rest);
 	else
 		gf2_gemm_m4rm_kernel<4><<<dim3(ceil_div(nw, G2_TW), ceil_div(m, 256 * 4)), 256, 0, ctx->stream>>>(
-			A, lda, B, ldb, C, ldc, (int) m, (int) n, (int) k, (a & 1) && k > 0, b & 1);
+			A, lda, B, ldb, C, ldc, (int) m, (int) n, (int) k, (a & 1) && k > 0, b & 1, kz, is_pivot, rest_w,
+			rest);
 	return check_launch(ctx, "gf2_gemm_m4rm_kernel");
+}
+
+int gf2_gemm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda,
+		 const uint64_t *B, size_t ldb, int b, uint64_t *C, size_t ldc)
+{
+	if (m == 0 || n == 0)
+		return MLZ_OK;
+	if (ctx->gf2_kernel == 1)
+		return gf2_gemm_dot(ctx, m, n, k, a, A, lda, B, ldb, b, C, ldc);
+	return gf2_gemm_m4rm_dev(ctx, m, n, k, a, A, lda, B, ldb, b, C, ldc, nullptr);
 }
 
 // ===========================================================================
@@ -264,6 +288,8 @@ constexpr int G2_MAXLVL = 24;
 struct Gf2State {
 	int *is_pivot, *rank, *piv_rows, *piv_cols, *k, *J, *prow, *lists_rows, *lists_cnt, *counters, *need_tree;
 	int *inv_done;   // the fast path also produced P_J^-1
+	int *finished;   // no live row has a bit left: every later panel is empty
+	int *rest;       // [panel & 1]: the panel's update left a bit in a live row right of it
 };
 
 __device__ inline int bit_of(const uint64_t *row, int col)
@@ -413,10 +439,13 @@ __device__ inline uint64_t g2_readlane64(uint64_t x, int l)
 // XORs its entry once.  Four barriers per strip instead of two per column.
 // The result is gf2_reg_eliminate's with reduced = true: every row is
 // cleared in every pivot column, and the same pivot rows are chosen.
+// IW carried words per row (x): the identity part on the fast path, none in
+// the tournament.
+template <int IW>
 __device__ int gf2_strip_eliminate(uint64_t (&v)[G2_SRPT][G2P], bool (&alive)[G2_SRPT], int nrows, int nsearch,
-				   int *out_rows, int *out_cols, uint64_t (&x)[G2_SRPT][G2_IW])
+				   int *out_rows, int *out_cols, uint64_t (&x)[G2_SRPT][IW > 0 ? IW : 1])
 {
-	constexpr int RW = G2P + G2_IW;   // words of a row: panel, then carried
+	constexpr int RW = G2P + IW;   // words of a row: panel, then carried
 	static_assert(G2_CHUNK == 512 && G2_SRPT == 2, "one wave holds the strip patterns of 512 rows");
 	__shared__ __align__(8) uint8_t sb[G2_CHUNK], sa[G2_CHUNK], sc[G2_CHUNK], sp[G2_CHUNK];
 	__shared__ uint64_t prow[8][RW];
@@ -535,7 +564,7 @@ __device__ int gf2_strip_eliminate(uint64_t (&v)[G2_SRPT][G2P], bool (&alive)[G2
 				for (int ww = 0; ww < G2P; ++ww)
 					prow[t][ww] = v[q][ww];
 #pragma unroll
-				for (int ww = 0; ww < G2_IW; ++ww)
+				for (int ww = 0; ww < IW; ++ww)
 					prow[t][G2P + ww] = x[q][ww];
 				alive[q] = false;
 			}
@@ -563,7 +592,7 @@ __device__ int gf2_strip_eliminate(uint64_t (&v)[G2_SRPT][G2P], bool (&alive)[G2
 				for (int ww = 0; ww < G2P; ++ww)
 					v[q][ww] ^= T[0][lo][ww] ^ T[1][hi][ww];
 #pragma unroll
-				for (int ww = 0; ww < G2_IW; ++ww)
+				for (int ww = 0; ww < IW; ++ww)
 					x[q][ww] ^= T[0][lo][G2P + ww] ^ T[1][hi][G2P + ww];
 			}
 		}
@@ -593,8 +622,24 @@ __global__ __launch_bounds__(256) void gf2_fast_kernel(const uint64_t *__restric
 	__shared__ int rowid[G2_CHUNK], outr[G2_PB], outc[G2_PB], pos[G2_CHUNK];
 	__shared__ int s_cnt, s_scan[G2_SCAN][4];
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-	if (tid == 0)
+	// the previous panel's pivot count; when it had pivots, its update
+	// recorded whether any live row kept a bit right of it
+	const int panel = cw / G2P;
+	const int prev_k = panel > 0 ? *st.k : 0;
+	if (*st.finished || (prev_k > 0 && st.rest[(panel - 1) & 1] == 0)) {
+		// every live row is zero to the end of the rows: no pivots are left
+		if (tid == 0) {
+			*st.finished = 1;
+			*st.k = 0;
+			*st.need_tree = 0;
+			*st.inv_done = 1;
+		}
+		return;
+	}
+	if (tid == 0) {
 		s_cnt = 0;
+		st.rest[panel & 1] = 0;   // for this panel's update
+	}
 	__syncthreads();
 	// the first G2_CHUNK live rows, G2_SCAN groups of 256 rows per pass (their
 	// flags loaded together, one prefix count per pass)
@@ -649,7 +694,7 @@ __global__ __launch_bounds__(256) void gf2_fast_kernel(const uint64_t *__restric
 		for (int w = 0; w < G2_IW; ++w)
 			xi[q][w] = (r >> 6) == w ? 1ull << (r & 63) : 0ull;
 	}
-	const int cnt = gf2_strip_eliminate(v, alive, nr, wb, outr, outc, xi);
+	const int cnt = gf2_strip_eliminate<G2_IW>(v, alive, nr, wb, outr, outc, xi);
 	const bool full = cnt == wb;
 	bool all_zero = false;
 	if (cnt == 0 && !scanned_all) {
@@ -680,6 +725,27 @@ __global__ __launch_bounds__(256) void gf2_fast_kernel(const uint64_t *__restric
 		all_zero = s_any == 0;
 	}
 	const bool done = full || scanned_all || all_zero;
+	if (cnt == 0 && done && prev_k == 0) {
+		// the panel has no pivot and no update ran before it: if no live row
+		// has a bit right of it either, the rank is complete and the
+		// remaining panels are empty
+		__shared__ int s_rest;
+		if (tid == 0)
+			s_rest = 0;
+		__syncthreads();
+		const int w0 = cw + G2P;
+		int any = 0;
+		for (int base = 0; base < m && !any && w0 < nw; base += 256) {
+			const int row = base + tid;
+			uint64_t acc = 0ull;
+			if (row < m && st.is_pivot[row] == 0)
+				for (int w = w0; w < nw; ++w)
+					acc |= A[(size_t) row * lda + w];
+			any = __syncthreads_or(acc != 0ull);
+		}
+		if (tid == 0 && !any)
+			*st.finished = 1;
+	}
 	if (full) {
 		// P_J^-1 row t (pivot of column outc[t]) from the identity part of row outr[t]
 		for (int s = tid; s < G2_CHUNK; s += blockDim.x)
@@ -734,8 +800,7 @@ __global__ __launch_bounds__(256) void gf2_tourney_kernel(const uint64_t *__rest
 							  int cw, int nw, int wb, Gf2State st, int nodes0)
 {
 	__shared__ int rowid[G2_CHUNK], outr[G2_PB], outc[G2_PB];
-	__shared__ uint64_t pivot[G2P];
-	__shared__ int s_min[2], s_go, s_nr, s_n0;
+	__shared__ int s_go, s_nr, s_n0;
 	if (*st.need_tree == 0)
 		return;
 	const int tid = threadIdx.x;
@@ -755,7 +820,8 @@ __global__ __launch_bounds__(256) void gf2_tourney_kernel(const uint64_t *__rest
 			v[q][w] = (r < nr0 && cw + w < nw) ? A[(size_t) (r0 + r) * lda + cw + w] : 0ull;
 	}
 	__syncthreads();
-	int cnt = gf2_reg_eliminate<G2_SRPT, G2P>(v, alive, nr0, wb, false, pivot, s_min, outr, outc);
+	uint64_t none[G2_SRPT][1];
+	int cnt = gf2_strip_eliminate<0>(v, alive, nr0, wb, outr, outc, none);
 	for (int t = tid; t < cnt; t += blockDim.x)
 		outr[t] = rowid[outr[t]];
 	__syncthreads();
@@ -813,7 +879,7 @@ __global__ __launch_bounds__(256) void gf2_tourney_kernel(const uint64_t *__rest
 			for (int w = 0; w < G2P; ++w)
 				v[q][w] = (r < nr && cw + w < nw) ? A[(size_t) rowid[r] * lda + cw + w] : 0ull;
 		}
-		cnt = gf2_reg_eliminate<G2_SRPT, G2P>(v, alive, nr, wb, false, pivot, s_min, outr, outc);
+		cnt = gf2_strip_eliminate<0>(v, alive, nr, wb, outr, outc, none);
 		for (int t = tid; t < cnt; t += blockDim.x)
 			outr[t] = rowid[outr[t]];
 		__syncthreads();
@@ -828,6 +894,8 @@ __global__ void gf2_gather_kernel(const uint64_t *__restrict__ A, size_t lda, in
 {
 	const int t = blockIdx.y;
 	const int k = *st.k;
+	if (k == 0)
+		return;                 // no pivots: P is not read
 	const int w = blockIdx.x * blockDim.x + threadIdx.x;
 	if (w < nwc)
 		P[(size_t) t * nwc + w] = t < k ? A[(size_t) st.prow[t] * lda + cw + w] : 0ull;
@@ -840,42 +908,59 @@ __global__ void gf2_gather_kernel(const uint64_t *__restrict__ A, size_t lda, in
 }
 
 // Inverse of P_J (k x k bits, identity-padded to G2_PB) by Gauss-Jordan on
-// [P_J | I], one row of 2*G2P words per thread.
+// [P_J | I] (strip elimination, the identity part carried): row t of thread
+// t; the pivot row of column c ends as row c of the inverse.
 __global__ __launch_bounds__(256) void gf2_inv_kernel(const uint64_t *__restrict__ P, int nwc, Gf2State st,
 						      uint64_t *__restrict__ Pinv)
 {
-	__shared__ int outr[G2_PB], outc[G2_PB], pos[G2_PB];
-	__shared__ uint64_t pivot[2 * G2P];
-	__shared__ int s_min[2];
+	__shared__ int outr[G2_PB], outc[G2_PB], pos[G2_PB], sJ[G2_PB];
 	if (*st.inv_done)
 		return;
 	const int tid = threadIdx.x;
 	const int k = *st.k;
-	uint64_t v[1][2 * G2P];
-	bool alive[1] = {true};
+	uint64_t v[G2_SRPT][G2P], x[G2_SRPT][G2P];
+	bool alive[G2_SRPT];
+#pragma unroll
+	for (int q = 0; q < G2_SRPT; ++q) {
+		alive[q] = q == 0;
+#pragma unroll
+		for (int w = 0; w < G2P; ++w)
+			v[q][w] = x[q][w] = 0ull;
+	}
+	sJ[tid] = tid < k ? st.J[tid] : 0;
+	__syncthreads();
 	{
 		const int r = tid;
-#pragma unroll
-		for (int w = 0; w < 2 * G2P; ++w)
-			v[0][w] = 0;
 		if (r < k) {
-			const uint64_t *src = P + (size_t) r * nwc;
-			for (int s = 0; s < k; ++s)
-				if (bit_of(src, st.J[s]))
-					v[0][s >> 6] |= 1ull << (s & 63);
+			// the row's panel words, then its bits on the pivot columns
+			uint64_t pw[G2P];
+#pragma unroll
+			for (int w = 0; w < G2P; ++w)
+				pw[w] = w < nwc ? P[(size_t) r * nwc + w] : 0ull;
+			for (int s = 0; s < k; ++s) {
+				const int j = sJ[s];
+				uint64_t word = pw[0];
+#pragma unroll
+				for (int w = 1; w < G2P; ++w)
+					word = (j >> 6) == w ? pw[w] : word;
+				const uint64_t bit = ((word >> (j & 63)) & 1ull) << (s & 63);
+#pragma unroll
+				for (int w = 0; w < G2P; ++w)
+					v[0][w] |= (s >> 6) == w ? bit : 0ull;
+			}
 		} else {
 			v[0][r >> 6] |= 1ull << (r & 63);
 		}
-		v[0][G2P + (r >> 6)] |= 1ull << (r & 63);
+		x[0][r >> 6] |= 1ull << (r & 63);
 		pos[r] = r;
 	}
-	const int cnt = gf2_reg_eliminate<1, 2 * G2P>(v, alive, G2_PB, k, true, pivot, s_min, outr, outc);
+	const int cnt = gf2_strip_eliminate<G2P>(v, alive, G2_PB, k, outr, outc, x);
 	for (int s = tid; s < cnt; s += blockDim.x)
 		pos[outr[s]] = outc[s];
 	__syncthreads();
 #pragma unroll
 	for (int w = 0; w < G2P; ++w)
-		Pinv[(size_t) pos[tid] * G2P + w] = v[0][G2P + w];
+		Pinv[(size_t) pos[tid] * G2P + w] = x[0][w];
 }
 
 // G[i] = bits J of row i (compressed to k bits), zero on this panel's pivots.
@@ -886,9 +971,9 @@ __global__ void gf2_gatherG_kernel(const uint64_t *__restrict__ A, size_t lda, i
 				   Gf2State st, uint64_t *__restrict__ G)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= m)
-		return;
 	const int k = *st.k;
+	if (i >= m || k == 0)
+		return;                 // no pivots: G is not read
 	uint64_t row[G2P], g[G2P];
 	for (int w = 0; w < G2P; ++w) {
 		row[w] = cw + w < nw ? A[(size_t) i * lda + cw + w] : 0ull;
@@ -937,6 +1022,8 @@ __global__ void gf2_init_kernel(Gf2State st, int m, int cnt_counters)
 		*st.k = 0;
 		*st.need_tree = 0;
 		*st.inv_done = 0;
+		*st.finished = 0;
+		st.rest[0] = st.rest[1] = 1;
 	}
 }
 
@@ -979,7 +1066,7 @@ int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size
 	MLZ_TRY(ws_get(ctx, WS_GF2_1, 2 * (size_t) G2_PB * nw * 8 + (size_t) G2_PB * G2P * 8, &w1));
 	MLZ_TRY(ws_get(ctx, WS_GF2_2, m * G2P * 8, &w2));
 	const size_t ints = m + 1 + 2 * rmax + 1 + 2 * G2_PB + (size_t) G2_MAXLVL * nodes0 * G2_PB +
-			    2 * (size_t) G2_MAXLVL * nodes0 + 2;
+			    2 * (size_t) G2_MAXLVL * nodes0 + 5;
 	MLZ_TRY(ws_get(ctx, WS_SMALL, ints * sizeof(int), &w3));
 	uint64_t *Wm = (uint64_t *) w0;
 	uint64_t *P = (uint64_t *) w1, *Wb = P + (size_t) G2_PB * nw, *Pinv = Wb + (size_t) G2_PB * nw;
@@ -997,6 +1084,8 @@ int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size
 	st.lists_cnt = q; q += (size_t) G2_MAXLVL * nodes0;
 	st.need_tree = q; q += 1;
 	st.inv_done = q; q += 1;
+	st.finished = q; q += 1;
+	st.rest = q; q += 2;
 	st.counters = q;
 	{
 		const int cc = G2_MAXLVL * nodes0;
@@ -1017,16 +1106,18 @@ int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size
 		gf2_gather_kernel<<<dim3(ceil_div(nwc, 256), G2_PB), 256, 0, ctx->stream>>>(Wm, nw, nwc, (int) cw, panel, st, P);
 		gf2_inv_kernel<<<1, 256, 0, ctx->stream>>>(P, nwc, st, Pinv);
 		MLZ_TRY(check_launch(ctx, "gf2_inv_kernel"));
-		MLZ_TRY(gf2_gemm_dev(ctx, G2_PB, (size_t) nwc * 64, G2_PB, 1, Pinv, G2P, P, nwc, 0, Wb, nwc));
+		MLZ_TRY(gf2_gemm_m4rm_dev(ctx, G2_PB, (size_t) nwc * 64, G2_PB, 1, Pinv, G2P, P, nwc, 0, Wb, nwc, st.k));
 		gf2_gatherG_kernel<<<ceil_div(m, 256), 256, 0, ctx->stream>>>(Wm, nw, (int) m, (int) cw, (int) nw, panel, st, G);
-		MLZ_TRY(gf2_gemm_dev(ctx, m, (size_t) nwc * 64, G2_PB, 1, G, G2P, Wb, nwc, 1, Wm + cw, nw));
+		MLZ_TRY(gf2_gemm_m4rm_dev(ctx, m, (size_t) nwc * 64, G2_PB, 1, G, G2P, Wb, nwc, 1, Wm + cw, nw, st.k,
+					  st.is_pivot, G2P, st.rest + (panel & 1)));
 		gf2_scatter_kernel<<<dim3(ceil_div(nwc, 256), G2_PB), 256, 0, ctx->stream>>>(Wm, nw, nwc, (int) cw, st, Wb);
 		MLZ_TRY(check_launch(ctx, "gf2_scatter_kernel"));
 		if ((panel & 3) == 3) {
-			int r = 0;
+			int r = 0, fin = 0;
 			MLZ_HIP(ctx, hipMemcpyAsync(&r, st.rank, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+			MLZ_HIP(ctx, hipMemcpyAsync(&fin, st.finished, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
 			MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
-			if ((size_t) r >= m)
+			if ((size_t) r >= m || fin)
 				break;
 		}
 	}
