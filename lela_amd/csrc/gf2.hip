@@ -246,20 +246,23 @@ static int gf2_gemm_dot(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const
 // later of its result sets *rest.
 static int gf2_gemm_m4rm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, const uint64_t *A, size_t lda,
 			     const uint64_t *B, size_t ldb, int b, uint64_t *C, size_t ldc, const int *kz,
-			     const int *is_pivot = nullptr, int rest_w = 0, int *rest = nullptr)
+			     const int *is_pivot = nullptr, int rest_w = 0, int *rest = nullptr,
+			     hipStream_t stream = nullptr)
 {
 	if (m == 0 || n == 0)
 		return MLZ_OK;
+	if (!stream)
+		stream = ctx->stream;
 	const size_t nw = (n + 63) / 64;
 	ProfScope ps(ctx, "gf2_gemm_m4rm");
 	// 8 rows per thread where that still leaves 512 workgroups (two per CU):
 	// 16384^3 in 4.65 ms against 4.89 ms with 4
 	dim3 g8(ceil_div(nw, G2_TW), ceil_div(m, 256 * 8));
 	if ((size_t) g8.x * g8.y >= 512)
-		gf2_gemm_m4rm_kernel<8><<<g8, 256, 0, ctx->stream>>>(A, lda, B, ldb, C, ldc, (int) m, (int) n, (int) k,
-								     (a & 1) && k > 0, b & 1, kz, is_pivot, rest_w, rest);
+		gf2_gemm_m4rm_kernel<8><<<g8, 256, 0, stream>>>(A, lda, B, ldb, C, ldc, (int) m, (int) n, (int) k,
+								(a & 1) && k > 0, b & 1, kz, is_pivot, rest_w, rest);
 	else
-		gf2_gemm_m4rm_kernel<4><<<dim3(ceil_div(nw, G2_TW), ceil_div(m, 256 * 4)), 256, 0, ctx->stream>>>(
+		gf2_gemm_m4rm_kernel<4><<<dim3(ceil_div(nw, G2_TW), ceil_div(m, 256 * 4)), 256, 0, stream>>>(
 			A, lda, B, ldb, C, ldc, (int) m, (int) n, (int) k, (a & 1) && k > 0, b & 1, kz, is_pivot, rest_w,
 			rest);
 	return check_launch(ctx, "gf2_gemm_m4rm_kernel");
@@ -289,7 +292,9 @@ struct Gf2State {
 	int *is_pivot, *rank, *piv_rows, *piv_cols, *k, *J, *prow, *lists_rows, *lists_cnt, *counters, *need_tree;
 	int *inv_done;   // the fast path also produced P_J^-1
 	int *finished;   // no live row has a bit left: every later panel is empty
-	int *rest;       // [panel & 1]: the panel's update left a bit in a live row right of it
+	int *rest;       // [panel & 3]: the panel's update left a bit in a live row right of it
+	int *kh;         // [panel & 3]: the panel's pivot count (st.k is overwritten by the next panel)
+	int *prow_h;     // [panel & 1][G2_PB]: the panel's pivot rows, likewise
 };
 
 __device__ inline int bit_of(const uint64_t *row, int col)
@@ -622,11 +627,12 @@ __global__ __launch_bounds__(256) void gf2_fast_kernel(const uint64_t *__restric
 	__shared__ int rowid[G2_CHUNK], outr[G2_PB], outc[G2_PB], pos[G2_CHUNK];
 	__shared__ int s_cnt, s_scan[G2_SCAN][4];
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-	// the previous panel's pivot count; when it had pivots, its update
-	// recorded whether any live row kept a bit right of it
+	// A panel with pivots records in its update whether a live row kept a
+	// bit right of it.  The previous panel's update may still be running
+	// (its columns right of this panel's), so the one before it is used.
 	const int panel = cw / G2P;
-	const int prev_k = panel > 0 ? *st.k : 0;
-	if (*st.finished || (prev_k > 0 && st.rest[(panel - 1) & 1] == 0)) {
+	const int prev_k = panel > 0 ? st.kh[(panel - 1) & 3] : 0;
+	if (*st.finished || (panel >= 2 && st.kh[(panel - 2) & 3] > 0 && st.rest[(panel - 2) & 3] == 0)) {
 		// every live row is zero to the end of the rows: no pivots are left
 		if (tid == 0) {
 			*st.finished = 1;
@@ -638,7 +644,7 @@ __global__ __launch_bounds__(256) void gf2_fast_kernel(const uint64_t *__restric
 	}
 	if (tid == 0) {
 		s_cnt = 0;
-		st.rest[panel & 1] = 0;   // for this panel's update
+		st.rest[panel & 3] = 0;   // for this panel's update
 	}
 	__syncthreads();
 	// the first G2_CHUNK live rows, G2_SCAN groups of 256 rows per pass (their
@@ -726,9 +732,10 @@ __global__ __launch_bounds__(256) void gf2_fast_kernel(const uint64_t *__restric
 	}
 	const bool done = full || scanned_all || all_zero;
 	if (cnt == 0 && done && prev_k == 0) {
-		// the panel has no pivot and no update ran before it: if no live row
-		// has a bit right of it either, the rank is complete and the
-		// remaining panels are empty
+		// the panel has no pivot and the previous one none either (so no
+		// update runs now, and none will tell): if no live row has a bit
+		// right of it, the rank is complete and the remaining panels are
+		// empty
 		__shared__ int s_rest;
 		if (tid == 0)
 			s_rest = 0;
@@ -894,6 +901,13 @@ __global__ void gf2_gather_kernel(const uint64_t *__restrict__ A, size_t lda, in
 {
 	const int t = blockIdx.y;
 	const int k = *st.k;
+	if (blockIdx.x == 0 && threadIdx.x == 0) {
+		// the panel's count and pivot rows, for work still running when the
+		// next panel's search overwrites st.k and st.prow
+		if (t == 0)
+			st.kh[panel & 3] = k;
+		st.prow_h[(panel & 1) * G2_PB + t] = t < k ? st.prow[t] : 0;
+	}
 	if (k == 0)
 		return;                 // no pivots: P is not read
 	const int w = blockIdx.x * blockDim.x + threadIdx.x;
@@ -997,17 +1011,20 @@ __global__ void gf2_gatherG_kernel(const uint64_t *__restrict__ A, size_t lda, i
 		G[(size_t) i * G2P + w] = g[w];
 }
 
-__global__ void gf2_scatter_kernel(uint64_t *__restrict__ A, size_t lda, int nwc, int cw, Gf2State st,
-				   const uint64_t *__restrict__ W)
+// The panel's pivot rows <- W on words [w0, w1) of the nwc from cw; kp and
+// prow: the panel's count and pivot rows.  count: the pivots join the rank.
+__global__ void gf2_scatter_kernel(uint64_t *__restrict__ A, size_t lda, int nwc, int cw, int w0, int w1,
+				   const int *__restrict__ kp, const int *__restrict__ prow, int *__restrict__ rank,
+				   int count, const uint64_t *__restrict__ W)
 {
 	const int t = blockIdx.y;
-	const int k = *st.k;
-	const int w = blockIdx.x * blockDim.x + threadIdx.x;
-	if (t < k && w < nwc)
-		A[(size_t) st.prow[t] * lda + cw + w] = W[(size_t) t * nwc + w];
-	// the panel's pivots join the rank (no workgroup of this launch reads it)
-	if (blockIdx.x == 0 && t == 0 && threadIdx.x == 0)
-		*st.rank += k;
+	const int k = *kp;
+	const int w = w0 + blockIdx.x * blockDim.x + threadIdx.x;
+	if (t < k && w < w1)
+		A[(size_t) prow[t] * lda + cw + w] = W[(size_t) t * nwc + w];
+	// (no workgroup of this launch reads the rank)
+	if (count && blockIdx.x == 0 && t == 0 && threadIdx.x == 0)
+		*rank += k;
 }
 
 __global__ void gf2_init_kernel(Gf2State st, int m, int cnt_counters)
@@ -1023,7 +1040,10 @@ __global__ void gf2_init_kernel(Gf2State st, int m, int cnt_counters)
 		*st.need_tree = 0;
 		*st.inv_done = 0;
 		*st.finished = 0;
-		st.rest[0] = st.rest[1] = 1;
+		for (int j = 0; j < 4; ++j) {
+			st.rest[j] = 1;
+			st.kh[j] = 0;
+		}
 	}
 }
 
@@ -1063,10 +1083,10 @@ int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size
 	const size_t rmax = std::min(m, n);
 	void *w0, *w1, *w2, *w3;
 	MLZ_TRY(ws_get(ctx, WS_GF2_0, m * nw * 8, &w0));
-	MLZ_TRY(ws_get(ctx, WS_GF2_1, 2 * (size_t) G2_PB * nw * 8 + (size_t) G2_PB * G2P * 8, &w1));
+	MLZ_TRY(ws_get(ctx, WS_GF2_1, 2 * (size_t) G2_PB * nw * 8 + 2 * (size_t) G2_PB * G2P * 8, &w1));
 	MLZ_TRY(ws_get(ctx, WS_GF2_2, m * G2P * 8, &w2));
 	const size_t ints = m + 1 + 2 * rmax + 1 + 2 * G2_PB + (size_t) G2_MAXLVL * nodes0 * G2_PB +
-			    2 * (size_t) G2_MAXLVL * nodes0 + 5;
+			    2 * (size_t) G2_MAXLVL * nodes0 + 11 + 2 * G2_PB;
 	MLZ_TRY(ws_get(ctx, WS_SMALL, ints * sizeof(int), &w3));
 	uint64_t *Wm = (uint64_t *) w0;
 	uint64_t *P = (uint64_t *) w1, *Wb = P + (size_t) G2_PB * nw, *Pinv = Wb + (size_t) G2_PB * nw;
@@ -1085,7 +1105,9 @@ int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size
 	st.need_tree = q; q += 1;
 	st.inv_done = q; q += 1;
 	st.finished = q; q += 1;
-	st.rest = q; q += 2;
+	st.rest = q; q += 4;
+	st.kh = q; q += 4;
+	st.prow_h = q; q += 2 * G2_PB;
 	st.counters = q;
 	{
 		const int cc = G2_MAXLVL * nodes0;
@@ -1093,25 +1115,73 @@ int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size
 		gf2_load_kernel<<<ceil_div(m * nw, 256), 256, 0, ctx->stream>>>(A, lda, (int) m, (int) n, Wm);
 		MLZ_TRY(check_launch(ctx, "gf2_load_kernel"));
 	}
+	// Look-ahead: a panel's update of its own and the next panel's words
+	// (the narrow part) runs on the context's stream, the rest (the wide
+	// part: Wb's remaining words, the update and the pivot rows' scatter
+	// there) on the second stream, beside the next panel's search.  The next
+	// panel's gather waits for the wide part; Pinv alternates between two
+	// buffers (the search writes it), k and the pivot rows are read from the
+	// panel's snapshots.
+	struct Overlap {
+		hipStream_t s1 = nullptr;
+		hipEvent_t en = nullptr, ew = nullptr;
+		~Overlap()
+		{
+			if (s1)
+				(void) hipStreamSynchronize(s1);
+			if (en)
+				(void) hipEventDestroy(en);
+			if (ew)
+				(void) hipEventDestroy(ew);
+		}
+	} ov;
+	if (nw > 2 * (size_t) G2P) {
+		MLZ_TRY(aux_stream(ctx, &ov.s1));
+		MLZ_HIP(ctx, hipEventCreateWithFlags(&ov.en, hipEventDisableTiming));
+		MLZ_HIP(ctx, hipEventCreateWithFlags(&ov.ew, hipEventDisableTiming));
+	}
+	bool wide_live = false;
 	int panel = 0;
 	for (size_t cw = 0; cw < nw; cw += G2P, ++panel) {
 		const int wb = (int) std::min((size_t) G2_PB, n - cw * 64);
 		const int nwc = (int) (nw - cw);
+		const int nn = std::min(nwc, 2 * G2P);
+		uint64_t *Pi = Pinv + (size_t) (panel & 1) * G2_PB * G2P;
+		const int *kp = st.kh + (panel & 3);
+		const int *prow = st.prow_h + (panel & 1) * G2_PB;
 		{
 			ProfScope ps(ctx, "gf2_pivots");
-			gf2_fast_kernel<<<1, 256, 0, ctx->stream>>>(Wm, nw, (int) m, (int) cw, (int) nw, wb, st, Pinv);
+			gf2_fast_kernel<<<1, 256, 0, ctx->stream>>>(Wm, nw, (int) m, (int) cw, (int) nw, wb, st, Pi);
 			gf2_tourney_kernel<<<nodes0, 256, 0, ctx->stream>>>(Wm, nw, (int) m, (int) cw, (int) nw, wb, st, nodes0);
 		}
 		MLZ_TRY(check_launch(ctx, "gf2_tourney_kernel"));
+		if (wide_live) {
+			MLZ_HIP(ctx, hipStreamWaitEvent(ctx->stream, ov.ew, 0));
+			wide_live = false;
+		}
 		gf2_gather_kernel<<<dim3(ceil_div(nwc, 256), G2_PB), 256, 0, ctx->stream>>>(Wm, nw, nwc, (int) cw, panel, st, P);
-		gf2_inv_kernel<<<1, 256, 0, ctx->stream>>>(P, nwc, st, Pinv);
+		gf2_inv_kernel<<<1, 256, 0, ctx->stream>>>(P, nwc, st, Pi);
 		MLZ_TRY(check_launch(ctx, "gf2_inv_kernel"));
-		MLZ_TRY(gf2_gemm_m4rm_dev(ctx, G2_PB, (size_t) nwc * 64, G2_PB, 1, Pinv, G2P, P, nwc, 0, Wb, nwc, st.k));
+		MLZ_TRY(gf2_gemm_m4rm_dev(ctx, G2_PB, (size_t) nn * 64, G2_PB, 1, Pi, G2P, P, nwc, 0, Wb, nwc, kp));
 		gf2_gatherG_kernel<<<ceil_div(m, 256), 256, 0, ctx->stream>>>(Wm, nw, (int) m, (int) cw, (int) nw, panel, st, G);
-		MLZ_TRY(gf2_gemm_m4rm_dev(ctx, m, (size_t) nwc * 64, G2_PB, 1, G, G2P, Wb, nwc, 1, Wm + cw, nw, st.k,
-					  st.is_pivot, G2P, st.rest + (panel & 1)));
-		gf2_scatter_kernel<<<dim3(ceil_div(nwc, 256), G2_PB), 256, 0, ctx->stream>>>(Wm, nw, nwc, (int) cw, st, Wb);
+		MLZ_TRY(gf2_gemm_m4rm_dev(ctx, m, (size_t) nn * 64, G2_PB, 1, G, G2P, Wb, nwc, 1, Wm + cw, nw, kp,
+					  st.is_pivot, G2P, st.rest + (panel & 3)));
+		gf2_scatter_kernel<<<dim3(ceil_div(nn, 256), G2_PB), 256, 0, ctx->stream>>>(Wm, nw, nwc, (int) cw, 0, nn, kp,
+												prow, st.rank, 1, Wb);
 		MLZ_TRY(check_launch(ctx, "gf2_scatter_kernel"));
+		if (nwc > nn) {
+			MLZ_HIP(ctx, hipEventRecord(ov.en, ctx->stream));
+			MLZ_HIP(ctx, hipStreamWaitEvent(ov.s1, ov.en, 0));
+			MLZ_TRY(gf2_gemm_m4rm_dev(ctx, G2_PB, (size_t) (nwc - nn) * 64, G2_PB, 1, Pi, G2P, P + nn, nwc, 0,
+						  Wb + nn, nwc, kp, nullptr, 0, nullptr, ov.s1));
+			MLZ_TRY(gf2_gemm_m4rm_dev(ctx, m, (size_t) (nwc - nn) * 64, G2_PB, 1, G, G2P, Wb + nn, nwc, 1,
+						  Wm + cw + nn, nw, kp, st.is_pivot, 0, st.rest + (panel & 3), ov.s1));
+			gf2_scatter_kernel<<<dim3(ceil_div(nwc - nn, 256), G2_PB), 256, 0, ov.s1>>>(
+				Wm, nw, nwc, (int) cw, nn, nwc, kp, prow, st.rank, 0, Wb);
+			MLZ_TRY(check_launch(ctx, "gf2_scatter_kernel"));
+			MLZ_HIP(ctx, hipEventRecord(ov.ew, ov.s1));
+			wide_live = true;
+		}
 		if ((panel & 3) == 3) {
 			int r = 0, fin = 0;
 			MLZ_HIP(ctx, hipMemcpyAsync(&r, st.rank, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -1121,6 +1191,8 @@ int gf2_rref_dev(mlz_ctx *ctx, size_t m, size_t n, uint64_t *A, size_t lda, size
 				break;
 		}
 	}
+	if (wide_live)
+		MLZ_HIP(ctx, hipStreamWaitEvent(ctx->stream, ov.ew, 0));
 	int r = 0;
 	MLZ_HIP(ctx, hipMemcpyAsync(&r, st.rank, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
 	MLZ_HIP(ctx, hipStreamSynchronize(ctx->stream));
