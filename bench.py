@@ -551,7 +551,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "int8",
-        "data": "synthetic (SplitMix64 uniform residues, seeds 0x4C454C4100000020..22)",
+        "data": "synthetic (SplitMix64 uniform residues, seeds 0x4C454C4100000020..22; every secondary "
+                "workload also drawn from SplitMix64 streams)",
         "config": {"workload": "Z/p gemm C <- A*B + C, m=n=k=8192, p=65521, Modular<uint16> "
                                "storage, inputs resident in HBM",
                    "m": N, "n": N, "k": N, "p": P, "storage": "uint16",
