@@ -104,16 +104,17 @@ def host_cores():
 
 
 def cpu_baselines(seconds):
-    """The CPU restatements (oracle) of LELA's paths for configs 2b, 3a, 3b, 4
-    and 5, each on a bounded sample of the workload, one core, extrapolated
-    as each entry's `sample` says."""
+    """The CPU restatements (oracle) of LELA's paths for the trsm, the two
+    GF(2) workloads, the Faugere-Lachartre reduction and the uneven batch,
+    each on a bounded sample of the workload, one core, extrapolated as each
+    entry's `sample` says."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc
     from lela_amd import synth
     lib = orc.load()
     out = {}
-    # 2b: trsm columns are independent -- solve a block of right-hand sides
+    # trsm: right-hand-side columns are independent -- solve a block of right-hand sides
     L = np.tril(synth.zp_matrix(synth.seed_for(2, 3), N, N, P), -1)
     L[np.arange(N), np.arange(N)] = 1
     cols = 32
@@ -124,12 +125,12 @@ def cpu_baselines(seconds):
     lib.orc_zp_trsm(P, 0, 1, N, cols, 1, L.ctypes.data, N, Bs.ctypes.data, cols, ctypes.byref(bad))
     t = time.perf_counter() - t0
     W = N * (N - 1) / 2 * cols
-    out["2b_trsm_8192x1024"] = {"value": W / t, "unit": "modular MACs/s", "cores": 1, "kind": "port",
+    out["trsm_8192x1024"] = {"value": W / t, "unit": "modular MACs/s", "cores": 1, "kind": "port",
                                 "est_full_s": t * 1024 / cols,
                                 "sample": f"{cols} of 1024 right-hand sides ({t:.1f} s), recursive trsm "
                                           "restatement, -O2"}
     del L, Bs
-    # 3a: GF(2) gemm rows are independent; the transposition of B is a fixed cost
+    # GF(2) gemm: rows are independent; the transposition of B is a fixed cost
     n = 16384
     A = synth.gf2_matrix(synth.seed_for(3, 0), n, n)
     B = synth.gf2_matrix(synth.seed_for(3, 1), n, n)
@@ -142,12 +143,12 @@ def cpu_baselines(seconds):
     t1, t2 = gf2_rows(1), gf2_rows(129)
     per_row = max(1e-9, (t2 - t1) / 128)
     est = t1 - per_row + per_row * n
-    out["3a_gf2_gemm_16384"] = {"value": float(n) ** 3 / est, "unit": "bit-MACs/s", "cores": 1, "kind": "port",
+    out["gf2_gemm_16384"] = {"value": float(n) ** 3 / est, "unit": "bit-MACs/s", "cores": 1, "kind": "port",
                                 "est_full_s": est,
                                 "sample": f"rows 1 and 129 of {n} ({t1:.1f} s, {t2:.1f} s: transposition of B plus "
                                           f"{per_row * 1e3:.0f} ms per row), word AND/XOR-parity dots, -O2"}
     del A, B
-    # 3b: the echelon form of a 4096^2 rank-3000 product, scaled by m*r*n to 16384^2 rank 12000
+    # GF(2) echelon form: the echelon form of a 4096^2 rank-3000 product, scaled by m*r*n to 16384^2 rank 12000
     m4, r4 = 4096, 3000
     X = synth.gf2_matrix(synth.seed_for(3, 2), m4, r4)
     Y = synth.gf2_matrix(synth.seed_for(3, 3), r4, m4)
@@ -158,11 +159,11 @@ def cpu_baselines(seconds):
     lib.orc_gf2_echelonize(m4, m4, M.ctypes.data, m4 // 64, 1, ctypes.byref(rank), piv.ctypes.data)
     t = time.perf_counter() - t0
     scale = (16384 * 12000 * 16384) / (m4 * r4 * m4)
-    out["3b_gf2_rref_16384_rank12000"] = {"value": 16384 * 12000 * 16384 / (t * scale), "unit": "bit-ops/s",
+    out["gf2_rref_16384_rank12000"] = {"value": 16384 * 12000 * 16384 / (t * scale), "unit": "bit-ops/s",
                                           "cores": 1, "kind": "port", "est_full_s": t * scale,
                                           "sample": f"{m4}^2 rank {rank.value} product ({t:.1f} s), scaled by "
                                                     f"m*r*n ({scale:.0f}x), Gauss-Jordan restatement, -O2"}
-    # 4: FL phases -- the dense Schur product D - C A^-1 B on sampled rows of C,
+    # Faugere-Lachartre phases -- the dense Schur product D - C A^-1 B on sampled rows of C,
     # the other phases at the same rate from their MAC counts
     rows = 2
     Cs = synth.zp_matrix(synth.seed_for(4, 1), rows, 15000, P)
@@ -176,12 +177,12 @@ def cpu_baselines(seconds):
     phases = {"trsm_A^-1B": 2.25e6 * 15000, "schur": 5000.0 * 15000 * 15000,
               "rref_D": 5000.0 * 5000 * 12500, "back_substitution": 15000.0 * 5000 * 10000}
     tot = sum(phases.values())
-    out["4_fl_20000x30000"] = {"value": tot / (tot / rate), "unit": "modular MACs/s", "cores": 1, "kind": "port",
+    out["fl_20000x30000"] = {"value": tot / (tot / rate), "unit": "modular MACs/s", "cores": 1, "kind": "port",
                                "est_full_s": tot / rate, "phase_MACs": phases,
                                "sample": f"{rows} of 5000 rows of the Schur product ({t:.1f} s, uint32 ZpModule "
                                          "dots); the other phases extrapolated at that rate from their MAC counts"}
     del Bf
-    # 5: the first 16 instances of the 4096-instance batch, one at a time
+    # uneven batch: the first 16 instances of the 4096-instance batch, one at a time
     raw = synth.batch_instances(synth.seed_for(5, 0), count=16, p=P)
     t0 = time.perf_counter()
     for d in raw:
@@ -190,7 +191,7 @@ def cpu_baselines(seconds):
         else:
             orc.zp_echelonize(P, d["A"].astype(np.uint32))
     t = time.perf_counter() - t0
-    out["5_batch_4096"] = {"value": len(raw) / t, "unit": "instances/s", "cores": 1, "kind": "port",
+    out["batch_4096"] = {"value": len(raw) / t, "unit": "instances/s", "cores": 1, "kind": "port",
                            "est_full_s": t * 4096 / len(raw),
                            "sample": f"instances 0..15 of the 4096 ({t:.1f} s; gemm via ZpModule dots, echelon "
                                      "forms via Elimination), one core"}
