@@ -700,15 +700,16 @@ __global__ void lu_iota_kernel(int *orig, int m)
 
 // Working copy: canonical residues of A in columns 0 .. n-1, for compute_L the
 // identity in columns n .. n+m-1.
+// Columns ntot .. ldx-1 (the padding of a row) are zero.
 template <int E>
 __global__ void lu_load_kernel(const typename Elem<E>::T *__restrict__ A, size_t lda, int m, int n, int ntot, uint32_t p,
 			       uint32_t *__restrict__ X, size_t ldx)
 {
 	const int col = blockIdx.x * blockDim.x + threadIdx.x, row = blockIdx.y;
-	if (col >= ntot)
+	if ((size_t) col >= ldx)
 		return;
 	X[(size_t) row * ldx + col] = col < n ? Elem<E>::load(A + (size_t) row * lda + col, p)
-					       : (col - n == row ? 1u : 0u);
+					       : (col - n == row && col < ntot ? 1u : 0u);
 }
 
 // Output row t.  mode U: the working copy.  compute_L: U from the diagonal
@@ -750,8 +751,9 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 	const ModP mp = make_modp(p);
 	const uint32_t pp = (uint32_t) p;
 	const size_t ntot = mode == MLZ_ECH_TRANSFORM ? n + m : n;
-	// rows of the working copy start on 128-byte boundaries
-	const size_t ldx = round_up(ntot, 32);
+	// rows of the working copy start on 256-byte boundaries; the columns past
+	// ntot are zero, so the packed update can run over whole 64-column strips
+	const size_t ldx = round_up(ntot, 64);
 	const size_t rmax = std::min(m, n);
 	const bool pluq = mode == MLZ_ECH_PLUQ;
 	// workspace: X (m x ntot); the window temporary (LU_K x max(ntot, rmax)) and
@@ -792,7 +794,7 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 	{
 		ProfScope ps(ctx, "lu_load");
 		lu_iota_kernel<<<ceil_div(m, 256), 256, 0, ctx->stream>>>(orig, (int) m);
-		dim3 g(ceil_div(ntot, 256), (unsigned) m);
+		dim3 g(ceil_div(ldx, 256), (unsigned) m);
 		if (elem == MLZ_U16)
 			lu_load_kernel<MLZ_U16><<<g, 256, 0, ctx->stream>>>((const uint16_t *) A, lda, (int) m, (int) n,
 									     (int) ntot, pp, X, ldx);
@@ -853,12 +855,10 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 		{
 			if (s1)
 				(void) hipStreamSynchronize(s1);
-			for (int b = 0; b < 2; ++b) {
-				if (nev[b])
-					(void) hipEventDestroy(nev[b]);
-				if (uev[b])
-					(void) hipEventDestroy(uev[b]);
-			}
+			for (int b = 0; b < 2; ++b)
+				for (hipEvent_t e : {nev[b], uev[b]})
+					if (e)
+						(void) hipEventDestroy(e);
 		}
 	} ov;
 	bool wide_live = false;      // a wide part is queued on the second stream
@@ -887,9 +887,18 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 		wide_live = false;
 		return MLZ_OK;
 	};
+	// Columns c .. ldx-1, the zero padding included, so that the wide update
+	// runs over whole 64-column strips (a separate launch for the last columns
+	// was on the critical path: 24.3 -> 24.1 ms for the 5000 x 15000 D of the
+	// 20000 x 30000 reduction).  Starting the wide part's W, or all of the wide
+	// part, beside the narrow chain measured slower (25.8 and 24.5 ms): the
+	// narrow chain is latency-bound and loses more to the shared CUs than the
+	// overlap gains.
 	auto update_fused = [&](int kw, int b) -> int {
-		const size_t nc = ntot - c, mb = m - i;
+		const size_t nc = ldx - c, mb = m - i;
 		const size_t nn = std::min(nc, (size_t) LU_NARROW);
+		const int nbn = (int) (nn / 32);
+		const bool wide = nc > nn;
 		MLZ_TRY(join_wide());
 		{
 			ProfScope ps(ctx, "lu_update");
@@ -900,19 +909,17 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 											Gpk[b], pp);
 			MLZ_TRY(check_launch(ctx, "lu_gpack_i8_kernel"));
 			MLZ_TRY(zp_rank_update_i8_packed(ctx, pp, mb, nn, 2, Gpk[b], Wpk[b], X + i * ldx + c, ldx));
-		}
-		if (nc > nn) {
-			const int nbn = (int) (nn / 32);
-			MLZ_HIP(ctx, hipEventRecord(ov.nev[b], ctx->stream));
-			MLZ_HIP(ctx, hipStreamWaitEvent(ov.s1, ov.nev[b], 0));
-			lu_wb_i8_kernel<32><<<ceil_div(nc, 32) - nbn, 256, 0, ov.s1>>>(X, ldx, (int) nc, (int) c, (int) i, kw,
-											    st, LiM, Wpk[b], mp, nbn);
-			MLZ_TRY(check_launch(ctx, "lu_wb_i8_kernel"));
-			MLZ_TRY(zp_rank_update_i8_packed(ctx, pp, mb, nc - nn, 2, Gpk[b], Wpk[b] + (size_t) nbn * 4096,
-							 X + i * ldx + c + nn, ldx, ov.s1));
-			MLZ_HIP(ctx, hipEventRecord(ov.uev[b], ov.s1));
-			wide_live = true;
-			wide_b = b;
+			if (wide) {
+				MLZ_HIP(ctx, hipEventRecord(ov.nev[b], ctx->stream));
+				MLZ_HIP(ctx, hipStreamWaitEvent(ov.s1, ov.nev[b], 0));
+				lu_wb_i8_kernel<32><<<ceil_div(nc, 32) - nbn, 256, 0, ov.s1>>>(X, ldx, (int) nc, (int) c, (int) i, kw,
+												    st, LiM, Wpk[b], mp, nbn);
+				MLZ_TRY(zp_rank_update_i8_packed(ctx, pp, mb, nc - nn, 2, Gpk[b], Wpk[b] + (size_t) nbn * 4096,
+								 X + i * ldx + c + nn, ldx, ov.s1));
+				MLZ_HIP(ctx, hipEventRecord(ov.uev[b], ov.s1));
+				wide_live = true;
+				wide_b = b;
+			}
 		}
 		return MLZ_OK;
 	};
