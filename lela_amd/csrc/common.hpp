@@ -308,6 +308,7 @@ struct mlz_ctx {
 	int device;
 	hipStream_t stream;
 	hipStream_t aux_stream;  // second stream (default priority) for overlapped work; created on first use
+	hipStream_t copy_stream; // third stream (default priority) for small result copies; created on first use
 	int ptr_mode;
 	std::string last_error;
 	void *ws[mlz::WS_NUM_SLOTS];
@@ -451,6 +452,7 @@ int zp_rank_update_i8_packed(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, int K
 			     const int8_t *Wp, uint32_t *C, size_t ldc, hipStream_t stream = nullptr);
 // The context's second stream, created on first use (default priority).
 int aux_stream(mlz_ctx *ctx, hipStream_t *out);
+int copy_stream(mlz_ctx *ctx, hipStream_t *out);
 int check_modulus(mlz_ctx *ctx, int elem, uint64_t p);
 void tuner_free(mlz_ctx *ctx);
 // Looks up (once per device, with the device current) the address of the

@@ -138,10 +138,24 @@ int aux_stream(mlz_ctx *ctx, hipStream_t *out)
 		const hipError_t e = hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking);
 		if (e != hipSuccess) {
 			ctx->aux_stream = nullptr;
+	ctx->copy_stream = nullptr;
 			return set_error(ctx, MLZ_E_HIP, "hipStreamCreateWithFlags: %s", hipGetErrorString(e));
 		}
 	}
 	*out = ctx->aux_stream;
+	return MLZ_OK;
+}
+
+int copy_stream(mlz_ctx *ctx, hipStream_t *out)
+{
+	if (!ctx->copy_stream) {
+		const hipError_t e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking);
+		if (e != hipSuccess) {
+			ctx->copy_stream = nullptr;
+			return set_error(ctx, MLZ_E_HIP, "hipStreamCreateWithFlags: %s", hipGetErrorString(e));
+		}
+	}
+	*out = ctx->copy_stream;
 	return MLZ_OK;
 }
 
@@ -231,6 +245,8 @@ int mlz_ctx_destroy(mlz_ctx *ctx)
 		(void) hipHostFree(ctx->host_pinned);
 	if (ctx->aux_stream)
 		(void) hipStreamDestroy(ctx->aux_stream);
+	if (ctx->copy_stream)
+		(void) hipStreamDestroy(ctx->copy_stream);
 	(void) hipStreamDestroy(ctx->stream);
 	delete ctx;
 	return MLZ_OK;

@@ -819,9 +819,14 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 	// host can queue the panel's update behind the window kernel first.
 	struct Res {
 		LuPanel *h = nullptr;
-		hipEvent_t ev = nullptr;
+		hipEvent_t ev = nullptr, wev = nullptr;
+		hipStream_t cs = nullptr;
 		~Res()
 		{
+			if (cs)
+				(void) hipStreamSynchronize(cs);
+			if (wev)
+				(void) hipEventDestroy(wev);
 			if (ev)
 				(void) hipEventDestroy(ev);
 			if (h)
@@ -830,6 +835,12 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 	} res;
 	MLZ_HIP(ctx, hipHostMalloc((void **) &res.h, sizeof(LuPanel), hipHostMallocDefault));
 	MLZ_HIP(ctx, hipEventCreateWithFlags(&res.ev, hipEventDisableTiming));
+	MLZ_HIP(ctx, hipEventCreateWithFlags(&res.wev, hipEventDisableTiming));
+	// the state goes back on a stream of its own, so that the copy does not
+	// hold up the panel's update queued behind the window kernel
+	hipStream_t cs;
+	MLZ_TRY(copy_stream(ctx, &cs));
+	res.cs = cs;
 	auto swap_rows = [&](int a, int b) -> int {
 		lu_swap_rows_kernel<uint32_t><<<ceil_div(ntot, 256), 256, 0, ctx->stream>>>(X, ldx, (int) ntot, a, b);
 		lu_swap_rows_kernel<int><<<1, 1, 0, ctx->stream>>>(orig, 1, 1, a, b);
@@ -902,7 +913,8 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 		MLZ_TRY(join_wide());
 		{
 			ProfScope ps(ctx, "lu_update");
-			lu_window_orig_kernel<<<1, LU_K, 0, ctx->stream>>>(orig, (int) i, kw, st);
+			if (mode == MLZ_ECH_TRANSFORM)   // the row origins matter for T only
+				lu_window_orig_kernel<<<1, LU_K, 0, ctx->stream>>>(orig, (int) i, kw, st);
 			lu_wb_i8_kernel<32><<<ceil_div(nn, 32), 256, 0, ctx->stream>>>(X, ldx, (int) nc, (int) c, (int) i, kw,
 											    st, LiM, Wpk[b], mp, 0);
 			lu_gpack_i8_kernel<<<ceil_div(mb, 64), 256, 0, ctx->stream>>>(X, ldx, (int) mb, (int) i, (int) c, st,
@@ -933,7 +945,8 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 			X, ldx, (int) ntot, (int) i, st, Xw);
 		lu_window_put_kernel<uint32_t><<<dim3(ceil_div(ntot, 256), kw), 256, 0, ctx->stream>>>(
 			X, ldx, (int) ntot, (int) i, Xw);
-		lu_window_orig_kernel<<<1, LU_K, 0, ctx->stream>>>(orig, (int) i, kw, st);
+		if (mode == MLZ_ECH_TRANSFORM)
+			lu_window_orig_kernel<<<1, LU_K, 0, ctx->stream>>>(orig, (int) i, kw, st);
 		if (pluq && i > 0) {
 			lu_window_gather_kernel<uint32_t><<<dim3(ceil_div(rmax, 256), kw), 256, 0, ctx->stream>>>(
 				Lb, rmax, (int) rmax, (int) i, st, Xw);
@@ -993,8 +1006,10 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 											      (int) i, mp, invtab, st, L11, LiM);
 		}
 		MLZ_TRY(check_launch(ctx, "lu_window_kernel"));
-		MLZ_HIP(ctx, hipMemcpyAsync(res.h, st, sizeof(LuPanel), hipMemcpyDeviceToHost, ctx->stream));
-		MLZ_HIP(ctx, hipEventRecord(res.ev, ctx->stream));
+		MLZ_HIP(ctx, hipEventRecord(res.wev, ctx->stream));
+		MLZ_HIP(ctx, hipStreamWaitEvent(cs, res.wev, 0));
+		MLZ_HIP(ctx, hipMemcpyAsync(res.h, st, sizeof(LuPanel), hipMemcpyDeviceToHost, cs));
+		MLZ_HIP(ctx, hipEventRecord(res.ev, cs));
 		// pluq's sizes depend on rp; otherwise the update is queued at once
 		const bool early = !pluq;
 		if (early)
