@@ -554,15 +554,29 @@ __global__ __launch_bounds__(256) void lu_wb_i8_kernel(uint32_t *__restrict__ X,
 	const int cbk = cb0 + (int) blockIdx.x;     // the slice's block index (W fragments too)
 	const int j0 = cbk * TC;
 	const int rp = st->rp;
-	if (tid < LU_K)
-		wq[tid] = st->wperm[tid];
+	// LiM (32 KB) as 16-byte loads, issued before the window order is waited
+	// for, so that its latency overlaps the dependent loads of the rows
+	{
+		uint4 lv[2 * LU_W * LU_W / 4 / 256];
+#pragma unroll
+		for (int u = 0; u < 2 * LU_W * LU_W / 4 / 256; ++u)
+			lv[u] = reinterpret_cast<const uint4 *>(LiM)[tid + 256 * u];
+		if (tid < LU_K)
+			wq[tid] = st->wperm[tid];
+#pragma unroll
+		for (int u = 0; u < 2 * LU_W * LU_W / 4 / 256; ++u) {
+			const int e = 4 * (tid + 256 * u), r = e / LU_W, cc = e % LU_W;
+			Pm[r][cc] = lv[u].x;
+			Pm[r][cc + 1] = lv[u].y;
+			Pm[r][cc + 2] = lv[u].z;
+			Pm[r][cc + 3] = lv[u].w;
+		}
+	}
 	__syncthreads();
 	for (int e = tid; e < LU_K * TC; e += 256) {
 		const int q = e / TC, col = e % TC, gc = j0 + col;
 		Ps[q][col] = (q < kw && gc < nc) ? X[(size_t) (i0 + wq[q]) * ldx + c + gc] : 0u;
 	}
-	for (int e = tid; e < 2 * LU_W * LU_W; e += 256)
-		Pm[e / LU_W][e % LU_W] = LiM[e];
 	__syncthreads();
 	// wave w: output rows 32w .. 32w+31 as two 16-row blocks, TC/16 column
 	// blocks each.  A fragment: lane l holds (row l%16, k l/16); B fragment:
