@@ -775,10 +775,21 @@ __global__ __launch_bounds__(256) void rref_wb_i8_kernel(uint32_t *__restrict__ 
 	const uint32_t p = (uint32_t) mp.p;
 	const int tid = threadIdx.x;
 	const int j0 = blockIdx.x * TC;
+	// P_J^-1 loaded first, all at once, so that its latency overlaps the
+	// dependent loads of the pivot rows
+	uint32_t pv[W * W / 256];
+#pragma unroll
+	for (int u = 0; u < W * W / 256; ++u)
+		pv[u] = Pinv[tid + 256 * u];
 	const int k = *st.k;
 	const int kb = *st.rank - *st.rank0;
 	if (tid < W)
 		prow[tid] = tid < k ? st.prow[tid] : -1;
+#pragma unroll
+	for (int u = 0; u < W * W / 256; ++u) {
+		const int e = tid + 256 * u;
+		Pi[e / W][e % W] = pv[u];
+	}
 	__syncthreads();
 	for (int e = tid; e < W * TC; e += 256) {
 		const int t = e / TC, col = e % TC, gc = j0 + col;
@@ -790,8 +801,6 @@ __global__ __launch_bounds__(256) void rref_wb_i8_kernel(uint32_t *__restrict__ 
 		}
 		Ps[t][col] = x;
 	}
-	for (int e = tid; e < W * W; e += 256)
-		Pi[e / W][e % W] = Pinv[e];
 	__syncthreads();
 	// W on v_mfma_f64_16x16x4f64: wave w computes rows 16w .. 16w+15 as
 	// TC/16 16 x 16 blocks.  A fragment: lane l holds (row l%16, k l/16); B
