@@ -32,8 +32,7 @@ __device__ inline uint64_t tail_mask64(int bits)
 }
 
 // k_bits: inner dimension in bits; nbits: C columns.  G2_RPT rows per thread
-// (256 * G2_RPT per workgroup): 8 halves the table builds per row of C,
-// 4 gives twice the workgroups to narrow products.
+// (256 * G2_RPT per workgroup).
 template <int G2_RPT>
 __global__ __launch_bounds__(256) void gf2_gemm_m4rm_kernel(const uint64_t *__restrict__ A, size_t lda,
 							    const uint64_t *__restrict__ B, size_t ldb,
@@ -264,16 +263,11 @@ static int gf2_gemm_m4rm_dev(mlz_ctx *ctx, size_t m, size_t n, size_t k, int a, 
 		stream = ctx->stream;
 	const size_t nw = (n + 63) / 64;
 	ProfScope ps(ctx, "gf2_gemm_m4rm");
-	// 8 rows per thread where that still leaves 512 workgroups (two per CU):
-	// 16384^3 in 4.65 ms against 4.89 ms with 4
-	dim3 g8(ceil_div(nw, G2_TW), ceil_div(m, 256 * 8));
-	if ((size_t) g8.x * g8.y >= 512)
-		gf2_gemm_m4rm_kernel<8><<<g8, 256, 0, stream>>>(A, lda, B, ldb, C, ldc, (int) m, (int) n, (int) k,
-								(a & 1) && k > 0, b & 1, kz, is_pivot, rest_w, rest);
-	else
-		gf2_gemm_m4rm_kernel<4><<<dim3(ceil_div(nw, G2_TW), ceil_div(m, 256 * 4)), 256, 0, stream>>>(
-			A, lda, B, ldb, C, ldc, (int) m, (int) n, (int) k, (a & 1) && k > 0, b & 1, kz, is_pivot, rest_w,
-			rest);
+	// 4 rows per thread (1024 per workgroup): 16384^3 in 2.72 ms, against
+	// 3.87 ms with 8 (which spills registers) and 3.71 ms with 2 (twice the
+	// table builds per row)
+	gf2_gemm_m4rm_kernel<4><<<dim3(ceil_div(nw, G2_TW), ceil_div(m, 256 * 4)), 256, 0, stream>>>(
+		A, lda, B, ldb, C, ldc, (int) m, (int) n, (int) k, (a & 1) && k > 0, b & 1, kz, is_pivot, rest_w, rest);
 	return check_launch(ctx, "gf2_gemm_m4rm_kernel");
 }
 
