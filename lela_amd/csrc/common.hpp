@@ -450,6 +450,8 @@ int zp_rank_update_i8_dev(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k
 // limb fragments ([32-index block][k block][limb][64 lanes][16 bytes]).
 int zp_rank_update_i8_packed(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, int KB, const int8_t *Gp,
 			     const int8_t *Wp, uint32_t *C, size_t ldc, hipStream_t stream = nullptr);
+int zp_rank_update_i8_packed(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, int KB, const int8_t *Gp,
+			     const int8_t *Wp, uint16_t *C, size_t ldc, hipStream_t stream = nullptr);
 // The context's second stream, created on first use (default priority).
 int aux_stream(mlz_ctx *ctx, hipStream_t *out);
 int copy_stream(mlz_ctx *ctx, hipStream_t *out);

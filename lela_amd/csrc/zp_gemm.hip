@@ -775,10 +775,10 @@ int zp_pack_i8_a_dev(mlz_ctx *ctx, int elem, uint32_t p, const I8Batch &bt, size
 // epilogue subtracts in place.  No barriers: a wave past the last row block
 // just returns.
 // ===========================================================================
-template <int UPD_NBLK>
+template <int UPD_NBLK, typename TC>
 __global__ __launch_bounds__(256) void rank_update_i8_kernel(const int8_t *__restrict__ Gp,
 							     const int8_t *__restrict__ Wp, int KB,
-							     uint32_t *__restrict__ C, size_t ldc, int m, int n,
+							     TC *__restrict__ C, size_t ldc, int m, int n,
 							     ModP mp, int strips_n)
 {
 	const int lane = threadIdx.x & 63;
@@ -798,7 +798,7 @@ __global__ __launch_bounds__(256) void rank_update_i8_kernel(const int8_t *__res
 #pragma unroll
 		for (int r = 0; r < 16; ++r) {
 			const int row = rb * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-			cv[j][r] = (j < nblk && row < m && col < n) ? C[(size_t) row * ldc + col] : 0u;
+			cv[j][r] = (j < nblk && row < m && col < n) ? (uint32_t) C[(size_t) row * ldc + col] : 0u;
 		}
 	}
 	v16i acc[3][UPD_NBLK];
@@ -836,7 +836,7 @@ __global__ __launch_bounds__(256) void rank_update_i8_kernel(const int8_t *__res
 						     fma((double) acc[1][j][r], 256.0, (double) acc[2][j][r]));
 				const uint32_t d = mp.mod_f64(v);
 				const uint32_t c = cv[j][r];
-				C[(size_t) row * ldc + col] = c >= d ? c - d : c + p - d;
+				C[(size_t) row * ldc + col] = (TC) (c >= d ? c - d : c + p - d);
 			}
 		}
 	}
@@ -844,38 +844,44 @@ __global__ __launch_bounds__(256) void rank_update_i8_kernel(const int8_t *__res
 
 // The same update for full 32 x 32 tiles of a 16-byte aligned C: each wave
 // takes TPW tiles side by side (one 32-row strip), loads them straight into
-// LDS first (global_load_lds_dwordx4, four loads of eight rows per tile; no
-// registers hold them, so the loads of all TPW tiles are in flight while the
-// MFMAs run), then per tile reads C in the accumulator layout, writes the
-// results back to LDS and stores them as 16-byte rows.  The G fragments are
-// loaded once per strip.  A wave touches only its own LDS tiles and its LDS
-// operations complete in order: no barriers.
-template <int TPW>
+// LDS first (global_load_lds_dwordx4: a 16-byte piece per lane, NQ loads of
+// RPL rows per tile; no registers hold them, so the loads of all TPW tiles are
+// in flight while the MFMAs run), then per tile reads C in the accumulator
+// layout, writes the results back to LDS and stores them as 16-byte pieces.
+// The G fragments are loaded once per strip.  A wave touches only its own LDS
+// tiles and its LDS operations complete in order: no barriers.  C holds
+// uint32 or (p <= 2^16) uint16 entries.
+template <int TPW, typename TC>
 __global__ __launch_bounds__(256) void rank_update_i8_lds_kernel(const int8_t *__restrict__ Gp,
 								 const int8_t *__restrict__ Wp, int KB,
-								 uint32_t *__restrict__ C, size_t ldc, int m, int sns,
+								 TC *__restrict__ C, size_t ldc, int m, int sns,
 								 ModP mp)
 {
-	__shared__ __attribute__((aligned(16))) uint32_t tile[4 * TPW * 32 * 32];
+	constexpr int EPL = 16 / (int) sizeof(TC);   // entries per 16-byte piece
+	constexpr int LPR = 32 / EPL;                // lanes per tile row
+	constexpr int RPL = 64 / LPR;                // tile rows per load
+	constexpr int NQ = 32 / RPL;                 // loads per tile
+	__shared__ __attribute__((aligned(16))) TC tile[4 * TPW * 32 * 32];
 	const int lane = threadIdx.x & 63;
 	const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const int wid = blockIdx.x * 4 + wv;
 	const int rb = wid / sns, sn = wid - rb * sns;
 	if (rb * 32 >= m)
 		return;
-	uint32_t *T = tile + wv * (TPW * 1024);
-	uint32_t *Cw = C + (size_t) (rb * 32) * ldc + sn * (32 * TPW);
+	TC *T = tile + wv * (TPW * 1024);
+	TC *Cw = C + (size_t) (rb * 32) * ldc + sn * (32 * TPW);
 	const int rows = min(32, m - rb * 32);     // a last partial row block: rows past m are
 						   // loaded from the last row and not stored
-	// tile j, load q: rows 8q .. 8q+7; lane l: row 8q + l/8, columns
-	// 32j + 4(l%8) .. +3, landing at T + 1024j + 256q + 4l (row-major tiles)
+	// tile j, load q: rows RPL q .. RPL q + RPL-1; lane l: row RPL q + l/LPR,
+	// columns 32j + EPL(l%LPR) .. +EPL-1, landing at T + 1024j + 32 RPL q + EPL l
+	// (row-major tiles)
 #pragma unroll
 	for (int j = 0; j < TPW; ++j)
 #pragma unroll
-		for (int q = 0; q < 4; ++q)
-			__builtin_amdgcn_global_load_lds((const MLZ_AS_GLOBAL void *) (Cw + (size_t) min(8 * q + (lane >> 3), rows - 1) *
-										  ldc + 32 * j + 4 * (lane & 7)),
-							 (MLZ_AS_LDS void *) (T + 1024 * j + 256 * q), 16, 0, 0);
+		for (int q = 0; q < NQ; ++q)
+			__builtin_amdgcn_global_load_lds((const MLZ_AS_GLOBAL void *) (Cw + (size_t) min(RPL * q + lane / LPR, rows - 1) *
+										  ldc + 32 * j + EPL * (lane % LPR)),
+							 (MLZ_AS_LDS void *) (T + 1024 * j + 32 * RPL * q), 16, 0, 0);
 	const size_t panel = (size_t) KB * 2 * I8_FRAG;
 	const int8_t *ga = Gp + (size_t) rb * panel + lane * 16;
 	const uint32_t p = (uint32_t) mp.p;
@@ -896,20 +902,20 @@ __global__ __launch_bounds__(256) void rank_update_i8_lds_kernel(const int8_t *_
 		// all of this wave's loads are done, the tiles among them
 		wait_vmcnt<0>();
 		asm volatile("" ::: "memory");
-		uint32_t *Tj = T + 1024 * j;
+		TC *Tj = T + 1024 * j;
 #pragma unroll
 		for (int r = 0; r < 16; ++r) {
 			const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
 			const double v = fma((double) acc0[r], 65536.0, fma((double) acc1[r], 256.0, (double) acc2[r]));
 			const uint32_t d = mp.mod_f64(v);
 			const uint32_t c = Tj[row * 32 + col];
-			Tj[row * 32 + col] = c >= d ? c - d : c + p - d;
+			Tj[row * 32 + col] = (TC) (c >= d ? c - d : c + p - d);
 		}
 #pragma unroll
-		for (int q = 0; q < 4; ++q)
-			if (8 * q + (lane >> 3) < rows)
-				*(uint4 *) (Cw + (size_t) (8 * q + (lane >> 3)) * ldc + 32 * j + 4 * (lane & 7)) =
-					*(const uint4 *) (Tj + 256 * q + 4 * lane);
+		for (int q = 0; q < NQ; ++q)
+			if (RPL * q + lane / LPR < rows)
+				*(uint4 *) (Cw + (size_t) (RPL * q + lane / LPR) * ldc + 32 * j + EPL * (lane % LPR)) =
+					*(const uint4 *) (Tj + 32 * RPL * q + EPL * lane);
 	}
 }
 
@@ -951,11 +957,13 @@ int zp_rank_update_i8_dev(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k
 	return zp_rank_update_i8_packed(ctx, p, m, n, KB, (const int8_t *) gp, (const int8_t *) wp, C, ldc);
 }
 
+template <typename TC>
 static int rank_update_i8_reg(mlz_ctx *ctx, const ModP &mp, size_t m, size_t n, int KB, const int8_t *gp,
-			      const int8_t *wp, uint32_t *C, size_t ldc, hipStream_t stream);
+			      const int8_t *wp, TC *C, size_t ldc, hipStream_t stream);
 
-int zp_rank_update_i8_packed(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, int KB, const int8_t *gp,
-			     const int8_t *wp, uint32_t *C, size_t ldc, hipStream_t stream)
+template <typename TC>
+static int rank_update_i8_packed(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, int KB, const int8_t *gp,
+				 const int8_t *wp, TC *C, size_t ldc, hipStream_t stream)
 {
 	if (!stream)
 		stream = ctx->stream;
@@ -966,16 +974,16 @@ int zp_rank_update_i8_packed(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, int K
 	// full tiles of an aligned C on the LDS-staged kernel, the edge strips below
 	// tiles per wave: two when the rows start on 128-byte boundaries, else one
 	// (measured); the context option fixes it
-	const int tpw = ctx->upd_lds >= 0 ? ctx->upd_lds : ((ldc & 31) == 0 ? 2 : 1);
+	const int tpw = ctx->upd_lds >= 0 ? ctx->upd_lds : ((ldc * sizeof(TC)) % 128 == 0 ? 2 : 1);
 	const size_t rbs = ceil_div(m, 32), sns = tpw > 0 ? n / (32 * (size_t) tpw) : 0, cbs = sns * (size_t) tpw;
-	if (tpw > 0 && sns && (ldc & 3) == 0 && ((uintptr_t) C & 15) == 0) {
+	if (tpw > 0 && sns && (ldc * sizeof(TC)) % 16 == 0 && ((uintptr_t) C & 15) == 0) {
 		const unsigned grid = ceil_div(rbs * sns, 4);
 		if (tpw == 1)
-			rank_update_i8_lds_kernel<1><<<grid, 256, 0, stream>>>(gp, wp, KB, C, ldc, (int) m, (int) sns, mp);
+			rank_update_i8_lds_kernel<1, TC><<<grid, 256, 0, stream>>>(gp, wp, KB, C, ldc, (int) m, (int) sns, mp);
 		else if (tpw == 2)
-			rank_update_i8_lds_kernel<2><<<grid, 256, 0, stream>>>(gp, wp, KB, C, ldc, (int) m, (int) sns, mp);
+			rank_update_i8_lds_kernel<2, TC><<<grid, 256, 0, stream>>>(gp, wp, KB, C, ldc, (int) m, (int) sns, mp);
 		else
-			rank_update_i8_lds_kernel<4><<<grid, 256, 0, stream>>>(gp, wp, KB, C, ldc, (int) m, (int) sns, mp);
+			rank_update_i8_lds_kernel<4, TC><<<grid, 256, 0, stream>>>(gp, wp, KB, C, ldc, (int) m, (int) sns, mp);
 		MLZ_TRY(check_launch(ctx, "rank_update_i8_lds_kernel"));
 		const size_t panel = (size_t) KB * 2 * I8_FRAG;
 		if (n > cbs * 32)        // the last columns
@@ -985,9 +993,22 @@ int zp_rank_update_i8_packed(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, int K
 	return rank_update_i8_reg(ctx, mp, m, n, KB, gp, wp, C, ldc, stream);
 }
 
+int zp_rank_update_i8_packed(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, int KB, const int8_t *gp,
+			     const int8_t *wp, uint32_t *C, size_t ldc, hipStream_t stream)
+{
+	return rank_update_i8_packed(ctx, p, m, n, KB, gp, wp, C, ldc, stream);
+}
+
+int zp_rank_update_i8_packed(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, int KB, const int8_t *gp,
+			     const int8_t *wp, uint16_t *C, size_t ldc, hipStream_t stream)
+{
+	return rank_update_i8_packed(ctx, p, m, n, KB, gp, wp, C, ldc, stream);
+}
+
 // The register-staged kernel over all of C.
+template <typename TC>
 static int rank_update_i8_reg(mlz_ctx *ctx, const ModP &mp, size_t m, size_t n, int KB, const int8_t *gp,
-			      const int8_t *wp, uint32_t *C, size_t ldc, hipStream_t stream)
+			      const int8_t *wp, TC *C, size_t ldc, hipStream_t stream)
 {
 	const size_t mpad = round_up(m, 64);
 	const int nblk = ctx->upd_nblk == 2 || ctx->upd_nblk == 4 ? ctx->upd_nblk : 1;
@@ -995,14 +1016,14 @@ static int rank_update_i8_reg(mlz_ctx *ctx, const ModP &mp, size_t m, size_t n, 
 	const size_t waves = (mpad / 32) * (size_t) strips_n;
 	const unsigned grid = ceil_div(waves, 4);
 	if (nblk == 1)
-		rank_update_i8_kernel<1><<<grid, 256, 0, stream>>>((const int8_t *) gp, (const int8_t *) wp, KB, C, ldc,
-								   (int) m, (int) n, mp, strips_n);
+		rank_update_i8_kernel<1, TC><<<grid, 256, 0, stream>>>((const int8_t *) gp, (const int8_t *) wp, KB, C, ldc,
+								       (int) m, (int) n, mp, strips_n);
 	else if (nblk == 4)
-		rank_update_i8_kernel<4><<<grid, 256, 0, stream>>>((const int8_t *) gp, (const int8_t *) wp, KB, C, ldc,
-								   (int) m, (int) n, mp, strips_n);
+		rank_update_i8_kernel<4, TC><<<grid, 256, 0, stream>>>((const int8_t *) gp, (const int8_t *) wp, KB, C, ldc,
+								       (int) m, (int) n, mp, strips_n);
 	else
-		rank_update_i8_kernel<2><<<grid, 256, 0, stream>>>((const int8_t *) gp, (const int8_t *) wp, KB, C, ldc,
-								   (int) m, (int) n, mp, strips_n);
+		rank_update_i8_kernel<2, TC><<<grid, 256, 0, stream>>>((const int8_t *) gp, (const int8_t *) wp, KB, C, ldc,
+								       (int) m, (int) n, mp, strips_n);
 	return check_launch(ctx, "rank_update_i8_kernel");
 }
 

@@ -88,8 +88,8 @@ struct LuPanel {
 // multipliers is replayed afterwards.  Then L11^-1 and P_J^-1 = U11^-1 L11^-1
 // come from a blocked substitution over 8 x 8 blocks with all threads: the
 // diagonal blocks are inverted first, then each block row is one product.
-template <bool SMALL, bool TAB>
-__global__ __launch_bounds__(LU_NT) void lu_window_kernel(const uint32_t *__restrict__ X, size_t ldx, int m, int c,
+template <bool SMALL, bool TAB, typename TX>
+__global__ __launch_bounds__(LU_NT) void lu_window_kernel(const TX *__restrict__ X, size_t ldx, int m, int c,
 							  int wc, int i0, ModP mp, const uint16_t *__restrict__ invtab,
 							  LuPanel *__restrict__ out, uint32_t *__restrict__ L11,
 							  uint32_t *__restrict__ LiM)
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(LU_NT) void lu_window_kernel(const uint32_t *__rest
 #pragma unroll
 	for (int jj = 0; jj < 8; ++jj) {
 		const int col = 8 * w + jj;
-		x[jj] = (l < kw && col < wc) ? X[(size_t) (i0 + l) * ldx + c + col] : 0u;
+		x[jj] = (l < kw && col < wc) ? (uint32_t) X[(size_t) (i0 + l) * ldx + c + col] : 0u;
 	}
 	if (tid < LU_K) {
 		ord[tid] = tid;
@@ -201,7 +201,7 @@ __global__ __launch_bounds__(LU_NT) void lu_window_kernel(const uint32_t *__rest
 		if (sfail) {             // start over from the window as loaded
 #pragma unroll
 			for (int jj = 0; jj < 8; ++jj)
-				x[jj] = X[(size_t) (i0 + l) * ldx + c + 8 * w + jj];
+				x[jj] = (uint32_t) X[(size_t) (i0 + l) * ldx + c + 8 * w + jj];
 			__syncthreads();     // every wave is past its reads of the fast path's multipliers
 		} else {
 			cur = LU_K;
@@ -540,8 +540,8 @@ __global__ void lu_gather_mult_kernel(const uint32_t *__restrict__ X, size_t ldx
 // back in that order, and packs W = P_J^-1 P as the update's INT8 B fragments
 // (KB = 2).  Every read of a slice precedes its writes, and slices are
 // disjoint, so the reordering is done in place.
-template <int TC>
-__global__ __launch_bounds__(256) void lu_wb_i8_kernel(uint32_t *__restrict__ X, size_t ldx, int nc, int c, int i0,
+template <int TC, typename TX>
+__global__ __launch_bounds__(256) void lu_wb_i8_kernel(TX *__restrict__ X, size_t ldx, int nc, int c, int i0,
 						       int kw, const LuPanel *__restrict__ st,
 						       const uint32_t *__restrict__ LiM, int8_t *__restrict__ Wp, ModP mp,
 						       int cb0)
@@ -575,7 +575,7 @@ __global__ __launch_bounds__(256) void lu_wb_i8_kernel(uint32_t *__restrict__ X,
 	__syncthreads();
 	for (int e = tid; e < LU_K * TC; e += 256) {
 		const int q = e / TC, col = e % TC, gc = j0 + col;
-		Ps[q][col] = (q < kw && gc < nc) ? X[(size_t) (i0 + wq[q]) * ldx + c + gc] : 0u;
+		Ps[q][col] = (q < kw && gc < nc) ? (uint32_t) X[(size_t) (i0 + wq[q]) * ldx + c + gc] : 0u;
 	}
 	__syncthreads();
 	// wave w: output rows 32w .. 32w+31 as two 16-row blocks, TC/16 column
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(256) void lu_wb_i8_kernel(uint32_t *__restrict__ X,
 	for (int e = tid; e < LU_K * TC; e += 256) {
 		const int q = e / TC, col = e % TC, gc = j0 + col;
 		if (q < kw && gc < nc)
-			X[(size_t) (i0 + q) * ldx + c + gc] = q < rp ? Os[q][col] : Ps[q][col];
+			X[(size_t) (i0 + q) * ldx + c + gc] = (TX) (q < rp ? Os[q][col] : Ps[q][col]);
 	}
 	// B fragments: piece (column block, k block q, lane) holds column
 	// 32 cb + lane%32 and k = 32q + 16(lane/32) .. +15
@@ -637,7 +637,8 @@ __global__ __launch_bounds__(256) void lu_wb_i8_kernel(uint32_t *__restrict__ X,
 // zero on the new pivot rows (r < rp), packed as the update's INT8 A fragments
 // (KB = 2).  Thread: the 16-byte piece (row block, k block, lane) of one
 // workgroup's 64 rows.
-__global__ __launch_bounds__(256) void lu_gpack_i8_kernel(const uint32_t *__restrict__ X, size_t ldx, int mb, int i0,
+template <typename TX>
+__global__ __launch_bounds__(256) void lu_gpack_i8_kernel(const TX *__restrict__ X, size_t ldx, int mb, int i0,
 							  int c, const LuPanel *__restrict__ st, int8_t *__restrict__ Gp,
 							  uint32_t p)
 {
@@ -645,24 +646,25 @@ __global__ __launch_bounds__(256) void lu_gpack_i8_kernel(const uint32_t *__rest
 	const int rp = st->rp;
 	const int r = 32 * rb + (lane & 31), s0 = 32 * q + 16 * (lane >> 5);
 	const bool live = r < mb && r >= rp;
-	const uint32_t *row = X + (size_t) (i0 + r) * ldx;
+	const TX *row = X + (size_t) (i0 + r) * ldx;
 	uint32_t xv[16];
-	// a full-rank panel has J_t = c + t: sixteen consecutive entries, four
-	// 16-byte loads when the row stride and c keep them aligned
-	if (rp == LU_W && st->J[LU_W - 1] == c + LU_W - 1 && (ldx & 3) == 0 && (c & 3) == 0) {
+	// a full-rank panel has J_t = c + t: sixteen consecutive entries, 16-byte
+	// loads when the row stride and c keep them aligned
+	constexpr int EPL = 16 / (int) sizeof(TX);
+	if (rp == LU_W && st->J[LU_W - 1] == c + LU_W - 1 && (ldx % EPL) == 0 && (c % EPL) == 0) {
 		const uint4 *src = (const uint4 *) (row + c + s0);
 #pragma unroll
-		for (int qq = 0; qq < 4; ++qq) {
+		for (int qq = 0; qq < 16 / EPL; ++qq) {
 			const uint4 u = live ? src[qq] : make_uint4(0u, 0u, 0u, 0u);
-			xv[4 * qq] = u.x;
-			xv[4 * qq + 1] = u.y;
-			xv[4 * qq + 2] = u.z;
-			xv[4 * qq + 3] = u.w;
+			const uint32_t wv[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+			for (int e = 0; e < EPL; ++e)
+				xv[EPL * qq + e] = EPL == 4 ? wv[e] : (wv[e >> 1] >> (16 * (e & 1))) & 0xffffu;
 		}
 	} else {
 #pragma unroll
 		for (int b = 0; b < 16; ++b)
-			xv[b] = (live && s0 + b < rp) ? row[st->J[s0 + b]] : 0u;
+			xv[b] = (live && s0 + b < rp) ? (uint32_t) row[st->J[s0 + b]] : 0u;
 	}
 	uint32_t hw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
 #pragma unroll
@@ -677,14 +679,15 @@ __global__ __launch_bounds__(256) void lu_gpack_i8_kernel(const uint32_t *__rest
 }
 
 // First nonzero at positions >= i0, columns [c0, c1): key = col << 32 | pos.
-__global__ void lu_search_kernel(const uint32_t *__restrict__ X, size_t ldx, int m, int i0, int c0, int c1,
+template <typename TX>
+__global__ void lu_search_kernel(const TX *__restrict__ X, size_t ldx, int m, int i0, int c0, int c1,
 				 unsigned long long *__restrict__ key)
 {
 	const int col = c0 + blockIdx.x * blockDim.x + threadIdx.x;
 	if (col >= c1)
 		return;
 	for (int pos = i0 + blockIdx.y; pos < m; pos += gridDim.y) {
-		if (X[(size_t) pos * ldx + col] != 0u) {
+		if (X[(size_t) pos * ldx + col] != 0) {
 			atomicMin(key, ((unsigned long long) col << 32) | (unsigned) pos);
 			return;
 		}
@@ -715,22 +718,22 @@ __global__ void lu_iota_kernel(int *orig, int m)
 // Working copy: canonical residues of A in columns 0 .. n-1, for compute_L the
 // identity in columns n .. n+m-1.
 // Columns ntot .. ldx-1 (the padding of a row) are zero.
-template <int E>
+template <int E, typename TX>
 __global__ void lu_load_kernel(const typename Elem<E>::T *__restrict__ A, size_t lda, int m, int n, int ntot, uint32_t p,
-			       uint32_t *__restrict__ X, size_t ldx)
+			       TX *__restrict__ X, size_t ldx)
 {
 	const int col = blockIdx.x * blockDim.x + threadIdx.x, row = blockIdx.y;
 	if ((size_t) col >= ldx)
 		return;
-	X[(size_t) row * ldx + col] = col < n ? Elem<E>::load(A + (size_t) row * lda + col, p)
-					       : (col - n == row && col < ntot ? 1u : 0u);
+	X[(size_t) row * ldx + col] = (TX) (col < n ? Elem<E>::load(A + (size_t) row * lda + col, p)
+						    : (col - n == row && col < ntot ? 1u : 0u));
 }
 
 // Output row t.  mode U: the working copy.  compute_L: U from the diagonal
 // on, T[t][s] = X[t][n + orig[s]] left of it.  pluq: L left of the diagonal
 // (first `rank` columns), U in the column order corder.
-template <int E>
-__global__ void lu_store_kernel(const uint32_t *__restrict__ X, size_t ldx, int m, int n, int rank, int mode,
+template <int E, typename TX>
+__global__ void lu_store_kernel(const TX *__restrict__ X, size_t ldx, int m, int n, int rank, int mode,
 				const int *__restrict__ orig, const int *__restrict__ corder,
 				const uint32_t *__restrict__ L, size_t ldl, uint32_t p, typename Elem<E>::T *__restrict__ A,
 				size_t lda)
@@ -749,9 +752,13 @@ __global__ void lu_store_kernel(const uint32_t *__restrict__ X, size_t ldx, int 
 	Elem<E>::store(A + (size_t) t * lda + col, v, p);
 }
 
-int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A, size_t lda, int mode,
-		   size_t *rank_out, uint64_t *det_out, uint32_t *perm_rows, uint32_t *perm_cols, int algo, bool store,
-		   uint32_t *pivcols)
+// TX: the working copy's entry type.  uint16 serves the fused path (p <= 2^16,
+// INT8 update, no pluq) and halves the bytes each panel's update moves; every
+// other path keeps uint32.
+template <typename TX>
+static int echelon_impl(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A, size_t lda, int mode,
+			size_t *rank_out, uint64_t *det_out, uint32_t *perm_rows, uint32_t *perm_cols, int algo,
+			bool store, uint32_t *pivcols)
 {
 	if (m == 0 || n == 0) {
 		if (rank_out)
@@ -765,8 +772,8 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 	const ModP mp = make_modp(p);
 	const uint32_t pp = (uint32_t) p;
 	const size_t ntot = mode == MLZ_ECH_TRANSFORM ? n + m : n;
-	// rows of the working copy start on 256-byte boundaries; the columns past
-	// ntot are zero, so the packed update can run over whole 64-column strips
+	// rows of the working copy start on whole 64-entry boundaries; the columns
+	// past ntot are zero, so the packed update can run over whole 64-column strips
 	const size_t ldx = round_up(ntot, 64);
 	const size_t rmax = std::min(m, n);
 	const bool pluq = mode == MLZ_ECH_PLUQ;
@@ -774,7 +781,7 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 	// UW (2 LU_W x ntot: the new pivot rows, then W = P_J^-1 P); G (m x LU_W)
 	// and for pluq Lb (m x rmax); L11, LiM (2 LU_W x LU_W), Ui, orig, corder, state
 	void *w0, *w1, *w2, *w3;
-	MLZ_TRY(ws_get(ctx, WS_RREF_0, m * ldx * sizeof(uint32_t), &w0));
+	MLZ_TRY(ws_get(ctx, WS_RREF_0, m * ldx * sizeof(TX), &w0));
 	const size_t sXw = round_up(LU_K * std::max(ntot, rmax) * 4, 256), sUW = round_up(2 * LU_W * ntot * 4, 256);
 	MLZ_TRY(ws_get(ctx, WS_RREF_1, sXw + sUW, &w1));
 	const size_t sG = round_up(m * LU_W * 4, 256);
@@ -782,7 +789,7 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 	const size_t sq = round_up(LU_W * LU_W * 4, 256);
 	const size_t small = 6 * sq + round_up(m * 4, 256) + round_up(n * 4, 256) + 2 * round_up(sizeof(LuPanel), 256) + 256;
 	MLZ_TRY(ws_get(ctx, WS_RREF_3, small, &w3));
-	uint32_t *X = (uint32_t *) w0;
+	TX *X = (TX *) w0;
 	uint32_t *Xw = (uint32_t *) w1, *UW = (uint32_t *) ((char *) w1 + sXw);
 	uint32_t *G = (uint32_t *) w2, *Lb = (uint32_t *) ((char *) w2 + sG);
 	char *q = (char *) w3;
@@ -810,13 +817,13 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 		lu_iota_kernel<<<ceil_div(m, 256), 256, 0, ctx->stream>>>(orig, (int) m);
 		dim3 g(ceil_div(ldx, 256), (unsigned) m);
 		if (elem == MLZ_U16)
-			lu_load_kernel<MLZ_U16><<<g, 256, 0, ctx->stream>>>((const uint16_t *) A, lda, (int) m, (int) n,
+			lu_load_kernel<MLZ_U16, TX><<<g, 256, 0, ctx->stream>>>((const uint16_t *) A, lda, (int) m, (int) n,
 									     (int) ntot, pp, X, ldx);
 		else if (elem == MLZ_U32)
-			lu_load_kernel<MLZ_U32><<<g, 256, 0, ctx->stream>>>((const uint32_t *) A, lda, (int) m, (int) n,
+			lu_load_kernel<MLZ_U32, TX><<<g, 256, 0, ctx->stream>>>((const uint32_t *) A, lda, (int) m, (int) n,
 									     (int) ntot, pp, X, ldx);
 		else
-			lu_load_kernel<MLZ_F64_SYM><<<g, 256, 0, ctx->stream>>>((const double *) A, lda, (int) m, (int) n,
+			lu_load_kernel<MLZ_F64_SYM, TX><<<g, 256, 0, ctx->stream>>>((const double *) A, lda, (int) m, (int) n,
 										 (int) ntot, pp, X, ldx);
 		MLZ_TRY(check_launch(ctx, "lu_load_kernel"));
 	}
@@ -856,7 +863,7 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 	MLZ_TRY(copy_stream(ctx, &cs));
 	res.cs = cs;
 	auto swap_rows = [&](int a, int b) -> int {
-		lu_swap_rows_kernel<uint32_t><<<ceil_div(ntot, 256), 256, 0, ctx->stream>>>(X, ldx, (int) ntot, a, b);
+		lu_swap_rows_kernel<TX><<<ceil_div(ntot, 256), 256, 0, ctx->stream>>>(X, ldx, (int) ntot, a, b);
 		lu_swap_rows_kernel<int><<<1, 1, 0, ctx->stream>>>(orig, 1, 1, a, b);
 		if (pluq)
 			lu_swap_rows_kernel<uint32_t><<<ceil_div(rmax, 256), 256, 0, ctx->stream>>>(Lb, rmax, (int) rmax, a, b);
@@ -929,7 +936,7 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 			ProfScope ps(ctx, "lu_update");
 			if (mode == MLZ_ECH_TRANSFORM)   // the row origins matter for T only
 				lu_window_orig_kernel<<<1, LU_K, 0, ctx->stream>>>(orig, (int) i, kw, st);
-			lu_wb_i8_kernel<32><<<ceil_div(nn, 32), 256, 0, ctx->stream>>>(X, ldx, (int) nc, (int) c, (int) i, kw,
+			lu_wb_i8_kernel<32, TX><<<ceil_div(nn, 32), 256, 0, ctx->stream>>>(X, ldx, (int) nc, (int) c, (int) i, kw,
 											    st, LiM, Wpk[b], mp, 0);
 			lu_gpack_i8_kernel<<<ceil_div(mb, 64), 256, 0, ctx->stream>>>(X, ldx, (int) mb, (int) i, (int) c, st,
 											Gpk[b], pp);
@@ -938,7 +945,7 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 			if (wide) {
 				MLZ_HIP(ctx, hipEventRecord(ov.nev[b], ctx->stream));
 				MLZ_HIP(ctx, hipStreamWaitEvent(ov.s1, ov.nev[b], 0));
-				lu_wb_i8_kernel<32><<<ceil_div(nc, 32) - nbn, 256, 0, ov.s1>>>(X, ldx, (int) nc, (int) c, (int) i, kw,
+				lu_wb_i8_kernel<32, TX><<<ceil_div(nc, 32) - nbn, 256, 0, ov.s1>>>(X, ldx, (int) nc, (int) c, (int) i, kw,
 												    st, LiM, Wpk[b], mp, nbn);
 				MLZ_TRY(zp_rank_update_i8_packed(ctx, pp, mb, nc - nn, 2, Gpk[b], Wpk[b] + (size_t) nbn * 4096,
 								 X + i * ldx + c + nn, ldx, ov.s1));
@@ -950,54 +957,60 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 		return MLZ_OK;
 	};
 	auto update = [&](int kw, int rp) -> int {
-		const size_t nc = ntot - c;
-		const size_t rb = rp < 0 ? i : i + (size_t) rp, mb = m - rb;
-		const size_t kk = rp < 0 ? (size_t) kw : (size_t) rp;
-		ProfScope ps(ctx, "lu_update");
-		// the window rows in their new order (the working copy and, for pluq, the multipliers)
-		lu_window_gather_kernel<uint32_t><<<dim3(ceil_div(ntot, 256), kw), 256, 0, ctx->stream>>>(
-			X, ldx, (int) ntot, (int) i, st, Xw);
-		lu_window_put_kernel<uint32_t><<<dim3(ceil_div(ntot, 256), kw), 256, 0, ctx->stream>>>(
-			X, ldx, (int) ntot, (int) i, Xw);
-		if (mode == MLZ_ECH_TRANSFORM)
-			lu_window_orig_kernel<<<1, LU_K, 0, ctx->stream>>>(orig, (int) i, kw, st);
-		if (pluq && i > 0) {
-			lu_window_gather_kernel<uint32_t><<<dim3(ceil_div(rmax, 256), kw), 256, 0, ctx->stream>>>(
-				Lb, rmax, (int) rmax, (int) i, st, Xw);
-			lu_window_put_kernel<uint32_t><<<dim3(ceil_div(rmax, 256), kw), 256, 0, ctx->stream>>>(
-				Lb, rmax, (int) rmax, (int) i, Xw);
-		}
-		MLZ_TRY(check_launch(ctx, "lu_window_put_kernel"));
-		// UW = [L11^-1; P_J^-1] P: the new pivot rows and W
-		MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, 2 * LU_W, nc, kk, 1, LiM, LU_W, X + i * ldx + c, ldx, 0, UW, nc,
-				    algo));
-		if (mb) {
-			lu_gather_mult_kernel<<<ceil_div(mb, 256 / LU_W), 256, 0, ctx->stream>>>(X, ldx, (int) mb, (int) rb,
-												   (int) i, st, G);
-			MLZ_TRY(check_launch(ctx, "lu_gather_mult_kernel"));
-		}
-		if (pluq) {
-			if (mb) {
-				// the rows below: x U11 = X_j[J], U11^-1 = P_J^-1 L11
-				MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, LU_W, LU_W, LU_W, 1, LiM + LU_W * LU_W, LU_W, L11, LU_W, 0,
-						    Ui, LU_W, algo));
-				MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, mb, kk, kk, 1, G, LU_W, Ui, LU_W, 0, Lb + rb * rmax + i,
-						    rmax, algo));
+		if constexpr (sizeof(TX) != sizeof(uint32_t)) {   // not reached: narrow copies are fused
+			(void) kw;
+			(void) rp;
+			return set_error(ctx, MLZ_E_ARG, "lu: the general update needs a uint32 working copy");
+		} else {
+			const size_t nc = ntot - c;
+			const size_t rb = rp < 0 ? i : i + (size_t) rp, mb = m - rb;
+			const size_t kk = rp < 0 ? (size_t) kw : (size_t) rp;
+			ProfScope ps(ctx, "lu_update");
+			// the window rows in their new order (the working copy and, for pluq, the multipliers)
+			lu_window_gather_kernel<uint32_t><<<dim3(ceil_div(ntot, 256), kw), 256, 0, ctx->stream>>>(
+				X, ldx, (int) ntot, (int) i, st, Xw);
+			lu_window_put_kernel<uint32_t><<<dim3(ceil_div(ntot, 256), kw), 256, 0, ctx->stream>>>(
+				X, ldx, (int) ntot, (int) i, Xw);
+			if (mode == MLZ_ECH_TRANSFORM)
+				lu_window_orig_kernel<<<1, LU_K, 0, ctx->stream>>>(orig, (int) i, kw, st);
+			if (pluq && i > 0) {
+				lu_window_gather_kernel<uint32_t><<<dim3(ceil_div(rmax, 256), kw), 256, 0, ctx->stream>>>(
+					Lb, rmax, (int) rmax, (int) i, st, Xw);
+				lu_window_put_kernel<uint32_t><<<dim3(ceil_div(rmax, 256), kw), 256, 0, ctx->stream>>>(
+					Lb, rmax, (int) rmax, (int) i, Xw);
 			}
-			lu_pivot_mult_kernel<<<LU_W, LU_W, 0, ctx->stream>>>(L11, (int) i, st, Lb, rmax);
-			MLZ_TRY(check_launch(ctx, "lu_pivot_mult_kernel"));
+			MLZ_TRY(check_launch(ctx, "lu_window_put_kernel"));
+			// UW = [L11^-1; P_J^-1] P: the new pivot rows and W
+			MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, 2 * LU_W, nc, kk, 1, LiM, LU_W, X + i * ldx + c, ldx, 0, UW, nc,
+					    algo));
+			if (mb) {
+				lu_gather_mult_kernel<<<ceil_div(mb, 256 / LU_W), 256, 0, ctx->stream>>>(X, ldx, (int) mb, (int) rb,
+													   (int) i, st, G);
+				MLZ_TRY(check_launch(ctx, "lu_gather_mult_kernel"));
+			}
+			if (pluq) {
+				if (mb) {
+					// the rows below: x U11 = X_j[J], U11^-1 = P_J^-1 L11
+					MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, LU_W, LU_W, LU_W, 1, LiM + LU_W * LU_W, LU_W, L11, LU_W, 0,
+							    Ui, LU_W, algo));
+					MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, mb, kk, kk, 1, G, LU_W, Ui, LU_W, 0, Lb + rb * rmax + i,
+							    rmax, algo));
+				}
+				lu_pivot_mult_kernel<<<LU_W, LU_W, 0, ctx->stream>>>(L11, (int) i, st, Lb, rmax);
+				MLZ_TRY(check_launch(ctx, "lu_pivot_mult_kernel"));
+			}
+			if (mb) {
+				uint32_t *Xb = X + rb * ldx + c;
+				const uint32_t *Wp = UW + (size_t) LU_W * nc;
+				if (upd_i8)
+					MLZ_TRY(zp_rank_update_i8_dev(ctx, pp, mb, nc, LU_W, G, LU_W, Wp, nc, Xb, ldx));
+				else
+					MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, mb, nc, LU_W, p - 1, G, LU_W, Wp, nc, 1, Xb, ldx, algo));
+			}
+			lu_scatter_pivots_kernel<<<dim3(ceil_div(nc, 256), LU_W), 256, 0, ctx->stream>>>(X, ldx, (int) c, (int) nc,
+													(int) i, st, UW);
+			return check_launch(ctx, "lu_scatter_pivots_kernel");
 		}
-		if (mb) {
-			uint32_t *Xb = X + rb * ldx + c;
-			const uint32_t *Wp = UW + (size_t) LU_W * nc;
-			if (upd_i8)
-				MLZ_TRY(zp_rank_update_i8_dev(ctx, pp, mb, nc, LU_W, G, LU_W, Wp, nc, Xb, ldx));
-			else
-				MLZ_TRY(zp_gemm_dev(ctx, MLZ_U32, p, mb, nc, LU_W, p - 1, G, LU_W, Wp, nc, 1, Xb, ldx, algo));
-		}
-		lu_scatter_pivots_kernel<<<dim3(ceil_div(nc, 256), LU_W), 256, 0, ctx->stream>>>(X, ldx, (int) c, (int) nc,
-												(int) i, st, UW);
-		return check_launch(ctx, "lu_scatter_pivots_kernel");
 	};
 	int panel = 0;
 	while (i < m && c < n) {
@@ -1009,15 +1022,18 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 		LiM = LiMb[pb];
 		{
 			ProfScope ps(ctx, "lu_window");
-			if (invtab)
-				lu_window_kernel<true, true><<<1, LU_NT, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, (int) i,
-											    mp, invtab, st, L11, LiM);
+			if constexpr (sizeof(TX) != sizeof(uint32_t))   // p <= 2^16: the table is there
+				lu_window_kernel<true, true, TX><<<1, LU_NT, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc,
+												(int) i, mp, invtab, st, L11, LiM);
+			else if (invtab)
+				lu_window_kernel<true, true, TX><<<1, LU_NT, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc,
+												(int) i, mp, invtab, st, L11, LiM);
 			else if (mp.small)
-				lu_window_kernel<true, false><<<1, LU_NT, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc, (int) i,
-											     mp, invtab, st, L11, LiM);
+				lu_window_kernel<true, false, TX><<<1, LU_NT, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc,
+												 (int) i, mp, invtab, st, L11, LiM);
 			else
-				lu_window_kernel<false, false><<<1, LU_NT, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc,
-											      (int) i, mp, invtab, st, L11, LiM);
+				lu_window_kernel<false, false, TX><<<1, LU_NT, 0, ctx->stream>>>(X, ldx, (int) m, (int) c, wc,
+												  (int) i, mp, invtab, st, L11, LiM);
 		}
 		MLZ_TRY(check_launch(ctx, "lu_window_kernel"));
 		MLZ_HIP(ctx, hipEventRecord(res.wev, ctx->stream));
@@ -1109,7 +1125,7 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 	if (store) {
 		ProfScope ps(ctx, "lu_store");
 		dim3 g(ceil_div(n, 256), (unsigned) m);
-#define STORE(EE, TT) lu_store_kernel<EE><<<g, 256, 0, ctx->stream>>>(X, ldx, (int) m, (int) n, (int) rank, mode, \
+#define STORE(EE, TT) lu_store_kernel<EE, TX><<<g, 256, 0, ctx->stream>>>(X, ldx, (int) m, (int) n, (int) rank, mode, \
 								     orig, corder, Lb, rmax, pp, (TT *) A, lda)
 		if (elem == MLZ_U16)
 			STORE(MLZ_U16, uint16_t);
@@ -1126,6 +1142,17 @@ int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void 
 	if (det_out)
 		*det_out = det;
 	return MLZ_OK;
+}
+
+int zp_echelon_dev(mlz_ctx *ctx, int elem, uint64_t p, size_t m, size_t n, void *A, size_t lda, int mode,
+		   size_t *rank_out, uint64_t *det_out, uint32_t *perm_rows, uint32_t *perm_cols, int algo, bool store,
+		   uint32_t *pivcols)
+{
+	const bool fused = p <= 65536 && (algo == MLZ_ALGO_AUTO || algo == MLZ_ALGO_INT8) && mode != MLZ_ECH_PLUQ;
+	return fused ? echelon_impl<uint16_t>(ctx, elem, p, m, n, A, lda, mode, rank_out, det_out, perm_rows, perm_cols,
+					      algo, store, pivcols)
+		     : echelon_impl<uint32_t>(ctx, elem, p, m, n, A, lda, mode, rank_out, det_out, perm_rows, perm_cols,
+					      algo, store, pivcols);
 }
 
 }  // namespace mlz
