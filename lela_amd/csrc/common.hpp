@@ -439,11 +439,24 @@ int pick_gemm_algo(uint64_t p, int elem, size_t m, size_t n, size_t k, int algo)
 // batch's offsets from A, B and C (element storage E, device pointers).
 // elemA: A's storage (any of the three; elem is B's and C's).
 // Apre (may be null): A already packed by zp_pack_i8_a_dev (then A is unused).
+// Packed operands across products (one product, k <= I8_KMAX): Bpre, B
+// already packed with its 32-column blocks bpanel bytes apart (a k-range of a
+// larger packed operand: the pointer is offset by 2048 bytes per 32 rows of
+// k); Cpk, C also written in that layout with blocks cpanel bytes apart (m, n
+// multiples of 32).  B's columns up to the tile padding must exist in Bpre.
+struct I8PackIO {
+	const int8_t *Bpre = nullptr;
+	size_t bpanel = 0;
+	int8_t *Cpk = nullptr;
+	size_t cpanel = 0;
+};
 int zp_gemm_i8_batched_dev(mlz_ctx *ctx, int elem, int elemA, uint32_t p, const I8Batch &bt, size_t m, size_t n,
 			   size_t k, uint32_t a, const void *A, size_t lda, const void *B, size_t ldb, uint32_t b,
-			   void *C, size_t ldc, const int8_t *Apre = nullptr);
+			   void *C, size_t ldc, const int8_t *Apre = nullptr, const I8PackIO &io = I8PackIO());
 int zp_pack_i8_a_dev(mlz_ctx *ctx, int elem, uint32_t p, const I8Batch &bt, size_t rows, size_t k, const void *A,
 		     size_t lda, int8_t *dst, size_t *item_bytes, hipStream_t stream = nullptr);
+int zp_pack_i8_b_dev(mlz_ctx *ctx, int elem, uint32_t p, size_t k, size_t cols, const void *B, size_t ldb,
+		     int8_t *dst, hipStream_t stream = nullptr);
 int zp_rank_update_i8_dev(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, const uint32_t *G, size_t ldg,
 			  const uint32_t *W, size_t ldw, uint32_t *C, size_t ldc);
 // The same update with G (m x 32*KB) and W (32*KB x n) already packed as INT8

@@ -226,12 +226,54 @@ __device__ __forceinline__ void i8_stage_mfma(const int8_t *la, const int8_t *lb
 	}
 }
 
+// The 32 x 32 block a wave holds (lane l, register r: row 8*(r/4) +
+// 4*(l/32) + r%4, column l%32) written as the B operand fragments of a later
+// product whose k runs along C's rows: fragment (column block, row block)
+// lane L holds column L%32, rows 16*(L/32) .. +15.  Lanes l and l^32 swap
+// half of their rows (one exchange of 8 values), then each lane writes its
+// two limb words as 16 bytes each.
+__device__ __forceinline__ void i8_pack_out(const uint32_t (&ov)[16], uint32_t p, int lane, int8_t *__restrict__ fr)
+{
+	const bool hi = lane >= 32;
+	uint32_t rv[8];
+#pragma unroll
+	for (int r = 0; r < 8; ++r)
+		rv[r] = (uint32_t) __shfl_xor((int) (hi ? ov[r] : ov[r + 8]), 32, 64);
+	// rows 16*(l/32) + 4q .. +3 for q = 0..3
+	uint32_t v[16];
+#pragma unroll
+	for (int t = 0; t < 4; ++t) {
+		v[t] = hi ? rv[t] : ov[t];
+		v[4 + t] = hi ? ov[8 + t] : rv[t];
+		v[8 + t] = hi ? rv[4 + t] : ov[4 + t];
+		v[12 + t] = hi ? ov[12 + t] : rv[4 + t];
+	}
+	uint32_t hw[4], lw[4];
+#pragma unroll
+	for (int q = 0; q < 4; ++q) {
+		uint32_t h = 0, l = 0;
+#pragma unroll
+		for (int t = 0; t < 4; ++t) {
+			const uint32_t w = i8_limbs(v[4 * q + t], p);
+			h |= ((w >> 8) & 0xffu) << (8 * t);
+			l |= (w & 0xffu) << (8 * t);
+		}
+		hw[q] = h;
+		lw[q] = l;
+	}
+	*(v4i *) (fr + lane * 16) = v4i{(int) hw[0], (int) hw[1], (int) hw[2], (int) hw[3]};
+	*(v4i *) (fr + I8_FRAG + lane * 16) = v4i{(int) lw[0], (int) lw[1], (int) lw[2], (int) lw[3]};
+}
+
 // Epilogue of a workgroup tile.  32x32 accumulator layout: lane l,
-// register r holds row 8*(r/4) + 4*(l/32) + r%4, column l%32.
+// register r holds row 8*(r/4) + 4*(l/32) + r%4, column l%32.  With Cpk the
+// results are also written as packed B fragments (i8_pack_out), 32-column
+// blocks panelC bytes apart; the caller guarantees whole 32 x 32 blocks.
 template <int E, class T, int WTM, int WTN>
 __device__ __forceinline__ void i8_epilogue(const v16i (&acc)[3][WTM][WTN], typename Elem<E>::T *__restrict__ C,
 					    size_t ldc, int m, int n, uint32_t a, uint32_t b, const ModP &mp, int tm,
-					    int tn, int wr, int wc, int lane)
+					    int tn, int wr, int wc, int lane, int8_t *__restrict__ Cpk = nullptr,
+					    size_t panelC = 0)
 {
 	const uint32_t p = (uint32_t) mp.p;
 #pragma unroll
@@ -250,6 +292,7 @@ __device__ __forceinline__ void i8_epilogue(const v16i (&acc)[3][WTM][WTN], type
 					for (int r = 0; r < 16; ++r)
 						cv[r] = Elem<E>::load(cb + (size_t) (8 * (r >> 2) + (r & 3)) * ldc, p);
 				}
+				uint32_t ov[16];
 #pragma unroll
 				for (int r = 0; r < 16; ++r) {
 					const double v = fma((double) acc[0][i][j][r], 65536.0,
@@ -258,8 +301,12 @@ __device__ __forceinline__ void i8_epilogue(const v16i (&acc)[3][WTM][WTN], type
 					double t = a == 1 ? v : (double) a * (double) mp.mod_f64(v);
 					if (b)
 						t = fma((double) b, (double) cv[r], t);
-					Elem<E>::store(cb + (size_t) (8 * (r >> 2) + (r & 3)) * ldc, mp.mod_f64(t), p);
+					ov[r] = mp.mod_f64(t);
+					Elem<E>::store(cb + (size_t) (8 * (r >> 2) + (r & 3)) * ldc, ov[r], p);
 				}
+				if (Cpk)
+					i8_pack_out(ov, p, lane,
+						    Cpk + (size_t) ((col - (lane & 31)) >> 5) * panelC + (size_t) (row0 >> 5) * 2 * I8_FRAG);
 				continue;
 			}
 #pragma unroll
@@ -393,7 +440,8 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_async_kernel(const 
 								 const int8_t *__restrict__ Bp, int KB,
 								 typename Elem<E>::T *__restrict__ C, size_t ldc,
 								 int m, int n, uint32_t a, uint32_t b, ModP mp,
-								 int tiles_m, int tiles_n, I8Batch bt, size_t sAp, size_t sBp)
+								 int tiles_m, int tiles_n, I8Batch bt, size_t sAp, size_t sBp,
+								 size_t panelB, int8_t *__restrict__ Cpk, size_t panelC)
 {
 	typedef I8Tile<WTM, WTN, KK, WM, WN> T;
 	{
@@ -411,15 +459,17 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_async_kernel(const 
 	const int wr = wave / WN, wc = wave % WN;
 	const int wave_off = __builtin_amdgcn_readfirstlane(wave) * 1024;
 
+	// A's 32-row blocks are panel bytes apart; B's 32-column blocks panelB
+	// (more than panel when B is a k-range of a larger packed operand)
 	const size_t panel = (size_t) KB * 2 * I8_FRAG;
 	const int8_t *Ag = Ap + (size_t) (tm * T::RBA) * panel;
-	const int8_t *Bg = Bp + (size_t) (tn * T::RBB) * panel;
+	const int8_t *Bg = Bp + (size_t) (tn * T::RBB) * panelB;
 	const int8_t *src[T::LOADS];
 #pragma unroll
 	for (int c = 0; c < T::LOADS; ++c) {
 		const int g = c * 16 * T::THREADS + tid * 16;
 		const int blk = g / T::CHUNK, o = g % T::CHUNK;
-		src[c] = blk < T::RBA ? Ag + (size_t) blk * panel + o : Bg + (size_t) (blk - T::RBA) * panel + o;
+		src[c] = blk < T::RBA ? Ag + (size_t) blk * panel + o : Bg + (size_t) (blk - T::RBA) * panelB + o;
 	}
 	auto buf = [&](int s) -> int8_t * { return s == 0 ? s0 : s == 1 ? s1 : s == 2 ? s2 : s3; };
 	auto issue = [&](int step, int8_t *dst) {
@@ -467,7 +517,7 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_async_kernel(const 
 		if (ks0 + s < KS)
 			step(ks0 + s, s);
 	wait_vmcnt<0>();   // no load into LDS outlives the workgroup
-	i8_epilogue<E, T, WTM, WTN>(acc, C, ldc, m, n, a, b, mp, tm, tn, wr, wc, lane);
+	i8_epilogue<E, T, WTM, WTN>(acc, C, ldc, m, n, a, b, mp, tm, tn, wr, wc, lane, Cpk, panelC);
 }
 
 // ===========================================================================
@@ -545,15 +595,28 @@ static int pack_buffers(mlz_ctx *ctx, size_t abytes, size_t bbytes, void **ap, v
 enum { I8V_AUTO = 0, I8V_COUNT = 12 };
 
 // EA: storage of the A operand when it differs from B's and C's (E).
+// Bpre: B already packed, its 32-column blocks bpanel bytes apart (a k-range
+// of a larger packed operand); Cpk: C also written as packed B fragments,
+// 32-column blocks cpanel bytes apart (i8_epilogue).  Both need one k chunk,
+// one product and the multi-buffered kernels; Cpk needs m and n multiples of 32.
 template <int E, int WTM, int WTN, int KK, int OCC, int WM = 2, int WN = 2, int NS = 0, int EA = E>
 static int gemm_i8_launch_t(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k, uint32_t a,
 			    const void *A, size_t lda, const void *B, size_t ldb, uint32_t b, void *C,
-			    size_t ldc, const I8Batch &bt = I8Batch::single(), const int8_t *Apre = nullptr)
+			    size_t ldc, const I8Batch &bt = I8Batch::single(), const int8_t *Apre = nullptr,
+			    const int8_t *Bpre = nullptr, size_t bpanel = 0, int8_t *Cpk = nullptr, size_t cpanel = 0)
 {
 	typedef typename Elem<E>::T T;
 	typedef typename Elem<EA>::T TA;
 	if (Apre && k > (size_t) I8_KMAX)
 		return set_error(ctx, MLZ_E_ARG, "a pre-packed A operand needs k <= %d", I8_KMAX);
+	if ((Bpre || Cpk) && (k > (size_t) I8_KMAX || bt.count != 1 || NS == 0))
+		return set_error(ctx, MLZ_E_ARG, "packed B input or packed output: one product with k <= %d", I8_KMAX);
+	if (Cpk && (m % 32 || n % 32))
+		return set_error(ctx, MLZ_E_ARG, "packed output needs m and n in multiples of 32");
+	if (Bpre && bpanel < round_up(k, 64) * 64)
+		return set_error(ctx, MLZ_E_ARG, "packed B: block stride %zu below its k", bpanel);
+	if ((!Apre && !A) || (!Bpre && !B))
+		return set_error(ctx, MLZ_E_ARG, "INT8 gemm: an operand is neither given nor pre-packed");
 	typedef I8Tile<WTM, WTN, KK, WM, WN> Tile;
 	const ModP mp = make_modp(p);
 	const size_t mpad = round_up(m, Tile::BM), npad = round_up(n, Tile::BN);
@@ -565,7 +628,9 @@ static int gemm_i8_launch_t(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t
 		const int KB = (int) (kpad / 32);
 		const size_t sAp = mpad * kpad * 2, sBp = npad * kpad * 2;   // bytes per packed item
 		void *ap = nullptr, *bp = nullptr;
-		MLZ_TRY(pack_buffers(ctx, Apre ? 0 : sAp * bt.count, sBp * bt.count, &ap, &bp));
+		MLZ_TRY(pack_buffers(ctx, Apre ? 0 : sAp * bt.count, Bpre ? 0 : sBp * bt.count, &ap, &bp));
+		if (Bpre)
+			bp = (void *) Bpre;
 		if (Apre)
 			ap = (void *) Apre;
 		else {
@@ -582,7 +647,7 @@ static int gemm_i8_launch_t(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t
 											     (int8_t *) ap, KB, bt, sAp);
 		}
 		MLZ_TRY(check_launch(ctx, "pack_i8_kernel(A)"));
-		{
+		if (!Bpre) {
 			ProfScope ps(ctx, "pack_i8_B");
 			dim3 g(npad / 64, kpad / 64, bt.count);
 			const T *src = (const T *) B + k0 * ldb;
@@ -612,7 +677,8 @@ static int gemm_i8_launch_t(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t
 				gemm_i8_async_kernel<E, WTM, WTN, KK, OCC, WM, WN, NS>
 					<<<g, Tile::THREADS, 0, ctx->stream>>>(
 						(const int8_t *) ap, (const int8_t *) bp, KB, (T *) C, ldc, (int) m, (int) n,
-						a, k0 == 0 ? b : 1u, mp, tiles_m, tiles_n, bt, sAp, sBp);
+						a, k0 == 0 ? b : 1u, mp, tiles_m, tiles_n, bt, sAp, sBp,
+						Bpre ? bpanel : (size_t) KB * 2 * I8_FRAG, Cpk, cpanel);
 		}
 		MLZ_TRY(check_launch(ctx, "gemm_i8_kernel"));
 	}
@@ -675,31 +741,34 @@ static int gemm_i8_launch(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t k
 template <int E, int WTM, int WTN, int KK, int OCC, int WM, int WN, int NS>
 static int gemm_i8_batched_ea(mlz_ctx *ctx, int elemA, uint32_t p, const I8Batch &bt, size_t m, size_t n, size_t k,
 			      uint32_t a, const void *A, size_t lda, const void *B, size_t ldb, uint32_t b, void *C,
-			      size_t ldc, const int8_t *Apre)
+			      size_t ldc, const int8_t *Apre, const I8PackIO &io)
 {
 	if (elemA == MLZ_U16)
 		return gemm_i8_launch_t<E, WTM, WTN, KK, OCC, WM, WN, NS, MLZ_U16>(ctx, p, m, n, k, a, A, lda, B, ldb, b,
-										    C, ldc, bt, Apre);
+										    C, ldc, bt, Apre, io.Bpre, io.bpanel,
+										    io.Cpk, io.cpanel);
 	if (elemA == MLZ_U32)
 		return gemm_i8_launch_t<E, WTM, WTN, KK, OCC, WM, WN, NS, MLZ_U32>(ctx, p, m, n, k, a, A, lda, B, ldb, b,
-										    C, ldc, bt, Apre);
+										    C, ldc, bt, Apre, io.Bpre, io.bpanel,
+										    io.Cpk, io.cpanel);
 	return gemm_i8_launch_t<E, WTM, WTN, KK, OCC, WM, WN, NS, MLZ_F64_SYM>(ctx, p, m, n, k, a, A, lda, B, ldb, b, C,
-										ldc, bt, Apre);
+										ldc, bt, Apre, io.Bpre, io.bpanel, io.Cpk,
+										io.cpanel);
 }
 
 template <int WTM, int WTN, int KK, int OCC, int WM, int WN, int NS>
 static int gemm_i8_batched_e(mlz_ctx *ctx, int elem, int elemA, uint32_t p, const I8Batch &bt, size_t m, size_t n,
 			     size_t k, uint32_t a, const void *A, size_t lda, const void *B, size_t ldb, uint32_t b,
-			     void *C, size_t ldc, const int8_t *Apre)
+			     void *C, size_t ldc, const int8_t *Apre, const I8PackIO &io)
 {
 	if (elem == MLZ_U16)
 		return gemm_i8_batched_ea<MLZ_U16, WTM, WTN, KK, OCC, WM, WN, NS>(ctx, elemA, p, bt, m, n, k, a, A, lda, B,
-										   ldb, b, C, ldc, Apre);
+										   ldb, b, C, ldc, Apre, io);
 	if (elem == MLZ_U32)
 		return gemm_i8_batched_ea<MLZ_U32, WTM, WTN, KK, OCC, WM, WN, NS>(ctx, elemA, p, bt, m, n, k, a, A, lda, B,
-										   ldb, b, C, ldc, Apre);
+										   ldb, b, C, ldc, Apre, io);
 	return gemm_i8_batched_ea<MLZ_F64_SYM, WTM, WTN, KK, OCC, WM, WN, NS>(ctx, elemA, p, bt, m, n, k, a, A, lda, B,
-									       ldb, b, C, ldc, Apre);
+									       ldb, b, C, ldc, Apre, io);
 }
 
 // Batches use the 128 x 128 tile through four LDS buffers (variant 10), or
@@ -708,7 +777,7 @@ static int gemm_i8_batched_e(mlz_ctx *ctx, int elem, int elemA, uint32_t p, cons
 // elemA: storage of A.
 int zp_gemm_i8_batched_dev(mlz_ctx *ctx, int elem, int elemA, uint32_t p, const I8Batch &bt, size_t m, size_t n,
 			   size_t k, uint32_t a, const void *A, size_t lda, const void *B, size_t ldb, uint32_t b,
-			   void *C, size_t ldc, const int8_t *Apre)
+			   void *C, size_t ldc, const int8_t *Apre, const I8PackIO &io)
 {
 	if (elem_size(elem) == 0 || elem_size(elemA) == 0)
 		return set_error(ctx, MLZ_E_ARG, "batched gemm: unknown element storage");
@@ -725,8 +794,9 @@ int zp_gemm_i8_batched_dev(mlz_ctx *ctx, int elem, int elemA, uint32_t p, const 
 	// the 64 x 64 tile unless the batch has 256 or more 128 x 128 tiles
 	if ((size_t) bt.count * ceil_div(m, 128) * ceil_div(n, 128) < 256)
 		return gemm_i8_batched_e<1, 1, 1, 4, 2, 2, 4>(ctx, elem, elemA, p, bt, m, n, k, a, A, lda, B, ldb, b, C, ldc,
-							      Apre);
-	return gemm_i8_batched_e<2, 2, 1, 2, 2, 2, 4>(ctx, elem, elemA, p, bt, m, n, k, a, A, lda, B, ldb, b, C, ldc, Apre);
+							      Apre, io);
+	return gemm_i8_batched_e<2, 2, 1, 2, 2, 2, 4>(ctx, elem, elemA, p, bt, m, n, k, a, A, lda, B, ldb, b, C, ldc, Apre,
+						      io);
 }
 
 // Packs A (rows x k per item, the A operand layout of the INT8 kernels) for a
@@ -763,6 +833,33 @@ int zp_pack_i8_a_dev(mlz_ctx *ctx, int elem, uint32_t p, const I8Batch &bt, size
 	}
 #undef PACK
 	return check_launch(ctx, "pack_i8_kernel(A, batch)");
+}
+
+// Packs B (k x cols, the B operand layout of the INT8 kernels: 32-column
+// blocks of round_up(k, 64) * 64 bytes) into dst.
+int zp_pack_i8_b_dev(mlz_ctx *ctx, int elem, uint32_t p, size_t k, size_t cols, const void *B, size_t ldb,
+		     int8_t *dst, hipStream_t stream)
+{
+	if (k > (size_t) I8_KMAX || p > 65536)
+		return set_error(ctx, MLZ_E_ARG, "INT8 packing needs k <= %d and p <= 2^16", I8_KMAX);
+	if (!stream)
+		stream = ctx->stream;
+	const size_t npad = round_up(cols, 64), kpad = round_up(k, 64);
+	const int KB = (int) (kpad / 32);
+	dim3 g(npad / 64, kpad / 64, 1);
+	const bool vec = aligned16(B, ldb * elem_size(elem));
+	ProfScope ps(ctx, "pack_i8_B");
+#define PACK(EE, TT, V) pack_i8_kernel<EE, true, V><<<g, 256, 0, stream>>>((const TT *) B, ldb, (int) cols, (int) k, \
+									  p, dst, KB, I8Batch::single(), 0)
+	if (elem == MLZ_U16) {
+		if (vec) PACK(MLZ_U16, uint16_t, true); else PACK(MLZ_U16, uint16_t, false);
+	} else if (elem == MLZ_U32) {
+		if (vec) PACK(MLZ_U32, uint32_t, true); else PACK(MLZ_U32, uint32_t, false);
+	} else {
+		if (vec) PACK(MLZ_F64_SYM, double, true); else PACK(MLZ_F64_SYM, double, false);
+	}
+#undef PACK
+	return check_launch(ctx, "pack_i8_kernel(B)");
 }
 
 // ===========================================================================

@@ -777,13 +777,35 @@ static void trsm_updates(size_t n, size_t base, int upper, std::vector<TrsmUpd> 
 	}
 }
 
+// The right-hand sides kept as packed INT8 B operands between the products
+// of the recursion (every block a full TRSM_INV_BLOCK leaf): PB holds B's
+// current rows, PX the solved rows X; both in the packed B layout over all n
+// rows (32-column blocks `panel` bytes apart, 2048 bytes per 32 rows).  A
+// leaf reads B_I from PB and writes X_I also into PX; an update reads X from
+// PX and writes its rows also into PB.  Each product thus finds its B operand
+// packed by the one before it, with no packing launch between them.
+struct TrsmPk {
+	int8_t *PB, *PX;
+	size_t panel;
+	I8PackIO io(const int8_t *in, size_t in_row, int8_t *out, size_t out_row) const
+	{
+		I8PackIO r;
+		r.Bpre = in + in_row / 32 * 2048;
+		r.bpanel = panel;
+		r.Cpk = out + out_row / 32 * 2048;
+		r.cpanel = panel;
+		return r;
+	}
+};
+
 // Upk: the packed update blocks in trsm_updates' order (with Zp), *ui the
-// next one to use.
+// next one to use; pk (with Upk): the packed right-hand sides, row0 the
+// first row of B in them.
 template <int E>
 static int trsm_rec(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, size_t nrhs, uint32_t a,
 		    const typename Elem<E>::T *A, size_t lda, typename Elem<E>::T *B, size_t ldb,
 		    const uint32_t *W, int algo, const int8_t *Zp = nullptr, const int8_t *const *Upk = nullptr,
-		    size_t *ui = nullptr)
+		    size_t *ui = nullptr, const TrsmPk *pk = nullptr, size_t row0 = 0)
 {
 	if (n == 0)
 		return MLZ_OK;
@@ -792,7 +814,7 @@ static int trsm_rec(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, siz
 		// product is written over B_I, which the gemm reads only through its
 		// packed copy.
 		return zp_gemm_i8_batched_dev(ctx, E, MLZ_U32, p, I8Batch::single(), n, nrhs, n, a, nullptr, n, B, ldb, 0, B,
-					      ldb, Zp);
+					      ldb, Zp, pk ? pk->io(pk->PB, row0, pk->PX, row0) : I8PackIO());
 	}
 	if (n <= (size_t) TRSM_LEAF)
 		return trsm_leaf<E>(ctx, p, n, nrhs, a, W, B, ldb, algo);
@@ -811,24 +833,26 @@ static int trsm_rec(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n, siz
 	const typename Elem<E>::T *A11 = A, *A22 = A + l * lda + l;
 	typename Elem<E>::T *B1 = B, *B2 = B + l * ldb;
 	const int8_t *Zp2 = Zp ? Zp + (l / TRSM_INV_BLOCK) * TRSM_ZP_BYTES : nullptr;
-	// the update C <- -Aoff X + a C, its A block packed ahead when Upk is given
+	// the update C <- -Aoff X + a C, its A block packed ahead when Upk is
+	// given; X and C start at rows xr and cr of the packed right-hand sides
 	auto update = [&](size_t rows, size_t cols, const typename Elem<E>::T *Aoff, const typename Elem<E>::T *X,
-			  typename Elem<E>::T *Cu) -> int {
+			  typename Elem<E>::T *Cu, size_t xr, size_t cr) -> int {
 		if (Upk)
 			return zp_gemm_i8_batched_dev(ctx, E, E, p, I8Batch::single(), rows, nrhs, cols, p - 1, nullptr, lda,
-						      X, ldb, a, Cu, ldb, Upk[(*ui)++]);
+						      X, ldb, a, Cu, ldb, Upk[(*ui)++],
+						      pk ? pk->io(pk->PX, xr, pk->PB, cr) : I8PackIO());
 		return zp_gemm_dev(ctx, E, p, rows, nrhs, cols, p - 1, Aoff, lda, X, ldb, a, Cu, ldb, algo);
 	};
 	if (!upper) {
-		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, a, A11, lda, B1, ldb, W, algo, Zp, Upk, ui));
-		MLZ_TRY(update(n - l, l, A + l * lda, B1, B2));
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, a, A11, lda, B1, ldb, W, algo, Zp, Upk, ui, pk, row0));
+		MLZ_TRY(update(n - l, l, A + l * lda, B1, B2, row0, row0 + l));
 		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n - l, nrhs, 1, A22, lda, B2, ldb, W + l * TRSM_LEAF, algo, Zp2,
-				    Upk, ui));
+				    Upk, ui, pk, row0 + l));
 	} else {
 		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n - l, nrhs, a, A22, lda, B2, ldb, W + l * TRSM_LEAF, algo, Zp2,
-				    Upk, ui));
-		MLZ_TRY(update(l, n - l, A + l, B2, B1));
-		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, 1, A11, lda, B1, ldb, W, algo, Zp, Upk, ui));
+				    Upk, ui, pk, row0 + l));
+		MLZ_TRY(update(l, n - l, A + l, B2, B1, row0 + l, row0));
+		MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, l, nrhs, 1, A11, lda, B1, ldb, W, algo, Zp, Upk, ui, pk, row0));
 	}
 	return MLZ_OK;
 }
@@ -957,12 +981,17 @@ static int trsm_apply_t(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n,
 		u.off = ubytes;
 		ubytes += round_up(item, 256);
 	}
+	// the right-hand sides stay packed between the products when every block
+	// is a full leaf and the columns fill whole 128-column tiles
+	const bool packed_rhs = n % S == 0 && nrhs % 128 == 0 && !upd.empty();
+	const size_t pkbytes = packed_rhs ? round_up(n * nrhs * 2, 256) : 0;
 	void *w = nullptr;
-	MLZ_TRY(ws_get(ctx, WS_TRSM_3, zbytes + tbytes + nfull * TRSM_ZP_BYTES + ubytes, &w));
+	MLZ_TRY(ws_get(ctx, WS_TRSM_3, zbytes + tbytes + nfull * TRSM_ZP_BYTES + ubytes + 2 * pkbytes, &w));
 	uint32_t *Z = (uint32_t *) w;
 	uint32_t *Tb = (uint32_t *) ((char *) w + zbytes);
 	int8_t *Zp = (int8_t *) w + zbytes + tbytes;
 	int8_t *Up = Zp + nfull * TRSM_ZP_BYTES;
+	TrsmPk pk{Up + ubytes, Up + ubytes + pkbytes, n * 64};
 	std::vector<const int8_t *> upk(upd.size());
 	struct Events {
 		hipEvent_t e[2] = {nullptr, nullptr};
@@ -992,6 +1021,8 @@ static int trsm_apply_t(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n,
 			MLZ_TRY(zp_pack_i8_a_dev(ctx, E, p, I8Batch::single(), u.rows, u.cols, A + u.r0 * lda + u.c0, lda,
 						 Up + u.off, nullptr, s1));
 		}
+		if (packed_rhs)     // all of B as one packed B operand (k = n)
+			MLZ_TRY(zp_pack_i8_b_dev(ctx, E, p, n, nrhs, B, ldb, pk.PB, s1));
 		MLZ_HIP(ctx, hipEventRecord(ev.e[1], s1));
 	}
 	// the inverses as INT8 A operands, packed once for the leaves
@@ -1007,7 +1038,8 @@ static int trsm_apply_t(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n,
 		return trsm_rec<E>(ctx, p, upper, unit, n, nrhs, a, A, lda, B, ldb, W, algo, Zp);
 	MLZ_HIP(ctx, hipStreamWaitEvent(ctx->stream, ev.e[1], 0));
 	size_t ui = 0;
-	MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n, nrhs, a, A, lda, B, ldb, W, algo, Zp, upk.data(), &ui));
+	MLZ_TRY(trsm_rec<E>(ctx, p, upper, unit, n, nrhs, a, A, lda, B, ldb, W, algo, Zp, upk.data(), &ui,
+			    packed_rhs ? &pk : nullptr, 0));
 	if (ui != upd.size())
 		return set_error(ctx, MLZ_E_ARG, "trsm used %zu of %zu packed update blocks", ui, upd.size());
 	return MLZ_OK;

@@ -122,6 +122,23 @@ def test_trsm_block_inverse_path(gpu_ctx, upper, unit, storage):
     assert rc == 0 and np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("upper,unit,storage", [(False, True, "uint16"), (True, False, "uint16"),
+                                                (False, False, "uint32"), (True, True, "double")])
+def test_trsm_packed_rhs_path(gpu_ctx, upper, unit, storage):
+    """n = 2 * 1024 and 128 right-hand sides: every block is an inverted
+    leaf and the columns fill whole tiles, so the right-hand sides stay
+    packed between the products (each product's epilogue packs the next
+    one's operand); bit-exact against the oracle."""
+    p = 65521
+    n, nrhs = 2048, 128
+    print(f"trsm packed rhs upper={upper} unit={unit} {storage}", flush=True)
+    A = tri(35 + upper, n, p, upper, unit)
+    B = synth.zp_matrix(37, n, nrhs, p)
+    got = run_trsm(gpu_ctx, p, storage, upper, unit, 7, A, B)
+    rc, _, want = orc.zp_trsm(p, upper, unit, 7, A, B)
+    assert rc == 0 and np.array_equal(got, want)
+
+
 def test_trsm_upper_nonunit_4096(gpu_ctx):
     """Upper, non-unit diagonal, n = 4096 (four inverted blocks), checked by
     Freivalds: A X == a B."""
