@@ -133,69 +133,108 @@ __global__ __launch_bounds__(64) void trsm_diag_inv_col_kernel(const typename El
 		W[(size_t) (b0 + r) * L + j] = j < nb ? X[r][j] : 0u;
 }
 
+// Four canonical residues (p <= 2^16) as INT8 limb bytes: hi holds the high
+// limbs of v0..v3 in bytes 0..3, lo the low ones.  The limb pair of v is the
+// 16-bit value v + 32896 (less p above 32639) with both bytes offset by 0x80:
+// the signed split of i8_limbs.  Two byte permutes gather the bytes of the
+// four 16-bit values.
+__device__ __forceinline__ void limb_pack4(uint32_t v0, uint32_t v1, uint32_t v2, uint32_t v3, uint32_t p,
+					   uint32_t &hi, uint32_t &lo)
+{
+	auto w = [&](uint32_t v) -> uint32_t { return v > 32639u ? v + 32896u - p : v + 32896u; };
+	const uint32_t x01 = w(v0) | (w(v1) << 16), x23 = w(v2) | (w(v3) << 16);
+	lo = __builtin_amdgcn_perm(x23, x01, 0x06040200u) ^ 0x80808080u;
+	hi = __builtin_amdgcn_perm(x23, x01, 0x07050301u) ^ 0x80808080u;
+}
+
 // One product of the doubling steps of trsm_block_inverses:
 // C_z = a * A_z * B_z for the s x s blocks of batch member z (s a multiple of
 // 64), A in the caller's storage, B and C canonical uint32 (p <= 2^16), on
 // INT8 MFMA with the limb split done in LDS, so that no separate packing
 // launch is needed.  A workgroup computes one 64 x 64 tile of C (4 waves,
-// 32 x 32 each) over k-chunks of 64; each thread turns four k-consecutive
-// entries of A and of B per step into one 32-bit word per limb, in the
-// fragment layout of trsm_leaf_i8_kernel.  k <= 512 keeps every limb sum far
-// below 2^31 and the combined value an exact double.
-template <int EA>
+// 32 x 32 each) over k-chunks of 64.  The limb split is the kernel's main
+// VALU work (every workgroup splits the entries it reads), so the loads are
+// 16-byte ones where they can be: B as 4 x 4 blocks (four rows of four
+// columns; a thread then holds four k-consecutive entries of each column), A
+// as runs of four k-consecutive entries when AV (rows aligned to four
+// entries); limb_pack4 forms each 32-bit fragment word.  k <= 512 keeps every
+// limb sum far below 2^31 and the combined value an exact double.
+template <int EA, bool AV>
 __global__ __launch_bounds__(256) void trsm_dbl_kernel(const typename Elem<EA>::T *__restrict__ A, size_t lda,
 						       const uint32_t *__restrict__ B, size_t ldb,
 						       uint32_t *__restrict__ C, size_t ldc, int s, uint32_t a,
 						       I8Batch bt, ModP mp)
 {
+	typedef typename Elem<EA>::T TA;
 	__shared__ __attribute__((aligned(16))) int8_t Af[2][2][2][64][16];   // [row block][k block][limb]
 	__shared__ __attribute__((aligned(16))) int8_t Bf[2][2][2][64][16];   // [column block][k block][limb]
 	const uint32_t p = (uint32_t) mp.p;
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int nt = s >> 6, z = blockIdx.y;
 	const int r0 = (blockIdx.x / nt) * 64, c0 = (blockIdx.x % nt) * 64;
-	const typename Elem<EA>::T *Az = A + bt.offset(z, 0) + (size_t) r0 * lda;
+	const TA *Az = A + bt.offset(z, 0) + (size_t) r0 * lda;
 	const uint32_t *Bz = B + bt.offset(z, 1) + c0;
 	uint32_t *Cz = C + bt.offset(z, 2) + (size_t) r0 * ldc + c0;
 	const int rb = wave >> 1, cb = wave & 1;
 	v16i acc0 = v16i{0}, acc1 = v16i{0}, acc2 = v16i{0};
-	// thread tid handles A groups (row g >> 4, k 4 (g & 15) ..) and B groups
-	// (column g & 63, k 4 (g >> 6) ..) of the chunk, g = tid + 256 q; the next
-	// chunk's entries are loaded into registers while this one is multiplied
-	// (two chunks ahead, in two LDS buffers, measured slower: 120 against 90
-	// us for the eight launches of an 8192-row solve)
+	// A: groups g = tid + 256 q (row g >> 4, k 4 (g & 15) ..); B: the 4 x 4
+	// block at k 4 (tid >> 4) .., columns 4 (tid & 15) ..  The next chunk's
+	// entries are loaded into registers while this one is multiplied (two
+	// chunks ahead, in two LDS buffers, measured slower: 120 against 90 us for
+	// the eight launches of an 8192-row solve; chunks of 128 measured equal)
+	const int kq = 4 * (tid >> 4), cq = 4 * (tid & 15);
 	uint32_t ra[4][4], rbv[4][4];
 	auto load = [&](int k0) {
 #pragma unroll
 		for (int q = 0; q < 4; ++q) {
 			const int g = tid + 256 * q;
-			const int r = g >> 4, ka = 4 * (g & 15), c = g & 63, kb = 4 * (g >> 6);
+			const TA *src = Az + (size_t) (g >> 4) * lda + k0 + 4 * (g & 15);
+			if constexpr (AV && sizeof(TA) == 2) {
+				const uint2 u = *(const uint2 *) src;
+				ra[q][0] = u.x & 0xffffu;
+				ra[q][1] = u.x >> 16;
+				ra[q][2] = u.y & 0xffffu;
+				ra[q][3] = u.y >> 16;
+			} else if constexpr (AV && sizeof(TA) == 4) {
+				const uint4 u = *(const uint4 *) src;
+				ra[q][0] = u.x;
+				ra[q][1] = u.y;
+				ra[q][2] = u.z;
+				ra[q][3] = u.w;
+			} else {
 #pragma unroll
-			for (int t = 0; t < 4; ++t) {
-				ra[q][t] = Elem<EA>::load(Az + (size_t) r * lda + k0 + ka + t, p);
-				rbv[q][t] = Bz[(size_t) (k0 + kb + t) * ldb + c];
+				for (int t = 0; t < 4; ++t)
+					ra[q][t] = Elem<EA>::load(src + t, p);
 			}
+		}
+#pragma unroll
+		for (int t = 0; t < 4; ++t) {
+			const uint4 u = *(const uint4 *) (Bz + (size_t) (k0 + kq + t) * ldb + cq);
+			rbv[t][0] = u.x;
+			rbv[t][1] = u.y;
+			rbv[t][2] = u.z;
+			rbv[t][3] = u.w;
 		}
 	};
 	auto put = [&]() {
 #pragma unroll
 		for (int q = 0; q < 4; ++q) {
 			const int g = tid + 256 * q;
-			const int r = g >> 4, ka = 4 * (g & 15), c = g & 63, kb = 4 * (g >> 6);
-			uint32_t ahi = 0, alo = 0, bhi = 0, blo = 0;
+			const int r = g >> 4, ka = 4 * (g & 15);
+			uint32_t hi, lo;
+			limb_pack4(ra[q][0], ra[q][1], ra[q][2], ra[q][3], p, hi, lo);
+			const int lna = (r & 31) + 32 * ((ka & 31) >> 4);
+			*(uint32_t *) &Af[r >> 5][ka >> 5][0][lna][ka & 15] = hi;
+			*(uint32_t *) &Af[r >> 5][ka >> 5][1][lna][ka & 15] = lo;
+		}
 #pragma unroll
-			for (int t = 0; t < 4; ++t) {
-				const uint32_t la = i8_limbs(ra[q][t], p), lb = i8_limbs(rbv[q][t], p);
-				ahi |= (la >> 8) << (8 * t);
-				alo |= (la & 0xffu) << (8 * t);
-				bhi |= (lb >> 8) << (8 * t);
-				blo |= (lb & 0xffu) << (8 * t);
-			}
-			const int lna = (r & 31) + 32 * ((ka & 31) >> 4), lnb = (c & 31) + 32 * ((kb & 31) >> 4);
-			*(uint32_t *) &Af[r >> 5][ka >> 5][0][lna][ka & 15] = ahi;
-			*(uint32_t *) &Af[r >> 5][ka >> 5][1][lna][ka & 15] = alo;
-			*(uint32_t *) &Bf[c >> 5][kb >> 5][0][lnb][kb & 15] = bhi;
-			*(uint32_t *) &Bf[c >> 5][kb >> 5][1][lnb][kb & 15] = blo;
+		for (int j = 0; j < 4; ++j) {
+			const int c = cq + j;
+			uint32_t hi, lo;
+			limb_pack4(rbv[0][j], rbv[1][j], rbv[2][j], rbv[3][j], p, hi, lo);
+			const int lnb = (c & 31) + 32 * ((kq & 31) >> 4);
+			*(uint32_t *) &Bf[c >> 5][kq >> 5][0][lnb][kq & 15] = hi;
+			*(uint32_t *) &Bf[c >> 5][kq >> 5][1][lnb][kq & 15] = lo;
 		}
 	};
 	load(0);
@@ -721,6 +760,10 @@ static int trsm_block_inverses(mlz_ctx *ctx, uint32_t p, int upper, size_t nfull
 			trsm_inv_init_kernel<false><<<ceil_div(total, 256), 256, 0, ctx->stream>>>(W, total, Z);
 	}
 	MLZ_TRY(check_launch(ctx, "trsm_inv_init_kernel"));
+	// A's rows as runs of four entries: every block offset below is a multiple
+	// of 64 entries, so aligned rows of A suffice (uint16 / uint32 storage)
+	typedef typename Elem<E>::T TE;
+	const bool av = sizeof(TE) <= 4 && lda % 4 == 0 && (uintptr_t) A % (4 * sizeof(TE)) == 0;
 	for (size_t s = TRSM_LEAF; 2 * s <= S; s *= 2) {
 		// pair j of block I: rows and columns o .. o + 2s of the block, o = 2sj.
 		// Entry (o + r, o + c) of block I is at I*S*S + o*(S + 1) + r*S + c in
@@ -739,7 +782,10 @@ static int trsm_block_inverses(mlz_ctx *ctx, uint32_t p, int upper, size_t nfull
 		const typename Elem<E>::T *offA = upper ? A + s : A + s * lda;
 		const uint32_t *inv_r = upper ? Z + s * S + s : Z;
 		const dim3 grid((unsigned) ((s / 64) * (s / 64)), (unsigned) bt.count);
-		trsm_dbl_kernel<E><<<grid, 256, 0, ctx->stream>>>(offA, lda, inv_r, S, Tb, s, (int) s, 1u, bt, mp);
+		if (av)
+			trsm_dbl_kernel<E, true><<<grid, 256, 0, ctx->stream>>>(offA, lda, inv_r, S, Tb, s, (int) s, 1u, bt, mp);
+		else
+			trsm_dbl_kernel<E, false><<<grid, 256, 0, ctx->stream>>>(offA, lda, inv_r, S, Tb, s, (int) s, 1u, bt, mp);
 		MLZ_TRY(check_launch(ctx, "trsm_dbl_kernel"));
 		// off-diagonal block of the inverse = -(other diagonal inverse) * T
 		bt.outer[0] = zo; bt.in[0] = zi;
@@ -747,7 +793,7 @@ static int trsm_block_inverses(mlz_ctx *ctx, uint32_t p, int upper, size_t nfull
 		bt.outer[2] = zo; bt.in[2] = zi;
 		const uint32_t *inv_l = upper ? Z : Z + s * S + s;             // U11^-1 / L22^-1
 		uint32_t *out = upper ? Z + s : Z + s * S;
-		trsm_dbl_kernel<MLZ_U32><<<grid, 256, 0, ctx->stream>>>(inv_l, S, Tb, s, out, S, (int) s, p - 1, bt, mp);
+		trsm_dbl_kernel<MLZ_U32, true><<<grid, 256, 0, ctx->stream>>>(inv_l, S, Tb, s, out, S, (int) s, p - 1, bt, mp);
 		MLZ_TRY(check_launch(ctx, "trsm_dbl_kernel"));
 	}
 	return MLZ_OK;
@@ -1008,8 +1054,11 @@ static int trsm_apply_t(mlz_ctx *ctx, uint32_t p, int upper, int unit, size_t n,
 		// A may have been written by work queued before this call
 		MLZ_HIP(ctx, hipEventRecord(ev.e[0], ctx->stream));
 	}
-	// the inverses are queued first, so that the context's stream has work
-	// while the packing launches are issued
+	// The inverses are queued first, so that the context's stream has work
+	// while the packing launches are issued.  (Packing from a point before the
+	// 64-row diagonal inverses, beside them, measured no faster: 0.418-0.427
+	// against 0.413-0.418 ms for 8192 x 1024; the packs slow the inverse
+	// kernels as much as they gain.)
 	MLZ_TRY(trsm_block_inverses<E>(ctx, p, upper, nfull, A, lda, W, Z, Tb));
 	if (!upd.empty()) {
 		hipStream_t s1;
