@@ -61,76 +61,114 @@ __global__ void init_bad_kernel(unsigned long long *bad)
 	bad[1] = 0ull;
 }
 
-// Inverts every diagonal block of TRSM_LEAF rows in one launch, one wave per
-// block (W receives block b's inverse in rows b*L .. b*L+nb-1, L columns
-// each).  Lane j computes column j of A_bb^-1, which depends on itself only:
-// x_i = (e_ij - sum_k A_ik x_k) / A_ii over the rows k solved before i, so
-// the substitution needs no barrier.  Column j lives in LDS (X[k][j]: the
-// lane's own entries, conflict free across lanes); row i of the block is
-// read as 16-byte broadcasts over the solved part only (4-aligned groups:
-// As is zero on and beyond the diagonal, X zero where unsolved), and four
-// independent partial sums keep the products from forming one dependent
-// chain.  The sum is delayed, one reduction per entry: raw 64-bit products
-// for p < 2^26 (64 terms below 2^58), reduced products otherwise (64 terms
-// below 2^38).  (Entries as doubles with FP64 fmas for p <= 2^16 measured
-// slower, 62 against 35 us for 128 blocks: the step is latency-bound.)
+// Inverts every diagonal block of TRSM_LEAF rows in one launch, one
+// workgroup of 256 threads per block (W receives block b's inverse in rows
+// b*L .. b*L+nb-1, L columns each), in log depth: the eight 8 x 8 diagonal
+// blocks first (64 threads, one column each, 8 substitution steps:
+// x_i = (e_ij - sum_k A_ik x_k) / A_ii), then three doubling levels, each
+// joining pairs of inverted s x s blocks:
+//   lower [[P, 0], [C, Q]]^-1 = [[P^-1, 0], [-Q^-1 C P^-1, Q^-1]],
+//   upper [[P, B], [0, Q]]^-1 = [[P^-1, -P^-1 B Q^-1], [0, Q^-1]],
+// as two products T = C P^-1 (B Q^-1), then -Q^-1 T (-P^-1 T), all pairs of
+// a level at once, one barrier after each.  A 64-step substitution is one
+// dependent chain of reductions; here the longest chain is 8 steps plus six
+// sums of at most 32 terms.  Sums are delayed, one reduction per entry: raw
+// 64-bit products for p < 2^26 (32 terms below 2^57), reduced products
+// otherwise (below 2^37).  A partial block is completed by the identity.
 template <int E, bool UPPER, bool UNIT, bool SMALL>
-__global__ __launch_bounds__(64) void trsm_diag_inv_col_kernel(const typename Elem<E>::T *__restrict__ A, size_t lda,
-							       int n, const uint32_t *__restrict__ dinv,
-							       uint32_t *__restrict__ W, ModP mp)
+__global__ __launch_bounds__(256) void trsm_diag_inv_blk_kernel(const typename Elem<E>::T *__restrict__ A, size_t lda,
+								int n, const uint32_t *__restrict__ dinv,
+								uint32_t *__restrict__ W, ModP mp)
 {
 	constexpr int L = TRSM_LEAF;
-	__shared__ __attribute__((aligned(16))) uint32_t As[L][L];
-	__shared__ uint32_t X[L][L];
-	__shared__ uint32_t Ds[L];
+	__shared__ uint32_t As[L][L + 1];      // the block's strict triangle
+	__shared__ uint32_t X[L][L + 1];       // its inverse as it is formed
+	__shared__ uint32_t Tm[L * L / 4];     // the first products of a level
+	__shared__ uint32_t Ds[L];             // inverses of the diagonal entries
 	const uint32_t p = (uint32_t) mp.p;
-	const int j = threadIdx.x;
+	const int tid = threadIdx.x;
 	const int b0 = blockIdx.x * L;
 	const int nb = min(L, n - b0);
-	// a partial block is completed by the identity (zero off-diagonal
-	// entries, unit diagonal): its first nb rows and columns are unchanged
-	// (all loads issued before the first store: one memory latency, not 64)
-	uint32_t v[L];
+	auto prod = [&](uint32_t x, uint32_t y) -> uint64_t {
+		return SMALL ? (uint64_t) x * y : (uint64_t) mp.mul(x, y);
+	};
+	{
+		// row tid/4, columns 16 (tid%4) .. +15; all loads before the stores
+		const int r = tid >> 2, c0 = 16 * (tid & 3);
+		uint32_t v[16];
 #pragma unroll
-	for (int r = 0; r < L; ++r) {
-		const bool tri = UPPER ? (j > r) : (j < r);
-		v[r] = (r < nb && j < nb && tri) ? Elem<E>::load(A + (size_t) (b0 + r) * lda + b0 + j, p) : 0u;
-	}
-#pragma unroll
-	for (int r = 0; r < L; ++r) {
-		As[r][j] = v[r];
-		X[r][j] = 0u;
-	}
-	Ds[j] = (UNIT || j >= nb) ? 1u : dinv[b0 + j];
-	__syncthreads();
-#pragma unroll 1
-	for (int s = 0; s < L; ++s) {
-		const int i = UPPER ? L - 1 - s : s;
-		const int k0 = UPPER ? ((i + 1) & ~3) : 0, k1 = UPPER ? L : ((i + 3) & ~3);
-		uint64_t acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
-#pragma unroll 4
-		for (int k = k0; k < k1; k += 4) {
-			const uint4 a4 = *(const uint4 *) &As[i][k];
-			const uint32_t x0 = X[k][j], x1 = X[k + 1][j], x2 = X[k + 2][j], x3 = X[k + 3][j];
-			if (SMALL) {
-				acc0 += (uint64_t) a4.x * x0;
-				acc1 += (uint64_t) a4.y * x1;
-				acc2 += (uint64_t) a4.z * x2;
-				acc3 += (uint64_t) a4.w * x3;
-			} else {
-				acc0 += mp.mul(a4.x, x0);
-				acc1 += mp.mul(a4.y, x1);
-				acc2 += mp.mul(a4.z, x2);
-				acc3 += mp.mul(a4.w, x3);
-			}
+		for (int q = 0; q < 16; ++q) {
+			const int c = c0 + q;
+			const bool tri = UPPER ? (c > r) : (c < r);
+			v[q] = (r < nb && c < nb && tri) ? Elem<E>::load(A + (size_t) (b0 + r) * lda + b0 + c, p) : 0u;
 		}
-		uint32_t x = mp.sub(i == j ? 1u : 0u, (uint32_t) mp.reduce((acc0 + acc1) + (acc2 + acc3)));
-		if (!UNIT)
-			x = mp.mul(x, Ds[i]);
-		X[i][j] = x;
+#pragma unroll
+		for (int q = 0; q < 16; ++q) {
+			As[r][c0 + q] = v[q];
+			X[r][c0 + q] = 0u;
+		}
+		if (tid < L)
+			Ds[tid] = (UNIT || tid >= nb) ? 1u : dinv[b0 + tid];
 	}
-	for (int r = 0; r < nb; ++r)
-		W[(size_t) (b0 + r) * L + j] = j < nb ? X[r][j] : 0u;
+	__syncthreads();
+	if (tid < L) {           // column c of the inverse of diagonal block d (8 x 8)
+		const int o = 8 * (tid >> 3), c = tid & 7;
+		uint32_t x[8];
+#pragma unroll
+		for (int s_ = 0; s_ < 8; ++s_) {
+			const int i = UPPER ? 7 - s_ : s_;
+			uint64_t acc = 0;
+#pragma unroll
+			for (int t = 0; t < s_; ++t) {
+				const int k = UPPER ? 7 - t : t;     // the rows solved before i
+				acc += prod(As[o + i][o + k], x[k]);
+			}
+			uint32_t xi = mp.sub(i == c ? 1u : 0u, (uint32_t) mp.reduce(acc));
+			if (!UNIT)
+				xi = mp.mul(xi, Ds[o + i]);
+			x[i] = xi;
+		}
+#pragma unroll
+		for (int i = 0; i < 8; ++i)
+			X[o + i][o + c] = x[i];
+	}
+	__syncthreads();
+#pragma unroll
+	for (int s = 8; s < L; s *= 2) {
+		constexpr int NT = 256;
+		const int per = (L / (2 * s)) * s * s;     // outputs per product: 32 s
+		// T = C P^-1 (lower) or B Q^-1 (upper) for every pair
+		for (int idx = tid; idx < per; idx += NT) {
+			const int pr = idx / (s * s), rem = idx % (s * s), r = rem / s, c = rem % s, o = 2 * s * pr;
+			uint64_t acc = 0;
+			for (int k = 0; k < s; ++k)
+				acc += UPPER ? prod(As[o + r][o + s + k], X[o + s + k][o + s + c])
+					     : prod(As[o + s + r][o + k], X[o + k][o + c]);
+			Tm[idx] = (uint32_t) mp.reduce(acc);
+		}
+		__syncthreads();
+		// the off-diagonal block of the pair's inverse: -Q^-1 T (lower) or -P^-1 T (upper)
+		for (int idx = tid; idx < per; idx += NT) {
+			const int pr = idx / (s * s), rem = idx % (s * s), r = rem / s, c = rem % s, o = 2 * s * pr;
+			const uint32_t *Tp = Tm + pr * s * s;
+			uint64_t acc = 0;
+			for (int k = 0; k < s; ++k)
+				acc += UPPER ? prod(X[o + r][o + k], Tp[k * s + c]) : prod(X[o + s + r][o + s + k], Tp[k * s + c]);
+			const uint32_t v = mp.neg((uint32_t) mp.reduce(acc));
+			if (UPPER)
+				X[o + r][o + s + c] = v;
+			else
+				X[o + s + r][o + c] = v;
+		}
+		__syncthreads();
+	}
+	{
+		const int r = tid >> 2, c0 = 16 * (tid & 3);
+		if (r < nb)
+#pragma unroll
+			for (int q = 0; q < 16; ++q)
+				W[(size_t) (b0 + r) * L + c0 + q] = c0 + q < nb ? X[r][c0 + q] : 0u;
+	}
 }
 
 // Four canonical residues (p <= 2^16) as INT8 limb bytes: hi holds the high
@@ -366,7 +404,7 @@ static int trsm_diag_inverses(mlz_ctx *ctx, uint32_t p, int upper, int unit, siz
 	const ModP mp = make_modp(p);
 	unsigned g = ceil_div(n, TRSM_LEAF);
 	ProfScope ps(ctx, "trsm_diag_block_inv");
-#define INV(U, N, S) trsm_diag_inv_col_kernel<E, U, N, S><<<g, 64, 0, ctx->stream>>>((const T *) A, lda, (int) n, dinv, W, mp)
+#define INV(U, N, S) trsm_diag_inv_blk_kernel<E, U, N, S><<<g, 256, 0, ctx->stream>>>((const T *) A, lda, (int) n, dinv, W, mp)
 	if (mp.small) {
 		if (upper) { if (unit) INV(true, true, true); else INV(true, false, true); }
 		else       { if (unit) INV(false, true, true); else INV(false, false, true); }
@@ -375,7 +413,7 @@ static int trsm_diag_inverses(mlz_ctx *ctx, uint32_t p, int upper, int unit, siz
 		else       { if (unit) INV(false, true, false); else INV(false, false, false); }
 	}
 #undef INV
-	return check_launch(ctx, "trsm_diag_inv_col_kernel");
+	return check_launch(ctx, "trsm_diag_inv_blk_kernel");
 }
 
 template <int E>
