@@ -266,10 +266,12 @@ __device__ __forceinline__ void i8_pack_out(const uint32_t (&ov)[16], uint32_t p
 }
 
 // Epilogue of a workgroup tile.  32x32 accumulator layout: lane l,
-// register r holds row 8*(r/4) + 4*(l/32) + r%4, column l%32.  With Cpk the
-// results are also written as packed B fragments (i8_pack_out), 32-column
-// blocks panelC bytes apart; the caller guarantees whole 32 x 32 blocks.
-template <int E, class T, int WTM, int WTN>
+// register r holds row 8*(r/4) + 4*(l/32) + r%4, column l%32.  PK: the
+// results are also written to Cpk as packed B fragments (i8_pack_out),
+// 32-column blocks panelC bytes apart; the caller guarantees whole 32 x 32
+// blocks.  (A template flag, not a null test: kept in every instantiation,
+// the packing's registers slowed the 8192^3 product from 1.80 to 1.90 ms.)
+template <int E, class T, int WTM, int WTN, bool PK = false>
 __device__ __forceinline__ void i8_epilogue(const v16i (&acc)[3][WTM][WTN], typename Elem<E>::T *__restrict__ C,
 					    size_t ldc, int m, int n, uint32_t a, uint32_t b, const ModP &mp, int tm,
 					    int tn, int wr, int wc, int lane, int8_t *__restrict__ Cpk = nullptr,
@@ -301,10 +303,12 @@ __device__ __forceinline__ void i8_epilogue(const v16i (&acc)[3][WTM][WTN], type
 					double t = a == 1 ? v : (double) a * (double) mp.mod_f64(v);
 					if (b)
 						t = fma((double) b, (double) cv[r], t);
-					ov[r] = mp.mod_f64(t);
-					Elem<E>::store(cb + (size_t) (8 * (r >> 2) + (r & 3)) * ldc, ov[r], p);
+					const uint32_t o = mp.mod_f64(t);
+					Elem<E>::store(cb + (size_t) (8 * (r >> 2) + (r & 3)) * ldc, o, p);
+					if constexpr (PK)
+						ov[r] = o;
 				}
-				if (Cpk)
+				if constexpr (PK)
 					i8_pack_out(ov, p, lane,
 						    Cpk + (size_t) ((col - (lane & 31)) >> 5) * panelC + (size_t) (row0 >> 5) * 2 * I8_FRAG);
 				continue;
@@ -435,7 +439,7 @@ __device__ __forceinline__ void lds_barrier()
 	asm volatile("" ::: "memory");
 }
 
-template <int E, int WTM, int WTN, int KK, int OCC, int WM, int WN, int NS>
+template <int E, int WTM, int WTN, int KK, int OCC, int WM, int WN, int NS, bool PK = false>
 __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_async_kernel(const int8_t *__restrict__ Ap,
 								 const int8_t *__restrict__ Bp, int KB,
 								 typename Elem<E>::T *__restrict__ C, size_t ldc,
@@ -517,7 +521,7 @@ __global__ __launch_bounds__(64 * WM * WN, OCC) void gemm_i8_async_kernel(const 
 		if (ks0 + s < KS)
 			step(ks0 + s, s);
 	wait_vmcnt<0>();   // no load into LDS outlives the workgroup
-	i8_epilogue<E, T, WTM, WTN>(acc, C, ldc, m, n, a, b, mp, tm, tn, wr, wc, lane, Cpk, panelC);
+	i8_epilogue<E, T, WTM, WTN, PK>(acc, C, ldc, m, n, a, b, mp, tm, tn, wr, wc, lane, Cpk, panelC);
 }
 
 // ===========================================================================
@@ -673,12 +677,22 @@ static int gemm_i8_launch_t(mlz_ctx *ctx, uint32_t p, size_t m, size_t n, size_t
 				gemm_i8_kernel<E, WTM, WTN, KK, OCC, WM, WN><<<g, Tile::THREADS, 0, ctx->stream>>>(
 					(const int8_t *) ap, (const int8_t *) bp, KB, (T *) C, ldc, (int) m, (int) n, a,
 					k0 == 0 ? b : 1u, mp, tiles_m, tiles_n, bt, sAp, sBp);
-			else
+			else if (Cpk) {
+				// the batched entry's two tiles (64 x 64 and 128 x 128, four buffers)
+				if constexpr (WM == 2 && WN == 2 && NS == 4 && WTM == WTN)
+					gemm_i8_async_kernel<E, WTM, WTN, KK, OCC, WM, WN, NS, true>
+						<<<g, Tile::THREADS, 0, ctx->stream>>>(
+							(const int8_t *) ap, (const int8_t *) bp, KB, (T *) C, ldc, (int) m,
+							(int) n, a, k0 == 0 ? b : 1u, mp, tiles_m, tiles_n, bt, sAp, sBp,
+							Bpre ? bpanel : (size_t) KB * 2 * I8_FRAG, Cpk, cpanel);
+				else
+					return set_error(ctx, MLZ_E_ARG, "packed output: not for this tile variant");
+			} else
 				gemm_i8_async_kernel<E, WTM, WTN, KK, OCC, WM, WN, NS>
 					<<<g, Tile::THREADS, 0, ctx->stream>>>(
 						(const int8_t *) ap, (const int8_t *) bp, KB, (T *) C, ldc, (int) m, (int) n,
 						a, k0 == 0 ? b : 1u, mp, tiles_m, tiles_n, bt, sAp, sBp,
-						Bpre ? bpanel : (size_t) KB * 2 * I8_FRAG, Cpk, cpanel);
+						Bpre ? bpanel : (size_t) KB * 2 * I8_FRAG, nullptr, 0);
 		}
 		MLZ_TRY(check_launch(ctx, "gemm_i8_kernel"));
 	}

@@ -768,7 +768,9 @@ constexpr size_t TRSM_INV_BLOCK = 1024;
 constexpr size_t TRSM_ZP_BYTES = TRSM_INV_BLOCK * TRSM_INV_BLOCK * 2;   // one inverse as INT8 limbs
 
 // Z <- the 64-row diagonal inverses of W in place, zero elsewhere (one
-// thread per entry of the nfull S x S arrays).
+// thread per entry of the nfull S x S arrays).  The 64-blocks on the
+// triangle's side of the diagonal are left alone: the doubling steps write
+// each of them before anything reads it.
 template <bool UPPER>
 __global__ __launch_bounds__(256) void trsm_inv_init_kernel(const uint32_t *__restrict__ W, size_t total,
 							    uint32_t *__restrict__ Z)
@@ -779,6 +781,8 @@ __global__ __launch_bounds__(256) void trsm_inv_init_kernel(const uint32_t *__re
 		return;
 	const size_t I = idx / (S * S), rem = idx % (S * S);
 	const int r = (int) (rem / S), c = (int) (rem % S);
+	if (UPPER ? (c >> 6) > (r >> 6) : (c >> 6) < (r >> 6))
+		return;
 	const bool diag = (r >> 6) == (c >> 6) && (UPPER ? c >= r : c <= r);
 	Z[idx] = diag ? W[(I * S + r) * TRSM_LEAF + (c & (TRSM_LEAF - 1))] : 0u;
 }
